@@ -1,0 +1,187 @@
+/*
+ * mfv.h -- C ABI of libmfv.so, the MI355X (gfx950) kernels of manapy's explicit
+ * finite-volume step.
+ *
+ * Conventions
+ *   - Every pointer argument is a DEVICE pointer unless its name ends in _host.
+ *   - Indices are int32, floating-point values float64 unless a name says f32.
+ *   - Lengths are element counts. No C++ types, no exceptions cross the ABI.
+ *   - Every entry point returns 0 on success or a negative MFV_E* code; the
+ *     message of the last failure on the calling thread is mfv_last_error().
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream). All
+ *     calls are asynchronous on that stream except those with a *_host output,
+ *     which synchronise it.
+ *   - A mesh descriptor (mfv_mesh) holds device pointers owned by the caller; the
+ *     library never frees them.
+ *
+ * Each kernel-level entry point replaces the reference function cited beside
+ * it, with the reference's argument order and in-place semantics (outputs are
+ * arguments). The mesh arrays of the reference signature are taken from the
+ * descriptor instead of being passed one by one.
+ */
+#ifndef MFV_H
+#define MFV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFV_OK 0
+#define MFV_EINVAL (-1)
+#define MFV_ENOMEM (-2)
+#define MFV_EHIP (-3)
+#define MFV_ENOTSUP (-4)
+
+/* Device-resident mesh, manapy layout (ddm/domain.py) regrouped for the GPU:
+ * per-cell tables are column-major [slot][cell] so that a wavefront reading slot
+ * s of 64 consecutive cells reads 256 contiguous bytes. */
+typedef struct mfv_mesh {
+    int32_t dim;            /* 2 or 3 */
+    int32_t nb_cells;       /* C */
+    int32_t nb_faces;       /* F */
+    int32_t max_cell_faces; /* MF: columns of cell_faces */
+    int32_t lsq_width;      /* K: columns of lsq_cells */
+    int32_t ghost_width;    /* G: columns of lsq_ghosts */
+    const int32_t *face_cells;   /* [F][2]   faces.cellid (-1 on boundary faces) */
+    const int32_t *face_name;    /* [F]      faces.name */
+    const double *face_normal;   /* [F][3]   faces.normal, non-unit, out of face_cells[f][0] */
+    const double *face_mesure;   /* [F]      faces.mesure */
+    const double *face_center;   /* [F][3]   faces.center */
+    const double *face_ghost;    /* [F][3]   faces.ghostcenter (x, y[, z]) of boundary faces */
+    const double *cell_center;   /* [C][3]   cells.center */
+    const double *cell_volume;   /* [C]      cells.volume */
+    const int32_t *cell_faces;   /* [MF][C]  cells.faceid (reference order), -1 padded */
+    const int32_t *cell_nfaces;  /* [C]      cells.faceid[:, -1] */
+    const uint8_t *order_split;  /* [MF][C]  residual order of inner-then-boundary face loops:
+                                             slot r -> column of cell_faces */
+    const uint8_t *order_sorted; /* [MF][C]  residual order of all-faces-in-id-order loops */
+    const int32_t *lsq_cells;    /* [K][C]   cells.cellnid */
+    const int32_t *lsq_ncells;   /* [C]      cells.cellnid[:, -1] */
+    const int32_t *lsq_ghosts;   /* [G][C]   cells.ghostnid */
+    const int32_t *lsq_nghosts;  /* [C]      cells.ghostnid[:, -1] */
+    const int32_t *bnd_faces;    /* [nb_bnd] domain.boundaryfaces */
+    int32_t nb_bnd;
+    int32_t pad_;
+} mfv_mesh;
+
+const char *mfv_version(void);
+const char *mfv_last_error(void);
+int mfv_device_count(int *count_host);
+
+/* ---------------- generic operators ----------------------------------------------- */
+
+/* ghost_value_neumann / _nonslip / _dirichlet (boundary/bc_utils.py:45-68).
+ * kind 0: w_ghost[f] = src[cellid[f][0]]; 1: w_ghost[f] = -src[cellid[f][0]];
+ * 2: w_ghost[f] = src[f] (src = BC value array). Acts on the faces listed. */
+int mfv_ghost_value(const mfv_mesh *m, int kind, const double *src, double *w_ghost,
+                    const int32_t *faces, int nfaces, void *stream);
+
+/* facetocell (ast/ast_utils.py:8-18). */
+int mfv_facetocell(const mfv_mesh *m, const double *u_face, double *u_c, void *stream);
+
+/* cell_gradient_2d / cell_gradient_3d (ast/functions2d.py:5-104, functions3d.py:4-120),
+ * single-subdomain non-periodic terms. */
+int mfv_cell_gradient(const mfv_mesh *m, const double *w_c, const double *w_ghost,
+                      double *w_x, double *w_y, double *w_z, void *stream);
+
+/* barthlimiter_2d / barthlimiter_3d (ast/functions2d.py:280-325, functions3d.py:364-408). */
+int mfv_barthlimiter(const mfv_mesh *m, const double *w_c, const double *w_ghost,
+                     const double *w_x, const double *w_y, const double *w_z, double *psi, void *stream);
+
+/* ---------------- advection (solvers/advec) --------------------------------------- */
+
+/* time_step (solvers/advec/fvm_utils.py:293-309): dt = min_K cfl |K| / sum_f |u_K.n_f|.
+ * Writes the minimum to dt_dev[0] (device) and, if dt_host != NULL, to *dt_host (sync). */
+int mfv_time_step(const mfv_mesh *m, const double *u, const double *v, const double *w, double cfl,
+                  double *dt_dev, double *dt_host, void *stream);
+
+/* explicitscheme_convective_2d / _3d (solvers/advec/fvm_utils.py:19-152, :156-290) with the
+ * residual formed by a deterministic cell-centric gather in the reference's face order. */
+int mfv_explicitscheme_convective(const mfv_mesh *m, double *rez_w, const double *w_c,
+                                  const double *w_ghost, const double *u_face, const double *v_face,
+                                  const double *w_face, const double *w_x, const double *w_y,
+                                  const double *w_z, const double *psi, int order, void *stream);
+
+/* update_new_value (solvers/advec/fvm_utils.py:311-315); dtime read from dt_dev[0]
+ * when dt_dev != NULL, else the value argument. */
+int mfv_update_new_value(const mfv_mesh *m, double *ne_c, const double *rez, const double *dissip,
+                         const double *src, double dtime, const double *dt_dev, void *stream);
+
+/* One fused explicit upwind step for linear advection (time_step + ghost Neumann +
+ * explicitscheme_convective(order) + update_new_value, order 1 or 2 with LSQ gradient and
+ * Barth-Jespersen limiter). w_in -> w_out; dt_cur[0] is the dt of this step; the dt of the
+ * next step (from w_out) is min-reduced into dt_next[0], which must hold 1e6 on entry.
+ * scratch: 3*C doubles (gradient and limiter), unused at order 1. */
+int mfv_advection_step(const mfv_mesh *m, const double *w_in, double *w_out, const double *u_c,
+                       const double *v_c, const double *w_c, const double *u_face,
+                       const double *v_face, const double *w_face, double cfl, int order,
+                       const double *dt_cur, double *dt_next, double *scratch, void *stream);
+
+/* ---------------- shallow water (solvers/shallowater) ----------------------------- */
+
+/* time_step_SW (solvers/shallowater/fvm_utils.py:16-42). Output as mfv_time_step. */
+int mfv_time_step_sw(const mfv_mesh *m, const double *h, const double *hu, const double *hv,
+                     double cfl, double grav, double Dxx, double Dyy, double *dt_dev,
+                     double *dt_host, void *stream);
+
+/* Limited least-squares gradients of eta = h + Z, hu, hv for the MUSCL shallow-water step:
+ * cell_gradient_2d + barthlimiter_2d applied to the three fields with wall ghosts
+ * (h Neumann, hu/hv nonslip, Z Neumann; boundary/bc_utils.py:45-55).
+ * grad: [9][C] = eta_x, eta_y, eta_psi, hu_x, hu_y, hu_psi, hv_x, hv_y, hv_psi. */
+int mfv_sw_gradients(const mfv_mesh *m, const double *h, const double *hu, const double *hv,
+                     const double *Z, double *grad, void *stream);
+
+/* Residuals of the MUSCL/Rusanov shallow-water step with hydrostatic reconstruction
+ * (the loop structure of explicitscheme_convective_SW, shallowater/fvm_utils.py:344-496;
+ * the reference's flux is SRNH, this one is Rusanov). rez/src: [3][C] (h, hu, hv);
+ * zgrad: [2][C] unlimited LSQ gradient of Z. */
+int mfv_sw_residuals(const mfv_mesh *m, const double *h, const double *hu, const double *hv,
+                     const double *Z, const double *grad, const double *zgrad, double grav,
+                     int order, double *rez, double *src, void *stream);
+
+/* update_SW (solvers/shallowater/fvm_utils.py:3-14) for h, hu, hv; dt from dt_dev[0]. */
+int mfv_update_sw(const mfv_mesh *m, double *h, double *hu, double *hv, const double *rez,
+                  const double *src, const double *dt_dev, void *stream);
+
+/* Fused MUSCL/Rusanov step: residuals + update_SW + the next step's time_step_SW.
+ * state_in/state_out: [3][C] (h, hu, hv); dt_cur[0] this step's dt; dt_next[0] receives
+ * the min-reduced dt of state_out and must hold 1e6 on entry (mfv_dt_reset). */
+int mfv_sw_step_fused(const mfv_mesh *m, const double *state_in, double *state_out,
+                      const double *Z, const double *grad, const double *zgrad, double grav,
+                      double cfl, int order, const double *dt_cur, double *dt_next, void *stream);
+
+/* Sets dt[0] = 1e6, the initial value of the reference's dt min-loops. */
+int mfv_dt_reset(double *dt, void *stream);
+
+/* ---------------- 3D Euler (solvers/euler/fvm_utils3d.py) ------------------------- */
+
+/* ghost_value_TubeSchok3D (fvm_utils3d.py:441-500); quirk=1 keeps the reference's
+ * rhovg overwrite (rhowg untouched), quirk=0 the corrected slip wall.
+ * U: [6][C] (rho, P, rhou, rhov, rhow, rhoE); G: [6][F] ghosts. */
+int mfv_euler3d_ghost(const mfv_mesh *m, const double *U, double *G, int quirk, void *stream);
+
+/* time_step_euler_3d (fvm_utils3d.py:809-825) min-reduced: dt_dev[0] = min_K dt_c[K]. */
+int mfv_euler3d_time_step(const mfv_mesh *m, const double *U, double cfl, double gamma,
+                          double *dt_dev, double *dt_host, void *stream);
+
+/* explicitscheme_euler_3d_rusanov (fvm_utils3d.py:1083-1170) + compute_flux_euler_3d_rusanov
+ * (:838-886), cell-centric gather in face-id order. rez: [5][C]. */
+int mfv_euler3d_rusanov(const mfv_mesh *m, const double *U, const double *G, double gamma,
+                        int quirk, double *rez, void *stream);
+
+/* update_euler_3d_fvc (fvm_utils3d.py:1316-1327); dt from dt_dev[0]. */
+int mfv_euler3d_update(const mfv_mesh *m, double *U, const double *rez, double gamma,
+                       const double *dt_dev, void *stream);
+
+/* Fused step: ghosts on the fly + Rusanov fluxes + update + next dt.
+ * U_in -> U_out ([6][C]); dt_next[0] must hold 1e6 on entry. */
+int mfv_euler3d_step_fused(const mfv_mesh *m, const double *U_in, double *U_out, double gamma,
+                           double cfl, int quirk, const double *dt_cur, double *dt_next,
+                           void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFV_H */

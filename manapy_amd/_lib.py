@@ -1,0 +1,100 @@
+"""ctypes binding of libmfv.so (declarations in include/mfv.h).
+
+The library is loaded from the package directory only; there is no fallback. A
+missing or unloadable library raises ``MfvError`` at first use, so a run can
+never silently compute on the CPU.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libmfv.so')
+
+c_int = ctypes.c_int
+c_double = ctypes.c_double
+c_void_p = ctypes.c_void_p
+c_char_p = ctypes.c_char_p
+P = ctypes.c_void_p  # every array argument is a raw device address
+
+
+class MfvError(RuntimeError):
+    pass
+
+
+class MfvMesh(ctypes.Structure):
+    """Mirror of ``struct mfv_mesh`` (include/mfv.h)."""
+    _fields_ = [
+        ('dim', ctypes.c_int32), ('nb_cells', ctypes.c_int32), ('nb_faces', ctypes.c_int32),
+        ('max_cell_faces', ctypes.c_int32), ('lsq_width', ctypes.c_int32), ('ghost_width', ctypes.c_int32),
+        ('face_cells', P), ('face_name', P), ('face_normal', P), ('face_mesure', P), ('face_center', P),
+        ('face_ghost', P), ('cell_center', P), ('cell_volume', P), ('cell_faces', P), ('cell_nfaces', P),
+        ('order_split', P), ('order_sorted', P), ('lsq_cells', P), ('lsq_ncells', P), ('lsq_ghosts', P),
+        ('lsq_nghosts', P), ('bnd_faces', P), ('nb_bnd', ctypes.c_int32), ('pad_', ctypes.c_int32),
+    ]
+
+
+MESH_P = ctypes.POINTER(MfvMesh)
+DP = ctypes.POINTER(ctypes.c_double)
+
+# name -> argtypes (restype is int unless listed in _RESTYPE)
+_SIGNATURES = {
+    'mfv_version': [],
+    'mfv_last_error': [],
+    'mfv_device_count': [ctypes.POINTER(c_int)],
+    'mfv_ghost_value': [MESH_P, c_int, P, P, P, c_int, P],
+    'mfv_facetocell': [MESH_P, P, P, P],
+    'mfv_cell_gradient': [MESH_P, P, P, P, P, P, P],
+    'mfv_barthlimiter': [MESH_P, P, P, P, P, P, P, P],
+    'mfv_time_step': [MESH_P, P, P, P, c_double, P, DP, P],
+    'mfv_explicitscheme_convective': [MESH_P, P, P, P, P, P, P, P, P, P, P, c_int, P],
+    'mfv_update_new_value': [MESH_P, P, P, P, P, c_double, P, P],
+    'mfv_advection_step': [MESH_P, P, P, P, P, P, P, P, P, c_double, c_int, P, P, P, P],
+    'mfv_time_step_sw': [MESH_P, P, P, P, c_double, c_double, c_double, c_double, P, DP, P],
+    'mfv_sw_gradients': [MESH_P, P, P, P, P, P, P],
+    'mfv_sw_residuals': [MESH_P, P, P, P, P, P, P, c_double, c_int, P, P, P],
+    'mfv_update_sw': [MESH_P, P, P, P, P, P, P, P],
+    'mfv_sw_step_fused': [MESH_P, P, P, P, P, P, c_double, c_double, c_int, P, P, P],
+    'mfv_dt_reset': [P, P],
+    'mfv_euler3d_ghost': [MESH_P, P, P, c_int, P],
+    'mfv_euler3d_time_step': [MESH_P, P, c_double, c_double, P, DP, P],
+    'mfv_euler3d_rusanov': [MESH_P, P, P, c_double, c_int, P, P],
+    'mfv_euler3d_update': [MESH_P, P, P, c_double, P, P],
+    'mfv_euler3d_step_fused': [MESH_P, P, P, c_double, c_double, c_int, P, P, P],
+}
+_RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p}
+
+_lib = None
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def load():
+    """Load libmfv.so and bind every entry point of include/mfv.h."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MfvError("libmfv.so not built (%s): run __graft_entry__.build() or "
+                       "python manapy_amd/build.py" % LIB_PATH)
+    try:
+        lib = ctypes.CDLL(LIB_PATH)
+    except OSError as exc:
+        raise MfvError("cannot load %s: %s" % (LIB_PATH, exc))
+    for name, args in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, c_int)
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call an int-returning entry point; raise MfvError with mfv_last_error() on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.mfv_last_error().decode(errors='replace')
+        raise MfvError("%s failed (%d): %s" % (name, rc, msg))
+    return rc

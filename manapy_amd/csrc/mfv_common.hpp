@@ -1,0 +1,110 @@
+// Shared device helpers for libmfv (gfx950, wave64).
+//
+// Arithmetic in every kernel follows the reference expression order term by term
+// and the library is compiled with -ffp-contract=off: with IEEE-correct division and
+// square root, a cell-centric gather that adds the face contributions in the
+// reference's face order reproduces the reference's face-ordered scatter bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/mfv.h"
+
+namespace mfv {
+
+void set_error(const char *fmt, ...);
+int check_launch(const char *what);
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
+
+#define MFV_REQUIRE(cond, ...)            \
+    do {                                  \
+        if (!(cond)) {                    \
+            ::mfv::set_error(__VA_ARGS__); \
+            return MFV_EINVAL;            \
+        }                                 \
+    } while (0)
+
+// Python min(a, b) / max(a, b): the second argument wins only on a strict comparison.
+__device__ __forceinline__ double pmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double pmax(double a, double b) { return (b > a) ? b : a; }
+
+// Positive doubles order like their bit patterns, so a min over dt values is an unsigned
+// 64-bit atomicMin. NaN patterns sort above +inf and are ignored, as Python's min(dt, nan)
+// keeps dt. The result does not depend on the order of the atomics.
+__device__ __forceinline__ void atomic_min_positive(double *addr, double v)
+{
+    if (!(v >= 0.0)) return;
+    unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+    atomicMin(reinterpret_cast<unsigned long long *>(addr), bits);
+}
+
+// Wave64 min of non-negative doubles (NaN ignored) followed by one atomic per wave.
+__device__ __forceinline__ void wave_min_to(double *addr, double v)
+{
+    double m = (v >= 0.0) ? v : __longlong_as_double(0x7ff0000000000000LL);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(m, off, 64);
+        m = (o < m) ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomic_min_positive(addr, m);
+}
+
+// Mesh descriptor passed by value to kernels.
+struct Mesh {
+    int C, F, MF, K, G;
+    const int32_t *__restrict__ face_cells;
+    const int32_t *__restrict__ face_name;
+    const double *__restrict__ face_normal;
+    const double *__restrict__ face_mesure;
+    const double *__restrict__ face_center;
+    const double *__restrict__ face_ghost;
+    const double *__restrict__ cell_center;
+    const double *__restrict__ cell_volume;
+    const int32_t *__restrict__ cell_faces;
+    const int32_t *__restrict__ cell_nfaces;
+    const uint8_t *__restrict__ order_split;
+    const uint8_t *__restrict__ order_sorted;
+    const int32_t *__restrict__ lsq_cells;
+    const int32_t *__restrict__ lsq_ncells;
+    const int32_t *__restrict__ lsq_ghosts;
+    const int32_t *__restrict__ lsq_nghosts;
+};
+
+inline Mesh to_kernel(const mfv_mesh *m)
+{
+    Mesh k;
+    k.C = m->nb_cells;
+    k.F = m->nb_faces;
+    k.MF = m->max_cell_faces;
+    k.K = m->lsq_width;
+    k.G = m->ghost_width;
+    k.face_cells = m->face_cells;
+    k.face_name = m->face_name;
+    k.face_normal = m->face_normal;
+    k.face_mesure = m->face_mesure;
+    k.face_center = m->face_center;
+    k.face_ghost = m->face_ghost;
+    k.cell_center = m->cell_center;
+    k.cell_volume = m->cell_volume;
+    k.cell_faces = m->cell_faces;
+    k.cell_nfaces = m->cell_nfaces;
+    k.order_split = m->order_split;
+    k.order_sorted = m->order_sorted;
+    k.lsq_cells = m->lsq_cells;
+    k.lsq_ncells = m->lsq_ncells;
+    k.lsq_ghosts = m->lsq_ghosts;
+    k.lsq_nghosts = m->lsq_nghosts;
+    return k;
+}
+
+int validate_mesh(const mfv_mesh *m, int need_lsq, int need_order_split, int need_order_sorted);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace mfv

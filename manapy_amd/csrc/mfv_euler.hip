@@ -1,0 +1,304 @@
+// 3D compressible Euler, first-order Rusanov on tetrahedra (solvers/euler/fvm_utils3d.py).
+//
+// State layout: [6][C] = rho, P, rhou, rhov, rhow, rhoE. Ghost layout: [6][F].
+// quirk = 1 reproduces the reference's two slips on this path:
+//   Q1 compute_flux_euler_3d_rusanov (:842): qr = rhou_R n0 + rhov_L n1 + rhow_L n2;
+//   Q2 ghost_value_TubeSchok3D (:493-494): rhovg is overwritten by rho*w_g, rhowg never set
+//      (it keeps its previous content: zero for zero-initialised ghost arrays).
+// quirk = 0 uses the right state in qr and the mirrored velocity in both components.
+#include "mfv_common.hpp"
+
+using namespace mfv;
+
+namespace {
+
+struct Prim6 {
+    double rho, P, ru, rv, rw, rE;
+};
+
+__device__ __forceinline__ void euler_flux(const Prim6 &L, const Prim6 &R, const double *n, double mesure,
+                                           double gamma, int quirk, double (&fl)[5])
+{
+    const double rhol = L.rho, Pl = L.P, rhoul = L.ru, rhovl = L.rv, rhowl = L.rw, rhoEl = L.rE;
+    const double rhor = R.rho, Pr = R.P, rhour = R.ru, rhovr = R.rv, rhowr = R.rw, rhoEr = R.rE;
+    double ql = (rhoul * n[0] + rhovl * n[1] + rhowl * n[2]);
+    double qr = quirk ? (rhour * n[0] + rhovl * n[1] + rhowl * n[2]) : (rhour * n[0] + rhovr * n[1] + rhowr * n[2]);
+    double cl = sqrt(gamma * Pl / rhol);
+    double cr = sqrt(gamma * Pr / rhor);
+    double fl_rho = (rhoul * n[0] + rhovl * n[1] + rhowl * n[2]);
+    double fl_rhou = ((rhoul * rhoul / rhol + Pl) * n[0] + (rhoul * rhovl / rhol) * n[1] + (rhoul * rhowl / rhol) * n[2]);
+    double fl_rhov = ((rhovl * rhoul / rhol) * n[0] + (rhovl * rhovl / rhol + Pl) * n[1] + (rhovl * rhowl / rhol) * n[2]);
+    double fl_rhow = ((rhoul * rhowl / rhol) * n[0] + (rhovl * rhowl / rhol) * n[1] + (rhowl * rhowl / rhol + Pl) * n[2]);
+    double fl_rhoE =
+        (rhoul / rhol * (rhoEl + Pl) * n[0] + rhovl / rhol * (rhoEl + Pl) * n[1] + rhowl / rhol * (rhoEl + Pl) * n[2]);
+    double fr_rho = (rhour * n[0] + rhovr * n[1] + rhowr * n[2]);
+    double fr_rhou = ((rhour * rhour / rhor + Pr) * n[0] + (rhour * rhovr / rhor) * n[1] + (rhour * rhowr / rhor) * n[2]);
+    double fr_rhov = ((rhovr * rhour / rhor) * n[0] + (rhovr * rhovr / rhor + Pr) * n[1] + (rhovr * rhowr / rhor) * n[2]);
+    double fr_rhow = ((rhour * rhowr / rhor) * n[0] + (rhovr * rhowr / rhor) * n[1] + (rhowr * rhowr / rhor + Pr) * n[2]);
+    double fr_rhoE =
+        (rhour / rhor * (rhoEr + Pr) * n[0] + rhovr / rhor * (rhoEr + Pr) * n[1] + rhowr / rhor * (rhoEr + Pr) * n[2]);
+    double l1 = fabs((ql) / rhol - cl), l2 = fabs((ql) / rhol), l3 = fabs((ql) / rhol + cl);
+    double r1 = fabs((qr) / rhor - cr), r2 = fabs((qr) / rhor), r3 = fabs((qr) / rhor + cr);
+    double Ll = pmax(pmax(l1, l2), l3);
+    double Lr = pmax(pmax(r1, r2), r3);
+    double S = (Ll > Lr) ? Ll : Lr;
+    fl[0] = (0.5 * (fl_rho + fr_rho) - 0.5 * S * (rhor - rhol)) * mesure;
+    fl[1] = (0.5 * (fl_rhou + fr_rhou) - 0.5 * S * (rhour - rhoul)) * mesure;
+    fl[2] = (0.5 * (fl_rhov + fr_rhov) - 0.5 * S * (rhovr - rhovl)) * mesure;
+    fl[3] = (0.5 * (fl_rhow + fr_rhow) - 0.5 * S * (rhowr - rhowl)) * mesure;
+    fl[4] = (0.5 * (fl_rhoE + fr_rhoE) - 0.5 * S * (rhoEr - rhoEl)) * mesure;
+}
+
+__device__ __forceinline__ Prim6 load6(const double *__restrict__ U, size_t n, int c)
+{
+    Prim6 p;
+    p.rho = U[c];
+    p.P = U[n + c];
+    p.ru = U[2 * n + c];
+    p.rv = U[3 * n + c];
+    p.rw = U[4 * n + c];
+    p.rE = U[5 * n + c];
+    return p;
+}
+
+// ghost_value_TubeSchok3D for one face from its owner state. rhowg_prev is the ghost
+// array's previous content, kept under quirk 1 on slip faces.
+__device__ __forceinline__ Prim6 euler_ghost(const Prim6 &c, const double *n, int name, int quirk,
+                                             double rhowg_prev)
+{
+    Prim6 g = c;
+    if (name == 3 || name == 4 || name == 5 || name == 6) {
+        double u_i = c.ru / c.rho, v_i = c.rv / c.rho, w_i = c.rw / c.rho;
+        double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        double s0 = n[0] / nn, s1 = n[1] / nn, s2 = n[2] / nn;
+        double un = u_i * s0 + v_i * s1 + w_i * s2;
+        double u_g = u_i - 2 * un * s0;
+        double v_g = v_i - 2 * un * s1;
+        double w_g = w_i - 2 * un * s2;
+        g.ru = c.rho * u_g;
+        if (quirk) {
+            g.rv = c.rho * w_g;
+            g.rw = rhowg_prev;
+        } else {
+            g.rv = c.rho * v_g;
+            g.rw = c.rho * w_g;
+        }
+    }
+    return g;
+}
+
+__global__ void __launch_bounds__(kBlock) k_euler_ghost(Mesh m, const int32_t *__restrict__ bnd, int nb,
+                                                         const double *__restrict__ U, double *__restrict__ G,
+                                                         int quirk)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    int f = bnd[k];
+    int name = m.face_name[f];
+    if (!(name >= 1 && name <= 6)) return;
+    const size_t C = m.C, F = m.F;
+    Prim6 c = load6(U, C, m.face_cells[2 * f]);
+    Prim6 g = euler_ghost(c, m.face_normal + 3 * (size_t)f, name, quirk, G[4 * F + f]);
+    G[f] = g.rho;
+    G[F + f] = g.P;
+    G[2 * F + f] = g.ru;
+    G[3 * F + f] = g.rv;
+    G[4 * F + f] = g.rw;
+    G[5 * F + f] = g.rE;
+}
+
+// time_step_euler_3d (fvm_utils3d.py:809-825): first four faces of the cell.
+__device__ __forceinline__ double euler_cell_dt(const Mesh &m, int i, double rho, double P, double ru, double rv,
+                                                double rw, double cfl, double gamma)
+{
+    double lam = 0.;
+    double velson = sqrt(gamma * fabs(P / rho));
+    for (int j = 0; j < 4; j++) {
+        int f = m.cell_faces[(size_t)j * m.C + i];
+        const double *n = m.face_normal + 3 * (size_t)f;
+        double u_n = fabs((ru * n[0] + rv * n[1] + rw * n[2]) / rho) / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        double lam_convect = u_n + velson;
+        lam += lam_convect * m.face_mesure[f];
+    }
+    return cfl * m.cell_volume[i] / lam;
+}
+
+__global__ void __launch_bounds__(kBlock) k_euler_dt(Mesh m, const double *__restrict__ U, double cfl, double gamma,
+                                                      double *dt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 1e300;
+    if (i < m.C) {
+        const size_t C = m.C;
+        d = euler_cell_dt(m, i, U[i], U[C + i], U[2 * C + i], U[3 * C + i], U[4 * C + i], cfl, gamma);
+    }
+    wave_min_to(dt, d);
+}
+
+// Residual of cell i, faces in increasing id (the single all-faces loop of
+// explicitscheme_euler_3d_rusanov). GHOST_ARRAY: right state of boundary faces from G; else
+// computed from the owner with euler_ghost (rhowg_prev = 0 under quirk).
+template <bool GHOST_ARRAY>
+__device__ __forceinline__ void euler_cell_residual(const Mesh &m, int i, const double *__restrict__ U,
+                                                    const double *__restrict__ G, double gamma, int quirk,
+                                                    double (&rez)[5])
+{
+    for (int v = 0; v < 5; v++) rez[v] = 0.;
+    const size_t C = m.C, F = m.F;
+    const int nf = m.cell_nfaces[i];
+    for (int r = 0; r < nf; r++) {
+        int s = m.order_sorted[(size_t)r * C + i];
+        int f = m.cell_faces[(size_t)s * C + i];
+        int a = m.face_cells[2 * f];
+        const double mesu = m.face_mesure[f];
+        const double *nrm = m.face_normal + 3 * (size_t)f;
+        double n[3] = {nrm[0] / mesu, nrm[1] / mesu, nrm[2] / mesu};
+        Prim6 L = load6(U, C, a);
+        Prim6 R;
+        int name = m.face_name[f];
+        if (name == 0) {
+            R = load6(U, C, m.face_cells[2 * f + 1]);
+        } else if (GHOST_ARRAY) {
+            R = load6(G, F, f);
+        } else {
+            R = euler_ghost(L, nrm, name, quirk, 0.);
+        }
+        double fl[5];
+        euler_flux(L, R, n, mesu, gamma, quirk, fl);
+        if (a == i) {
+            for (int v = 0; v < 5; v++) rez[v] -= fl[v];
+        } else {
+            for (int v = 0; v < 5; v++) rez[v] += fl[v];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_euler_rusanov(Mesh m, const double *__restrict__ U,
+                                                           const double *__restrict__ G, double gamma, int quirk,
+                                                           double *__restrict__ rez)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    double r[5];
+    euler_cell_residual<true>(m, i, U, G, gamma, quirk, r);
+    const size_t C = m.C;
+    for (int v = 0; v < 5; v++) rez[v * C + i] = r[v];
+}
+
+__device__ __forceinline__ void euler_update_cell(double (&u)[6], const double (&r)[5], double gamma, double dt,
+                                                  double vol)
+{
+    // u = rho, P, rhou, rhov, rhow, rhoE ; r = rho, rhou, rhov, rhow, rhoE
+    u[0] += dt * ((r[0]) / vol);
+    u[2] += dt * ((r[1]) / vol);
+    u[3] += dt * ((r[2]) / vol);
+    u[4] += dt * ((r[3]) / vol);
+    u[5] += dt * ((r[4]) / vol);
+    u[1] = (gamma - 1) * (u[5] - 0.5 * (u[2] * u[2] + u[3] * u[3] + u[4] * u[4]) / u[0]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_euler_update(int C, double *__restrict__ U,
+                                                          const double *__restrict__ rez, double gamma,
+                                                          const double *__restrict__ dt_dev,
+                                                          const double *__restrict__ vol)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const size_t n = C;
+    double u[6], r[5];
+    for (int v = 0; v < 6; v++) u[v] = U[v * n + i];
+    for (int v = 0; v < 5; v++) r[v] = rez[v * n + i];
+    euler_update_cell(u, r, gamma, dt_dev[0], vol[i]);
+    for (int v = 0; v < 6; v++) U[v * n + i] = u[v];
+}
+
+__global__ void __launch_bounds__(kBlock) k_euler_step(Mesh m, const double *__restrict__ Uin,
+                                                        double *__restrict__ Uout, double gamma, double cfl,
+                                                        int quirk, const double *__restrict__ dt_cur,
+                                                        double *dt_next)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 1e300;
+    if (i < m.C) {
+        const size_t C = m.C;
+        double r[5];
+        euler_cell_residual<false>(m, i, Uin, nullptr, gamma, quirk, r);
+        double u[6];
+        for (int v = 0; v < 6; v++) u[v] = Uin[v * C + i];
+        euler_update_cell(u, r, gamma, dt_cur[0], m.cell_volume[i]);
+        for (int v = 0; v < 6; v++) Uout[v * C + i] = u[v];
+        d = euler_cell_dt(m, i, u[0], u[1], u[2], u[3], u[4], cfl, gamma);
+    }
+    wave_min_to(dt_next, d);
+}
+
+}  // namespace
+
+extern "C" int mfv_euler3d_ghost(const mfv_mesh *m, const double *U, double *G, int quirk, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    MFV_REQUIRE(m->dim == 3, "Euler 3D needs a 3D mesh");
+    MFV_REQUIRE(U && G, "NULL array");
+    if (m->nb_bnd == 0) return MFV_OK;
+    MFV_REQUIRE(m->bnd_faces, "mesh descriptor lacks bnd_faces");
+    hipLaunchKernelGGL(k_euler_ghost, dim3(grid_for(m->nb_bnd)), dim3(kBlock), 0, as_stream(stream), to_kernel(m),
+                       m->bnd_faces, m->nb_bnd, U, G, quirk);
+    return check_launch("k_euler_ghost");
+}
+
+extern "C" int mfv_euler3d_time_step(const mfv_mesh *m, const double *U, double cfl, double gamma, double *dt_dev,
+                                     double *dt_host, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    MFV_REQUIRE(m->dim == 3 && m->max_cell_faces >= 4, "Euler 3D time step needs tetrahedra");
+    MFV_REQUIRE(U && dt_dev, "NULL array");
+    if (int rc = mfv_dt_reset(dt_dev, stream)) return rc;
+    if (m->nb_cells)
+        hipLaunchKernelGGL(k_euler_dt, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m),
+                           U, cfl, gamma, dt_dev);
+    if (int rc = check_launch("k_euler_dt")) return rc;
+    if (dt_host) {
+        hipError_t e = hipMemcpyAsync(dt_host, dt_dev, sizeof(double), hipMemcpyDeviceToHost, as_stream(stream));
+        if (e == hipSuccess) e = hipStreamSynchronize(as_stream(stream));
+        if (e != hipSuccess) {
+            set_error("dt readback: %s", hipGetErrorString(e));
+            return MFV_EHIP;
+        }
+    }
+    return MFV_OK;
+}
+
+extern "C" int mfv_euler3d_rusanov(const mfv_mesh *m, const double *U, const double *G, double gamma, int quirk,
+                                   double *rez, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 1)) return rc;
+    MFV_REQUIRE(m->dim == 3, "Euler 3D needs a 3D mesh");
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(U && G && rez, "NULL array");
+    hipLaunchKernelGGL(k_euler_rusanov, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                       to_kernel(m), U, G, gamma, quirk, rez);
+    return check_launch("k_euler_rusanov");
+}
+
+extern "C" int mfv_euler3d_update(const mfv_mesh *m, double *U, const double *rez, double gamma, const double *dt_dev,
+                                  void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(U && rez && dt_dev, "NULL array");
+    hipLaunchKernelGGL(k_euler_update, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), m->nb_cells,
+                       U, rez, gamma, dt_dev, m->cell_volume);
+    return check_launch("k_euler_update");
+}
+
+extern "C" int mfv_euler3d_step_fused(const mfv_mesh *m, const double *U_in, double *U_out, double gamma, double cfl,
+                                      int quirk, const double *dt_cur, double *dt_next, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 1)) return rc;
+    MFV_REQUIRE(m->dim == 3 && m->max_cell_faces >= 4, "Euler 3D needs tetrahedra");
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(U_in && U_out && U_in != U_out && dt_cur && dt_next, "NULL or aliased array");
+    hipLaunchKernelGGL(k_euler_step, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m),
+                       U_in, U_out, gamma, cfl, quirk, dt_cur, dt_next);
+    return check_launch("k_euler_step");
+}

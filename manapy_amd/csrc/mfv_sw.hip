@@ -1,0 +1,415 @@
+// Shallow water on triangles/quads: second-order MUSCL with least-squares gradients,
+// Barth-Jespersen limiting, hydrostatic reconstruction and the Rusanov flux.
+//
+// Wall boundaries as in manapy's shallow-water driver: h Neumann, hu/hv "nonslip"
+// ghosts (w_ghost = -w of the owner cell, boundary/bc_utils.py:45-55), Z Neumann. The
+// gradients use these cell ghosts; the wall flux applies the same mirror to the
+// reconstructed face state, which keeps the walls exactly mass-conserving at second
+// order. Ghosts are computed from the owner on the fly, so no ghost arrays are stored.
+//
+// State layout: [3][C] = h, hu, hv. Gradient layout: [9][C] = (x, y, psi) of eta = h+Z,
+// hu, hv. Z gradient: [2][C], unlimited, computed once per mesh.
+#include "mfv_common.hpp"
+
+using namespace mfv;
+
+namespace {
+
+constexpr double kDry = 1e-12;
+
+// Hydrostatic-reconstruction Rusanov flux of one face (same expression order as the
+// oracle's orc_sw_face). flux: from L to R, |n| included. bL/bR: bathymetry term of each
+// side, to be multiplied by that side's outward normal and subtracted.
+__device__ __forceinline__ void sw_face(double etaL, double huL, double hvL, double zfL, double hcL, double zcL,
+                                        double etaR, double huR, double hvR, double zfR, double hcR, double zcR,
+                                        double n0, double n1, double mes, double g, double &F0, double &F1,
+                                        double &F2, double &bL, double &bR)
+{
+    double hL = etaL - zfL;
+    if (hL < 0.) hL = 0.;
+    double hR = etaR - zfR;
+    if (hR < 0.) hR = 0.;
+    double uL = 0., vL = 0., uR = 0., vR = 0.;
+    if (hL > kDry) {
+        uL = huL / hL;
+        vL = hvL / hL;
+    }
+    if (hR > kDry) {
+        uR = huR / hR;
+        vR = hvR / hR;
+    }
+    double zs = pmax(zfL, zfR);
+    double hsL = hL + zfL - zs;
+    if (hsL < 0.) hsL = 0.;
+    double hsR = hR + zfR - zs;
+    if (hsR < 0.) hsR = 0.;
+    double qL = uL * n0 + vL * n1;
+    double qR = uR * n0 + vR * n1;
+    double pL = 0.5 * g * hsL * hsL;
+    double pR = 0.5 * g * hsR * hsR;
+    double fL0 = hsL * qL, fL1 = hsL * uL * qL + pL * n0, fL2 = hsL * vL * qL + pL * n1;
+    double fR0 = hsR * qR, fR1 = hsR * uR * qR + pR * n0, fR2 = hsR * vR * qR + pR * n1;
+    double sL = fabs(qL) / mes + sqrt(g * hsL);
+    double sR = fabs(qR) / mes + sqrt(g * hsR);
+    double s = pmax(sL, sR) * mes;
+    F0 = 0.5 * (fL0 + fR0) - 0.5 * s * (hsR - hsL);
+    F1 = 0.5 * (fL1 + fR1) - 0.5 * s * (hsR * uR - hsL * uL);
+    F2 = 0.5 * (fL2 + fR2) - 0.5 * s * (hsR * vR - hsL * vL);
+    bL = 0.5 * g * (hL * hL - hsL * hsL) + g * (hL + hcL) * 0.5 * (zfL - zcL);
+    bR = 0.5 * g * (hR * hR - hsR * hsR) + g * (hR + hcR) * 0.5 * (zfR - zcR);
+}
+
+// time_step_SW (shallowater/fvm_utils.py:16-42): per-cell candidate, faces in faceid order.
+__device__ __forceinline__ double sw_cell_dt(const Mesh &m, int i, double h, double hu, double hv, double cfl,
+                                             double grav, double Dxx, double Dyy)
+{
+    const double vol = m.cell_volume[i];
+    const double velson = sqrt(grav * h);
+    const int nf = m.cell_nfaces[i];
+    double lam = 0.;
+    for (int j = 0; j < nf; j++) {
+        int f = m.cell_faces[(size_t)j * m.C + i];
+        const double *n = m.face_normal + 3 * (size_t)f;
+        double mesu = m.face_mesure[f];
+        double u_n = fabs(hu / h * n[0] + hv / h * n[1]);
+        double lam_convect = u_n / mesu + velson;
+        lam += lam_convect * mesu;
+        double mes = sqrt(n[0] * n[0] + n[1] * n[1]);
+        double lam_diff = Dxx * (mes * mes) + Dyy * (mes * mes);
+        lam += lam_diff / vol;
+    }
+    return cfl * vol / lam;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sw_time_step(Mesh m, const double *__restrict__ h,
+                                                          const double *__restrict__ hu,
+                                                          const double *__restrict__ hv, double cfl, double grav,
+                                                          double Dxx, double Dyy, double *dt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = (i < m.C) ? sw_cell_dt(m, i, h[i], hu[i], hv[i], cfl, grav, Dxx, Dyy) : 1e300;
+    wave_min_to(dt, d);
+}
+
+// cell_gradient_2d + barthlimiter_2d on eta = h + Z, hu, hv with wall ghosts.
+__global__ void __launch_bounds__(kBlock) k_sw_gradients(Mesh m, const double *__restrict__ h,
+                                                          const double *__restrict__ hu,
+                                                          const double *__restrict__ hv,
+                                                          const double *__restrict__ Z,
+                                                          double *__restrict__ grad)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const size_t C = m.C;
+    const double cx = m.cell_center[3 * i], cy = m.cell_center[3 * i + 1];
+    const double ei = h[i] + Z[i], ui = hu[i], vi = hv[i];
+    double ixx = 0., iyy = 0., ixy = 0.;
+    double je0 = 0., je1 = 0., ju0 = 0., ju1 = 0., jv0 = 0., jv1 = 0.;
+    const int nn = m.lsq_ncells[i];
+    for (int j = 0; j < nn; j++) {
+        int c = m.lsq_cells[(size_t)j * C + i];
+        const double *p = m.cell_center + 3 * (size_t)c;
+        double jx = p[0] - cx, jy = p[1] - cy;
+        ixx += jx * jx;
+        iyy += jy * jy;
+        ixy += (jx * jy);
+        double de = (h[c] + Z[c]) - ei, du = hu[c] - ui, dv = hv[c] - vi;
+        je0 += (jx * de);
+        je1 += (jy * de);
+        ju0 += (jx * du);
+        ju1 += (jy * du);
+        jv0 += (jx * dv);
+        jv1 += (jy * dv);
+    }
+    const int ng = m.lsq_nghosts[i];
+    for (int j = 0; j < ng; j++) {
+        int f = m.lsq_ghosts[(size_t)j * C + i];
+        int o = m.face_cells[2 * f];
+        const double *p = m.face_ghost + 3 * (size_t)f;
+        double jx = p[0] - cx, jy = p[1] - cy;
+        ixx += jx * jx;
+        iyy += jy * jy;
+        ixy += (jx * jy);
+        double de = (h[o] + Z[o]) - ei, du = (-1 * hu[o]) - ui, dv = (-1 * hv[o]) - vi;
+        je0 += (jx * de);
+        je1 += (jy * de);
+        ju0 += (jx * du);
+        ju1 += (jy * du);
+        jv0 += (jx * dv);
+        jv1 += (jy * dv);
+    }
+    const double dia = ixx * iyy - ixy * ixy;
+    const double ex = (iyy * je0 - ixy * je1) / dia, ey = (ixx * je1 - ixy * je0) / dia;
+    const double ux = (iyy * ju0 - ixy * ju1) / dia, uy = (ixx * ju1 - ixy * ju0) / dia;
+    const double vx = (iyy * jv0 - ixy * jv1) / dia, vy = (ixx * jv1 - ixy * jv0) / dia;
+
+    // limiter bounds over the face neighbours (barthlimiter_2d, functions2d.py:289-304)
+    double emax = ei, emin = ei, umax = ui, umin = ui, vmax = vi, vmin = vi;
+    const int nf = m.cell_nfaces[i];
+    for (int j = 0; j < nf; j++) {
+        int f = m.cell_faces[(size_t)j * C + i];
+        int a = m.face_cells[2 * f], b = m.face_cells[2 * f + 1];
+        double ea = h[a] + Z[a], ua = hu[a], va = hv[a];
+        double eb, ub, vb;
+        if (b >= 0) {
+            eb = h[b] + Z[b];
+            ub = hu[b];
+            vb = hv[b];
+        } else {
+            eb = h[a] + Z[a];
+            ub = -1 * hu[a];
+            vb = -1 * hv[a];
+        }
+        emax = pmax(pmax(emax, ea), eb);
+        emin = pmin(pmin(emin, ea), eb);
+        umax = pmax(pmax(umax, ua), ub);
+        umin = pmin(pmin(umin, ua), ub);
+        vmax = pmax(pmax(vmax, va), vb);
+        vmin = pmin(pmin(vmin, va), vb);
+    }
+    double pe = 1., pu = 1., pv = 1., qe = 1., qu = 1., qv = 1.;
+    for (int j = 0; j < nf; j++) {
+        int f = m.cell_faces[(size_t)j * C + i];
+        const double *fc = m.face_center + 3 * (size_t)f;
+        double r1 = fc[0] - cx, r2 = fc[1] - cy;
+        double d;
+        d = ex * r1 + ey * r2;
+        if (fabs(d) < 1e-8) qe = 1.;
+        else {
+            if (d > 0.) qe = pmin(1., (emax - ei) / d);
+            if (d < 0.) qe = pmin(1., (emin - ei) / d);
+        }
+        pe = pmin(pe, qe);
+        d = ux * r1 + uy * r2;
+        if (fabs(d) < 1e-8) qu = 1.;
+        else {
+            if (d > 0.) qu = pmin(1., (umax - ui) / d);
+            if (d < 0.) qu = pmin(1., (umin - ui) / d);
+        }
+        pu = pmin(pu, qu);
+        d = vx * r1 + vy * r2;
+        if (fabs(d) < 1e-8) qv = 1.;
+        else {
+            if (d > 0.) qv = pmin(1., (vmax - vi) / d);
+            if (d < 0.) qv = pmin(1., (vmin - vi) / d);
+        }
+        pv = pmin(pv, qv);
+    }
+    grad[0 * C + i] = ex;
+    grad[1 * C + i] = ey;
+    grad[2 * C + i] = pe;
+    grad[3 * C + i] = ux;
+    grad[4 * C + i] = uy;
+    grad[5 * C + i] = pu;
+    grad[6 * C + i] = vx;
+    grad[7 * C + i] = vy;
+    grad[8 * C + i] = pv;
+}
+
+struct SwSide {
+    double e, u, v, z;
+};
+
+// MUSCL value of cell c at face centre fc (same expressions as orc_sw_residuals).
+__device__ __forceinline__ SwSide sw_reconstruct(const Mesh &m, int c, const double *fc,
+                                                 const double *__restrict__ h, const double *__restrict__ hu,
+                                                 const double *__restrict__ hv, const double *__restrict__ Z,
+                                                 const double *__restrict__ grad,
+                                                 const double *__restrict__ zgrad, double om)
+{
+    const size_t C = m.C;
+    const double *cc = m.cell_center + 3 * (size_t)c;
+    double r0 = fc[0] - cc[0], r1 = fc[1] - cc[1];
+    SwSide s;
+    s.e = (h[c] + Z[c]) + om * grad[2 * C + c] * (grad[0 * C + c] * r0 + grad[1 * C + c] * r1);
+    s.u = hu[c] + om * grad[5 * C + c] * (grad[3 * C + c] * r0 + grad[4 * C + c] * r1);
+    s.v = hv[c] + om * grad[8 * C + c] * (grad[6 * C + c] * r0 + grad[7 * C + c] * r1);
+    s.z = Z[c] + om * (zgrad[c] * r0 + zgrad[C + c] * r1);
+    return s;
+}
+
+// Residual of cell i (convective rez and bathymetry src), face contributions added in the
+// order of the reference's face loops (inner faces by id, then boundary faces by id).
+__device__ __forceinline__ void sw_cell_residual(const Mesh &m, int i, const double *__restrict__ h,
+                                                 const double *__restrict__ hu, const double *__restrict__ hv,
+                                                 const double *__restrict__ Z, const double *__restrict__ grad,
+                                                 const double *__restrict__ zgrad, double g, double om,
+                                                 double (&rez)[3], double (&src)[3])
+{
+    rez[0] = rez[1] = rez[2] = 0.;
+    src[0] = src[1] = src[2] = 0.;
+    const int nf = m.cell_nfaces[i];
+    for (int r = 0; r < nf; r++) {
+        int s = m.order_split[(size_t)r * m.C + i];
+        int f = m.cell_faces[(size_t)s * m.C + i];
+        int a = m.face_cells[2 * f], b = m.face_cells[2 * f + 1];
+        const double *fc = m.face_center + 3 * (size_t)f;
+        const double *n = m.face_normal + 3 * (size_t)f;
+        const double mes = m.face_mesure[f];
+        SwSide L = sw_reconstruct(m, a, fc, h, hu, hv, Z, grad, zgrad, om);
+        double F0, F1, F2, bL, bR;
+        if (b >= 0) {
+            SwSide R = sw_reconstruct(m, b, fc, h, hu, hv, Z, grad, zgrad, om);
+            sw_face(L.e, L.u, L.v, L.z, h[a], Z[a], R.e, R.u, R.v, R.z, h[b], Z[b], n[0], n[1], mes, g, F0, F1, F2,
+                    bL, bR);
+        } else {
+            // wall: Neumann (eta, Z) / nonslip (hu, hv) mirror of the reconstructed face state
+            sw_face(L.e, L.u, L.v, L.z, h[a], Z[a], L.e, -1 * L.u, -1 * L.v, L.z, h[a], Z[a], n[0], n[1], mes, g, F0,
+                    F1, F2, bL, bR);
+        }
+        if (a == i) {
+            rez[0] -= F0;
+            rez[1] -= F1;
+            rez[2] -= F2;
+            src[1] -= bL * n[0];
+            src[2] -= bL * n[1];
+        } else {
+            rez[0] += F0;
+            rez[1] += F1;
+            rez[2] += F2;
+            src[1] -= -(bR * n[0]);
+            src[2] -= -(bR * n[1]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sw_residuals(Mesh m, const double *__restrict__ h,
+                                                          const double *__restrict__ hu,
+                                                          const double *__restrict__ hv,
+                                                          const double *__restrict__ Z,
+                                                          const double *__restrict__ grad,
+                                                          const double *__restrict__ zgrad, double g, double om,
+                                                          double *__restrict__ rez, double *__restrict__ src)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    double r[3], s[3];
+    sw_cell_residual(m, i, h, hu, hv, Z, grad, zgrad, g, om, r, s);
+    const size_t C = m.C;
+    for (int k = 0; k < 3; k++) {
+        rez[k * C + i] = r[k];
+        src[k * C + i] = s[k];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_update_sw(int C, double *__restrict__ h, double *__restrict__ hu,
+                                                       double *__restrict__ hv, const double *__restrict__ rez,
+                                                       const double *__restrict__ src,
+                                                       const double *__restrict__ dt_dev,
+                                                       const double *__restrict__ vol)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const double dt = dt_dev[0];
+    const size_t n = C;
+    h[i] += dt * (rez[i] + src[i]) / vol[i];
+    hu[i] += dt * ((rez[n + i] + src[n + i]) / vol[i] + 0. + 0.);
+    hv[i] += dt * ((rez[2 * n + i] + src[2 * n + i]) / vol[i] + 0. + 0.);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sw_step(Mesh m, const double *__restrict__ Sin,
+                                                     double *__restrict__ Sout, const double *__restrict__ Z,
+                                                     const double *__restrict__ grad,
+                                                     const double *__restrict__ zgrad, double g, double cfl,
+                                                     double om, const double *__restrict__ dt_cur, double *dt_next)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double dloc = 1e300;
+    if (i < m.C) {
+        const size_t C = m.C;
+        const double *h = Sin, *hu = Sin + C, *hv = Sin + 2 * C;
+        double r[3], s[3];
+        sw_cell_residual(m, i, h, hu, hv, Z, grad, zgrad, g, om, r, s);
+        const double dt = dt_cur[0];
+        const double vol = m.cell_volume[i];
+        double hn = h[i] + dt * (r[0] + s[0]) / vol;
+        double un = hu[i] + dt * ((r[1] + s[1]) / vol + 0. + 0.);
+        double vn = hv[i] + dt * ((r[2] + s[2]) / vol + 0. + 0.);
+        Sout[i] = hn;
+        Sout[C + i] = un;
+        Sout[2 * C + i] = vn;
+        dloc = sw_cell_dt(m, i, hn, un, vn, cfl, g, 0., 0.);
+    }
+    wave_min_to(dt_next, dloc);
+}
+
+int finish_dt(double *dt_dev, double *dt_host, void *stream)
+{
+    if (int rc = check_launch("dt kernel")) return rc;
+    if (dt_host) {
+        hipError_t e = hipMemcpyAsync(dt_host, dt_dev, sizeof(double), hipMemcpyDeviceToHost, as_stream(stream));
+        if (e == hipSuccess) e = hipStreamSynchronize(as_stream(stream));
+        if (e != hipSuccess) {
+            set_error("dt readback: %s", hipGetErrorString(e));
+            return MFV_EHIP;
+        }
+    }
+    return MFV_OK;
+}
+
+}  // namespace
+
+extern "C" int mfv_time_step_sw(const mfv_mesh *m, const double *h, const double *hu, const double *hv,
+                                double cfl, double grav, double Dxx, double Dyy, double *dt_dev, double *dt_host,
+                                void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    MFV_REQUIRE(h && hu && hv && dt_dev, "NULL array");
+    if (int rc = mfv_dt_reset(dt_dev, stream)) return rc;
+    if (m->nb_cells)
+        hipLaunchKernelGGL(k_sw_time_step, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                           to_kernel(m), h, hu, hv, cfl, grav, Dxx, Dyy, dt_dev);
+    return finish_dt(dt_dev, dt_host, stream);
+}
+
+extern "C" int mfv_sw_gradients(const mfv_mesh *m, const double *h, const double *hu, const double *hv,
+                                const double *Z, double *grad, void *stream)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(h && hu && hv && Z && grad, "NULL array");
+    hipLaunchKernelGGL(k_sw_gradients, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                       to_kernel(m), h, hu, hv, Z, grad);
+    return check_launch("k_sw_gradients");
+}
+
+extern "C" int mfv_sw_residuals(const mfv_mesh *m, const double *h, const double *hu, const double *hv,
+                                const double *Z, const double *grad, const double *zgrad, double grav, int order,
+                                double *rez, double *src, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 1, 0)) return rc;
+    MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(h && hu && hv && Z && grad && zgrad && rez && src, "NULL array");
+    hipLaunchKernelGGL(k_sw_residuals, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                       to_kernel(m), h, hu, hv, Z, grad, zgrad, grav, (double)(order - 1), rez, src);
+    return check_launch("k_sw_residuals");
+}
+
+extern "C" int mfv_update_sw(const mfv_mesh *m, double *h, double *hu, double *hv, const double *rez,
+                             const double *src, const double *dt_dev, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(h && hu && hv && rez && src && dt_dev, "NULL array");
+    hipLaunchKernelGGL(k_update_sw, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                       m->nb_cells, h, hu, hv, rez, src, dt_dev, m->cell_volume);
+    return check_launch("k_update_sw");
+}
+
+extern "C" int mfv_sw_step_fused(const mfv_mesh *m, const double *state_in, double *state_out, const double *Z,
+                                 const double *grad, const double *zgrad, double grav, double cfl, int order,
+                                 const double *dt_cur, double *dt_next, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 1, 0)) return rc;
+    MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(state_in && state_out && state_in != state_out && Z && grad && zgrad && dt_cur && dt_next,
+                "NULL or aliased array");
+    hipLaunchKernelGGL(k_sw_step, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m),
+                       state_in, state_out, Z, grad, zgrad, grav, cfl, (double)(order - 1), dt_cur, dt_next);
+    return check_launch("k_sw_step");
+}

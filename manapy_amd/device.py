@@ -1,0 +1,147 @@
+"""Device residency: GPU layout of a Domain and the mfv_mesh descriptor.
+
+``gpu_layout(domain)`` regroups the reference arrays for coalesced access
+(per-cell tables become column-major ``[slot][cell]``) and precomputes the two
+residual orders that make the cell-centric gather reproduce the reference's
+face-ordered scatters:
+
+* ``order_split`` -- inner faces by increasing id, then boundary faces by
+  increasing id (the loop order of ``explicitscheme_convective_2d``,
+  ``solvers/advec/fvm_utils.py:52-152``, and ``explicitscheme_convective_SW``,
+  ``solvers/shallowater/fvm_utils.py:363-496``);
+* ``order_sorted`` -- all faces by increasing id (the single loop of
+  ``explicitscheme_euler_3d_rusanov``, ``solvers/euler/fvm_utils3d.py:1101``).
+
+``DeviceMesh`` uploads the layout with PyTorch (device memory and streams
+only) and exposes the ``mfv_mesh`` descriptor passed to every C-ABI call.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+I32 = np.int32
+
+
+def require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise _lib.MfvError("no ROCm GPU visible: the manapy_amd kernels run on the GPU only")
+    return torch
+
+
+def gpu_layout(domain, lsq=True):
+    """Host-side arrays of the device mesh (numpy)."""
+    dim = int(domain.dim)
+    C = int(domain.nbcells)
+    F = int(domain.nbfaces)
+    faceid = domain.cells.faceid
+    MF = faceid.shape[1] - 1
+    nf = faceid[:, -1].astype(I32)
+    fids = faceid[:, :MF]
+    name = domain.faces.name.astype(I32)
+    bad = np.nonzero((domain.faces.cellid[:, 1] == -1) & ~np.isin(name, [1, 2, 3, 4, 5, 6]))[0]
+    if len(bad):
+        raise ValueError("%d boundary faces have no boundary tag (name %s): the reference skips such faces "
+                         "in every loop; tag the mesh boundary" % (len(bad), np.unique(name[bad])))
+    valid = np.arange(MF)[None, :] < nf[:, None]
+    big = np.int64(1) << 40
+    fid64 = np.where(valid, fids, big).astype(np.int64)
+    is_bnd = (name[np.where(valid, fids, 0)] != 0) & valid
+    order_split = np.argsort(fid64 + is_bnd.astype(np.int64) * (np.int64(1) << 32), axis=1, kind='stable')
+    order_sorted = np.argsort(fid64, axis=1, kind='stable')
+    out = {
+        'dim': dim, 'C': C, 'F': F, 'MF': MF,
+        'face_cells': np.ascontiguousarray(domain.faces.cellid, dtype=I32),
+        'face_name': name,
+        'face_normal': np.ascontiguousarray(domain.faces.normal[:, :3], dtype=np.float64),
+        'face_mesure': np.ascontiguousarray(domain.faces.mesure, dtype=np.float64),
+        'face_center': np.ascontiguousarray(domain.faces.center[:, :3], dtype=np.float64),
+        'cell_center': np.ascontiguousarray(domain.cells.center[:, :3], dtype=np.float64),
+        'cell_volume': np.ascontiguousarray(domain.cells.volume, dtype=np.float64),
+        'cell_faces': np.ascontiguousarray(fids.T, dtype=I32),
+        'cell_nfaces': nf,
+        'order_split': np.ascontiguousarray(order_split.T, dtype=np.uint8),
+        'order_sorted': np.ascontiguousarray(order_sorted.T, dtype=np.uint8),
+        'bnd_faces': np.ascontiguousarray(domain.boundaryfaces, dtype=I32),
+    }
+    gc = domain.faces.ghostcenter
+    ghost = np.zeros((F, 3), dtype=np.float64)
+    ghost[:, :min(3, gc.shape[1]) if dim == 3 else 2] = gc[:, :3 if dim == 3 else 2]
+    out['face_ghost'] = ghost
+    if lsq:
+        cnid = domain.cells.cellnid
+        gnid = domain.cells.ghostnid
+        K = cnid.shape[1] - 1
+        G = gnid.shape[1] - 1
+        out['K'] = K
+        out['G'] = G
+        out['lsq_cells'] = np.ascontiguousarray(cnid[:, :K].T, dtype=I32) if K else np.zeros((1, C), I32)
+        out['lsq_ncells'] = np.ascontiguousarray(cnid[:, -1], dtype=I32)
+        out['lsq_ghosts'] = np.ascontiguousarray(gnid[:, :G].T, dtype=I32) if G else np.zeros((1, C), I32)
+        out['lsq_nghosts'] = np.ascontiguousarray(gnid[:, -1], dtype=I32)
+    return out
+
+
+def dptr(t):
+    """Device address of a contiguous CUDA/ROCm tensor (0 for None)."""
+    if t is None:
+        return None
+    if not t.is_cuda or not t.is_contiguous():
+        raise _lib.MfvError("expected a contiguous device tensor")
+    return t.data_ptr()
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+class DeviceMesh(object):
+    """Device copy of a Domain's hot-path arrays and its ``mfv_mesh`` descriptor."""
+
+    def __init__(self, domain, lsq=True, device=None):
+        torch = require_gpu()
+        self.domain = domain
+        self.device = torch.device(device if device is not None else 'cuda')
+        host = gpu_layout(domain, lsq=lsq)
+        self.host = host
+        self.dim = host['dim']
+        self.C = host['C']
+        self.F = host['F']
+        self.MF = host['MF']
+        self.t = {}
+        for k, v in host.items():
+            if isinstance(v, np.ndarray):
+                self.t[k] = torch.from_numpy(np.ascontiguousarray(v)).to(self.device)
+        d = _lib.MfvMesh()
+        d.dim = self.dim
+        d.nb_cells = self.C
+        d.nb_faces = self.F
+        d.max_cell_faces = self.MF
+        d.lsq_width = host.get('K', 0)
+        d.ghost_width = host.get('G', 0)
+        for k in ('face_cells', 'face_name', 'face_normal', 'face_mesure', 'face_center', 'face_ghost',
+                  'cell_center', 'cell_volume', 'cell_faces', 'cell_nfaces', 'order_split', 'order_sorted',
+                  'bnd_faces'):
+            setattr(d, k, self.t[k].data_ptr())
+        if lsq:
+            for k in ('lsq_cells', 'lsq_ncells', 'lsq_ghosts', 'lsq_nghosts'):
+                setattr(d, k, self.t[k].data_ptr())
+        d.nb_bnd = len(host['bnd_faces'])
+        self.desc = d
+        self.lsq = lsq
+
+    @property
+    def ptr(self):
+        return ctypes.byref(self.desc)
+
+    def zeros(self, *shape, dtype=None):
+        import torch
+        return torch.zeros(*shape, dtype=dtype or torch.float64, device=self.device)
+
+    def to_device(self, a):
+        import torch
+        return torch.as_tensor(np.ascontiguousarray(a), device=self.device)

@@ -1,0 +1,254 @@
+"""ORACLE (test infrastructure only): numpy front-end of liboracle.so.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg use this module, and only as the checker / CPU baseline -- the product path
+(``manapy_amd``) never imports it.
+
+Each wrapper takes the reference's arrays (``Domain`` attributes and numpy
+fields) and runs the C restatement in ``oracle/mfv_oracle.c``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, 'liboracle.so')
+_lib = None
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_P = ctypes.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            from .build import build_oracle
+            build_oracle()
+        _lib = ctypes.CDLL(LIB)
+        _lib.orc_time_step.restype = _D
+        _lib.orc_time_step_sw.restype = _D
+    return _lib
+
+
+def _p(a, dtype):
+    assert a.dtype == dtype and a.flags['C_CONTIGUOUS'], (a.dtype, dtype)
+    return a.ctypes.data_as(_P)
+
+
+def f64(a):
+    return _p(a, np.float64)
+
+
+def i32(a):
+    return _p(a, np.int32)
+
+
+def c3(a):
+    """[n, >=3] float array -> contiguous [n, 3]."""
+    return np.ascontiguousarray(a[:, :3], dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------------------------
+def time_step(d, u, v, w, cfl):
+    fid = d.cells.faceid
+    return lib().orc_time_step(f64(u), f64(v), f64(w), _D(cfl), f64(c3(d.faces.normal)), f64(d.cells.volume),
+                               i32(fid), _I(fid.shape[1]), _I(int(d.nbcells)))
+
+
+def update_new_value(ne_c, rez, dissip, src, dtime, vol):
+    lib().orc_update_new_value(f64(ne_c), f64(rez), f64(dissip), f64(src), _D(dtime), f64(vol), _I(len(ne_c)))
+
+
+def facetocell(d, u_face, u_c):
+    fid = d.cells.faceid
+    lib().orc_facetocell(f64(u_face), f64(u_c), i32(fid), _I(fid.shape[1]), _I(int(d.nbcells)))
+
+
+def ghost_value(d, kind, src, w_ghost, faces):
+    faces = np.ascontiguousarray(faces, dtype=np.int32)
+    lib().orc_ghost_value(_I(kind), f64(src), f64(w_ghost), i32(d.faces.cellid), i32(faces), _I(len(faces)))
+
+
+def explicitscheme_convective(d, rez, w_c, w_ghost, uf, vf, wf, wx, wy, wz, psi, order):
+    lib().orc_explicitscheme_convective(
+        _I(int(d.dim)), f64(rez), f64(w_c), f64(w_ghost), f64(uf), f64(vf), f64(wf), f64(wx), f64(wy), f64(wz),
+        f64(psi), f64(c3(d.cells.center)), f64(c3(d.faces.center)), i32(d.faces.cellid), f64(c3(d.faces.normal)),
+        i32(d.innerfaces), _I(len(d.innerfaces)), i32(d.boundaryfaces), _I(len(d.boundaryfaces)), _I(order),
+        _I(int(d.nbcells)))
+
+
+def cell_gradient(d, w_c, w_ghost, wx, wy, wz):
+    cn = d.cells.cellnid
+    gn = d.cells.ghostnid
+    gc = np.ascontiguousarray(d.faces.ghostcenter, dtype=np.float64)
+    fn = lib().orc_cell_gradient_2d if int(d.dim) == 2 else lib().orc_cell_gradient_3d
+    fn(f64(w_c), f64(w_ghost), f64(c3(d.cells.center)), i32(cn), _I(cn.shape[1]), i32(gn), _I(gn.shape[1]),
+       f64(gc), _I(gc.shape[1]), f64(wx), f64(wy), f64(wz), _I(int(d.nbcells)))
+
+
+def barthlimiter(d, w_c, w_ghost, wx, wy, wz, psi):
+    fid = d.cells.faceid
+    if int(d.dim) == 2:
+        lib().orc_barthlimiter_2d(f64(w_c), f64(w_ghost), f64(wx), f64(wy), f64(psi), i32(d.faces.cellid), i32(fid),
+                                  _I(fid.shape[1]), i32(d.faces.name), f64(c3(d.cells.center)),
+                                  f64(c3(d.faces.center)), _I(int(d.nbcells)))
+    else:
+        lib().orc_barthlimiter_3d(f64(w_c), f64(w_ghost), f64(wx), f64(wy), f64(wz), f64(psi), i32(d.faces.cellid),
+                                  i32(fid), _I(fid.shape[1]), i32(d.faces.name), f64(c3(d.cells.center)),
+                                  f64(c3(d.faces.center)), _I(int(d.nbcells)))
+
+
+# ---------------------------------------------------------------------------------------------
+def time_step_sw(d, h, hu, hv, cfl, grav, Dxx=0., Dyy=0.):
+    fid = d.cells.faceid
+    return lib().orc_time_step_sw(f64(h), f64(hu), f64(hv), _D(cfl), f64(c3(d.faces.normal)), f64(d.faces.mesure),
+                                  f64(d.cells.volume), i32(fid), _I(fid.shape[1]), _D(grav), _D(Dxx), _D(Dyy),
+                                  _I(int(d.nbcells)))
+
+
+def update_sw(h, hu, hv, rez, src, dt, vol):
+    lib().orc_update_sw(f64(h), f64(hu), f64(hv), f64(rez[0]), f64(rez[1]), f64(rez[2]), f64(src[0]), f64(src[1]),
+                        f64(src[2]), _D(dt), f64(vol), _I(len(h)))
+
+
+def sw_face(etaL, huL, hvL, zfL, hcL, zcL, etaR, huR, hvR, zfR, hcR, zcR, n0, n1, mes, g):
+    fl = np.zeros(3)
+    sL = np.zeros(2)
+    sR = np.zeros(2)
+    lib().orc_sw_face(*[_D(x) for x in (etaL, huL, hvL, zfL, hcL, zcL, etaR, huR, hvR, zfR, hcR, zcR, n0, n1, mes, g)],
+                      f64(fl), f64(sL), f64(sR))
+    return fl, sL, sR
+
+
+def sw_ghosts(d, h, hu, hv, Z):
+    """Wall ghosts: h Neumann, hu/hv nonslip, Z Neumann, on every boundary face."""
+    F = int(d.nbfaces)
+    bf = d.boundaryfaces
+    out = []
+    for kind, w in ((0, h), (1, hu), (1, hv), (0, Z)):
+        g = np.zeros(F)
+        ghost_value(d, kind, w, g, bf)
+        out.append(g)
+    return out
+
+
+def sw_gradients(d, h, hu, hv, Z):
+    """Limited LSQ gradients of eta = h + Z, hu, hv ([9][C]) with wall ghosts."""
+    C = int(d.nbcells)
+    hg, hug, hvg, zg = sw_ghosts(d, h, hu, hv, Z)
+    eta = h + Z
+    etag = hg + zg
+    grad = np.zeros((9, C))
+    wz = np.zeros(C)
+    for k, (w, wg) in enumerate(((eta, etag), (hu, hug), (hv, hvg))):
+        gx = np.zeros(C)
+        gy = np.zeros(C)
+        psi = np.zeros(C)
+        cell_gradient(d, w, wg, gx, gy, wz)
+        barthlimiter(d, w, wg, gx, gy, wz, psi)
+        grad[3 * k], grad[3 * k + 1], grad[3 * k + 2] = gx, gy, psi
+    return grad
+
+
+def z_gradient(d, Z):
+    C = int(d.nbcells)
+    zg = np.zeros(int(d.nbfaces))
+    ghost_value(d, 0, Z, zg, d.boundaryfaces)
+    gx, gy, gz = np.zeros(C), np.zeros(C), np.zeros(C)
+    cell_gradient(d, Z, zg, gx, gy, gz)
+    return np.stack([gx, gy])
+
+
+def sw_residuals(d, h, hu, hv, Z, grad, zgrad, grav, order):
+    C = int(d.nbcells)
+    hg, hug, hvg, zg = sw_ghosts(d, h, hu, hv, Z)
+    rez = np.zeros((3, C))
+    src = np.zeros((3, C))
+    g = [np.ascontiguousarray(x) for x in grad]
+    zx, zy = np.ascontiguousarray(zgrad[0]), np.ascontiguousarray(zgrad[1])
+    lib().orc_sw_residuals(
+        f64(rez[0]), f64(rez[1]), f64(rez[2]), f64(src[0]), f64(src[1]), f64(src[2]), f64(h), f64(hu), f64(hv),
+        f64(Z), f64(hg), f64(hug), f64(hvg), f64(zg), *[f64(x) for x in g], f64(zx), f64(zy),
+        f64(c3(d.cells.center)), f64(c3(d.faces.center)), i32(d.faces.cellid), f64(c3(d.faces.normal)),
+        f64(d.faces.mesure), i32(d.innerfaces), _I(len(d.innerfaces)), i32(d.boundaryfaces),
+        _I(len(d.boundaryfaces)), _D(grav), _I(order), _I(C))
+    return rez, src
+
+
+def sw_step(d, h, hu, hv, Z, zgrad, grav, cfl, order):
+    """One step of the driver sequence stepper -> compute_fluxes -> compute_new_val
+    (examples/2D/shallowwater2d.py:99-107), in place. Returns the dt used."""
+    dt = time_step_sw(d, h, hu, hv, cfl, grav)
+    if order == 2:
+        grad = sw_gradients(d, h, hu, hv, Z)
+    else:
+        grad = np.zeros((9, int(d.nbcells)))
+    rez, src = sw_residuals(d, h, hu, hv, Z, grad, zgrad, grav, order)
+    update_sw(h, hu, hv, rez, src, dt, d.cells.volume)
+    return dt
+
+
+def advection_step(d, w, u_c, v_c, w_c, uf, vf, wf, cfl, order):
+    """AdvectionSolver.stepper -> compute_fluxes -> compute_new_val with all-Neumann BCs
+    (solvers/advec/system.py:129-147), in place. Returns dt."""
+    C = int(d.nbcells)
+    dt = time_step(d, u_c, v_c, w_c, cfl)
+    ghost = np.zeros(int(d.nbfaces))
+    ghost_value(d, 0, w, ghost, d.boundaryfaces)
+    wx, wy, wz, psi = np.zeros(C), np.zeros(C), np.zeros(C), np.zeros(C)
+    if order == 2:
+        cell_gradient(d, w, ghost, wx, wy, wz)
+        barthlimiter(d, w, ghost, wx, wy, wz, psi)
+    rez = np.zeros(C)
+    explicitscheme_convective(d, rez, w, ghost, uf, vf, wf, wx, wy, wz, psi, order)
+    z = np.zeros(C)
+    update_new_value(w, rez, z, z, dt, d.cells.volume)
+    return dt
+
+
+# ---------------------------------------------------------------------------------------------
+def euler_ghost(d, U, G, quirk):
+    C = int(d.nbcells)
+    F = int(d.nbfaces)
+    Uv = [U[k] for k in range(6)]
+    Gv = [G[k] for k in range(6)]
+    # argument order of the C function: rhog, Pg, rhoug, rhovg, rhowg, rhoEg, then cell arrays
+    lib().orc_ghost_tubeschok3d(*[f64(g) for g in Gv], *[f64(u) for u in Uv], i32(d.faces.cellid),
+                                i32(d.faces.name.astype(np.int32)), f64(c3(d.faces.normal)), _I(F), _I(quirk))
+    del C
+
+
+def euler_dt(d, U, cfl, gamma):
+    C = int(d.nbcells)
+    fid = d.cells.faceid
+    dtc = np.zeros(C)
+    lib().orc_time_step_euler_3d(f64(U[0]), f64(U[1]), f64(U[2]), f64(U[3]), f64(U[4]), _D(cfl),
+                                 f64(c3(d.faces.normal)), f64(d.faces.mesure), f64(d.cells.volume), i32(fid),
+                                 _I(fid.shape[1]), _D(gamma), f64(dtc), _I(C))
+    return dtc
+
+
+def euler_rusanov(d, U, G, gamma, quirk):
+    C = int(d.nbcells)
+    rez = np.zeros((5, C))
+    lib().orc_euler3d_rusanov(f64(rez), f64(np.ascontiguousarray(U)), f64(np.ascontiguousarray(G)),
+                              i32(d.faces.cellid), f64(c3(d.faces.normal)), f64(d.faces.mesure),
+                              i32(d.faces.name.astype(np.int32)), _D(gamma), _I(quirk), _I(C), _I(int(d.nbfaces)))
+    return rez
+
+
+def euler_update(d, U, rez, gamma, dt):
+    lib().orc_update_euler_3d(f64(U), f64(np.ascontiguousarray(rez)), _D(gamma), _D(dt), f64(d.cells.volume),
+                              _I(int(d.nbcells)))
+
+
+def euler_step(d, U, G, cfl, gamma, quirk):
+    """ghost -> dt (min of dt_c) -> Rusanov residual -> update, in place on U [6][C] and G [6][F]."""
+    euler_ghost(d, U, G, quirk)
+    dt = float(np.min(np.concatenate([[1e6], euler_dt(d, U, cfl, gamma)])))
+    rez = euler_rusanov(d, U, G, gamma, quirk)
+    euler_update(d, U, rez, gamma, dt)
+    return dt

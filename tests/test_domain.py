@@ -1,0 +1,169 @@
+"""Domain construction: the vectorised builder against the loop restatement of
+manapy's construction, plus geometric invariants that pin both."""
+import numpy as np
+import pytest
+
+from manapy_amd.ddm import Domain
+from manapy_amd.meshgen import jittered_triangles, kuhn_tetrahedra
+from oracle.ddm_loops import build_domain_loops
+
+from conftest import make_tri_domain, make_tet_domain
+
+KEYS_2D = [
+    ('center', lambda d: d.cells.center), ('volume', lambda d: d.cells.volume),
+    ('nodes_cellid', lambda d: d.nodes.cellid), ('cellnid', lambda d: d.cells.cellnid),
+    ('faces_nodeid_unoriented', lambda d: d.faces.nodeid), ('faceid', lambda d: d.cells.faceid),
+    ('faces_cellid', lambda d: d.faces.cellid), ('cellfid', lambda d: d.cells.cellfid),
+    ('faces_name', lambda d: d.faces.name), ('faces_normal', lambda d: d.faces.normal),
+    ('faces_mesure', lambda d: d.faces.mesure), ('faces_center', lambda d: d.faces.center),
+    ('innerfaces', lambda d: d.innerfaces), ('boundaryfaces', lambda d: d.boundaryfaces),
+    ('faces_ghostcenter', lambda d: d.faces.ghostcenter), ('ghostid', lambda d: d.nodes.ghostid),
+    ('ghostnid', lambda d: d.cells.ghostnid), ('number', lambda d: d.nodes.number),
+    ('R_x', lambda d: d.nodes.R_x), ('R_y', lambda d: d.nodes.R_y),
+    ('lambda_x', lambda d: d.nodes.lambda_x), ('lambda_y', lambda d: d.nodes.lambda_y),
+    ('airDiamond', lambda d: d.faces.airDiamond),
+]
+
+
+@pytest.mark.parametrize("nx,ny,lx,ly,seed", [(3, 2, 1.0, 1.0, 0), (7, 5, 2.0, 1.0, 3), (12, 12, 100., 100., 7)])
+def test_domain_2d_matches_loop_restatement(nx, ny, lx, ly, seed):
+    el, nd = jittered_triangles(nx, ny, lx, ly, 0.1, seed)
+    d = Domain(2, mesh=(el, nd))
+    o = build_domain_loops(el, nd, 2)
+    for key, get in KEYS_2D:
+        a, b = get(d), o[key]
+        assert a.shape == b.shape, key
+        assert np.array_equal(a, b, equal_nan=True), key
+    for k, name in enumerate(('param1', 'param2', 'param3', 'param4')):
+        assert np.array_equal(getattr(d.faces, name), o['params'][k]), name
+
+
+def test_domain_3d_matches_loop_restatement():
+    el, nd = kuhn_tetrahedra(3, 1.0, 0.08, 3)
+    d = Domain(3, mesh=(el, nd))
+    o = build_domain_loops(el, nd, 3)
+    pairs = [('center', d.cells.center), ('volume', d.cells.volume), ('faceid', d.cells.faceid),
+             ('faces_cellid', d.faces.cellid), ('cellfid', d.cells.cellfid), ('faces_name', d.faces.name),
+             ('faces_normal', d.faces.normal), ('faces_mesure', d.faces.mesure), ('faces_center', d.faces.center),
+             ('cellnid', d.cells.cellnid), ('nodes_cellid', d.nodes.cellid), ('boundaryfaces', d.boundaryfaces),
+             ('innerfaces', d.innerfaces)]
+    for key, a in pairs:
+        assert np.array_equal(a, o[key]), key
+
+
+def _closure(d):
+    """sum over a cell's faces of the outward normal (must vanish for closed cells)."""
+    C = int(d.nbcells)
+    s = np.zeros((C, 3))
+    fid = d.cells.faceid
+    for j in range(fid.shape[1] - 1):
+        rows = np.nonzero(fid[:, -1] > j)[0]
+        f = fid[rows, j]
+        sign = np.where(d.faces.cellid[f, 0] == rows, 1.0, -1.0)
+        s[rows] += sign[:, None] * d.faces.normal[f]
+    return s
+
+
+def test_invariants_2d(dom2d):
+    d = dom2d
+    # |n| = edge length, normals point out of cellid[f][0]
+    fn = d.faces.nodeid
+    v = d.nodes.vertex
+    elen = np.hypot(v[fn[:, 0], 0] - v[fn[:, 1], 0], v[fn[:, 0], 1] - v[fn[:, 1], 1])
+    assert np.allclose(d.faces.mesure, elen, rtol=1e-14, atol=0)
+    out = np.einsum('ij,ij->i', d.faces.center[:, :2] - d.cells.center[d.faces.cellid[:, 0], :2],
+                    d.faces.normal[:, :2])
+    assert np.all(out > 0)
+    assert np.abs(_closure(d)).max() < 1e-14
+    # total area = box area, boundary faces are exactly the perimeter
+    assert abs(d.cells.volume.sum() - 1.0) < 1e-13
+    assert abs(d.faces.mesure[d.boundaryfaces].sum() - 4.0) < 1e-13
+    assert set(np.unique(d.faces.name[d.boundaryfaces])) == {1, 2, 3, 4}
+    # ghost centre = mirror of the owner centre across the face line: midpoint on the line,
+    # displacement parallel to the normal
+    bf = d.boundaryfaces
+    g = d.faces.ghostcenter[bf, :2]
+    c = d.cells.center[d.faces.cellid[bf, 0], :2]
+    mid = 0.5 * (g + c)
+    p0 = v[fn[bf, 0], :2]
+    n = d.faces.normal[bf, :2]
+    assert np.abs(np.einsum('ij,ij->i', mid - p0, n)).max() < 1e-14
+    t = np.stack([-n[:, 1], n[:, 0]], axis=1)
+    assert np.abs(np.einsum('ij,ij->i', g - c, t)).max() < 1e-14
+    # cellnid: symmetric vertex adjacency, sorted, self excluded
+    cn = d.cells.cellnid
+    for i in range(0, int(d.nbcells), 37):
+        row = cn[i, :cn[i, -1]]
+        assert np.all(np.diff(row) > 0) and i not in row
+        for j in row:
+            assert i in cn[j, :cn[j, -1]]
+
+
+def test_pseudo_laplacian_weights_reproduce_affine_fields(dom2d):
+    """centertovertex_2d weights (ast/functions2d.py:199-277) interpolate affine fields exactly."""
+    d = dom2d
+    a, b, c = 0.3, -1.7, 2.2
+    f = lambda x, y: a + b * x + c * y
+    wc = f(d.cells.center[:, 0], d.cells.center[:, 1])
+    gc = d.faces.ghostcenter
+    wg = np.where(gc[:, 0] != -1, f(gc[:, 0], gc[:, 1]), 0.)
+    N = int(d.nbnodes)
+    wn = np.zeros(N)
+    vx, vy = d.nodes.vertex[:, 0], d.nodes.vertex[:, 1]
+    lx, ly = d.nodes.lambda_x, d.nodes.lambda_y
+    den = d.nodes.number + lx * d.nodes.R_x + ly * d.nodes.R_y
+    for arr, pts, vals in ((d.nodes.cellid, d.cells.center, wc), (d.nodes.ghostid, gc, wg)):
+        for j in range(arr.shape[1] - 1):
+            rows = np.nonzero(arr[:, -1] > j)[0]
+            k = arr[rows, j]
+            alpha = (1. + lx[rows] * (pts[k, 0] - vx[rows]) + ly[rows] * (pts[k, 1] - vy[rows])) / den[rows]
+            wn[rows] += alpha * vals[k]
+    assert np.abs(wn - f(vx, vy)).max() < 1e-12
+
+
+def test_invariants_3d(dom3d):
+    d = dom3d
+    assert abs(d.cells.volume.sum() - 1.0) < 1e-13
+    assert np.abs(_closure(d)).max() < 1e-14
+    out = np.einsum('ij,ij->i', d.faces.center - d.cells.center[d.faces.cellid[:, 0]], d.faces.normal)
+    assert np.all(out > 0)
+    assert abs(d.faces.mesure[d.boundaryfaces].sum() - 6.0) < 1e-12
+    assert set(np.unique(d.faces.name[d.boundaryfaces])) == {1, 2, 3, 4, 5, 6}
+    # Kuhn split: every interior tet has 4 face neighbours
+    assert d.cells.faceid[:, -1].min() == 4
+
+
+def test_boundary_tags_follow_partition_priority():
+    """x tags (1, 2) win over y tags (3, 4) at corners (partitions_utils.py:39-51)."""
+    el, nd = jittered_triangles(4, 4, 1.0, 1.0, 0.1, 0)
+    tag = nd[:, 3]
+    corners = [(0., 0., 1), (1., 0., 2), (1., 1., 2), (0., 1., 1)]
+    for x, y, t in corners:
+        k = np.nonzero((nd[:, 0] == x) & (nd[:, 1] == y))[0][0]
+        assert tag[k] == t
+
+
+def test_unclassified_boundary_rejected_by_device_layout():
+    from manapy_amd.device import gpu_layout
+    el, nd = jittered_triangles(3, 3, 1.0, 1.0, 0.0, 0)
+    nd = nd.copy()
+    nd[:, 3] = 0  # untagged boundary: its faces get name 0 with a single neighbour cell
+    d = Domain(2, mesh=(el, nd))
+    with pytest.raises(ValueError):
+        gpu_layout(d)
+
+
+def test_order_tables(dom2d):
+    from manapy_amd.device import gpu_layout
+    d = dom2d
+    L = gpu_layout(d)
+    C = int(d.nbcells)
+    fid = d.cells.faceid
+    for i in range(0, C, 13):
+        nf = fid[i, -1]
+        faces = fid[i, :nf]
+        split = [faces[L['order_split'][r, i]] for r in range(nf)]
+        inner = sorted(f for f in faces if d.faces.name[f] == 0)
+        bnd = sorted(f for f in faces if d.faces.name[f] != 0)
+        assert split == inner + bnd
+        assert [faces[L['order_sorted'][r, i]] for r in range(nf)] == sorted(faces)

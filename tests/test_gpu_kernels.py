@@ -1,0 +1,315 @@
+"""GPU parity: every libmfv entry point, called through the C ABI, against the CPU
+restatement (oracle) on the same seeded inputs. The bar is bit-exact float64 for
+each kernel and after N fused steps: the device kernels evaluate the reference's
+expressions in the reference's order with no contraction."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available, make_tri_domain, make_tet_domain
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    return t
+
+
+@pytest.fixture(scope="module")
+def d2():
+    return make_tri_domain(40, 30, 1.0, 1.0, 0.1, 21)
+
+
+@pytest.fixture(scope="module")
+def d3():
+    return make_tet_domain(6, 0.08, 9)
+
+
+@pytest.fixture(scope="module")
+def m2(d2):
+    from manapy_amd.device import DeviceMesh
+    return DeviceMesh(d2)
+
+
+@pytest.fixture(scope="module")
+def m3(d3):
+    from manapy_amd.device import DeviceMesh
+    return DeviceMesh(d3)
+
+
+def dev(torch, a):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def call(name, *args):
+    from manapy_amd import _lib
+    from manapy_amd.device import stream_ptr
+    _lib.call(name, *args, stream_ptr())
+
+
+def P(t):
+    """Device address of a tensor the caller keeps alive until the call has run: a temporary
+    would go back to the caching allocator and its block could be handed to the next one."""
+    return t.data_ptr()
+
+
+def same(a, b, what=""):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, what
+    diff = np.nonzero(~((a == b) | (np.isnan(a) & np.isnan(b))))[0] if a.ndim == 1 else None
+    assert np.array_equal(a, b, equal_nan=True), "%s: %d differ, max |diff| %g" % (
+        what, int(np.sum(a != b)), float(np.nanmax(np.abs(a - b))))
+    del diff
+
+
+def field(d, seed, scale=1.0, offset=0.0):
+    rng = np.random.default_rng(seed)
+    c = d.cells.center
+    return offset + scale * (np.sin(5 * c[:, 0] + 1) * np.cos(3 * c[:, 1]) + 0.05 * rng.standard_normal(len(c)))
+
+
+# ------------------------------------------------------------------------------- generic operators
+def test_ghost_and_facetocell(torch, d2, m2, oracle_lib):
+    C, F = int(d2.nbcells), int(d2.nbfaces)
+    w = field(d2, 1)
+    bf = d2.boundaryfaces
+    tbf = torch.as_tensor(bf).cuda()
+    for kind, src in ((0, w), (1, w), (2, np.random.default_rng(2).random(F))):
+        g_ref = np.full(F, 3.0)
+        oracle_lib.ghost_value(d2, kind, src, g_ref, bf)
+        g = dev(torch, np.full(F, 3.0))
+        tsrc = dev(torch, src)
+        call('mfv_ghost_value', m2.ptr, kind, P(tsrc), P(g), P(tbf), len(bf))
+        same(host(g), g_ref, "ghost kind %d" % kind)
+    uf = np.random.default_rng(3).random(F)
+    uc_ref = np.zeros(C)
+    oracle_lib.facetocell(d2, uf, uc_ref)
+    uc = dev(torch, np.zeros(C))
+    tuf = dev(torch, uf)
+    call('mfv_facetocell', m2.ptr, P(tuf), P(uc))
+    same(host(uc), uc_ref, "facetocell")
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gradient_and_limiter(torch, dim, d2, m2, d3, m3, oracle_lib):
+    d, m = (d2, m2) if dim == 2 else (d3, m3)
+    C, F = int(d.nbcells), int(d.nbfaces)
+    w = field(d, 4)
+    g = np.zeros(F)
+    oracle_lib.ghost_value(d, 0, w, g, d.boundaryfaces)
+    ref = [np.zeros(C) for _ in range(4)]
+    oracle_lib.cell_gradient(d, w, g, ref[0], ref[1], ref[2])
+    oracle_lib.barthlimiter(d, w, g, ref[0], ref[1], ref[2], ref[3])
+    out = [dev(torch, np.zeros(C)) for _ in range(4)]
+    tw, tg = dev(torch, w), dev(torch, g)
+    call('mfv_cell_gradient', m.ptr, P(tw), P(tg), P(out[0]), P(out[1]), P(out[2]))
+    call('mfv_barthlimiter', m.ptr, P(tw), P(tg), P(out[0]), P(out[1]), P(out[2]), P(out[3]))
+    for k, nm in enumerate(('w_x', 'w_y', 'w_z', 'psi')):
+        same(host(out[k]), ref[k], nm)
+    assert ref[3].min() < 1.0
+
+
+# ------------------------------------------------------------------------------- advection
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2)])
+def test_advection_kernels(torch, dim, order, d2, m2, d3, m3, oracle_lib):
+    d, m = (d2, m2) if dim == 2 else (d3, m3)
+    C, F = int(d.nbcells), int(d.nbfaces)
+    w = field(d, 5)
+    rng = np.random.default_rng(6)
+    uf, vf, wf = rng.uniform(-1, 1, F), rng.uniform(-1, 1, F), (rng.uniform(-1, 1, F) if dim == 3 else np.zeros(F))
+    g = np.zeros(F)
+    oracle_lib.ghost_value(d, 0, w, g, d.boundaryfaces)
+    wx, wy, wz, psi = (np.zeros(C) for _ in range(4))
+    oracle_lib.cell_gradient(d, w, g, wx, wy, wz)
+    oracle_lib.barthlimiter(d, w, g, wx, wy, wz, psi)
+    rez_ref = np.zeros(C)
+    oracle_lib.explicitscheme_convective(d, rez_ref, w, g, uf, vf, wf, wx, wy, wz, psi, order)
+    rez = dev(torch, np.zeros(C))
+    args = [dev(torch, a) for a in (w, g, uf, vf, wf, wx, wy, wz, psi)]
+    call('mfv_explicitscheme_convective', m.ptr, P(rez), *[P(a) for a in args], order)
+    same(host(rez), rez_ref, "rez")
+    # time step (cell velocity) and update
+    uc, vc, wc = rng.uniform(-1, 1, C), rng.uniform(-1, 1, C), rng.uniform(-1, 1, C)
+    dt_ref = oracle_lib.time_step(d, uc, vc, wc, 0.8)
+    dtd = dev(torch, np.zeros(1))
+    dth = ctypes.c_double(0.)
+    tuc, tvc, twc = dev(torch, uc), dev(torch, vc), dev(torch, wc)
+    call('mfv_time_step', m.ptr, P(tuc), P(tvc), P(twc), 0.8, P(dtd), ctypes.byref(dth))
+    assert dth.value == dt_ref and host(dtd)[0] == dt_ref
+    dis, src = rng.random(C), rng.random(C)
+    w_ref = w.copy()
+    oracle_lib.update_new_value(w_ref, rez_ref, dis, src, dt_ref, d.cells.volume)
+    tw = dev(torch, w)
+    tdis, tsrc = dev(torch, dis), dev(torch, src)
+    call('mfv_update_new_value', m.ptr, P(tw), P(rez), P(tdis), P(tsrc), 0.0, P(dtd))
+    same(host(tw), w_ref, "update")
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_advection_fused_100_steps(torch, order, d2, m2, oracle_lib):
+    """Config-1 style run (Gaussian blob, u = (1, 0.5), all-Neumann, CFL 0.8) for 100 steps:
+    the fused device step equals the reference step sequence bit for bit."""
+    d, m = d2, m2
+    C, F = int(d.nbcells), int(d.nbfaces)
+    c = d.cells.center
+    w0 = np.exp(-((c[:, 0] - .3) ** 2 + (c[:, 1] - .3) ** 2) / .005)
+    uf, vf, wf = np.full(F, 1.0), np.full(F, 0.5), np.zeros(F)
+    uc, vc, wc = np.zeros(C), np.zeros(C), np.zeros(C)
+    oracle_lib.facetocell(d, uf, uc)
+    oracle_lib.facetocell(d, vf, vc)
+    w_ref = w0.copy()
+    dts = [oracle_lib.advection_step(d, w_ref, uc, vc, wc, uf, vf, wf, 0.8, order) for _ in range(100)]
+    a = dev(torch, w0)
+    b = dev(torch, np.zeros(C))
+    scratch = dev(torch, np.zeros(3 * C))
+    dt = dev(torch, np.zeros(2))
+    tv = [dev(torch, x) for x in (uc, vc, wc, uf, vf, wf)]
+    call('mfv_time_step', m.ptr, P(tv[0]), P(tv[1]), P(tv[2]), 0.8, P(dt[0:1]), None)
+    for s in range(100):
+        cur, nxt = dt[s % 2: s % 2 + 1], dt[(s + 1) % 2: (s + 1) % 2 + 1]
+        call('mfv_dt_reset', P(nxt))
+        call('mfv_advection_step', m.ptr, P(a), P(b), *[P(x) for x in tv], 0.8, order, P(cur), P(nxt),
+             P(scratch))
+        a, b = b, a
+    assert host(dt[0:1])[0] == dts[-1]
+    same(host(a), w_ref, "w after 100 steps")
+    assert abs(w_ref.max() - w0.max()) > 1e-3  # the blob moved
+
+
+# ------------------------------------------------------------------------------- shallow water
+@pytest.fixture(scope="module")
+def sw_case():
+    d = make_tri_domain(36, 30, 100., 100., 0.1, 8)
+    from manapy_amd.device import DeviceMesh
+    from manapy_amd.meshgen import gaussian_bumps
+    c = d.cells.center
+    Z = gaussian_bumps(c, 8, np.random.default_rng(3), [(0., 100.), (0., 100.)])
+    h = np.where(c[:, 0] < 50., 2.0, 1.0)
+    return d, DeviceMesh(d), Z, h
+
+
+def test_sw_kernels(torch, sw_case, oracle_lib):
+    d, m, Z, h0 = sw_case
+    C = int(d.nbcells)
+    rng = np.random.default_rng(9)
+    h = h0 + 0.1 * rng.random(C)
+    hu, hv = 0.3 * rng.standard_normal(C), 0.3 * rng.standard_normal(C)
+    G = 9.81
+    dt_ref = oracle_lib.time_step_sw(d, h, hu, hv, 0.8, G)
+    th, thu, thv, tZ = (dev(torch, x) for x in (h, hu, hv, Z))
+    dtd = dev(torch, np.zeros(1))
+    dth = ctypes.c_double()
+    call('mfv_time_step_sw', m.ptr, P(th), P(thu), P(thv), 0.8, G, 0., 0., P(dtd), ctypes.byref(dth))
+    assert dth.value == dt_ref
+    zg_ref = oracle_lib.z_gradient(d, Z)
+    grad_ref = oracle_lib.sw_gradients(d, h, hu, hv, Z)
+    grad = dev(torch, np.zeros((9, C)))
+    call('mfv_sw_gradients', m.ptr, P(th), P(thu), P(thv), P(tZ), P(grad))
+    same(host(grad), grad_ref, "sw grad")
+    zg = dev(torch, zg_ref)
+    rez_ref, src_ref = oracle_lib.sw_residuals(d, h, hu, hv, Z, grad_ref, zg_ref, G, 2)
+    rez, src = dev(torch, np.zeros((3, C))), dev(torch, np.zeros((3, C)))
+    call('mfv_sw_residuals', m.ptr, P(th), P(thu), P(thv), P(tZ), P(grad), P(zg), G, 2, P(rez), P(src))
+    same(host(rez), rez_ref, "sw rez")
+    same(host(src), src_ref, "sw src")
+    hr, hur, hvr = h.copy(), hu.copy(), hv.copy()
+    oracle_lib.update_sw(hr, hur, hvr, rez_ref, src_ref, dt_ref, d.cells.volume)
+    call('mfv_update_sw', m.ptr, P(th), P(thu), P(thv), P(rez), P(src), P(dtd))
+    same(host(th), hr, "h")
+    same(host(thu), hur, "hu")
+    same(host(thv), hvr, "hv")
+
+
+def test_sw_fused_60_steps(torch, sw_case, oracle_lib):
+    """Dam break over bumps, MUSCL + LSQ + Barth-Jespersen + Rusanov, walls, CFL 0.8: 60 fused
+    device steps equal 60 reference-sequence oracle steps bit for bit."""
+    d, m, Z, h0 = sw_case
+    C = int(d.nbcells)
+    G = 9.81
+    h, hu, hv = h0.copy(), np.zeros(C), np.zeros(C)
+    zg_ref = oracle_lib.z_gradient(d, Z)
+    for _ in range(60):
+        dt_last = oracle_lib.sw_step(d, h, hu, hv, Z, zg_ref, G, 0.8, 2)
+    S = [dev(torch, np.stack([h0, np.zeros(C), np.zeros(C)])), dev(torch, np.zeros((3, C)))]
+    tZ, zg = dev(torch, Z), dev(torch, zg_ref)
+    grad = dev(torch, np.zeros((9, C)))
+    dt = dev(torch, np.zeros(2))
+    call('mfv_time_step_sw', m.ptr, P(S[0][0]), P(S[0][1]), P(S[0][2]), 0.8, G, 0., 0., P(dt[0:1]), None)
+    for s in range(60):
+        a, b = S[s % 2], S[(s + 1) % 2]
+        cur, nxt = dt[s % 2: s % 2 + 1], dt[(s + 1) % 2: (s + 1) % 2 + 1]
+        call('mfv_dt_reset', P(nxt))
+        call('mfv_sw_gradients', m.ptr, P(a[0]), P(a[1]), P(a[2]), P(tZ), P(grad))
+        call('mfv_sw_step_fused', m.ptr, P(a), P(b), P(tZ), P(grad), P(zg), G, 0.8, 2, P(cur), P(nxt))
+    out = host(S[0])
+    same(out[0], h, "h")
+    same(out[1], hu, "hu")
+    same(out[2], hv, "hv")
+    assert np.abs(hu).max() > 0.1
+
+
+# ------------------------------------------------------------------------------- Euler 3D
+def sod(d, gamma=1.4):
+    C = int(d.nbcells)
+    left = d.cells.center[:, 0] < 0.5
+    U = np.zeros((6, C))
+    U[0] = np.where(left, 1.0, 0.125)
+    U[1] = np.where(left, 1.0, 0.1)
+    U[5] = 0.5 * (U[2] ** 2 + U[3] ** 2 + U[4] ** 2) / U[0] + U[1] / (gamma - 1)
+    return U
+
+
+@pytest.mark.parametrize("quirk", [0, 1])
+def test_euler_kernels(torch, quirk, d3, m3, oracle_lib):
+    d, m = d3, m3
+    C, F = int(d.nbcells), int(d.nbfaces)
+    rng = np.random.default_rng(12)
+    U = sod(d)
+    U[2:5] = 0.2 * rng.standard_normal((3, C))
+    U[5] = 0.5 * (U[2] ** 2 + U[3] ** 2 + U[4] ** 2) / U[0] + U[1] / 0.4
+    G_ref = np.zeros((6, F))
+    oracle_lib.euler_ghost(d, U, G_ref, quirk)
+    tU, tG = dev(torch, U), dev(torch, np.zeros((6, F)))
+    call('mfv_euler3d_ghost', m.ptr, P(tU), P(tG), quirk)
+    same(host(tG), G_ref, "ghost")
+    dtc = oracle_lib.euler_dt(d, U, 0.8, 1.4)
+    dtd = dev(torch, np.zeros(1))
+    dth = ctypes.c_double()
+    call('mfv_euler3d_time_step', m.ptr, P(tU), 0.8, 1.4, P(dtd), ctypes.byref(dth))
+    assert dth.value == min(1e6, dtc.min())
+    rez_ref = oracle_lib.euler_rusanov(d, U, G_ref, 1.4, quirk)
+    rez = dev(torch, np.zeros((5, C)))
+    call('mfv_euler3d_rusanov', m.ptr, P(tU), P(tG), 1.4, quirk, P(rez))
+    same(host(rez), rez_ref, "rez")
+    U_ref = U.copy()
+    oracle_lib.euler_update(d, U_ref, rez_ref, 1.4, dth.value)
+    call('mfv_euler3d_update', m.ptr, P(tU), P(rez), 1.4, P(dtd))
+    same(host(tU), U_ref, "update")
+
+
+@pytest.mark.parametrize("quirk", [0, 1])
+def test_euler_fused_40_steps(torch, quirk, d3, m3, oracle_lib):
+    d, m = d3, m3
+    C, F = int(d.nbcells), int(d.nbfaces)
+    U0 = sod(d)
+    U = U0.copy()
+    G_ = np.zeros((6, F))
+    for _ in range(40):
+        oracle_lib.euler_step(d, U, G_, 0.8, 1.4, quirk)
+    S = [dev(torch, U0), dev(torch, np.zeros((6, C)))]
+    dt = dev(torch, np.zeros(2))
+    call('mfv_euler3d_time_step', m.ptr, P(S[0]), 0.8, 1.4, P(dt[0:1]), None)
+    for s in range(40):
+        cur, nxt = dt[s % 2: s % 2 + 1], dt[(s + 1) % 2: (s + 1) % 2 + 1]
+        call('mfv_dt_reset', P(nxt))
+        call('mfv_euler3d_step_fused', m.ptr, P(S[s % 2]), P(S[(s + 1) % 2]), 1.4, 0.8, quirk, P(cur), P(nxt))
+    same(host(S[0]), U, "U after 40 steps")

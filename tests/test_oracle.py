@@ -1,0 +1,378 @@
+"""Pins for the C restatement (oracle/mfv_oracle.c): closed forms, exactness on
+affine fields, conservation, limiter characterisation, well-balancing and flux
+consistency. None of these re-derives the oracle's formulas; each is a property
+the reference algorithm must satisfy, chosen so that a dropped term, a wrong
+sign or a swapped index fails."""
+import numpy as np
+import pytest
+
+from manapy_amd.ddm import Domain
+from conftest import make_tri_domain, make_tet_domain
+
+
+def two_triangles():
+    # unit square, nodes (0,0) (1,0) (1,1) (0,1); triangles (0,1,2) and (0,2,3)
+    el = np.array([[0, 1, 2, 3], [0, 2, 3, 3]], dtype=np.int32)
+    nd = np.array([[0., 0., 0., 1], [1., 0., 0., 2], [1., 1., 0., 2], [0., 1., 0., 1]])
+    return Domain(2, mesh=(el, nd))
+
+
+def test_time_step_closed_form(oracle_lib):
+    d = two_triangles()
+    u = np.array([1.0, 1.0])
+    v = np.array([0.5, 0.5])
+    w = np.zeros(2)
+    # triangle (0,0)(1,0)(1,1): outward non-unit normals (0,-1), (1,0), (-1,1) -> sum |u.n| = 0.5+1+0.5 = 2
+    # triangle (0,0)(1,1)(0,1): (1,-1), (0,1), (-1,0) -> 0.5 + 0.5 + 1 = 2 ; |K| = 0.5
+    dt = oracle_lib.time_step(d, u, v, w, 0.8)
+    assert dt == 0.8 * 0.5 / 2.0
+
+
+def test_time_step_sw_closed_form(oracle_lib):
+    d = two_triangles()
+    g = 9.81
+    h = np.array([2.0, 2.0])
+    hu = np.array([0., 0.])
+    hv = np.array([0., 0.])
+    # at rest: lam = sum_f sqrt(g h) |f| = sqrt(2g) (1 + 1 + sqrt 2)
+    dt = oracle_lib.time_step_sw(d, h, hu, hv, 0.8, g)
+    assert np.isclose(dt, 0.8 * 0.5 / (np.sqrt(2 * g) * (2 + np.sqrt(2))), rtol=1e-15)
+
+
+def _affine(d, a=0.4, b=-1.3, c=2.1):
+    f = lambda x, y: a + b * x + c * y
+    wc = f(d.cells.center[:, 0], d.cells.center[:, 1])
+    gc = d.faces.ghostcenter
+    wg = np.zeros(int(d.nbfaces))
+    bf = d.boundaryfaces
+    wg[bf] = f(gc[bf, 0], gc[bf, 1])
+    return f, wc, wg, (b, c)
+
+
+def test_lsq_gradient_exact_on_affine_fields(oracle_lib, dom2d):
+    d = dom2d
+    f, wc, wg, (gx_true, gy_true) = _affine(d)
+    C = int(d.nbcells)
+    gx, gy, gz = np.zeros(C), np.zeros(C), np.ones(C)
+    oracle_lib.cell_gradient(d, wc, wg, gx, gy, gz)
+    assert np.abs(gx - gx_true).max() < 1e-11
+    assert np.abs(gy - gy_true).max() < 1e-11
+    assert np.all(gz == 0.)
+
+
+def test_lsq_gradient_3d_exact_on_affine_fields(oracle_lib, dom3d):
+    d = dom3d
+    f = lambda p: 0.2 + 1.1 * p[:, 0] - 0.7 * p[:, 1] + 0.3 * p[:, 2]
+    wc = f(d.cells.center)
+    wg = np.zeros(int(d.nbfaces))
+    bf = d.boundaryfaces
+    wg[bf] = f(d.faces.ghostcenter[bf, :3])
+    C = int(d.nbcells)
+    g = [np.zeros(C) for _ in range(3)]
+    oracle_lib.cell_gradient(d, wc, wg, *g)
+    for k, t in enumerate((1.1, -0.7, 0.3)):
+        assert np.abs(g[k] - t).max() < 1e-10
+
+
+def test_barth_limiter_characterisation(oracle_lib, dom2d):
+    """BJ: psi in [0, 1]; limited face values stay inside the face-neighbour bounds;
+    psi < 1 only where some face value reaches a bound."""
+    d = dom2d
+    C = int(d.nbcells)
+    rng = np.random.default_rng(4)
+    xc = d.cells.center
+    wc = np.sin(6 * xc[:, 0]) * np.cos(4 * xc[:, 1]) + 0.1 * rng.standard_normal(C)
+    wg = np.zeros(int(d.nbfaces))
+    oracle_lib.ghost_value(d, 0, wc, wg, d.boundaryfaces)
+    gx, gy, gz, psi = np.zeros(C), np.zeros(C), np.zeros(C), np.zeros(C)
+    oracle_lib.cell_gradient(d, wc, wg, gx, gy, gz)
+    oracle_lib.barthlimiter(d, wc, wg, gx, gy, gz, psi)
+    assert psi.min() >= 0 and psi.max() <= 1
+    assert psi.min() < 1  # the field is rough enough to limit somewhere
+    fid = d.cells.faceid
+    cid = d.faces.cellid
+    for i in range(C):
+        faces = fid[i, :fid[i, -1]]
+        nb = [wc[cid[f, 1]] if d.faces.name[f] == 0 else wg[f] for f in faces]
+        nb += [wc[cid[f, 0]] for f in faces]
+        lo, hi = min(nb + [wc[i]]), max(nb + [wc[i]])
+        r = d.faces.center[faces, :2] - xc[i, :2]
+        delta = gx[i] * r[:, 0] + gy[i] * r[:, 1]
+        vals = wc[i] + psi[i] * delta
+        tol = 1e-12 * (1 + abs(hi) + abs(lo))
+        assert np.all(vals <= hi + tol) and np.all(vals >= lo - tol)
+        if psi[i] < 1:
+            assert np.min(np.minimum(np.abs(vals - hi), np.abs(vals - lo))) < 1e-9
+
+
+def test_limiter_constant_field_is_one(oracle_lib, dom2d):
+    d = dom2d
+    C = int(d.nbcells)
+    wc = np.full(C, 3.0)
+    wg = np.full(int(d.nbfaces), 3.0)
+    gx, gy, gz, psi = np.zeros(C), np.zeros(C), np.zeros(C), np.zeros(C)
+    oracle_lib.cell_gradient(d, wc, wg, gx, gy, gz)
+    assert np.all(gx == 0) and np.all(gy == 0)
+    oracle_lib.barthlimiter(d, wc, wg, gx, gy, gz, psi)
+    assert np.all(psi == 1.)
+
+
+def _face_vel(d, u, v):
+    F = int(d.nbfaces)
+    return np.full(F, u), np.full(F, v), np.zeros(F)
+
+
+def test_upwind_second_order_affine_exactness(oracle_lib, dom2d):
+    """With exact gradients and psi = 1 both face sides reconstruct w(x_f), so
+    rez_K / |K| = -u . grad w (midpoint rule is exact for affine w on straight faces)."""
+    d = dom2d
+    C = int(d.nbcells)
+    f, wc, wg, (gx_t, gy_t) = _affine(d)
+    uf, vf, wf = _face_vel(d, 0.7, -0.4)
+    gx, gy = np.full(C, gx_t), np.full(C, gy_t)
+    # boundary faces take the ghost value: use the affine value at the face centre there
+    bf = d.boundaryfaces
+    wg = np.zeros(int(d.nbfaces))
+    wg[bf] = f(d.faces.center[bf, 0], d.faces.center[bf, 1])
+    rez = np.zeros(C)
+    oracle_lib.explicitscheme_convective(d, rez, wc, wg, uf, vf, wf, gx, gy, np.zeros(C), np.ones(C), 2)
+    expected = -(0.7 * gx_t - 0.4 * gy_t)
+    assert np.abs(rez / d.cells.volume - expected).max() < 1e-10
+
+
+def test_upwind_first_order_jump_flux(oracle_lib, dom2d):
+    """u = (1, 0), w = 1 left of x = 0.5, 0 right: only the cells just downstream of the
+    jump gain, and the total gain is the y-extent of the interface (1.0)."""
+    d = dom2d
+    C = int(d.nbcells)
+    wc = (d.cells.center[:, 0] < 0.5).astype(np.float64)
+    wg = np.zeros(int(d.nbfaces))
+    oracle_lib.ghost_value(d, 0, wc, wg, d.boundaryfaces)
+    uf, vf, wf = _face_vel(d, 1.0, 0.0)
+    rez = np.zeros(C)
+    z = np.zeros(C)
+    oracle_lib.explicitscheme_convective(d, rez, wc, wg, uf, vf, wf, z, z, z, z, 1)
+    # left wall: inflow of the ghost value 1 balances; right wall: outflow of 0
+    inner = np.abs(rez) > 1e-14
+    assert np.all(rez[inner] > 0)
+    assert np.all(wc[inner] == 0.0)
+    assert abs(rez.sum() - 1.0) < 1e-13
+
+
+def test_upwind_constant_state_and_conservation(oracle_lib, dom2d):
+    d = dom2d
+    C = int(d.nbcells)
+    wc = np.full(C, 2.5)
+    wg = np.full(int(d.nbfaces), 2.5)
+    uf, vf, wf = _face_vel(d, 0.3, 0.9)
+    rez = np.zeros(C)
+    z = np.zeros(C)
+    oracle_lib.explicitscheme_convective(d, rez, wc, wg, uf, vf, wf, z, z, z, z, 1)
+    assert np.abs(rez).max() < 1e-14
+    # random state: interior fluxes cancel in the sum
+    wc = np.random.default_rng(1).random(C)
+    oracle_lib.ghost_value(d, 0, wc, wg, d.boundaryfaces)
+    oracle_lib.explicitscheme_convective(d, rez, wc, wg, uf, vf, wf, z, z, z, z, 1)
+    bf = d.boundaryfaces
+    n = d.faces.normal[bf]
+    sign = 0.3 * n[:, 0] + 0.9 * n[:, 1]
+    bflux = sign * wc[d.faces.cellid[bf, 0]]  # Neumann ghost: both sides carry the owner value
+    assert abs(rez.sum() + bflux.sum()) < 1e-13
+
+
+def test_update_new_value_identity(oracle_lib):
+    rng = np.random.default_rng(0)
+    w, rez, dis, src = (rng.random(17) for _ in range(4))
+    vol = rng.random(17) + 0.5
+    w0 = w.copy()
+    oracle_lib.update_new_value(w, rez, dis, src, 0.01, vol)
+    assert np.allclose(w, w0 + 0.01 * ((rez + dis) / vol + src), rtol=1e-15, atol=0)
+
+
+def test_facetocell_average(oracle_lib, dom2d):
+    d = dom2d
+    uf = np.random.default_rng(3).random(int(d.nbfaces))
+    uc = np.zeros(int(d.nbcells))
+    oracle_lib.facetocell(d, uf, uc)
+    fid = d.cells.faceid
+    assert np.allclose(uc, uf[fid[:, :3]].mean(axis=1), rtol=1e-14)
+
+
+# --------------------------------------------------------------------------------- shallow water
+G = 9.81
+
+
+def test_sw_flux_consistency_and_antisymmetry(oracle_lib):
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        h, hu, hv = rng.uniform(0.5, 3), rng.uniform(-2, 2), rng.uniform(-2, 2)
+        n = rng.uniform(-1, 1, 2)
+        mes = np.hypot(*n)
+        fl, sL, sR = oracle_lib.sw_face(h, hu, hv, 0., h, 0., h, hu, hv, 0., h, 0., n[0], n[1], mes, G)
+        u, v = hu / h, hv / h
+        q = u * n[0] + v * n[1]
+        phys = np.array([h * q, hu * q + 0.5 * G * h * h * n[0], hv * q + 0.5 * G * h * h * n[1]])
+        assert np.allclose(fl, phys, rtol=1e-13, atol=1e-13)
+        assert np.all(sL == 0) and np.all(sR == 0)
+        # antisymmetry: F(L, R, n) = -F(R, L, -n)
+        a = rng.uniform(0.5, 3, 3) * np.array([1, 1, 1])
+        b = rng.uniform(0.5, 3, 3)
+        L = (a[0], a[1] - 1.5, a[2] - 1.5)
+        R = (b[0], b[1] - 1.5, b[2] - 1.5)
+        f1, _, _ = oracle_lib.sw_face(*L, 0., L[0], 0., *R, 0., R[0], 0., n[0], n[1], mes, G)
+        f2, _, _ = oracle_lib.sw_face(*R, 0., R[0], 0., *L, 0., L[0], 0., -n[0], -n[1], mes, G)
+        assert np.allclose(f1, -f2, rtol=1e-13, atol=1e-13)
+
+
+@pytest.fixture(scope="module")
+def sw_dom():
+    return make_tri_domain(20, 16, 100., 100., 0.1, 2)
+
+
+def _bumps(d, seed=1):
+    from manapy_amd.meshgen import gaussian_bumps
+    return gaussian_bumps(d.cells.center, 8, np.random.default_rng(seed), [(0., 100.), (0., 100.)],
+                          width=(8., 20.))
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_sw_lake_at_rest_is_preserved(oracle_lib, sw_dom, order):
+    d = sw_dom
+    Z = _bumps(d)
+    h = 3.0 - Z
+    hu = np.zeros_like(h)
+    hv = np.zeros_like(h)
+    zgrad = oracle_lib.z_gradient(d, Z)
+    grad = oracle_lib.sw_gradients(d, h, hu, hv, Z) if order == 2 else np.zeros((9, len(h)))
+    rez, src = oracle_lib.sw_residuals(d, h, hu, hv, Z, grad, zgrad, G, order)
+    scale = G * 9.0 * d.faces.mesure.max()
+    assert np.abs(rez[0]).max() < 1e-12 * scale
+    assert np.abs(rez[1:] + src[1:]).max() < 1e-11 * scale
+    assert np.abs(src[1:]).max() > 1e-3 * scale  # the bathymetry terms are not trivially zero
+    # one full step keeps the state at rest
+    h2, hu2, hv2 = h.copy(), hu.copy(), hv.copy()
+    oracle_lib.sw_step(d, h2, hu2, hv2, Z, zgrad, G, 0.8, order)
+    assert np.abs(h2 + Z - 3.0).max() < 1e-12
+    assert np.abs(hu2).max() < 1e-11 and np.abs(hv2).max() < 1e-11
+
+
+def test_sw_mass_conservation_first_order(oracle_lib, sw_dom):
+    d = sw_dom
+    rng = np.random.default_rng(3)
+    C = int(d.nbcells)
+    Z = _bumps(d, 4)
+    h = 2.0 + rng.random(C)
+    hu = rng.standard_normal(C)
+    hv = rng.standard_normal(C)
+    zgrad = oracle_lib.z_gradient(d, Z)
+    rez, src = oracle_lib.sw_residuals(d, h, hu, hv, Z, np.zeros((9, C)), zgrad, G, 1)
+    assert abs(rez[0].sum()) < 1e-10 * np.abs(rez[0]).max()
+    assert np.all(src[0] == 0)
+
+
+def test_sw_dam_break_mass_and_symmetry(oracle_lib):
+    """A dam break symmetric in y on a y-symmetric structured mesh stays symmetric and
+    conserves mass (walls) over several second-order steps."""
+    from manapy_amd.meshgen import jittered_triangles
+    el, nd = jittered_triangles(30, 10, 100., 20., 0.0, 0)
+    d = Domain(2, mesh=(el, nd))
+    C = int(d.nbcells)
+    h = np.where(d.cells.center[:, 0] < 50., 2.0, 1.0)
+    hu, hv, Z = np.zeros(C), np.zeros(C), np.zeros(C)
+    zgrad = oracle_lib.z_gradient(d, Z)
+    m0 = (h * d.cells.volume).sum()
+    for _ in range(20):
+        oracle_lib.sw_step(d, h, hu, hv, Z, zgrad, G, 0.8, 2)
+    assert abs((h * d.cells.volume).sum() - m0) < 1e-9 * m0
+    assert h.min() > 0.9 and h.max() < 2.1
+    assert hu.max() > 0.1  # the dam has started to move right
+
+
+# --------------------------------------------------------------------------------- Euler 3D
+GAMMA = 1.4
+
+
+def _phys_flux(U, n):
+    rho, P, ru, rv, rw, rE = U
+    q = (ru * n[0] + rv * n[1] + rw * n[2]) / rho
+    return np.array([rho * q, ru * q + P * n[0], rv * q + P * n[1], rw * q + P * n[2], (rE + P) * q])
+
+
+def _flux(U1, U2, n, mes, quirk):
+    from oracle.oracle import lib, f64
+    import ctypes
+    out = np.zeros(5)
+    nn = np.ascontiguousarray(n, dtype=np.float64)
+    lib().orc_flux_euler_rusanov(*[ctypes.c_double(x) for x in U1], *[ctypes.c_double(x) for x in U2], f64(nn),
+                                 ctypes.c_double(mes), ctypes.c_double(GAMMA), ctypes.c_int(quirk), f64(out))
+    return out
+
+
+def _rand_state(rng):
+    rho, P = rng.uniform(0.2, 2), rng.uniform(0.2, 2)
+    u = rng.uniform(-1, 1, 3)
+    return (rho, P, rho * u[0], rho * u[1], rho * u[2], P / (GAMMA - 1) + 0.5 * rho * (u @ u))
+
+
+def test_euler_rusanov_consistency_and_quirk(oracle_lib):
+    rng = np.random.default_rng(5)
+    for _ in range(40):
+        U = _rand_state(rng)
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        mes = rng.uniform(0.1, 2)
+        for quirk in (0, 1):
+            assert np.allclose(_flux(U, U, n, mes, quirk), _phys_flux(U, n) * mes, rtol=1e-12, atol=1e-12)
+        UL, UR = _rand_state(rng), _rand_state(rng)
+        f0 = _flux(UL, UR, n, mes, 0)
+        # the corrected flux is antisymmetric; Q1 only changes the right wave speed through v, w
+        assert np.allclose(f0, -_flux(UR, UL, -n, mes, 0), rtol=1e-12, atol=1e-12)
+        UR_same_vw = (UR[0], UR[1], UR[2], UL[3] / UL[0] * UR[0], UL[4] / UL[0] * UR[0], UR[5])
+        UR_qr = (UR_same_vw[0], UR_same_vw[1], UR_same_vw[2], UL[3], UL[4], UR_same_vw[5])
+        # with rhov_R = rhov_L and rhow_R = rhow_L the quirk is invisible
+        UR_eq = (UR[0], UR[1], UR[2], UL[3], UL[4], UR[5])
+        assert np.array_equal(_flux(UL, UR_eq, n, mes, 1), _flux(UL, UR_eq, n, mes, 0))
+        del UR_qr
+
+
+def test_euler_ghost_slip(oracle_lib, dom3d):
+    d = dom3d
+    rng = np.random.default_rng(2)
+    C, F = int(d.nbcells), int(d.nbfaces)
+    U = np.array([_rand_state(rng) for _ in range(C)]).T.copy()
+    for quirk in (0, 1):
+        G_ = np.full((6, F), 7.0)
+        oracle_lib.euler_ghost(d, U, G_, quirk)
+        bf = d.boundaryfaces
+        c = d.faces.cellid[bf, 0]
+        n = d.faces.normal[bf] / d.faces.mesure[bf][:, None]
+        wall = np.isin(d.faces.name[bf], [3, 4, 5, 6])
+        ug = G_[2:5, bf] / G_[0, bf]
+        uc = U[2:5, c] / U[0, c]
+        assert np.array_equal(G_[0, bf], U[0, c]) and np.array_equal(G_[5, bf], U[5, c])
+        if quirk == 0:
+            # mirrored normal velocity, tangential velocity kept
+            un_g = np.einsum('ij,ji->i', n[wall], ug[:, wall])
+            un_c = np.einsum('ij,ji->i', n[wall], uc[:, wall])
+            assert np.allclose(un_g, -un_c, atol=1e-13)
+            t_g = ug[:, wall] - un_g * n[wall].T
+            t_c = uc[:, wall] - un_c * n[wall].T
+            assert np.allclose(t_g, t_c, atol=1e-13)
+        else:
+            # Q2: rhowg untouched on walls, rhovg holds rho * w_g
+            assert np.all(G_[4, bf[wall]] == 7.0)
+        # transmissive faces copy the owner state
+        tr = ~wall
+        assert np.array_equal(G_[:, bf[tr]], U[:, c[tr]])
+
+
+def test_euler_uniform_state_is_steady(oracle_lib, dom3d):
+    d = dom3d
+    C, F = int(d.nbcells), int(d.nbfaces)
+    U = np.zeros((6, C))
+    U[0], U[1], U[5] = 1.2, 0.9, 0.9 / (GAMMA - 1)
+    G_ = np.zeros((6, F))
+    U0 = U.copy()
+    dt = oracle_lib.euler_step(d, U, G_, 0.8, GAMMA, 0)
+    assert dt > 0
+    assert np.abs(U - U0).max() < 1e-13
