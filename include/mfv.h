@@ -109,15 +109,20 @@ int mfv_explicitscheme_convective(const mfv_mesh *m, double *rez_w, const double
 int mfv_update_new_value(const mfv_mesh *m, double *ne_c, const double *rez, const double *dissip,
                          const double *src, double dtime, const double *dt_dev, void *stream);
 
+/* Fused steps keep dt on the device in a ring of three doubles: step k uses
+ * dt_ring[k % 3], min-reduces the next step's dt into dt_ring[(k+1) % 3] (which must hold
+ * 1e6 when step k starts) and sets dt_ring[(k+2) % 3] = 1e6 for step k+1. Initialise the
+ * ring to {dt_0, 1e6, 1e6} (mfv_dt_ring_init) and pass k = 0, 1, 2, ... */
+int mfv_dt_ring_init(double *dt_ring, const double *dt0, void *stream);
+
 /* One fused explicit upwind step for linear advection (time_step + ghost Neumann +
  * explicitscheme_convective(order) + update_new_value, order 1 or 2 with LSQ gradient and
- * Barth-Jespersen limiter). w_in -> w_out; dt_cur[0] is the dt of this step; the dt of the
- * next step (from w_out) is min-reduced into dt_next[0], which must hold 1e6 on entry.
- * scratch: 3*C doubles (gradient and limiter), unused at order 1. */
+ * Barth-Jespersen limiter). w_in -> w_out. scratch: (dim+1)*C doubles (gradient and limiter),
+ * unused at order 1. */
 int mfv_advection_step(const mfv_mesh *m, const double *w_in, double *w_out, const double *u_c,
                        const double *v_c, const double *w_c, const double *u_face,
                        const double *v_face, const double *w_face, double cfl, int order,
-                       const double *dt_cur, double *dt_next, double *scratch, void *stream);
+                       double *dt_ring, int step, double *scratch, void *stream);
 
 /* ---------------- shallow water (solvers/shallowater) ----------------------------- */
 
@@ -146,11 +151,11 @@ int mfv_update_sw(const mfv_mesh *m, double *h, double *hu, double *hv, const do
                   const double *src, const double *dt_dev, void *stream);
 
 /* Fused MUSCL/Rusanov step: residuals + update_SW + the next step's time_step_SW.
- * state_in/state_out: [3][C] (h, hu, hv); dt_cur[0] this step's dt; dt_next[0] receives
- * the min-reduced dt of state_out and must hold 1e6 on entry (mfv_dt_reset). */
+ * state_in/state_out: [3][C] (h, hu, hv); grad from mfv_sw_gradients(state_in); dt ring as
+ * for mfv_advection_step. */
 int mfv_sw_step_fused(const mfv_mesh *m, const double *state_in, double *state_out,
                       const double *Z, const double *grad, const double *zgrad, double grav,
-                      double cfl, int order, const double *dt_cur, double *dt_next, void *stream);
+                      double cfl, int order, double *dt_ring, int step, void *stream);
 
 /* Sets dt[0] = 1e6, the initial value of the reference's dt min-loops. */
 int mfv_dt_reset(double *dt, void *stream);
@@ -176,10 +181,9 @@ int mfv_euler3d_update(const mfv_mesh *m, double *U, const double *rez, double g
                        const double *dt_dev, void *stream);
 
 /* Fused step: ghosts on the fly + Rusanov fluxes + update + next dt.
- * U_in -> U_out ([6][C]); dt_next[0] must hold 1e6 on entry. */
+ * U_in -> U_out ([6][C]); dt ring as for mfv_advection_step. */
 int mfv_euler3d_step_fused(const mfv_mesh *m, const double *U_in, double *U_out, double gamma,
-                           double cfl, int quirk, const double *dt_cur, double *dt_next,
-                           void *stream);
+                           double cfl, int quirk, double *dt_ring, int step, void *stream);
 
 #ifdef __cplusplus
 }

@@ -48,18 +48,19 @@ _SIGNATURES = {
     'mfv_time_step': [MESH_P, P, P, P, c_double, P, DP, P],
     'mfv_explicitscheme_convective': [MESH_P, P, P, P, P, P, P, P, P, P, P, c_int, P],
     'mfv_update_new_value': [MESH_P, P, P, P, P, c_double, P, P],
-    'mfv_advection_step': [MESH_P, P, P, P, P, P, P, P, P, c_double, c_int, P, P, P, P],
+    'mfv_dt_ring_init': [P, P, P],
+    'mfv_advection_step': [MESH_P, P, P, P, P, P, P, P, P, c_double, c_int, P, c_int, P, P],
     'mfv_time_step_sw': [MESH_P, P, P, P, c_double, c_double, c_double, c_double, P, DP, P],
     'mfv_sw_gradients': [MESH_P, P, P, P, P, P, P],
     'mfv_sw_residuals': [MESH_P, P, P, P, P, P, P, c_double, c_int, P, P, P],
     'mfv_update_sw': [MESH_P, P, P, P, P, P, P, P],
-    'mfv_sw_step_fused': [MESH_P, P, P, P, P, P, c_double, c_double, c_int, P, P, P],
+    'mfv_sw_step_fused': [MESH_P, P, P, P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_dt_reset': [P, P],
     'mfv_euler3d_ghost': [MESH_P, P, P, c_int, P],
     'mfv_euler3d_time_step': [MESH_P, P, c_double, c_double, P, DP, P],
     'mfv_euler3d_rusanov': [MESH_P, P, P, c_double, c_int, P, P],
     'mfv_euler3d_update': [MESH_P, P, P, c_double, P, P],
-    'mfv_euler3d_step_fused': [MESH_P, P, P, c_double, c_double, c_int, P, P, P],
+    'mfv_euler3d_step_fused': [MESH_P, P, P, c_double, c_double, c_int, P, c_int, P],
 }
 _RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p}
 

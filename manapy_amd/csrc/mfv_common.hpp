@@ -55,6 +55,23 @@ __device__ __forceinline__ void wave_min_to(double *addr, double v)
     if ((threadIdx.x & 63) == 0) atomic_min_positive(addr, m);
 }
 
+// dt ring of the fused steps: slot k%3 is this step's dt, (k+1)%3 receives the next dt,
+// (k+2)%3 is reset for the step after (nothing reads or writes it during step k).
+struct DtRing {
+    const double *cur;
+    double *next;
+    double *spare;
+};
+inline DtRing dt_ring(double *ring, int step)
+{
+    int k = ((step % 3) + 3) % 3;
+    return DtRing{ring + k, ring + (k + 1) % 3, ring + (k + 2) % 3};
+}
+__device__ __forceinline__ void ring_reset_spare(double *spare)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) spare[0] = 1e6;
+}
+
 // Mesh descriptor passed by value to kernels.
 struct Mesh {
     int C, F, MF, K, G;

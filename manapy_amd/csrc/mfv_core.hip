@@ -266,6 +266,20 @@ extern "C" int mfv_barthlimiter(const mfv_mesh *m, const double *w_c, const doub
 
 __global__ void k_dt_reset(double *dt) { dt[0] = 1e6; }
 
+__global__ void k_dt_ring_init(double *ring, const double *dt0)
+{
+    ring[0] = dt0[0];
+    ring[1] = 1e6;
+    ring[2] = 1e6;
+}
+
+extern "C" int mfv_dt_ring_init(double *dt_ring_p, const double *dt0, void *stream)
+{
+    MFV_REQUIRE(dt_ring_p && dt0, "NULL array");
+    hipLaunchKernelGGL(k_dt_ring_init, dim3(1), dim3(1), 0, as_stream(stream), dt_ring_p, dt0);
+    return check_launch("k_dt_ring_init");
+}
+
 extern "C" int mfv_dt_reset(double *dt, void *stream)
 {
     MFV_REQUIRE(dt, "dt is NULL");
@@ -503,9 +517,11 @@ __global__ void __launch_bounds__(kBlock) k_adv_step(Mesh m, const double *__res
                                                       const double *__restrict__ vf,
                                                       const double *__restrict__ wf, double cfl, double om,
                                                       const double *__restrict__ scratch,
-                                                      const double *__restrict__ dt_cur, double *dt_next)
+                                                      const double *__restrict__ dt_cur, double *dt_next,
+                                                      double *dt_spare)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    ring_reset_spare(dt_spare);
     double dloc = 1e300;
     if (i < m.C) {
         const double *gx = scratch, *gy = scratch + m.C, *gz = scratch + 2 * (size_t)m.C;
@@ -522,13 +538,13 @@ __global__ void __launch_bounds__(kBlock) k_adv_step(Mesh m, const double *__res
 extern "C" int mfv_advection_step(const mfv_mesh *m, const double *w_in, double *w_out, const double *u_c,
                                   const double *v_c, const double *w_c, const double *u_face,
                                   const double *v_face, const double *w_face, double cfl, int order,
-                                  const double *dt_cur, double *dt_next, double *scratch, void *stream)
+                                  double *dt_ring_p, int step, double *scratch, void *stream)
 {
     if (int rc = validate_mesh(m, order == 2, 1, 0)) return rc;
     MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
-    MFV_REQUIRE(w_in && w_out && w_in != w_out && u_c && v_c && w_c && u_face && v_face && w_face && dt_cur &&
-                    dt_next,
+    MFV_REQUIRE(w_in && w_out && w_in != w_out && u_c && v_c && w_c && u_face && v_face && w_face && dt_ring_p,
                 "NULL or aliased array");
+    DtRing r = dt_ring(dt_ring_p, step);
     MFV_REQUIRE(order == 1 || scratch, "order 2 needs a scratch of (dim+1)*C doubles");
     MFV_REQUIRE(!(m->dim == 2 && m->max_cell_faces > 4), "2D cells have at most 4 faces");
     if (m->nb_cells == 0) return MFV_OK;
@@ -542,6 +558,6 @@ extern "C" int mfv_advection_step(const mfv_mesh *m, const double *w_in, double 
     auto ks = (m->dim == 2) ? ((order == 2) ? k_adv_step<2, 2> : k_adv_step<2, 1>)
                             : ((order == 2) ? k_adv_step<3, 2> : k_adv_step<3, 1>);
     hipLaunchKernelGGL(ks, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, st, k, w_in, w_out, u_c, v_c, w_c,
-                       u_face, v_face, w_face, cfl, (double)(order - 1), scratch, dt_cur, dt_next);
+                       u_face, v_face, w_face, cfl, (double)(order - 1), scratch, r.cur, r.next, r.spare);
     return check_launch("k_adv_step");
 }

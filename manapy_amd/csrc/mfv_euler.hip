@@ -215,9 +215,10 @@ __global__ void __launch_bounds__(kBlock) k_euler_update(int C, double *__restri
 __global__ void __launch_bounds__(kBlock) k_euler_step(Mesh m, const double *__restrict__ Uin,
                                                         double *__restrict__ Uout, double gamma, double cfl,
                                                         int quirk, const double *__restrict__ dt_cur,
-                                                        double *dt_next)
+                                                        double *dt_next, double *dt_spare)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    ring_reset_spare(dt_spare);
     double d = 1e300;
     if (i < m.C) {
         const size_t C = m.C;
@@ -292,13 +293,14 @@ extern "C" int mfv_euler3d_update(const mfv_mesh *m, double *U, const double *re
 }
 
 extern "C" int mfv_euler3d_step_fused(const mfv_mesh *m, const double *U_in, double *U_out, double gamma, double cfl,
-                                      int quirk, const double *dt_cur, double *dt_next, void *stream)
+                                      int quirk, double *dt_ring_p, int step, void *stream)
 {
     if (int rc = validate_mesh(m, 0, 0, 1)) return rc;
     MFV_REQUIRE(m->dim == 3 && m->max_cell_faces >= 4, "Euler 3D needs tetrahedra");
     if (m->nb_cells == 0) return MFV_OK;
-    MFV_REQUIRE(U_in && U_out && U_in != U_out && dt_cur && dt_next, "NULL or aliased array");
+    MFV_REQUIRE(U_in && U_out && U_in != U_out && dt_ring_p, "NULL or aliased array");
+    DtRing r = dt_ring(dt_ring_p, step);
     hipLaunchKernelGGL(k_euler_step, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m),
-                       U_in, U_out, gamma, cfl, quirk, dt_cur, dt_next);
+                       U_in, U_out, gamma, cfl, quirk, r.cur, r.next, r.spare);
     return check_launch("k_euler_step");
 }

@@ -311,9 +311,11 @@ __global__ void __launch_bounds__(kBlock) k_sw_step(Mesh m, const double *__rest
                                                      double *__restrict__ Sout, const double *__restrict__ Z,
                                                      const double *__restrict__ grad,
                                                      const double *__restrict__ zgrad, double g, double cfl,
-                                                     double om, const double *__restrict__ dt_cur, double *dt_next)
+                                                     double om, const double *__restrict__ dt_cur, double *dt_next,
+                                                     double *dt_spare)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    ring_reset_spare(dt_spare);
     double dloc = 1e300;
     if (i < m.C) {
         const size_t C = m.C;
@@ -401,15 +403,16 @@ extern "C" int mfv_update_sw(const mfv_mesh *m, double *h, double *hu, double *h
 
 extern "C" int mfv_sw_step_fused(const mfv_mesh *m, const double *state_in, double *state_out, const double *Z,
                                  const double *grad, const double *zgrad, double grav, double cfl, int order,
-                                 const double *dt_cur, double *dt_next, void *stream)
+                                 double *dt_ring_p, int step, void *stream)
 {
     if (int rc = validate_mesh(m, 0, 1, 0)) return rc;
     MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
     MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
     if (m->nb_cells == 0) return MFV_OK;
-    MFV_REQUIRE(state_in && state_out && state_in != state_out && Z && grad && zgrad && dt_cur && dt_next,
+    MFV_REQUIRE(state_in && state_out && state_in != state_out && Z && grad && zgrad && dt_ring_p,
                 "NULL or aliased array");
+    DtRing r = dt_ring(dt_ring_p, step);
     hipLaunchKernelGGL(k_sw_step, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m),
-                       state_in, state_out, Z, grad, zgrad, grav, cfl, (double)(order - 1), dt_cur, dt_next);
+                       state_in, state_out, Z, grad, zgrad, grav, cfl, (double)(order - 1), r.cur, r.next, r.spare);
     return check_launch("k_sw_step");
 }

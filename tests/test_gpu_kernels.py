@@ -171,16 +171,15 @@ def test_advection_fused_100_steps(torch, order, d2, m2, oracle_lib):
     a = dev(torch, w0)
     b = dev(torch, np.zeros(C))
     scratch = dev(torch, np.zeros(3 * C))
-    dt = dev(torch, np.zeros(2))
+    dt0 = dev(torch, np.zeros(1))
+    ring = dev(torch, np.zeros(3))
     tv = [dev(torch, x) for x in (uc, vc, wc, uf, vf, wf)]
-    call('mfv_time_step', m.ptr, P(tv[0]), P(tv[1]), P(tv[2]), 0.8, P(dt[0:1]), None)
+    call('mfv_time_step', m.ptr, P(tv[0]), P(tv[1]), P(tv[2]), 0.8, P(dt0), None)
+    call('mfv_dt_ring_init', P(ring), P(dt0))
     for s in range(100):
-        cur, nxt = dt[s % 2: s % 2 + 1], dt[(s + 1) % 2: (s + 1) % 2 + 1]
-        call('mfv_dt_reset', P(nxt))
-        call('mfv_advection_step', m.ptr, P(a), P(b), *[P(x) for x in tv], 0.8, order, P(cur), P(nxt),
-             P(scratch))
+        call('mfv_advection_step', m.ptr, P(a), P(b), *[P(x) for x in tv], 0.8, order, P(ring), s, P(scratch))
         a, b = b, a
-    assert host(dt[0:1])[0] == dts[-1]
+    assert host(ring)[99 % 3] == dts[-1]
     same(host(a), w_ref, "w after 100 steps")
     assert abs(w_ref.max() - w0.max()) > 1e-3  # the blob moved
 
@@ -242,14 +241,15 @@ def test_sw_fused_60_steps(torch, sw_case, oracle_lib):
     S = [dev(torch, np.stack([h0, np.zeros(C), np.zeros(C)])), dev(torch, np.zeros((3, C)))]
     tZ, zg = dev(torch, Z), dev(torch, zg_ref)
     grad = dev(torch, np.zeros((9, C)))
-    dt = dev(torch, np.zeros(2))
-    call('mfv_time_step_sw', m.ptr, P(S[0][0]), P(S[0][1]), P(S[0][2]), 0.8, G, 0., 0., P(dt[0:1]), None)
+    dt0 = dev(torch, np.zeros(1))
+    ring = dev(torch, np.zeros(3))
+    call('mfv_time_step_sw', m.ptr, P(S[0][0]), P(S[0][1]), P(S[0][2]), 0.8, G, 0., 0., P(dt0), None)
+    call('mfv_dt_ring_init', P(ring), P(dt0))
     for s in range(60):
         a, b = S[s % 2], S[(s + 1) % 2]
-        cur, nxt = dt[s % 2: s % 2 + 1], dt[(s + 1) % 2: (s + 1) % 2 + 1]
-        call('mfv_dt_reset', P(nxt))
         call('mfv_sw_gradients', m.ptr, P(a[0]), P(a[1]), P(a[2]), P(tZ), P(grad))
-        call('mfv_sw_step_fused', m.ptr, P(a), P(b), P(tZ), P(grad), P(zg), G, 0.8, 2, P(cur), P(nxt))
+        call('mfv_sw_step_fused', m.ptr, P(a), P(b), P(tZ), P(grad), P(zg), G, 0.8, 2, P(ring), s)
+    assert host(ring)[59 % 3] == dt_last
     out = host(S[0])
     same(out[0], h, "h")
     same(out[1], hu, "hu")
@@ -306,10 +306,10 @@ def test_euler_fused_40_steps(torch, quirk, d3, m3, oracle_lib):
     for _ in range(40):
         oracle_lib.euler_step(d, U, G_, 0.8, 1.4, quirk)
     S = [dev(torch, U0), dev(torch, np.zeros((6, C)))]
-    dt = dev(torch, np.zeros(2))
-    call('mfv_euler3d_time_step', m.ptr, P(S[0]), 0.8, 1.4, P(dt[0:1]), None)
+    dt0 = dev(torch, np.zeros(1))
+    ring = dev(torch, np.zeros(3))
+    call('mfv_euler3d_time_step', m.ptr, P(S[0]), 0.8, 1.4, P(dt0), None)
+    call('mfv_dt_ring_init', P(ring), P(dt0))
     for s in range(40):
-        cur, nxt = dt[s % 2: s % 2 + 1], dt[(s + 1) % 2: (s + 1) % 2 + 1]
-        call('mfv_dt_reset', P(nxt))
-        call('mfv_euler3d_step_fused', m.ptr, P(S[s % 2]), P(S[(s + 1) % 2]), 1.4, 0.8, quirk, P(cur), P(nxt))
+        call('mfv_euler3d_step_fused', m.ptr, P(S[s % 2]), P(S[(s + 1) % 2]), 1.4, 0.8, quirk, P(ring), s)
     same(host(S[0]), U, "U after 40 steps")
