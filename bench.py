@@ -29,6 +29,24 @@ G = 9.81
 CFL = 0.8
 
 
+def kernel_bytes_per_cell_packed(MF, K_avg, G_avg):
+    """Algorithmic HBM bytes per cell of the packed kernels: every array a kernel needs,
+    counted once per cell at its element width (a neighbour's record is another cell's own
+    record, already counted)."""
+    recon = (8 * 8                # own record: cx, cy, eta, hu, hv (+h, Z, vol in the same line)
+             + 4 * K_avg + 4      # cellnid entries + count
+             + G_avg * (4 + 8 + 24) + 4  # ghostnid entries, face_cells, ghost centre + count
+             + MF * 4 + 4         # neighbour codes + face count
+             + MF * 32            # cfrec: rx, ry, z_f
+             + MF * 32)           # write facerec
+    flux = (8 * 8                 # own record (read) -- h, hu, hv, Z, vol
+            + 4 * 8               # write eta, hu, hv, h
+            + MF * 32             # own facerec slots (neighbours read theirs)
+            + MF * 32             # cfgeo: n0, n1, mesure
+            + MF * 4 + 1 + 4)     # neighbour codes, residual order, face count
+    return recon, flux
+
+
 def kernel_bytes_per_cell(F, C, K_avg, G_avg):
     """Algorithmic HBM bytes per cell of each kernel (every array the kernel needs, counted
     once, at its element width; face arrays scaled by F/C)."""
@@ -88,7 +106,20 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-steps', type=int, default=20)
     ap.add_argument('--cpu-budget', type=float, default=20.0)
+    ap.add_argument('--path', choices=('packed', 'unpacked'), default='packed',
+                    help='packed: per-cell records + per-face MUSCL values (default); unpacked: SoA kernels')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='only run a device-to-device copy of 1 GiB five times (calibrates HBM counters)')
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 2 * src.numel() * 8, "read_bytes": src.numel() * 8}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU = one subdomain); "
               "run with --gpus 1", file=sys.stderr)
@@ -121,18 +152,39 @@ def main():
     _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
 
     step_no = [0]
+    if args.path == 'packed':
+        from manapy_amd.device import SwPacked
+        pk = SwPacked(m)
+        _lib.call('mfv_sw_pack', m.ptr, pk.ptr, S[0][0].data_ptr(), S[0][1].data_ptr(), S[0][2].data_ptr(),
+                  tZ.data_ptr(), zgrad.data_ptr(), 2, s)
+        names = ('k_sw_recon', 'k_sw_flux_packed')
+
+        def launch_a(k):
+            _lib.call('mfv_sw_recon', m.ptr, pk.ptr, 2, s)
+
+        def launch_b(k):
+            _lib.call('mfv_sw_flux_packed', m.ptr, pk.ptr, G, CFL, ring.data_ptr(), k, s)
+    else:
+        names = ('k_sw_gradients', 'k_sw_step')
+
+        def launch_a(k):
+            a = S[k % 2]
+            _lib.call('mfv_sw_gradients', m.ptr, a[0].data_ptr(), a[1].data_ptr(), a[2].data_ptr(), tZ.data_ptr(),
+                      grad.data_ptr(), s)
+
+        def launch_b(k):
+            a, b = S[k % 2], S[(k + 1) % 2]
+            _lib.call('mfv_sw_step_fused', m.ptr, a.data_ptr(), b.data_ptr(), tZ.data_ptr(), grad.data_ptr(),
+                      zgrad.data_ptr(), G, CFL, 2, ring.data_ptr(), k, s)
 
     def step(ev=None):
         k = step_no[0]
-        a, b = S[k % 2], S[(k + 1) % 2]
         if ev is not None:
             ev[0].record()
-        _lib.call('mfv_sw_gradients', m.ptr, a[0].data_ptr(), a[1].data_ptr(), a[2].data_ptr(), tZ.data_ptr(),
-                  grad.data_ptr(), s)
+        launch_a(k)
         if ev is not None:
             ev[1].record()
-        _lib.call('mfv_sw_step_fused', m.ptr, a.data_ptr(), b.data_ptr(), tZ.data_ptr(), grad.data_ptr(),
-                  zgrad.data_ptr(), G, CFL, 2, ring.data_ptr(), k, s)
+        launch_b(k)
         if ev is not None:
             ev[2].record()
         step_no[0] = k + 1
@@ -159,7 +211,10 @@ def main():
 
     K_avg = float(d.cells.cellnid[:, -1].mean())
     G_avg = float(d.cells.ghostnid[:, -1].mean())
-    bg, bfx = kernel_bytes_per_cell(F, C, K_avg, G_avg)
+    if args.path == 'packed':
+        bg, bfx = kernel_bytes_per_cell_packed(int(d.cells.faceid.shape[1] - 1), K_avg, G_avg)
+    else:
+        bg, bfx = kernel_bytes_per_cell(F, C, K_avg, G_avg)
     ach_flux = bfx * C / t_flux.mean() / 1e9
     ach_grad = bg * C / t_grad.mean() / 1e9
     dom_is_flux = t_flux.mean() >= t_grad.mean()
@@ -181,15 +236,16 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded jittered mesh, dam break over seeded Gaussian bumps)",
         "config": {"workload": "shallow-water dam break, 2D, MUSCL + LSQ + Barth-Jespersen + Rusanov, walls",
-                   "cells": C, "faces": F, "nx": args.nx, "cfl": CFL, "order": 2, "precision": "fp64 plain"},
-        "roofline": {"bound": "hbm", "kernel": "k_sw_step (flux+update+dt)" if dom_is_flux else "k_sw_gradients",
+                   "cells": C, "faces": F, "nx": args.nx, "cfl": CFL, "order": 2, "precision": "fp64 plain",
+                   "path": args.path},
+        "roofline": {"bound": "hbm", "kernel": names[1] if dom_is_flux else names[0],
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_cell": round(bpc, 1), "avg_launch_us": round(float(tk.mean()) * 1e6, 2)},
-        "kernels": {"k_sw_gradients": {"avg_us": round(float(t_grad.mean()) * 1e6, 2),
-                                       "bytes_per_cell": round(bg, 1), "GB/s": round(ach_grad, 1)},
-                    "k_sw_step": {"avg_us": round(float(t_flux.mean()) * 1e6, 2),
-                                  "bytes_per_cell": round(bfx, 1), "GB/s": round(ach_flux, 1)}},
+        "kernels": {names[0]: {"avg_us": round(float(t_grad.mean()) * 1e6, 2),
+                               "bytes_per_cell": round(bg, 1), "GB/s": round(ach_grad, 1)},
+                    names[1]: {"avg_us": round(float(t_flux.mean()) * 1e6, 2),
+                               "bytes_per_cell": round(bfx, 1), "GB/s": round(ach_flux, 1)}},
         "step_hbm_frac_356B": round(step_bytes_survey * C * args.steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
         "mesh_build_s": round(t_mesh, 2),
     }

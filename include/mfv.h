@@ -157,6 +157,37 @@ int mfv_sw_step_fused(const mfv_mesh *m, const double *state_in, double *state_o
                       const double *Z, const double *grad, const double *zgrad, double grav,
                       double cfl, int order, double *dt_ring, int step, void *stream);
 
+/* Packed shallow-water path: the same step as mfv_sw_gradients + mfv_sw_step_fused with
+ * the state held in one 64-byte record per cell and the MUSCL values stored per face slot,
+ * so that every neighbour access is a contiguous load. Bit-identical results. */
+typedef struct mfv_sw_packed {
+    int32_t nb_cells;       /* C */
+    int32_t max_cell_faces; /* MF (3 or 4), equal to the mesh's */
+    double *cellrec;        /* [C][8]     cx, cy, eta, hu, hv, h, Z, vol */
+    double *facerec;        /* [C][MF][4] eta_f, hu_f, hv_f, z_f at each face centre */
+    double *cfrec;          /* [C][MF][4] rx, ry (face centre - cell centre), z_f, 0 */
+    const double *cfgeo;    /* [C][MF][4] n0, n1, mesure, 0 of the face (faces.normal orientation) */
+    const int32_t *nbr;     /* [MF][C]    (j << 3) | (k << 1) | side, -1 on walls: j = cell across
+                                          the face, k = the face's slot in j's faceid row,
+                                          side = 1 when the cell is faces.cellid[f][1] */
+    const uint8_t *order;   /* [C]        slot of the r-th residual contribution in bits 2r..2r+1 */
+} mfv_sw_packed;
+
+/* Fills cellrec from (h, hu, hv, Z) and z_f = Z + (order-1)(Z_x rx + Z_y ry) into cfrec. */
+int mfv_sw_pack(const mfv_mesh *m, const mfv_sw_packed *p, const double *h, const double *hu,
+                const double *hv, const double *Z, const double *zgrad, int order, void *stream);
+
+/* Copies h, hu, hv out of cellrec. */
+int mfv_sw_unpack(const mfv_mesh *m, const mfv_sw_packed *p, double *h, double *hu, double *hv,
+                  void *stream);
+
+/* Gradients + limiter + MUSCL face values into facerec (as mfv_sw_gradients). */
+int mfv_sw_recon(const mfv_mesh *m, const mfv_sw_packed *p, int order, void *stream);
+
+/* Fluxes + update_SW + next dt, in place on cellrec (as mfv_sw_step_fused). */
+int mfv_sw_flux_packed(const mfv_mesh *m, const mfv_sw_packed *p, double grav, double cfl,
+                       double *dt_ring, int step, void *stream);
+
 /* Sets dt[0] = 1e6, the initial value of the reference's dt min-loops. */
 int mfv_dt_reset(double *dt, void *stream);
 

@@ -33,7 +33,14 @@ class MfvMesh(ctypes.Structure):
     ]
 
 
+class MfvSwPacked(ctypes.Structure):
+    """Mirror of ``struct mfv_sw_packed`` (include/mfv.h)."""
+    _fields_ = [('nb_cells', ctypes.c_int32), ('max_cell_faces', ctypes.c_int32), ('cellrec', P), ('facerec', P),
+                ('cfrec', P), ('cfgeo', P), ('nbr', P), ('order', P)]
+
+
 MESH_P = ctypes.POINTER(MfvMesh)
+SWP_P = ctypes.POINTER(MfvSwPacked)
 DP = ctypes.POINTER(ctypes.c_double)
 
 # name -> argtypes (restype is int unless listed in _RESTYPE)
@@ -56,6 +63,10 @@ _SIGNATURES = {
     'mfv_update_sw': [MESH_P, P, P, P, P, P, P, P],
     'mfv_sw_step_fused': [MESH_P, P, P, P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_dt_reset': [P, P],
+    'mfv_sw_pack': [MESH_P, SWP_P, P, P, P, P, P, c_int, P],
+    'mfv_sw_unpack': [MESH_P, SWP_P, P, P, P, P],
+    'mfv_sw_recon': [MESH_P, SWP_P, c_int, P],
+    'mfv_sw_flux_packed': [MESH_P, SWP_P, c_double, c_double, P, c_int, P],
     'mfv_euler3d_ghost': [MESH_P, P, P, c_int, P],
     'mfv_euler3d_time_step': [MESH_P, P, c_double, c_double, P, DP, P],
     'mfv_euler3d_rusanov': [MESH_P, P, P, c_double, c_int, P, P],

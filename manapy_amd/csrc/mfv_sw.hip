@@ -416,3 +416,351 @@ extern "C" int mfv_sw_step_fused(const mfv_mesh *m, const double *state_in, doub
                        state_in, state_out, Z, grad, zgrad, grav, cfl, (double)(order - 1), r.cur, r.next, r.spare);
     return check_launch("k_sw_step");
 }
+
+// =============================================================================================
+// Packed shallow-water path.
+//
+// Per-cell and per-cell-face records so that each gather is one or two contiguous 16-byte
+// loads instead of one scattered load per array:
+//   cellrec [C][8]      cx, cy, eta, hu, hv, h, Z, vol   (64 bytes, one record per cell)
+//   facerec [C][MF][4]  eta_f, hu_f, hv_f, z_f           (MUSCL values at the face centres)
+//   cfrec   [C][MF][4]  rx, ry, z_f, 0                   (face centre - cell centre; static)
+//   cfgeo   [C][MF][4]  n0, n1, mesure, 0                (faces.normal / mesure; static)
+//   nbr     [MF][C]     (j << 3) | (k << 1) | side, or -1 on a wall: j = cell across the face,
+//                       k = the slot of the face in j's faceid row, side = 1 when the cell is
+//                       faces.cellid[f][1]
+//   order   [C]         residual order: slot of the r-th contribution in bits 2r..2r+1
+// Slots follow cells.faceid order. The arithmetic is the unpacked kernels' expression for
+// expression, so both paths give the same bits.
+namespace {
+
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_pack(Mesh m, mfv_sw_packed p, const double *__restrict__ h,
+                                                     const double *__restrict__ hu, const double *__restrict__ hv,
+                                                     const double *__restrict__ Z, const double *__restrict__ zgrad,
+                                                     double om)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const size_t C = m.C;
+    double *r = p.cellrec + 8 * (size_t)i;
+    r[0] = m.cell_center[3 * (size_t)i];
+    r[1] = m.cell_center[3 * (size_t)i + 1];
+    r[2] = h[i] + Z[i];
+    r[3] = hu[i];
+    r[4] = hv[i];
+    r[5] = h[i];
+    r[6] = Z[i];
+    r[7] = m.cell_volume[i];
+    const int nf = m.cell_nfaces[i];
+    for (int s = 0; s < MF && s < nf; s++) {
+        double *q = p.cfrec + 4 * (MF * (size_t)i + s);
+        q[2] = Z[i] + om * (zgrad[i] * q[0] + zgrad[C + i] * q[1]);
+    }
+}
+
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_unpack(int C, const double *__restrict__ cellrec,
+                                                       double *__restrict__ h, double *__restrict__ hu,
+                                                       double *__restrict__ hv)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const double *r = cellrec + 8 * (size_t)i;
+    h[i] = r[5];
+    hu[i] = r[3];
+    hv[i] = r[4];
+}
+
+__device__ __forceinline__ void load_ehv(const double *__restrict__ cellrec, int c, double &cx, double &cy,
+                                         double &e, double &u, double &v)
+{
+    const double2 *r = reinterpret_cast<const double2 *>(cellrec + 8 * (size_t)c);
+    double2 a = r[0], b = r[1], d = r[2];
+    cx = a.x;
+    cy = a.y;
+    e = b.x;
+    u = b.y;
+    v = d.x;
+}
+
+// Gradients, limiter and MUSCL face values (k_sw_gradients + the reconstruction of
+// sw_reconstruct, written per face slot).
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, double om)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const size_t C = m.C;
+    double cx, cy, ei, ui, vi;
+    load_ehv(p.cellrec, i, cx, cy, ei, ui, vi);
+    double ixx = 0., iyy = 0., ixy = 0.;
+    double je0 = 0., je1 = 0., ju0 = 0., ju1 = 0., jv0 = 0., jv1 = 0.;
+    const int nn = m.lsq_ncells[i];
+    for (int j = 0; j < nn; j++) {
+        int c = m.lsq_cells[(size_t)j * C + i];
+        double px, py, ec, uc, vc;
+        load_ehv(p.cellrec, c, px, py, ec, uc, vc);
+        double jx = px - cx, jy = py - cy;
+        ixx += jx * jx;
+        iyy += jy * jy;
+        ixy += (jx * jy);
+        double de = ec - ei, du = uc - ui, dv = vc - vi;
+        je0 += (jx * de);
+        je1 += (jy * de);
+        ju0 += (jx * du);
+        ju1 += (jy * du);
+        jv0 += (jx * dv);
+        jv1 += (jy * dv);
+    }
+    const int ng = m.lsq_nghosts[i];
+    for (int j = 0; j < ng; j++) {
+        int f = m.lsq_ghosts[(size_t)j * C + i];
+        int o = m.face_cells[2 * f];
+        const double *g = m.face_ghost + 3 * (size_t)f;
+        double px, py, eo, uo, vo;
+        load_ehv(p.cellrec, o, px, py, eo, uo, vo);
+        double jx = g[0] - cx, jy = g[1] - cy;
+        ixx += jx * jx;
+        iyy += jy * jy;
+        ixy += (jx * jy);
+        double de = eo - ei, du = (-1 * uo) - ui, dv = (-1 * vo) - vi;
+        je0 += (jx * de);
+        je1 += (jy * de);
+        ju0 += (jx * du);
+        ju1 += (jy * du);
+        jv0 += (jx * dv);
+        jv1 += (jy * dv);
+    }
+    const double dia = ixx * iyy - ixy * ixy;
+    const double ex = (iyy * je0 - ixy * je1) / dia, ey = (ixx * je1 - ixy * je0) / dia;
+    const double ux = (iyy * ju0 - ixy * ju1) / dia, uy = (ixx * ju1 - ixy * ju0) / dia;
+    const double vx = (iyy * jv0 - ixy * jv1) / dia, vy = (ixx * jv1 - ixy * jv0) / dia;
+
+    const int nf = m.cell_nfaces[i];
+    double emax = ei, emin = ei, umax = ui, umin = ui, vmax = vi, vmin = vi;
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        if (s >= nf) break;
+        int code = p.nbr[(size_t)s * C + i];
+        double ea, ua, va, eb, ub, vb;
+        if (code >= 0) {
+            double px, py, ej, uj, vj;
+            load_ehv(p.cellrec, code >> 3, px, py, ej, uj, vj);
+            if (code & 1) {
+                ea = ej; ua = uj; va = vj;
+                eb = ei; ub = ui; vb = vi;
+            } else {
+                ea = ei; ua = ui; va = vi;
+                eb = ej; ub = uj; vb = vj;
+            }
+        } else {
+            ea = ei; ua = ui; va = vi;
+            eb = ei; ub = -1 * ui; vb = -1 * vi;
+        }
+        emax = pmax(pmax(emax, ea), eb);
+        emin = pmin(pmin(emin, ea), eb);
+        umax = pmax(pmax(umax, ua), ub);
+        umin = pmin(pmin(umin, ua), ub);
+        vmax = pmax(pmax(vmax, va), vb);
+        vmin = pmin(pmin(vmin, va), vb);
+    }
+    double rx[MF], ry[MF], zf[MF];
+    double pe = 1., pu = 1., pv = 1., qe = 1., qu = 1., qv = 1.;
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        const double2 *q = reinterpret_cast<const double2 *>(p.cfrec + 4 * (MF * (size_t)i + s));
+        double2 a = q[0], b = q[1];
+        rx[s] = a.x;
+        ry[s] = a.y;
+        zf[s] = b.x;
+        if (s >= nf) continue;
+        double r1 = rx[s], r2 = ry[s], d;
+        d = ex * r1 + ey * r2;
+        if (fabs(d) < 1e-8) qe = 1.;
+        else {
+            if (d > 0.) qe = pmin(1., (emax - ei) / d);
+            if (d < 0.) qe = pmin(1., (emin - ei) / d);
+        }
+        pe = pmin(pe, qe);
+        d = ux * r1 + uy * r2;
+        if (fabs(d) < 1e-8) qu = 1.;
+        else {
+            if (d > 0.) qu = pmin(1., (umax - ui) / d);
+            if (d < 0.) qu = pmin(1., (umin - ui) / d);
+        }
+        pu = pmin(pu, qu);
+        d = vx * r1 + vy * r2;
+        if (fabs(d) < 1e-8) qv = 1.;
+        else {
+            if (d > 0.) qv = pmin(1., (vmax - vi) / d);
+            if (d < 0.) qv = pmin(1., (vmin - vi) / d);
+        }
+        pv = pmin(pv, qv);
+    }
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        if (s >= nf) break;
+        double2 *o = reinterpret_cast<double2 *>(p.facerec + 4 * (MF * (size_t)i + s));
+        double2 a, b;
+        a.x = ei + om * pe * (ex * rx[s] + ey * ry[s]);
+        a.y = ui + om * pu * (ux * rx[s] + uy * ry[s]);
+        b.x = vi + om * pv * (vx * rx[s] + vy * ry[s]);
+        b.y = zf[s];
+        o[0] = a;
+        o[1] = b;
+    }
+}
+
+// Fluxes + bathymetry terms + update_SW + next dt, in place on cellrec.
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_flux_packed(Mesh m, mfv_sw_packed p, double g, double cfl,
+                                                            const double *__restrict__ dt_cur, double *dt_next,
+                                                            double *dt_spare)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    ring_reset_spare(dt_spare);
+    double dloc = 1e300;
+    if (i < m.C) {
+        const size_t C = m.C;
+        double *rec = p.cellrec + 8 * (size_t)i;
+        const double2 *r2 = reinterpret_cast<const double2 *>(rec);
+        double2 c1 = r2[1], c2 = r2[2], c3 = r2[3];
+        const double hu_i = c1.y, hv_i = c2.x, h_i = c2.y, Z_i = c3.x, vol = c3.y;
+        const int nf = m.cell_nfaces[i];
+        const unsigned ord = p.order[i];
+        double r0 = 0., r1 = 0., rr2 = 0., s1 = 0., s2 = 0.;
+        // contributions in the residual order; slot s of the r-th one is in bits 2r..2r+1
+#pragma unroll
+        for (int r = 0; r < MF; r++) {
+            if (r >= nf) break;
+            const int s = (ord >> (2 * r)) & 3;
+            const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
+            double2 ga = gq[0], gb = gq[1];
+            const double n0 = ga.x, n1 = ga.y, mes = gb.x;
+            const double2 *oq = reinterpret_cast<const double2 *>(p.facerec + 4 * (MF * (size_t)i + s));
+            double2 oa = oq[0], ob = oq[1];
+            const int code = p.nbr[(size_t)s * C + i];
+            double eR, uR, vR, zR;
+            if (code >= 0) {
+                int j = code >> 3, k = (code >> 1) & 3;
+                const double2 *nq = reinterpret_cast<const double2 *>(p.facerec + 4 * (MF * (size_t)j + k));
+                double2 na = nq[0], nb = nq[1];
+                eR = na.x; uR = na.y; vR = nb.x; zR = nb.y;
+            } else {
+                eR = oa.x; uR = -1 * oa.y; vR = -1 * ob.x; zR = ob.y;
+            }
+            double F0, F1, F2, bL, bR;
+            if (code < 0 || (code & 1) == 0) {
+                sw_face(oa.x, oa.y, ob.x, ob.y, h_i, Z_i, eR, uR, vR, zR, h_i, Z_i, n0, n1, mes, g, F0, F1, F2, bL,
+                        bR);
+                r0 -= F0; r1 -= F1; rr2 -= F2;
+                s1 -= bL * n0;
+                s2 -= bL * n1;
+            } else {
+                sw_face(eR, uR, vR, zR, h_i, Z_i, oa.x, oa.y, ob.x, ob.y, h_i, Z_i, n0, n1, mes, g, F0, F1, F2, bL,
+                        bR);
+                r0 += F0; r1 += F1; rr2 += F2;
+                s1 -= -(bR * n0);
+                s2 -= -(bR * n1);
+            }
+        }
+        const double dt = dt_cur[0];
+        const double s0 = 0.;
+        double hn = h_i + dt * (r0 + s0) / vol;
+        double un = hu_i + dt * ((r1 + s1) / vol + 0. + 0.);
+        double vn = hv_i + dt * ((rr2 + s2) / vol + 0. + 0.);
+        double2 w1, w2;
+        w1.x = hn + Z_i;
+        w1.y = un;
+        w2.x = vn;
+        w2.y = hn;
+        reinterpret_cast<double2 *>(rec)[1] = w1;
+        reinterpret_cast<double2 *>(rec)[2] = w2;
+        // time_step_SW of the new state, faces in faceid order
+        const double velson = sqrt(g * hn);
+        double lam = 0.;
+#pragma unroll
+        for (int s = 0; s < MF; s++) {
+            if (s >= nf) break;
+            const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
+            double2 ga = gq[0], gb = gq[1];
+            double n0 = ga.x, n1 = ga.y, mesu = gb.x;
+            double u_n = fabs(un / hn * n0 + vn / hn * n1);
+            double lam_convect = u_n / mesu + velson;
+            lam += lam_convect * mesu;
+            double mes = sqrt(n0 * n0 + n1 * n1);
+            double lam_diff = 0. * (mes * mes) + 0. * (mes * mes);
+            lam += lam_diff / vol;
+        }
+        dloc = cfl * vol / lam;
+    }
+    wave_min_to(dt_next, dloc);
+}
+
+}  // namespace
+
+static int check_packed(const mfv_mesh *m, const mfv_sw_packed *p)
+{
+    MFV_REQUIRE(p != nullptr, "packed descriptor is NULL");
+    MFV_REQUIRE(p->nb_cells == m->nb_cells && p->max_cell_faces == m->max_cell_faces,
+                "packed descriptor does not match the mesh");
+    MFV_REQUIRE(m->max_cell_faces == 3 || m->max_cell_faces == 4, "packed path supports 3 or 4 faces per cell");
+    MFV_REQUIRE(p->cellrec && p->facerec && p->cfrec && p->cfgeo && p->nbr && p->order, "packed descriptor "
+                "has NULL arrays");
+    return MFV_OK;
+}
+
+extern "C" int mfv_sw_pack(const mfv_mesh *m, const mfv_sw_packed *p, const double *h, const double *hu,
+                           const double *hv, const double *Z, const double *zgrad, int order, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_packed(m, p)) return rc;
+    MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    MFV_REQUIRE(h && hu && hv && Z && zgrad, "NULL array");
+    if (m->nb_cells == 0) return MFV_OK;
+    auto k = (m->max_cell_faces == 3) ? k_sw_pack<3> : k_sw_pack<4>;
+    hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p, h, hu,
+                       hv, Z, zgrad, (double)(order - 1));
+    return check_launch("k_sw_pack");
+}
+
+extern "C" int mfv_sw_unpack(const mfv_mesh *m, const mfv_sw_packed *p, double *h, double *hu, double *hv,
+                             void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_packed(m, p)) return rc;
+    MFV_REQUIRE(h && hu && hv, "NULL array");
+    if (m->nb_cells == 0) return MFV_OK;
+    hipLaunchKernelGGL(k_sw_unpack<3>, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                       m->nb_cells, p->cellrec, h, hu, hv);
+    return check_launch("k_sw_unpack");
+}
+
+extern "C" int mfv_sw_recon(const mfv_mesh *m, const mfv_sw_packed *p, int order, void *stream)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_packed(m, p)) return rc;
+    MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    if (m->nb_cells == 0) return MFV_OK;
+    auto k = (m->max_cell_faces == 3) ? k_sw_recon<3> : k_sw_recon<4>;
+    hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p,
+                       (double)(order - 1));
+    return check_launch("k_sw_recon");
+}
+
+extern "C" int mfv_sw_flux_packed(const mfv_mesh *m, const mfv_sw_packed *p, double grav, double cfl,
+                                  double *dt_ring_p, int step, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_packed(m, p)) return rc;
+    MFV_REQUIRE(dt_ring_p, "dt ring is NULL");
+    if (m->nb_cells == 0) return MFV_OK;
+    DtRing r = dt_ring(dt_ring_p, step);
+    auto k = (m->max_cell_faces == 3) ? k_sw_flux_packed<3> : k_sw_flux_packed<4>;
+    hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p, grav,
+                       cfl, r.cur, r.next, r.spare);
+    return check_launch("k_sw_flux_packed");
+}

@@ -145,3 +145,81 @@ class DeviceMesh(object):
     def to_device(self, a):
         import torch
         return torch.as_tensor(np.ascontiguousarray(a), device=self.device)
+
+
+def sw_packed_layout(domain):
+    """Host tables of the packed shallow-water path (see mfv_sw_packed in include/mfv.h)."""
+    C = int(domain.nbcells)
+    faceid = domain.cells.faceid
+    MF = faceid.shape[1] - 1
+    if MF not in (3, 4):
+        raise ValueError("packed shallow-water path needs 3 or 4 faces per cell")
+    nf = faceid[:, -1]
+    cellid = domain.faces.cellid
+    slot_mask = np.arange(MF)[None, :] < nf[:, None]
+    fids = np.where(slot_mask, faceid[:, :MF], 0).astype(np.int64)
+    rows = np.repeat(np.arange(C, dtype=np.int64), MF).reshape(C, MF)
+    a = cellid[fids, 0].astype(np.int64)
+    b = cellid[fids, 1].astype(np.int64)
+    side = (b == rows) & (b >= 0)
+    other = np.where(side, a, b)
+    # slot of face f in the other cell's row
+    F = int(domain.nbfaces)
+    r_idx, s_idx = np.nonzero(slot_mask)
+    key = fids[r_idx, s_idx] * 2 + (cellid[fids[r_idx, s_idx], 1] == r_idx)
+    lut = np.full(2 * F, -1, dtype=np.int64)
+    lut[key] = s_idx
+    wall = (b < 0) | ~slot_mask
+    other_side = (~side).astype(np.int64)  # the other cell is cellid[1] when we are cellid[0]
+    k = lut[fids * 2 + other_side]
+    code = (other << 3) | (k << 1) | side.astype(np.int64)
+    code = np.where(wall, -1, code)
+    if np.any(code[~wall] < 0) or np.any(k[~wall] < 0):
+        raise ValueError("inconsistent face/cell tables")
+    nbr = np.ascontiguousarray(code.T, dtype=np.int32)
+    cf = domain.faces.center
+    cc = domain.cells.center
+    cfrec = np.zeros((C, MF, 4))
+    cfrec[:, :, 0] = np.where(slot_mask, cf[fids, 0] - cc[:, None, 0], 0.)
+    cfrec[:, :, 1] = np.where(slot_mask, cf[fids, 1] - cc[:, None, 1], 0.)
+    cfgeo = np.zeros((C, MF, 4))
+    cfgeo[:, :, 0] = np.where(slot_mask, domain.faces.normal[fids, 0], 0.)
+    cfgeo[:, :, 1] = np.where(slot_mask, domain.faces.normal[fids, 1], 0.)
+    cfgeo[:, :, 2] = np.where(slot_mask, domain.faces.mesure[fids], 1.)
+    L = gpu_layout(domain, lsq=False)
+    osplit = L['order_split'].T.astype(np.uint32)  # [C][MF]
+    order = np.zeros(C, dtype=np.uint32)
+    for r in range(MF):
+        order |= (osplit[:, r] & 3) << (2 * r)
+    return {'nbr': nbr, 'cfrec': cfrec, 'cfgeo': cfgeo, 'order': order.astype(np.uint8), 'MF': MF}
+
+
+class SwPacked(object):
+    """Device tables and records of the packed shallow-water path for a DeviceMesh."""
+
+    def __init__(self, dmesh):
+        import torch
+        host = sw_packed_layout(dmesh.domain)
+        C = dmesh.C
+        MF = host['MF']
+        dev = dmesh.device
+        self.cellrec = torch.zeros((C, 8), dtype=torch.float64, device=dev)
+        self.facerec = torch.zeros((C, MF, 4), dtype=torch.float64, device=dev)
+        self.cfrec = torch.as_tensor(host['cfrec'], device=dev)
+        self.cfgeo = torch.as_tensor(host['cfgeo'], device=dev)
+        self.nbr = torch.as_tensor(host['nbr'], device=dev)
+        self.order = torch.as_tensor(host['order'], device=dev)
+        d = _lib.MfvSwPacked()
+        d.nb_cells = C
+        d.max_cell_faces = MF
+        d.cellrec = self.cellrec.data_ptr()
+        d.facerec = self.facerec.data_ptr()
+        d.cfrec = self.cfrec.data_ptr()
+        d.cfgeo = self.cfgeo.data_ptr()
+        d.nbr = self.nbr.data_ptr()
+        d.order = self.order.data_ptr()
+        self.desc = d
+
+    @property
+    def ptr(self):
+        return ctypes.byref(self.desc)

@@ -313,3 +313,77 @@ def test_euler_fused_40_steps(torch, quirk, d3, m3, oracle_lib):
     for s in range(40):
         call('mfv_euler3d_step_fused', m.ptr, P(S[s % 2]), P(S[(s + 1) % 2]), 1.4, 0.8, quirk, P(ring), s)
     same(host(S[0]), U, "U after 40 steps")
+
+
+def test_sw_packed_60_steps(torch, sw_case, oracle_lib):
+    """The packed path (one 64-byte record per cell, MUSCL values per face slot) gives the
+    same bits as the reference step sequence."""
+    from manapy_amd.device import SwPacked
+    d, m, Z, h0 = sw_case
+    C = int(d.nbcells)
+    G = 9.81
+    h, hu, hv = h0.copy(), np.zeros(C), np.zeros(C)
+    zg_ref = oracle_lib.z_gradient(d, Z)
+    dts = [oracle_lib.sw_step(d, h, hu, hv, Z, zg_ref, G, 0.8, 2) for _ in range(60)]
+    p = SwPacked(m)
+    th, thu, thv, tZ, zg = (dev(torch, x) for x in (h0, np.zeros(C), np.zeros(C), Z, zg_ref))
+    call('mfv_sw_pack', m.ptr, p.ptr, P(th), P(thu), P(thv), P(tZ), P(zg), 2)
+    dt0 = dev(torch, np.zeros(1))
+    ring = dev(torch, np.zeros(3))
+    call('mfv_time_step_sw', m.ptr, P(th), P(thu), P(thv), 0.8, G, 0., 0., P(dt0), None)
+    call('mfv_dt_ring_init', P(ring), P(dt0))
+    for s in range(60):
+        call('mfv_sw_recon', m.ptr, p.ptr, 2)
+        call('mfv_sw_flux_packed', m.ptr, p.ptr, G, 0.8, P(ring), s)
+    oh, ohu, ohv = (dev(torch, np.zeros(C)) for _ in range(3))
+    call('mfv_sw_unpack', m.ptr, p.ptr, P(oh), P(ohu), P(ohv))
+    same(host(oh), h, "h")
+    same(host(ohu), hu, "hu")
+    same(host(ohv), hv, "hv")
+    assert host(ring)[59 % 3] == dts[-1]
+    # eta stored in the record is h + Z of the new state
+    rec = host(p.cellrec)
+    same(rec[:, 2], h + Z, "eta")
+
+
+def test_sw_packed_first_order_and_quads(torch, oracle_lib):
+    """Order 1 on a mixed triangle/quad mesh (max 4 faces per cell)."""
+    from manapy_amd.ddm import Domain
+    from manapy_amd.device import DeviceMesh, SwPacked
+    from manapy_amd.meshgen import jittered_triangles
+    el, nd = jittered_triangles(20, 14, 100., 60., 0.0, 4)
+    # merge every other triangle pair of the first rows into a quad (a, b, c, d)
+    quads = []
+    keep = np.ones(len(el), dtype=bool)
+    for q in range(0, 40, 2):
+        t1, t2 = el[2 * q], el[2 * q + 1]
+        quads.append([t1[0], t1[1], t1[2], t2[2], 4])
+        keep[2 * q] = keep[2 * q + 1] = False
+    el4 = np.concatenate([np.concatenate([el[keep][:, :3], -np.ones((keep.sum(), 1), np.int32),
+                                          el[keep][:, 3:]], axis=1), np.array(quads, dtype=np.int32)])
+    d = Domain(2, mesh=(el4, nd))
+    assert d.cells.faceid.shape[1] == 5
+    C = int(d.nbcells)
+    c = d.cells.center
+    Z = 0.1 * np.sin(c[:, 0] / 10.)
+    h0 = np.where(c[:, 0] < 50., 2.0, 1.0)
+    h, hu, hv = h0.copy(), np.zeros(C), np.zeros(C)
+    zg_ref = oracle_lib.z_gradient(d, Z)
+    for _ in range(30):
+        oracle_lib.sw_step(d, h, hu, hv, Z, zg_ref, 9.81, 0.8, 1)
+    m = DeviceMesh(d)
+    p = SwPacked(m)
+    th, thu, thv, tZ, zg = (dev(torch, x) for x in (h0, np.zeros(C), np.zeros(C), Z, zg_ref))
+    call('mfv_sw_pack', m.ptr, p.ptr, P(th), P(thu), P(thv), P(tZ), P(zg), 1)
+    dt0 = dev(torch, np.zeros(1))
+    ring = dev(torch, np.zeros(3))
+    call('mfv_time_step_sw', m.ptr, P(th), P(thu), P(thv), 0.8, 9.81, 0., 0., P(dt0), None)
+    call('mfv_dt_ring_init', P(ring), P(dt0))
+    for s in range(30):
+        call('mfv_sw_recon', m.ptr, p.ptr, 1)
+        call('mfv_sw_flux_packed', m.ptr, p.ptr, 9.81, 0.8, P(ring), s)
+    oh, ohu, ohv = (dev(torch, np.zeros(C)) for _ in range(3))
+    call('mfv_sw_unpack', m.ptr, p.ptr, P(oh), P(ohu), P(ohv))
+    same(host(oh), h, "h")
+    same(host(ohu), hu, "hu")
+    same(host(ohv), hv, "hv")
