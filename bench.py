@@ -146,7 +146,7 @@ def main():
               zgrad[1].data_ptr(), zgrad[2].data_ptr(), s)
     grad = dv(np.zeros((9, C)))
     dt0 = dv(np.zeros(1))
-    ring = dv(np.zeros(3))
+    ring = dv(np.zeros(4))
     _lib.call('mfv_time_step_sw', m.ptr, S[0][0].data_ptr(), S[0][1].data_ptr(), S[0][2].data_ptr(), CFL, G, 0.,
               0., dt0.data_ptr(), None, s)
     _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)

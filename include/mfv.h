@@ -109,10 +109,11 @@ int mfv_explicitscheme_convective(const mfv_mesh *m, double *rez_w, const double
 int mfv_update_new_value(const mfv_mesh *m, double *ne_c, const double *rez, const double *dissip,
                          const double *src, double dtime, const double *dt_dev, void *stream);
 
-/* Fused steps keep dt on the device in a ring of three doubles: step k uses
- * dt_ring[k % 3], min-reduces the next step's dt into dt_ring[(k+1) % 3] (which must hold
- * 1e6 when step k starts) and sets dt_ring[(k+2) % 3] = 1e6 for step k+1. Initialise the
- * ring to {dt_0, 1e6, 1e6} (mfv_dt_ring_init) and pass k = 0, 1, 2, ... */
+/* Fused steps keep dt on the device in a ring of four doubles: step k uses dt_ring[k % 3],
+ * min-reduces the next step's dt into dt_ring[(k+1) % 3] (which must hold 1e6 when step k
+ * starts), sets dt_ring[(k+2) % 3] = 1e6 for step k+1 and adds its dt to dt_ring[3] (the
+ * simulated time). Initialise the ring to {dt_0, 1e6, 1e6, 0} (mfv_dt_ring_init) and pass
+ * k = 0, 1, 2, ... */
 int mfv_dt_ring_init(double *dt_ring, const double *dt0, void *stream);
 
 /* One fused explicit upwind step for linear advection (time_step + ghost Neumann +
@@ -187,6 +188,33 @@ int mfv_sw_recon(const mfv_mesh *m, const mfv_sw_packed *p, int order, void *str
 /* Fluxes + update_SW + next dt, in place on cellrec (as mfv_sw_step_fused). */
 int mfv_sw_flux_packed(const mfv_mesh *m, const mfv_sw_packed *p, double grav, double cfl,
                        double *dt_ring, int step, void *stream);
+
+/* Mixed-precision shallow-water path: float32 records, gradients and fluxes, Kahan-compensated
+ * float32 residual sums, float64 state update and dt; same scheme as the float64 paths. */
+typedef struct mfv_sw_mixed {
+    int32_t nb_cells;       /* C */
+    int32_t max_cell_faces; /* MF (3 or 4) */
+    double *state;          /* [C][4]     h, hu, hv, vol (float64) */
+    float *cellrec;         /* [C][8]     cx, cy, eta, hu, hv, h, Z, 0 */
+    float *facerec;         /* [C][MF][4] eta_f, hu_f, hv_f, z_f */
+    float *cfrec;           /* [C][MF][4] rx, ry, z_f, 0 */
+    const float *cfgeo;     /* [C][MF][4] n0, n1, mesure, 0 */
+    const int32_t *nbr;     /* [MF][C]    as mfv_sw_packed */
+    const uint8_t *order;   /* [C]        as mfv_sw_packed */
+} mfv_sw_mixed;
+
+int mfv_sw_pack_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, const double *h, const double *hu,
+                      const double *hv, const double *Z, const double *zgrad, int order, void *stream);
+int mfv_sw_unpack_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, double *h, double *hu, double *hv,
+                        void *stream);
+int mfv_sw_recon_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, int order, void *stream);
+int mfv_sw_flux_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, double grav, double cfl,
+                      double *dt_ring, int step, void *stream);
+
+/* Deterministic float64 totals sum_K (h, hu, hv)_K |K| of the mixed state into out3[0..2];
+ * partial: workspace of at least 3*ceil(C/256) doubles. */
+int mfv_sw_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, double *partial, int n_partial,
+                        double *out3, void *stream);
 
 /* Sets dt[0] = 1e6, the initial value of the reference's dt min-loops. */
 int mfv_dt_reset(double *dt, void *stream);

@@ -39,8 +39,15 @@ class MfvSwPacked(ctypes.Structure):
                 ('cfrec', P), ('cfgeo', P), ('nbr', P), ('order', P)]
 
 
+class MfvSwMixed(ctypes.Structure):
+    """Mirror of ``struct mfv_sw_mixed`` (include/mfv.h)."""
+    _fields_ = [('nb_cells', ctypes.c_int32), ('max_cell_faces', ctypes.c_int32), ('state', P), ('cellrec', P),
+                ('facerec', P), ('cfrec', P), ('cfgeo', P), ('nbr', P), ('order', P)]
+
+
 MESH_P = ctypes.POINTER(MfvMesh)
 SWP_P = ctypes.POINTER(MfvSwPacked)
+SWM_P = ctypes.POINTER(MfvSwMixed)
 DP = ctypes.POINTER(ctypes.c_double)
 
 # name -> argtypes (restype is int unless listed in _RESTYPE)
@@ -67,6 +74,11 @@ _SIGNATURES = {
     'mfv_sw_unpack': [MESH_P, SWP_P, P, P, P, P],
     'mfv_sw_recon': [MESH_P, SWP_P, c_int, P],
     'mfv_sw_flux_packed': [MESH_P, SWP_P, c_double, c_double, P, c_int, P],
+    'mfv_sw_pack_mixed': [MESH_P, SWM_P, P, P, P, P, P, c_int, P],
+    'mfv_sw_unpack_mixed': [MESH_P, SWM_P, P, P, P, P],
+    'mfv_sw_recon_mixed': [MESH_P, SWM_P, c_int, P],
+    'mfv_sw_flux_mixed': [MESH_P, SWM_P, c_double, c_double, P, c_int, P],
+    'mfv_sw_conservation': [MESH_P, SWM_P, P, c_int, P, P],
     'mfv_euler3d_ghost': [MESH_P, P, P, c_int, P],
     'mfv_euler3d_time_step': [MESH_P, P, c_double, c_double, P, DP, P],
     'mfv_euler3d_rusanov': [MESH_P, P, P, c_double, c_int, P, P],

@@ -1,0 +1,297 @@
+"""``HIPBackend``: the drop-in for manapy's ``Backend.compile`` seam.
+
+manapy binds every kernel once with ``backend.compile(func, signature=...)``
+(``backends/cpu/backend.py:69-93``) and calls the result with the kernel's own
+positional arguments. ``HIPBackend.compile`` looks the function up by name and
+returns a callable with the same positional signature that runs the libmfv entry
+point on the GPU. Mesh arrays in the argument list (centres, normals, connectivity,
+face lists) are taken from the Domain's device copy; their shapes are checked
+against the Domain. Field arguments may be device tensors (used in place) or NumPy
+arrays (copied to the device, outputs copied back in place), so a reference driver
+runs unchanged.
+
+A kernel with no device implementation raises ``NotImplementedError``: there is no
+CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .device import stream_ptr
+
+
+class _Args(object):
+    """Converts reference arguments to device tensors and writes outputs back."""
+
+    def __init__(self, device):
+        self.device = device
+        self.back = []
+
+    def dev(self, a, out=False):
+        import torch
+        if torch.is_tensor(a):
+            if not a.is_cuda or not a.is_contiguous() or a.dtype != torch.float64:
+                raise _lib.MfvError("tensors must be contiguous float64 device tensors (or pass numpy arrays)")
+            return a
+        arr = np.ascontiguousarray(a, dtype=np.float64)
+        t = torch.as_tensor(arr, device=self.device)
+        if out:
+            self.back.append((a, t))
+        return t
+
+    def finish(self):
+        import torch
+        if self.back:
+            torch.cuda.current_stream().synchronize()
+        for host, t in self.back:
+            host[...] = t.cpu().numpy()
+        self.back = []
+
+
+class HIPBackend(object):
+    """GPU backend of a Domain (``backends/base.py:4-6`` interface)."""
+
+    _registry = {}
+
+    def __init__(self, domain):
+        self.domain = domain
+        self.backend = "hip"
+        self.float_precision = 'f8'
+        self.int_precision = 'i4'
+
+    @classmethod
+    def for_domain(cls, domain):
+        return cls(domain)
+
+    @classmethod
+    def register(cls, *names):
+        def deco(fn):
+            for n in names:
+                cls._registry[n] = fn
+            return fn
+        return deco
+
+    def compile(self, func, signature=False, forcedbackend=None):
+        name = func if isinstance(func, str) else getattr(func, '__name__', None)
+        impl = self._registry.get(name)
+        if impl is None:
+            raise NotImplementedError("no device kernel for %r (the GPU backend covers the explicit finite-volume "
+                                      "step: %s)" % (name, ', '.join(sorted(self._registry))))
+        domain = self.domain
+
+        def kernel(*args, **kwargs):
+            return impl(domain, *args, **kwargs)
+
+        kernel.__name__ = name
+        kernel.__doc__ = impl.__doc__
+        return kernel
+
+    def supported(self):
+        return sorted(self._registry)
+
+
+def _m(domain):
+    return domain.device_mesh
+
+
+def _check_len(a, n, what):
+    if len(a) != n:
+        raise ValueError("%s has %d rows, the Domain has %d" % (what, len(a), n))
+
+
+@HIPBackend.register('explicitscheme_convective_2d', 'explicitscheme_convective_3d')
+def _explicitscheme_convective(domain, rez_w, w_c, w_ghost, w_halo, u_face, v_face, w_face, w_x, w_y, w_z,
+                               wx_halo, wy_halo, wz_halo, psi, psi_halo, centerc, centerf, centerh, centerg,
+                               cellidf, normalf, halofid, name, innerfaces, halofaces, boundaryfaces,
+                               periodicboundaryfaces, shift, order):
+    """solvers/advec/fvm_utils.py:19-152 / 156-290."""
+    m = _m(domain)
+    _check_len(centerc, m.C, "centerc")
+    _check_len(normalf, m.F, "normalf")
+    if len(periodicboundaryfaces) or len(halofaces):
+        raise NotImplementedError("one subdomain, non-periodic: halo/periodic face lists must be empty")
+    A = _Args(m.device)
+    rez = A.dev(rez_w, out=True)
+    ts = [A.dev(x) for x in (w_c, w_ghost, u_face, v_face, w_face, w_x, w_y, w_z, psi)]
+    _lib.call('mfv_explicitscheme_convective', m.ptr, rez.data_ptr(), *[t.data_ptr() for t in ts], int(order),
+              stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('time_step')
+def _time_step(domain, u, v, w, cfl, normal, mesure, volume, faceid, dim):
+    """solvers/advec/fvm_utils.py:293-309; returns dt (float)."""
+    import torch
+    m = _m(domain)
+    _check_len(faceid, m.C, "faceid")
+    A = _Args(m.device)
+    ts = [A.dev(x) for x in (u, v, w)]
+    dt = torch.zeros(1, dtype=torch.float64, device=m.device)
+    h = ctypes.c_double(0.)
+    _lib.call('mfv_time_step', m.ptr, *[t.data_ptr() for t in ts], float(cfl), dt.data_ptr(), ctypes.byref(h),
+              stream_ptr())
+    return h.value
+
+
+@HIPBackend.register('update_new_value')
+def _update_new_value(domain, ne_c, rez_ne, dissip_ne, src_ne, dtime, vol):
+    """solvers/advec/fvm_utils.py:311-315."""
+    m = _m(domain)
+    A = _Args(m.device)
+    ne = A.dev(ne_c, out=True)
+    ts = [A.dev(x) for x in (rez_ne, dissip_ne, src_ne)]
+    _lib.call('mfv_update_new_value', m.ptr, ne.data_ptr(), *[t.data_ptr() for t in ts], float(dtime), None,
+              stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('cell_gradient_2d', 'cell_gradient_3d')
+def _cell_gradient(domain, w_c, w_ghost, w_halo, w_haloghost, centerc, cellnid, ghostnid, haloghostnid, halonid,
+                   nodecid, periodicn, periodic, centergf, halocenterg, vertexn, centerh, shift, w_x, w_y, w_z):
+    """ast/functions2d.py:5-104, functions3d.py:4-120."""
+    m = _m(domain)
+    m.ensure_lsq()
+    _check_len(cellnid, m.C, "cellnid")
+    A = _Args(m.device)
+    wc, wg = A.dev(w_c), A.dev(w_ghost)
+    outs = [A.dev(x, out=True) for x in (w_x, w_y, w_z)]
+    _lib.call('mfv_cell_gradient', m.ptr, wc.data_ptr(), wg.data_ptr(), *[t.data_ptr() for t in outs], stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('barthlimiter_2d', 'barthlimiter_3d')
+def _barthlimiter(domain, w_c, w_ghost, w_halo, w_x, w_y, w_z, psi, cellid, faceid, namef, halofid, centerc,
+                  centerf):
+    """ast/functions2d.py:280-325, functions3d.py:364-408."""
+    m = _m(domain)
+    _check_len(faceid, m.C, "faceid")
+    A = _Args(m.device)
+    ts = [A.dev(x) for x in (w_c, w_ghost, w_x, w_y, w_z)]
+    p = A.dev(psi, out=True)
+    _lib.call('mfv_barthlimiter', m.ptr, *[t.data_ptr() for t in ts], p.data_ptr(), stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('facetocell')
+def _facetocell(domain, u_face, u_c, faceidc, dim):
+    """ast/ast_utils.py:8-18."""
+    m = _m(domain)
+    A = _Args(m.device)
+    uf = A.dev(u_face)
+    uc = A.dev(u_c, out=True)
+    _lib.call('mfv_facetocell', m.ptr, uf.data_ptr(), uc.data_ptr(), stream_ptr())
+    A.finish()
+
+
+def _ghost(kind):
+    def run(domain, w_c, w_ghost, cellid, faces, cst, dist):
+        import torch
+        m = _m(domain)
+        A = _Args(m.device)
+        src = A.dev(w_c)
+        g = A.dev(w_ghost, out=True)
+        f = torch.as_tensor(np.asarray(faces, dtype=np.int32) if not torch.is_tensor(faces) else faces,
+                            dtype=torch.int32, device=m.device)
+        _lib.call('mfv_ghost_value', m.ptr, kind, src.data_ptr(), g.data_ptr(), f.data_ptr(), len(f), stream_ptr())
+        A.finish()
+    run.__doc__ = "boundary/bc_utils.py:45-68"
+    return run
+
+
+HIPBackend.register('ghost_value_neumann')(_ghost(0))
+HIPBackend.register('ghost_value_nonslip')(_ghost(1))
+HIPBackend.register('ghost_value_dirichlet')(_ghost(2))
+
+
+@HIPBackend.register('time_step_SW')
+def _time_step_sw(domain, h_c, hu_c, hv_c, cfl, normal, mesure, volume, faceid, grav, Dxx, Dyy):
+    """solvers/shallowater/fvm_utils.py:16-42; returns dt (float)."""
+    import torch
+    m = _m(domain)
+    A = _Args(m.device)
+    ts = [A.dev(x) for x in (h_c, hu_c, hv_c)]
+    dt = torch.zeros(1, dtype=torch.float64, device=m.device)
+    h = ctypes.c_double(0.)
+    _lib.call('mfv_time_step_sw', m.ptr, *[t.data_ptr() for t in ts], float(cfl), float(grav), float(Dxx), float(Dyy),
+              dt.data_ptr(), ctypes.byref(h), stream_ptr())
+    return h.value
+
+
+@HIPBackend.register('update_SW')
+def _update_sw(domain, h_c, hu_c, hv_c, hc_c, Z_c, rez_h, rez_hu, rez_hv, rez_hc, rez_Z, src_h, src_hu, src_hv,
+               src_hc, src_Z, dissip_hc, corio_hu, corio_hv, wind_hu, wind_hv, dtime, vol):
+    """solvers/shallowater/fvm_utils.py:3-14 for h, hu, hv; the passive hc and Z must have zero
+    residual, source, dissipation, Coriolis and wind terms (the accelerated path's scope)."""
+    import torch
+    m = _m(domain)
+    A = _Args(m.device)
+    for arr in (rez_hc, rez_Z, src_hc, src_Z, dissip_hc, corio_hu, corio_hv):
+        a = arr if torch.is_tensor(arr) else torch.as_tensor(np.asarray(arr))
+        if bool((a != 0).any()):
+            raise NotImplementedError("update_SW on the device handles h, hu, hv with zero hc/Z/Coriolis terms")
+    if wind_hu != 0 or wind_hv != 0:
+        raise NotImplementedError("wind terms are not on the accelerated path")
+    hs = [A.dev(x, out=True) for x in (h_c, hu_c, hv_c)]
+    rez = torch.stack([A.dev(x) for x in (rez_h, rez_hu, rez_hv)]).contiguous()
+    src = torch.stack([A.dev(x) for x in (src_h, src_hu, src_hv)]).contiguous()
+    dt = torch.full((1,), float(dtime), dtype=torch.float64, device=m.device)
+    _lib.call('mfv_update_sw', m.ptr, *[t.data_ptr() for t in hs], rez.data_ptr(), src.data_ptr(), dt.data_ptr(),
+              stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('explicitscheme_euler_3d_rusanov')
+def _euler_rusanov(domain, rez_rho, rez_rhou, rez_rhov, rez_rhow, rez_rhoE, rho_c, P_c, rhou_c, rhov_c, rhow_c,
+                   rhoE_c, rho_g, P_g, rhou_g, rhov_g, rhow_g, rhoE_g, rho_h, P_h, rhou_h, rhov_h, rhow_h, rhoE_h,
+                   cellidf, halofid, normal, mesurf, name, gamma, tangent, binormal, quirk=1):
+    """solvers/euler/fvm_utils3d.py:1083-1170 (quirk=1: the reference's flux, Q1)."""
+    import torch
+    m = _m(domain)
+    A = _Args(m.device)
+    U = torch.stack([A.dev(x) for x in (rho_c, P_c, rhou_c, rhov_c, rhow_c, rhoE_c)]).contiguous()
+    G = torch.stack([A.dev(x) for x in (rho_g, P_g, rhou_g, rhov_g, rhow_g, rhoE_g)]).contiguous()
+    rez = torch.empty((5, m.C), dtype=torch.float64, device=m.device)
+    _lib.call('mfv_euler3d_rusanov', m.ptr, U.data_ptr(), G.data_ptr(), float(gamma), int(quirk), rez.data_ptr(),
+              stream_ptr())
+    for k, out in enumerate((rez_rho, rez_rhou, rez_rhov, rez_rhow, rez_rhoE)):
+        t = A.dev(out, out=True)
+        t.copy_(rez[k])
+    A.finish()
+
+
+@HIPBackend.register('update_euler_3d_fvc')
+def _euler_update(domain, rho_c, P_c, rhou_c, rhov_c, rhow_c, rhoE_c, rez_rho, rez_rhou, rez_rhov, rez_rhow,
+                  rez_rhoE, gamma, dtime, vol):
+    """solvers/euler/fvm_utils3d.py:1316-1327."""
+    import torch
+    m = _m(domain)
+    A = _Args(m.device)
+    outs = [A.dev(x, out=True) for x in (rho_c, P_c, rhou_c, rhov_c, rhow_c, rhoE_c)]
+    U = torch.stack(outs).contiguous()
+    rez = torch.stack([A.dev(x) for x in (rez_rho, rez_rhou, rez_rhov, rez_rhow, rez_rhoE)]).contiguous()
+    dt = torch.full((1,), float(dtime), dtype=torch.float64, device=m.device)
+    _lib.call('mfv_euler3d_update', m.ptr, U.data_ptr(), rez.data_ptr(), float(gamma), dt.data_ptr(), stream_ptr())
+    for k, t in enumerate(outs):
+        t.copy_(U[k])
+    A.finish()
+
+
+@HIPBackend.register('ghost_value_TubeSchok3D')
+def _euler_ghost(domain, rhog, Pg, rhoug, rhovg, rhowg, ug, vg, wg, rhoEg, rhoc, Pc, rhouc, rhovc, rhowc, rhoEc,
+                 cellid, name, normal, mesure, quirk=1):
+    """solvers/euler/fvm_utils3d.py:441-500 (quirk=1: the reference's slip ghost, Q2)."""
+    import torch
+    m = _m(domain)
+    A = _Args(m.device)
+    gouts = [A.dev(x, out=True) for x in (rhog, Pg, rhoug, rhovg, rhowg, rhoEg)]
+    G = torch.stack(gouts).contiguous()
+    U = torch.stack([A.dev(x) for x in (rhoc, Pc, rhouc, rhovc, rhowc, rhoEc)]).contiguous()
+    _lib.call('mfv_euler3d_ghost', m.ptr, U.data_ptr(), G.data_ptr(), int(quirk), stream_ptr())
+    for k, t in enumerate(gouts):
+        t.copy_(G[k])
+    bf = torch.as_tensor(domain.boundaryfaces, dtype=torch.long, device=m.device)
+    for vel, mom in ((ug, 2), (vg, 3), (wg, 4)):
+        t = A.dev(vel, out=True)
+        t[bf] = G[mom][bf] / G[0][bf]
+    A.finish()

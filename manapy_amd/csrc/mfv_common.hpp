@@ -56,20 +56,25 @@ __device__ __forceinline__ void wave_min_to(double *addr, double v)
 }
 
 // dt ring of the fused steps: slot k%3 is this step's dt, (k+1)%3 receives the next dt,
-// (k+2)%3 is reset for the step after (nothing reads or writes it during step k).
+// (k+2)%3 is reset for the step after (nothing reads or writes it during step k); slot 3
+// accumulates the simulated time (t += dt of each step).
 struct DtRing {
     const double *cur;
     double *next;
     double *spare;
+    double *time;
 };
 inline DtRing dt_ring(double *ring, int step)
 {
     int k = ((step % 3) + 3) % 3;
-    return DtRing{ring + k, ring + (k + 1) % 3, ring + (k + 2) % 3};
+    return DtRing{ring + k, ring + (k + 1) % 3, ring + (k + 2) % 3, ring + 3};
 }
-__device__ __forceinline__ void ring_reset_spare(double *spare)
+__device__ __forceinline__ void ring_advance(const DtRing &r)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) spare[0] = 1e6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        r.spare[0] = 1e6;
+        r.time[0] += r.cur[0];
+    }
 }
 
 // Mesh descriptor passed by value to kernels.

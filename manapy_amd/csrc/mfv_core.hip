@@ -271,6 +271,7 @@ __global__ void k_dt_ring_init(double *ring, const double *dt0)
     ring[0] = dt0[0];
     ring[1] = 1e6;
     ring[2] = 1e6;
+    ring[3] = 0.;
 }
 
 extern "C" int mfv_dt_ring_init(double *dt_ring_p, const double *dt0, void *stream)
@@ -517,22 +518,21 @@ __global__ void __launch_bounds__(kBlock) k_adv_step(Mesh m, const double *__res
                                                       const double *__restrict__ vf,
                                                       const double *__restrict__ wf, double cfl, double om,
                                                       const double *__restrict__ scratch,
-                                                      const double *__restrict__ dt_cur, double *dt_next,
-                                                      double *dt_spare)
+                                                      DtRing ring)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    ring_reset_spare(dt_spare);
+    ring_advance(ring);
     double dloc = 1e300;
     if (i < m.C) {
         const double *gx = scratch, *gy = scratch + m.C, *gz = scratch + 2 * (size_t)m.C;
         const double *psi = scratch + (size_t)DIM * m.C;
         double rez = adv_residual<DIM, ORDER, true>(m, i, w_in, nullptr, uf, vf, wf, gx, gy, gz, psi, om);
-        double dt = dt_cur[0];
+        double dt = ring.cur[0];
         // update_new_value with the dissipative and source terms zero
         w_out[i] = w_in[i] + dt * ((rez + 0.) / m.cell_volume[i] + 0.);
         dloc = adv_cell_dt(m, i, uc[i], vc[i], wc[i], cfl);
     }
-    wave_min_to(dt_next, dloc);
+    wave_min_to(ring.next, dloc);
 }
 
 extern "C" int mfv_advection_step(const mfv_mesh *m, const double *w_in, double *w_out, const double *u_c,
@@ -558,6 +558,6 @@ extern "C" int mfv_advection_step(const mfv_mesh *m, const double *w_in, double 
     auto ks = (m->dim == 2) ? ((order == 2) ? k_adv_step<2, 2> : k_adv_step<2, 1>)
                             : ((order == 2) ? k_adv_step<3, 2> : k_adv_step<3, 1>);
     hipLaunchKernelGGL(ks, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, st, k, w_in, w_out, u_c, v_c, w_c,
-                       u_face, v_face, w_face, cfl, (double)(order - 1), scratch, r.cur, r.next, r.spare);
+                       u_face, v_face, w_face, cfl, (double)(order - 1), scratch, r);
     return check_launch("k_adv_step");
 }

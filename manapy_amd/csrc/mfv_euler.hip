@@ -214,11 +214,10 @@ __global__ void __launch_bounds__(kBlock) k_euler_update(int C, double *__restri
 
 __global__ void __launch_bounds__(kBlock) k_euler_step(Mesh m, const double *__restrict__ Uin,
                                                         double *__restrict__ Uout, double gamma, double cfl,
-                                                        int quirk, const double *__restrict__ dt_cur,
-                                                        double *dt_next, double *dt_spare)
+                                                        int quirk, DtRing ring)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    ring_reset_spare(dt_spare);
+    ring_advance(ring);
     double d = 1e300;
     if (i < m.C) {
         const size_t C = m.C;
@@ -226,11 +225,11 @@ __global__ void __launch_bounds__(kBlock) k_euler_step(Mesh m, const double *__r
         euler_cell_residual<false>(m, i, Uin, nullptr, gamma, quirk, r);
         double u[6];
         for (int v = 0; v < 6; v++) u[v] = Uin[v * C + i];
-        euler_update_cell(u, r, gamma, dt_cur[0], m.cell_volume[i]);
+        euler_update_cell(u, r, gamma, ring.cur[0], m.cell_volume[i]);
         for (int v = 0; v < 6; v++) Uout[v * C + i] = u[v];
         d = euler_cell_dt(m, i, u[0], u[1], u[2], u[3], u[4], cfl, gamma);
     }
-    wave_min_to(dt_next, d);
+    wave_min_to(ring.next, d);
 }
 
 }  // namespace
@@ -301,6 +300,6 @@ extern "C" int mfv_euler3d_step_fused(const mfv_mesh *m, const double *U_in, dou
     MFV_REQUIRE(U_in && U_out && U_in != U_out && dt_ring_p, "NULL or aliased array");
     DtRing r = dt_ring(dt_ring_p, step);
     hipLaunchKernelGGL(k_euler_step, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m),
-                       U_in, U_out, gamma, cfl, quirk, r.cur, r.next, r.spare);
+                       U_in, U_out, gamma, cfl, quirk, r);
     return check_launch("k_euler_step");
 }

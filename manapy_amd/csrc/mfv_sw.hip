@@ -20,43 +20,57 @@ constexpr double kDry = 1e-12;
 // Hydrostatic-reconstruction Rusanov flux of one face (same expression order as the
 // oracle's orc_sw_face). flux: from L to R, |n| included. bL/bR: bathymetry term of each
 // side, to be multiplied by that side's outward normal and subtracted.
+template <typename R>
+__device__ __forceinline__ R pminT(R a, R b) { return (b < a) ? b : a; }
+template <typename R>
+__device__ __forceinline__ R pmaxT(R a, R b) { return (b > a) ? b : a; }
+
+template <typename R>
+__device__ __forceinline__ void sw_face_t(R etaL, R huL, R hvL, R zfL, R hcL, R zcL, R etaR, R huR, R hvR, R zfR,
+                                          R hcR, R zcR, R n0, R n1, R mes, R g, R &F0, R &F1, R &F2, R &bL, R &bR)
+{
+    const R zero = R(0), half = R(0.5);
+    R hL = etaL - zfL;
+    if (hL < zero) hL = zero;
+    R hR = etaR - zfR;
+    if (hR < zero) hR = zero;
+    R uL = zero, vL = zero, uR = zero, vR = zero;
+    if (hL > R(kDry)) {
+        uL = huL / hL;
+        vL = hvL / hL;
+    }
+    if (hR > R(kDry)) {
+        uR = huR / hR;
+        vR = hvR / hR;
+    }
+    R zs = pmaxT(zfL, zfR);
+    R hsL = hL + zfL - zs;
+    if (hsL < zero) hsL = zero;
+    R hsR = hR + zfR - zs;
+    if (hsR < zero) hsR = zero;
+    R qL = uL * n0 + vL * n1;
+    R qR = uR * n0 + vR * n1;
+    R pL = half * g * hsL * hsL;
+    R pR = half * g * hsR * hsR;
+    R fL0 = hsL * qL, fL1 = hsL * uL * qL + pL * n0, fL2 = hsL * vL * qL + pL * n1;
+    R fR0 = hsR * qR, fR1 = hsR * uR * qR + pR * n0, fR2 = hsR * vR * qR + pR * n1;
+    R sL = fabs(qL) / mes + sqrt(g * hsL);
+    R sR = fabs(qR) / mes + sqrt(g * hsR);
+    R s = pmaxT(sL, sR) * mes;
+    F0 = half * (fL0 + fR0) - half * s * (hsR - hsL);
+    F1 = half * (fL1 + fR1) - half * s * (hsR * uR - hsL * uL);
+    F2 = half * (fL2 + fR2) - half * s * (hsR * vR - hsL * vL);
+    bL = half * g * (hL * hL - hsL * hsL) + g * (hL + hcL) * half * (zfL - zcL);
+    bR = half * g * (hR * hR - hsR * hsR) + g * (hR + hcR) * half * (zfR - zcR);
+}
+
 __device__ __forceinline__ void sw_face(double etaL, double huL, double hvL, double zfL, double hcL, double zcL,
                                         double etaR, double huR, double hvR, double zfR, double hcR, double zcR,
                                         double n0, double n1, double mes, double g, double &F0, double &F1,
                                         double &F2, double &bL, double &bR)
 {
-    double hL = etaL - zfL;
-    if (hL < 0.) hL = 0.;
-    double hR = etaR - zfR;
-    if (hR < 0.) hR = 0.;
-    double uL = 0., vL = 0., uR = 0., vR = 0.;
-    if (hL > kDry) {
-        uL = huL / hL;
-        vL = hvL / hL;
-    }
-    if (hR > kDry) {
-        uR = huR / hR;
-        vR = hvR / hR;
-    }
-    double zs = pmax(zfL, zfR);
-    double hsL = hL + zfL - zs;
-    if (hsL < 0.) hsL = 0.;
-    double hsR = hR + zfR - zs;
-    if (hsR < 0.) hsR = 0.;
-    double qL = uL * n0 + vL * n1;
-    double qR = uR * n0 + vR * n1;
-    double pL = 0.5 * g * hsL * hsL;
-    double pR = 0.5 * g * hsR * hsR;
-    double fL0 = hsL * qL, fL1 = hsL * uL * qL + pL * n0, fL2 = hsL * vL * qL + pL * n1;
-    double fR0 = hsR * qR, fR1 = hsR * uR * qR + pR * n0, fR2 = hsR * vR * qR + pR * n1;
-    double sL = fabs(qL) / mes + sqrt(g * hsL);
-    double sR = fabs(qR) / mes + sqrt(g * hsR);
-    double s = pmax(sL, sR) * mes;
-    F0 = 0.5 * (fL0 + fR0) - 0.5 * s * (hsR - hsL);
-    F1 = 0.5 * (fL1 + fR1) - 0.5 * s * (hsR * uR - hsL * uL);
-    F2 = 0.5 * (fL2 + fR2) - 0.5 * s * (hsR * vR - hsL * vL);
-    bL = 0.5 * g * (hL * hL - hsL * hsL) + g * (hL + hcL) * 0.5 * (zfL - zcL);
-    bR = 0.5 * g * (hR * hR - hsR * hsR) + g * (hR + hcR) * 0.5 * (zfR - zcR);
+    sw_face_t<double>(etaL, huL, hvL, zfL, hcL, zcL, etaR, huR, hvR, zfR, hcR, zcR, n0, n1, mes, g, F0, F1, F2, bL,
+                      bR);
 }
 
 // time_step_SW (shallowater/fvm_utils.py:16-42): per-cell candidate, faces in faceid order.
@@ -311,18 +325,17 @@ __global__ void __launch_bounds__(kBlock) k_sw_step(Mesh m, const double *__rest
                                                      double *__restrict__ Sout, const double *__restrict__ Z,
                                                      const double *__restrict__ grad,
                                                      const double *__restrict__ zgrad, double g, double cfl,
-                                                     double om, const double *__restrict__ dt_cur, double *dt_next,
-                                                     double *dt_spare)
+                                                     double om, DtRing ring)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    ring_reset_spare(dt_spare);
+    ring_advance(ring);
     double dloc = 1e300;
     if (i < m.C) {
         const size_t C = m.C;
         const double *h = Sin, *hu = Sin + C, *hv = Sin + 2 * C;
         double r[3], s[3];
         sw_cell_residual(m, i, h, hu, hv, Z, grad, zgrad, g, om, r, s);
-        const double dt = dt_cur[0];
+        const double dt = ring.cur[0];
         const double vol = m.cell_volume[i];
         double hn = h[i] + dt * (r[0] + s[0]) / vol;
         double un = hu[i] + dt * ((r[1] + s[1]) / vol + 0. + 0.);
@@ -332,7 +345,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_step(Mesh m, const double *__rest
         Sout[2 * C + i] = vn;
         dloc = sw_cell_dt(m, i, hn, un, vn, cfl, g, 0., 0.);
     }
-    wave_min_to(dt_next, dloc);
+    wave_min_to(ring.next, dloc);
 }
 
 int finish_dt(double *dt_dev, double *dt_host, void *stream)
@@ -413,7 +426,7 @@ extern "C" int mfv_sw_step_fused(const mfv_mesh *m, const double *state_in, doub
                 "NULL or aliased array");
     DtRing r = dt_ring(dt_ring_p, step);
     hipLaunchKernelGGL(k_sw_step, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m),
-                       state_in, state_out, Z, grad, zgrad, grav, cfl, (double)(order - 1), r.cur, r.next, r.spare);
+                       state_in, state_out, Z, grad, zgrad, grav, cfl, (double)(order - 1), r);
     return check_launch("k_sw_step");
 }
 
@@ -615,11 +628,10 @@ __global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, do
 // Fluxes + bathymetry terms + update_SW + next dt, in place on cellrec.
 template <int MF>
 __global__ void __launch_bounds__(kBlock) k_sw_flux_packed(Mesh m, mfv_sw_packed p, double g, double cfl,
-                                                            const double *__restrict__ dt_cur, double *dt_next,
-                                                            double *dt_spare)
+                                                            DtRing ring)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    ring_reset_spare(dt_spare);
+    ring_advance(ring);
     double dloc = 1e300;
     if (i < m.C) {
         const size_t C = m.C;
@@ -665,7 +677,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_flux_packed(Mesh m, mfv_sw_packed
                 s2 -= -(bR * n1);
             }
         }
-        const double dt = dt_cur[0];
+        const double dt = ring.cur[0];
         const double s0 = 0.;
         double hn = h_i + dt * (r0 + s0) / vol;
         double un = hu_i + dt * ((r1 + s1) / vol + 0. + 0.);
@@ -695,7 +707,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_flux_packed(Mesh m, mfv_sw_packed
         }
         dloc = cfl * vol / lam;
     }
-    wave_min_to(dt_next, dloc);
+    wave_min_to(ring.next, dloc);
 }
 
 }  // namespace
@@ -761,6 +773,397 @@ extern "C" int mfv_sw_flux_packed(const mfv_mesh *m, const mfv_sw_packed *p, dou
     DtRing r = dt_ring(dt_ring_p, step);
     auto k = (m->max_cell_faces == 3) ? k_sw_flux_packed<3> : k_sw_flux_packed<4>;
     hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p, grav,
-                       cfl, r.cur, r.next, r.spare);
+                       cfl, r);
     return check_launch("k_sw_flux_packed");
+}
+
+// =============================================================================================
+// Mixed-precision shallow-water path: float32 records, gradients and fluxes; Kahan-compensated
+// float32 residual sums; float64 state update, dt reduction and conservation sums.
+//   state   [C][4]      h, hu, hv, vol              (float64)
+//   cellrec [C][8]      cx, cy, eta, hu, hv, h, Z, 0 (float32)
+//   facerec [C][MF][4]  eta_f, hu_f, hv_f, z_f      (float32)
+//   cfrec   [C][MF][4]  rx, ry, z_f, 0              (float32)
+//   cfgeo   [C][MF][4]  n0, n1, mesure, 0           (float32)
+//   nbr, order: as the packed float64 path.
+namespace {
+
+__device__ __forceinline__ void kahan_add(float &s, float &c, float x)
+{
+    float y = x - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_pack32(Mesh m, mfv_sw_mixed p, const double *__restrict__ h,
+                                                       const double *__restrict__ hu,
+                                                       const double *__restrict__ hv, const double *__restrict__ Z,
+                                                       const double *__restrict__ zgrad, double om)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const size_t C = m.C;
+    double *st = p.state + 4 * (size_t)i;
+    st[0] = h[i];
+    st[1] = hu[i];
+    st[2] = hv[i];
+    st[3] = m.cell_volume[i];
+    float *r = p.cellrec + 8 * (size_t)i;
+    r[0] = (float)m.cell_center[3 * (size_t)i];
+    r[1] = (float)m.cell_center[3 * (size_t)i + 1];
+    r[2] = (float)(h[i] + Z[i]);
+    r[3] = (float)hu[i];
+    r[4] = (float)hv[i];
+    r[5] = (float)h[i];
+    r[6] = (float)Z[i];
+    r[7] = 0.f;
+    const int nf = m.cell_nfaces[i];
+    for (int s = 0; s < MF && s < nf; s++) {
+        float *q = p.cfrec + 4 * (MF * (size_t)i + s);
+        // z_f from the float64 geometry, rounded once
+        const int f = m.cell_faces[(size_t)s * C + i];
+        const double rx = m.face_center[3 * (size_t)f] - m.cell_center[3 * (size_t)i];
+        const double ry = m.face_center[3 * (size_t)f + 1] - m.cell_center[3 * (size_t)i + 1];
+        q[2] = (float)(Z[i] + om * (zgrad[i] * rx + zgrad[C + i] * ry));
+    }
+}
+
+__device__ __forceinline__ void load_ehv32(const float *__restrict__ cellrec, int c, float &cx, float &cy, float &e,
+                                           float &u, float &v)
+{
+    const float4 *r = reinterpret_cast<const float4 *>(cellrec + 8 * (size_t)c);
+    float4 a = r[0];
+    float b = cellrec[8 * (size_t)c + 4];
+    cx = a.x;
+    cy = a.y;
+    e = a.z;
+    u = a.w;
+    v = b;
+}
+
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_recon32(Mesh m, mfv_sw_mixed p, float om)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const size_t C = m.C;
+    float cx, cy, ei, ui, vi;
+    load_ehv32(p.cellrec, i, cx, cy, ei, ui, vi);
+    float ixx = 0.f, iyy = 0.f, ixy = 0.f;
+    float je0 = 0.f, je1 = 0.f, ju0 = 0.f, ju1 = 0.f, jv0 = 0.f, jv1 = 0.f;
+    const int nn = m.lsq_ncells[i];
+    for (int j = 0; j < nn; j++) {
+        int c = m.lsq_cells[(size_t)j * C + i];
+        float px, py, ec, uc, vc;
+        load_ehv32(p.cellrec, c, px, py, ec, uc, vc);
+        float jx = px - cx, jy = py - cy;
+        ixx += jx * jx;
+        iyy += jy * jy;
+        ixy += jx * jy;
+        float de = ec - ei, du = uc - ui, dv = vc - vi;
+        je0 += jx * de;
+        je1 += jy * de;
+        ju0 += jx * du;
+        ju1 += jy * du;
+        jv0 += jx * dv;
+        jv1 += jy * dv;
+    }
+    const int ng = m.lsq_nghosts[i];
+    for (int j = 0; j < ng; j++) {
+        int f = m.lsq_ghosts[(size_t)j * C + i];
+        int o = m.face_cells[2 * f];
+        const double *g = m.face_ghost + 3 * (size_t)f;
+        float px, py, eo, uo, vo;
+        load_ehv32(p.cellrec, o, px, py, eo, uo, vo);
+        float jx = (float)(g[0] - (double)cx), jy = (float)(g[1] - (double)cy);
+        ixx += jx * jx;
+        iyy += jy * jy;
+        ixy += jx * jy;
+        float de = eo - ei, du = -uo - ui, dv = -vo - vi;
+        je0 += jx * de;
+        je1 += jy * de;
+        ju0 += jx * du;
+        ju1 += jy * du;
+        jv0 += jx * dv;
+        jv1 += jy * dv;
+    }
+    const float dia = ixx * iyy - ixy * ixy;
+    const float ex = (iyy * je0 - ixy * je1) / dia, ey = (ixx * je1 - ixy * je0) / dia;
+    const float ux = (iyy * ju0 - ixy * ju1) / dia, uy = (ixx * ju1 - ixy * ju0) / dia;
+    const float vx = (iyy * jv0 - ixy * jv1) / dia, vy = (ixx * jv1 - ixy * jv0) / dia;
+    const int nf = m.cell_nfaces[i];
+    float emax = ei, emin = ei, umax = ui, umin = ui, vmax = vi, vmin = vi;
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        if (s >= nf) break;
+        int code = p.nbr[(size_t)s * C + i];
+        float eb, ub, vb;
+        if (code >= 0) {
+            float px, py;
+            load_ehv32(p.cellrec, code >> 3, px, py, eb, ub, vb);
+        } else {
+            eb = ei;
+            ub = -ui;
+            vb = -vi;
+        }
+        emax = fmaxf(emax, eb);
+        emin = fminf(emin, eb);
+        umax = fmaxf(umax, ub);
+        umin = fminf(umin, ub);
+        vmax = fmaxf(vmax, vb);
+        vmin = fminf(vmin, vb);
+    }
+    float rx[MF], ry[MF], zf[MF];
+    float pe = 1.f, pu = 1.f, pv = 1.f;
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        float4 q = reinterpret_cast<const float4 *>(p.cfrec)[MF * (size_t)i + s];
+        rx[s] = q.x;
+        ry[s] = q.y;
+        zf[s] = q.z;
+        if (s >= nf) continue;
+        float d = ex * rx[s] + ey * ry[s];
+        if (!(fabsf(d) < 1e-8f)) pe = fminf(pe, d > 0.f ? (emax - ei) / d : (emin - ei) / d);
+        d = ux * rx[s] + uy * ry[s];
+        if (!(fabsf(d) < 1e-8f)) pu = fminf(pu, d > 0.f ? (umax - ui) / d : (umin - ui) / d);
+        d = vx * rx[s] + vy * ry[s];
+        if (!(fabsf(d) < 1e-8f)) pv = fminf(pv, d > 0.f ? (vmax - vi) / d : (vmin - vi) / d);
+    }
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        if (s >= nf) break;
+        float4 o;
+        o.x = ei + om * pe * (ex * rx[s] + ey * ry[s]);
+        o.y = ui + om * pu * (ux * rx[s] + uy * ry[s]);
+        o.z = vi + om * pv * (vx * rx[s] + vy * ry[s]);
+        o.w = zf[s];
+        reinterpret_cast<float4 *>(p.facerec)[MF * (size_t)i + s] = o;
+    }
+}
+
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_flux32(Mesh m, mfv_sw_mixed p, float g, double cfl, DtRing ring)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    ring_advance(ring);
+    double dloc = 1e300;
+    if (i < m.C) {
+        const size_t C = m.C;
+        const float *rec = p.cellrec + 8 * (size_t)i;
+        const float h_i = rec[5], Z_i = rec[6];
+        const int nf = m.cell_nfaces[i];
+        const unsigned ord = p.order[i];
+        float r0 = 0.f, c0 = 0.f, r1 = 0.f, c1 = 0.f, r2 = 0.f, c2 = 0.f, s1 = 0.f, d1 = 0.f, s2 = 0.f, d2 = 0.f;
+#pragma unroll
+        for (int r = 0; r < MF; r++) {
+            if (r >= nf) break;
+            const int s = (ord >> (2 * r)) & 3;
+            float4 geo = reinterpret_cast<const float4 *>(p.cfgeo)[MF * (size_t)i + s];
+            float4 own = reinterpret_cast<const float4 *>(p.facerec)[MF * (size_t)i + s];
+            const int code = p.nbr[(size_t)s * C + i];
+            float4 nb;
+            if (code >= 0) {
+                nb = reinterpret_cast<const float4 *>(p.facerec)[MF * (size_t)(code >> 3) + ((code >> 1) & 3)];
+            } else {
+                nb = make_float4(own.x, -own.y, -own.z, own.w);
+            }
+            float F0, F1, F2, bL, bR;
+            if (code < 0 || (code & 1) == 0) {
+                sw_face_t<float>(own.x, own.y, own.z, own.w, h_i, Z_i, nb.x, nb.y, nb.z, nb.w, h_i, Z_i, geo.x, geo.y,
+                                 geo.z, g, F0, F1, F2, bL, bR);
+                kahan_add(r0, c0, -F0);
+                kahan_add(r1, c1, -F1);
+                kahan_add(r2, c2, -F2);
+                kahan_add(s1, d1, -(bL * geo.x));
+                kahan_add(s2, d2, -(bL * geo.y));
+            } else {
+                sw_face_t<float>(nb.x, nb.y, nb.z, nb.w, h_i, Z_i, own.x, own.y, own.z, own.w, h_i, Z_i, geo.x, geo.y,
+                                 geo.z, g, F0, F1, F2, bL, bR);
+                kahan_add(r0, c0, F0);
+                kahan_add(r1, c1, F1);
+                kahan_add(r2, c2, F2);
+                kahan_add(s1, d1, bR * geo.x);
+                kahan_add(s2, d2, bR * geo.y);
+            }
+        }
+        const double dt = ring.cur[0];
+        double *st = p.state + 4 * (size_t)i;
+        const double2 a = reinterpret_cast<const double2 *>(st)[0], b = reinterpret_cast<const double2 *>(st)[1];
+        const double vol = b.y;
+        const double R0 = (double)r0 - (double)c0;
+        const double R1 = ((double)r1 - (double)c1) + ((double)s1 - (double)d1);
+        const double R2 = ((double)r2 - (double)c2) + ((double)s2 - (double)d2);
+        const double hn = a.x + dt * R0 / vol;
+        const double un = a.y + dt * (R1 / vol);
+        const double vn = b.x + dt * (R2 / vol);
+        reinterpret_cast<double2 *>(st)[0] = make_double2(hn, un);
+        st[2] = vn;
+        float *w = p.cellrec + 8 * (size_t)i;
+        reinterpret_cast<float4 *>(w)[0] = make_float4(rec[0], rec[1], (float)(hn + (double)Z_i), (float)un);
+        w[4] = (float)vn;
+        w[5] = (float)hn;
+        // time_step_SW of the float64 state (float32 geometry), faces in faceid order
+        const double velson = sqrt((double)g * hn);
+        double lam = 0.;
+#pragma unroll
+        for (int s = 0; s < MF; s++) {
+            if (s >= nf) break;
+            float4 geo = reinterpret_cast<const float4 *>(p.cfgeo)[MF * (size_t)i + s];
+            double n0 = geo.x, n1 = geo.y, mesu = geo.z;
+            double u_n = fabs(un / hn * n0 + vn / hn * n1);
+            lam += (u_n / mesu + velson) * mesu;
+        }
+        dloc = cfl * vol / lam;
+    }
+    wave_min_to(ring.next, dloc);
+}
+
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_unpack32(int C, const double *__restrict__ state,
+                                                         double *__restrict__ h, double *__restrict__ hu,
+                                                         double *__restrict__ hv)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    h[i] = state[4 * (size_t)i];
+    hu[i] = state[4 * (size_t)i + 1];
+    hv[i] = state[4 * (size_t)i + 2];
+}
+
+// Deterministic float64 sums of (h, hu, hv) * vol: fixed-shape block tree, then a single-block
+// pass over the block partials in index order.
+constexpr int kSumBlock = 256;
+
+__global__ void __launch_bounds__(kSumBlock) k_sw_sums_partial(int C, const double *__restrict__ state,
+                                                               double *__restrict__ partial)
+{
+    __shared__ double sh[3][kSumBlock];
+    int i = blockIdx.x * kSumBlock + threadIdx.x;
+    double a = 0., b = 0., c = 0.;
+    if (i < C) {
+        const double *s = state + 4 * (size_t)i;
+        a = s[0] * s[3];
+        b = s[1] * s[3];
+        c = s[2] * s[3];
+    }
+    sh[0][threadIdx.x] = a;
+    sh[1][threadIdx.x] = b;
+    sh[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int w = kSumBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) partial[3 * (size_t)blockIdx.x + k] = sh[k][0];
+}
+
+__global__ void __launch_bounds__(kSumBlock) k_sw_sums_final(int nb, const double *__restrict__ partial,
+                                                             double *__restrict__ out)
+{
+    __shared__ double sh[3][kSumBlock];
+    double a = 0., b = 0., c = 0.;
+    for (int j = threadIdx.x; j < nb; j += kSumBlock) {
+        a += partial[3 * (size_t)j];
+        b += partial[3 * (size_t)j + 1];
+        c += partial[3 * (size_t)j + 2];
+    }
+    sh[0][threadIdx.x] = a;
+    sh[1][threadIdx.x] = b;
+    sh[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int w = kSumBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) out[k] = sh[k][0];
+}
+
+}  // namespace
+
+static int check_mixed(const mfv_mesh *m, const mfv_sw_mixed *p)
+{
+    MFV_REQUIRE(p != nullptr, "mixed descriptor is NULL");
+    MFV_REQUIRE(p->nb_cells == m->nb_cells && p->max_cell_faces == m->max_cell_faces,
+                "mixed descriptor does not match the mesh");
+    MFV_REQUIRE(m->max_cell_faces == 3 || m->max_cell_faces == 4, "mixed path supports 3 or 4 faces per cell");
+    MFV_REQUIRE(p->state && p->cellrec && p->facerec && p->cfrec && p->cfgeo && p->nbr && p->order,
+                "mixed descriptor has NULL arrays");
+    return MFV_OK;
+}
+
+extern "C" int mfv_sw_pack_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, const double *h, const double *hu,
+                                 const double *hv, const double *Z, const double *zgrad, int order, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_mixed(m, p)) return rc;
+    MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    MFV_REQUIRE(h && hu && hv && Z && zgrad, "NULL array");
+    if (m->nb_cells == 0) return MFV_OK;
+    auto k = (m->max_cell_faces == 3) ? k_sw_pack32<3> : k_sw_pack32<4>;
+    hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p, h, hu,
+                       hv, Z, zgrad, (double)(order - 1));
+    return check_launch("k_sw_pack32");
+}
+
+extern "C" int mfv_sw_unpack_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, double *h, double *hu, double *hv,
+                                   void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_mixed(m, p)) return rc;
+    MFV_REQUIRE(h && hu && hv, "NULL array");
+    if (m->nb_cells == 0) return MFV_OK;
+    hipLaunchKernelGGL(k_sw_unpack32<3>, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                       m->nb_cells, p->state, h, hu, hv);
+    return check_launch("k_sw_unpack32");
+}
+
+extern "C" int mfv_sw_recon_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, int order, void *stream)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_mixed(m, p)) return rc;
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    if (m->nb_cells == 0) return MFV_OK;
+    auto k = (m->max_cell_faces == 3) ? k_sw_recon32<3> : k_sw_recon32<4>;
+    hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p,
+                       (float)(order - 1));
+    return check_launch("k_sw_recon32");
+}
+
+extern "C" int mfv_sw_flux_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, double grav, double cfl,
+                                 double *dt_ring_p, int step, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_mixed(m, p)) return rc;
+    MFV_REQUIRE(dt_ring_p, "dt ring is NULL");
+    if (m->nb_cells == 0) return MFV_OK;
+    DtRing r = dt_ring(dt_ring_p, step);
+    auto k = (m->max_cell_faces == 3) ? k_sw_flux32<3> : k_sw_flux32<4>;
+    hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p,
+                       (float)grav, cfl, r);
+    return check_launch("k_sw_flux32");
+}
+
+extern "C" int mfv_sw_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, double *partial, int n_partial,
+                                   double *out3, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_mixed(m, p)) return rc;
+    const int nb = (m->nb_cells + kSumBlock - 1) / kSumBlock;
+    MFV_REQUIRE(partial && out3, "NULL array");
+    MFV_REQUIRE(n_partial >= 3 * nb, "partial needs 3*ceil(C/256) = %d doubles, got %d", 3 * nb, n_partial);
+    if (nb == 0) return MFV_OK;
+    hipLaunchKernelGGL(k_sw_sums_partial, dim3(nb), dim3(kSumBlock), 0, as_stream(stream), m->nb_cells, p->state,
+                       partial);
+    if (int rc = check_launch("k_sw_sums_partial")) return rc;
+    hipLaunchKernelGGL(k_sw_sums_final, dim3(1), dim3(kSumBlock), 0, as_stream(stream), nb, partial, out3);
+    return check_launch("k_sw_sums_final");
 }

@@ -993,6 +993,15 @@ class Domain(object):
     maxcellfid = property(lambda self: int(self._maxcellfid))
 
     @property
+    def device_mesh(self):
+        """The GPU copy of the hot-path arrays (built on first use). 2D meshes carry the
+        least-squares stencil; 3D meshes upload it on the first gradient call."""
+        if getattr(self, '_device_mesh', None) is None:
+            from .device import DeviceMesh
+            self._device_mesh = DeviceMesh(self, lsq=(int(self._dim) == 2))
+        return self._device_mesh
+
+    @property
     def backend(self):
         if self._backend is None:
             from .backend import HIPBackend

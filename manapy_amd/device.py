@@ -138,6 +138,26 @@ class DeviceMesh(object):
     def ptr(self):
         return ctypes.byref(self.desc)
 
+    def ensure_lsq(self):
+        """Upload the least-squares stencil (cellnid / ghostnid) if it is not resident yet."""
+        if self.lsq:
+            return
+        import torch
+        host = gpu_layout(self.domain, lsq=True)
+        for k in ('lsq_cells', 'lsq_ncells', 'lsq_ghosts', 'lsq_nghosts'):
+            self.t[k] = torch.from_numpy(host[k]).to(self.device)
+            setattr(self.desc, k, self.t[k].data_ptr())
+        self.desc.lsq_width = host['K']
+        self.desc.ghost_width = host['G']
+        self.lsq = True
+
+    @property
+    def dist_ortho(self):
+        import torch
+        if 'dist_ortho' not in self.t:
+            self.t['dist_ortho'] = torch.as_tensor(self.domain.faces.dist_ortho, device=self.device)
+        return self.t['dist_ortho']
+
     def zeros(self, *shape, dtype=None):
         import torch
         return torch.zeros(*shape, dtype=dtype or torch.float64, device=self.device)
@@ -218,6 +238,37 @@ class SwPacked(object):
         d.cfgeo = self.cfgeo.data_ptr()
         d.nbr = self.nbr.data_ptr()
         d.order = self.order.data_ptr()
+        self.desc = d
+
+    @property
+    def ptr(self):
+        return ctypes.byref(self.desc)
+
+
+class SwMixed(object):
+    """Device tables of the mixed-precision shallow-water path (mfv_sw_mixed)."""
+
+    def __init__(self, dmesh):
+        import torch
+        host = sw_packed_layout(dmesh.domain)
+        C = dmesh.C
+        MF = host['MF']
+        dev = dmesh.device
+        f32 = torch.float32
+        self.state = torch.zeros((C, 4), dtype=torch.float64, device=dev)
+        self.cellrec = torch.zeros((C, 8), dtype=f32, device=dev)
+        self.facerec = torch.zeros((C, MF, 4), dtype=f32, device=dev)
+        self.cfrec = torch.as_tensor(host['cfrec'].astype(np.float32), device=dev)
+        self.cfgeo = torch.as_tensor(host['cfgeo'].astype(np.float32), device=dev)
+        self.nbr = torch.as_tensor(host['nbr'], device=dev)
+        self.order = torch.as_tensor(host['order'], device=dev)
+        self.partial = torch.zeros(3 * ((C + 255) // 256) + 3, dtype=torch.float64, device=dev)
+        self.sums = torch.zeros(3, dtype=torch.float64, device=dev)
+        d = _lib.MfvSwMixed()
+        d.nb_cells = C
+        d.max_cell_faces = MF
+        for k in ('state', 'cellrec', 'facerec', 'cfrec', 'cfgeo', 'nbr', 'order'):
+            setattr(d, k, getattr(self, k).data_ptr())
         self.desc = d
 
     @property
