@@ -1,0 +1,125 @@
+"""Pins for the diamond-scheme oracle (oracle/diamond.py): the properties that define the
+scheme, checked independently of its formulas."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from conftest import make_tri_domain
+from oracle import diamond as dm
+
+DIR_ALL = {"in": "dirichlet", "out": "dirichlet", "upper": "dirichlet", "bottom": "dirichlet"}
+NEU_ALL = {"in": "neumann", "out": "neumann", "upper": "neumann", "bottom": "neumann"}
+
+
+@pytest.fixture(scope="module")
+def dd():
+    return make_tri_domain(18, 14, 1.0, 1.0, 0.1, 3)
+
+
+def _dirichlet_lists(d, bcs):
+    idx = {"in": 1, "out": 2, "upper": 3, "bottom": 4}
+    faces = {"in": d.infaces, "out": d.outfaces, "upper": d.upperfaces, "bottom": d.bottomfaces}
+    BCdir = [idx[k] for k, v in bcs.items() if v == "dirichlet"]
+    dirfaces = np.sort(np.concatenate([faces[k] for k, v in bcs.items() if v == "dirichlet"] or
+                                      [np.zeros(0, np.int32)])).astype(np.int32)
+    return BCdir, dirfaces
+
+
+def _matrix(d, bcs):
+    BCdir, dirfaces = _dirichlet_lists(d, bcs)
+    mif = dm.matrixinnerfaces(d)
+    data, row, col = dm.get_triplet_2d(d, BCdir, mif, dirfaces)
+    assert len(data) == dm.compute_2dmatrix_size(d, BCdir, mif, dirfaces)
+    C = int(d.nbcells)
+    return sp.csc_matrix((data, (row, col)), shape=(C, C)), BCdir, dirfaces, mif
+
+
+def test_diamond_affine_is_exact(dd):
+    """The diamond scheme reproduces affine solutions of the Laplace problem whenever its
+    boundary treatment is exact: u = a + b x with Dirichlet data on the x = const sides (the
+    reference's mirror ghost uses the face-centre value, functions2d.py:736-737, exact when u
+    has no tangential gradient there) and Neumann on the y = const sides (ghost = owner value,
+    exact for u independent of y). Every row of A u - rhs0 vanishes and SuperLU (the
+    reference ScipySolver) returns u."""
+    d = dd
+    bcs = {"in": "dirichlet", "out": "dirichlet", "upper": "neumann", "bottom": "neumann"}
+    f = lambda x, y, z: 0.3 + 1.7 * x
+    A, BCdir, dirfaces, mif = _matrix(d, bcs)
+    Pn, Pf = dm.update_ghost_values(d, bcs, {"in": f, "out": f})
+    rhs0 = dm.get_rhs_glob_2d(d, Pn, Pf, BCdir, mif, dirfaces)
+    u = f(d.cells.center[:, 0], d.cells.center[:, 1], 0.)
+    assert np.abs(A @ u - rhs0).max() < 1e-9 * np.abs(rhs0).max()
+    x = sla.spsolve(A, rhs0)
+    assert np.abs(x - u).max() < 1e-11
+
+
+def test_diamond_dirichlet_tangential_gradient_is_not_exact(dd):
+    """...and with a tangential gradient on Dirichlet faces the boundary rows are not exact
+    (interior rows still are): the reference's property, kept."""
+    d = dd
+    f = lambda x, y, z: 0.3 + 1.7 * x - 0.9 * y
+    A, BCdir, dirfaces, mif = _matrix(d, DIR_ALL)
+    Pn, Pf = dm.update_ghost_values(d, DIR_ALL, {k: f for k in DIR_ALL})
+    rhs0 = dm.get_rhs_glob_2d(d, Pn, Pf, BCdir, mif, dirfaces)
+    u = f(d.cells.center[:, 0], d.cells.center[:, 1], 0.)
+    r = np.abs(A @ u - rhs0)
+    bcells = np.unique(d.faces.cellid[d.boundaryfaces, 0])
+    far = np.setdiff1d(np.arange(int(d.nbcells)), bcells)
+    assert r[bcells].max() > 1.0
+    near_nodes = np.unique(d.cells.nodeid[bcells, :3])
+    touch = np.isin(d.cells.nodeid[:, :3], near_nodes).any(axis=1)
+    interior = np.nonzero(~touch)[0]
+    assert len(interior) > 100 and r[interior].max() < 1e-9 * np.abs(rhs0).max()
+
+
+def test_diamond_neumann_rows_sum_to_zero(dd):
+    A, _, _, _ = _matrix(dd, NEU_ALL)
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    assert np.abs(rs).max() < 1e-9 * abs(A).max()
+    # and the operator is not trivially zero, with negative diagonal (a discrete Laplacian)
+    assert (A.diagonal() < 0).all()
+
+
+def test_diamond_mixed_bc_converges_to_quadratic(dd):
+    """Poisson with Dirichlet data of u = x^2 + y^2 and source 4 converges (first order in the
+    max norm: the boundary treatment above is first order)."""
+    errs = []
+    for n in (10, 20):
+        d = make_tri_domain(n, n, 1.0, 1.0, 0.1, 1)
+        f = lambda x, y, z: x * x + y * y
+        A, BCdir, dirfaces, mif = _matrix(d, DIR_ALL)
+        Pn, Pf = dm.update_ghost_values(d, DIR_ALL, {k: f for k in DIR_ALL})
+        rhs0 = dm.get_rhs_glob_2d(d, Pn, Pf, BCdir, mif, dirfaces)
+        x = sla.spsolve(A, rhs0 + 4.0)
+        u = f(d.cells.center[:, 0], d.cells.center[:, 1], 0.)
+        errs.append(np.abs(x - u).max())
+    assert errs[1] < errs[0] / 1.6
+
+
+def test_node_interpolation_and_face_gradient_affine(dd):
+    d = dd
+    g = lambda x, y: 0.5 - 2.0 * x + 3.0 * y
+    wc = g(d.cells.center[:, 0], d.cells.center[:, 1])
+    wg = np.zeros(int(d.nbfaces))
+    bf = d.boundaryfaces
+    wg[bf] = g(d.faces.ghostcenter[bf, 0], d.faces.ghostcenter[bf, 1])
+    wn = dm.centertovertex_2d(d, wc, wg)
+    assert np.abs(wn - g(d.nodes.vertex[:, 0], d.nodes.vertex[:, 1])).max() < 1e-12
+    wx, wy = dm.face_gradient_2d(d, wc, wg, wn, bf)
+    assert np.abs(wx + 2.0).max() < 1e-10
+    assert np.abs(wy - 3.0).max() < 1e-10
+    dis = dm.explicitscheme_dissipative(d, wx, wy, 0.7, 0.7)
+    # affine field: the discrete divergence of a constant flux vanishes cell by cell
+    # (boundary faces carry the same constant gradient)
+    assert np.abs(dis).max() < 1e-11
+
+
+def test_diffusion_time_step_closed_form(dd):
+    d = dd
+    dt = dm.time_step_diffusion(d, 0.1, 0.1, 0.8)
+    lam = np.zeros(int(d.nbcells))
+    for j in range(3):
+        m = d.faces.mesure[d.cells.faceid[:, j]]
+        lam += 0.2 * m * m / d.cells.volume
+    assert np.isclose(dt, (0.8 * d.cells.volume / lam).min(), rtol=1e-13)
