@@ -216,6 +216,36 @@ int mfv_sw_flux_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, double grav, dou
 int mfv_sw_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, double *partial, int n_partial,
                         double *out3, void *stream);
 
+/* Ensemble of independent shallow-water instances concatenated into one mesh (mfv_mesh +
+ * mfv_sw_mixed over all cells). Instance i owns cells inst_cell0[i] .. + inst_ncells[i] - 1 with
+ * inst_cell0[i] a multiple of 64; cells up to the next multiple of 64 are padding (no faces,
+ * volume 1, zero state). Each instance has its own dt ring rings[i][0..3] (as mfv_dt_ring_init). */
+typedef struct mfv_sw_batch {
+    int32_t nb_inst;
+    int32_t nb_waves;             /* nb_cells / 64 */
+    const int32_t *wave_inst;     /* [nb_waves] instance of each 64-cell group */
+    const int32_t *inst_cell0;    /* [I] */
+    const int32_t *inst_ncells;   /* [I] */
+    double *rings;                /* [I][4] */
+} mfv_sw_batch;
+
+/* rings[i] = {dt_i, 1e6, 1e6, 0} with dt_i the time_step_SW of instance i's packed state. */
+int mfv_sw_batch_dt_init(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b, double grav,
+                         double cfl, void *stream);
+/* One step of every instance (reconstruction + fluxes + update + next dt, per-instance rings);
+ * block = threads per block of the two cell kernels (64, 128, 256 or 512). */
+int mfv_sw_batch_step(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b, double grav,
+                      double cfl, int order, int step, int block, void *stream);
+/* Deterministic float64 totals per instance: out[i][0..2] = sum (h, hu, hv) |K|. */
+int mfv_sw_batch_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b,
+                              double *out, void *stream);
+/* Captures nsteps (a multiple of 3) ensemble steps into an executable graph (kernel nodes only);
+ * each mfv_graph_launch advances every instance by nsteps steps. */
+int mfv_sw_batch_graph(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b, double grav,
+                       double cfl, int order, int nsteps, int block, void **graph_out);
+int mfv_graph_launch(void *graph, void *stream);
+int mfv_graph_destroy(void *graph);
+
 /* Sets dt[0] = 1e6, the initial value of the reference's dt min-loops. */
 int mfv_dt_reset(double *dt, void *stream);
 
@@ -243,6 +273,153 @@ int mfv_euler3d_update(const mfv_mesh *m, double *U, const double *rez, double g
  * U_in -> U_out ([6][C]); dt ring as for mfv_advection_step. */
 int mfv_euler3d_step_fused(const mfv_mesh *m, const double *U_in, double *U_out, double gamma,
                            double cfl, int quirk, double *dt_ring, int step, void *stream);
+
+/* ---------------- diamond scheme, 2D (ast/functions2d.py, solvers/diffusion) --------- */
+
+/* Diamond-scheme geometry. The pseudo-Laplacian weight of a node incidence is
+ *   alpha = (1 + lambda_x (x - x_n) + lambda_y (y - y_n)) / (number + lambda_x R_x + lambda_y R_y)
+ * (ast/functions2d.py:221, :362), evaluated once per incidence with that expression order. */
+typedef struct mfv_diamond {
+    int32_t nb_cells, nb_faces, nb_nodes, max_cell_faces;
+    const int32_t *cell_faces;      /* [MF][C] as mfv_mesh */
+    const int32_t *cell_nfaces;     /* [C] */
+    const uint8_t *order_sorted;    /* [MF][C] as mfv_mesh (faces of a cell by ascending id) */
+    const double *cell_volume;      /* [C] */
+    const int32_t *face_cells;      /* [F][2] */
+    const int32_t *face_nodes;      /* [F][2] faces.nodeid[:, :2] */
+    const double *face_param;       /* [F][4] faces.param1..param4 */
+    const double *face_fk;          /* [F][8] faces.f_1..f_4 (x, y) */
+    const double *face_air;         /* [F]    faces.airDiamond */
+    const double *face_normal;      /* [F][3] */
+    const int32_t *node_cell_ptr;   /* [N+1]  CSR of nodes.cellid */
+    const int32_t *node_cell;       /*        incident cells, reference order */
+    const double *node_cell_alpha;  /*        alpha of each incident cell centre */
+    const int32_t *node_ghost_ptr;  /* [N+1]  CSR of the valid nodes.ghostcenter entries */
+    const int32_t *node_ghost_face; /*        boundary face of the ghost centre */
+    const int32_t *node_ghost_owner;/*        its owner cell (ghostcenter[..][2]) */
+    const double *node_ghost_alpha; /*        alpha of the ghost centre */
+} mfv_diamond;
+
+/* Boundary data of a variable for the diamond scheme:
+ *   node_dir[N]      1 if nodes.oldname is in BCdirichlet (search_element(BCdirichlet, ..) == 1)
+ *   face_dir_rank[F] position of the face in the variable's dirichletfaces list, -1 if absent */
+
+/* get_triplet_2d + compute_2dmatrix_size (ast/functions2d.py:328-664), first pass: for each
+ * 64-row slice the largest number of distinct columns in its rows -> slice_w[ceil(C/64)];
+ * info[0] |= 1 if a row has more than 32 distinct columns, info[1] = max(info[1], width). */
+int mfv_diamond_widths(const mfv_diamond *d, const uint8_t *node_dir, const int32_t *face_dir_rank,
+                       int32_t *slice_w, int32_t *info, void *stream);
+
+/* get_triplet_2d, second pass: the matrix rows with the reference's duplicate triplets summed
+ * in insertion order, written as ELL slices (entry j of row K at ((K/64)*width + j)*64 + K%64,
+ * columns ascending and offset by row0, unused slots (row0+K, 0)), vals32 = (float)vals,
+ * diag[K] = A[K][K]. Rows C..64*ceil(C/64)-1 are padding (no entries, diag 0). */
+int mfv_diamond_assemble(const mfv_diamond *d, const uint8_t *node_dir, const int32_t *face_dir_rank,
+                         int row0, int width, const int32_t *slice_w, int32_t *cols, double *vals,
+                         float *vals32, double *diag, void *stream);
+
+/* get_rhs_glob_2d (ast/functions2d.py:740-817): Dirichlet lifting rhs[C]. */
+int mfv_diamond_rhs(const mfv_diamond *d, const uint8_t *node_dir, const int32_t *face_dir_rank,
+                    const double *Pbordnode, const double *Pbordface, double *rhs, void *stream);
+
+/* centertovertex_2d (ast/functions2d.py:199-277): w_node[N]. */
+int mfv_centertovertex(const mfv_diamond *d, const double *w_c, const double *w_ghost, double *w_node,
+                       void *stream);
+
+/* kind 0: face_gradient_2d (ast/functions2d.py:108-196); kind 1: compute_P_gradient_2d_diamond
+ * (:820-936). face_bc[F] on boundary faces: 0 not in the face lists (left untouched), 1 in the
+ * Neumann list, 2 in the Dirichlet list. node_dir/Pbord* are read by kind 1 only. */
+int mfv_face_gradient(const mfv_diamond *d, int kind, const double *w_c, const double *w_ghost,
+                      const double *w_node, const int8_t *face_bc, const uint8_t *node_dir,
+                      const double *Pbordnode, const double *Pbordface, double *wx, double *wy,
+                      void *stream);
+
+/* explicitscheme_dissipative (solvers/diffusion/fvm_utils.py:5-26); wz may be NULL (zeros).
+ * Needs mesh->order_sorted. */
+int mfv_explicitscheme_dissipative(const mfv_mesh *m, const double *wx, const double *wy,
+                                   const double *wz, double Dxx, double Dyy, double Dzz,
+                                   double *dissip, void *stream);
+
+/* time_step of solvers/diffusion/fvm_utils.py:30-49 (u = v = w = NULL) or of
+ * solvers/advecdiff/fvm_utils.py:317-340 (cell velocities). Output as mfv_time_step. */
+int mfv_time_step_diffusion(const mfv_mesh *m, const double *u, const double *v, const double *w,
+                            double cfl, double Dxx, double Dyy, double Dzz, double *dt_dev,
+                            double *dt_host, void *stream);
+
+/* ---------------- batched sparse solves (replaces solvers/ls/system.py:624-744) ------- */
+
+/* Many independent systems (sigma_i I + tau_i A_i) x_i = b_i in one block-diagonal ELL matrix
+ * of 64-row slices. Instance i owns the contiguous slices inst_slice0[i] ..
+ * inst_slice0[i] + inst_nslices[i] - 1; its columns are global row indices inside its block.
+ * live_slices / live_inst list the slices and instances the kernels visit (their counts are
+ * device scalars so that a captured graph stays valid as instances join and leave). */
+typedef struct mfv_ell_batch {
+    int32_t nb_slices;        /* slice capacity S (rows = 64 S) */
+    int32_t width;            /* ELL slots per row */
+    int32_t nb_inst;          /* instance capacity I */
+    int32_t pad_;
+    const int32_t *slice_inst;    /* [S] owning instance, -1 if free */
+    const int32_t *slice_w;       /* [S] slots used in the slice */
+    const int32_t *cols;          /* [S][width][64] */
+    const double *vals;           /* [S][width][64] */
+    const float *vals32;          /* [S][width][64] */
+    const double *diag;           /* [64 S] */
+    const int32_t *inst_slice0;   /* [I] */
+    const int32_t *inst_nslices;  /* [I] */
+    const int32_t *live_slices;   /* [S] */
+    const int32_t *n_live_slices; /* device scalar */
+    const int32_t *live_inst;     /* [I] */
+    const int32_t *n_live_inst;   /* device scalar */
+} mfv_ell_batch;
+
+/* Per-instance solver state: status[i] and scal[i][MFV_KS_NSCAL]. The caller sets SIGMA, TAU,
+ * TOL_OUT, TOL_IN, MAXIT_IN, MAXIT_OUT, fills x (initial guess) and b (zero on padding rows)
+ * and sets status START (which restarts the refinement count OUTER; ITTOT keeps adding up).
+ * Ticks then move it through INNER/CORRECT to DONE (||b - op(x)|| <= TOL_OUT ||b||) or FAILED
+ * (more than MAXIT_OUT refinements). */
+#define MFV_KS_IDLE 0
+#define MFV_KS_START 1
+#define MFV_KS_INNER 2
+#define MFV_KS_CORRECT 3
+#define MFV_KS_DONE 4
+#define MFV_KS_FAILED 5
+
+#define MFV_KS_SIGMA 0
+#define MFV_KS_TAU 1
+#define MFV_KS_TOL_OUT 2
+#define MFV_KS_TOL_IN 3
+#define MFV_KS_MAXIT_IN 4
+#define MFV_KS_MAXIT_OUT 5
+#define MFV_KS_RHO 6
+#define MFV_KS_ALPHA 7
+#define MFV_KS_OMEGA 8
+#define MFV_KS_BETA 9
+#define MFV_KS_R0N2 10
+#define MFV_KS_RN2 11
+#define MFV_KS_BN2 12
+#define MFV_KS_RESN2 13
+#define MFV_KS_IT 14
+#define MFV_KS_OUTER 15
+#define MFV_KS_ITTOT 16
+#define MFV_KS_NSCAL 24
+
+typedef struct mfv_krylov_work {
+    double *x, *b;                                  /* [64 S] FP64 solution and right-hand side */
+    float *d, *r, *rh, *p, *v, *yz, *t, *dinv;      /* [64 S] FP32 inner-solve vectors */
+    double *partial;                                /* [S][4] per-slice partial sums */
+    double *scal;                                   /* [I][MFV_KS_NSCAL] */
+    int32_t *status;                                /* [I] */
+} mfv_krylov_work;
+
+/* One tick: refinement step (x += d, r = b - op(x) in FP64, convergence test) for instances in
+ * START/CORRECT, then n_iter BiCGStab iterations (FP32, Jacobi) for instances in INNER.
+ * grid_slices / grid_inst are the persistent grid sizes (blocks of 256) of the per-slice and
+ * per-instance kernels. Asynchronous; poll status[] between ticks. */
+int mfv_krylov_tick(const mfv_ell_batch *e, const mfv_krylov_work *w, int n_iter, int grid_slices,
+                    int grid_inst, void *stream);
+
+/* y = A x (FP64) on the live slices. */
+int mfv_ell_spmv(const mfv_ell_batch *e, const double *x, double *y, int grid_slices, void *stream);
 
 #ifdef __cplusplus
 }

@@ -17,7 +17,9 @@ def __getattr__(name):
     if name in ('Variable', 'Boundary'):
         from . import variable
         return getattr(variable, name)
-    if name in ('AdvectionSolver', 'ShallowWaterSolver', 'EulerSolver3D', 'initialisation_euler_3d'):
+    if name in ('AdvectionSolver', 'ShallowWaterSolver', 'EulerSolver3D', 'initialisation_euler_3d',
+                'DiffusionSolver', 'AdvectionDiffusionSolver', 'LinearSolver', 'HIPKrylovSolver', 'ScipySolver',
+                'MUMPSSolver', 'PETScKrylovSolver'):
         from . import solvers
         return getattr(solvers, name)
     if name == 'HIPBackend':
@@ -27,5 +29,6 @@ def __getattr__(name):
 
 
 __all__ = ['Struct', 'make_get_conf', 'Domain', 'Variable', 'Boundary', 'AdvectionSolver', 'ShallowWaterSolver',
-           'EulerSolver3D', 'initialisation_euler_3d', 'HIPBackend', 'jittered_triangles', 'kuhn_tetrahedra',
-           'gaussian_bumps']
+           'EulerSolver3D', 'initialisation_euler_3d', 'DiffusionSolver', 'AdvectionDiffusionSolver', 'LinearSolver',
+           'HIPKrylovSolver', 'ScipySolver', 'MUMPSSolver', 'PETScKrylovSolver', 'HIPBackend', 'jittered_triangles',
+           'kuhn_tetrahedra', 'gaussian_bumps']

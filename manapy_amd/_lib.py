@@ -45,7 +45,47 @@ class MfvSwMixed(ctypes.Structure):
                 ('facerec', P), ('cfrec', P), ('cfgeo', P), ('nbr', P), ('order', P)]
 
 
+class MfvDiamond(ctypes.Structure):
+    """Mirror of ``struct mfv_diamond`` (include/mfv.h)."""
+    _fields_ = [('nb_cells', ctypes.c_int32), ('nb_faces', ctypes.c_int32), ('nb_nodes', ctypes.c_int32),
+                ('max_cell_faces', ctypes.c_int32), ('cell_faces', P), ('cell_nfaces', P), ('order_sorted', P),
+                ('cell_volume', P), ('face_cells', P), ('face_nodes', P), ('face_param', P), ('face_fk', P),
+                ('face_air', P), ('face_normal', P), ('node_cell_ptr', P), ('node_cell', P),
+                ('node_cell_alpha', P), ('node_ghost_ptr', P), ('node_ghost_face', P), ('node_ghost_owner', P),
+                ('node_ghost_alpha', P)]
+
+
+class MfvEllBatch(ctypes.Structure):
+    """Mirror of ``struct mfv_ell_batch`` (include/mfv.h)."""
+    _fields_ = [('nb_slices', ctypes.c_int32), ('width', ctypes.c_int32), ('nb_inst', ctypes.c_int32),
+                ('pad_', ctypes.c_int32), ('slice_inst', P), ('slice_w', P), ('cols', P), ('vals', P),
+                ('vals32', P), ('diag', P), ('inst_slice0', P), ('inst_nslices', P), ('live_slices', P),
+                ('n_live_slices', P), ('live_inst', P), ('n_live_inst', P)]
+
+
+class MfvKrylovWork(ctypes.Structure):
+    """Mirror of ``struct mfv_krylov_work`` (include/mfv.h)."""
+    _fields_ = [('x', P), ('b', P), ('d', P), ('r', P), ('rh', P), ('p', P), ('v', P), ('yz', P), ('t', P),
+                ('dinv', P), ('partial', P), ('scal', P), ('status', P)]
+
+
+class MfvSwBatch(ctypes.Structure):
+    """Mirror of ``struct mfv_sw_batch`` (include/mfv.h)."""
+    _fields_ = [('nb_inst', ctypes.c_int32), ('nb_waves', ctypes.c_int32), ('wave_inst', P), ('inst_cell0', P),
+                ('inst_ncells', P), ('rings', P)]
+
+
+# per-instance solver states and scalar slots (MFV_KS_* in include/mfv.h)
+KS_IDLE, KS_START, KS_INNER, KS_CORRECT, KS_DONE, KS_FAILED = 0, 1, 2, 3, 4, 5
+KS_SIGMA, KS_TAU, KS_TOL_OUT, KS_TOL_IN, KS_MAXIT_IN, KS_MAXIT_OUT = 0, 1, 2, 3, 4, 5
+KS_RHO, KS_ALPHA, KS_OMEGA, KS_BETA, KS_R0N2, KS_RN2, KS_BN2, KS_RESN2 = 6, 7, 8, 9, 10, 11, 12, 13
+KS_IT, KS_OUTER, KS_ITTOT, KS_NSCAL = 14, 15, 16, 24
+
 MESH_P = ctypes.POINTER(MfvMesh)
+DIA_P = ctypes.POINTER(MfvDiamond)
+ELL_P = ctypes.POINTER(MfvEllBatch)
+KW_P = ctypes.POINTER(MfvKrylovWork)
+SWB_P = ctypes.POINTER(MfvSwBatch)
 SWP_P = ctypes.POINTER(MfvSwPacked)
 SWM_P = ctypes.POINTER(MfvSwMixed)
 DP = ctypes.POINTER(ctypes.c_double)
@@ -84,6 +124,21 @@ _SIGNATURES = {
     'mfv_euler3d_rusanov': [MESH_P, P, P, c_double, c_int, P, P],
     'mfv_euler3d_update': [MESH_P, P, P, c_double, P, P],
     'mfv_euler3d_step_fused': [MESH_P, P, P, c_double, c_double, c_int, P, c_int, P],
+    'mfv_diamond_widths': [DIA_P, P, P, P, P, P],
+    'mfv_diamond_assemble': [DIA_P, P, P, c_int, c_int, P, P, P, P, P, P],
+    'mfv_diamond_rhs': [DIA_P, P, P, P, P, P, P],
+    'mfv_centertovertex': [DIA_P, P, P, P, P],
+    'mfv_face_gradient': [DIA_P, c_int, P, P, P, P, P, P, P, P, P, P],
+    'mfv_explicitscheme_dissipative': [MESH_P, P, P, P, c_double, c_double, c_double, P, P],
+    'mfv_time_step_diffusion': [MESH_P, P, P, P, c_double, c_double, c_double, c_double, P, DP, P],
+    'mfv_sw_batch_dt_init': [MESH_P, SWM_P, SWB_P, c_double, c_double, P],
+    'mfv_sw_batch_step': [MESH_P, SWM_P, SWB_P, c_double, c_double, c_int, c_int, c_int, P],
+    'mfv_sw_batch_conservation': [MESH_P, SWM_P, SWB_P, P, P],
+    'mfv_sw_batch_graph': [MESH_P, SWM_P, SWB_P, c_double, c_double, c_int, c_int, c_int, ctypes.POINTER(c_void_p)],
+    'mfv_graph_launch': [c_void_p, P],
+    'mfv_graph_destroy': [c_void_p],
+    'mfv_krylov_tick': [ELL_P, KW_P, c_int, c_int, c_int, P],
+    'mfv_ell_spmv': [ELL_P, P, P, c_int, P],
 }
 _RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p}
 

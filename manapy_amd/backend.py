@@ -74,6 +74,13 @@ class HIPBackend(object):
 
     def compile(self, func, signature=False, forcedbackend=None):
         name = func if isinstance(func, str) else getattr(func, '__name__', None)
+        # kernels that share a name across solver modules are keyed '<module>.<name>'
+        # (time_step of solvers/diffusion and solvers/advecdiff differ from solvers/advec's)
+        module = '' if isinstance(func, str) else str(getattr(func, '__module__', '') or '')
+        parts = module.split('.')
+        qual = (parts[-2] + '.' + name) if len(parts) >= 2 else None
+        if qual in self._registry:
+            name = qual
         impl = self._registry.get(name)
         if impl is None:
             raise NotImplementedError("no device kernel for %r (the GPU backend covers the explicit finite-volume "
@@ -295,3 +302,178 @@ def _euler_ghost(domain, rhog, Pg, rhoug, rhovg, rhowg, ug, vg, wg, rhoEg, rhoc,
         t = A.dev(vel, out=True)
         t[bf] = G[mom][bf] / G[0][bf]
     A.finish()
+
+
+# ---------------------------------------------------------------------------------------------
+# diamond scheme (ast/functions2d.py, solvers/diffusion, solvers/advecdiff)
+
+def _i32dev(a, device):
+    import torch
+    if torch.is_tensor(a):
+        return a.to(device=device, dtype=torch.int32).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
+
+
+def _bc_dev(domain, BCdirichlet, dirichletfaces, neumannfaces):
+    import torch
+    from .diamond import bc_tables
+    dev = _m(domain).device
+    return tuple(torch.from_numpy(a).to(dev) for a in bc_tables(domain, np.asarray(BCdirichlet),
+                                                                 np.asarray(dirichletfaces), np.asarray(neumannfaces)))
+
+
+def _no_halo(*lists):
+    if any(len(x) for x in lists):
+        raise NotImplementedError("one subdomain, non-periodic: halo/periodic face lists must be empty")
+
+
+@HIPBackend.register('compute_2dmatrix_size')
+def _matrix_size(domain, nodeidf, halofid, cellnid, halonid, periodicnid, centergn, halocentergn, oldnamen,
+                 BCdirichlet, matrixinnerfaces, halofaces, dirichletfaces):
+    """ast/functions2d.py:582-664 (an index count, evaluated on the host)."""
+    from .solvers.ls import compute_2dmatrix_size
+    _no_halo(halofaces)
+    return compute_2dmatrix_size(domain, BCdirichlet, matrixinnerfaces, dirichletfaces)
+
+
+@HIPBackend.register('get_triplet_2d')
+def _get_triplet_2d(domain, cellfid, nodeidf, vertexn, halofid, haloext, oldnamen, volume, cellnid, centerc,
+                    centerh, halonid, periodicnid, centergn, halocentergn, airDiamond, lambda_x, lambda_y,
+                    lambda_z, number, R_x, R_y, R_z, param1, param2, param3, param4, shift, nbelements,
+                    loctoglob, BCdirichlet, a_loc, irn_loc, jcn_loc, matrixinnerfaces, halofaces,
+                    dirichletfaces):
+    """ast/functions2d.py:328-580. The triplets come back with duplicates already summed (in
+    the reference's insertion order), one per distinct (row, col); the remaining slots of the
+    caller's arrays are (0, 0, 0.0), so any sparse constructor that sums duplicates builds the
+    same matrix."""
+    import torch
+    from .krylov import EllBatch
+    _no_halo(halofaces)
+    m = _m(domain)
+    C = m.C
+    if not np.array_equal(np.sort(np.asarray(matrixinnerfaces)), np.sort(np.asarray(domain.innerfaces))):
+        raise NotImplementedError("matrixinnerfaces must be the domain's inner faces (no periodic faces)")
+    node_dir, rank, _ = _bc_dev(domain, BCdirichlet, dirichletfaces, [])
+    batch = EllBatch((C + 63) // 64, max_inst=1, device=m.device)
+    inst = batch.alloc(C)
+    batch.assemble_diamond(inst, domain.device_diamond, node_dir, rank)
+    W = batch.W
+    ns = (C + 63) // 64
+    sw = batch.slice_w[:ns].cpu().numpy()
+    cols = batch.cols[:ns * W * 64].cpu().numpy().reshape(ns, W, 64)
+    vals = batch.vals[:ns * W * 64].cpu().numpy().reshape(ns, W, 64)
+    rows = np.broadcast_to(np.arange(ns)[:, None, None] * 64 + np.arange(64)[None, None, :], cols.shape)
+    is_diag = cols == rows
+    keep = (np.arange(W)[None, :, None] < sw[:, None, None]) & (rows < C) & \
+        (~is_diag | (np.cumsum(is_diag, axis=1) == 1))
+    v, r, c = vals[keep], rows[keep], cols[keep]
+    n = len(v)
+    if n > len(a_loc):
+        raise ValueError("a_loc holds %d entries, the matrix has %d" % (len(a_loc), n))
+    l2g = np.asarray(loctoglob)
+    a_loc[:] = 0.
+    irn_loc[:] = 0
+    jcn_loc[:] = 0
+    a_loc[:n] = v
+    irn_loc[:n] = l2g[r]
+    jcn_loc[:n] = l2g[c]
+    return n
+
+
+@HIPBackend.register('get_rhs_glob_2d')
+def _get_rhs_glob_2d(domain, cellfid, nodeidf, oldname, volume, centergn, loctoglob, param1, param2, param3, param4,
+                     Pbordnode, Pbordface, rhs, BCdirichlet, centergf, matrixinnerfaces, halofaces, dirichletfaces):
+    """ast/functions2d.py:740-817."""
+    _no_halo(halofaces)
+    m = _m(domain)
+    dd = domain.device_diamond
+    node_dir, rank, _ = _bc_dev(domain, BCdirichlet, dirichletfaces, [])
+    A = _Args(m.device)
+    pn, pf = A.dev(Pbordnode), A.dev(Pbordface)
+    out = A.dev(rhs, out=True)
+    _lib.call('mfv_diamond_rhs', dd.ptr, node_dir.data_ptr(), rank.data_ptr(), pn.data_ptr(), pf.data_ptr(),
+              out.data_ptr(), stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('centertovertex_2d')
+def _centertovertex_2d(domain, w_c, w_ghost, w_halo, w_haloghost, centerc, centerh, cellid, ghostid, haloghostid,
+                       periodicid, haloid, vertexn, centergf, halocenterg, R_x, R_y, R_z, lambda_x, lambda_y,
+                       lambda_z, number, shift, w_n):
+    """ast/functions2d.py:199-277."""
+    m = _m(domain)
+    dd = domain.device_diamond
+    A = _Args(m.device)
+    wc, wg = A.dev(w_c), A.dev(w_ghost)
+    out = A.dev(w_n, out=True)
+    _lib.call('mfv_centertovertex', dd.ptr, wc.data_ptr(), wg.data_ptr(), out.data_ptr(), stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('face_gradient_2d')
+def _face_gradient_2d(domain, w_c, w_ghost, w_halo, w_node, cellidf, nodeidf, centergf, halofid, centerc, centerh,
+                      vertexn, airDiamond, normalf, f_1, f_2, f_3, f_4, shift, wx_face, wy_face, wz_face, innerfaces,
+                      halofaces, dirichletfaces, neumann, periodicfaces):
+    """ast/functions2d.py:108-196."""
+    _no_halo(halofaces, periodicfaces)
+    m = _m(domain)
+    dd = domain.device_diamond
+    _, _, face_bc = _bc_dev(domain, [], dirichletfaces, neumann)
+    A = _Args(m.device)
+    wc, wg, wn = A.dev(w_c), A.dev(w_ghost), A.dev(w_node)
+    wx, wy = A.dev(wx_face, out=True), A.dev(wy_face, out=True)
+    _lib.call('mfv_face_gradient', dd.ptr, 0, wc.data_ptr(), wg.data_ptr(), wn.data_ptr(), face_bc.data_ptr(), None,
+              None, None, wx.data_ptr(), wy.data_ptr(), stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('compute_P_gradient_2d_diamond')
+def _p_gradient_2d(domain, P_c, P_ghost, P_halo, P_node, cellidf, nodeidf, centergf, halofid, centerc, centerh,
+                   oldname, airDiamond, f_1, f_2, f_3, f_4, normalf, shift, Pbordnode, Pbordface, Px_face, Py_face,
+                   Pz_face, BCdirichlet, innerfaces, halofaces, neumannfaces, dirichletfaces, periodicfaces):
+    """ast/functions2d.py:820-936."""
+    _no_halo(halofaces, periodicfaces)
+    m = _m(domain)
+    dd = domain.device_diamond
+    node_dir, _, face_bc = _bc_dev(domain, BCdirichlet, dirichletfaces, neumannfaces)
+    A = _Args(m.device)
+    pc, pg, pn = A.dev(P_c), A.dev(P_ghost), A.dev(P_node)
+    bn, bf = A.dev(Pbordnode), A.dev(Pbordface)
+    px, py = A.dev(Px_face, out=True), A.dev(Py_face, out=True)
+    _lib.call('mfv_face_gradient', dd.ptr, 1, pc.data_ptr(), pg.data_ptr(), pn.data_ptr(), face_bc.data_ptr(),
+              node_dir.data_ptr(), bn.data_ptr(), bf.data_ptr(), px.data_ptr(), py.data_ptr(), stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('explicitscheme_dissipative')
+def _dissipative(domain, wx_face, wy_face, wz_face, cellidf, normalf, namef, dissip_w, Dxx, Dyy, Dzz):
+    """solvers/diffusion/fvm_utils.py:5-26 (same function in solvers/advecdiff)."""
+    m = _m(domain)
+    A = _Args(m.device)
+    wx, wy, wz = A.dev(wx_face), A.dev(wy_face), A.dev(wz_face)
+    out = A.dev(dissip_w, out=True)
+    _lib.call('mfv_explicitscheme_dissipative', m.ptr, wx.data_ptr(), wy.data_ptr(), wz.data_ptr(), float(Dxx),
+              float(Dyy), float(Dzz), out.data_ptr(), stream_ptr())
+    A.finish()
+
+
+def _time_step_diff(with_velocity):
+    def run(domain, u, v, w, cfl, normal, mesure, volume, faceid, dim, Dxx, Dyy, Dzz):
+        import torch
+        m = _m(domain)
+        A = _Args(m.device)
+        uvw = [A.dev(x).data_ptr() for x in (u, v, w)] if with_velocity else [None, None, None]
+        dt = torch.zeros(1, dtype=torch.float64, device=m.device)
+        h = ctypes.c_double(0.)
+        _lib.call('mfv_time_step_diffusion', m.ptr, *uvw, float(cfl), float(Dxx), float(Dyy), float(Dzz),
+                  dt.data_ptr(), ctypes.byref(h), stream_ptr())
+        return h.value
+    run.__doc__ = ("solvers/advecdiff/fvm_utils.py:317-340" if with_velocity
+                   else "solvers/diffusion/fvm_utils.py:30-49") + "; returns dt (float)."
+    return run
+
+
+HIPBackend.register('diffusion.time_step')(_time_step_diff(False))
+HIPBackend.register('advecdiff.time_step')(_time_step_diff(True))
+HIPBackend.register('diffusion.update_new_value', 'advecdiff.update_new_value')(_update_new_value)
+HIPBackend.register('advecdiff.explicitscheme_convective_2d')(_explicitscheme_convective)

@@ -127,6 +127,9 @@ inline Mesh to_kernel(const mfv_mesh *m)
 
 int validate_mesh(const mfv_mesh *m, int need_lsq, int need_order_split, int need_order_sorted);
 
+// Checks the last launch and, if dt_host != NULL, copies dt_dev[0] to it (synchronising).
+int finish_dt(double *dt_dev, double *dt_host, void *stream);
+
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 }  // namespace mfv

@@ -288,7 +288,7 @@ extern "C" int mfv_dt_reset(double *dt, void *stream)
     return check_launch("k_dt_reset");
 }
 
-static int finish_dt(double *dt_dev, double *dt_host, void *stream)
+int mfv::finish_dt(double *dt_dev, double *dt_host, void *stream)
 {
     if (int rc = check_launch("dt kernel")) return rc;
     if (dt_host) {

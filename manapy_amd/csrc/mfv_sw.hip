@@ -348,20 +348,6 @@ __global__ void __launch_bounds__(kBlock) k_sw_step(Mesh m, const double *__rest
     wave_min_to(ring.next, dloc);
 }
 
-int finish_dt(double *dt_dev, double *dt_host, void *stream)
-{
-    if (int rc = check_launch("dt kernel")) return rc;
-    if (dt_host) {
-        hipError_t e = hipMemcpyAsync(dt_host, dt_dev, sizeof(double), hipMemcpyDeviceToHost, as_stream(stream));
-        if (e == hipSuccess) e = hipStreamSynchronize(as_stream(stream));
-        if (e != hipSuccess) {
-            set_error("dt readback: %s", hipGetErrorString(e));
-            return MFV_EHIP;
-        }
-    }
-    return MFV_OK;
-}
-
 }  // namespace
 
 extern "C" int mfv_time_step_sw(const mfv_mesh *m, const double *h, const double *hu, const double *hv,
@@ -843,10 +829,10 @@ __device__ __forceinline__ void load_ehv32(const float *__restrict__ cellrec, in
     v = b;
 }
 
-template <int MF>
-__global__ void __launch_bounds__(kBlock) k_sw_recon32(Mesh m, mfv_sw_mixed p, float om)
+template <int MF, int BS = kBlock>
+__global__ void __launch_bounds__(BS) k_sw_recon32(Mesh m, mfv_sw_mixed p, float om)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.x * BS + threadIdx.x;
     if (i >= m.C) return;
     const size_t C = m.C;
     float cx, cy, ei, ui, vi;
@@ -943,80 +929,93 @@ __global__ void __launch_bounds__(kBlock) k_sw_recon32(Mesh m, mfv_sw_mixed p, f
     }
 }
 
+// time_step_SW of a float64 state with the float32 face geometry, faces in faceid order.
+template <int MF>
+__device__ __forceinline__ double sw_dt32(const mfv_sw_mixed &p, int i, int nf, double hn, double un, double vn,
+                                          double vol, float g, double cfl)
+{
+    const double velson = sqrt((double)g * hn);
+    double lam = 0.;
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        if (s >= nf) break;
+        float4 geo = reinterpret_cast<const float4 *>(p.cfgeo)[MF * (size_t)i + s];
+        double n0 = geo.x, n1 = geo.y, mesu = geo.z;
+        double u_n = fabs(un / hn * n0 + vn / hn * n1);
+        lam += (u_n / mesu + velson) * mesu;
+    }
+    return cfl * vol / lam;
+}
+
+// One cell of the mixed step: Kahan-compensated float32 residual, float64 update with dt, new
+// records, and the cell's time_step_SW term of the new state (returned).
+template <int MF>
+__device__ __forceinline__ double sw_flux32_cell(const Mesh &m, const mfv_sw_mixed &p, float g, double cfl, int i,
+                                                 double dt)
+{
+    const size_t C = m.C;
+    const float *rec = p.cellrec + 8 * (size_t)i;
+    const float h_i = rec[5], Z_i = rec[6];
+    const int nf = m.cell_nfaces[i];
+    const unsigned ord = p.order[i];
+    float r0 = 0.f, c0 = 0.f, r1 = 0.f, c1 = 0.f, r2 = 0.f, c2 = 0.f, s1 = 0.f, d1 = 0.f, s2 = 0.f, d2 = 0.f;
+#pragma unroll
+    for (int r = 0; r < MF; r++) {
+        if (r >= nf) break;
+        const int s = (ord >> (2 * r)) & 3;
+        float4 geo = reinterpret_cast<const float4 *>(p.cfgeo)[MF * (size_t)i + s];
+        float4 own = reinterpret_cast<const float4 *>(p.facerec)[MF * (size_t)i + s];
+        const int code = p.nbr[(size_t)s * C + i];
+        float4 nb;
+        if (code >= 0) {
+            nb = reinterpret_cast<const float4 *>(p.facerec)[MF * (size_t)(code >> 3) + ((code >> 1) & 3)];
+        } else {
+            nb = make_float4(own.x, -own.y, -own.z, own.w);
+        }
+        float F0, F1, F2, bL, bR;
+        if (code < 0 || (code & 1) == 0) {
+            sw_face_t<float>(own.x, own.y, own.z, own.w, h_i, Z_i, nb.x, nb.y, nb.z, nb.w, h_i, Z_i, geo.x, geo.y,
+                             geo.z, g, F0, F1, F2, bL, bR);
+            kahan_add(r0, c0, -F0);
+            kahan_add(r1, c1, -F1);
+            kahan_add(r2, c2, -F2);
+            kahan_add(s1, d1, -(bL * geo.x));
+            kahan_add(s2, d2, -(bL * geo.y));
+        } else {
+            sw_face_t<float>(nb.x, nb.y, nb.z, nb.w, h_i, Z_i, own.x, own.y, own.z, own.w, h_i, Z_i, geo.x, geo.y,
+                             geo.z, g, F0, F1, F2, bL, bR);
+            kahan_add(r0, c0, F0);
+            kahan_add(r1, c1, F1);
+            kahan_add(r2, c2, F2);
+            kahan_add(s1, d1, bR * geo.x);
+            kahan_add(s2, d2, bR * geo.y);
+        }
+    }
+    double *st = p.state + 4 * (size_t)i;
+    const double2 a = reinterpret_cast<const double2 *>(st)[0], b = reinterpret_cast<const double2 *>(st)[1];
+    const double vol = b.y;
+    const double R0 = (double)r0 - (double)c0;
+    const double R1 = ((double)r1 - (double)c1) + ((double)s1 - (double)d1);
+    const double R2 = ((double)r2 - (double)c2) + ((double)s2 - (double)d2);
+    const double hn = a.x + dt * R0 / vol;
+    const double un = a.y + dt * (R1 / vol);
+    const double vn = b.x + dt * (R2 / vol);
+    reinterpret_cast<double2 *>(st)[0] = make_double2(hn, un);
+    st[2] = vn;
+    float *w = p.cellrec + 8 * (size_t)i;
+    reinterpret_cast<float4 *>(w)[0] = make_float4(rec[0], rec[1], (float)(hn + (double)Z_i), (float)un);
+    w[4] = (float)vn;
+    w[5] = (float)hn;
+    return sw_dt32<MF>(p, i, nf, hn, un, vn, vol, g, cfl);
+}
+
 template <int MF>
 __global__ void __launch_bounds__(kBlock) k_sw_flux32(Mesh m, mfv_sw_mixed p, float g, double cfl, DtRing ring)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     ring_advance(ring);
     double dloc = 1e300;
-    if (i < m.C) {
-        const size_t C = m.C;
-        const float *rec = p.cellrec + 8 * (size_t)i;
-        const float h_i = rec[5], Z_i = rec[6];
-        const int nf = m.cell_nfaces[i];
-        const unsigned ord = p.order[i];
-        float r0 = 0.f, c0 = 0.f, r1 = 0.f, c1 = 0.f, r2 = 0.f, c2 = 0.f, s1 = 0.f, d1 = 0.f, s2 = 0.f, d2 = 0.f;
-#pragma unroll
-        for (int r = 0; r < MF; r++) {
-            if (r >= nf) break;
-            const int s = (ord >> (2 * r)) & 3;
-            float4 geo = reinterpret_cast<const float4 *>(p.cfgeo)[MF * (size_t)i + s];
-            float4 own = reinterpret_cast<const float4 *>(p.facerec)[MF * (size_t)i + s];
-            const int code = p.nbr[(size_t)s * C + i];
-            float4 nb;
-            if (code >= 0) {
-                nb = reinterpret_cast<const float4 *>(p.facerec)[MF * (size_t)(code >> 3) + ((code >> 1) & 3)];
-            } else {
-                nb = make_float4(own.x, -own.y, -own.z, own.w);
-            }
-            float F0, F1, F2, bL, bR;
-            if (code < 0 || (code & 1) == 0) {
-                sw_face_t<float>(own.x, own.y, own.z, own.w, h_i, Z_i, nb.x, nb.y, nb.z, nb.w, h_i, Z_i, geo.x, geo.y,
-                                 geo.z, g, F0, F1, F2, bL, bR);
-                kahan_add(r0, c0, -F0);
-                kahan_add(r1, c1, -F1);
-                kahan_add(r2, c2, -F2);
-                kahan_add(s1, d1, -(bL * geo.x));
-                kahan_add(s2, d2, -(bL * geo.y));
-            } else {
-                sw_face_t<float>(nb.x, nb.y, nb.z, nb.w, h_i, Z_i, own.x, own.y, own.z, own.w, h_i, Z_i, geo.x, geo.y,
-                                 geo.z, g, F0, F1, F2, bL, bR);
-                kahan_add(r0, c0, F0);
-                kahan_add(r1, c1, F1);
-                kahan_add(r2, c2, F2);
-                kahan_add(s1, d1, bR * geo.x);
-                kahan_add(s2, d2, bR * geo.y);
-            }
-        }
-        const double dt = ring.cur[0];
-        double *st = p.state + 4 * (size_t)i;
-        const double2 a = reinterpret_cast<const double2 *>(st)[0], b = reinterpret_cast<const double2 *>(st)[1];
-        const double vol = b.y;
-        const double R0 = (double)r0 - (double)c0;
-        const double R1 = ((double)r1 - (double)c1) + ((double)s1 - (double)d1);
-        const double R2 = ((double)r2 - (double)c2) + ((double)s2 - (double)d2);
-        const double hn = a.x + dt * R0 / vol;
-        const double un = a.y + dt * (R1 / vol);
-        const double vn = b.x + dt * (R2 / vol);
-        reinterpret_cast<double2 *>(st)[0] = make_double2(hn, un);
-        st[2] = vn;
-        float *w = p.cellrec + 8 * (size_t)i;
-        reinterpret_cast<float4 *>(w)[0] = make_float4(rec[0], rec[1], (float)(hn + (double)Z_i), (float)un);
-        w[4] = (float)vn;
-        w[5] = (float)hn;
-        // time_step_SW of the float64 state (float32 geometry), faces in faceid order
-        const double velson = sqrt((double)g * hn);
-        double lam = 0.;
-#pragma unroll
-        for (int s = 0; s < MF; s++) {
-            if (s >= nf) break;
-            float4 geo = reinterpret_cast<const float4 *>(p.cfgeo)[MF * (size_t)i + s];
-            double n0 = geo.x, n1 = geo.y, mesu = geo.z;
-            double u_n = fabs(un / hn * n0 + vn / hn * n1);
-            lam += (u_n / mesu + velson) * mesu;
-        }
-        dloc = cfl * vol / lam;
-    }
+    if (i < m.C) dloc = sw_flux32_cell<MF>(m, p, g, cfl, i, ring.cur[0]);
     wave_min_to(ring.next, dloc);
 }
 
@@ -1084,6 +1083,117 @@ __global__ void __launch_bounds__(kSumBlock) k_sw_sums_final(int nb, const doubl
     }
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) out[k] = sh[k][0];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Ensembles: many independent instances concatenated into one mesh, each instance's cells
+// starting at a multiple of 64 so that every wavefront belongs to one instance. The step
+// kernels are the mixed-precision ones; only the dt ring is per instance.
+
+struct BatchK {
+    int I, W;
+    const int32_t *__restrict__ wave_inst;
+    const int32_t *__restrict__ inst_cell0;
+    const int32_t *__restrict__ inst_ncells;
+    double *__restrict__ rings;
+};
+
+template <int MF, int BS>
+__global__ void __launch_bounds__(BS) k_sw_flux32_batch(Mesh m, mfv_sw_mixed p, float g, double cfl, BatchK b,
+                                                        int kcur, int knext)
+{
+    const int i = blockIdx.x * BS + threadIdx.x;
+    // C is a multiple of 64: all lanes of a wave agree on i < C and on the instance
+    if (i >= m.C) return;
+    const int inst = b.wave_inst[i >> 6];
+    if (inst < 0) return;
+    double *ring = b.rings + 4 * (size_t)inst;
+    const double dloc = sw_flux32_cell<MF>(m, p, g, cfl, i, ring[kcur]);
+    wave_min_to(ring + knext, dloc);
+}
+
+__global__ void k_ring_advance_batch(double *__restrict__ rings, int I, int kcur, int kspare)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= I) return;
+    rings[4 * (size_t)t + kspare] = 1e6;
+    rings[4 * (size_t)t + 3] += rings[4 * (size_t)t + kcur];
+}
+
+__global__ void k_ring_reset_batch(double *__restrict__ rings, int I)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= I) return;
+    rings[4 * (size_t)t + 0] = 1e6;
+    rings[4 * (size_t)t + 1] = 1e6;
+    rings[4 * (size_t)t + 2] = 1e6;
+    rings[4 * (size_t)t + 3] = 0.;
+}
+
+// dt of the current state into ring slot 0 (after k_ring_reset_batch)
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_dt_batch(Mesh m, mfv_sw_mixed p, float g, double cfl, BatchK b)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m.C) return;
+    const int inst = b.wave_inst[i >> 6];
+    if (inst < 0) return;
+    const double *st = p.state + 4 * (size_t)i;
+    const double dloc = sw_dt32<MF>(p, i, m.cell_nfaces[i], st[0], st[1], st[2], st[3], g, cfl);
+    wave_min_to(b.rings + 4 * (size_t)inst, dloc);
+}
+
+// Per-instance float64 totals sum (h, hu, hv) vol: one block per instance, fixed order.
+__global__ void __launch_bounds__(kSumBlock) k_sw_sums_batch(const double *__restrict__ state, BatchK b,
+                                                             double *__restrict__ out)
+{
+    __shared__ double sh[3][kSumBlock];
+    const int inst = blockIdx.x;
+    const int c0 = b.inst_cell0[inst], n = b.inst_ncells[inst];
+    double a = 0., bb = 0., c = 0.;
+    for (int j = threadIdx.x; j < n; j += kSumBlock) {
+        const double *s = state + 4 * (size_t)(c0 + j);
+        a += s[0] * s[3];
+        bb += s[1] * s[3];
+        c += s[2] * s[3];
+    }
+    sh[0][threadIdx.x] = a;
+    sh[1][threadIdx.x] = bb;
+    sh[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int w = kSumBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) out[3 * (size_t)inst + k] = sh[k][0];
+}
+
+template <int MF, int BS>
+void launch_batch_step(const Mesh &m, const mfv_sw_mixed &p, const BatchK &b, float g, double cfl, float om,
+                       int step, hipStream_t st)
+{
+    const int k = ((step % 3) + 3) % 3;
+    const unsigned grid = (unsigned)((m.C + BS - 1) / BS);
+    hipLaunchKernelGGL((k_sw_recon32<MF, BS>), dim3(grid), dim3(BS), 0, st, m, p, om);
+    hipLaunchKernelGGL((k_sw_flux32_batch<MF, BS>), dim3(grid), dim3(BS), 0, st, m, p, g, cfl, b, k, (k + 1) % 3);
+    hipLaunchKernelGGL(k_ring_advance_batch, dim3((b.I + 255) / 256), dim3(256), 0, st, b.rings, b.I, k, (k + 2) % 3);
+}
+
+template <int MF>
+int batch_step_bs(const Mesh &m, const mfv_sw_mixed &p, const BatchK &b, float g, double cfl, float om, int step,
+                  int block, hipStream_t st)
+{
+    switch (block) {
+    case 64: launch_batch_step<MF, 64>(m, p, b, g, cfl, om, step, st); break;
+    case 128: launch_batch_step<MF, 128>(m, p, b, g, cfl, om, step, st); break;
+    case 256: launch_batch_step<MF, 256>(m, p, b, g, cfl, om, step, st); break;
+    case 512: launch_batch_step<MF, 512>(m, p, b, g, cfl, om, step, st); break;
+    default: set_error("block size %d not in {64, 128, 256, 512}", block); return MFV_EINVAL;
+    }
+    return check_launch("ensemble step");
 }
 
 }  // namespace
@@ -1166,4 +1276,144 @@ extern "C" int mfv_sw_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, dou
     if (int rc = check_launch("k_sw_sums_partial")) return rc;
     hipLaunchKernelGGL(k_sw_sums_final, dim3(1), dim3(kSumBlock), 0, as_stream(stream), nb, partial, out3);
     return check_launch("k_sw_sums_final");
+}
+
+// ---------------------------------------------------------------------------------------------
+// ensembles of independent shallow-water instances (mfv_sw_batch)
+
+static int check_batch(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b)
+{
+    if (int rc = check_mixed(m, p)) return rc;
+    MFV_REQUIRE(b != nullptr, "batch descriptor is NULL");
+    MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
+    MFV_REQUIRE(m->nb_cells % 64 == 0, "batched mesh must have a multiple of 64 cells, got %d", m->nb_cells);
+    MFV_REQUIRE(b->nb_waves == m->nb_cells / 64, "nb_waves %d != cells/64 %d", b->nb_waves, m->nb_cells / 64);
+    MFV_REQUIRE(b->nb_inst >= 1, "nb_inst must be positive");
+    MFV_REQUIRE(b->wave_inst && b->inst_cell0 && b->inst_ncells && b->rings, "batch descriptor has NULL arrays");
+    return MFV_OK;
+}
+
+static BatchK to_batch(const mfv_sw_batch *b)
+{
+    return BatchK{b->nb_inst, b->nb_waves, b->wave_inst, b->inst_cell0, b->inst_ncells, b->rings};
+}
+
+extern "C" int mfv_sw_batch_dt_init(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b, double grav,
+                                    double cfl, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_batch(m, p, b)) return rc;
+    hipStream_t st = as_stream(stream);
+    const BatchK bk = to_batch(b);
+    hipLaunchKernelGGL(k_ring_reset_batch, dim3((b->nb_inst + 255) / 256), dim3(256), 0, st, b->rings, b->nb_inst);
+    if (m->nb_cells) {
+        auto k = (m->max_cell_faces == 3) ? k_sw_dt_batch<3> : k_sw_dt_batch<4>;
+        hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, st, to_kernel(m), *p, (float)grav, cfl,
+                           bk);
+    }
+    return check_launch("k_sw_dt_batch");
+}
+
+extern "C" int mfv_sw_batch_step(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b, double grav,
+                                 double cfl, int order, int step, int block, void *stream)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_batch(m, p, b)) return rc;
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    if (m->nb_cells == 0) return MFV_OK;
+    const Mesh mk = to_kernel(m);
+    const BatchK bk = to_batch(b);
+    const float om = (float)(order - 1);
+    if (m->max_cell_faces == 3)
+        return batch_step_bs<3>(mk, *p, bk, (float)grav, cfl, om, step, block, as_stream(stream));
+    return batch_step_bs<4>(mk, *p, bk, (float)grav, cfl, om, step, block, as_stream(stream));
+}
+
+extern "C" int mfv_sw_batch_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b,
+                                         double *out, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_batch(m, p, b)) return rc;
+    MFV_REQUIRE(out, "NULL array");
+    hipLaunchKernelGGL(k_sw_sums_batch, dim3(b->nb_inst), dim3(kSumBlock), 0, as_stream(stream), p->state,
+                       to_batch(b), out);
+    return check_launch("k_sw_sums_batch");
+}
+
+namespace {
+struct MfvGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+void graph_free(MfvGraph *g)
+{
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+}  // namespace
+
+extern "C" int mfv_sw_batch_graph(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b, double grav,
+                                  double cfl, int order, int nsteps, int block, void **graph_out)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_batch(m, p, b)) return rc;
+    MFV_REQUIRE(graph_out, "graph_out is NULL");
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    MFV_REQUIRE(nsteps >= 3 && nsteps % 3 == 0 && nsteps <= 3000, "nsteps %d must be a multiple of 3 in [3, 3000]",
+                nsteps);
+    MFV_REQUIRE(block == 64 || block == 128 || block == 256 || block == 512, "block size %d not in {64..512}",
+                block);
+    *graph_out = nullptr;
+    MfvGraph *g = new MfvGraph();
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        set_error("graph capture setup: %s", hipGetErrorString(e));
+        graph_free(g);
+        return MFV_EHIP;
+    }
+    const Mesh mk = to_kernel(m);
+    const BatchK bk = to_batch(b);
+    int rc = MFV_OK;
+    for (int k = 0; k < nsteps && rc == MFV_OK; k++) {
+        rc = (m->max_cell_faces == 3)
+                 ? batch_step_bs<3>(mk, *p, bk, (float)grav, cfl, (float)(order - 1), k, block, g->stream)
+                 : batch_step_bs<4>(mk, *p, bk, (float)grav, cfl, (float)(order - 1), k, block, g->stream);
+    }
+    e = hipStreamEndCapture(g->stream, &g->graph);
+    if (rc != MFV_OK || e != hipSuccess) {
+        if (rc == MFV_OK) set_error("graph capture: %s", hipGetErrorString(e));
+        graph_free(g);
+        return rc != MFV_OK ? rc : MFV_EHIP;
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        set_error("graph instantiate: %s", hipGetErrorString(e));
+        graph_free(g);
+        return MFV_EHIP;
+    }
+    *graph_out = g;
+    return MFV_OK;
+}
+
+extern "C" int mfv_graph_launch(void *graph, void *stream)
+{
+    MFV_REQUIRE(graph, "graph is NULL");
+    hipError_t e = hipGraphLaunch(static_cast<MfvGraph *>(graph)->exec, as_stream(stream));
+    if (e != hipSuccess) {
+        set_error("hipGraphLaunch: %s", hipGetErrorString(e));
+        return MFV_EHIP;
+    }
+    return MFV_OK;
+}
+
+extern "C" int mfv_graph_destroy(void *graph)
+{
+    graph_free(static_cast<MfvGraph *>(graph));
+    return MFV_OK;
 }

@@ -1002,6 +1002,14 @@ class Domain(object):
         return self._device_mesh
 
     @property
+    def device_diamond(self):
+        """The GPU copy of the diamond-scheme geometry (2D, built on first use)."""
+        if getattr(self, '_device_diamond', None) is None:
+            from .diamond import DeviceDiamond
+            self._device_diamond = DeviceDiamond(self.device_mesh, host=getattr(self, '_diamond_host', None))
+        return self._device_diamond
+
+    @property
     def backend(self):
         if self._backend is None:
             from .backend import HIPBackend

@@ -102,11 +102,11 @@ def stream_ptr(stream=None):
 class DeviceMesh(object):
     """Device copy of a Domain's hot-path arrays and its ``mfv_mesh`` descriptor."""
 
-    def __init__(self, domain, lsq=True, device=None):
+    def __init__(self, domain, lsq=True, device=None, host=None):
         torch = require_gpu()
         self.domain = domain
         self.device = torch.device(device if device is not None else 'cuda')
-        host = gpu_layout(domain, lsq=lsq)
+        host = host if host is not None else gpu_layout(domain, lsq=lsq)
         self.host = host
         self.dim = host['dim']
         self.C = host['C']

@@ -1,0 +1,219 @@
+"""Continuous-batching service of implicit advection-diffusion solves.
+
+A request carries a mesh (a ``Domain``), an initial field, a constant velocity and a
+diffusivity D, and asks for ``nsteps`` time steps of
+
+    u*      = u^n + dt * upwind_advection(u^n)        (explicitscheme_convective_2d, order 1)
+    (I - dt D A) u^{n+1} = u*                         (diamond matrix A, get_triplet_2d)
+
+with all-Neumann boundaries (so the Dirichlet lifting rhs0 is zero) and dt the CFL step
+of the advection (constant, the velocity is constant). The implicit system is solved to
+``||b - M x|| <= tol ||b||`` (default 1e-10) by the arena solver of ``krylov.EllBatch``.
+
+``ServingEngine.run`` admits requests as they arrive while the arena has room for them
+(HBM footprint: contiguous matrix slices plus the request's mesh tables, under a byte
+budget), keeps every admitted request's state on the device between steps, and advances
+all of them together: each engine tick launches the advection of the requests entering a
+new step and one solver tick for every system in the arena, whatever stage it is at.
+Requests therefore overlap on the device; each request's arithmetic is independent of the
+others (per-request kernels, position-independent batched solver), so a result is the
+same alone and in any mix.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceMesh, require_gpu, stream_ptr
+from .krylov import EllBatch
+
+PHASE_WAIT, PHASE_ADVECT, PHASE_SOLVE, PHASE_DONE = 0, 1, 2, 3
+
+
+class AdvDiffRequest(object):
+    def __init__(self, rid, domain, u0, velocity=(1.0, 0.5), D=1e-2, nsteps=20, arrival=0., cfl=0.8):
+        self.rid = rid
+        self.domain = domain
+        self.u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        self.velocity = (float(velocity[0]), float(velocity[1]))
+        self.D = float(D)
+        self.nsteps = int(nsteps)
+        self.arrival = float(arrival)
+        self.cfl = float(cfl)
+        self.C = int(domain.nbcells)
+        self._mesh_host = None
+        # filled by the engine
+        self.inst = None
+        self.phase = PHASE_WAIT
+        self.step = 0
+        self.dt = None
+        self.t_admit = self.t_done = None
+        self.result = None
+        self.iterations = 0
+        self._dev = None
+
+    def prepare(self):
+        """Host-side tables of the request's mesh (GPU layout and diamond weights), built
+        before the request is submitted so that admission only uploads them."""
+        from .device import gpu_layout
+        from .diamond import diamond_layout
+        self._mesh_host = gpu_layout(self.domain, lsq=False)
+        self.domain._diamond_host = diamond_layout(self.domain)
+        return self
+
+    def footprint(self, arena):
+        """Bytes the request holds on the device while admitted."""
+        slices = (self.C + 63) // 64
+        F = int(self.domain.nbfaces)
+        mesh = self.C * (8 * 4 + 4 * 4 + 2) + F * (4 * 3 + 8 * 8)   # mfv_mesh tables (no LSQ)
+        fields = 8 * (3 * F + 3 * self.C) + 64
+        return slices * arena.bytes_per_slice() + mesh + fields
+
+
+class ServingEngine(object):
+    def __init__(self, max_cells=4 << 20, max_inst=128, width=16, hbm_budget=None, n_iter=6, tol=1e-10,
+                 tol_inner=1e-4, device=None):
+        torch = require_gpu()
+        self.arena = EllBatch((int(max_cells) + 63) // 64, max_inst=max_inst, width=width, device=device)
+        self.device = self.arena.device
+        self.hbm_budget = int(hbm_budget) if hbm_budget else None
+        self.n_iter = int(n_iter)
+        self.tol, self.tol_inner = float(tol), float(tol_inner)
+        self.active = []
+        self.in_use = 0
+        self.ticks = 0
+        self.max_active = 0
+
+    # -- admission --------------------------------------------------------------------------------
+    def can_admit(self, req):
+        if req.footprint(self.arena) + self.in_use > (self.hbm_budget or 1 << 62):
+            return False
+        if not self.arena._free_ids:
+            return False
+        return self.arena._ranges.largest() >= (req.C + 63) // 64
+
+    def admit(self, req, now):
+        import torch
+        inst = self.arena.alloc(req.C)
+        if inst is None:
+            raise RuntimeError("admit called without room")
+        d = req.domain
+        dm = d._device_mesh if getattr(d, '_device_mesh', None) is not None else \
+            DeviceMesh(d, lsq=False, host=req._mesh_host)
+        d._device_mesh = dm
+        dev = self.device
+        C, F = req.C, int(d.nbfaces)
+        s = stream_ptr()
+        uf = torch.full((F,), req.velocity[0], dtype=torch.float64, device=dev)
+        vf = torch.full((F,), req.velocity[1], dtype=torch.float64, device=dev)
+        wf = torch.zeros(F, dtype=torch.float64, device=dev)
+        uc, vc, wc = (torch.zeros(C, dtype=torch.float64, device=dev) for _ in range(3))
+        _lib.call('mfv_facetocell', dm.ptr, uf.data_ptr(), uc.data_ptr(), s)
+        _lib.call('mfv_facetocell', dm.ptr, vf.data_ptr(), vc.data_ptr(), s)
+        dt_dev = torch.zeros(1, dtype=torch.float64, device=dev)
+        h = ctypes.c_double(0.)
+        _lib.call('mfv_time_step', dm.ptr, uc.data_ptr(), vc.data_ptr(), wc.data_ptr(), req.cfl, dt_dev.data_ptr(),
+                  ctypes.byref(h), s)
+        req.dt = h.value
+        ring = torch.zeros(4, dtype=torch.float64, device=dev)
+        _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt_dev.data_ptr(), s)
+        # matrix of the all-Neumann diamond operator, then the geometry tables are dropped
+        dd = d.device_diamond
+        N = int(d.nbnodes)
+        node_dir = torch.zeros(N, dtype=torch.uint8, device=dev)
+        rank = torch.full((F,), -1, dtype=torch.int32, device=dev)
+        self.arena.assemble_diamond(inst, dd, node_dir, rank)
+        d._device_diamond = None
+        row0, _, ns = self.arena.rows(inst)
+        self.arena.x[row0:row0 + 64 * ns].zero_()
+        self.arena.b[row0:row0 + 64 * ns].zero_()
+        self.arena.x[row0:row0 + C] = torch.from_numpy(req.u0).to(dev)
+        self.arena.configure(inst, 1., -req.dt * req.D, tol=self.tol, tol_in=self.tol_inner)
+        req._dev = {'mesh': dm, 'uf': uf, 'vf': vf, 'wf': wf, 'uc': uc, 'vc': vc, 'wc': wc, 'ring': ring,
+                    'row0': row0}
+        req.inst = inst
+        req.phase = PHASE_ADVECT
+        req.t_admit = now
+        self.in_use += req.footprint(self.arena)
+        self.active.append(req)
+
+    def _advect(self, req):
+        """u* = u^n + dt * convective(u^n) into b, initial guess x = u*, status START."""
+        a = self.arena
+        g = req._dev
+        row0, C = g['row0'], req.C
+        xs, bs = a.x[row0:row0 + C], a.b[row0:row0 + C]
+        _lib.call('mfv_advection_step', g['mesh'].ptr, xs.data_ptr(), bs.data_ptr(), g['uc'].data_ptr(),
+                  g['vc'].data_ptr(), g['wc'].data_ptr(), g['uf'].data_ptr(), g['vf'].data_ptr(), g['wf'].data_ptr(),
+                  req.cfl, 1, g['ring'].data_ptr(), req.step, None, stream_ptr())
+        xs.copy_(bs)
+        a.status[req.inst] = _lib.KS_START
+        req.phase = PHASE_SOLVE
+
+    def _finish(self, req, now):
+        row0 = req._dev['row0']
+        req.result = self.arena.x[row0:row0 + req.C].cpu().numpy().copy()
+        req.iterations = int(self.arena.scalars(req.inst)[_lib.KS_ITTOT])
+        self.arena.release(req.inst)
+        self.arena.publish()
+        self.in_use -= req.footprint(self.arena)
+        req._dev = None
+        req.domain._device_mesh = None
+        req.phase = PHASE_DONE
+        req.t_done = now
+        self.active.remove(req)
+
+    # -- the loop -------------------------------------------------------------------------------
+    def run(self, requests, clock=None, max_ticks=10 ** 7):
+        """Serve ``requests`` (sorted by arrival time, seconds from the start); returns them
+        with t_admit / t_done set and ``result`` filled."""
+        import torch
+        pending = sorted(requests, key=lambda r: r.arrival)
+        clock = clock or time.perf_counter
+        t0 = clock()
+        done = []
+        for _ in range(max_ticks):
+            now = clock() - t0
+            admitted = False
+            while pending and pending[0].arrival <= now and self.can_admit(pending[0]):
+                self.admit(pending.pop(0), now)
+                admitted = True
+            if admitted:
+                self.arena.publish()
+                self.max_active = max(self.max_active, len(self.active))
+            if not self.active:
+                if not pending:
+                    break
+                wait = pending[0].arrival - (clock() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+                continue
+            for r in self.active:
+                if r.phase == PHASE_ADVECT:
+                    self._advect(r)
+            self.arena.tick(self.n_iter)
+            self.ticks += 1
+            st = self.arena.status.cpu().numpy()
+            now = clock() - t0
+            for r in list(self.active):
+                s = st[r.inst]
+                if s == _lib.KS_DONE:
+                    r.step += 1
+                    if r.step == r.nsteps:
+                        self._finish(r, now)
+                        done.append(r)
+                    else:
+                        r.phase = PHASE_ADVECT
+                elif s == _lib.KS_FAILED:
+                    raise _lib.MfvError("request %s: implicit solve failed at step %d" % (r.rid, r.step))
+        else:
+            raise _lib.MfvError("serving loop did not finish in %d ticks" % max_ticks)
+        torch.cuda.synchronize()
+        return done
+
+
+def latency_stats(done):
+    lat = np.array([r.t_done - r.arrival for r in done])
+    return {'p50_s': float(np.percentile(lat, 50)), 'p99_s': float(np.percentile(lat, 99)),
+            'mean_s': float(lat.mean()), 'max_s': float(lat.max())}

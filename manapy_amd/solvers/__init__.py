@@ -2,5 +2,9 @@
 from .advec import AdvectionSolver
 from .shallowater import ShallowWaterSolver
 from .euler import EulerSolver3D, initialisation_euler_3d
+from .diffusion import DiffusionSolver, AdvectionDiffusionSolver
+from .ls import LinearSolver, HIPKrylovSolver, ScipySolver, MUMPSSolver, PETScKrylovSolver
 
-__all__ = ['AdvectionSolver', 'ShallowWaterSolver', 'EulerSolver3D', 'initialisation_euler_3d']
+__all__ = ['AdvectionSolver', 'ShallowWaterSolver', 'EulerSolver3D', 'initialisation_euler_3d', 'DiffusionSolver',
+           'AdvectionDiffusionSolver', 'LinearSolver', 'HIPKrylovSolver', 'ScipySolver', 'MUMPSSolver',
+           'PETScKrylovSolver']

@@ -39,6 +39,7 @@ class Boundary(object):
         nodes = {"in": domain.innodes, "out": domain.outnodes, "bottom": domain.bottomnodes,
                  "upper": domain.uppernodes}
         self._BCfaces = np.asarray(faces[BCloc], dtype=np.int32)
+        self.BCvaluenode = None
         self._BCnodes = np.asarray(nodes.get(BCloc, np.zeros(0, np.int32)), dtype=np.int32)
         dm = domain.device_mesh
         self._faces_dev = torch.as_tensor(self._BCfaces, device=dm.device)
@@ -55,8 +56,17 @@ class Boundary(object):
                     valueface[i] = values(fc[i][0], fc[i][1], fc[i][2])
             else:
                 valueface[self._BCfaces] = float(values)
+            valuenode = np.zeros(int(domain.nbnodes))
+            vx = domain.nodes.vertex
+            own = np.nonzero(domain.nodes.oldname == BCtypeindex)[0]
+            if isinstance(values, types.LambdaType):
+                for i in own:
+                    valuenode[i] = values(vx[i][0], vx[i][1], vx[i][2])
+            else:
+                valuenode[own] = float(values)
             if BCtype == "dirichlet":
                 self.BCvalueface = torch.as_tensor(valueface, device=dm.device)
+                self.BCvaluenode = torch.as_tensor(valuenode, device=dm.device)
             else:
                 self.constNH = torch.as_tensor(valueface, device=dm.device)
 
@@ -124,6 +134,7 @@ class Variable(object):
         locs = ["in", "out", "bottom", "upper"] + (["front", "back"] if int(self._dim) == 3 else [])
         self._BCs = {}
         self.dirichletfaces, self.neumannfaces, self.neumannNHfaces = [], [], []
+        self.BCdirichlet, self.BCneumann, self.BCneumannNH = [], [], []
         for loc in locs:
             bct = "neumann" if self._BC is None else self._BC.get(loc, "neumann")
             if bct == "periodic":
@@ -139,13 +150,30 @@ class Variable(object):
             self._BCs[loc] = b
             if bct == "dirichlet":
                 self.dirichletfaces.extend(b.BCfaces)
+                self.BCdirichlet.append(b.BCtypeindex)
             elif bct == "neumannNH":
                 self.neumannNHfaces.extend(b.BCfaces)
+                self.BCneumannNH.append(b.BCtypeindex)
             else:
                 self.neumannfaces.extend(b.BCfaces)
+                self.BCneumann.append(b.BCtypeindex)
         self.dirichletfaces = np.sort(np.asarray(self.dirichletfaces, dtype=np.int32))
         self.neumannfaces = np.sort(np.asarray(self.neumannfaces, dtype=np.int32))
         self.neumannNHfaces = np.asarray(self.neumannNHfaces, dtype=np.int32)
+        self.BCdirichlet = np.asarray(self.BCdirichlet, dtype=np.int32)
+        self.BCneumann = np.asarray(self.BCneumann, dtype=np.int32)
+        self.BCneumannNH = np.asarray(self.BCneumannNH, dtype=np.int32)
+        self._diamond_bc = None
+
+    def diamond_bc(self):
+        """Device tables (node_dir, face_dir_rank, face_bc) of the diamond kernels."""
+        if self._diamond_bc is None:
+            import torch
+            from .diamond import bc_tables
+            dev = self._domain.device_mesh.device
+            self._diamond_bc = tuple(torch.from_numpy(a).to(dev) for a in bc_tables(
+                self._domain, self.BCdirichlet, self.dirichletfaces, self.neumannfaces))
+        return self._diamond_bc
 
     # -- operators ---------------------------------------------------------------------------
     def update_halo_value(self):
@@ -160,6 +188,21 @@ class Variable(object):
         """``ast/core.py:422-423`` -> ``ast_utils.facetocell``."""
         dm = self._domain.device_mesh
         _lib.call('mfv_facetocell', dm.ptr, self.face.data_ptr(), self.cell.data_ptr(), stream_ptr())
+
+    def interpolate_celltonode(self):
+        """``ast/core.py:429-440`` -> ``centertovertex_2d`` (pseudo-Laplacian weights)."""
+        dd = self._domain.device_diamond
+        _lib.call('mfv_centertovertex', dd.ptr, self.cell.data_ptr(), self.ghost.data_ptr(), self.node.data_ptr(),
+                  stream_ptr())
+
+    def compute_face_gradient(self):
+        """``ast/core.py:480-490`` -> ``face_gradient_2d`` on the inner, Dirichlet and Neumann
+        faces (uses ``node``: call interpolate_celltonode first, as the reference does)."""
+        dd = self._domain.device_diamond
+        node_dir, _, face_bc = self.diamond_bc()
+        _lib.call('mfv_face_gradient', dd.ptr, 0, self.cell.data_ptr(), self.ghost.data_ptr(), self.node.data_ptr(),
+                  face_bc.data_ptr(), None, None, None, self.gradfacex.data_ptr(), self.gradfacey.data_ptr(),
+                  stream_ptr())
 
     def compute_cell_gradient(self):
         """``ast/core.py:442-478``: least-squares gradient then Barth-Jespersen limiter."""

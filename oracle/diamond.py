@@ -9,6 +9,9 @@ Follows, loop for loop and in insertion order:
   * face_gradient_2d           ast/functions2d.py:108-196
   * explicitscheme_dissipative solvers/diffusion/fvm_utils.py:5-26
   * time_step (diffusion)      solvers/diffusion/fvm_utils.py:30-49
+  * time_step (advecdiff)      solvers/advecdiff/fvm_utils.py:317-340
+  * compute_P_gradient_2d_diamond  ast/functions2d.py:820-936
+  * insertion-order duplicate sum of the COO triplets (what summing them one by one gives)
 One subdomain, no periodic faces: the halo/periodic loops are empty.
 
 Pinned in tests/test_diamond.py: the assembled operator is exact on affine fields
@@ -211,3 +214,58 @@ def time_step_diffusion(d, Dxx, Dyy, cfl):
             lam += (Dxx * (mes * mes) + Dyy * (mes * mes) + 0. * (mes * mes)) / d.cells.volume[i]
         dt = min(dt, cfl * d.cells.volume[i] / lam)
     return dt
+
+
+def time_step_advdiff(d, u, v, w, Dxx, Dyy, cfl):
+    """``solvers/advecdiff/fvm_utils.py:317-340`` (2D: Dzz = 0)."""
+    dt = 1e6
+    nrm = d.faces.normal
+    for i in range(int(d.nbcells)):
+        lam = 0.
+        for j in range(d.cells.faceid[i][-1]):
+            n = nrm[d.cells.faceid[i][j]]
+            lam += abs(u[i] * n[0] + v[i] * n[1] + w[i] * n[2])
+            mes = np.sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2])
+            lam += (Dxx * (mes * mes) + Dyy * (mes * mes) + 0. * (mes * mes)) / d.cells.volume[i]
+        if lam != 0:
+            dt = min(dt, cfl * d.cells.volume[i] / lam)
+    return dt
+
+
+def p_gradient_2d(d, P_c, P_ghost, P_node, Pbordnode, Pbordface, BCdirichlet, neumannfaces, dirichletfaces):
+    """``compute_P_gradient_2d_diamond``: inner faces, Neumann faces (ghost values) and
+    Dirichlet faces (mirror 2 V_K - P_c, Dirichlet node values)."""
+    F = int(d.nbfaces)
+    px, py = np.zeros(F), np.zeros(F)
+    cid = d.faces.cellid
+    fn = d.faces.nodeid
+    on = d.nodes.oldname
+    f1, f2, f3, f4 = d.faces.f_1, d.faces.f_2, d.faces.f_3, d.faces.f_4
+    air = d.faces.airDiamond
+
+    def nodeval(n):
+        return Pbordnode[n] if on[n] in BCdirichlet else P_node[n]
+
+    def one(i, vi1, vi2, vv1, vv2):
+        px[i] = -1 / (2 * air[i]) * ((vi1 + vv1) * f1[i][1] + (vv1 + vi2) * f2[i][1] + (vi2 + vv2) * f3[i][1]
+                                     + (vv2 + vi1) * f4[i][1])
+        py[i] = 1 / (2 * air[i]) * ((vi1 + vv1) * f1[i][0] + (vv1 + vi2) * f2[i][0] + (vi2 + vv2) * f3[i][0]
+                                    + (vv2 + vi1) * f4[i][0])
+
+    for i in d.innerfaces:
+        one(i, nodeval(fn[i][0]), nodeval(fn[i][1]), P_c[cid[i][0]], P_c[cid[i][1]])
+    for i in neumannfaces:
+        one(i, nodeval(fn[i][0]), nodeval(fn[i][1]), P_c[cid[i][0]], P_ghost[i])
+    for i in dirichletfaces:
+        vv1 = P_c[cid[i][0]]
+        one(i, Pbordnode[fn[i][0]], Pbordnode[fn[i][1]], vv1, 2. * Pbordface[i] - vv1)
+    return px, py
+
+
+def coo_sum_in_order(data, row, col):
+    """{(row, col): value} with duplicates added in insertion order."""
+    out = {}
+    for v, r, c in zip(data.tolist(), row.tolist(), col.tolist()):
+        k = (r, c)
+        out[k] = out[k] + v if k in out else v
+    return out
