@@ -25,6 +25,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mcell-updates/s + %peak HBM GB/s at stated FP64 error; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    """Progress on stderr (the JSON result line is the only stdout output)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 G = 9.81
 CFL = 0.8
 
@@ -97,6 +102,237 @@ def cpu_baseline(d, Z, h0, steps, budget_s):
     return C * done / t / 1e6, done, t
 
 
+def mixed_bytes_per_cell(MF, K_avg, G_avg):
+    """Algorithmic HBM bytes per cell of the two mixed-precision kernels (each array the
+    kernel needs counted once at its width; neighbour records are other cells' own)."""
+    recon = (32 + 16                    # own state (h, hu, hv, Z), centre (x, y)
+             + 4 * K_avg + 4            # cellnid entries + count
+             + G_avg * (4 + 4 + 16) + 4  # ghostnid entries, owner, ghost centre + count
+             + 4 * MF + 4               # neighbour codes + face count
+             + 4 * MF + 8 * MF          # geoA.w (z_f - Z), geoB (rx, ry)
+             + 16 * MF)                 # write face records
+    flux = (32 + 24 + 8                 # state read + write, volume
+            + 4 + 1                     # face count, residual order
+            + 16 * MF + 16 * MF         # own face records, geoA
+            + 4 * MF)                   # neighbour codes
+    return recon, flux
+
+
+def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64):
+    """The same case on the mixed-precision path: throughput, per-kernel times and the
+    per-field error against the float64 run after the same number of steps."""
+    import torch
+    from manapy_amd import _lib
+    from manapy_amd.device import SwMixed, stream_ptr
+    C = int(d.nbcells)
+    s = stream_ptr()
+    p = SwMixed(m)
+    dv = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device='cuda')
+    th, tz, tZ = dv(h0), dv(np.zeros(C)), dv(Z)
+    _lib.call('mfv_sw_pack_mixed', m.ptr, p.ptr, th.data_ptr(), tz.data_ptr(), tz.data_ptr(), tZ.data_ptr(),
+              zgrad.data_ptr(), 2, s)
+    _lib.call('mfv_sw_conservation', m.ptr, p.ptr, p.partial.data_ptr(), p.partial.numel(), p.sums.data_ptr(), s)
+    mass0 = float(p.sums[0])
+    dt0 = dv(np.zeros(1))
+    ring = dv(np.zeros(4))
+    _lib.call('mfv_time_step_sw', m.ptr, th.data_ptr(), tz.data_ptr(), tz.data_ptr(), CFL, G, 0., 0., dt0.data_ptr(),
+              None, s)
+    _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
+    k = [0]
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record()
+        _lib.call('mfv_sw_recon_mixed', m.ptr, p.ptr, 2, s)
+        if ev is not None:
+            ev[1].record()
+        _lib.call('mfv_sw_flux_mixed', m.ptr, p.ptr, G, CFL, ring.data_ptr(), k[0], s)
+        if ev is not None:
+            ev[2].record()
+        k[0] += 1
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t1
+    oh, ohu, ohv = (torch.empty(C, dtype=torch.float64, device='cuda') for _ in range(3))
+    _lib.call('mfv_sw_unpack_mixed', m.ptr, p.ptr, oh.data_ptr(), ohu.data_ptr(), ohv.data_ptr(), s)
+    _lib.call('mfv_sw_conservation', m.ptr, p.ptr, p.partial.data_ptr(), p.partial.numel(), p.sums.data_ptr(), s)
+    mass1 = float(p.sums[0])
+    vol = d.cells.volume
+    errs = {}
+    for nm, a, b in (('h', oh, fp64_state[0]), ('hu', ohu, fp64_state[1]), ('hv', ohv, fp64_state[2])):
+        a = a.cpu().numpy()
+        errs[nm] = {"rel_L1": float(np.sum(np.abs(a - b) * vol) / np.sum(np.abs(b) * vol)),
+                    "rel_Linf": float(np.abs(a - b).max() / np.abs(b).max())}
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for e in evs:
+        step(e)
+    torch.cuda.synchronize()
+    t_rec = np.array([a.elapsed_time(b) for a, b, _ in evs]) * 1e-3
+    t_flx = np.array([b.elapsed_time(c) for _, b, c in evs]) * 1e-3
+    K_avg = float(d.cells.cellnid[:, -1].mean())
+    G_avg = float(d.cells.ghostnid[:, -1].mean())
+    br, bf = mixed_bytes_per_cell(int(d.cells.faceid.shape[1] - 1), K_avg, G_avg)
+    mcell = C * steps / t_loop / 1e6
+    dom = 'k_sw_flux32' if t_flx.mean() >= t_rec.mean() else 'k_sw_recon32'
+    tk, bk = (t_flx, bf) if dom == 'k_sw_flux32' else (t_rec, br)
+    return {
+        "value": round(mcell, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop / steps * 1e3, 5),
+        "speedup_vs_fp64": round(mcell / mcell_fp64, 3), "dtype": "f32 flux/gradient arithmetic, f64 state/dt/sums",
+        "steps_compared": warmup + steps, "error_vs_fp64": errs,
+        "mass_rel_drift": (mass1 - mass0) / mass0,
+        "kernels": {"k_sw_recon32": {"avg_us": round(float(t_rec.mean()) * 1e6, 2), "bytes_per_cell": round(br, 1),
+                                     "GB/s": round(br * C / t_rec.mean() / 1e9, 1)},
+                    "k_sw_flux32": {"avg_us": round(float(t_flx.mean()) * 1e6, 2), "bytes_per_cell": round(bf, 1),
+                                    "GB/s": round(bf * C / t_flx.mean() / 1e9, 1)}},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(bk * C / tk.mean() / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(bk * C / tk.mean() / 1e9 / HBM_PEAK_GBS, 4)},
+        "step_hbm_frac_214B": round(214.0 * C * steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+
+def run_ensemble(n_inst, steps, seed=0, n_check=4):
+    """Batched ensemble: n_inst shallow-water instances (sizes log-uniform in [5k, 200k]
+    triangles, aspect ratio 1-4, 1-10 bumps, random dam) advanced together by a captured graph
+    with the autotuned block size; error of sampled instances against their float64 runs."""
+    import torch
+    from manapy_amd import _lib
+    from manapy_amd.ddm import Domain
+    from manapy_amd.device import DeviceMesh, SwPacked, stream_ptr
+    from manapy_amd.ensemble import SwEnsemble, random_instance, instance_state
+    if steps % 3:
+        raise ValueError("ensemble steps must be a multiple of 3")
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    sizes = np.exp(rng.uniform(np.log(5e3), np.log(2e5), n_inst))
+    doms, states = [], []
+    for k, c in enumerate(sizes):
+        if k % 32 == 0:
+            log("ensemble: building instance %d / %d" % (k, n_inst))
+        el, nd, lx, ly, r = random_instance(rng, c)
+        d = Domain(2, mesh=(el, nd))
+        doms.append(d)
+        states.append(instance_state(d, lx, ly, r))
+    t_build = time.perf_counter() - t0
+    log("ensemble: uploading %d instances" % n_inst)
+    e = SwEnsemble(doms, states)
+    tune = e.autotune(steps=6)
+    log("ensemble: block %d" % tune['block'])
+    e.step(3)
+    torch.cuda.synchronize()
+    g = e.graph(steps)
+    t1 = time.perf_counter()
+    e.run_graph(steps, chunk=steps)
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t1
+    cells = int(sum(int(d.nbcells) for d in doms))
+    mcell = cells * steps / t_loop / 1e6
+    # float64 reference for a sample of instances (same initial state, same 3 + steps steps)
+    errs = []
+    pick = np.linspace(0, n_inst - 1, n_check).astype(int)
+    s = stream_ptr()
+    for i in pick:
+        d = doms[i]
+        h0, hu0, hv0, Z = states[i]
+        C, F = int(d.nbcells), int(d.nbfaces)
+        m = DeviceMesh(d)
+        pk = SwPacked(m)
+        dv = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device='cuda')
+        tZ, zgh, zg = dv(Z), dv(np.zeros(F)), dv(np.zeros((3, C)))
+        bf = torch.as_tensor(d.boundaryfaces, device='cuda')
+        _lib.call('mfv_ghost_value', m.ptr, 0, tZ.data_ptr(), zgh.data_ptr(), bf.data_ptr(), len(bf), s)
+        _lib.call('mfv_cell_gradient', m.ptr, tZ.data_ptr(), zgh.data_ptr(), zg[0].data_ptr(), zg[1].data_ptr(),
+                  zg[2].data_ptr(), s)
+        th, thu, thv = dv(h0), dv(hu0), dv(hv0)
+        _lib.call('mfv_sw_pack', m.ptr, pk.ptr, th.data_ptr(), thu.data_ptr(), thv.data_ptr(), tZ.data_ptr(),
+                  zg.data_ptr(), 2, s)
+        dt0, ring = dv(np.zeros(1)), dv(np.zeros(4))
+        _lib.call('mfv_time_step_sw', m.ptr, th.data_ptr(), thu.data_ptr(), thv.data_ptr(), CFL, G, 0., 0.,
+                  dt0.data_ptr(), None, s)
+        _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
+        for k in range(3 + steps):
+            _lib.call('mfv_sw_recon', m.ptr, pk.ptr, 2, s)
+            _lib.call('mfv_sw_flux_packed', m.ptr, pk.ptr, G, CFL, ring.data_ptr(), k, s)
+        _lib.call('mfv_sw_unpack', m.ptr, pk.ptr, th.data_ptr(), thu.data_ptr(), thv.data_ptr(), s)
+        ref = [x.cpu().numpy() for x in (th, thu, thv)]
+        got = e.instance_state(i)
+        vol = d.cells.volume
+        errs.append({nm: float(np.sum(np.abs(a - b) * vol) / np.sum(np.abs(b) * vol))
+                     for nm, a, b in zip(('h', 'hu', 'hv'), got, ref)})
+    sums = e.conservation()
+    return {
+        "value": round(mcell, 3), "unit": "Mcell-updates/s", "instances": n_inst, "cells": cells,
+        "steps": steps, "ms_per_step": round(t_loop / steps * 1e3, 4), "graph": "one captured graph of all steps",
+        "block": int(tune['block']), "autotune_ms_per_step": tune['ms_per_step'],
+        "step_hbm_frac_214B": round(214.0 * cells * steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
+        "error_vs_fp64_sampled": errs, "max_rel_L1": max(max(x.values()) for x in errs),
+        "mass_totals_finite": bool(np.all(np.isfinite(sums))), "build_s": round(t_build, 1),
+    }
+
+
+def run_serving(n_req, seed=1, nsteps=20):
+    """Implicit advection-diffusion requests (meshes log-uniform in [10k, 1M] cells, D
+    log-uniform in [1e-3, 1e-1], 20 steps each, FP64 refinement to 1e-10) served with
+    continuous batching: first back to back (capacity), then with Poisson arrivals at 80 % of
+    that capacity; p50 / p99 latency from arrival to result."""
+    from manapy_amd.ddm import Domain
+    from manapy_amd.meshgen import jittered_triangles
+    from manapy_amd.serving import AdvDiffRequest, ServingEngine, latency_stats
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    specs = []
+    for k in range(n_req):
+        if k % 8 == 0:
+            log("serving: building request mesh %d / %d" % (k, n_req))
+        cells = np.exp(rng.uniform(np.log(1e4), np.log(1e6)))
+        n = max(8, int(round(np.sqrt(cells / 2.))))
+        el, nd = jittered_triangles(n, n, 1.0, 1.0, 0.1, int(rng.integers(1 << 30)))
+        d = Domain(2, mesh=(el, nd))
+        c = d.cells.center
+        x0, y0 = rng.uniform(.2, .5, 2)
+        u0 = np.exp(-((c[:, 0] - x0) ** 2 + (c[:, 1] - y0) ** 2) / .01)
+        ang = rng.uniform(0, 2 * np.pi)
+        D = float(np.exp(rng.uniform(np.log(1e-3), np.log(1e-1))))
+        specs.append((d, u0, (np.cos(ang), np.sin(ang)), D))
+    t_build = time.perf_counter() - t0
+    total_cells = sum(int(sp[0].nbcells) for sp in specs)
+
+    def make(arrivals):
+        return [AdvDiffRequest(k, sp[0], sp[1], velocity=sp[2], D=sp[3], nsteps=nsteps, arrival=a).prepare()
+                for k, (sp, a) in enumerate(zip(specs, arrivals))]
+
+    budget = int(3.0e9)
+    eng = ServingEngine(max_cells=max(1 << 22, total_cells), max_inst=64, hbm_budget=budget)
+    t1 = time.perf_counter()
+    log("serving: capacity run")
+    done = eng.run(make(np.zeros(n_req)))
+    makespan = time.perf_counter() - t1
+    log("serving: Poisson run")
+    capacity = n_req / makespan
+    rate = 0.8 * capacity
+    arrivals = np.cumsum(rng.exponential(1. / rate, n_req))
+    eng = ServingEngine(max_cells=max(1 << 22, total_cells), max_inst=64, hbm_budget=budget)
+    t2 = time.perf_counter()
+    done = eng.run(make(arrivals))
+    span = time.perf_counter() - t2
+    lat = latency_stats(done)
+    return {
+        "requests": n_req, "steps_per_request": nsteps, "cells_total": total_cells,
+        "capacity_req_per_s": round(capacity, 3), "arrival_rate_req_per_s": round(rate, 3),
+        "throughput_req_per_s": round(n_req / span, 3),
+        "throughput_Mcell_steps_per_s": round(total_cells * nsteps / span / 1e6, 3),
+        "latency_p50_s": round(lat['p50_s'], 4), "latency_p99_s": round(lat['p99_s'], 4),
+        "latency_mean_s": round(lat['mean_s'], 4), "max_concurrent": eng.max_active,
+        "hbm_budget_bytes": budget, "krylov_iterations_mean": float(np.mean([r.iterations for r in done])),
+        "solve_tol": 1e-10, "mesh_build_s": round(t_build, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -108,6 +344,10 @@ def main():
     ap.add_argument('--cpu-budget', type=float, default=20.0)
     ap.add_argument('--path', choices=('packed', 'unpacked'), default='packed',
                     help='packed: per-cell records + per-face MUSCL values (default); unpacked: SoA kernels')
+    ap.add_argument('--no-mixed', action='store_true', help='skip the mixed-precision run of the same case')
+    ap.add_argument('--ensemble', type=int, default=256, help='instances of the ensemble run (0: skip)')
+    ap.add_argument('--ensemble-steps', type=int, default=201)
+    ap.add_argument('--serving', type=int, default=32, help='requests of the serving run (0: skip)')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run a device-to-device copy of 1 GiB five times (calibrates HBM counters)')
     args = ap.parse_args()
@@ -200,6 +440,15 @@ def main():
         step()
     torch.cuda.synchronize()
     t_loop = time.perf_counter() - t1
+    n_fp64_steps = step_no[0]
+    h64 = torch.empty(C, dtype=torch.float64, device='cuda')
+    hu64, hv64 = torch.empty_like(h64), torch.empty_like(h64)
+    if args.path == 'packed':
+        _lib.call('mfv_sw_unpack', m.ptr, pk.ptr, h64.data_ptr(), hu64.data_ptr(), hv64.data_ptr(), s)
+    else:
+        last = S[n_fp64_steps % 2]
+        h64.copy_(last[0]), hu64.copy_(last[1]), hv64.copy_(last[2])
+    fp64_state = [x.cpu().numpy() for x in (h64, hu64, hv64)]
 
     # per-kernel durations: HIP events on the launch stream over a second timed pass
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
@@ -249,6 +498,16 @@ def main():
         "step_hbm_frac_356B": round(step_bytes_survey * C * args.steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
         "mesh_build_s": round(t_mesh, 2),
     }
+    log("fp64: %.1f Mcell-updates/s" % mcell)
+    if not args.no_mixed:
+        out["mixed"] = run_mixed(d, m, Z, h0, zgrad, args.warmup, args.steps, fp64_state, mcell)
+        log("mixed: %.1f Mcell-updates/s" % out["mixed"]["value"])
+    if args.ensemble:
+        out["ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps)
+        log("ensemble: %.1f Mcell-updates/s" % out["ensemble"]["value"])
+    if args.serving:
+        out["serving"] = run_serving(args.serving)
+        log("serving: p50 %.3f s, p99 %.3f s" % (out["serving"]["latency_p50_s"], out["serving"]["latency_p99_s"]))
     if not args.no_cpu:
         v, n, t = cpu_baseline(d, Z, h0, args.cpu_steps, args.cpu_budget)
         out["cpu_baseline"] = {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",

@@ -189,20 +189,24 @@ int mfv_sw_recon(const mfv_mesh *m, const mfv_sw_packed *p, int order, void *str
 int mfv_sw_flux_packed(const mfv_mesh *m, const mfv_sw_packed *p, double grav, double cfl,
                        double *dt_ring, int step, void *stream);
 
-/* Mixed-precision shallow-water path: float32 records, gradients and fluxes, Kahan-compensated
- * float32 residual sums, float64 state update and dt; same scheme as the float64 paths. */
+/* Mixed-precision shallow-water path: the scheme of the float64 paths with float64 cell
+ * states, float32 reconstruction, face records and flux arithmetic on deviations from the cell
+ * states, TwoSum-compensated float32 residual gathers and float64 updates, dt and totals.
+ * The face record of slot s of cell i is (a, hu_f, hv_f, w): a = face depth minus the depth of
+ * the face's left cell, w = that depth (i is the left cell) or h_i minus it (i is the right cell). */
 typedef struct mfv_sw_mixed {
     int32_t nb_cells;       /* C */
     int32_t max_cell_faces; /* MF (3 or 4) */
-    double *state;          /* [C][4]     h, hu, hv, vol (float64) */
-    float *cellrec;         /* [C][8]     cx, cy, eta, hu, hv, h, Z, 0 */
-    float *facerec;         /* [C][MF][4] eta_f, hu_f, hv_f, z_f */
-    float *cfrec;           /* [C][MF][4] rx, ry, z_f, 0 */
-    const float *cfgeo;     /* [C][MF][4] n0, n1, mesure, 0 */
+    double *state;          /* [C][4]     h, hu, hv, Z (float64) */
+    const double *vol;      /* [C]        cell volumes (float64) */
+    float *facerec;         /* [C][MF][4] face records (mfv_sw_recon_mixed) */
+    float *geoA;            /* [C][MF][4] n0, n1, zf_R - zf_L, zf_own - Z_i (mfv_sw_pack_mixed) */
+    float *geoB;            /* [C][MF][2] face centre - cell centre (mfv_sw_pack_mixed) */
     const int32_t *nbr;     /* [MF][C]    as mfv_sw_packed */
     const uint8_t *order;   /* [C]        as mfv_sw_packed */
 } mfv_sw_mixed;
 
+/* Loads (h, hu, hv, Z) and builds geoA/geoB from the mesh and the bathymetry gradient zgrad [3][C]. */
 int mfv_sw_pack_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, const double *h, const double *hu,
                       const double *hv, const double *Z, const double *zgrad, int order, void *stream);
 int mfv_sw_unpack_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, double *h, double *hu, double *hv,

@@ -41,8 +41,8 @@ class MfvSwPacked(ctypes.Structure):
 
 class MfvSwMixed(ctypes.Structure):
     """Mirror of ``struct mfv_sw_mixed`` (include/mfv.h)."""
-    _fields_ = [('nb_cells', ctypes.c_int32), ('max_cell_faces', ctypes.c_int32), ('state', P), ('cellrec', P),
-                ('facerec', P), ('cfrec', P), ('cfgeo', P), ('nbr', P), ('order', P)]
+    _fields_ = [('nb_cells', ctypes.c_int32), ('max_cell_faces', ctypes.c_int32), ('state', P), ('vol', P),
+                ('facerec', P), ('geoA', P), ('geoB', P), ('nbr', P), ('order', P)]
 
 
 class MfvDiamond(ctypes.Structure):

@@ -462,7 +462,10 @@ def _time_step_diff(with_velocity):
         import torch
         m = _m(domain)
         A = _Args(m.device)
-        uvw = [A.dev(x).data_ptr() for x in (u, v, w)] if with_velocity else [None, None, None]
+        # the device copies must outlive the call: the caching allocator would hand a freed
+        # temporary's memory to the next allocation before the kernel has read it
+        ts = [A.dev(x) for x in (u, v, w)] if with_velocity else []
+        uvw = [t.data_ptr() for t in ts] if with_velocity else [None, None, None]
         dt = torch.zeros(1, dtype=torch.float64, device=m.device)
         h = ctypes.c_double(0.)
         _lib.call('mfv_time_step_diffusion', m.ptr, *uvw, float(cfl), float(Dxx), float(Dyy), float(Dzz),

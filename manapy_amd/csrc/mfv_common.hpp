@@ -33,18 +33,12 @@ inline unsigned grid_for(int n, int block = kBlock) { return (unsigned)((n + blo
 __device__ __forceinline__ double pmin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double pmax(double a, double b) { return (b > a) ? b : a; }
 
-// Positive doubles order like their bit patterns, so a min over dt values is an unsigned
-// 64-bit atomicMin. NaN patterns sort above +inf and are ignored, as Python's min(dt, nan)
-// keeps dt. The result does not depend on the order of the atomics.
-__device__ __forceinline__ void atomic_min_positive(double *addr, double v)
-{
-    if (!(v >= 0.0)) return;
-    unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-    atomicMin(reinterpret_cast<unsigned long long *>(addr), bits);
-}
-
-// Wave64 min of non-negative doubles (NaN ignored) followed by one atomic per wave.
-__device__ __forceinline__ void wave_min_to(double *addr, double v)
+// dt reductions. Every dt-producing kernel reduces its wavefront in registers and lane 0
+// stores the wave's minimum (non-negative values; NaN and negative candidates ignored) at
+// part[global wave index] -- no atomics: thousands of same-address atomics queue at the L2
+// for longer than the kernel's own work. dt_finish then folds the partials into the target
+// (out[0] = min(out[0], min part)) in one block, deterministically.
+__device__ __forceinline__ void wave_min_part(double *__restrict__ part, double v)
 {
     double m = (v >= 0.0) ? v : __longlong_as_double(0x7ff0000000000000LL);
 #pragma unroll
@@ -52,8 +46,19 @@ __device__ __forceinline__ void wave_min_to(double *addr, double v)
         double o = __shfl_xor(m, off, 64);
         m = (o < m) ? o : m;
     }
-    if ((threadIdx.x & 63) == 0) atomic_min_positive(addr, m);
+    if ((threadIdx.x & 63) == 0) part[((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6] = m;
 }
+
+// Waves of a launch of grid_for(n, block) blocks.
+inline size_t waves_for(int n, int block = kBlock) { return (size_t)grid_for(n, block) * (size_t)(block / 64); }
+
+// Library-owned scratch for the per-wave partials, one buffer per (current device, stream),
+// grown on demand; work on one stream runs in order, so reuse does not race. NULL if the
+// allocation fails (error set).
+double *dt_scratch(size_t nwaves, hipStream_t st);
+
+// out[0] = min(out[0], part[0..n)) on the stream (one block).
+int dt_finish(const double *part, size_t n, double *out, hipStream_t st);
 
 // dt ring of the fused steps: slot k%3 is this step's dt, (k+1)%3 receives the next dt,
 // (k+2)%3 is reset for the step after (nothing reads or writes it during step k); slot 3

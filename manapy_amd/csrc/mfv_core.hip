@@ -6,6 +6,9 @@
 // deterministic and independent of the launch shape.
 #include <stdarg.h>
 
+#include <mutex>
+#include <vector>
+
 #include "mfv_common.hpp"
 
 namespace mfv {
@@ -288,6 +291,82 @@ extern "C" int mfv_dt_reset(double *dt, void *stream)
     return check_launch("k_dt_reset");
 }
 
+namespace mfv {
+
+// Per-wave dt partial minima live in one buffer per (device, stream): work queued on one
+// stream runs in order, so the buffer is reused without a race, and calls on different
+// streams never share one. A buffer only grows (hipFree waits for the device).
+struct DtScratch {
+    int dev;
+    hipStream_t stream;
+    double *ptr;
+    size_t cap;
+};
+static std::mutex g_dt_mu;
+static std::vector<DtScratch> g_dt_scr;
+
+double *dt_scratch(size_t nwaves, hipStream_t st)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("dt scratch: no current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_dt_mu);
+    DtScratch *e = nullptr;
+    for (auto &x : g_dt_scr)
+        if (x.dev == dev && x.stream == st) e = &x;
+    if (!e) {
+        g_dt_scr.push_back(DtScratch{dev, st, nullptr, 0});
+        e = &g_dt_scr.back();
+    }
+    if (e->cap < nwaves) {
+        const size_t cap = e->cap * 2 > nwaves ? e->cap * 2 : nwaves;
+        if (e->ptr) (void)hipFree(e->ptr);
+        e->ptr = nullptr;
+        e->cap = 0;
+        if (hipMalloc(reinterpret_cast<void **>(&e->ptr), cap * sizeof(double)) != hipSuccess) {
+            set_error("dt scratch: hipMalloc of %zu doubles failed", cap);
+            e->ptr = nullptr;
+            return nullptr;
+        }
+        e->cap = cap;
+    }
+    return e->ptr;
+}
+
+__global__ void __launch_bounds__(1024) k_dt_finish(const double *__restrict__ part, size_t n, double *__restrict__ out)
+{
+    __shared__ double sh[1024];
+    double m = __longlong_as_double(0x7ff0000000000000LL);
+    for (size_t j = threadIdx.x; j < n; j += 1024) {
+        const double v = part[j];
+        m = (v < m) ? v : m;
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double o = sh[threadIdx.x + w];
+            sh[threadIdx.x] = (o < sh[threadIdx.x]) ? o : sh[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double r = sh[0];
+        if (r < out[0]) out[0] = r;
+    }
+}
+
+int dt_finish(const double *part, size_t n, double *out, hipStream_t st)
+{
+    if (int rc = check_launch("dt kernel")) return rc;
+    hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(1024), 0, st, part, n, out);
+    return check_launch("k_dt_finish");
+}
+
+}  // namespace mfv
+
 int mfv::finish_dt(double *dt_dev, double *dt_host, void *stream)
 {
     if (int rc = check_launch("dt kernel")) return rc;
@@ -320,11 +399,11 @@ __device__ __forceinline__ double adv_cell_dt(const Mesh &m, int i, double u, do
 __global__ void __launch_bounds__(kBlock) k_adv_time_step(Mesh m, const double *__restrict__ u,
                                                            const double *__restrict__ v,
                                                            const double *__restrict__ w, double cfl,
-                                                           double *dt)
+                                                           double *__restrict__ dtpart)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d = (i < m.C) ? adv_cell_dt(m, i, u[i], v[i], w[i], cfl) : 1e300;
-    wave_min_to(dt, d);
+    wave_min_part(dtpart, d);
 }
 
 extern "C" int mfv_time_step(const mfv_mesh *m, const double *u, const double *v, const double *w, double cfl,
@@ -333,9 +412,14 @@ extern "C" int mfv_time_step(const mfv_mesh *m, const double *u, const double *v
     if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
     MFV_REQUIRE(u && v && w && dt_dev, "NULL array");
     if (int rc = mfv_dt_reset(dt_dev, stream)) return rc;
-    if (m->nb_cells)
+    if (m->nb_cells) {
+        const size_t nw = waves_for(m->nb_cells);
+        double *part = dt_scratch(nw, as_stream(stream));
+        if (!part) return MFV_ENOMEM;
         hipLaunchKernelGGL(k_adv_time_step, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
-                           to_kernel(m), u, v, w, cfl, dt_dev);
+                           to_kernel(m), u, v, w, cfl, part);
+        if (int rc = dt_finish(part, nw, dt_dev, as_stream(stream))) return rc;
+    }
     return finish_dt(dt_dev, dt_host, stream);
 }
 
@@ -518,7 +602,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_step(Mesh m, const double *__res
                                                       const double *__restrict__ vf,
                                                       const double *__restrict__ wf, double cfl, double om,
                                                       const double *__restrict__ scratch,
-                                                      DtRing ring)
+                                                      DtRing ring, double *__restrict__ dtpart)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     ring_advance(ring);
@@ -532,7 +616,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_step(Mesh m, const double *__res
         w_out[i] = w_in[i] + dt * ((rez + 0.) / m.cell_volume[i] + 0.);
         dloc = adv_cell_dt(m, i, uc[i], vc[i], wc[i], cfl);
     }
-    wave_min_to(ring.next, dloc);
+    wave_min_part(dtpart, dloc);
 }
 
 extern "C" int mfv_advection_step(const mfv_mesh *m, const double *w_in, double *w_out, const double *u_c,
@@ -555,9 +639,13 @@ extern "C" int mfv_advection_step(const mfv_mesh *m, const double *w_in, double 
         hipLaunchKernelGGL(kg, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, st, k, w_in, scratch);
         if (int rc = check_launch("k_adv_grad_limit")) return rc;
     }
+    const size_t nw = waves_for(m->nb_cells);
+    double *part = dt_scratch(nw, as_stream(stream));
+    if (!part) return MFV_ENOMEM;
     auto ks = (m->dim == 2) ? ((order == 2) ? k_adv_step<2, 2> : k_adv_step<2, 1>)
                             : ((order == 2) ? k_adv_step<3, 2> : k_adv_step<3, 1>);
     hipLaunchKernelGGL(ks, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, st, k, w_in, w_out, u_c, v_c, w_c,
-                       u_face, v_face, w_face, cfl, (double)(order - 1), scratch, r);
-    return check_launch("k_adv_step");
+                       u_face, v_face, w_face, cfl, (double)(order - 1), scratch, r, part);
+    if (int rc = check_launch("k_adv_step")) return rc;
+    return dt_finish(part, nw, r.next, st);
 }

@@ -5,8 +5,7 @@
 // Every output element is produced by one thread that gathers its terms in the order the
 // reference adds them (faces in the reference loop order, nodes in face order, cells and
 // ghost centres in node-list order), so sums are bit-identical to the reference's
-// sequential loops and do not depend on the launch shape. Nothing uses atomics except the
-// dt minimum (order-independent).
+// sequential loops and do not depend on the launch shape. Nothing uses atomics.
 #include "mfv_common.hpp"
 
 namespace mfv {
@@ -396,7 +395,7 @@ __global__ void __launch_bounds__(kBlock) k_time_step_diff(Mesh m, const double 
                                                             const double *__restrict__ v,
                                                             const double *__restrict__ w, double cfl,
                                                             double Dxx, double Dyy, double Dzz,
-                                                            double *__restrict__ dt_out)
+                                                            double *__restrict__ dtpart)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double dt = 1e6;
@@ -413,7 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_time_step_diff(Mesh m, const double 
         }
         if (!u || lam != 0) dt = pmin(dt, cfl * vol / lam);
     }
-    wave_min_to(dt_out, dt);
+    wave_min_part(dtpart, dt);
 }
 
 }  // namespace mfv
@@ -504,8 +503,13 @@ extern "C" int mfv_time_step_diffusion(const mfv_mesh *m, const double *u, const
     MFV_REQUIRE(dt_dev, "NULL dt_dev");
     MFV_REQUIRE((u && v && w) || (!u && !v && !w), "u, v, w must all be given or all be NULL");
     if (int rc = mfv_dt_reset(dt_dev, stream)) return rc;
-    if (m->nb_cells)
+    if (m->nb_cells) {
+        const size_t nw = waves_for(m->nb_cells);
+        double *part = dt_scratch(nw, as_stream(stream));
+        if (!part) return MFV_ENOMEM;
         hipLaunchKernelGGL(k_time_step_diff, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
-                           to_kernel(m), u, v, w, cfl, Dxx, Dyy, Dzz, dt_dev);
+                           to_kernel(m), u, v, w, cfl, Dxx, Dyy, Dzz, part);
+        if (int rc = dt_finish(part, nw, dt_dev, as_stream(stream))) return rc;
+    }
     return finish_dt(dt_dev, dt_host, stream);
 }

@@ -124,7 +124,7 @@ __device__ __forceinline__ double euler_cell_dt(const Mesh &m, int i, double rho
 }
 
 __global__ void __launch_bounds__(kBlock) k_euler_dt(Mesh m, const double *__restrict__ U, double cfl, double gamma,
-                                                      double *dt)
+                                                      double *__restrict__ dtpart)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d = 1e300;
@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(kBlock) k_euler_dt(Mesh m, const double *__res
         const size_t C = m.C;
         d = euler_cell_dt(m, i, U[i], U[C + i], U[2 * C + i], U[3 * C + i], U[4 * C + i], cfl, gamma);
     }
-    wave_min_to(dt, d);
+    wave_min_part(dtpart, d);
 }
 
 // Residual of cell i, faces in increasing id (the single all-faces loop of
@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_euler_update(int C, double *__restri
 
 __global__ void __launch_bounds__(kBlock) k_euler_step(Mesh m, const double *__restrict__ Uin,
                                                         double *__restrict__ Uout, double gamma, double cfl,
-                                                        int quirk, DtRing ring)
+                                                        int quirk, DtRing ring, double *__restrict__ dtpart)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     ring_advance(ring);
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(kBlock) k_euler_step(Mesh m, const double *__r
         for (int v = 0; v < 6; v++) Uout[v * C + i] = u[v];
         d = euler_cell_dt(m, i, u[0], u[1], u[2], u[3], u[4], cfl, gamma);
     }
-    wave_min_to(ring.next, d);
+    wave_min_part(dtpart, d);
 }
 
 }  // namespace
@@ -253,9 +253,14 @@ extern "C" int mfv_euler3d_time_step(const mfv_mesh *m, const double *U, double 
     MFV_REQUIRE(m->dim == 3 && m->max_cell_faces >= 4, "Euler 3D time step needs tetrahedra");
     MFV_REQUIRE(U && dt_dev, "NULL array");
     if (int rc = mfv_dt_reset(dt_dev, stream)) return rc;
-    if (m->nb_cells)
+    if (m->nb_cells) {
+        const size_t nw = waves_for(m->nb_cells);
+        double *part = dt_scratch(nw, as_stream(stream));
+        if (!part) return MFV_ENOMEM;
         hipLaunchKernelGGL(k_euler_dt, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m),
-                           U, cfl, gamma, dt_dev);
+                           U, cfl, gamma, part);
+        if (int rc = dt_finish(part, nw, dt_dev, as_stream(stream))) return rc;
+    }
     if (int rc = check_launch("k_euler_dt")) return rc;
     if (dt_host) {
         hipError_t e = hipMemcpyAsync(dt_host, dt_dev, sizeof(double), hipMemcpyDeviceToHost, as_stream(stream));
@@ -299,7 +304,11 @@ extern "C" int mfv_euler3d_step_fused(const mfv_mesh *m, const double *U_in, dou
     if (m->nb_cells == 0) return MFV_OK;
     MFV_REQUIRE(U_in && U_out && U_in != U_out && dt_ring_p, "NULL or aliased array");
     DtRing r = dt_ring(dt_ring_p, step);
+    const size_t nw = waves_for(m->nb_cells);
+    double *part = dt_scratch(nw, as_stream(stream));
+    if (!part) return MFV_ENOMEM;
     hipLaunchKernelGGL(k_euler_step, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m),
-                       U_in, U_out, gamma, cfl, quirk, r);
-    return check_launch("k_euler_step");
+                       U_in, U_out, gamma, cfl, quirk, r, part);
+    if (int rc = check_launch("k_euler_step")) return rc;
+    return dt_finish(part, nw, r.next, as_stream(stream));
 }

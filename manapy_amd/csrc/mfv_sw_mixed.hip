@@ -1,0 +1,753 @@
+// libmfv mixed-precision shallow-water path and its ensembles.
+//
+// Same scheme as the float64 paths (MUSCL on eta, hu, hv with LSQ gradients and the
+// Barth-Jespersen limiter, hydrostatic reconstruction, Rusanov flux, centred bathymetry term,
+// mirrored walls), arranged so that float32 arithmetic only ever sees small quantities:
+//   * the cell states (h, hu, hv, Z) stay in float64; the reconstruction takes float64
+//     differences to its neighbours and runs the least-squares gradient and limiter in float32;
+//   * a face record holds the face depth as a deviation from the depth of the face's left cell
+//     (the face's canonical base), the face momenta, and that base (left side) or the deviation
+//     of the own cell from it (right side): both cells of a face evaluate the face from the
+//     same float32 numbers, so the mass flux is the same bits on both sides;
+//   * pressures enter the momentum fluxes as deviations from the own cell's hydrostatic
+//     pressure 1/2 g h_K^2 (whose contribution sum_f n_f vanishes on a closed cell);
+//   * the residual gathers add the float32 face terms with TwoSum compensation (exact error
+//     terms, any order of magnitudes) and finish in float64, so the float64 update conserves
+//     mass to round-off.
+// dt and the conservation totals are float64.
+#include "mfv_common.hpp"
+
+namespace {
+
+using namespace mfv;
+
+// Python max(a, b) on floats.
+__device__ __forceinline__ float pmaxf(float a, float b) { return (b > a) ? b : a; }
+
+// Knuth's TwoSum accumulation: s + c carries the sum with the rounding errors in c.
+__device__ __forceinline__ void twosum_add(float &s, float &c, float x)
+{
+    const float t = s + x;
+    const float bp = t - s;
+    const float e = (s - (t - bp)) + (x - bp);
+    s = t;
+    c += e;
+}
+
+// One face in its canonical orientation L -> R (normal n out of L, |n| = mes).
+//   hb          depth of the L cell (float32)
+//   aL, aR      face depths (eta_f - z_f) minus hb
+//   dz          z_f(R) - z_f(L)
+//   huL..hvR    face momenta
+//   dzo         z_f - Z of the own side
+//   shift       hb - h_own (0 when the own cell is L)
+// Outputs: mass and momentum fluxes L -> R (momentum with pressures relative to the own cell)
+// and the own side's bathymetry term.
+__device__ __forceinline__ void face_dev(float hb, float aL, float aR, float dz, float huL, float hvL, float huR,
+                                         float hvR, float dzo, float n0, float n1, float mes, float g, float shift,
+                                         bool own_is_L, float &F0, float &F1, float &F2, float &bo)
+{
+    float hL = hb + aL, hR = hb + aR;
+    const float cL = pmaxf(0.f, dz), cR = pmaxf(0.f, -dz);
+    const bool wetL = hL > 0.f, wetR = hR > 0.f;
+    float eL = aL - cL, eR = aR - cR;  // hydrostatic depths minus hb
+    float hsL = hb + eL, hsR = hb + eR;
+    if (!wetL) hL = 0.f;
+    if (!(hsL > 0.f)) {
+        hsL = 0.f;
+        eL = -hb;
+    }
+    if (!wetR) hR = 0.f;
+    if (!(hsR > 0.f)) {
+        hsR = 0.f;
+        eR = -hb;
+    }
+    float uL = 0.f, vL = 0.f, uR = 0.f, vR = 0.f;
+    if (hL > 1e-12f) {
+        uL = huL / hL;
+        vL = hvL / hL;
+    }
+    if (hR > 1e-12f) {
+        uR = huR / hR;
+        vR = hvR / hR;
+    }
+    const float qL = uL * n0 + vL * n1, qR = uR * n0 + vR * n1;
+    const float sL = fabsf(qL) / mes + sqrtf(g * hsL), sR = fabsf(qR) / mes + sqrtf(g * hsR);
+    const float s = pmaxf(sL, sR) * mes;
+    const float ho = hb - shift;
+    const float dL = eL + shift, dR = eR + shift;  // hs - h_own
+    const float PL = 0.5f * g * dL * (2.f * ho + dL);
+    const float PR = 0.5f * g * dR * (2.f * ho + dR);
+    F0 = 0.5f * (hsL * qL + hsR * qR) - 0.5f * s * (eR - eL);
+    F1 = 0.5f * (hsL * uL * qL + hsR * uR * qR) + 0.5f * (PL + PR) * n0 - 0.5f * s * (hsR * uR - hsL * uL);
+    F2 = 0.5f * (hsL * vL * qL + hsR * vR * qR) + 0.5f * (PL + PR) * n1 - 0.5f * s * (hsR * vR - hsL * vL);
+    if (own_is_L) {
+        const float dh = hL - hsL;
+        bo = 0.5f * g * dh * (hL + hsL) + g * (hL + ho) * 0.5f * dzo;
+    } else {
+        const float dh = hR - hsR;
+        bo = 0.5f * g * dh * (hR + hsR) + g * (hR + ho) * 0.5f * dzo;
+    }
+}
+
+// Static face geometry and the constant bathymetry differences (Z gradient from zgrad [3][C]):
+//   geoA[i][s] = (n0, n1, zf_R - zf_L, zf_own - Z_i), geoB[i][s] = (rx, ry)
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_pack32(Mesh m, mfv_sw_mixed p, const double *__restrict__ h,
+                                                       const double *__restrict__ hu,
+                                                       const double *__restrict__ hv, const double *__restrict__ Z,
+                                                       const double *__restrict__ zgrad, double om)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const size_t C = m.C;
+    reinterpret_cast<double4 *>(p.state)[i] = make_double4(h[i], hu[i], hv[i], Z[i]);
+    const int nf = m.cell_nfaces[i];
+    const double cx = m.cell_center[3 * (size_t)i], cy = m.cell_center[3 * (size_t)i + 1];
+    for (int s = 0; s < MF && s < nf; s++) {
+        const int f = m.cell_faces[(size_t)s * C + i];
+        const double fx = m.face_center[3 * (size_t)f], fy = m.face_center[3 * (size_t)f + 1];
+        const double rx = fx - cx, ry = fy - cy;
+        const double zo = Z[i] + om * (zgrad[i] * rx + zgrad[C + i] * ry);
+        const int L = m.face_cells[2 * f], R = m.face_cells[2 * f + 1];
+        double dz = 0.;
+        if (R >= 0) {
+            const int j = (L == i) ? R : L;
+            const double rjx = fx - m.cell_center[3 * (size_t)j], rjy = fy - m.cell_center[3 * (size_t)j + 1];
+            const double zt = Z[j] + om * (zgrad[j] * rjx + zgrad[C + j] * rjy);
+            dz = (L == i) ? zt - zo : zo - zt;
+        }
+        const double *n = m.face_normal + 3 * (size_t)f;
+        reinterpret_cast<float4 *>(p.geoA)[MF * (size_t)i + s] =
+            make_float4((float)n[0], (float)n[1], (float)dz, (float)(zo - Z[i]));
+        reinterpret_cast<float2 *>(p.geoB)[MF * (size_t)i + s] = make_float2((float)rx, (float)ry);
+    }
+}
+
+__device__ __forceinline__ void accum_grad(float jx, float jy, float de, float du, float dv, float &ixx, float &iyy,
+                                           float &ixy, float &je0, float &je1, float &ju0, float &ju1, float &jv0,
+                                           float &jv1)
+{
+    ixx += jx * jx;
+    iyy += jy * jy;
+    ixy += jx * jy;
+    je0 += jx * de;
+    je1 += jy * de;
+    ju0 += jx * du;
+    ju1 += jy * du;
+    jv0 += jx * dv;
+    jv1 += jy * dv;
+}
+
+// Reconstruction: float32 LSQ gradients of eta, hu, hv from float64 differences, Barth-Jespersen
+// limiter on deviations, face records (a, hu_f, hv_f, w) -- see the file comment.
+template <int MF, int BS = kBlock>
+__global__ void __launch_bounds__(BS) k_sw_recon32(Mesh m, mfv_sw_mixed p, float om)
+{
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= m.C) return;
+    const size_t C = m.C;
+    const double4 *st = reinterpret_cast<const double4 *>(p.state);
+    const double4 si = st[i];
+    const double hi = si.x, ei = si.x + si.w, ui = si.y, vi = si.z;
+    const double cx = m.cell_center[3 * (size_t)i], cy = m.cell_center[3 * (size_t)i + 1];
+    float ixx = 0.f, iyy = 0.f, ixy = 0.f, je0 = 0.f, je1 = 0.f, ju0 = 0.f, ju1 = 0.f, jv0 = 0.f, jv1 = 0.f;
+    const int nn = m.lsq_ncells[i];
+    for (int j = 0; j < nn; j++) {
+        const int c = m.lsq_cells[(size_t)j * C + i];
+        const double4 sc = st[c];
+        const float jx = (float)(m.cell_center[3 * (size_t)c] - cx);
+        const float jy = (float)(m.cell_center[3 * (size_t)c + 1] - cy);
+        accum_grad(jx, jy, (float)((sc.x + sc.w) - ei), (float)(sc.y - ui), (float)(sc.z - vi), ixx, iyy, ixy, je0,
+                   je1, ju0, ju1, jv0, jv1);
+    }
+    const int ng = m.lsq_nghosts[i];
+    for (int j = 0; j < ng; j++) {
+        const int f = m.lsq_ghosts[(size_t)j * C + i];
+        const double4 so = st[m.face_cells[2 * f]];
+        const double *gc = m.face_ghost + 3 * (size_t)f;
+        // wall ghost: eta and Z mirrored as Neumann, momenta as nonslip (-owner)
+        accum_grad((float)(gc[0] - cx), (float)(gc[1] - cy), (float)((so.x + so.w) - ei), (float)(-so.y - ui),
+                   (float)(-so.z - vi), ixx, iyy, ixy, je0, je1, ju0, ju1, jv0, jv1);
+    }
+    const float dia = ixx * iyy - ixy * ixy;
+    const float ex = (iyy * je0 - ixy * je1) / dia, ey = (ixx * je1 - ixy * je0) / dia;
+    const float ux = (iyy * ju0 - ixy * ju1) / dia, uy = (ixx * ju1 - ixy * ju0) / dia;
+    const float vx = (iyy * jv0 - ixy * jv1) / dia, vy = (ixx * jv1 - ixy * jv0) / dia;
+    const int nf = m.cell_nfaces[i];
+    float emax = 0.f, emin = 0.f, umax = 0.f, umin = 0.f, vmax = 0.f, vmin = 0.f;  // bounds as deviations
+    float dh[MF];
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        dh[s] = 0.f;
+        if (s >= nf) continue;
+        const int code = p.nbr[(size_t)s * C + i];
+        float eb, ub, vb;
+        if (code >= 0) {
+            const double4 sb = st[code >> 3];
+            eb = (float)((sb.x + sb.w) - ei);
+            ub = (float)(sb.y - ui);
+            vb = (float)(sb.z - vi);
+            if (code & 1) dh[s] = (float)(hi - sb.x);  // own cell is the face's R: deviation from the L depth
+        } else {
+            eb = 0.f;
+            ub = (float)(-2. * ui);
+            vb = (float)(-2. * vi);
+        }
+        emax = fmaxf(emax, eb);
+        emin = fminf(emin, eb);
+        umax = fmaxf(umax, ub);
+        umin = fminf(umin, ub);
+        vmax = fmaxf(vmax, vb);
+        vmin = fminf(vmin, vb);
+    }
+    float rx[MF], ry[MF];
+    float pe = 1.f, pu = 1.f, pv = 1.f;
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        const float2 q = reinterpret_cast<const float2 *>(p.geoB)[MF * (size_t)i + s];
+        rx[s] = q.x;
+        ry[s] = q.y;
+        if (s >= nf) continue;
+        float d = ex * rx[s] + ey * ry[s];
+        if (!(fabsf(d) < 1e-8f)) pe = fminf(pe, d > 0.f ? emax / d : emin / d);
+        d = ux * rx[s] + uy * ry[s];
+        if (!(fabsf(d) < 1e-8f)) pu = fminf(pu, d > 0.f ? umax / d : umin / d);
+        d = vx * rx[s] + vy * ry[s];
+        if (!(fabsf(d) < 1e-8f)) pv = fminf(pv, d > 0.f ? vmax / d : vmin / d);
+    }
+    const float hu32 = (float)ui, hv32 = (float)vi, h32 = (float)hi;
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        if (s >= nf) break;
+        const float dzo = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s].w;
+        const float de = om * pe * (ex * rx[s] + ey * ry[s]);
+        const int code = p.nbr[(size_t)s * C + i];
+        const bool right = code >= 0 && (code & 1);
+        float4 o;
+        o.x = right ? dh[s] + (de - dzo) : de - dzo;
+        o.y = hu32 + om * pu * (ux * rx[s] + uy * ry[s]);
+        o.z = hv32 + om * pv * (vx * rx[s] + vy * ry[s]);
+        o.w = right ? dh[s] : h32;
+        reinterpret_cast<float4 *>(p.facerec)[MF * (size_t)i + s] = o;
+    }
+}
+
+// time_step_SW of a float64 state with the float32 face normals (mesure = |n|), faces in
+// faceid order.
+template <int MF>
+__device__ __forceinline__ double sw_dt32(const mfv_sw_mixed &p, int i, int nf, double hn, double un, double vn,
+                                          double vol, float g, double cfl)
+{
+    const double velson = sqrt((double)g * hn);
+    double lam = 0.;
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        if (s >= nf) break;
+        const float4 ga = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
+        const double n0 = ga.x, n1 = ga.y, mesu = (double)sqrtf(ga.x * ga.x + ga.y * ga.y);
+        const double u_n = fabs(un / hn * n0 + vn / hn * n1);
+        lam += (u_n / mesu + velson) * mesu;
+    }
+    return cfl * vol / lam;
+}
+
+// One cell of the step: compensated float32 residual, float64 update, next-dt candidate.
+template <int MF>
+__device__ __forceinline__ double sw_flux32_cell(const Mesh &m, const mfv_sw_mixed &p, float g, double cfl, int i,
+                                                 double dt)
+{
+    const size_t C = m.C;
+    double4 *st = reinterpret_cast<double4 *>(p.state);
+    const double4 si = st[i];
+    const int nf = m.cell_nfaces[i];
+    const unsigned ord = p.order[i];
+    const float4 *fr = reinterpret_cast<const float4 *>(p.facerec);
+    float r0 = 0.f, c0 = 0.f, r1 = 0.f, c1 = 0.f, r2 = 0.f, c2 = 0.f, s1 = 0.f, d1 = 0.f, s2 = 0.f, d2 = 0.f;
+#pragma unroll
+    for (int r = 0; r < MF; r++) {
+        if (r >= nf) break;
+        const int s = (ord >> (2 * r)) & 3;
+        const float4 ga = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
+        const float mes = sqrtf(ga.x * ga.x + ga.y * ga.y);
+        const float4 own = fr[MF * (size_t)i + s];
+        const int code = p.nbr[(size_t)s * C + i];
+        float F0, F1, F2, bo;
+        if (code < 0) {
+            face_dev(own.w, own.x, own.x, 0.f, own.y, own.z, -own.y, -own.z, ga.w, ga.x, ga.y, mes, g, 0.f, true, F0,
+                     F1, F2, bo);
+        } else {
+            const float4 oth = fr[MF * (size_t)(code >> 3) + ((code >> 1) & 3)];
+            if ((code & 1) == 0)
+                face_dev(own.w, own.x, oth.x, ga.z, own.y, own.z, oth.y, oth.z, ga.w, ga.x, ga.y, mes, g, 0.f, true,
+                         F0, F1, F2, bo);
+            else
+                face_dev(oth.w, oth.x, own.x, ga.z, oth.y, oth.z, own.y, own.z, ga.w, ga.x, ga.y, mes, g, -own.w,
+                         false, F0, F1, F2, bo);
+        }
+        if (code < 0 || (code & 1) == 0) {
+            twosum_add(r0, c0, -F0);
+            twosum_add(r1, c1, -F1);
+            twosum_add(r2, c2, -F2);
+            twosum_add(s1, d1, -(bo * ga.x));
+            twosum_add(s2, d2, -(bo * ga.y));
+        } else {
+            twosum_add(r0, c0, F0);
+            twosum_add(r1, c1, F1);
+            twosum_add(r2, c2, F2);
+            twosum_add(s1, d1, bo * ga.x);
+            twosum_add(s2, d2, bo * ga.y);
+        }
+    }
+    const double vol = p.vol[i];
+    const double R0 = (double)r0 + (double)c0;
+    const double R1 = ((double)r1 + (double)c1) + ((double)s1 + (double)d1);
+    const double R2 = ((double)r2 + (double)c2) + ((double)s2 + (double)d2);
+    const double hn = si.x + dt * R0 / vol;
+    const double un = si.y + dt * (R1 / vol);
+    const double vn = si.z + dt * (R2 / vol);
+    st[i] = make_double4(hn, un, vn, si.w);
+    return sw_dt32<MF>(p, i, nf, hn, un, vn, vol, g, cfl);
+}
+
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_flux32(Mesh m, mfv_sw_mixed p, float g, double cfl, DtRing ring,
+                                                       double *__restrict__ dtpart)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    ring_advance(ring);
+    double dloc = 1e300;
+    if (i < m.C) dloc = sw_flux32_cell<MF>(m, p, g, cfl, i, ring.cur[0]);
+    wave_min_part(dtpart, dloc);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sw_unpack32(int C, const double *__restrict__ state,
+                                                         double *__restrict__ h, double *__restrict__ hu,
+                                                         double *__restrict__ hv)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const double4 s = reinterpret_cast<const double4 *>(state)[i];
+    h[i] = s.x;
+    hu[i] = s.y;
+    hv[i] = s.z;
+}
+
+// Deterministic float64 sums of (h, hu, hv) * vol: fixed-shape block tree, then a single-block
+// pass over the block partials in index order.
+constexpr int kSumBlock = 256;
+
+__device__ __forceinline__ void block_tree3(double (*sh)[kSumBlock])
+{
+    __syncthreads();
+    for (int w = kSumBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kSumBlock) k_sw_sums_partial(int C, const double *__restrict__ state,
+                                                               const double *__restrict__ vol,
+                                                               double *__restrict__ partial)
+{
+    __shared__ double sh[3][kSumBlock];
+    const int i = blockIdx.x * kSumBlock + threadIdx.x;
+    double a = 0., b = 0., c = 0.;
+    if (i < C) {
+        const double4 s = reinterpret_cast<const double4 *>(state)[i];
+        a = s.x * vol[i];
+        b = s.y * vol[i];
+        c = s.z * vol[i];
+    }
+    sh[0][threadIdx.x] = a;
+    sh[1][threadIdx.x] = b;
+    sh[2][threadIdx.x] = c;
+    block_tree3(sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) partial[3 * (size_t)blockIdx.x + k] = sh[k][0];
+}
+
+__global__ void __launch_bounds__(kSumBlock) k_sw_sums_final(int nb, const double *__restrict__ partial,
+                                                             double *__restrict__ out)
+{
+    __shared__ double sh[3][kSumBlock];
+    double a = 0., b = 0., c = 0.;
+    for (int j = threadIdx.x; j < nb; j += kSumBlock) {
+        a += partial[3 * (size_t)j];
+        b += partial[3 * (size_t)j + 1];
+        c += partial[3 * (size_t)j + 2];
+    }
+    sh[0][threadIdx.x] = a;
+    sh[1][threadIdx.x] = b;
+    sh[2][threadIdx.x] = c;
+    block_tree3(sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) out[k] = sh[k][0];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Ensembles: many independent instances concatenated into one mesh, each instance's cells
+// starting at a multiple of 64 so that every wavefront belongs to one instance. The step
+// kernels are the ones above; only the dt ring is per instance.
+
+struct BatchK {
+    int I, W;
+    const int32_t *__restrict__ wave_inst;
+    const int32_t *__restrict__ inst_cell0;
+    const int32_t *__restrict__ inst_ncells;
+    double *__restrict__ rings;
+};
+
+// Each wave stores its dt minimum at part[global wave] (wave w covers cells 64w..64w+63);
+// k_batch_dt_fold then reduces the waves of each instance.
+template <int MF, int BS>
+__global__ void __launch_bounds__(BS) k_sw_flux32_batch(Mesh m, mfv_sw_mixed p, float g, double cfl, BatchK b,
+                                                        int kcur, double *__restrict__ part)
+{
+    const int i = blockIdx.x * BS + threadIdx.x;
+    // C is a multiple of 64: all lanes of a wave agree on i < C and on the instance
+    if (i >= m.C) return;
+    const int inst = b.wave_inst[i >> 6];
+    if (inst < 0) return;
+    const double dloc = sw_flux32_cell<MF>(m, p, g, cfl, i, b.rings[4 * (size_t)inst + kcur]);
+    wave_min_part(part, dloc);
+}
+
+// One wave per instance: ring[knext] = min(ring[knext], min over the instance's waves).
+// With advance != 0 it also closes the step: ring[kspare] = 1e6, ring[3] += ring[kcur].
+__global__ void __launch_bounds__(256) k_batch_dt_fold(const double *__restrict__ part, BatchK b, int kcur,
+                                                       int knext, int kspare, int advance)
+{
+    const int inst = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (inst >= b.I) return;
+    const int w0 = b.inst_cell0[inst] >> 6;
+    const int w1 = (b.inst_cell0[inst] + b.inst_ncells[inst] + 63) >> 6;
+    double m = __longlong_as_double(0x7ff0000000000000LL);
+    for (int w = w0 + lane; w < w1; w += 64) {
+        const double v = part[w];
+        m = (v < m) ? v : m;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = (o < m) ? o : m;
+    }
+    if (lane == 0) {
+        double *ring = b.rings + 4 * (size_t)inst;
+        if (m < ring[knext]) ring[knext] = m;
+        if (advance) {
+            ring[kspare] = 1e6;
+            ring[3] += ring[kcur];
+        }
+    }
+}
+
+__global__ void k_ring_reset_batch(double *__restrict__ rings, int I)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= I) return;
+    rings[4 * (size_t)t + 0] = 1e6;
+    rings[4 * (size_t)t + 1] = 1e6;
+    rings[4 * (size_t)t + 2] = 1e6;
+    rings[4 * (size_t)t + 3] = 0.;
+}
+
+// dt of the current state into ring slot 0 (after k_ring_reset_batch)
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_dt_batch(Mesh m, mfv_sw_mixed p, float g, double cfl, BatchK b,
+                                                        double *__restrict__ part)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m.C) return;
+    const int inst = b.wave_inst[i >> 6];
+    if (inst < 0) return;
+    const double4 s = reinterpret_cast<const double4 *>(p.state)[i];
+    const double dloc = sw_dt32<MF>(p, i, m.cell_nfaces[i], s.x, s.y, s.z, p.vol[i], g, cfl);
+    wave_min_part(part, dloc);
+}
+
+// Per-instance float64 totals sum (h, hu, hv) vol: one block per instance, fixed order.
+__global__ void __launch_bounds__(kSumBlock) k_sw_sums_batch(const double *__restrict__ state,
+                                                             const double *__restrict__ vol, BatchK b,
+                                                             double *__restrict__ out)
+{
+    __shared__ double sh[3][kSumBlock];
+    const int inst = blockIdx.x;
+    const int c0 = b.inst_cell0[inst], n = b.inst_ncells[inst];
+    double a = 0., bb = 0., c = 0.;
+    for (int j = threadIdx.x; j < n; j += kSumBlock) {
+        const double4 s = reinterpret_cast<const double4 *>(state)[c0 + j];
+        const double v = vol[c0 + j];
+        a += s.x * v;
+        bb += s.y * v;
+        c += s.z * v;
+    }
+    sh[0][threadIdx.x] = a;
+    sh[1][threadIdx.x] = bb;
+    sh[2][threadIdx.x] = c;
+    block_tree3(sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) out[3 * (size_t)inst + k] = sh[k][0];
+}
+
+inline unsigned fold_grid(int I) { return (unsigned)((I + 3) / 4); }  // 4 instances per 256-thread block
+
+template <int MF, int BS>
+void launch_batch_step(const Mesh &m, const mfv_sw_mixed &p, const BatchK &b, float g, double cfl, float om,
+                       int step, double *part, hipStream_t st)
+{
+    const int k = ((step % 3) + 3) % 3;
+    const unsigned grid = (unsigned)((m.C + BS - 1) / BS);
+    hipLaunchKernelGGL((k_sw_recon32<MF, BS>), dim3(grid), dim3(BS), 0, st, m, p, om);
+    hipLaunchKernelGGL((k_sw_flux32_batch<MF, BS>), dim3(grid), dim3(BS), 0, st, m, p, g, cfl, b, k, part);
+    hipLaunchKernelGGL(k_batch_dt_fold, dim3(fold_grid(b.I)), dim3(256), 0, st, part, b, k, (k + 1) % 3,
+                       (k + 2) % 3, 1);
+}
+
+template <int MF>
+int batch_step_bs(const Mesh &m, const mfv_sw_mixed &p, const BatchK &b, float g, double cfl, float om, int step,
+                  int block, double *part, hipStream_t st)
+{
+    switch (block) {
+    case 64: launch_batch_step<MF, 64>(m, p, b, g, cfl, om, step, part, st); break;
+    case 128: launch_batch_step<MF, 128>(m, p, b, g, cfl, om, step, part, st); break;
+    case 256: launch_batch_step<MF, 256>(m, p, b, g, cfl, om, step, part, st); break;
+    case 512: launch_batch_step<MF, 512>(m, p, b, g, cfl, om, step, part, st); break;
+    default: set_error("block size %d not in {64, 128, 256, 512}", block); return MFV_EINVAL;
+    }
+    return check_launch("ensemble step");
+}
+
+// A captured ensemble graph owns its dt partials buffer: the library's shared scratch may be
+// reallocated by later calls, which must not move memory a graph still refers to.
+struct MfvGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+    double *part = nullptr;
+};
+
+void graph_free(MfvGraph *g)
+{
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->part) (void)hipFree(g->part);
+    delete g;
+}
+
+int check_mixed(const mfv_mesh *m, const mfv_sw_mixed *p)
+{
+    MFV_REQUIRE(p != nullptr, "mixed descriptor is NULL");
+    MFV_REQUIRE(p->nb_cells == m->nb_cells && p->max_cell_faces == m->max_cell_faces,
+                "mixed descriptor does not match the mesh");
+    MFV_REQUIRE(m->max_cell_faces == 3 || m->max_cell_faces == 4, "mixed path supports 3 or 4 faces per cell");
+    MFV_REQUIRE(p->state && p->vol && p->facerec && p->geoA && p->geoB && p->nbr && p->order,
+                "mixed descriptor has NULL arrays");
+    return MFV_OK;
+}
+
+int check_batch(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b)
+{
+    if (int rc = check_mixed(m, p)) return rc;
+    MFV_REQUIRE(b != nullptr, "batch descriptor is NULL");
+    MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
+    MFV_REQUIRE(m->nb_cells % 64 == 0, "batched mesh must have a multiple of 64 cells, got %d", m->nb_cells);
+    MFV_REQUIRE(b->nb_waves == m->nb_cells / 64, "nb_waves %d != cells/64 %d", b->nb_waves, m->nb_cells / 64);
+    MFV_REQUIRE(b->nb_inst >= 1, "nb_inst must be positive");
+    MFV_REQUIRE(b->wave_inst && b->inst_cell0 && b->inst_ncells && b->rings, "batch descriptor has NULL arrays");
+    return MFV_OK;
+}
+
+BatchK to_batch(const mfv_sw_batch *b)
+{
+    return BatchK{b->nb_inst, b->nb_waves, b->wave_inst, b->inst_cell0, b->inst_ncells, b->rings};
+}
+
+}  // namespace
+
+extern "C" int mfv_sw_pack_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, const double *h, const double *hu,
+                                 const double *hv, const double *Z, const double *zgrad, int order, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_mixed(m, p)) return rc;
+    MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    MFV_REQUIRE(h && hu && hv && Z && zgrad, "NULL array");
+    if (m->nb_cells == 0) return MFV_OK;
+    auto k = (m->max_cell_faces == 3) ? k_sw_pack32<3> : k_sw_pack32<4>;
+    hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p, h, hu,
+                       hv, Z, zgrad, (double)(order - 1));
+    return check_launch("k_sw_pack32");
+}
+
+extern "C" int mfv_sw_unpack_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, double *h, double *hu, double *hv,
+                                   void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_mixed(m, p)) return rc;
+    MFV_REQUIRE(h && hu && hv, "NULL array");
+    if (m->nb_cells == 0) return MFV_OK;
+    hipLaunchKernelGGL(k_sw_unpack32, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), m->nb_cells,
+                       p->state, h, hu, hv);
+    return check_launch("k_sw_unpack32");
+}
+
+extern "C" int mfv_sw_recon_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, int order, void *stream)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_mixed(m, p)) return rc;
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    if (m->nb_cells == 0) return MFV_OK;
+    auto k = (m->max_cell_faces == 3) ? k_sw_recon32<3> : k_sw_recon32<4>;
+    hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p,
+                       (float)(order - 1));
+    return check_launch("k_sw_recon32");
+}
+
+extern "C" int mfv_sw_flux_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, double grav, double cfl,
+                                 double *dt_ring_p, int step, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_mixed(m, p)) return rc;
+    MFV_REQUIRE(dt_ring_p, "dt ring is NULL");
+    if (m->nb_cells == 0) return MFV_OK;
+    DtRing r = dt_ring(dt_ring_p, step);
+    const size_t nw = waves_for(m->nb_cells);
+    double *part = dt_scratch(nw, as_stream(stream));
+    if (!part) return MFV_ENOMEM;
+    auto k = (m->max_cell_faces == 3) ? k_sw_flux32<3> : k_sw_flux32<4>;
+    hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p,
+                       (float)grav, cfl, r, part);
+    if (int rc = check_launch("k_sw_flux32")) return rc;
+    return dt_finish(part, nw, r.next, as_stream(stream));
+}
+
+extern "C" int mfv_sw_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, double *partial, int n_partial,
+                                   double *out3, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_mixed(m, p)) return rc;
+    const int nb = (m->nb_cells + kSumBlock - 1) / kSumBlock;
+    MFV_REQUIRE(partial && out3, "NULL array");
+    MFV_REQUIRE(n_partial >= 3 * nb, "partial needs 3*ceil(C/256) = %d doubles, got %d", 3 * nb, n_partial);
+    if (nb == 0) return MFV_OK;
+    hipLaunchKernelGGL(k_sw_sums_partial, dim3(nb), dim3(kSumBlock), 0, as_stream(stream), m->nb_cells, p->state,
+                       p->vol, partial);
+    if (int rc = check_launch("k_sw_sums_partial")) return rc;
+    hipLaunchKernelGGL(k_sw_sums_final, dim3(1), dim3(kSumBlock), 0, as_stream(stream), nb, partial, out3);
+    return check_launch("k_sw_sums_final");
+}
+
+extern "C" int mfv_sw_batch_dt_init(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b, double grav,
+                                    double cfl, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_batch(m, p, b)) return rc;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_ring_reset_batch, dim3((b->nb_inst + 255) / 256), dim3(256), 0, st, b->rings, b->nb_inst);
+    if (m->nb_cells) {
+        const size_t nw = waves_for(m->nb_cells);
+        double *part = dt_scratch(nw, st);
+        if (!part) return MFV_ENOMEM;
+        auto k = (m->max_cell_faces == 3) ? k_sw_dt_batch<3> : k_sw_dt_batch<4>;
+        hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, st, to_kernel(m), *p, (float)grav, cfl,
+                           to_batch(b), part);
+        hipLaunchKernelGGL(k_batch_dt_fold, dim3(fold_grid(b->nb_inst)), dim3(256), 0, st, part, to_batch(b), 0, 0,
+                           0, 0);
+    }
+    return check_launch("k_sw_dt_batch");
+}
+
+extern "C" int mfv_sw_batch_step(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b, double grav,
+                                 double cfl, int order, int step, int block, void *stream)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_batch(m, p, b)) return rc;
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    if (m->nb_cells == 0) return MFV_OK;
+    const Mesh mk = to_kernel(m);
+    const BatchK bk = to_batch(b);
+    const float om = (float)(order - 1);
+    double *part = dt_scratch((size_t)m->nb_cells / 64, as_stream(stream));
+    if (!part) return MFV_ENOMEM;
+    if (m->max_cell_faces == 3)
+        return batch_step_bs<3>(mk, *p, bk, (float)grav, cfl, om, step, block, part, as_stream(stream));
+    return batch_step_bs<4>(mk, *p, bk, (float)grav, cfl, om, step, block, part, as_stream(stream));
+}
+
+extern "C" int mfv_sw_batch_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b,
+                                         double *out, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (int rc = check_batch(m, p, b)) return rc;
+    MFV_REQUIRE(out, "NULL array");
+    hipLaunchKernelGGL(k_sw_sums_batch, dim3(b->nb_inst), dim3(kSumBlock), 0, as_stream(stream), p->state, p->vol,
+                       to_batch(b), out);
+    return check_launch("k_sw_sums_batch");
+}
+
+extern "C" int mfv_sw_batch_graph(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b, double grav,
+                                  double cfl, int order, int nsteps, int block, void **graph_out)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_batch(m, p, b)) return rc;
+    MFV_REQUIRE(graph_out, "graph_out is NULL");
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    MFV_REQUIRE(nsteps >= 3 && nsteps % 3 == 0 && nsteps <= 3000, "nsteps %d must be a multiple of 3 in [3, 3000]",
+                nsteps);
+    MFV_REQUIRE(block == 64 || block == 128 || block == 256 || block == 512, "block size %d not in {64..512}",
+                block);
+    *graph_out = nullptr;
+    MfvGraph *g = new MfvGraph();
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&g->part), ((size_t)m->nb_cells / 64 + 1) * sizeof(double));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        set_error("graph capture setup: %s", hipGetErrorString(e));
+        graph_free(g);
+        return MFV_EHIP;
+    }
+    const Mesh mk = to_kernel(m);
+    const BatchK bk = to_batch(b);
+    int rc = MFV_OK;
+    for (int k = 0; k < nsteps && rc == MFV_OK; k++) {
+        rc = (m->max_cell_faces == 3)
+                 ? batch_step_bs<3>(mk, *p, bk, (float)grav, cfl, (float)(order - 1), k, block, g->part, g->stream)
+                 : batch_step_bs<4>(mk, *p, bk, (float)grav, cfl, (float)(order - 1), k, block, g->part, g->stream);
+    }
+    e = hipStreamEndCapture(g->stream, &g->graph);
+    if (rc != MFV_OK || e != hipSuccess) {
+        if (rc == MFV_OK) set_error("graph capture: %s", hipGetErrorString(e));
+        graph_free(g);
+        return rc != MFV_OK ? rc : MFV_EHIP;
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        set_error("graph instantiate: %s", hipGetErrorString(e));
+        graph_free(g);
+        return MFV_EHIP;
+    }
+    *graph_out = g;
+    return MFV_OK;
+}
+
+extern "C" int mfv_graph_launch(void *graph, void *stream)
+{
+    MFV_REQUIRE(graph, "graph is NULL");
+    hipError_t e = hipGraphLaunch(static_cast<MfvGraph *>(graph)->exec, as_stream(stream));
+    if (e != hipSuccess) {
+        set_error("hipGraphLaunch: %s", hipGetErrorString(e));
+        return MFV_EHIP;
+    }
+    return MFV_OK;
+}
+
+extern "C" int mfv_graph_destroy(void *graph)
+{
+    graph_free(static_cast<MfvGraph *>(graph));
+    return MFV_OK;
+}

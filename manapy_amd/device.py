@@ -248,18 +248,18 @@ class SwPacked(object):
 class SwMixed(object):
     """Device tables of the mixed-precision shallow-water path (mfv_sw_mixed)."""
 
-    def __init__(self, dmesh):
+    def __init__(self, dmesh, host=None):
         import torch
-        host = sw_packed_layout(dmesh.domain)
+        host = host if host is not None else sw_packed_layout(dmesh.domain)
         C = dmesh.C
         MF = host['MF']
         dev = dmesh.device
         f32 = torch.float32
         self.state = torch.zeros((C, 4), dtype=torch.float64, device=dev)
-        self.cellrec = torch.zeros((C, 8), dtype=f32, device=dev)
+        self.vol = dmesh.t['cell_volume']
         self.facerec = torch.zeros((C, MF, 4), dtype=f32, device=dev)
-        self.cfrec = torch.as_tensor(host['cfrec'].astype(np.float32), device=dev)
-        self.cfgeo = torch.as_tensor(host['cfgeo'].astype(np.float32), device=dev)
+        self.geoA = torch.zeros((C, MF, 4), dtype=f32, device=dev)
+        self.geoB = torch.zeros((C, MF, 2), dtype=f32, device=dev)
         self.nbr = torch.as_tensor(host['nbr'], device=dev)
         self.order = torch.as_tensor(host['order'], device=dev)
         self.partial = torch.zeros(3 * ((C + 255) // 256) + 3, dtype=torch.float64, device=dev)
@@ -267,7 +267,7 @@ class SwMixed(object):
         d = _lib.MfvSwMixed()
         d.nb_cells = C
         d.max_cell_faces = MF
-        for k in ('state', 'cellrec', 'facerec', 'cfrec', 'cfgeo', 'nbr', 'order'):
+        for k in ('state', 'vol', 'facerec', 'geoA', 'geoB', 'nbr', 'order'):
             setattr(d, k, getattr(self, k).data_ptr())
         self.desc = d
 

@@ -85,9 +85,6 @@ def ensemble_layout(domains):
     # neighbour codes (other << 3 | slot << 1 | side): shift the cell part by the instance offset
     out['nbr'] = _cat_cols([np.where(P['nbr'] >= 0, P['nbr'] + (c0[i] << 3), -1).astype(np.int32)
                             for i, P in enumerate(Ps)], pads, -1)
-    out['cfrec'] = _cat_rows([P['cfrec'].astype(np.float32) for P in Ps], pads, 0.)
-    cfgeo = [P['cfgeo'].astype(np.float32) for P in Ps]
-    out['cfgeo'] = _cat_rows(cfgeo, pads, 1.)
     out['order'] = _cat_rows([P['order'] for P in Ps], pads, 0)
     out['wave_inst'] = np.repeat(np.arange(len(domains), dtype=np.int32), Cp // 64)
     out['inst_cell0'] = c0.astype(np.int32)
@@ -122,12 +119,13 @@ class SwEnsemble(object):
         self.mesh = m
         f32 = torch.float32
         self.state = torch.zeros((self.C, 4), dtype=torch.float64, device=dev)
-        self.cellrec = torch.zeros((self.C, 8), dtype=f32, device=dev)
         self.facerec = torch.zeros((self.C, self.MF, 4), dtype=f32, device=dev)
+        self.geoA = torch.zeros((self.C, self.MF, 4), dtype=f32, device=dev)
+        self.geoB = torch.zeros((self.C, self.MF, 2), dtype=f32, device=dev)
         p = _lib.MfvSwMixed()
         p.nb_cells, p.max_cell_faces = self.C, self.MF
-        p.state, p.cellrec, p.facerec = self.state.data_ptr(), self.cellrec.data_ptr(), self.facerec.data_ptr()
-        p.cfrec, p.cfgeo = self.t['cfrec'].data_ptr(), self.t['cfgeo'].data_ptr()
+        p.state, p.vol, p.facerec = self.state.data_ptr(), self.t['cell_volume'].data_ptr(), self.facerec.data_ptr()
+        p.geoA, p.geoB = self.geoA.data_ptr(), self.geoB.data_ptr()
         p.nbr, p.order = self.t['nbr'].data_ptr(), self.t['order'].data_ptr()
         self.mixed = p
         self.rings = torch.zeros((self.n, 4), dtype=torch.float64, device=dev)
@@ -222,7 +220,7 @@ class SwEnsemble(object):
 
     def instance_state(self, i):
         c0, n = int(self.host['inst_cell0'][i]), int(self.host['inst_ncells'][i])
-        st = self.state[c0:c0 + n].cpu().numpy()
+        st = self.state[c0:c0 + n].cpu().numpy()  # h, hu, hv, Z
         return st[:, 0].copy(), st[:, 1].copy(), st[:, 2].copy()
 
     def times(self):
@@ -242,7 +240,7 @@ class SwEnsemble(object):
         if key in table:
             self.block = int(table[key]['block'])
             return table[key]
-        saved = (self.state.clone(), self.cellrec.clone(), self.facerec.clone(), self.rings.clone(), self.step_index)
+        saved = (self.state.clone(), self.facerec.clone(), self.rings.clone(), self.step_index)
         st = torch.cuda.current_stream()
         res = {}
         for blk in BLOCKS:
@@ -254,10 +252,9 @@ class SwEnsemble(object):
             e1.synchronize()
             res[blk] = e0.elapsed_time(e1) / steps
         self.state.copy_(saved[0])
-        self.cellrec.copy_(saved[1])
-        self.facerec.copy_(saved[2])
-        self.rings.copy_(saved[3])
-        self.step_index = saved[4]
+        self.facerec.copy_(saved[1])
+        self.rings.copy_(saved[2])
+        self.step_index = saved[3]
         best = min(res, key=res.get)
         self.block = best
         entry = {'block': best, 'ms_per_step': {str(k): v for k, v in res.items()}, 'when': time.time()}

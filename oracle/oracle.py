@@ -30,6 +30,8 @@ def lib():
         _lib = ctypes.CDLL(LIB)
         _lib.orc_time_step.restype = _D
         _lib.orc_time_step_sw.restype = _D
+        _lib.orc_mixed_flux.restype = _D
+        _lib.orc_mixed_dt.restype = _D
     return _lib
 
 
@@ -252,3 +254,45 @@ def euler_step(d, U, G, cfl, gamma, quirk):
     rez = euler_rusanov(d, U, G, gamma, quirk)
     euler_update(d, U, rez, gamma, dt)
     return dt
+
+
+# ---------------------------------------------------------------------------------------------
+# mixed-precision shallow-water kernels restated in C float arithmetic (mixed32.c)
+
+class MixedModel(object):
+    """CPU model of the mixed-precision shallow-water path on a Domain (layout as the GPU's)."""
+
+    def __init__(self, d, h, hu, hv, Z, zgrad, order=2):
+        from manapy_amd.device import gpu_layout, sw_packed_layout
+        L = gpu_layout(d, lsq=True)
+        Pk = sw_packed_layout(d)
+        self.L, self.Pk = L, Pk
+        self.C, self.MF = L['C'], L['MF']
+        C, MF = self.C, self.MF
+        self.state = np.zeros((C, 4))
+        self.geoA = np.zeros((C, MF, 4), np.float32)
+        self.geoB = np.zeros((C, MF, 2), np.float32)
+        self.facerec = np.zeros((C, MF, 4), np.float32)
+        self.vol = np.ascontiguousarray(d.cells.volume, dtype=np.float64)
+        self.om = float(order - 1)
+        zg = np.ascontiguousarray(zgrad, dtype=np.float64)
+        lib().orc_mixed_pack(_I(C), _I(MF), i32(L['cell_faces']), i32(L['cell_nfaces']), i32(L['face_cells']),
+                             f64(L['face_normal']), f64(L['face_center']), f64(L['cell_center']), f64(h), f64(hu),
+                             f64(hv), f64(Z), f64(zg), _D(self.om), f64(self.state), _p(self.geoA, np.float32),
+                             _p(self.geoB, np.float32))
+
+    def dt(self, g, cfl):
+        """time_step_SW of the current state with the float32 face normals."""
+        return lib().orc_mixed_dt(_I(self.C), _I(self.MF), f64(self.state), f64(self.vol), _p(self.geoA, np.float32),
+                                  i32(self.L['cell_nfaces']), ctypes.c_float(g), _D(cfl))
+
+    def step(self, g, cfl, dt):
+        """One step; returns the next dt."""
+        L, Pk = self.L, self.Pk
+        lib().orc_mixed_recon(_I(self.C), _I(self.MF), f64(self.state), f64(L['cell_center']), i32(L['lsq_cells']),
+                              i32(L['lsq_ncells']), i32(L['lsq_ghosts']), i32(L['lsq_nghosts']), i32(L['face_cells']),
+                              f64(L['face_ghost']), i32(L['cell_nfaces']), i32(Pk['nbr']), _p(self.geoA, np.float32),
+                              _p(self.geoB, np.float32), ctypes.c_float(self.om), _p(self.facerec, np.float32))
+        return lib().orc_mixed_flux(_I(self.C), _I(self.MF), f64(self.state), f64(self.vol),
+                                    _p(self.facerec, np.float32), _p(self.geoA, np.float32), i32(L['cell_nfaces']),
+                                    i32(Pk['nbr']), _p(Pk['order'], np.uint8), ctypes.c_float(g), _D(cfl), _D(dt))

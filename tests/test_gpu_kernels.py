@@ -390,9 +390,9 @@ def test_sw_packed_first_order_and_quads(torch, oracle_lib):
 
 
 def test_sw_mixed_precision_error_and_conservation(torch, oracle_lib):
-    """float32 fluxes with Kahan-compensated sums and a float64 state: relative L1 error
-    (Variable.norml2, order 1) against the float64 oracle <= 1e-5 per field after 200 steps
-    of the dam break over bumps; the float64 mass total is conserved to round-off."""
+    """Mixed-precision path: bit-identical to its CPU float32 model (oracle/mixed32.c) after
+    200 steps of the dam break over bumps; relative L1 error (Variable.norml2, order 1) of
+    every field against the float64 oracle <= 1e-5; the float64 mass total conserved."""
     from manapy_amd.device import DeviceMesh, SwMixed
     from manapy_amd.meshgen import gaussian_bumps
     d = make_tri_domain(60, 60, 100., 100., 0.1, 13)
@@ -405,10 +405,15 @@ def test_sw_mixed_precision_error_and_conservation(torch, oracle_lib):
     zg_ref = oracle_lib.z_gradient(d, Z)
     for _ in range(200):
         oracle_lib.sw_step(d, h, hu, hv, Z, zg_ref, G, 0.8, 2)
+    model = oracle_lib.MixedModel(d, h0, np.zeros(C), np.zeros(C), Z, zg_ref)
+    dt_m = oracle_lib.time_step_sw(d, h0, np.zeros(C), np.zeros(C), 0.8, G)
+    for _ in range(200):
+        dt_m = model.step(G, 0.8, dt_m)
     m = DeviceMesh(d)
     p = SwMixed(m)
     th, thu, thv, tZ, zg = (dev(torch, x) for x in (h0, np.zeros(C), np.zeros(C), Z, zg_ref))
     call('mfv_sw_pack_mixed', m.ptr, p.ptr, P(th), P(thu), P(thv), P(tZ), P(zg), 2)
+    assert np.array_equal(host(p.geoA), model.geoA) and np.array_equal(host(p.geoB), model.geoB)
     call('mfv_sw_conservation', m.ptr, p.ptr, P(p.partial), p.partial.numel(), P(p.sums))
     mass0 = host(p.sums)[0]
     dt0 = dev(torch, np.zeros(1))
@@ -418,6 +423,8 @@ def test_sw_mixed_precision_error_and_conservation(torch, oracle_lib):
     for s in range(200):
         call('mfv_sw_recon_mixed', m.ptr, p.ptr, 2)
         call('mfv_sw_flux_mixed', m.ptr, p.ptr, G, 0.8, P(ring), s)
+    same(host(p.state), model.state, "mixed state vs its float32 model")
+    assert host(ring)[200 % 3] == dt_m
     oh, ohu, ohv = (dev(torch, np.zeros(C)) for _ in range(3))
     call('mfv_sw_unpack_mixed', m.ptr, p.ptr, P(oh), P(ohu), P(ohv))
     call('mfv_sw_conservation', m.ptr, p.ptr, P(p.partial), p.partial.numel(), P(p.sums))
