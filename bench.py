@@ -348,6 +348,8 @@ def main():
     ap.add_argument('--ensemble', type=int, default=256, help='instances of the ensemble run (0: skip)')
     ap.add_argument('--ensemble-steps', type=int, default=201)
     ap.add_argument('--serving', type=int, default=32, help='requests of the serving run (0: skip)')
+    ap.add_argument('--renumber', choices=('hilbert', 'none'), default='hilbert',
+                    help='device cell order: Hilbert curve of the cell centres (default) or the mesh numbering')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run a device-to-device copy of 1 GiB five times (calibrates HBM counters)')
     args = ap.parse_args()
@@ -370,8 +372,14 @@ def main():
     from manapy_amd.device import DeviceMesh, stream_ptr
 
     t0 = time.perf_counter()
-    d, Z, h0 = build_case(args.nx)
+    d0, Z0, h00 = build_case(args.nx)
     t_mesh = time.perf_counter() - t0
+    if args.renumber == 'hilbert':
+        from manapy_amd.renumber import renumbered
+        d = renumbered(d0)
+        Z, h0 = d.to_new(Z0), d.to_new(h00)
+    else:
+        d, Z, h0 = d0, Z0, h00
     C, F = int(d.nbcells), int(d.nbfaces)
     m = DeviceMesh(d)
     s = stream_ptr()
@@ -486,7 +494,7 @@ def main():
         "data": "synthetic (seeded jittered mesh, dam break over seeded Gaussian bumps)",
         "config": {"workload": "shallow-water dam break, 2D, MUSCL + LSQ + Barth-Jespersen + Rusanov, walls",
                    "cells": C, "faces": F, "nx": args.nx, "cfl": CFL, "order": 2, "precision": "fp64 plain",
-                   "path": args.path},
+                   "path": args.path, "cell_order": args.renumber},
         "roofline": {"bound": "hbm", "kernel": names[1] if dom_is_flux else names[0],
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
@@ -509,7 +517,7 @@ def main():
         out["serving"] = run_serving(args.serving)
         log("serving: p50 %.3f s, p99 %.3f s" % (out["serving"]["latency_p50_s"], out["serving"]["latency_p99_s"]))
     if not args.no_cpu:
-        v, n, t = cpu_baseline(d, Z, h0, args.cpu_steps, args.cpu_budget)
+        v, n, t = cpu_baseline(d0, Z0, h00, args.cpu_steps, args.cpu_budget)
         out["cpu_baseline"] = {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
                                "sample": "%d steps of the same 2M-cell case (%.1f s), single-threaded C "
                                          "restatement of the reference loops" % (n, t)}

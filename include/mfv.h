@@ -151,6 +151,40 @@ int mfv_sw_residuals(const mfv_mesh *m, const double *h, const double *hu, const
 int mfv_update_sw(const mfv_mesh *m, double *h, double *hu, double *hv, const double *rez,
                   const double *src, const double *dt_dev, void *stream);
 
+/* The reference's own shallow-water scheme on five fields (h, hu, hv, hc, Z).
+ *
+ * explicitscheme_convective_SW + srnh_scheme (solvers/shallowater/fvm_utils.py:344-496,
+ * 152-342): SRNH Roe-type flux; MUSCL on h and hc only, both limited by psi (= hc.psi);
+ * boundary faces take the ghost state. Residuals are overwritten. */
+int mfv_explicitscheme_convective_sw(const mfv_mesh *m, double *rez_h, double *rez_hu, double *rez_hv,
+                                     double *rez_hc, double *rez_Z, const double *h_c, const double *hu_c,
+                                     const double *hv_c, const double *hc_c, const double *Z_c,
+                                     const double *h_ghost, const double *hu_ghost, const double *hv_ghost,
+                                     const double *hc_ghost, const double *Z_ghost, const double *h_x,
+                                     const double *h_y, const double *hc_x, const double *hc_y,
+                                     const double *psi, double grav, int order, void *stream);
+
+/* term_source_srnh_SW (solvers/shallowater/fvm_utils.py:44-148), triangles only.
+ * nodeid [C][nw], cellfid [C][fw] (cells.nodeid / cells.cellfid, row-major, count in
+ * the last column), nf [C][3][3] (cells.nf), vertex [N][4] (nodes.vertex).
+ * quirk = 1 reads the neighbour of face slot j as cellfid[i][j] like the reference
+ * (cellfid is compacted, so boundary cells read a shifted neighbour and -1 wraps to the
+ * last cell); quirk = 0 uses the cell across face j. src_h, src_hc, src_Z are set to 0. */
+int mfv_term_source_srnh_sw(const mfv_mesh *m, double *src_h, double *src_hu, double *src_hv,
+                            double *src_hc, double *src_Z, const double *h_c, const double *Z_c,
+                            const double *h_ghost, const double *Z_ghost, const int32_t *nodeid, int nw,
+                            const int32_t *cellfid, int fw, const double *nf, const double *vertex,
+                            double grav, int quirk, void *stream);
+
+/* update_SW (solvers/shallowater/fvm_utils.py:3-14), all five fields with the Coriolis,
+ * wind and hc dissipation terms. dt = dt_dev[0] if dt_dev is not NULL, else dtime. */
+int mfv_update_sw5(const mfv_mesh *m, double *h_c, double *hu_c, double *hv_c, double *hc_c, double *Z_c,
+                   const double *rez_h, const double *rez_hu, const double *rez_hv, const double *rez_hc,
+                   const double *rez_Z, const double *src_h, const double *src_hu, const double *src_hv,
+                   const double *src_hc, const double *src_Z, const double *dissip_hc,
+                   const double *corio_hu, const double *corio_hv, double wind_hu, double wind_hv,
+                   double dtime, const double *dt_dev, void *stream);
+
 /* Fused MUSCL/Rusanov step: residuals + update_SW + the next step's time_step_SW.
  * state_in/state_out: [3][C] (h, hu, hv); grad from mfv_sw_gradients(state_in); dt ring as
  * for mfv_advection_step. */

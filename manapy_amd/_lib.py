@@ -108,6 +108,9 @@ _SIGNATURES = {
     'mfv_sw_gradients': [MESH_P, P, P, P, P, P, P],
     'mfv_sw_residuals': [MESH_P, P, P, P, P, P, P, c_double, c_int, P, P, P],
     'mfv_update_sw': [MESH_P, P, P, P, P, P, P, P],
+    'mfv_explicitscheme_convective_sw': [MESH_P] + [P] * 20 + [c_double, c_int, P],
+    'mfv_term_source_srnh_sw': [MESH_P, P, P, P, P, P, P, P, P, P, P, c_int, P, c_int, P, P, c_double, c_int, P],
+    'mfv_update_sw5': [MESH_P] + [P] * 18 + [c_double, c_double, c_double, P, P],
     'mfv_sw_step_fused': [MESH_P, P, P, P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_dt_reset': [P, P],
     'mfv_sw_pack': [MESH_P, SWP_P, P, P, P, P, P, c_int, P],

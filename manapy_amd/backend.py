@@ -228,23 +228,62 @@ def _time_step_sw(domain, h_c, hu_c, hv_c, cfl, normal, mesure, volume, faceid, 
 @HIPBackend.register('update_SW')
 def _update_sw(domain, h_c, hu_c, hv_c, hc_c, Z_c, rez_h, rez_hu, rez_hv, rez_hc, rez_Z, src_h, src_hu, src_hv,
                src_hc, src_Z, dissip_hc, corio_hu, corio_hv, wind_hu, wind_hv, dtime, vol):
-    """solvers/shallowater/fvm_utils.py:3-14 for h, hu, hv; the passive hc and Z must have zero
-    residual, source, dissipation, Coriolis and wind terms (the accelerated path's scope)."""
+    """solvers/shallowater/fvm_utils.py:3-14, all five fields with the Coriolis, wind and hc
+    dissipation terms (``vol`` is the mesh's cell volume)."""
+    m = _m(domain)
+    A = _Args(m.device)
+    st = [A.dev(x, out=True) for x in (h_c, hu_c, hv_c, hc_c, Z_c)]
+    ins = [A.dev(x) for x in (rez_h, rez_hu, rez_hv, rez_hc, rez_Z, src_h, src_hu, src_hv, src_hc, src_Z,
+                              dissip_hc, corio_hu, corio_hv)]
+    _lib.call('mfv_update_sw5', m.ptr, *[t.data_ptr() for t in st], *[t.data_ptr() for t in ins], float(wind_hu),
+              float(wind_hv), float(dtime), None, stream_ptr())
+    A.finish()
+
+
+def _no_halo(halofaces):
+    if len(halofaces) != 0:
+        raise NotImplementedError("halo faces: one GPU holds one subdomain (no MPI halo path)")
+
+
+@HIPBackend.register('explicitscheme_convective_SW')
+def _convective_sw(domain, rez_h, rez_hu, rez_hv, rez_hc, rez_Z, h_c, hu_c, hv_c, hc_c, Z_c, h_ghost, hu_ghost,
+                   hv_ghost, hc_ghost, Z_ghost, h_halo, hu_halo, hv_halo, hc_halo, Z_halo, h_x, h_y, hx_halo, hy_halo,
+                   hc_x, hc_y, hcx_halo, hcy_halo, psi, psi_halo, centerc, centerf, centerh, centerg, cellidf,
+                   mesuref, normalf, halofid, innerfaces, halofaces, boundaryfaces, grav, order):
+    """solvers/shallowater/fvm_utils.py:344-496 (SRNH flux, srnh_scheme :152-342). The mesh
+    arrays are the Domain's own (its device copy is used)."""
+    _no_halo(halofaces)
+    m = _m(domain)
+    A = _Args(m.device)
+    rez = [A.dev(x, out=True) for x in (rez_h, rez_hu, rez_hv, rez_hc, rez_Z)]
+    ins = [A.dev(x) for x in (h_c, hu_c, hv_c, hc_c, Z_c, h_ghost, hu_ghost, hv_ghost, hc_ghost, Z_ghost, h_x, h_y,
+                              hc_x, hc_y, psi)]
+    _lib.call('mfv_explicitscheme_convective_sw', m.ptr, *[t.data_ptr() for t in rez], *[t.data_ptr() for t in ins],
+              float(grav), int(order), stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('term_source_srnh_SW')
+def _source_srnh(domain, src_h, src_hu, src_hv, src_hc, src_Z, h_c, hu_c, hv_c, hc_c, Z_c, h_ghost, hu_ghost,
+                 hv_ghost, hc_ghost, Z_ghost, h_halo, hu_halo, hv_halo, hc_halo, Z_halo, h_x, h_y, psi, hx_halo,
+                 hy_halo, psi_halo, nodeidc, faceidc, cellidc, cellidf, centerc, normalc, namef, centerf, centerh,
+                 vertexn, halofid, grav, order, quirk=1):
+    """solvers/shallowater/fvm_utils.py:44-148 (triangles). ``quirk=1`` reads the neighbour of
+    face slot j from ``cellidc`` (cells.cellfid) exactly as the reference does."""
     import torch
     m = _m(domain)
     A = _Args(m.device)
-    for arr in (rez_hc, rez_Z, src_hc, src_Z, dissip_hc, corio_hu, corio_hv):
-        a = arr if torch.is_tensor(arr) else torch.as_tensor(np.asarray(arr))
-        if bool((a != 0).any()):
-            raise NotImplementedError("update_SW on the device handles h, hu, hv with zero hc/Z/Coriolis terms")
-    if wind_hu != 0 or wind_hv != 0:
-        raise NotImplementedError("wind terms are not on the accelerated path")
-    hs = [A.dev(x, out=True) for x in (h_c, hu_c, hv_c)]
-    rez = torch.stack([A.dev(x) for x in (rez_h, rez_hu, rez_hv)]).contiguous()
-    src = torch.stack([A.dev(x) for x in (src_h, src_hu, src_hv)]).contiguous()
-    dt = torch.full((1,), float(dtime), dtype=torch.float64, device=m.device)
-    _lib.call('mfv_update_sw', m.ptr, *[t.data_ptr() for t in hs], rez.data_ptr(), src.data_ptr(), dt.data_ptr(),
-              stream_ptr())
+    out = [A.dev(x, out=True) for x in (src_h, src_hu, src_hv, src_hc, src_Z)]
+    ins = [A.dev(x) for x in (h_c, Z_c, h_ghost, Z_ghost)]
+    ti = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=m.device)
+    nid, cfid = ti(nodeidc), ti(cellidc)
+    nf = A.dev(np.ascontiguousarray(normalc, dtype=np.float64))
+    vx = A.dev(np.ascontiguousarray(vertexn, dtype=np.float64))
+    if nf.dim() != 3 or tuple(nf.shape[1:]) != (3, 3) or vx.dim() != 2 or vx.shape[1] != 4:
+        raise _lib.MfvError("normalc must be [C][3][3] and vertexn [N][4]")
+    _lib.call('mfv_term_source_srnh_sw', m.ptr, *[t.data_ptr() for t in out], *[t.data_ptr() for t in ins],
+              nid.data_ptr(), int(nid.shape[1]), cfid.data_ptr(), int(cfid.shape[1]), nf.data_ptr(), vx.data_ptr(),
+              float(grav), int(quirk), stream_ptr())
     A.finish()
 
 

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libmfv.so')
-SOURCES = ['mfv_core.hip', 'mfv_sw.hip', 'mfv_sw_mixed.hip', 'mfv_euler.hip', 'mfv_diamond.hip', 'mfv_krylov.hip']
+SOURCES = ['mfv_core.hip', 'mfv_sw.hip', 'mfv_sw_mixed.hip', 'mfv_srnh.hip', 'mfv_euler.hip', 'mfv_diamond.hip',
+           'mfv_krylov.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('MFV_ARCH', 'gfx950')
 

@@ -38,7 +38,8 @@ __device__ __forceinline__ double pmax(double a, double b) { return (b > a) ? b 
 // part[global wave index] -- no atomics: thousands of same-address atomics queue at the L2
 // for longer than the kernel's own work. dt_finish then folds the partials into the target
 // (out[0] = min(out[0], min part)) in one block, deterministically.
-__device__ __forceinline__ void wave_min_part(double *__restrict__ part, double v)
+// `gid` is the thread's global (logical) index, so the wave's slot is gid / 64.
+__device__ __forceinline__ void wave_min_part(double *__restrict__ part, size_t gid, double v)
 {
     double m = (v >= 0.0) ? v : __longlong_as_double(0x7ff0000000000000LL);
 #pragma unroll
@@ -46,7 +47,19 @@ __device__ __forceinline__ void wave_min_part(double *__restrict__ part, double 
         double o = __shfl_xor(m, off, 64);
         m = (o < m) ? o : m;
     }
-    if ((threadIdx.x & 63) == 0) part[((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6] = m;
+    if ((threadIdx.x & 63) == 0) part[gid >> 6] = m;
+}
+
+// XCD-aware block order. The dispatcher hands consecutive workgroups to the 8 XCDs in
+// turn and each XCD has its own L2, so with blockIdx order a cell's mesh neighbours (close
+// in cell number) are mostly processed, and cached, on other XCDs. The remap gives XCD x
+// (the blocks b with b % 8 == x) one contiguous range of logical blocks instead.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block()
+{
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int per = nb / kXcds, rem = nb % kXcds, x = b % kXcds;
+    return x * per + (x < rem ? x : rem) + b / kXcds;
 }
 
 // Waves of a launch of grid_for(n, block) blocks.

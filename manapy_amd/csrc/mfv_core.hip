@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_gradient(Mesh m, const double *
                                                            double *__restrict__ wx, double *__restrict__ wy,
                                                            double *__restrict__ wz)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     const double cx = m.cell_center[3 * i], cy = m.cell_center[3 * i + 1], cz = m.cell_center[3 * i + 2];
     const double wi = w_c[i];
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(kBlock) k_barthlimiter(Mesh m, const double *_
                                                           const double *__restrict__ wy,
                                                           const double *__restrict__ wz, double *__restrict__ psi)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     const double wi = w_c[i];
     double w_max = wi, w_min = wi;
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_time_step(Mesh m, const double *
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d = (i < m.C) ? adv_cell_dt(m, i, u[i], v[i], w[i], cfl) : 1e300;
-    wave_min_part(dtpart, d);
+    wave_min_part(dtpart, (size_t)i, d);
 }
 
 extern "C" int mfv_time_step(const mfv_mesh *m, const double *u, const double *v, const double *w, double cfl,
@@ -505,7 +505,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_residual(Mesh m, double *__restr
                                                           const double *__restrict__ gz,
                                                           const double *__restrict__ psi, double om)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     // the kernel-level entry point always evaluates the reference expression with its arrays
     rez_w[i] = adv_residual<DIM, 2, false>(m, i, w_c, w_ghost, uf, vf, wf, gx, gy, gz, psi, om);
@@ -558,7 +558,7 @@ template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_adv_grad_limit(Mesh m, const double *__restrict__ w,
                                                             double *__restrict__ scratch)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     const double cx = m.cell_center[3 * i], cy = m.cell_center[3 * i + 1], cz = m.cell_center[3 * i + 2];
     const double wi = w[i];
@@ -604,7 +604,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_step(Mesh m, const double *__res
                                                       const double *__restrict__ scratch,
                                                       DtRing ring, double *__restrict__ dtpart)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     ring_advance(ring);
     double dloc = 1e300;
     if (i < m.C) {
@@ -616,7 +616,7 @@ __global__ void __launch_bounds__(kBlock) k_adv_step(Mesh m, const double *__res
         w_out[i] = w_in[i] + dt * ((rez + 0.) / m.cell_volume[i] + 0.);
         dloc = adv_cell_dt(m, i, uc[i], vc[i], wc[i], cfl);
     }
-    wave_min_part(dtpart, dloc);
+    wave_min_part(dtpart, (size_t)i, dloc);
 }
 
 extern "C" int mfv_advection_step(const mfv_mesh *m, const double *w_in, double *w_out, const double *u_c,

@@ -412,7 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_time_step_diff(Mesh m, const double 
         }
         if (!u || lam != 0) dt = pmin(dt, cfl * vol / lam);
     }
-    wave_min_part(dtpart, dt);
+    wave_min_part(dtpart, (size_t)i, dt);
 }
 
 }  // namespace mfv

@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(kBlock) k_euler_dt(Mesh m, const double *__res
         const size_t C = m.C;
         d = euler_cell_dt(m, i, U[i], U[C + i], U[2 * C + i], U[3 * C + i], U[4 * C + i], cfl, gamma);
     }
-    wave_min_part(dtpart, d);
+    wave_min_part(dtpart, (size_t)i, d);
 }
 
 // Residual of cell i, faces in increasing id (the single all-faces loop of
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_euler_rusanov(Mesh m, const double *
                                                            const double *__restrict__ G, double gamma, int quirk,
                                                            double *__restrict__ rez)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     double r[5];
     euler_cell_residual<true>(m, i, U, G, gamma, quirk, r);
@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(kBlock) k_euler_step(Mesh m, const double *__r
                                                         double *__restrict__ Uout, double gamma, double cfl,
                                                         int quirk, DtRing ring, double *__restrict__ dtpart)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     ring_advance(ring);
     double d = 1e300;
     if (i < m.C) {
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(kBlock) k_euler_step(Mesh m, const double *__r
         for (int v = 0; v < 6; v++) Uout[v * C + i] = u[v];
         d = euler_cell_dt(m, i, u[0], u[1], u[2], u[3], u[4], cfl, gamma);
     }
-    wave_min_part(dtpart, d);
+    wave_min_part(dtpart, (size_t)i, d);
 }
 
 }  // namespace

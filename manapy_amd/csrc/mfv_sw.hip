@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_time_step(Mesh m, const double *_
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d = (i < m.C) ? sw_cell_dt(m, i, h[i], hu[i], hv[i], cfl, grav, Dxx, Dyy) : 1e300;
-    wave_min_part(dtpart, d);
+    wave_min_part(dtpart, (size_t)i, d);
 }
 
 // cell_gradient_2d + barthlimiter_2d on eta = h + Z, hu, hv with wall ghosts.
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_gradients(Mesh m, const double *_
                                                           const double *__restrict__ Z,
                                                           double *__restrict__ grad)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     const size_t C = m.C;
     const double cx = m.cell_center[3 * i], cy = m.cell_center[3 * i + 1];
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_residuals(Mesh m, const double *_
                                                           const double *__restrict__ zgrad, double g, double om,
                                                           double *__restrict__ rez, double *__restrict__ src)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     double r[3], s[3];
     sw_cell_residual(m, i, h, hu, hv, Z, grad, zgrad, g, om, r, s);
@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_step(Mesh m, const double *__rest
                                                      const double *__restrict__ zgrad, double g, double cfl,
                                                      double om, DtRing ring, double *__restrict__ dtpart)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     ring_advance(ring);
     double dloc = 1e300;
     if (i < m.C) {
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_step(Mesh m, const double *__rest
         Sout[2 * C + i] = vn;
         dloc = sw_cell_dt(m, i, hn, un, vn, cfl, g, 0., 0.);
     }
-    wave_min_part(dtpart, dloc);
+    wave_min_part(dtpart, (size_t)i, dloc);
 }
 
 }  // namespace
@@ -497,7 +497,7 @@ __device__ __forceinline__ void load_ehv(const double *__restrict__ cellrec, int
 template <int MF>
 __global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, double om)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     const size_t C = m.C;
     double cx, cy, ei, ui, vi;
@@ -625,7 +625,7 @@ template <int MF>
 __global__ void __launch_bounds__(kBlock) k_sw_flux_packed(Mesh m, mfv_sw_packed p, double g, double cfl,
                                                             DtRing ring, double *__restrict__ dtpart)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     ring_advance(ring);
     double dloc = 1e300;
     if (i < m.C) {
@@ -702,7 +702,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_flux_packed(Mesh m, mfv_sw_packed
         }
         dloc = cfl * vol / lam;
     }
-    wave_min_part(dtpart, dloc);
+    wave_min_part(dtpart, (size_t)i, dloc);
 }
 
 }  // namespace

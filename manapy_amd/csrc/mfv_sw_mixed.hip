@@ -144,7 +144,7 @@ __device__ __forceinline__ void accum_grad(float jx, float jy, float de, float d
 template <int MF, int BS = kBlock>
 __global__ void __launch_bounds__(BS) k_sw_recon32(Mesh m, mfv_sw_mixed p, float om)
 {
-    const int i = blockIdx.x * BS + threadIdx.x;
+    const int i = xcd_block() * BS + threadIdx.x;
     if (i >= m.C) return;
     const size_t C = m.C;
     const double4 *st = reinterpret_cast<const double4 *>(p.state);
@@ -314,11 +314,11 @@ template <int MF>
 __global__ void __launch_bounds__(kBlock) k_sw_flux32(Mesh m, mfv_sw_mixed p, float g, double cfl, DtRing ring,
                                                        double *__restrict__ dtpart)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
     ring_advance(ring);
     double dloc = 1e300;
     if (i < m.C) dloc = sw_flux32_cell<MF>(m, p, g, cfl, i, ring.cur[0]);
-    wave_min_part(dtpart, dloc);
+    wave_min_part(dtpart, (size_t)i, dloc);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sw_unpack32(int C, const double *__restrict__ state,
@@ -405,13 +405,13 @@ template <int MF, int BS>
 __global__ void __launch_bounds__(BS) k_sw_flux32_batch(Mesh m, mfv_sw_mixed p, float g, double cfl, BatchK b,
                                                         int kcur, double *__restrict__ part)
 {
-    const int i = blockIdx.x * BS + threadIdx.x;
+    const int i = xcd_block() * BS + threadIdx.x;
     // C is a multiple of 64: all lanes of a wave agree on i < C and on the instance
     if (i >= m.C) return;
     const int inst = b.wave_inst[i >> 6];
     if (inst < 0) return;
     const double dloc = sw_flux32_cell<MF>(m, p, g, cfl, i, b.rings[4 * (size_t)inst + kcur]);
-    wave_min_part(part, dloc);
+    wave_min_part(part, (size_t)i, dloc);
 }
 
 // One wave per instance: ring[knext] = min(ring[knext], min over the instance's waves).
@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_dt_batch(Mesh m, mfv_sw_mixed p, 
     if (inst < 0) return;
     const double4 s = reinterpret_cast<const double4 *>(p.state)[i];
     const double dloc = sw_dt32<MF>(p, i, m.cell_nfaces[i], s.x, s.y, s.z, p.vol[i], g, cfl);
-    wave_min_part(part, dloc);
+    wave_min_part(part, (size_t)i, dloc);
 }
 
 // Per-instance float64 totals sum (h, hu, hv) vol: one block per instance, fixed order.

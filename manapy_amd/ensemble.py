@@ -10,7 +10,9 @@ kernels on the actual ensemble and keeps the fastest.
 
 Each instance's arithmetic is that of the single-mesh mixed path and does not depend on
 the other instances (cell-local gathers, exact per-instance min for dt), so an instance
-gives the same bits alone and inside any ensemble.
+gives the same bits alone and inside any ensemble. By default each instance's cells are
+stored in Hilbert-curve order on the device (``renumber``); states go in and come out in
+the instance's own numbering, and the renumbering does not change any bit.
 """
 import ctypes
 import json
@@ -97,9 +99,16 @@ def ensemble_layout(domains):
 class SwEnsemble(object):
     """Device state of an ensemble; ``states`` is a list of (h, hu, hv, Z) numpy arrays."""
 
-    def __init__(self, domains, states, grav=9.81, cfl=0.8, order=2, device=None):
+    def __init__(self, domains, states, grav=9.81, cfl=0.8, order=2, device=None, renumber=True):
         torch = require_gpu()
         self.device = torch.device(device if device is not None else 'cuda')
+        if renumber:
+            from .renumber import renumbered
+            self.views = [renumbered(d) for d in domains]
+            domains = self.views
+            states = [tuple(v.to_new(np.asarray(a)) for a in st) for v, st in zip(self.views, states)]
+        else:
+            self.views = None
         host = ensemble_layout(domains)
         self.host = host
         self.n = len(domains)
@@ -221,7 +230,10 @@ class SwEnsemble(object):
     def instance_state(self, i):
         c0, n = int(self.host['inst_cell0'][i]), int(self.host['inst_ncells'][i])
         st = self.state[c0:c0 + n].cpu().numpy()  # h, hu, hv, Z
-        return st[:, 0].copy(), st[:, 1].copy(), st[:, 2].copy()
+        out = (st[:, 0].copy(), st[:, 1].copy(), st[:, 2].copy())
+        if self.views is not None:
+            out = tuple(self.views[i].to_old(a) for a in out)
+        return out
 
     def times(self):
         return self.rings[:, 3].cpu().numpy().copy()

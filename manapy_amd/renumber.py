@@ -1,0 +1,147 @@
+"""Cell renumbering along a space-filling curve for the device layout.
+
+The gather kernels read every neighbour of a cell. With the mesh generator's or a mesh
+file's numbering, neighbours can sit thousands of cells apart, so the cache lines a wave
+pulls in for its neighbours are evicted before the neighbouring waves use them. Ordering
+cells along a Hilbert curve (2D) or a Morton curve (3D) of their centres keeps a cell's
+neighbours within a few waves of it.
+
+``renumbered(domain, perm)`` is a read-only view of a Domain with the cells in the new
+order and every cell index remapped; faces keep their ids. Each cell keeps its own
+ordered lists (faces in ``cells.faceid`` order, the least-squares stencil in
+``cells.cellnid`` order), so every per-cell sum of the kernels is formed from the same
+terms in the same order and the results are bit-identical to the original numbering,
+permuted.
+"""
+import numpy as np
+
+from .base import Struct
+
+__all__ = ['hilbert_order_2d', 'morton_order_3d', 'cell_order', 'renumbered', 'RenumberedDomain']
+
+
+def _quantize(x, bits):
+    lo, hi = float(np.min(x)), float(np.max(x))
+    span = hi - lo if hi > lo else 1.0
+    q = np.floor((x - lo) / span * ((1 << bits) - 1) + 0.5)
+    return q.astype(np.int64)
+
+
+def hilbert_order_2d(x, y, bits=16):
+    """Indices that sort points by their position along a Hilbert curve of order ``bits``
+    (the iterative quadrant-rotation construction of the curve index)."""
+    xi = _quantize(np.asarray(x, dtype=np.float64), bits)
+    yi = _quantize(np.asarray(y, dtype=np.float64), bits)
+    d = np.zeros(len(xi), dtype=np.int64)
+    s = np.int64(1) << (bits - 1)
+    n = np.int64(1) << bits
+    while s > 0:
+        rx = ((xi & s) > 0).astype(np.int64)
+        ry = ((yi & s) > 0).astype(np.int64)
+        d += s * s * ((3 * rx) ^ ry)
+        # rotate the quadrant so the sub-curve is in standard orientation
+        flip = (ry == 0)
+        refl = flip & (rx == 1)
+        xi = np.where(refl, n - 1 - xi, xi)
+        yi = np.where(refl, n - 1 - yi, yi)
+        xs = np.where(flip, yi, xi)
+        ys = np.where(flip, xi, yi)
+        xi, yi = xs, ys
+        s >>= 1
+    return np.argsort(d, kind='stable')
+
+
+def _spread3(v):
+    v = v.astype(np.uint64) & np.uint64(0x1fffff)
+    v = (v | (v << np.uint64(32))) & np.uint64(0x1f00000000ffff)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x1f0000ff0000ff)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x100f00f00f00f00f)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x10c30c30c30c30c3)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x1249249249249249)
+    return v
+
+
+def morton_order_3d(x, y, z, bits=21):
+    """Indices that sort points by their Morton (Z-order) key."""
+    key = (_spread3(_quantize(x, bits)) | (_spread3(_quantize(y, bits)) << np.uint64(1)) |
+           (_spread3(_quantize(z, bits)) << np.uint64(2)))
+    return np.argsort(key, kind='stable')
+
+
+def cell_order(domain):
+    """New-to-old cell permutation of a Domain: Hilbert (2D) or Morton (3D) order of the
+    cell centres."""
+    c = domain.cells.center
+    if int(domain.dim) == 2:
+        return hilbert_order_2d(c[:, 0], c[:, 1])
+    return morton_order_3d(c[:, 0], c[:, 1], c[:, 2])
+
+
+def _remap_padded(a, iperm):
+    """Rows with the count in the last column: valid entries mapped through iperm."""
+    a = np.asarray(a)
+    out = a.copy()
+    W = a.shape[1] - 1
+    valid = np.arange(W)[None, :] < a[:, -1][:, None]
+    body = out[:, :W]
+    body[valid] = iperm[body[valid]]
+    return out
+
+
+class RenumberedDomain(object):
+    """Domain view with cells in ``perm`` order (perm[new] = old); see the module doc."""
+
+    def __init__(self, domain, perm):
+        perm = np.asarray(perm, dtype=np.int64)
+        C = int(domain.nbcells)
+        if perm.shape != (C,) or not np.array_equal(np.sort(perm), np.arange(C)):
+            raise ValueError("perm must be a permutation of the %d cells" % C)
+        iperm = np.empty(C, dtype=np.int64)
+        iperm[perm] = np.arange(C)
+        self.base = domain
+        self.perm = perm
+        self.iperm = iperm
+        self.dim = domain.dim
+        self.nbcells = domain.nbcells
+        self.nbfaces = domain.nbfaces
+        self.nbnodes = domain.nbnodes
+        self.maxcellfid = domain.maxcellfid
+        self.innerfaces = domain.innerfaces
+        self.boundaryfaces = domain.boundaryfaces
+        bc = domain.cells
+        cells = Struct()
+        cells.faceid = np.ascontiguousarray(bc.faceid[perm])
+        cells.center = np.ascontiguousarray(bc.center[perm])
+        cells.volume = np.ascontiguousarray(bc.volume[perm])
+        cells.nodeid = np.ascontiguousarray(bc.nodeid[perm])
+        cells.cellnid = _remap_padded(bc.cellnid[perm], iperm).astype(bc.cellnid.dtype)
+        cells.ghostnid = np.ascontiguousarray(bc.ghostnid[perm])
+        cells.cellfid = _remap_padded(bc.cellfid[perm], iperm).astype(bc.cellfid.dtype)
+        if hasattr(bc, 'nf'):
+            cells.nf = np.ascontiguousarray(bc.nf[perm])
+        self.cells = cells
+        bf = domain.faces
+        faces = Struct()
+        cid = np.asarray(bf.cellid)
+        faces.cellid = np.where(cid >= 0, iperm[np.maximum(cid, 0)], -1).astype(cid.dtype)
+        for k in ('name', 'normal', 'mesure', 'center', 'ghostcenter', 'nodeid'):
+            setattr(faces, k, getattr(bf, k))
+        self.faces = faces
+        self.nodes = domain.nodes
+
+    def to_new(self, a):
+        """Cell array in the original numbering -> the new numbering (axis 0 or -1)."""
+        a = np.asarray(a)
+        return np.ascontiguousarray(a[..., self.perm] if a.shape[-1] == len(self.perm) and a.ndim > 1
+                                    else a[self.perm])
+
+    def to_old(self, a):
+        """Inverse of to_new."""
+        a = np.asarray(a)
+        return np.ascontiguousarray(a[..., self.iperm] if a.shape[-1] == len(self.perm) and a.ndim > 1
+                                    else a[self.iperm])
+
+
+def renumbered(domain, perm=None):
+    """Renumbered view of ``domain`` (space-filling-curve order unless ``perm`` is given)."""
+    return RenumberedDomain(domain, cell_order(domain) if perm is None else perm)

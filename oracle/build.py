@@ -1,4 +1,4 @@
-"""ORACLE build recipe (test infrastructure): compiles the C restatements (mfv_oracle.c, mixed32.c)
+"""ORACLE build recipe (test infrastructure): compiles the C restatements (mfv_oracle.c, mixed32.c, srnh.c)
 to oracle/liboracle.so with strict IEEE semantics (no contraction, no fast-math)."""
 import os
 import subprocess
@@ -9,7 +9,7 @@ LIB = os.path.join(HERE, 'liboracle.so')
 
 def build_oracle(verbose=False):
     cmd = ['gcc', '-O2', '-fPIC', '-shared', '-std=c99', '-ffp-contract=off', '-fno-fast-math',
-           '-Wall', os.path.join(HERE, 'mfv_oracle.c'), os.path.join(HERE, 'mixed32.c'), '-lm', '-o', LIB + '.tmp']
+           '-Wall', os.path.join(HERE, 'mfv_oracle.c'), os.path.join(HERE, 'mixed32.c'), os.path.join(HERE, 'srnh.c'), '-lm', '-o', LIB + '.tmp']
     if verbose:
         print(' '.join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -193,6 +193,108 @@ def sw_step(d, h, hu, hv, Z, zgrad, grav, cfl, order):
     return dt
 
 
+# --- reference shallow-water path: SRNH flux + well-balanced source (oracle/srnh.c) ----------
+def srnh_roe(hroe, uroe, vroe, croe, grav):
+    """Eigenvalues (5,) and sign matrix (5, 5) of fvm_utils.py:220-315."""
+    lam = np.zeros(5)
+    r = np.zeros(25)
+    lib().orc_srnh_roe(_D(hroe), _D(uroe), _D(vroe), _D(croe), _D(grav), f64(lam), f64(r))
+    return lam, r.reshape(5, 5)
+
+
+def srnh_flux(hu_l, hu_r, hv_l, hv_r, h_l, h_r, hc_l, hc_r, Z_l, Z_r, normal, mesure, grav):
+    fl = np.zeros(5)
+    n = np.ascontiguousarray(normal, dtype=np.float64)
+    lib().orc_srnh_flux(*[_D(x) for x in (hu_l, hu_r, hv_l, hv_r, h_l, h_r, hc_l, hc_r, Z_l, Z_r)], f64(n),
+                        _D(mesure), _D(grav), f64(fl))
+    return fl
+
+
+SRNH_WALLS = (0, 1, 1, 0, 0)  # ghost kinds of h, hu, hv, hc, Z: Neumann / nonslip
+
+
+def srnh_ghosts(d, S, kinds=SRNH_WALLS):
+    F = int(d.nbfaces)
+    G = np.zeros((5, F))
+    for k in range(5):
+        ghost_value(d, kinds[k], np.ascontiguousarray(S[k]), G[k], d.boundaryfaces)
+    return G
+
+
+def srnh_convective(d, S, G, grads, grav, order):
+    """explicitscheme_convective_SW; grads = (h_x, h_y, hc_x, hc_y, psi). Returns rez [5][C]."""
+    C, F = int(d.nbcells), int(d.nbfaces)
+    rez = np.zeros((5, C))
+    g = [np.ascontiguousarray(x, dtype=np.float64) for x in grads]
+    lib().orc_convective_srnh(
+        f64(rez), f64(np.ascontiguousarray(S)), f64(np.ascontiguousarray(G)), *[f64(x) for x in g],
+        f64(c3(d.cells.center)), f64(c3(d.faces.center)), i32(d.faces.cellid), f64(d.faces.mesure),
+        f64(c3(d.faces.normal)), i32(d.innerfaces), _I(len(d.innerfaces)), i32(d.boundaryfaces),
+        _I(len(d.boundaryfaces)), _D(grav), _I(order), _I(C), _I(F))
+    return rez
+
+
+def srnh_source(d, S, G, grav, quirk=True):
+    """term_source_srnh_SW. Returns src [5][C] (only hu, hv non-zero). quirk=True reads the
+    neighbour of face slot j from the compacted cells.cellfid as the reference does."""
+    C = int(d.nbcells)
+    src = np.zeros((5, C))
+    nid, fid, cfid = d.cells.nodeid, d.cells.faceid, d.cells.cellfid
+    nf = np.ascontiguousarray(d.cells.nf, dtype=np.float64)
+    vx = np.ascontiguousarray(d.nodes.vertex, dtype=np.float64)
+    assert vx.shape[1] == 4 and nf.shape[1:] == (3, 3) and fid.shape[1] == cfid.shape[1]
+    lib().orc_source_srnh(f64(src[1]), f64(src[2]), f64(np.ascontiguousarray(S[0])), f64(np.ascontiguousarray(S[4])),
+                          f64(np.ascontiguousarray(G[0])), f64(np.ascontiguousarray(G[4])), i32(nid),
+                          _I(nid.shape[1]), i32(fid), i32(cfid), _I(fid.shape[1]), i32(d.faces.cellid),
+                          f64(c3(d.cells.center)), f64(nf), i32(d.faces.name), f64(vx), _D(grav), _I(int(quirk)),
+                          _I(C))
+    return src
+
+
+def srnh_update(d, S, rez, src, dt, dissip_hc=None, corio_hu=None, corio_hv=None, wind=(0., 0.)):
+    C = int(d.nbcells)
+    z = np.zeros(C)
+    dh = z if dissip_hc is None else np.ascontiguousarray(dissip_hc)
+    cu = z if corio_hu is None else np.ascontiguousarray(corio_hu)
+    cv = z if corio_hv is None else np.ascontiguousarray(corio_hv)
+    lib().orc_update_sw5(f64(S), f64(np.ascontiguousarray(rez)), f64(np.ascontiguousarray(src)), f64(dh), f64(cu),
+                         f64(cv), _D(wind[0]), _D(wind[1]), _D(dt), f64(d.cells.volume), _I(C))
+
+
+def srnh_gradients(d, S, G):
+    """h and hc gradients, each limited by its own Barth-Jespersen psi
+    (Variable.compute_cell_gradient, ast/core.py:442-456); the scheme uses hc.psi for both."""
+    C = int(d.nbcells)
+    wz = np.zeros(C)
+    out = []
+    for k in (0, 3):
+        gx, gy, psi = np.zeros(C), np.zeros(C), np.zeros(C)
+        w = np.ascontiguousarray(S[k])
+        cell_gradient(d, w, G[k], gx, gy, wz)
+        barthlimiter(d, w, G[k], gx, gy, wz, psi)
+        out.append((gx, gy, psi))
+    (hx, hy, _), (hcx, hcy, hcpsi) = out
+    return hx, hy, hcx, hcy, hcpsi
+
+
+def srnh_step(d, S, grav, cfl, order, kinds=SRNH_WALLS, quirk=True):
+    """ShallowWaterSolver: stepper -> compute_fluxes -> compute_new_val
+    (solvers/shallowater/system.py:186-281) with Dxx = Dyy = 0, no friction, Coriolis or
+    wind, in place on S [5][C] (h, hu, hv, hc, Z). Returns dt."""
+    C = int(d.nbcells)
+    dt = time_step_sw(d, np.ascontiguousarray(S[0]), np.ascontiguousarray(S[1]), np.ascontiguousarray(S[2]), cfl,
+                      grav)
+    G = srnh_ghosts(d, S, kinds)
+    if order == 2:
+        grads = srnh_gradients(d, S, G)
+    else:
+        grads = tuple(np.zeros(C) for _ in range(5))
+    rez = srnh_convective(d, S, G, grads, grav, order)
+    src = srnh_source(d, S, G, grav, quirk)
+    srnh_update(d, S, rez, src, dt)
+    return dt
+
+
 def advection_step(d, w, u_c, v_c, w_c, uf, vf, wf, cfl, order):
     """AdvectionSolver.stepper -> compute_fluxes -> compute_new_val with all-Neumann BCs
     (solvers/advec/system.py:129-147), in place. Returns dt."""

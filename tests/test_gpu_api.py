@@ -99,7 +99,7 @@ def test_shallow_water_solver(oracle_lib):
             Variable(domain=d)
         h.cell[:] = torch.as_tensor(h0)
         Z.cell[:] = torch.as_tensor(Zc)
-        S = ShallowWaterSolver(h=h, hvel=(hu, hv), Z=Z, conf=Struct(order=2, cfl=0.8))
+        S = ShallowWaterSolver(h=h, hvel=(hu, hv), Z=Z, conf=Struct(order=2, cfl=0.8, flux='rusanov'))
         if mode == "sequence":
             t = 0.
             for _ in range(30):

@@ -49,6 +49,12 @@ def test_alone_equals_mix_and_graph_equals_eager():
         for x, y in zip(mix.instance_state(j), st):
             assert np.array_equal(x, y), k
         assert mix.times()[j] == t
+    # the Hilbert relabelling of the device cells changes no bit
+    plain = SwEnsemble([doms[1]], [states[1]], renumber=False)
+    plain.step(30)
+    for x, y in zip(plain.instance_state(0), alone[1][0]):
+        assert np.array_equal(x, y)
+    assert plain.times()[0] == alone[1][1]
 
 
 def test_ensemble_error_vs_fp64_oracle_and_conservation(oracle_lib):

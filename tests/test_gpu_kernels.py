@@ -436,3 +436,51 @@ def test_sw_mixed_precision_error_and_conservation(torch, oracle_lib):
     assert max(errs.values()) <= 1e-5, errs
     assert abs(host(p.sums)[0] - mass0) <= 1e-12 * mass0
     assert abs(mass0 - np.sum(h0 * vol)) <= 1e-12 * mass0
+
+
+def test_sw_packed_and_mixed_on_hilbert_order(torch, sw_case, oracle_lib):
+    """The packed float64 path and the mixed path on the Hilbert-renumbered device mesh give
+    the original-numbering results permuted, bit for bit."""
+    from manapy_amd.device import DeviceMesh, SwPacked, SwMixed
+    from manapy_amd.renumber import renumbered
+    d, _, Z, h0 = sw_case
+    r = renumbered(d)
+    rm = DeviceMesh(r)
+    C = int(d.nbcells)
+    G = 9.81
+    h, hu, hv = h0.copy(), np.zeros(C), np.zeros(C)
+    zg_ref = oracle_lib.z_gradient(d, Z)
+    model = oracle_lib.MixedModel(d, h0, np.zeros(C), np.zeros(C), Z, zg_ref)
+    dts = [oracle_lib.sw_step(d, h, hu, hv, Z, zg_ref, G, 0.8, 2) for _ in range(30)]
+    th, tz, tZ = dev(torch, r.to_new(h0)), dev(torch, np.zeros(C)), dev(torch, r.to_new(Z))
+    zg = dev(torch, np.ascontiguousarray(zg_ref[:, r.perm]))
+    p = SwPacked(rm)
+    call('mfv_sw_pack', rm.ptr, p.ptr, P(th), P(tz), P(tz), P(tZ), P(zg), 2)
+    dt0 = dev(torch, np.zeros(1))
+    ring = dev(torch, np.zeros(4))
+    call('mfv_time_step_sw', rm.ptr, P(th), P(tz), P(tz), 0.8, G, 0., 0., P(dt0), None)
+    call('mfv_dt_ring_init', P(ring), P(dt0))
+    for s in range(30):
+        call('mfv_sw_recon', rm.ptr, p.ptr, 2)
+        call('mfv_sw_flux_packed', rm.ptr, p.ptr, G, 0.8, P(ring), s)
+    oh, ohu, ohv = (dev(torch, np.zeros(C)) for _ in range(3))
+    call('mfv_sw_unpack', rm.ptr, p.ptr, P(oh), P(ohu), P(ohv))
+    same(r.to_old(host(oh)), h, "h")
+    same(r.to_old(host(ohu)), hu, "hu")
+    same(r.to_old(host(ohv)), hv, "hv")
+    assert host(ring)[29 % 3] == dts[-1]
+    # mixed path
+    q = SwMixed(rm)
+    zg3 = dev(torch, np.vstack([zg_ref[:, r.perm], np.zeros((1, C))]))
+    call('mfv_sw_pack_mixed', rm.ptr, q.ptr, P(th), P(tz), P(tz), P(tZ), P(zg3), 2)
+    dt = model.dt(G, 0.8)
+    ring2 = dev(torch, np.zeros(4))
+    dt0[0] = dt
+    call('mfv_dt_ring_init', P(ring2), P(dt0))
+    for s in range(30):
+        dt = model.step(G, 0.8, dt)
+        call('mfv_sw_recon_mixed', rm.ptr, q.ptr, 2)
+        call('mfv_sw_flux_mixed', rm.ptr, q.ptr, G, 0.8, P(ring2), s)
+    st = host(q.state)
+    for k in range(3):
+        same(r.to_old(st[:, k]), model.state[:, k], "mixed %d" % k)
