@@ -118,9 +118,21 @@ def mixed_bytes_per_cell(MF, K_avg, G_avg):
     return recon, flux
 
 
-def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64):
+def tiled_bytes_per_cell(MF, K_avg, G_avg):
+    """Algorithmic HBM bytes per cell of the fused tiled mixed step (the halo's repeated
+    reconstruction is extra work, not extra algorithmic bytes)."""
+    return (32 + 32 + 16 + 8           # state read + write, centre, volume
+            + 4 + 1                     # face count, residual order
+            + 4 * K_avg + 4             # cellnid entries + count
+            + G_avg * (4 + 4 + 16) + 4  # ghostnid entries, owner, ghost centre + count
+            + 4 * MF + 4 * MF           # global and block-local neighbour codes
+            + 16 * MF + 8 * MF)         # geoA, geoB
+
+
+def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel='tiled', tile=256):
     """The same case on the mixed-precision path: throughput, per-kernel times and the
-    per-field error against the float64 run after the same number of steps."""
+    per-field error against the float64 run after the same number of steps. kernel='tiled'
+    runs the fused one-launch step, 'split' the reconstruction + flux kernels."""
     import torch
     from manapy_amd import _lib
     from manapy_amd.device import SwMixed, stream_ptr
@@ -143,10 +155,15 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64):
     def step(ev=None):
         if ev is not None:
             ev[0].record()
-        _lib.call('mfv_sw_recon_mixed', m.ptr, p.ptr, 2, s)
-        if ev is not None:
-            ev[1].record()
-        _lib.call('mfv_sw_flux_mixed', m.ptr, p.ptr, G, CFL, ring.data_ptr(), k[0], s)
+        if kernel == 'tiled':
+            p.step_tiled(m, G, CFL, 2, ring, k[0], T=tile, stream=s)
+            if ev is not None:
+                ev[1].record()
+        else:
+            _lib.call('mfv_sw_recon_mixed', m.ptr, p.ptr, 2, s)
+            if ev is not None:
+                ev[1].record()
+            _lib.call('mfv_sw_flux_mixed', m.ptr, p.ptr, G, CFL, ring.data_ptr(), k[0], s)
         if ev is not None:
             ev[2].record()
         k[0] += 1
@@ -177,21 +194,33 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64):
     t_flx = np.array([b.elapsed_time(c) for _, b, c in evs]) * 1e-3
     K_avg = float(d.cells.cellnid[:, -1].mean())
     G_avg = float(d.cells.ghostnid[:, -1].mean())
-    br, bf = mixed_bytes_per_cell(int(d.cells.faceid.shape[1] - 1), K_avg, G_avg)
+    MF = int(d.cells.faceid.shape[1] - 1)
     mcell = C * steps / t_loop / 1e6
-    dom = 'k_sw_flux32' if t_flx.mean() >= t_rec.mean() else 'k_sw_recon32'
-    tk, bk = (t_flx, bf) if dom == 'k_sw_flux32' else (t_rec, br)
+    if kernel == 'tiled':
+        # t_rec brackets the fused kernel, t_flx the one-block dt fold after it
+        bt = tiled_bytes_per_cell(MF, K_avg, G_avg)
+        kern = {"k_sw_step32_tiled": {"avg_us": round(float(t_rec.mean()) * 1e6, 2), "bytes_per_cell": round(bt, 1),
+                                      "GB/s": round(bt * C / t_rec.mean() / 1e9, 1), "tile": tile},
+                "k_dt_finish": {"avg_us": round(float(t_flx.mean()) * 1e6, 2)}}
+        dom, tk, bk = 'k_sw_step32_tiled', t_rec, bt
+    else:
+        br, bf = mixed_bytes_per_cell(MF, K_avg, G_avg)
+        kern = {"k_sw_recon32": {"avg_us": round(float(t_rec.mean()) * 1e6, 2), "bytes_per_cell": round(br, 1),
+                                 "GB/s": round(br * C / t_rec.mean() / 1e9, 1)},
+                "k_sw_flux32": {"avg_us": round(float(t_flx.mean()) * 1e6, 2), "bytes_per_cell": round(bf, 1),
+                                "GB/s": round(bf * C / t_flx.mean() / 1e9, 1)}}
+        dom = 'k_sw_flux32' if t_flx.mean() >= t_rec.mean() else 'k_sw_recon32'
+        tk, bk = (t_flx, bf) if dom == 'k_sw_flux32' else (t_rec, br)
     return {
         "value": round(mcell, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop / steps * 1e3, 5),
-        "speedup_vs_fp64": round(mcell / mcell_fp64, 3), "dtype": "f32 flux/gradient arithmetic, f64 state/dt/sums",
+        "kernel": kernel, "speedup_vs_fp64": round(mcell / mcell_fp64, 3),
+        "dtype": "f32 flux/gradient arithmetic, f64 state/dt/sums",
         "steps_compared": warmup + steps, "error_vs_fp64": errs,
         "mass_rel_drift": (mass1 - mass0) / mass0,
-        "kernels": {"k_sw_recon32": {"avg_us": round(float(t_rec.mean()) * 1e6, 2), "bytes_per_cell": round(br, 1),
-                                     "GB/s": round(br * C / t_rec.mean() / 1e9, 1)},
-                    "k_sw_flux32": {"avg_us": round(float(t_flx.mean()) * 1e6, 2), "bytes_per_cell": round(bf, 1),
-                                    "GB/s": round(bf * C / t_flx.mean() / 1e9, 1)}},
+        "kernels": kern,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(bk * C / tk.mean() / 1e9, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(bk * C / tk.mean() / 1e9 / HBM_PEAK_GBS, 4)},
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(bk * C / tk.mean() / 1e9 / HBM_PEAK_GBS, 4),
+                     "bytes_per_cell": round(bk, 1), "avg_launch_us": round(float(tk.mean()) * 1e6, 2)},
         "step_hbm_frac_214B": round(214.0 * C * steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
     }
 
@@ -199,14 +228,15 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64):
 def run_ensemble(n_inst, steps, seed=0, n_check=4):
     """Batched ensemble: n_inst shallow-water instances (sizes log-uniform in [5k, 200k]
     triangles, aspect ratio 1-4, 1-10 bumps, random dam) advanced together by a captured graph
-    with the autotuned block size; error of sampled instances against their float64 runs."""
+    with the autotuned launch configuration; error of sampled instances against their float64
+    runs."""
     import torch
     from manapy_amd import _lib
     from manapy_amd.ddm import Domain
     from manapy_amd.device import DeviceMesh, SwPacked, stream_ptr
     from manapy_amd.ensemble import SwEnsemble, random_instance, instance_state
-    if steps % 3:
-        raise ValueError("ensemble steps must be a multiple of 3")
+    if steps % 6:
+        raise ValueError("ensemble steps must be a multiple of 6 (dt ring period 3, two state buffers)")
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
     sizes = np.exp(rng.uniform(np.log(5e3), np.log(2e5), n_inst))
@@ -222,8 +252,8 @@ def run_ensemble(n_inst, steps, seed=0, n_check=4):
     log("ensemble: uploading %d instances" % n_inst)
     e = SwEnsemble(doms, states)
     tune = e.autotune(steps=6)
-    log("ensemble: block %d" % tune['block'])
-    e.step(3)
+    log("ensemble: launch configuration %s" % (tune['config'],))
+    e.step(6)
     torch.cuda.synchronize()
     g = e.graph(steps)
     t1 = time.perf_counter()
@@ -232,7 +262,7 @@ def run_ensemble(n_inst, steps, seed=0, n_check=4):
     t_loop = time.perf_counter() - t1
     cells = int(sum(int(d.nbcells) for d in doms))
     mcell = cells * steps / t_loop / 1e6
-    # float64 reference for a sample of instances (same initial state, same 3 + steps steps)
+    # float64 reference for a sample of instances (same initial state, same 6 + steps steps)
     errs = []
     pick = np.linspace(0, n_inst - 1, n_check).astype(int)
     s = stream_ptr()
@@ -255,7 +285,7 @@ def run_ensemble(n_inst, steps, seed=0, n_check=4):
         _lib.call('mfv_time_step_sw', m.ptr, th.data_ptr(), thu.data_ptr(), thv.data_ptr(), CFL, G, 0., 0.,
                   dt0.data_ptr(), None, s)
         _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
-        for k in range(3 + steps):
+        for k in range(6 + steps):
             _lib.call('mfv_sw_recon', m.ptr, pk.ptr, 2, s)
             _lib.call('mfv_sw_flux_packed', m.ptr, pk.ptr, G, CFL, ring.data_ptr(), k, s)
         _lib.call('mfv_sw_unpack', m.ptr, pk.ptr, th.data_ptr(), thu.data_ptr(), thv.data_ptr(), s)
@@ -268,7 +298,7 @@ def run_ensemble(n_inst, steps, seed=0, n_check=4):
     return {
         "value": round(mcell, 3), "unit": "Mcell-updates/s", "instances": n_inst, "cells": cells,
         "steps": steps, "ms_per_step": round(t_loop / steps * 1e3, 4), "graph": "one captured graph of all steps",
-        "block": int(tune['block']), "autotune_ms_per_step": tune['ms_per_step'],
+        "config": list(tune['config']), "autotune_ms_per_step": tune['ms_per_step'],
         "step_hbm_frac_214B": round(214.0 * cells * steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
         "error_vs_fp64_sampled": errs, "max_rel_L1": max(max(x.values()) for x in errs),
         "mass_totals_finite": bool(np.all(np.isfinite(sums))), "build_s": round(t_build, 1),
@@ -346,8 +376,9 @@ def main():
                     help='packed: per-cell records + per-face MUSCL values (default); unpacked: SoA kernels')
     ap.add_argument('--no-mixed', action='store_true', help='skip the mixed-precision run of the same case')
     ap.add_argument('--ensemble', type=int, default=256, help='instances of the ensemble run (0: skip)')
-    ap.add_argument('--ensemble-steps', type=int, default=201)
+    ap.add_argument('--ensemble-steps', type=int, default=204)
     ap.add_argument('--serving', type=int, default=32, help='requests of the serving run (0: skip)')
+    ap.add_argument('--tile', type=int, choices=(256, 512), default=256, help='cells per tile of the fused step')
     ap.add_argument('--renumber', choices=('hilbert', 'none'), default='hilbert',
                     help='device cell order: Hilbert curve of the cell centres (default) or the mesh numbering')
     ap.add_argument('--copy-calibration', action='store_true',
@@ -508,8 +539,11 @@ def main():
     }
     log("fp64: %.1f Mcell-updates/s" % mcell)
     if not args.no_mixed:
-        out["mixed"] = run_mixed(d, m, Z, h0, zgrad, args.warmup, args.steps, fp64_state, mcell)
-        log("mixed: %.1f Mcell-updates/s" % out["mixed"]["value"])
+        out["mixed"] = run_mixed(d, m, Z, h0, zgrad, args.warmup, args.steps, fp64_state, mcell, 'tiled',
+                                 args.tile)
+        log("mixed (fused tiled step): %.1f Mcell-updates/s" % out["mixed"]["value"])
+        out["mixed_split"] = run_mixed(d, m, Z, h0, zgrad, args.warmup, args.steps, fp64_state, mcell, 'split')
+        log("mixed (two kernels): %.1f Mcell-updates/s" % out["mixed_split"]["value"])
     if args.ensemble:
         out["ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps)
         log("ensemble: %.1f Mcell-updates/s" % out["ensemble"]["value"])

@@ -254,6 +254,33 @@ int mfv_sw_flux_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, double grav, dou
 int mfv_sw_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, double *partial, int n_partial,
                         double *out3, void *stream);
 
+/* Tiles of the fused mixed step. Tile t is the cell run tile_cell0[t] .. + tile_ncells[t] - 1
+ * (at most tile_size cells; a multiple of 64 cells except possibly the mesh's last tile),
+ * processed by one block of tile_size threads. Its halo -- the face neighbours outside the
+ * tile -- is halo_cells[halo_ptr[t] .. halo_ptr[t+1]). lnbr [MF][C] is nbr with the cell part
+ * replaced by the neighbour's slot in the tile's record array: cell - tile_cell0 inside the
+ * tile, tile_ncells + halo position outside. (tile_size + max_halo) * MF * 16 bytes of LDS
+ * must not exceed 64 KiB. */
+typedef struct mfv_sw_tiles {
+    int32_t nb_tiles;
+    int32_t tile_size;          /* 256 or 512 */
+    int32_t max_halo;
+    int32_t pad_;
+    const int32_t *tile_cell0;  /* [nb_tiles] */
+    const int32_t *tile_ncells; /* [nb_tiles] */
+    const int32_t *halo_ptr;    /* [nb_tiles + 1] */
+    const int32_t *halo_cells;  /* [halo_ptr[nb_tiles]] */
+    const int32_t *lnbr;        /* [MF][C] */
+} mfv_sw_tiles;
+
+/* One fused mixed step (reconstruction, fluxes, update, next dt) from p->state into
+ * state_out (a second [C][4] buffer); the face records stay in LDS. Gives the same bits as
+ * mfv_sw_recon_mixed followed by mfv_sw_flux_mixed. No counterpart call in the reference:
+ * it replaces the pair above (explicitscheme_convective_SW + update_SW in one launch). */
+int mfv_sw_step_mixed_tiled(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_tiles *t,
+                            double *state_out, double grav, double cfl, int order, double *dt_ring,
+                            int step, void *stream);
+
 /* Ensemble of independent shallow-water instances concatenated into one mesh (mfv_mesh +
  * mfv_sw_mixed over all cells). Instance i owns cells inst_cell0[i] .. + inst_ncells[i] - 1 with
  * inst_cell0[i] a multiple of 64; cells up to the next multiple of 64 are padding (no faces,
@@ -281,6 +308,15 @@ int mfv_sw_batch_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, const mf
  * each mfv_graph_launch advances every instance by nsteps steps. */
 int mfv_sw_batch_graph(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b, double grav,
                        double cfl, int order, int nsteps, int block, void **graph_out);
+/* Fused tiled ensemble step (tiles never straddle instances) from p->state into state_out,
+ * and a graph of nsteps (a multiple of 6) such steps alternating p->state and state2, which
+ * leaves the result in p->state. */
+int mfv_sw_batch_step_tiled(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b,
+                            const mfv_sw_tiles *t, double *state_out, double grav, double cfl,
+                            int order, int step, void *stream);
+int mfv_sw_batch_graph_tiled(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b,
+                             const mfv_sw_tiles *t, double *state2, double grav, double cfl, int order,
+                             int nsteps, void **graph_out);
 int mfv_graph_launch(void *graph, void *stream);
 int mfv_graph_destroy(void *graph);
 

@@ -33,6 +33,13 @@ class MfvMesh(ctypes.Structure):
     ]
 
 
+class MfvSwTiles(ctypes.Structure):
+    """Mirror of ``struct mfv_sw_tiles`` (include/mfv.h)."""
+    _fields_ = [('nb_tiles', ctypes.c_int32), ('tile_size', ctypes.c_int32), ('max_halo', ctypes.c_int32),
+                ('pad_', ctypes.c_int32), ('tile_cell0', P), ('tile_ncells', P), ('halo_ptr', P),
+                ('halo_cells', P), ('lnbr', P)]
+
+
 class MfvSwPacked(ctypes.Structure):
     """Mirror of ``struct mfv_sw_packed`` (include/mfv.h)."""
     _fields_ = [('nb_cells', ctypes.c_int32), ('max_cell_faces', ctypes.c_int32), ('cellrec', P), ('facerec', P),
@@ -88,6 +95,7 @@ KW_P = ctypes.POINTER(MfvKrylovWork)
 SWB_P = ctypes.POINTER(MfvSwBatch)
 SWP_P = ctypes.POINTER(MfvSwPacked)
 SWM_P = ctypes.POINTER(MfvSwMixed)
+SWT_P = ctypes.POINTER(MfvSwTiles)
 DP = ctypes.POINTER(ctypes.c_double)
 
 # name -> argtypes (restype is int unless listed in _RESTYPE)
@@ -136,6 +144,10 @@ _SIGNATURES = {
     'mfv_time_step_diffusion': [MESH_P, P, P, P, c_double, c_double, c_double, c_double, P, DP, P],
     'mfv_sw_batch_dt_init': [MESH_P, SWM_P, SWB_P, c_double, c_double, P],
     'mfv_sw_batch_step': [MESH_P, SWM_P, SWB_P, c_double, c_double, c_int, c_int, c_int, P],
+    'mfv_sw_step_mixed_tiled': [MESH_P, SWM_P, SWT_P, P, c_double, c_double, c_int, P, c_int, P],
+    'mfv_sw_batch_step_tiled': [MESH_P, SWM_P, SWB_P, SWT_P, P, c_double, c_double, c_int, c_int, P],
+    'mfv_sw_batch_graph_tiled': [MESH_P, SWM_P, SWB_P, SWT_P, P, c_double, c_double, c_int, c_int,
+                                 ctypes.POINTER(c_void_p)],
     'mfv_sw_batch_conservation': [MESH_P, SWM_P, SWB_P, P, P],
     'mfv_sw_batch_graph': [MESH_P, SWM_P, SWB_P, c_double, c_double, c_int, c_int, c_int, ctypes.POINTER(c_void_p)],
     'mfv_graph_launch': [c_void_p, P],

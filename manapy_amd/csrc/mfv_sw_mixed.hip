@@ -139,15 +139,14 @@ __device__ __forceinline__ void accum_grad(float jx, float jy, float de, float d
     jv1 += jy * dv;
 }
 
-// Reconstruction: float32 LSQ gradients of eta, hu, hv from float64 differences, Barth-Jespersen
-// limiter on deviations, face records (a, hu_f, hv_f, w) -- see the file comment.
-template <int MF, int BS = kBlock>
-__global__ void __launch_bounds__(BS) k_sw_recon32(Mesh m, mfv_sw_mixed p, float om)
+// Reconstruction of cell i: float32 LSQ gradients of eta, hu, hv from float64 differences,
+// Barth-Jespersen limiter on deviations, face records (a, hu_f, hv_f, w) of its nf faces in
+// out[0..nf) -- see the file comment. st is the state the step starts from.
+template <int MF>
+__device__ __forceinline__ int recon32_cell(const Mesh &m, const mfv_sw_mixed &p, const double4 *__restrict__ st,
+                                            int i, float om, float4 *out)
 {
-    const int i = xcd_block() * BS + threadIdx.x;
-    if (i >= m.C) return;
     const size_t C = m.C;
-    const double4 *st = reinterpret_cast<const double4 *>(p.state);
     const double4 si = st[i];
     const double hi = si.x, ei = si.x + si.w, ui = si.y, vi = si.z;
     const double cx = m.cell_center[3 * (size_t)i], cy = m.cell_center[3 * (size_t)i + 1];
@@ -177,11 +176,14 @@ __global__ void __launch_bounds__(BS) k_sw_recon32(Mesh m, mfv_sw_mixed p, float
     const int nf = m.cell_nfaces[i];
     float emax = 0.f, emin = 0.f, umax = 0.f, umin = 0.f, vmax = 0.f, vmin = 0.f;  // bounds as deviations
     float dh[MF];
+    int codes[MF];
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         dh[s] = 0.f;
+        codes[s] = -1;
         if (s >= nf) continue;
         const int code = p.nbr[(size_t)s * C + i];
+        codes[s] = code;
         float eb, ub, vb;
         if (code >= 0) {
             const double4 sb = st[code >> 3];
@@ -222,15 +224,28 @@ __global__ void __launch_bounds__(BS) k_sw_recon32(Mesh m, mfv_sw_mixed p, float
         if (s >= nf) break;
         const float dzo = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s].w;
         const float de = om * pe * (ex * rx[s] + ey * ry[s]);
-        const int code = p.nbr[(size_t)s * C + i];
-        const bool right = code >= 0 && (code & 1);
+        const bool right = codes[s] >= 0 && (codes[s] & 1);
         float4 o;
         o.x = right ? dh[s] + (de - dzo) : de - dzo;
         o.y = hu32 + om * pu * (ux * rx[s] + uy * ry[s]);
         o.z = hv32 + om * pv * (vx * rx[s] + vy * ry[s]);
         o.w = right ? dh[s] : h32;
-        reinterpret_cast<float4 *>(p.facerec)[MF * (size_t)i + s] = o;
+        out[s] = o;
     }
+    return nf;
+}
+
+template <int MF, int BS = kBlock>
+__global__ void __launch_bounds__(BS) k_sw_recon32(Mesh m, mfv_sw_mixed p, float om)
+{
+    const int i = xcd_block() * BS + threadIdx.x;
+    if (i >= m.C) return;
+    float4 rec[MF];
+    const int nf = recon32_cell<MF>(m, p, reinterpret_cast<const double4 *>(p.state), i, om, rec);
+    float4 *o = reinterpret_cast<float4 *>(p.facerec) + MF * (size_t)i;
+#pragma unroll
+    for (int s = 0; s < MF; s++)
+        if (s < nf) o[s] = rec[s];
 }
 
 // time_step_SW of a float64 state with the float32 face normals (mesure = |n|), faces in
@@ -252,17 +267,21 @@ __device__ __forceinline__ double sw_dt32(const mfv_sw_mixed &p, int i, int nf, 
     return cfl * vol / lam;
 }
 
-// One cell of the step: compensated float32 residual, float64 update, next-dt candidate.
-template <int MF>
-__device__ __forceinline__ double sw_flux32_cell(const Mesh &m, const mfv_sw_mixed &p, float g, double cfl, int i,
-                                                 double dt)
+// Face records of the flux gather: rec(k) is record k (= MF * cell + slot) of the array the
+// neighbour codes index: the global facerec, or a block's LDS copy with block-local codes.
+struct GlobalRecs {
+    const float4 *__restrict__ fr;
+    __device__ __forceinline__ float4 operator()(size_t k) const { return fr[k]; }
+};
+
+// One cell of the step: compensated float32 residual, float64 update into *out, next-dt
+// candidate. own: this cell's index in the record array; code_of(s): neighbour code of slot s
+// in the record array's numbering.
+template <int MF, class Recs, class Codes>
+__device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int own, const double4 si, int nf,
+                                              unsigned ord, Recs rec, Codes code_of, float g, double cfl,
+                                              double dt, double4 *out)
 {
-    const size_t C = m.C;
-    double4 *st = reinterpret_cast<double4 *>(p.state);
-    const double4 si = st[i];
-    const int nf = m.cell_nfaces[i];
-    const unsigned ord = p.order[i];
-    const float4 *fr = reinterpret_cast<const float4 *>(p.facerec);
     float r0 = 0.f, c0 = 0.f, r1 = 0.f, c1 = 0.f, r2 = 0.f, c2 = 0.f, s1 = 0.f, d1 = 0.f, s2 = 0.f, d2 = 0.f;
 #pragma unroll
     for (int r = 0; r < MF; r++) {
@@ -270,19 +289,19 @@ __device__ __forceinline__ double sw_flux32_cell(const Mesh &m, const mfv_sw_mix
         const int s = (ord >> (2 * r)) & 3;
         const float4 ga = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
         const float mes = sqrtf(ga.x * ga.x + ga.y * ga.y);
-        const float4 own = fr[MF * (size_t)i + s];
-        const int code = p.nbr[(size_t)s * C + i];
+        const float4 ownr = rec(MF * (size_t)own + s);
+        const int code = code_of(s);
         float F0, F1, F2, bo;
         if (code < 0) {
-            face_dev(own.w, own.x, own.x, 0.f, own.y, own.z, -own.y, -own.z, ga.w, ga.x, ga.y, mes, g, 0.f, true, F0,
-                     F1, F2, bo);
+            face_dev(ownr.w, ownr.x, ownr.x, 0.f, ownr.y, ownr.z, -ownr.y, -ownr.z, ga.w, ga.x, ga.y, mes, g, 0.f,
+                     true, F0, F1, F2, bo);
         } else {
-            const float4 oth = fr[MF * (size_t)(code >> 3) + ((code >> 1) & 3)];
+            const float4 oth = rec(MF * (size_t)(code >> 3) + ((code >> 1) & 3));
             if ((code & 1) == 0)
-                face_dev(own.w, own.x, oth.x, ga.z, own.y, own.z, oth.y, oth.z, ga.w, ga.x, ga.y, mes, g, 0.f, true,
-                         F0, F1, F2, bo);
+                face_dev(ownr.w, ownr.x, oth.x, ga.z, ownr.y, ownr.z, oth.y, oth.z, ga.w, ga.x, ga.y, mes, g, 0.f,
+                         true, F0, F1, F2, bo);
             else
-                face_dev(oth.w, oth.x, own.x, ga.z, oth.y, oth.z, own.y, own.z, ga.w, ga.x, ga.y, mes, g, -own.w,
+                face_dev(oth.w, oth.x, ownr.x, ga.z, oth.y, oth.z, ownr.y, ownr.z, ga.w, ga.x, ga.y, mes, g, -ownr.w,
                          false, F0, F1, F2, bo);
         }
         if (code < 0 || (code & 1) == 0) {
@@ -306,8 +325,21 @@ __device__ __forceinline__ double sw_flux32_cell(const Mesh &m, const mfv_sw_mix
     const double hn = si.x + dt * R0 / vol;
     const double un = si.y + dt * (R1 / vol);
     const double vn = si.z + dt * (R2 / vol);
-    st[i] = make_double4(hn, un, vn, si.w);
+    *out = make_double4(hn, un, vn, si.w);
     return sw_dt32<MF>(p, i, nf, hn, un, vn, vol, g, cfl);
+}
+
+// The cell step on the global records, in place on p.state.
+template <int MF>
+__device__ __forceinline__ double sw_flux32_cell(const Mesh &m, const mfv_sw_mixed &p, float g, double cfl, int i,
+                                                 double dt)
+{
+    const size_t C = m.C;
+    double4 *st = reinterpret_cast<double4 *>(p.state);
+    const int32_t *__restrict__ nbr = p.nbr;
+    auto code_of = [nbr, C, i](int s) { return nbr[(size_t)s * C + i]; };
+    return flux32_core<MF>(p, i, i, st[i], m.cell_nfaces[i], p.order[i],
+                           GlobalRecs{reinterpret_cast<const float4 *>(p.facerec)}, code_of, g, cfl, dt, st + i);
 }
 
 template <int MF>
@@ -567,6 +599,105 @@ BatchK to_batch(const mfv_sw_batch *b)
     return BatchK{b->nb_inst, b->nb_waves, b->wave_inst, b->inst_cell0, b->inst_ncells, b->rings};
 }
 
+// ---------------------------------------------------------------------------------------------
+// Fused step on tiles. A block takes one tile (a run of at most T cells) and first
+// reconstructs the face records of its cells and of its halo (the face neighbours outside the
+// tile) into LDS, then evaluates the fluxes and the update of its cells from LDS. The face
+// records never go to HBM; the halo's reconstruction is computed by both tiles that need it.
+// Every record is computed by recon32_cell and every update by flux32_core, the functions of
+// the two-kernel path, so the bits are the same. The state is double-buffered (the step reads
+// p.state and writes out) because neighbouring tiles read the old state while a tile writes
+// the new one.
+
+struct TilesK {
+    int n, T, H;
+    const int32_t *__restrict__ cell0;
+    const int32_t *__restrict__ ncells;
+    const int32_t *__restrict__ hptr;
+    const int32_t *__restrict__ halo;
+    const int32_t *__restrict__ lnbr;
+};
+
+template <int MF, int T, bool BATCH>
+__global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, TilesK t, double4 *__restrict__ out,
+                                                       float g, double cfl, float om, DtRing ring, BatchK b,
+                                                       int kcur, double *__restrict__ part)
+{
+    extern __shared__ float4 lfr[];  // [(T + t.H) * MF] face records of the tile, then of its halo
+    const int tile = xcd_block();
+    const int c0 = t.cell0[tile], nc = t.ncells[tile];
+    const int h0 = t.hptr[tile], nh = t.hptr[tile + 1] - h0;
+    const double4 *__restrict__ st = reinterpret_cast<const double4 *>(p.state);
+    if (!BATCH) ring_advance(ring);
+    for (int q = threadIdx.x; q < nc + nh; q += T) {
+        const int c = (q < nc) ? c0 + q : t.halo[h0 + (q - nc)];
+        float4 rec[MF];
+        const int nf = recon32_cell<MF>(m, p, st, c, om, rec);
+#pragma unroll
+        for (int s = 0; s < MF; s++)
+            if (s < nf) lfr[MF * q + s] = rec[s];
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if ((q & ~63) >= nc) return;  // the whole wave lies past the tile: it owns no partial
+    double dloc = 1e300;
+    if (q < nc) {
+        const int i = c0 + q;
+        const double dt = BATCH ? b.rings[4 * (size_t)b.wave_inst[i >> 6] + kcur] : ring.cur[0];
+        const int32_t *__restrict__ lnbr = t.lnbr;
+        const size_t C = m.C;
+        auto code_of = [lnbr, C, i](int s) { return lnbr[(size_t)s * C + i]; };
+        auto recs = [](size_t k) { return lfr[k]; };
+        dloc = flux32_core<MF>(p, i, q, st[i], m.cell_nfaces[i], p.order[i], recs, code_of, g, cfl, dt, out + i);
+    }
+    wave_min_part(part, (size_t)c0 + q, dloc);
+}
+
+int check_tiles(const mfv_mesh *m, const mfv_sw_tiles *t)
+{
+    MFV_REQUIRE(t != nullptr, "tiles descriptor is NULL");
+    MFV_REQUIRE(t->tile_size == 256 || t->tile_size == 512, "tile_size %d not in {256, 512}", t->tile_size);
+    MFV_REQUIRE(t->nb_tiles >= 1 && t->max_halo >= 0, "bad tile counts");
+    MFV_REQUIRE(t->tile_cell0 && t->tile_ncells && t->halo_ptr && t->halo_cells && t->lnbr,
+                "tiles descriptor has NULL arrays");
+    const size_t lds = (size_t)(t->tile_size + t->max_halo) * m->max_cell_faces * sizeof(float4);
+    MFV_REQUIRE(lds <= 64 * 1024, "tile of %d cells with a halo of %d needs %zu bytes of LDS (> 64 KiB): "
+                "renumber the mesh or use smaller tiles", t->tile_size, t->max_halo, lds);
+    return MFV_OK;
+}
+
+TilesK to_tiles(const mfv_sw_tiles *t)
+{
+    return TilesK{t->nb_tiles, t->tile_size, t->max_halo, t->tile_cell0, t->tile_ncells,
+                  t->halo_ptr, t->halo_cells, t->lnbr};
+}
+
+template <int MF, int T, bool BATCH>
+void launch_tiled(const Mesh &mk, const mfv_sw_mixed &p, const TilesK &tk, double *out, float g, double cfl,
+                  float om, DtRing r, const BatchK &bk, int kcur, double *part, hipStream_t st)
+{
+    const size_t lds = (size_t)(T + tk.H) * MF * sizeof(float4);
+    hipLaunchKernelGGL((k_sw_step32_tiled<MF, T, BATCH>), dim3(tk.n), dim3(T), lds, st, mk, p, tk,
+                       reinterpret_cast<double4 *>(out), g, cfl, om, r, bk, kcur, part);
+}
+
+template <bool BATCH>
+int tiled_step(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_tiles *t, double *out, float g, double cfl,
+               float om, DtRing r, const BatchK &bk, int kcur, double *part, hipStream_t st)
+{
+    const Mesh mk = to_kernel(m);
+    const TilesK tk = to_tiles(t);
+    const bool four = m->max_cell_faces == 4;
+    if (t->tile_size == 256) {
+        if (four) launch_tiled<4, 256, BATCH>(mk, *p, tk, out, g, cfl, om, r, bk, kcur, part, st);
+        else launch_tiled<3, 256, BATCH>(mk, *p, tk, out, g, cfl, om, r, bk, kcur, part, st);
+    } else {
+        if (four) launch_tiled<4, 512, BATCH>(mk, *p, tk, out, g, cfl, om, r, bk, kcur, part, st);
+        else launch_tiled<3, 512, BATCH>(mk, *p, tk, out, g, cfl, om, r, bk, kcur, part, st);
+    }
+    return check_launch("k_sw_step32_tiled");
+}
+
 }  // namespace
 
 extern "C" int mfv_sw_pack_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, const double *h, const double *hu,
@@ -749,5 +880,102 @@ extern "C" int mfv_graph_launch(void *graph, void *stream)
 extern "C" int mfv_graph_destroy(void *graph)
 {
     graph_free(static_cast<MfvGraph *>(graph));
+    return MFV_OK;
+}
+
+extern "C" int mfv_sw_step_mixed_tiled(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_tiles *t,
+                                       double *state_out, double grav, double cfl, int order, double *dt_ring_p,
+                                       int step, void *stream)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_mixed(m, p)) return rc;
+    if (int rc = check_tiles(m, t)) return rc;
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    MFV_REQUIRE(dt_ring_p && state_out && state_out != p->state, "state_out must be a second state buffer");
+    if (m->nb_cells == 0) return MFV_OK;
+    DtRing r = dt_ring(dt_ring_p, step);
+    const size_t nw = ((size_t)m->nb_cells + 63) / 64;
+    double *part = dt_scratch(nw, as_stream(stream));
+    if (!part) return MFV_ENOMEM;
+    const BatchK none{0, 0, nullptr, nullptr, nullptr, nullptr};
+    if (int rc = tiled_step<false>(m, p, t, state_out, (float)grav, cfl, (float)(order - 1), r, none, 0, part,
+                                   as_stream(stream)))
+        return rc;
+    return dt_finish(part, nw, r.next, as_stream(stream));
+}
+
+static int batch_tiled_launch(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b,
+                              const mfv_sw_tiles *t, double *out, float g, double cfl, float om, int step,
+                              double *part, hipStream_t st)
+{
+    const int k = ((step % 3) + 3) % 3;
+    const BatchK bk = to_batch(b);
+    const DtRing none{};
+    if (int rc = tiled_step<true>(m, p, t, out, g, cfl, om, none, bk, k, part, st)) return rc;
+    hipLaunchKernelGGL(k_batch_dt_fold, dim3(fold_grid(b->nb_inst)), dim3(256), 0, st, part, bk, k, (k + 1) % 3,
+                       (k + 2) % 3, 1);
+    return check_launch("k_batch_dt_fold");
+}
+
+extern "C" int mfv_sw_batch_step_tiled(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b,
+                                       const mfv_sw_tiles *t, double *state_out, double grav, double cfl,
+                                       int order, int step, void *stream)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_batch(m, p, b)) return rc;
+    if (int rc = check_tiles(m, t)) return rc;
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    MFV_REQUIRE(state_out && state_out != p->state, "state_out must be a second state buffer");
+    if (m->nb_cells == 0) return MFV_OK;
+    double *part = dt_scratch((size_t)m->nb_cells / 64, as_stream(stream));
+    if (!part) return MFV_ENOMEM;
+    return batch_tiled_launch(m, p, b, t, state_out, (float)grav, cfl, (float)(order - 1), step, part,
+                              as_stream(stream));
+}
+
+extern "C" int mfv_sw_batch_graph_tiled(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_batch *b,
+                                        const mfv_sw_tiles *t, double *state2, double grav, double cfl, int order,
+                                        int nsteps, void **graph_out)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_batch(m, p, b)) return rc;
+    if (int rc = check_tiles(m, t)) return rc;
+    MFV_REQUIRE(graph_out, "graph_out is NULL");
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    MFV_REQUIRE(state2 && state2 != p->state, "state2 must be a second state buffer");
+    MFV_REQUIRE(nsteps >= 6 && nsteps % 6 == 0 && nsteps <= 3000, "nsteps %d must be a multiple of 6 in [6, 3000]",
+                nsteps);
+    *graph_out = nullptr;
+    MfvGraph *g = new MfvGraph();
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&g->part), ((size_t)m->nb_cells / 64 + 1) * sizeof(double));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        set_error("graph capture setup: %s", hipGetErrorString(e));
+        graph_free(g);
+        return MFV_EHIP;
+    }
+    // steps alternate the two state buffers; nsteps is even, so a replay ends in p->state
+    mfv_sw_mixed pa = *p, pb = *p;
+    pb.state = state2;
+    int rc = MFV_OK;
+    for (int k = 0; k < nsteps && rc == MFV_OK; k++) {
+        const mfv_sw_mixed &pin = (k % 2 == 0) ? pa : pb;
+        double *out = (k % 2 == 0) ? state2 : p->state;
+        rc = batch_tiled_launch(m, &pin, b, t, out, (float)grav, cfl, (float)(order - 1), k, g->part, g->stream);
+    }
+    e = hipStreamEndCapture(g->stream, &g->graph);
+    if (rc != MFV_OK || e != hipSuccess) {
+        if (rc == MFV_OK) set_error("graph capture: %s", hipGetErrorString(e));
+        graph_free(g);
+        return rc != MFV_OK ? rc : MFV_EHIP;
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        set_error("graph instantiate: %s", hipGetErrorString(e));
+        graph_free(g);
+        return MFV_EHIP;
+    }
+    *graph_out = g;
     return MFV_OK;
 }

@@ -245,6 +245,68 @@ class SwPacked(object):
         return ctypes.byref(self.desc)
 
 
+def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None):
+    """Tiles of the fused mixed step (mfv_sw_tiles in include/mfv.h) from the neighbour codes
+    nbr [MF][C]: by default consecutive runs of T cells; halo lists and block-local codes."""
+    nbr = np.asarray(nbr)
+    MF = nbr.shape[0]
+    if tile_cell0 is None:
+        tile_cell0 = np.arange(0, C, T, dtype=np.int64)
+        tile_ncells = np.minimum(T, C - tile_cell0)
+    tile_cell0 = np.asarray(tile_cell0, dtype=np.int64)
+    tile_ncells = np.asarray(tile_ncells, dtype=np.int64)
+    if np.any(tile_ncells > T) or np.any(tile_ncells <= 0):
+        raise ValueError("tiles must hold 1..T cells")
+    nt = len(tile_cell0)
+    tile_of = np.full(C, -1, dtype=np.int64)
+    for_tile = np.repeat(np.arange(nt), tile_ncells)
+    cells = np.concatenate([np.arange(a, a + n) for a, n in zip(tile_cell0, tile_ncells)]) if nt else np.zeros(0, int)
+    tile_of[cells] = for_tile
+    code = nbr.astype(np.int64)                               # [MF][C]
+    valid = code >= 0
+    j = np.where(valid, code >> 3, 0)
+    ti = np.broadcast_to(tile_of[None, :], code.shape)       # tile of the cell owning the slot
+    covered = ti >= 0
+    outside = valid & covered & (tile_of[j] != ti)
+    # halo of each tile: the distinct outside neighbours, ascending
+    key = np.unique(ti[outside] * np.int64(C) + j[outside])
+    h_tile = key // C
+    h_cell = key % C
+    halo_ptr = np.zeros(nt + 1, dtype=np.int64)
+    np.add.at(halo_ptr, h_tile + 1, 1)
+    halo_ptr = np.cumsum(halo_ptr)
+    local = np.where(valid, j - tile_cell0[np.maximum(ti, 0)], 0)
+    pos = np.searchsorted(key, ti * np.int64(C) + j)
+    local = np.where(outside, tile_ncells[np.maximum(ti, 0)] + (pos - halo_ptr[np.maximum(ti, 0)]), local)
+    lnbr = np.where(valid, (local << 3) | (code & 7), -1)
+    max_halo = int(np.max(np.diff(halo_ptr))) if nt else 0
+    return {'T': int(T), 'nb_tiles': nt, 'max_halo': max_halo, 'MF': MF,
+            'tile_cell0': tile_cell0.astype(np.int32), 'tile_ncells': tile_ncells.astype(np.int32),
+            'halo_ptr': halo_ptr.astype(np.int32), 'halo_cells': h_cell.astype(np.int32),
+            'lnbr': np.ascontiguousarray(lnbr, dtype=np.int32)}
+
+
+class SwTiles(object):
+    """Device copy of a tiles layout and its mfv_sw_tiles descriptor."""
+
+    def __init__(self, host, device):
+        import torch
+        self.host = host
+        self.t = {k: torch.from_numpy(np.ascontiguousarray(host[k])).to(device)
+                  for k in ('tile_cell0', 'tile_ncells', 'halo_ptr', 'halo_cells', 'lnbr')}
+        if host['halo_cells'].size == 0:
+            self.t['halo_cells'] = torch.zeros(1, dtype=torch.int32, device=device)
+        d = _lib.MfvSwTiles()
+        d.nb_tiles, d.tile_size, d.max_halo = host['nb_tiles'], host['T'], host['max_halo']
+        for k in ('tile_cell0', 'tile_ncells', 'halo_ptr', 'halo_cells', 'lnbr'):
+            setattr(d, k, self.t[k].data_ptr())
+        self.desc = d
+
+    @property
+    def ptr(self):
+        return ctypes.byref(self.desc)
+
+
 class SwMixed(object):
     """Device tables of the mixed-precision shallow-water path (mfv_sw_mixed)."""
 
@@ -256,6 +318,8 @@ class SwMixed(object):
         dev = dmesh.device
         f32 = torch.float32
         self.state = torch.zeros((C, 4), dtype=torch.float64, device=dev)
+        self.state2 = torch.zeros((C, 4), dtype=torch.float64, device=dev)  # fused step's second buffer
+        self.nbr_host = host['nbr']
         self.vol = dmesh.t['cell_volume']
         self.facerec = torch.zeros((C, MF, 4), dtype=f32, device=dev)
         self.geoA = torch.zeros((C, MF, 4), dtype=f32, device=dev)
@@ -270,6 +334,21 @@ class SwMixed(object):
         for k in ('state', 'vol', 'facerec', 'geoA', 'geoB', 'nbr', 'order'):
             setattr(d, k, getattr(self, k).data_ptr())
         self.desc = d
+        self._tiles = {}
+
+    def tiles(self, T=256):
+        """The fused step's tiles of T cells (built on first use)."""
+        if T not in self._tiles:
+            self._tiles[T] = SwTiles(sw_tiles_layout(self.nbr_host, self.desc.nb_cells, T), self.state.device)
+        return self._tiles[T]
+
+    def step_tiled(self, dmesh, grav, cfl, order, ring, step, T=256, stream=None):
+        """One fused step: reads the current state buffer, writes the other, swaps them."""
+        _lib.call('mfv_sw_step_mixed_tiled', dmesh.ptr, self.ptr, self.tiles(T).ptr, self.state2.data_ptr(),
+                  float(grav), float(cfl), int(order), ring.data_ptr(), int(step),
+                  stream if stream is not None else stream_ptr())
+        self.state, self.state2 = self.state2, self.state
+        self.desc.state = self.state.data_ptr()
 
     @property
     def ptr(self):

@@ -25,6 +25,10 @@ from . import _lib
 from .device import gpu_layout, require_gpu, stream_ptr, sw_packed_layout
 
 BLOCKS = (64, 128, 256, 512)
+TILES = (256, 512)
+# launch configurations the autotuner compares: the two-kernel step with its block size, or
+# the fused tiled step with its tile size
+CONFIGS = tuple(('split', b) for b in BLOCKS) + tuple(('tiled', t) for t in TILES)
 
 
 def _cat_cols(arrs, pads, fill, offsets=None):
@@ -96,6 +100,19 @@ def ensemble_layout(domains):
     return out
 
 
+def ensemble_tiles(host, T=256):
+    """Tiles of the fused step for an ensemble layout: runs of at most T cells that start at
+    an instance's first cell and never cross into the next instance."""
+    from .device import sw_tiles_layout
+    c0s, ns = [], []
+    for a, n in zip(host['inst_cell0'], host['inst_ncells']):
+        npad = (int(n) + 63) // 64 * 64
+        for off in range(0, npad, T):
+            c0s.append(int(a) + off)
+            ns.append(min(T, npad - off))
+    return sw_tiles_layout(host['nbr'], host['C'], T, np.array(c0s), np.array(ns))
+
+
 class SwEnsemble(object):
     """Device state of an ensemble; ``states`` is a list of (h, hu, hv, Z) numpy arrays."""
 
@@ -128,6 +145,7 @@ class SwEnsemble(object):
         self.mesh = m
         f32 = torch.float32
         self.state = torch.zeros((self.C, 4), dtype=torch.float64, device=dev)
+        self.state2 = torch.zeros((self.C, 4), dtype=torch.float64, device=dev)
         self.facerec = torch.zeros((self.C, self.MF, 4), dtype=f32, device=dev)
         self.geoA = torch.zeros((self.C, self.MF, 4), dtype=f32, device=dev)
         self.geoB = torch.zeros((self.C, self.MF, 2), dtype=f32, device=dev)
@@ -144,7 +162,8 @@ class SwEnsemble(object):
         b.wave_inst, b.inst_cell0 = self.t['wave_inst'].data_ptr(), self.t['inst_cell0'].data_ptr()
         b.inst_ncells, b.rings = self.t['inst_ncells'].data_ptr(), self.rings.data_ptr()
         self.batch = b
-        self.block = 256
+        self.config = ('tiled', 256)
+        self._tiles = {}
         self.step_index = 0
         self._graph = None
         self._graph_key = None
@@ -180,31 +199,61 @@ class SwEnsemble(object):
         self.step_index = 0
 
     # -- stepping -------------------------------------------------------------------------------
-    def step(self, nsteps=1, block=None):
+    @property
+    def block(self):
+        return self.config[1]
+
+    def tiles(self, T):
+        if T not in self._tiles:
+            from .device import SwTiles
+            self._tiles[T] = SwTiles(ensemble_tiles(self.host, T), self.device)
+        return self._tiles[T]
+
+    def _swap(self):
+        self.state, self.state2 = self.state2, self.state
+        self.mixed.state = self.state.data_ptr()
+
+    def step(self, nsteps=1, config=None):
+        """Eager steps with ``config`` = ('split', block) or ('tiled', T) (default: self.config)."""
+        kind, size = config or self.config
         s = stream_ptr()
-        blk = int(block or self.block)
         for _ in range(nsteps):
-            _lib.call('mfv_sw_batch_step', ctypes.byref(self.mesh), ctypes.byref(self.mixed), ctypes.byref(self.batch),
-                      self.grav, self.cfl, self.order, self.step_index, blk, s)
+            if kind == 'split':
+                _lib.call('mfv_sw_batch_step', ctypes.byref(self.mesh), ctypes.byref(self.mixed),
+                          ctypes.byref(self.batch), self.grav, self.cfl, self.order, self.step_index, int(size), s)
+            else:
+                _lib.call('mfv_sw_batch_step_tiled', ctypes.byref(self.mesh), ctypes.byref(self.mixed),
+                          ctypes.byref(self.batch), self.tiles(int(size)).ptr, self.state2.data_ptr(), self.grav,
+                          self.cfl, self.order, self.step_index, s)
+                self._swap()
             self.step_index += 1
 
-    def graph(self, nsteps, block=None):
-        """The captured graph of ``nsteps`` steps (a multiple of 3), built on first use."""
-        key = (int(nsteps), int(block or self.block))
+    def graph(self, nsteps, config=None):
+        """The captured graph of ``nsteps`` steps (a multiple of 6), built on first use. A
+        replay ends in the state buffer it started from."""
+        key = (int(nsteps),) + tuple(config or self.config) + (self.state.data_ptr(),)
         if self._graph is None or self._graph_key != key:
             self.close_graph()
             h = ctypes.c_void_p()
-            _lib.call('mfv_sw_batch_graph', ctypes.byref(self.mesh), ctypes.byref(self.mixed), ctypes.byref(self.batch),
-                      self.grav, self.cfl, self.order, key[0], key[1], ctypes.byref(h))
+            if key[1] == 'split':
+                _lib.call('mfv_sw_batch_graph', ctypes.byref(self.mesh), ctypes.byref(self.mixed),
+                          ctypes.byref(self.batch), self.grav, self.cfl, self.order, key[0], key[2], ctypes.byref(h))
+            else:
+                _lib.call('mfv_sw_batch_graph_tiled', ctypes.byref(self.mesh), ctypes.byref(self.mixed),
+                          ctypes.byref(self.batch), self.tiles(key[2]).ptr, self.state2.data_ptr(), self.grav,
+                          self.cfl, self.order, key[0], ctypes.byref(h))
             self._graph, self._graph_key = h, key
+            self._graph_state = self.state.data_ptr()
         return self._graph
 
-    def run_graph(self, nsteps, chunk=30, block=None):
+    def run_graph(self, nsteps, chunk=30, config=None):
         """Advance ``nsteps`` steps by replaying a ``chunk``-step graph (steps must stay
-        aligned with the 3-slot dt ring)."""
-        if self.step_index % 3 or nsteps % chunk or chunk % 3:
-            raise ValueError("graph replays need step index, nsteps and chunk multiples of 3 (and of chunk)")
-        g = self.graph(chunk, block)
+        aligned with the 3-slot dt ring and the two state buffers)."""
+        if self.step_index % 6 or nsteps % chunk or chunk % 6:
+            raise ValueError("graph replays need step index, nsteps and chunk multiples of 6 (and of chunk)")
+        g = self.graph(chunk, config)
+        if self._graph_state != self.state.data_ptr():
+            raise RuntimeError("the graph was captured on the other state buffer")
         s = stream_ptr()
         for _ in range(nsteps // chunk):
             _lib.call('mfv_graph_launch', g, s)
@@ -240,8 +289,9 @@ class SwEnsemble(object):
 
     # -- tuning ---------------------------------------------------------------------------------
     def autotune(self, steps=6, cache=None):
-        """Times every block size on ``steps`` steps of this ensemble (state restored after)
-        and keeps the fastest; results are cached by (cells, instances) in ``cache`` (JSON)."""
+        """Times every launch configuration (CONFIGS) on ``steps`` steps of this ensemble (state
+        restored after) and keeps the fastest; results are cached by (cells, instances) in
+        ``cache`` (JSON)."""
         import torch
         key = "sw_ensemble:C=%d:I=%d:MF=%d" % (self.C, self.n, self.MF)
         path = cache or os.environ.get('MFV_TUNE_CACHE')
@@ -250,26 +300,26 @@ class SwEnsemble(object):
             with open(path) as fh:
                 table = json.load(fh)
         if key in table:
-            self.block = int(table[key]['block'])
+            self.config = tuple(table[key]['config'])
             return table[key]
-        saved = (self.state.clone(), self.facerec.clone(), self.rings.clone(), self.step_index)
+        saved = (self.state.clone(), self.rings.clone(), self.step_index)
         st = torch.cuda.current_stream()
         res = {}
-        for blk in BLOCKS:
-            self.step(3, blk)  # warm-up, aligned to the ring period
+        for cfg in CONFIGS:
+            self.step(6, cfg)  # warm-up, aligned to the ring period and the state buffers
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(st)
-            self.step(steps, blk)
+            self.step(steps, cfg)
             e1.record(st)
             e1.synchronize()
-            res[blk] = e0.elapsed_time(e1) / steps
+            res[cfg] = e0.elapsed_time(e1) / steps
         self.state.copy_(saved[0])
-        self.facerec.copy_(saved[1])
-        self.rings.copy_(saved[2])
-        self.step_index = saved[3]
+        self.rings.copy_(saved[1])
+        self.step_index = saved[2]
         best = min(res, key=res.get)
-        self.block = best
-        entry = {'block': best, 'ms_per_step': {str(k): v for k, v in res.items()}, 'when': time.time()}
+        self.config = best
+        entry = {'config': list(best), 'ms_per_step': {"%s:%d" % k: v for k, v in res.items()},
+                 'when': time.time()}
         if path:
             table[key] = entry
             os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)

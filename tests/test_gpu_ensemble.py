@@ -21,22 +21,24 @@ def _instances():
     return doms, states
 
 
-def test_alone_equals_mix_and_graph_equals_eager():
+@pytest.mark.parametrize("config", [("tiled", 256), ("split", 128), ("tiled", 512)])
+def test_alone_equals_mix_and_graph_equals_eager(config):
+    """Fused tiled and two-kernel steps, eager and in graphs, alone and mixed: all the same bits."""
     from manapy_amd.ensemble import SwEnsemble
     doms, states = _instances()
     alone = []
     for d, s in zip(doms, states):
         e = SwEnsemble([d], [s])
-        e.step(30)
+        e.step(36)
         alone.append((e.instance_state(0), e.times()[0]))
     order = [2, 0, 3, 1]
     mix = SwEnsemble([doms[k] for k in order], [states[k] for k in order])
-    mix.block = 128
+    mix.config = config
     eager = SwEnsemble([doms[k] for k in order], [states[k] for k in order])
-    # graph: two replays of 15 steps, eager steps in between on another ensemble
+    # graph: two replays of 18 steps, eager steps in between on another ensemble
     for rep in range(2):
-        mix.run_graph(15, chunk=15)
-        eager.step(15, block=64)
+        mix.run_graph(18, chunk=18)
+        eager.step(18, ("split", 64) if config[0] == "tiled" else ("tiled", 256))
         import torch
         torch.cuda.synchronize()
         for j in range(len(order)):
@@ -51,7 +53,7 @@ def test_alone_equals_mix_and_graph_equals_eager():
         assert mix.times()[j] == t
     # the Hilbert relabelling of the device cells changes no bit
     plain = SwEnsemble([doms[1]], [states[1]], renumber=False)
-    plain.step(30)
+    plain.step(36, ("split", 256))
     for x, y in zip(plain.instance_state(0), alone[1][0]):
         assert np.array_equal(x, y)
     assert plain.times()[0] == alone[1][1]
@@ -91,9 +93,10 @@ def test_autotune_restores_state(tmp_path):
     e.step(3)
     before = [e.instance_state(i) for i in range(e.n)]
     entry = e.autotune(steps=3, cache=str(tmp_path / "tune.json"))
-    assert entry['block'] in (64, 128, 256, 512)
+    from manapy_amd.ensemble import CONFIGS
+    assert tuple(entry['config']) in CONFIGS
     for i in range(e.n):
         for x, y in zip(e.instance_state(i), before[i]):
             assert np.array_equal(x, y)
     e2 = SwEnsemble(doms, states)
-    assert e2.autotune(cache=str(tmp_path / "tune.json"))['block'] == entry['block']
+    assert e2.autotune(cache=str(tmp_path / "tune.json"))['config'] == entry['config']

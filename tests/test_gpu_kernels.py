@@ -484,3 +484,41 @@ def test_sw_packed_and_mixed_on_hilbert_order(torch, sw_case, oracle_lib):
     st = host(q.state)
     for k in range(3):
         same(r.to_old(st[:, k]), model.state[:, k], "mixed %d" % k)
+
+
+@pytest.mark.parametrize("T,hilbert", [(256, True), (512, True), (256, False)])
+def test_sw_mixed_tiled_step_equals_two_kernel_step(torch, oracle_lib, T, hilbert):
+    """The fused tiled step (face records in LDS, double-buffered state) gives the bits of
+    reconstruction + flux kernels, step after step, including the dt ring."""
+    from manapy_amd.device import DeviceMesh, SwMixed
+    from manapy_amd.meshgen import gaussian_bumps
+    from manapy_amd.renumber import renumbered
+    d0 = make_tri_domain(45, 37, 100., 80., 0.1, 21)
+    d = renumbered(d0) if hilbert else d0
+    C = int(d.nbcells)
+    c = d.cells.center
+    Z = gaussian_bumps(c, 6, np.random.default_rng(3), [(0., 100.), (0., 80.)])
+    h0 = np.where(c[:, 0] < 40., 2.0, 1.0)
+    zg = oracle_lib.z_gradient(d, Z)
+    m = DeviceMesh(d)
+    runs = []
+    for tiled in (False, True):
+        p = SwMixed(m)
+        th, tz, tZ, tg = (dev(torch, x) for x in (h0, np.zeros(C), Z, zg))
+        call('mfv_sw_pack_mixed', m.ptr, p.ptr, P(th), P(tz), P(tz), P(tZ), P(tg), 2)
+        dt0 = dev(torch, np.zeros(1))
+        ring = dev(torch, np.zeros(4))
+        call('mfv_time_step_sw', m.ptr, P(th), P(tz), P(tz), 0.8, 9.81, 0., 0., P(dt0), None)
+        call('mfv_dt_ring_init', P(ring), P(dt0))
+        rings = []
+        for s in range(25):
+            if tiled:
+                p.step_tiled(m, 9.81, 0.8, 2, ring, s, T=T)
+            else:
+                call('mfv_sw_recon_mixed', m.ptr, p.ptr, 2)
+                call('mfv_sw_flux_mixed', m.ptr, p.ptr, 9.81, 0.8, P(ring), s)
+            rings.append(host(ring).copy())
+        runs.append((host(p.state), rings))
+    same(runs[1][0], runs[0][0], "tiled state")
+    for a, b in zip(runs[0][1], runs[1][1]):
+        same(b, a, "dt ring")

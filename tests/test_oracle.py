@@ -335,6 +335,38 @@ def test_euler_rusanov_consistency_and_quirk(oracle_lib):
         del UR_qr
 
 
+def test_euler_rusanov_q1_wave_speed_from_the_mass_flux(oracle_lib):
+    """Q1 (fvm_utils3d.py:842): the right state's normal momentum in the wave speed is
+    rhou_R n0 + rhov_L n1 + rhow_L n2. The dissipation speed S actually used is recovered
+    from the mass flux, F_rho = (1/2 (q_L + q_R) - 1/2 S (rho_R - rho_L)) |f| with the physical
+    mass fluxes q = rho u . n, and must equal max |q/rho - c|, |q/rho|, |q/rho + c| over the
+    left state and the (quirky or true) right state."""
+    rng = np.random.default_rng(11)
+    differs = 0
+    for _ in range(200):
+        UL, UR = _rand_state(rng), _rand_state(rng)
+        if abs(UR[0] - UL[0]) < 0.1:
+            continue
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        mes = rng.uniform(0.1, 2)
+        qL = UL[2] * n[0] + UL[3] * n[1] + UL[4] * n[2]
+        qR = UR[2] * n[0] + UR[3] * n[1] + UR[4] * n[2]
+        qR_q1 = UR[2] * n[0] + UL[3] * n[1] + UL[4] * n[2]
+        cL, cR = np.sqrt(GAMMA * UL[1] / UL[0]), np.sqrt(GAMMA * UR[1] / UR[0])
+
+        def speed(q, rho, c):
+            return max(abs(q / rho - c), abs(q / rho), abs(q / rho + c))
+        for quirk, qr in ((1, qR_q1), (0, qR)):
+            f = _flux(UL, UR, n, mes, quirk)
+            S = (0.5 * (qL + qR) - f[0] / mes) * 2. / (UR[0] - UL[0])
+            want = max(speed(qL, UL[0], cL), speed(qr, UR[0], cR))
+            assert abs(S - want) <= 1e-9 * want, (quirk, S, want)
+        if max(speed(qL, UL[0], cL), speed(qR_q1, UR[0], cR)) != max(speed(qL, UL[0], cL), speed(qR, UR[0], cR)):
+            differs += 1
+    assert differs > 20  # the check sees the quirk
+
+
 def test_euler_ghost_slip(oracle_lib, dom3d):
     d = dom3d
     rng = np.random.default_rng(2)
