@@ -27,6 +27,21 @@ METRIC = "Mcell-updates/s + %peak HBM GB/s at stated FP64 error; p50/p99 solve l
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(kernel, cells):
+    """HBM bytes per launch of ``kernel`` on this workload from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (scaled by the copy calibration), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'hbm_traffic_sw_dam_break_2M.json')
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if data.get('cells') != cells:
+        return None
+    k = data.get('kernels', {}).get(kernel)
+    return int(k['bytes']) if k else None
+
+
 def log(msg):
     """Progress on stderr (the JSON result line is the only stdout output)."""
     print("[bench] " + msg, file=sys.stderr, flush=True)
@@ -129,10 +144,11 @@ def tiled_bytes_per_cell(MF, K_avg, G_avg):
             + 16 * MF + 8 * MF)         # geoA, geoB
 
 
-def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel='tiled', tile=256):
+def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel='auto', tile=256):
     """The same case on the mixed-precision path: throughput, per-kernel times and the
-    per-field error against the float64 run after the same number of steps. kernel='tiled'
-    runs the fused one-launch step, 'split' the reconstruction + flux kernels."""
+    per-field error against the float64 run after the same number of steps. kernel='tiled' /
+    'staged' run the fused one-launch step (states from HBM / staged in LDS), 'split' the
+    reconstruction + flux kernels, 'auto' the fastest of them on this mesh."""
     import torch
     from manapy_amd import _lib
     from manapy_amd.device import SwMixed, stream_ptr
@@ -151,12 +167,16 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel=
               None, s)
     _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
     k = [0]
+    tune = None
+    if kernel == 'auto':
+        (kernel, tile), tune = p.autotune(m, G, CFL, 2, ring)
+        log("mixed: autotuned %s %d %s" % (kernel, tile, {"%s:%d" % c: round(v, 4) for c, v in tune.items()}))
 
     def step(ev=None):
         if ev is not None:
             ev[0].record()
-        if kernel == 'tiled':
-            p.step_tiled(m, G, CFL, 2, ring, k[0], T=tile, stream=s)
+        if kernel in ('tiled', 'staged'):
+            p.step_tiled(m, G, CFL, 2, ring, k[0], T=tile, stage=(kernel == 'staged'), stream=s)
             if ev is not None:
                 ev[1].record()
         else:
@@ -196,11 +216,12 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel=
     G_avg = float(d.cells.ghostnid[:, -1].mean())
     MF = int(d.cells.faceid.shape[1] - 1)
     mcell = C * steps / t_loop / 1e6
-    if kernel == 'tiled':
+    if kernel in ('tiled', 'staged'):
         # t_rec brackets the fused kernel, t_flx the one-block dt fold after it
         bt = tiled_bytes_per_cell(MF, K_avg, G_avg)
         kern = {"k_sw_step32_tiled": {"avg_us": round(float(t_rec.mean()) * 1e6, 2), "bytes_per_cell": round(bt, 1),
-                                      "GB/s": round(bt * C / t_rec.mean() / 1e9, 1), "tile": tile},
+                                      "GB/s": round(bt * C / t_rec.mean() / 1e9, 1), "tile": tile,
+                                      "states_in_lds": kernel == 'staged'},
                 "k_dt_finish": {"avg_us": round(float(t_flx.mean()) * 1e6, 2)}}
         dom, tk, bk = 'k_sw_step32_tiled', t_rec, bt
     else:
@@ -213,13 +234,16 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel=
         tk, bk = (t_flx, bf) if dom == 'k_sw_flux32' else (t_rec, br)
     return {
         "value": round(mcell, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop / steps * 1e3, 5),
-        "kernel": kernel, "speedup_vs_fp64": round(mcell / mcell_fp64, 3),
+        "kernel": kernel, "tile": tile if kernel != 'split' else None,
+        "autotune_ms_per_step": {"%s:%d" % c: round(v, 4) for c, v in tune.items()} if tune else None,
+        "speedup_vs_fp64": round(mcell / mcell_fp64, 3),
         "dtype": "f32 flux/gradient arithmetic, f64 state/dt/sums",
         "steps_compared": warmup + steps, "error_vs_fp64": errs,
         "mass_rel_drift": (mass1 - mass0) / mass0,
         "kernels": kern,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(bk * C / tk.mean() / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(bk * C / tk.mean() / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(dom if kernel == 'split' else "%s:%s:%d" % (dom, kernel, tile), C),
                      "bytes_per_cell": round(bk, 1), "avg_launch_us": round(float(tk.mean()) * 1e6, 2)},
         "step_hbm_frac_214B": round(214.0 * C * steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
     }
@@ -378,7 +402,9 @@ def main():
     ap.add_argument('--ensemble', type=int, default=256, help='instances of the ensemble run (0: skip)')
     ap.add_argument('--ensemble-steps', type=int, default=204)
     ap.add_argument('--serving', type=int, default=32, help='requests of the serving run (0: skip)')
-    ap.add_argument('--tile', type=int, choices=(256, 512), default=256, help='cells per tile of the fused step')
+    ap.add_argument('--tile', type=int, choices=(128, 256, 512), default=256, help='cells per tile of the fused step')
+    ap.add_argument('--mixed-kernel', choices=('auto', 'split', 'tiled', 'staged'), default='auto',
+                    help='mixed-precision step: autotuned (default), two kernels, fused, fused with LDS-staged states')
     ap.add_argument('--renumber', choices=('hilbert', 'none'), default='hilbert',
                     help='device cell order: Hilbert curve of the cell centres (default) or the mesh numbering')
     ap.add_argument('--copy-calibration', action='store_true',
@@ -528,7 +554,8 @@ def main():
                    "path": args.path, "cell_order": args.renumber},
         "roofline": {"bound": "hbm", "kernel": names[1] if dom_is_flux else names[0],
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(names[1] if dom_is_flux else names[0], C),
                      "bytes_per_cell": round(bpc, 1), "avg_launch_us": round(float(tk.mean()) * 1e6, 2)},
         "kernels": {names[0]: {"avg_us": round(float(t_grad.mean()) * 1e6, 2),
                                "bytes_per_cell": round(bg, 1), "GB/s": round(ach_grad, 1)},
@@ -539,9 +566,9 @@ def main():
     }
     log("fp64: %.1f Mcell-updates/s" % mcell)
     if not args.no_mixed:
-        out["mixed"] = run_mixed(d, m, Z, h0, zgrad, args.warmup, args.steps, fp64_state, mcell, 'tiled',
+        out["mixed"] = run_mixed(d, m, Z, h0, zgrad, args.warmup, args.steps, fp64_state, mcell, args.mixed_kernel,
                                  args.tile)
-        log("mixed (fused tiled step): %.1f Mcell-updates/s" % out["mixed"]["value"])
+        log("mixed (%s): %.1f Mcell-updates/s" % (out["mixed"]["kernel"], out["mixed"]["value"]))
         out["mixed_split"] = run_mixed(d, m, Z, h0, zgrad, args.warmup, args.steps, fp64_state, mcell, 'split')
         log("mixed (two kernels): %.1f Mcell-updates/s" % out["mixed_split"]["value"])
     if args.ensemble:

@@ -256,21 +256,32 @@ int mfv_sw_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, double *partia
 
 /* Tiles of the fused mixed step. Tile t is the cell run tile_cell0[t] .. + tile_ncells[t] - 1
  * (at most tile_size cells; a multiple of 64 cells except possibly the mesh's last tile),
- * processed by one block of tile_size threads. Its halo -- the face neighbours outside the
- * tile -- is halo_cells[halo_ptr[t] .. halo_ptr[t+1]). lnbr [MF][C] is nbr with the cell part
- * replaced by the neighbour's slot in the tile's record array: cell - tile_cell0 inside the
- * tile, tile_ncells + halo position outside. (tile_size + max_halo) * MF * 16 bytes of LDS
- * must not exceed 64 KiB. */
+ * processed by one block of tile_size threads. Its items are its cells followed by its halo
+ * (the tile_nhalo[t] face neighbours outside it); its ext list is its items followed by every
+ * other cell their reconstructions read (stencil cells, ghost owners, face neighbours):
+ * the halo and the extra cells are ext_cells[ext_ptr[t] .. ext_ptr[t+1]). Item q of tile t
+ * is row item_ptr[t] + q of the per-item tables lst [K][nb_items] (stencil), lgh
+ * [G][nb_items] (ghost owners) and lfn [MF][nb_items] (face neighbours, -1 on walls), which
+ * hold positions in the ext list. lnbr [MF][C] is nbr with the cell part replaced by the
+ * neighbour's position in the item list. LDS per block: (tile_size + max_halo) * MF * 16
+ * bytes, plus max_ext * 48 bytes with stage_states; at most 64 KiB. */
 typedef struct mfv_sw_tiles {
     int32_t nb_tiles;
-    int32_t tile_size;          /* 256 or 512 */
+    int32_t tile_size;          /* 128, 256 or 512 */
     int32_t max_halo;
-    int32_t pad_;
+    int32_t max_ext;
+    int32_t nb_items;
+    int32_t stage_states;       /* 1: copy every cell state a tile reads into LDS first */
     const int32_t *tile_cell0;  /* [nb_tiles] */
     const int32_t *tile_ncells; /* [nb_tiles] */
-    const int32_t *halo_ptr;    /* [nb_tiles + 1] */
-    const int32_t *halo_cells;  /* [halo_ptr[nb_tiles]] */
+    const int32_t *tile_nhalo;  /* [nb_tiles] */
+    const int32_t *ext_ptr;     /* [nb_tiles + 1] */
+    const int32_t *ext_cells;   /* [ext_ptr[nb_tiles]] */
+    const int32_t *item_ptr;    /* [nb_tiles + 1] */
     const int32_t *lnbr;        /* [MF][C] */
+    const uint16_t *lst;        /* [K][nb_items] */
+    const uint16_t *lgh;        /* [G][nb_items] */
+    const int16_t *lfn;         /* [MF][nb_items] */
 } mfv_sw_tiles;
 
 /* One fused mixed step (reconstruction, fluxes, update, next dt) from p->state into

@@ -36,8 +36,9 @@ class MfvMesh(ctypes.Structure):
 class MfvSwTiles(ctypes.Structure):
     """Mirror of ``struct mfv_sw_tiles`` (include/mfv.h)."""
     _fields_ = [('nb_tiles', ctypes.c_int32), ('tile_size', ctypes.c_int32), ('max_halo', ctypes.c_int32),
-                ('pad_', ctypes.c_int32), ('tile_cell0', P), ('tile_ncells', P), ('halo_ptr', P),
-                ('halo_cells', P), ('lnbr', P)]
+                ('max_ext', ctypes.c_int32), ('nb_items', ctypes.c_int32), ('stage_states', ctypes.c_int32),
+                ('tile_cell0', P), ('tile_ncells', P), ('tile_nhalo', P), ('ext_ptr', P), ('ext_cells', P),
+                ('item_ptr', P), ('lnbr', P), ('lst', P), ('lgh', P), ('lfn', P)]
 
 
 class MfvSwPacked(ctypes.Structure):

@@ -493,8 +493,11 @@ __device__ __forceinline__ void load_ehv(const double *__restrict__ cellrec, int
 }
 
 // Gradients, limiter and MUSCL face values (k_sw_gradients + the reconstruction of
-// sw_reconstruct, written per face slot).
-template <int MF>
+// sw_reconstruct, written per face slot). KMAX > 0: all stencil indices are loaded first and
+// the neighbour records in independent groups of four, so a cell waits for a few memory
+// latencies instead of two per stencil entry; the sums keep the stencil order. KMAX = 0:
+// plain loop (stencils wider than the largest KMAX).
+template <int MF, int KMAX>
 __global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, double om)
 {
     int i = xcd_block() * blockDim.x + threadIdx.x;
@@ -505,10 +508,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, do
     double ixx = 0., iyy = 0., ixy = 0.;
     double je0 = 0., je1 = 0., ju0 = 0., ju1 = 0., jv0 = 0., jv1 = 0.;
     const int nn = m.lsq_ncells[i];
-    for (int j = 0; j < nn; j++) {
-        int c = m.lsq_cells[(size_t)j * C + i];
-        double px, py, ec, uc, vc;
-        load_ehv(p.cellrec, c, px, py, ec, uc, vc);
+    auto accum = [&](double px, double py, double ec, double uc, double vc) {
         double jx = px - cx, jy = py - cy;
         ixx += jx * jx;
         iyy += jy * jy;
@@ -520,6 +520,28 @@ __global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, do
         ju1 += (jy * du);
         jv0 += (jx * dv);
         jv1 += (jy * dv);
+    };
+    if (KMAX > 0) {
+        int cidx[KMAX > 0 ? KMAX : 1];
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) cidx[j] = (j < nn) ? m.lsq_cells[(size_t)j * C + i] : i;
+#pragma unroll
+        for (int j0 = 0; j0 < KMAX; j0 += 4) {
+            double px[4], py[4], ec[4], uc[4], vc[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (j0 + k < nn) load_ehv(p.cellrec, cidx[j0 + k], px[k], py[k], ec[k], uc[k], vc[k]);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (j0 + k < nn) accum(px[k], py[k], ec[k], uc[k], vc[k]);
+        }
+    } else {
+        for (int j = 0; j < nn; j++) {
+            int c = m.lsq_cells[(size_t)j * C + i];
+            double px, py, ec, uc, vc;
+            load_ehv(p.cellrec, c, px, py, ec, uc, vc);
+            accum(px, py, ec, uc, vc);
+        }
     }
     const int ng = m.lsq_nghosts[i];
     for (int j = 0; j < ng; j++) {
@@ -752,7 +774,12 @@ extern "C" int mfv_sw_recon(const mfv_mesh *m, const mfv_sw_packed *p, int order
     MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
     MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
     if (m->nb_cells == 0) return MFV_OK;
-    auto k = (m->max_cell_faces == 3) ? k_sw_recon<3> : k_sw_recon<4>;
+    const bool tri = m->max_cell_faces == 3;
+    const int K = m->lsq_width;
+    auto k = (K <= 12)   ? (tri ? k_sw_recon<3, 12> : k_sw_recon<4, 12>)
+             : (K <= 16) ? (tri ? k_sw_recon<3, 16> : k_sw_recon<4, 16>)
+             : (K <= 24) ? (tri ? k_sw_recon<3, 24> : k_sw_recon<4, 24>)
+                         : (tri ? k_sw_recon<3, 0> : k_sw_recon<4, 0>);
     hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p,
                        (double)(order - 1));
     return check_launch("k_sw_recon");

@@ -64,15 +64,18 @@ __device__ __forceinline__ void face_dev(float hb, float aL, float aR, float dz,
     }
     float uL = 0.f, vL = 0.f, uR = 0.f, vR = 0.f;
     if (hL > 1e-12f) {
-        uL = huL / hL;
-        vL = hvL / hL;
+        const float r = 1.f / hL;
+        uL = huL * r;
+        vL = hvL * r;
     }
     if (hR > 1e-12f) {
-        uR = huR / hR;
-        vR = hvR / hR;
+        const float r = 1.f / hR;
+        uR = huR * r;
+        vR = hvR * r;
     }
     const float qL = uL * n0 + vL * n1, qR = uR * n0 + vR * n1;
-    const float sL = fabsf(qL) / mes + sqrtf(g * hsL), sR = fabsf(qR) / mes + sqrtf(g * hsR);
+    const float im = 1.f / mes;
+    const float sL = fabsf(qL) * im + sqrtf(g * hsL), sR = fabsf(qR) * im + sqrtf(g * hsR);
     const float s = pmaxf(sL, sR) * mes;
     const float ho = hb - shift;
     const float dL = eL + shift, dR = eR + shift;  // hs - h_own
@@ -142,37 +145,81 @@ __device__ __forceinline__ void accum_grad(float jx, float jy, float de, float d
 // Reconstruction of cell i: float32 LSQ gradients of eta, hu, hv from float64 differences,
 // Barth-Jespersen limiter on deviations, face records (a, hu_f, hv_f, w) of its nf faces in
 // out[0..nf) -- see the file comment. st is the state the step starts from.
-template <int MF>
-__device__ __forceinline__ int recon32_cell(const Mesh &m, const mfv_sw_mixed &p, const double4 *__restrict__ st,
-                                            int i, float om, float4 *out)
+// Where the reconstruction of one cell reads its inputs: cell states and centres by a
+// handle (a global cell index, or a position in a block's LDS copy), the stencil, ghost
+// owners and face neighbours as handles.
+struct GlobalCells {
+    const Mesh &m;
+    const double4 *__restrict__ st;
+    int i;
+    __device__ __forceinline__ int self() const { return i; }
+    __device__ __forceinline__ double4 state(int h) const { return st[h]; }
+    __device__ __forceinline__ double2 center(int h) const
+    {
+        return make_double2(m.cell_center[3 * (size_t)h], m.cell_center[3 * (size_t)h + 1]);
+    }
+    __device__ __forceinline__ int stencil(int j) const { return m.lsq_cells[(size_t)j * m.C + i]; }
+    __device__ __forceinline__ int ghost_owner(int, int f) const { return m.face_cells[2 * f]; }
+    __device__ __forceinline__ int face_nb(int, int code) const { return code >> 3; }
+};
+
+// KMAX > 0: the stencil handles are loaded first and the neighbour states in independent
+// groups of four (a few memory latencies per cell instead of two per stencil entry); the
+// sums keep the stencil order. KMAX = 0: plain loop, for stencils wider than KMAX.
+template <int MF, int KMAX, class Cells>
+__device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p, const Cells &a, int i, float om,
+                                            float4 *out)
 {
     const size_t C = m.C;
-    const double4 si = st[i];
+    const double4 si = a.state(a.self());
     const double hi = si.x, ei = si.x + si.w, ui = si.y, vi = si.z;
-    const double cx = m.cell_center[3 * (size_t)i], cy = m.cell_center[3 * (size_t)i + 1];
+    const double2 ci = a.center(a.self());
+    const double cx = ci.x, cy = ci.y;
     float ixx = 0.f, iyy = 0.f, ixy = 0.f, je0 = 0.f, je1 = 0.f, ju0 = 0.f, ju1 = 0.f, jv0 = 0.f, jv1 = 0.f;
     const int nn = m.lsq_ncells[i];
-    for (int j = 0; j < nn; j++) {
-        const int c = m.lsq_cells[(size_t)j * C + i];
-        const double4 sc = st[c];
-        const float jx = (float)(m.cell_center[3 * (size_t)c] - cx);
-        const float jy = (float)(m.cell_center[3 * (size_t)c + 1] - cy);
+    auto accum = [&](const double4 sc, const double2 cc) {
+        const float jx = (float)(cc.x - cx);
+        const float jy = (float)(cc.y - cy);
         accum_grad(jx, jy, (float)((sc.x + sc.w) - ei), (float)(sc.y - ui), (float)(sc.z - vi), ixx, iyy, ixy, je0,
                    je1, ju0, ju1, jv0, jv1);
+    };
+    if (KMAX > 0) {
+        int h[KMAX > 0 ? KMAX : 1];
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) h[j] = (j < nn) ? a.stencil(j) : a.self();
+#pragma unroll
+        for (int j0 = 0; j0 < KMAX; j0 += 4) {
+            double4 sc[4];
+            double2 cc[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (j0 + k < nn) {
+                    sc[k] = a.state(h[j0 + k]);
+                    cc[k] = a.center(h[j0 + k]);
+                }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (j0 + k < nn) accum(sc[k], cc[k]);
+        }
+    } else {
+        for (int j = 0; j < nn; j++) {
+            const int c = a.stencil(j);
+            accum(a.state(c), a.center(c));
+        }
     }
     const int ng = m.lsq_nghosts[i];
     for (int j = 0; j < ng; j++) {
         const int f = m.lsq_ghosts[(size_t)j * C + i];
-        const double4 so = st[m.face_cells[2 * f]];
+        const double4 so = a.state(a.ghost_owner(j, f));
         const double *gc = m.face_ghost + 3 * (size_t)f;
         // wall ghost: eta and Z mirrored as Neumann, momenta as nonslip (-owner)
         accum_grad((float)(gc[0] - cx), (float)(gc[1] - cy), (float)((so.x + so.w) - ei), (float)(-so.y - ui),
                    (float)(-so.z - vi), ixx, iyy, ixy, je0, je1, ju0, ju1, jv0, jv1);
     }
-    const float dia = ixx * iyy - ixy * ixy;
-    const float ex = (iyy * je0 - ixy * je1) / dia, ey = (ixx * je1 - ixy * je0) / dia;
-    const float ux = (iyy * ju0 - ixy * ju1) / dia, uy = (ixx * ju1 - ixy * ju0) / dia;
-    const float vx = (iyy * jv0 - ixy * jv1) / dia, vy = (ixx * jv1 - ixy * jv0) / dia;
+    const float idia = 1.f / (ixx * iyy - ixy * ixy);
+    const float ex = (iyy * je0 - ixy * je1) * idia, ey = (ixx * je1 - ixy * je0) * idia;
+    const float ux = (iyy * ju0 - ixy * ju1) * idia, uy = (ixx * ju1 - ixy * ju0) * idia;
+    const float vx = (iyy * jv0 - ixy * jv1) * idia, vy = (ixx * jv1 - ixy * jv0) * idia;
     const int nf = m.cell_nfaces[i];
     float emax = 0.f, emin = 0.f, umax = 0.f, umin = 0.f, vmax = 0.f, vmin = 0.f;  // bounds as deviations
     float dh[MF];
@@ -186,7 +233,7 @@ __device__ __forceinline__ int recon32_cell(const Mesh &m, const mfv_sw_mixed &p
         codes[s] = code;
         float eb, ub, vb;
         if (code >= 0) {
-            const double4 sb = st[code >> 3];
+            const double4 sb = a.state(a.face_nb(s, code));
             eb = (float)((sb.x + sb.w) - ei);
             ub = (float)(sb.y - ui);
             vb = (float)(sb.z - vi);
@@ -235,13 +282,24 @@ __device__ __forceinline__ int recon32_cell(const Mesh &m, const mfv_sw_mixed &p
     return nf;
 }
 
-template <int MF, int BS = kBlock>
+template <int MF, int KMAX>
+__device__ __forceinline__ int recon32_cell(const Mesh &m, const mfv_sw_mixed &p, const double4 *__restrict__ st,
+                                            int i, float om, float4 *out)
+{
+    return recon32_core<MF, KMAX>(m, p, GlobalCells{m, st, i}, i, om, out);
+}
+
+// Stencil width a reconstruction kernel is compiled for: 16 covers the vertex stencils of
+// triangle and quadrilateral meshes; wider ones take the plain loop.
+inline int kmax_for(int K) { return K <= 16 ? 16 : 0; }
+
+template <int MF, int BS, int KMAX>
 __global__ void __launch_bounds__(BS) k_sw_recon32(Mesh m, mfv_sw_mixed p, float om)
 {
     const int i = xcd_block() * BS + threadIdx.x;
     if (i >= m.C) return;
     float4 rec[MF];
-    const int nf = recon32_cell<MF>(m, p, reinterpret_cast<const double4 *>(p.state), i, om, rec);
+    const int nf = recon32_cell<MF, KMAX>(m, p, reinterpret_cast<const double4 *>(p.state), i, om, rec);
     float4 *o = reinterpret_cast<float4 *>(p.facerec) + MF * (size_t)i;
 #pragma unroll
     for (int s = 0; s < MF; s++)
@@ -255,14 +313,15 @@ __device__ __forceinline__ double sw_dt32(const mfv_sw_mixed &p, int i, int nf, 
                                           double vol, float g, double cfl)
 {
     const double velson = sqrt((double)g * hn);
+    const double ih = 1. / hn;
     double lam = 0.;
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         if (s >= nf) break;
         const float4 ga = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
         const double n0 = ga.x, n1 = ga.y, mesu = (double)sqrtf(ga.x * ga.x + ga.y * ga.y);
-        const double u_n = fabs(un / hn * n0 + vn / hn * n1);
-        lam += (u_n / mesu + velson) * mesu;
+        // (|u.n| / |n| + c) |n| of time_step_SW, with one division per cell
+        lam += fabs((un * n0 + vn * n1) * ih) + velson * mesu;
     }
     return cfl * vol / lam;
 }
@@ -322,9 +381,10 @@ __device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int 
     const double R0 = (double)r0 + (double)c0;
     const double R1 = ((double)r1 + (double)c1) + ((double)s1 + (double)d1);
     const double R2 = ((double)r2 + (double)c2) + ((double)s2 + (double)d2);
-    const double hn = si.x + dt * R0 / vol;
-    const double un = si.y + dt * (R1 / vol);
-    const double vn = si.z + dt * (R2 / vol);
+    const double dv = dt / vol;
+    const double hn = si.x + R0 * dv;
+    const double un = si.y + R1 * dv;
+    const double vn = si.z + R2 * dv;
     *out = make_double4(hn, un, vn, si.w);
     return sw_dt32<MF>(p, i, nf, hn, un, vn, vol, g, cfl);
 }
@@ -532,7 +592,10 @@ void launch_batch_step(const Mesh &m, const mfv_sw_mixed &p, const BatchK &b, fl
 {
     const int k = ((step % 3) + 3) % 3;
     const unsigned grid = (unsigned)((m.C + BS - 1) / BS);
-    hipLaunchKernelGGL((k_sw_recon32<MF, BS>), dim3(grid), dim3(BS), 0, st, m, p, om);
+    if (kmax_for(m.K) == 16)
+        hipLaunchKernelGGL((k_sw_recon32<MF, BS, 16>), dim3(grid), dim3(BS), 0, st, m, p, om);
+    else
+        hipLaunchKernelGGL((k_sw_recon32<MF, BS, 0>), dim3(grid), dim3(BS), 0, st, m, p, om);
     hipLaunchKernelGGL((k_sw_flux32_batch<MF, BS>), dim3(grid), dim3(BS), 0, st, m, p, g, cfl, b, k, part);
     hipLaunchKernelGGL(k_batch_dt_fold, dim3(fold_grid(b.I)), dim3(256), 0, st, part, b, k, (k + 1) % 3,
                        (k + 2) % 3, 1);
@@ -610,29 +673,72 @@ BatchK to_batch(const mfv_sw_batch *b)
 // the new one.
 
 struct TilesK {
-    int n, T, H;
+    int n, T, H, E, I;
     const int32_t *__restrict__ cell0;
     const int32_t *__restrict__ ncells;
-    const int32_t *__restrict__ hptr;
-    const int32_t *__restrict__ halo;
+    const int32_t *__restrict__ nhalo;
+    const int32_t *__restrict__ eptr;
+    const int32_t *__restrict__ ext;
+    const int32_t *__restrict__ iptr;
     const int32_t *__restrict__ lnbr;
+    const uint16_t *__restrict__ lst;
+    const uint16_t *__restrict__ lgh;
+    const int16_t *__restrict__ lfn;
 };
 
-template <int MF, int T, bool BATCH>
+// Reconstruction inputs from a block's LDS copy: handles are positions in the tile's ext
+// list (its cells, its halo, then every other cell its reconstructions read).
+struct StagedCells {
+    const double4 *st;
+    const double2 *cc;
+    const uint16_t *__restrict__ lst;
+    const uint16_t *__restrict__ lgh;
+    const int16_t *__restrict__ lfn;
+    size_t row, I;
+    int q;
+    __device__ __forceinline__ int self() const { return q; }
+    __device__ __forceinline__ double4 state(int h) const { return st[h]; }
+    __device__ __forceinline__ double2 center(int h) const { return cc[h]; }
+    __device__ __forceinline__ int stencil(int j) const { return lst[(size_t)j * I + row]; }
+    __device__ __forceinline__ int ghost_owner(int j, int) const { return lgh[(size_t)j * I + row]; }
+    __device__ __forceinline__ int face_nb(int s, int) const { return lfn[(size_t)s * I + row]; }
+};
+
+// STAGED = false: face records in LDS, cell states read from global memory.
+// STAGED = true: first every cell state and centre the tile reads is copied into LDS (one
+// gather per cell and tile instead of one per stencil entry), then both phases read LDS.
+template <int MF, int T, bool BATCH, bool STAGED, int KMAX>
 __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, TilesK t, double4 *__restrict__ out,
                                                        float g, double cfl, float om, DtRing ring, BatchK b,
                                                        int kcur, double *__restrict__ part)
 {
-    extern __shared__ float4 lfr[];  // [(T + t.H) * MF] face records of the tile, then of its halo
+    extern __shared__ double4 lds4[];
+    double4 *s_st = lds4;                                       // [E] when STAGED
+    double2 *s_cc = reinterpret_cast<double2 *>(lds4 + (STAGED ? t.E : 0));
+    float4 *lfr = reinterpret_cast<float4 *>(s_cc + (STAGED ? t.E : 0));  // [(T + H) * MF]
     const int tile = xcd_block();
-    const int c0 = t.cell0[tile], nc = t.ncells[tile];
-    const int h0 = t.hptr[tile], nh = t.hptr[tile + 1] - h0;
+    const int c0 = t.cell0[tile], nc = t.ncells[tile], nh = t.nhalo[tile];
+    const int e0 = t.eptr[tile], ne = t.eptr[tile + 1] - e0;
+    const size_t r0 = t.iptr[tile];
     const double4 *__restrict__ st = reinterpret_cast<const double4 *>(p.state);
     if (!BATCH) ring_advance(ring);
+    if (STAGED) {
+        for (int e = threadIdx.x; e < nc + ne; e += T) {
+            const int c = (e < nc) ? c0 + e : t.ext[e0 + (e - nc)];
+            s_st[e] = st[c];
+            s_cc[e] = make_double2(m.cell_center[3 * (size_t)c], m.cell_center[3 * (size_t)c + 1]);
+        }
+        __syncthreads();
+    }
     for (int q = threadIdx.x; q < nc + nh; q += T) {
-        const int c = (q < nc) ? c0 + q : t.halo[h0 + (q - nc)];
+        const int c = (q < nc) ? c0 + q : t.ext[e0 + (q - nc)];
         float4 rec[MF];
-        const int nf = recon32_cell<MF>(m, p, st, c, om, rec);
+        int nf;
+        if (STAGED)
+            nf = recon32_core<MF, KMAX>(m, p, StagedCells{s_st, s_cc, t.lst, t.lgh, t.lfn, r0 + q, (size_t)t.I, q},
+                                        c, om, rec);
+        else
+            nf = recon32_cell<MF, KMAX>(m, p, st, c, om, rec);
 #pragma unroll
         for (int s = 0; s < MF; s++)
             if (s < nf) lfr[MF * q + s] = rec[s];
@@ -647,38 +753,63 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
         const int32_t *__restrict__ lnbr = t.lnbr;
         const size_t C = m.C;
         auto code_of = [lnbr, C, i](int s) { return lnbr[(size_t)s * C + i]; };
-        auto recs = [](size_t k) { return lfr[k]; };
-        dloc = flux32_core<MF>(p, i, q, st[i], m.cell_nfaces[i], p.order[i], recs, code_of, g, cfl, dt, out + i);
+        auto recs = [lfr](size_t k) { return lfr[k]; };
+        const double4 si = STAGED ? s_st[q] : st[i];
+        dloc = flux32_core<MF>(p, i, q, si, m.cell_nfaces[i], p.order[i], recs, code_of, g, cfl, dt, out + i);
     }
     wave_min_part(part, (size_t)c0 + q, dloc);
+}
+
+size_t tiles_lds_bytes(const mfv_sw_tiles *t, int MF)
+{
+    const size_t rec = (size_t)(t->tile_size + t->max_halo) * MF * sizeof(float4);
+    return t->stage_states ? rec + (size_t)t->max_ext * (sizeof(double4) + sizeof(double2)) : rec;
 }
 
 int check_tiles(const mfv_mesh *m, const mfv_sw_tiles *t)
 {
     MFV_REQUIRE(t != nullptr, "tiles descriptor is NULL");
-    MFV_REQUIRE(t->tile_size == 256 || t->tile_size == 512, "tile_size %d not in {256, 512}", t->tile_size);
-    MFV_REQUIRE(t->nb_tiles >= 1 && t->max_halo >= 0, "bad tile counts");
-    MFV_REQUIRE(t->tile_cell0 && t->tile_ncells && t->halo_ptr && t->halo_cells && t->lnbr,
+    MFV_REQUIRE(t->tile_size == 128 || t->tile_size == 256 || t->tile_size == 512,
+                "tile_size %d not in {128, 256, 512}", t->tile_size);
+    MFV_REQUIRE(t->nb_tiles >= 1 && t->max_halo >= 0 && t->max_ext >= 0 && t->nb_items >= 1,
+                "bad tile counts");
+    MFV_REQUIRE(t->tile_cell0 && t->tile_ncells && t->tile_nhalo && t->ext_ptr && t->ext_cells && t->item_ptr &&
+                    t->lnbr && t->lst && t->lgh && t->lfn,
                 "tiles descriptor has NULL arrays");
-    const size_t lds = (size_t)(t->tile_size + t->max_halo) * m->max_cell_faces * sizeof(float4);
-    MFV_REQUIRE(lds <= 64 * 1024, "tile of %d cells with a halo of %d needs %zu bytes of LDS (> 64 KiB): "
-                "renumber the mesh or use smaller tiles", t->tile_size, t->max_halo, lds);
+    const size_t lds = tiles_lds_bytes(t, m->max_cell_faces);
+    MFV_REQUIRE(lds <= 64 * 1024, "tiles of %d cells need %zu bytes of LDS (> 64 KiB): renumber the mesh, use "
+                "smaller tiles or do not stage the states", t->tile_size, lds);
     return MFV_OK;
 }
 
 TilesK to_tiles(const mfv_sw_tiles *t)
 {
-    return TilesK{t->nb_tiles, t->tile_size, t->max_halo, t->tile_cell0, t->tile_ncells,
-                  t->halo_ptr, t->halo_cells, t->lnbr};
+    return TilesK{t->nb_tiles, t->tile_size, t->max_halo, t->max_ext, t->nb_items, t->tile_cell0,
+                  t->tile_ncells, t->tile_nhalo, t->ext_ptr, t->ext_cells, t->item_ptr, t->lnbr,
+                  t->lst, t->lgh, t->lfn};
 }
 
-template <int MF, int T, bool BATCH>
-void launch_tiled(const Mesh &mk, const mfv_sw_mixed &p, const TilesK &tk, double *out, float g, double cfl,
-                  float om, DtRing r, const BatchK &bk, int kcur, double *part, hipStream_t st)
+template <int MF, int T, bool BATCH, bool STAGED>
+void launch_tiled(const Mesh &mk, const mfv_sw_mixed &p, const TilesK &tk, size_t lds, double *out, float g,
+                  double cfl, float om, DtRing r, const BatchK &bk, int kcur, double *part, hipStream_t st)
 {
-    const size_t lds = (size_t)(T + tk.H) * MF * sizeof(float4);
-    hipLaunchKernelGGL((k_sw_step32_tiled<MF, T, BATCH>), dim3(tk.n), dim3(T), lds, st, mk, p, tk,
-                       reinterpret_cast<double4 *>(out), g, cfl, om, r, bk, kcur, part);
+    if (kmax_for(mk.K) == 16)
+        hipLaunchKernelGGL((k_sw_step32_tiled<MF, T, BATCH, STAGED, 16>), dim3(tk.n), dim3(T), lds, st, mk, p, tk,
+                           reinterpret_cast<double4 *>(out), g, cfl, om, r, bk, kcur, part);
+    else
+        hipLaunchKernelGGL((k_sw_step32_tiled<MF, T, BATCH, STAGED, 0>), dim3(tk.n), dim3(T), lds, st, mk, p, tk,
+                           reinterpret_cast<double4 *>(out), g, cfl, om, r, bk, kcur, part);
+}
+
+template <int MF, bool BATCH, bool STAGED>
+void launch_tiled_t(const Mesh &mk, const mfv_sw_mixed &p, const TilesK &tk, size_t lds, double *out, float g,
+                    double cfl, float om, DtRing r, const BatchK &bk, int kcur, double *part, hipStream_t st)
+{
+    switch (tk.T) {
+    case 128: launch_tiled<MF, 128, BATCH, STAGED>(mk, p, tk, lds, out, g, cfl, om, r, bk, kcur, part, st); break;
+    case 256: launch_tiled<MF, 256, BATCH, STAGED>(mk, p, tk, lds, out, g, cfl, om, r, bk, kcur, part, st); break;
+    default: launch_tiled<MF, 512, BATCH, STAGED>(mk, p, tk, lds, out, g, cfl, om, r, bk, kcur, part, st); break;
+    }
 }
 
 template <bool BATCH>
@@ -687,13 +818,14 @@ int tiled_step(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_tiles *t, 
 {
     const Mesh mk = to_kernel(m);
     const TilesK tk = to_tiles(t);
+    const size_t lds = tiles_lds_bytes(t, m->max_cell_faces);
     const bool four = m->max_cell_faces == 4;
-    if (t->tile_size == 256) {
-        if (four) launch_tiled<4, 256, BATCH>(mk, *p, tk, out, g, cfl, om, r, bk, kcur, part, st);
-        else launch_tiled<3, 256, BATCH>(mk, *p, tk, out, g, cfl, om, r, bk, kcur, part, st);
+    if (t->stage_states) {
+        if (four) launch_tiled_t<4, BATCH, true>(mk, *p, tk, lds, out, g, cfl, om, r, bk, kcur, part, st);
+        else launch_tiled_t<3, BATCH, true>(mk, *p, tk, lds, out, g, cfl, om, r, bk, kcur, part, st);
     } else {
-        if (four) launch_tiled<4, 512, BATCH>(mk, *p, tk, out, g, cfl, om, r, bk, kcur, part, st);
-        else launch_tiled<3, 512, BATCH>(mk, *p, tk, out, g, cfl, om, r, bk, kcur, part, st);
+        if (four) launch_tiled_t<4, BATCH, false>(mk, *p, tk, lds, out, g, cfl, om, r, bk, kcur, part, st);
+        else launch_tiled_t<3, BATCH, false>(mk, *p, tk, lds, out, g, cfl, om, r, bk, kcur, part, st);
     }
     return check_launch("k_sw_step32_tiled");
 }
@@ -733,7 +865,9 @@ extern "C" int mfv_sw_recon_mixed(const mfv_mesh *m, const mfv_sw_mixed *p, int 
     if (int rc = check_mixed(m, p)) return rc;
     MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
     if (m->nb_cells == 0) return MFV_OK;
-    auto k = (m->max_cell_faces == 3) ? k_sw_recon32<3> : k_sw_recon32<4>;
+    const bool k16 = kmax_for(m->lsq_width) == 16;
+    auto k = (m->max_cell_faces == 3) ? (k16 ? k_sw_recon32<3, kBlock, 16> : k_sw_recon32<3, kBlock, 0>)
+                                      : (k16 ? k_sw_recon32<4, kBlock, 16> : k_sw_recon32<4, kBlock, 0>);
     hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p,
                        (float)(order - 1));
     return check_launch("k_sw_recon32");

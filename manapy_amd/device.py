@@ -245,9 +245,13 @@ class SwPacked(object):
         return ctypes.byref(self.desc)
 
 
-def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None):
+def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None):
     """Tiles of the fused mixed step (mfv_sw_tiles in include/mfv.h) from the neighbour codes
-    nbr [MF][C]: by default consecutive runs of T cells; halo lists and block-local codes."""
+    nbr [MF][C] and the least-squares tables ``lsq`` (lsq_cells [K][C], lsq_ncells,
+    lsq_ghosts [G][C], lsq_nghosts, face_cells of a gpu_layout): by default consecutive runs
+    of T cells. A tile's items are its cells then its halo (face neighbours outside it);
+    its ext list is its items then every other cell their reconstruction reads (stencil
+    cells, ghost owners, face neighbours). Tables give positions in the ext list."""
     nbr = np.asarray(nbr)
     MF = nbr.shape[0]
     if tile_cell0 is None:
@@ -259,48 +263,145 @@ def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None):
         raise ValueError("tiles must hold 1..T cells")
     nt = len(tile_cell0)
     tile_of = np.full(C, -1, dtype=np.int64)
-    for_tile = np.repeat(np.arange(nt), tile_ncells)
     cells = np.concatenate([np.arange(a, a + n) for a, n in zip(tile_cell0, tile_ncells)]) if nt else np.zeros(0, int)
-    tile_of[cells] = for_tile
+    tile_of[cells] = np.repeat(np.arange(nt), tile_ncells)
     code = nbr.astype(np.int64)                               # [MF][C]
     valid = code >= 0
     j = np.where(valid, code >> 3, 0)
-    ti = np.broadcast_to(tile_of[None, :], code.shape)       # tile of the cell owning the slot
-    covered = ti >= 0
-    outside = valid & covered & (tile_of[j] != ti)
-    # halo of each tile: the distinct outside neighbours, ascending
-    key = np.unique(ti[outside] * np.int64(C) + j[outside])
-    h_tile = key // C
-    h_cell = key % C
-    halo_ptr = np.zeros(nt + 1, dtype=np.int64)
-    np.add.at(halo_ptr, h_tile + 1, 1)
-    halo_ptr = np.cumsum(halo_ptr)
-    local = np.where(valid, j - tile_cell0[np.maximum(ti, 0)], 0)
+    ti = np.broadcast_to(tile_of[None, :], code.shape)
+    outside = valid & (ti >= 0) & (tile_of[j] != ti)
+    key = np.unique(ti[outside] * np.int64(C) + j[outside])  # halo pairs (tile, cell), ascending
+    h_tile, h_cell = key // C, key % C
+    nhalo = np.bincount(h_tile, minlength=nt).astype(np.int64)
+    hptr = np.concatenate([[0], np.cumsum(nhalo)])
+    # block-local codes of the flux gather: positions in the item (= ext) list
+    t0 = np.maximum(ti, 0)
+    local = np.where(valid, j - tile_cell0[t0], 0)
     pos = np.searchsorted(key, ti * np.int64(C) + j)
-    local = np.where(outside, tile_ncells[np.maximum(ti, 0)] + (pos - halo_ptr[np.maximum(ti, 0)]), local)
+    local = np.where(outside, tile_ncells[t0] + (pos - hptr[t0]), local)
     lnbr = np.where(valid, (local << 3) | (code & 7), -1)
-    max_halo = int(np.max(np.diff(halo_ptr))) if nt else 0
-    return {'T': int(T), 'nb_tiles': nt, 'max_halo': max_halo, 'MF': MF,
-            'tile_cell0': tile_cell0.astype(np.int32), 'tile_ncells': tile_ncells.astype(np.int32),
-            'halo_ptr': halo_ptr.astype(np.int32), 'halo_cells': h_cell.astype(np.int32),
-            'lnbr': np.ascontiguousarray(lnbr, dtype=np.int32)}
+    out = {'T': int(T), 'nb_tiles': nt, 'MF': MF,
+           'tile_cell0': tile_cell0.astype(np.int32), 'tile_ncells': tile_ncells.astype(np.int32),
+           'tile_nhalo': nhalo.astype(np.int32), 'max_halo': int(nhalo.max()) if nt else 0,
+           'lnbr': np.ascontiguousarray(lnbr, dtype=np.int32)}
+    if lsq is None:
+        raise ValueError("the least-squares tables (lsq=gpu_layout(...)) are required")
+    # items: tile cells then halo, per tile
+    n_items = tile_ncells + nhalo
+    item_ptr = np.concatenate([[0], np.cumsum(n_items)])
+    I = int(item_ptr[-1])
+    item_tile = np.repeat(np.arange(nt), n_items)
+    item_cell = np.empty(I, dtype=np.int64)
+    first = item_ptr[:-1]
+    is_cell = np.zeros(I, dtype=bool)
+    for_cells = np.concatenate([np.arange(f, f + n) for f, n in zip(first, tile_ncells)]) if nt else np.zeros(0, int)
+    is_cell[for_cells] = True
+    item_cell[for_cells] = cells
+    item_cell[~is_cell] = h_cell                               # halo in (tile, cell) order
+    K = lsq['lsq_cells'].shape[0]
+    G = lsq['lsq_ghosts'].shape[0]
+    nn = lsq['lsq_ncells'][item_cell]
+    ng = lsq['lsq_nghosts'][item_cell]
+    sten = lsq['lsq_cells'][:, item_cell].astype(np.int64)     # [K][I]
+    sten_ok = np.arange(K)[:, None] < nn[None, :]
+    gf = lsq['lsq_ghosts'][:, item_cell].astype(np.int64)      # [G][I] face ids
+    g_ok = np.arange(G)[:, None] < ng[None, :]
+    gown = np.where(g_ok, lsq['face_cells'][np.where(g_ok, gf, 0), 0], 0).astype(np.int64)
+    fn_code = code[:, item_cell]                               # [MF][I]
+    fn_ok = fn_code >= 0
+    fn = np.where(fn_ok, fn_code >> 3, 0)
+    # ext list: items first (tile cells, halo), then every other cell read, ascending
+    want_t = np.concatenate([np.broadcast_to(item_tile, sten.shape)[sten_ok],
+                             np.broadcast_to(item_tile, gown.shape)[g_ok],
+                             np.broadcast_to(item_tile, fn.shape)[fn_ok]])
+    want_c = np.concatenate([sten[sten_ok], gown[g_ok], fn[fn_ok]])
+    wkey = np.unique(want_t * np.int64(C) + want_c)
+    ikey = item_tile * np.int64(C) + item_cell
+    order = np.argsort(ikey, kind='stable')
+    ikey_sorted = ikey[order]
+    in_items = np.isin(wkey, ikey_sorted, assume_unique=False)
+    xkey = wkey[~in_items]                                     # extras, ascending per tile
+    x_tile = xkey // C
+    nextra = np.bincount(x_tile, minlength=nt).astype(np.int64)
+    xptr = np.concatenate([[0], np.cumsum(nextra)])
+    n_ext = n_items + nextra
+    ext_ptr = np.concatenate([[0], np.cumsum(nhalo + nextra)])  # ext_cells holds halo + extras
+    ext_cells = np.empty(int(ext_ptr[-1]), dtype=np.int64)
+    hpos = ext_ptr[:-1]
+    for_halo = np.concatenate([np.arange(a, a + n) for a, n in zip(hpos, nhalo)]) if nt else np.zeros(0, int)
+    ext_cells[for_halo] = h_cell
+    for_x = np.concatenate([np.arange(a + h, a + h + n) for a, h, n in zip(hpos, nhalo, nextra)]) \
+        if nt else np.zeros(0, int)
+    ext_cells[for_x] = xkey % C
+
+    def ext_pos(t, c):
+        """Position of cell c in tile t's ext list (all queried pairs exist)."""
+        t = np.asarray(t, dtype=np.int64)
+        c = np.asarray(c, dtype=np.int64)
+        k = t * np.int64(C) + c
+        r = np.full(k.shape, -1, dtype=np.int64)
+        inc = (c >= tile_cell0[t]) & (c < tile_cell0[t] + tile_ncells[t])
+        r[inc] = (c - tile_cell0[t])[inc]
+        ph = np.searchsorted(key, k)
+        ph_ok = (~inc) & (ph < len(key))
+        ph_ok[ph_ok] = key[ph[ph_ok]] == k[ph_ok]
+        r[ph_ok] = (tile_ncells[t] + ph - hptr[t])[ph_ok]
+        px = np.searchsorted(xkey, k)
+        px_ok = (r < 0) & (px < len(xkey))
+        px_ok[px_ok] = xkey[px[px_ok]] == k[px_ok]
+        r[px_ok] = (n_items[t] + px - xptr[t])[px_ok]
+        if np.any(r < 0):
+            raise AssertionError("ext position lookup failed")
+        return r
+    it = np.broadcast_to(item_tile, sten.shape)
+    lst = np.zeros(sten.shape, dtype=np.int64)
+    lst[sten_ok] = ext_pos(it[sten_ok], sten[sten_ok])
+    lgh = np.zeros(gown.shape, dtype=np.int64)
+    itg = np.broadcast_to(item_tile, gown.shape)
+    lgh[g_ok] = ext_pos(itg[g_ok], gown[g_ok])
+    lfn = np.full(fn.shape, -1, dtype=np.int64)
+    itf = np.broadcast_to(item_tile, fn.shape)
+    lfn[fn_ok] = ext_pos(itf[fn_ok], fn[fn_ok])
+    if n_ext.max(initial=0) >= 65535:
+        raise ValueError("a tile reads more than 65535 cells")
+    out.update({'max_ext': int(n_ext.max()) if nt else 0, 'items': I,
+                'ext_ptr': ext_ptr.astype(np.int32), 'ext_cells': ext_cells.astype(np.int32),
+                'item_ptr': item_ptr.astype(np.int32), 'tile_next': n_ext.astype(np.int32),
+                'lst': np.ascontiguousarray(lst, dtype=np.uint16) if K else np.zeros((1, max(I, 1)), np.uint16),
+                'lgh': np.ascontiguousarray(lgh, dtype=np.uint16) if G else np.zeros((1, max(I, 1)), np.uint16),
+                'lfn': np.ascontiguousarray(lfn, dtype=np.int16)})
+    return out
 
 
 class SwTiles(object):
-    """Device copy of a tiles layout and its mfv_sw_tiles descriptor."""
+    """Device copy of a tiles layout and its mfv_sw_tiles descriptor (stage=True copies every
+    cell state a tile reads into LDS before reconstructing)."""
 
-    def __init__(self, host, device):
+    ARRAYS = ('tile_cell0', 'tile_ncells', 'tile_nhalo', 'ext_ptr', 'ext_cells', 'item_ptr', 'lnbr', 'lst', 'lgh',
+              'lfn')
+
+    def __init__(self, host, device, stage=False):
         import torch
         self.host = host
-        self.t = {k: torch.from_numpy(np.ascontiguousarray(host[k])).to(device)
-                  for k in ('tile_cell0', 'tile_ncells', 'halo_ptr', 'halo_cells', 'lnbr')}
-        if host['halo_cells'].size == 0:
-            self.t['halo_cells'] = torch.zeros(1, dtype=torch.int32, device=device)
+        self.t = {}
+        for k in self.ARRAYS:
+            a = np.ascontiguousarray(host[k])
+            if a.dtype == np.uint16:  # torch has no uint16: move the bits as int16
+                a = a.view(np.int16)
+            if a.size == 0:
+                a = np.zeros(1, dtype=a.dtype)
+            self.t[k] = torch.from_numpy(a).to(device)
         d = _lib.MfvSwTiles()
         d.nb_tiles, d.tile_size, d.max_halo = host['nb_tiles'], host['T'], host['max_halo']
-        for k in ('tile_cell0', 'tile_ncells', 'halo_ptr', 'halo_cells', 'lnbr'):
+        d.max_ext, d.nb_items, d.stage_states = host['max_ext'], host['items'], int(bool(stage))
+        for k in self.ARRAYS:
             setattr(d, k, self.t[k].data_ptr())
         self.desc = d
+
+    def lds_bytes(self):
+        MF = self.host['MF']
+        rec = (self.host['T'] + self.host['max_halo']) * MF * 16
+        return rec + (self.host['max_ext'] * 48 if self.desc.stage_states else 0)
 
     @property
     def ptr(self):
@@ -335,20 +436,63 @@ class SwMixed(object):
             setattr(d, k, getattr(self, k).data_ptr())
         self.desc = d
         self._tiles = {}
+        self._layouts = {}
+        self._lsq = dmesh.host
 
-    def tiles(self, T=256):
+    def tiles(self, T=256, stage=True):
         """The fused step's tiles of T cells (built on first use)."""
-        if T not in self._tiles:
-            self._tiles[T] = SwTiles(sw_tiles_layout(self.nbr_host, self.desc.nb_cells, T), self.state.device)
-        return self._tiles[T]
+        if (T, stage) not in self._tiles:
+            if T not in self._layouts:
+                self._layouts[T] = sw_tiles_layout(self.nbr_host, self.desc.nb_cells, T, lsq=self._lsq)
+            self._tiles[(T, stage)] = SwTiles(self._layouts[T], self.state.device, stage)
+        return self._tiles[(T, stage)]
 
-    def step_tiled(self, dmesh, grav, cfl, order, ring, step, T=256, stream=None):
+    def step_tiled(self, dmesh, grav, cfl, order, ring, step, T=256, stage=True, stream=None):
         """One fused step: reads the current state buffer, writes the other, swaps them."""
-        _lib.call('mfv_sw_step_mixed_tiled', dmesh.ptr, self.ptr, self.tiles(T).ptr, self.state2.data_ptr(),
+        _lib.call('mfv_sw_step_mixed_tiled', dmesh.ptr, self.ptr, self.tiles(T, stage).ptr, self.state2.data_ptr(),
                   float(grav), float(cfl), int(order), ring.data_ptr(), int(step),
                   stream if stream is not None else stream_ptr())
         self.state, self.state2 = self.state2, self.state
         self.desc.state = self.state.data_ptr()
+
+    CONFIGS = (('split', 256), ('tiled', 256), ('staged', 128), ('staged', 256))
+
+    def step(self, dmesh, grav, cfl, order, ring, step, config=('staged', 256), stream=None):
+        """One step with ``config``: ('split', _) runs the reconstruction and flux kernels,
+        ('tiled', T) / ('staged', T) the fused step on tiles of T cells."""
+        kind, T = config
+        s = stream if stream is not None else stream_ptr()
+        if kind == 'split':
+            _lib.call('mfv_sw_recon_mixed', dmesh.ptr, self.ptr, int(order), s)
+            _lib.call('mfv_sw_flux_mixed', dmesh.ptr, self.ptr, float(grav), float(cfl), ring.data_ptr(), int(step), s)
+        else:
+            self.step_tiled(dmesh, grav, cfl, order, ring, step, T=T, stage=(kind == 'staged'), stream=s)
+
+    def autotune(self, dmesh, grav, cfl, order, ring, steps=10):
+        """Times each configuration on ``steps`` steps (state and ring restored after) and
+        returns (best, {config: ms per step})."""
+        import torch
+        saved = (self.state.clone(), ring.clone())
+        st = torch.cuda.current_stream()
+        res = {}
+        for cfg in self.CONFIGS:
+            if cfg[0] != 'split':
+                try:
+                    if self.tiles(cfg[1], cfg[0] == 'staged').lds_bytes() > 64 * 1024:
+                        continue
+                except ValueError:
+                    continue
+            self.step(dmesh, grav, cfl, order, ring, 0, cfg)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for k in range(steps):
+                self.step(dmesh, grav, cfl, order, ring, 1 + k, cfg)
+            e1.record(st)
+            e1.synchronize()
+            res[cfg] = e0.elapsed_time(e1) / steps
+            self.state.copy_(saved[0])
+            ring.copy_(saved[1])
+        return min(res, key=res.get), res
 
     @property
     def ptr(self):

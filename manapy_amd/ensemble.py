@@ -26,9 +26,11 @@ from .device import gpu_layout, require_gpu, stream_ptr, sw_packed_layout
 
 BLOCKS = (64, 128, 256, 512)
 TILES = (256, 512)
-# launch configurations the autotuner compares: the two-kernel step with its block size, or
-# the fused tiled step with its tile size
-CONFIGS = tuple(('split', b) for b in BLOCKS) + tuple(('tiled', t) for t in TILES)
+# launch configurations the autotuner compares: the two-kernel step with its block size, the
+# fused tiled step with its tile size, and the fused step with every cell state a tile reads
+# staged in LDS
+CONFIGS = (tuple(('split', b) for b in BLOCKS) + tuple(('tiled', t) for t in TILES) +
+           tuple(('staged', t) for t in (128, 256)))
 
 
 def _cat_cols(arrs, pads, fill, offsets=None):
@@ -110,7 +112,7 @@ def ensemble_tiles(host, T=256):
         for off in range(0, npad, T):
             c0s.append(int(a) + off)
             ns.append(min(T, npad - off))
-    return sw_tiles_layout(host['nbr'], host['C'], T, np.array(c0s), np.array(ns))
+    return sw_tiles_layout(host['nbr'], host['C'], T, np.array(c0s), np.array(ns), lsq=host)
 
 
 class SwEnsemble(object):
@@ -203,11 +205,13 @@ class SwEnsemble(object):
     def block(self):
         return self.config[1]
 
-    def tiles(self, T):
-        if T not in self._tiles:
+    def tiles(self, T, stage=False):
+        if (T, stage) not in self._tiles:
             from .device import SwTiles
-            self._tiles[T] = SwTiles(ensemble_tiles(self.host, T), self.device)
-        return self._tiles[T]
+            lay = next((v.host for (t, _), v in self._tiles.items() if t == T), None)
+            self._tiles[(T, stage)] = SwTiles(lay if lay is not None else ensemble_tiles(self.host, T), self.device,
+                                              stage)
+        return self._tiles[(T, stage)]
 
     def _swap(self):
         self.state, self.state2 = self.state2, self.state
@@ -223,7 +227,8 @@ class SwEnsemble(object):
                           ctypes.byref(self.batch), self.grav, self.cfl, self.order, self.step_index, int(size), s)
             else:
                 _lib.call('mfv_sw_batch_step_tiled', ctypes.byref(self.mesh), ctypes.byref(self.mixed),
-                          ctypes.byref(self.batch), self.tiles(int(size)).ptr, self.state2.data_ptr(), self.grav,
+                          ctypes.byref(self.batch), self.tiles(int(size), kind == 'staged').ptr,
+                          self.state2.data_ptr(), self.grav,
                           self.cfl, self.order, self.step_index, s)
                 self._swap()
             self.step_index += 1
@@ -240,7 +245,8 @@ class SwEnsemble(object):
                           ctypes.byref(self.batch), self.grav, self.cfl, self.order, key[0], key[2], ctypes.byref(h))
             else:
                 _lib.call('mfv_sw_batch_graph_tiled', ctypes.byref(self.mesh), ctypes.byref(self.mixed),
-                          ctypes.byref(self.batch), self.tiles(key[2]).ptr, self.state2.data_ptr(), self.grav,
+                          ctypes.byref(self.batch), self.tiles(key[2], key[1] == 'staged').ptr,
+                          self.state2.data_ptr(), self.grav,
                           self.cfl, self.order, key[0], ctypes.byref(h))
             self._graph, self._graph_key = h, key
             self._graph_state = self.state.data_ptr()

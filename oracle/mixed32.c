@@ -47,15 +47,18 @@ static void face_dev(float hb, float aL, float aR, float dz, float huL, float hv
     }
     float uL = 0.f, vL = 0.f, uR = 0.f, vR = 0.f;
     if (hL > 1e-12f) {
-        uL = huL / hL;
-        vL = hvL / hL;
+        const float r = 1.f / hL;
+        uL = huL * r;
+        vL = hvL * r;
     }
     if (hR > 1e-12f) {
-        uR = huR / hR;
-        vR = hvR / hR;
+        const float r = 1.f / hR;
+        uR = huR * r;
+        vR = hvR * r;
     }
     const float qL = uL * n0 + vL * n1, qR = uR * n0 + vR * n1;
-    const float sL = fabsf(qL) / mes + sqrtf(g * hsL), sR = fabsf(qR) / mes + sqrtf(g * hsR);
+    const float im = 1.f / mes;
+    const float sL = fabsf(qL) * im + sqrtf(g * hsL), sR = fabsf(qR) * im + sqrtf(g * hsR);
     const float s = pmaxf_(sL, sR) * mes;
     const float ho = hb - shift;
     const float dL = eL + shift, dR = eR + shift;
@@ -147,9 +150,10 @@ void orc_mixed_recon(int C, int MF, const double *state, const double *cell_cent
         }
         const float ixx = a[0], iyy = a[1], ixy = a[2];
         const float dia = ixx * iyy - ixy * ixy;
-        const float ex = (iyy * a[3] - ixy * a[4]) / dia, ey = (ixx * a[4] - ixy * a[3]) / dia;
-        const float ux = (iyy * a[5] - ixy * a[6]) / dia, uy = (ixx * a[6] - ixy * a[5]) / dia;
-        const float vx = (iyy * a[7] - ixy * a[8]) / dia, vy = (ixx * a[8] - ixy * a[7]) / dia;
+        const float idia = 1.f / dia;
+        const float ex = (iyy * a[3] - ixy * a[4]) * idia, ey = (ixx * a[4] - ixy * a[3]) * idia;
+        const float ux = (iyy * a[5] - ixy * a[6]) * idia, uy = (ixx * a[6] - ixy * a[5]) * idia;
+        const float vx = (iyy * a[7] - ixy * a[8]) * idia, vy = (ixx * a[8] - ixy * a[7]) * idia;
         const int nf = cell_nfaces[i];
         float emax = 0.f, emin = 0.f, umax = 0.f, umin = 0.f, vmax = 0.f, vmin = 0.f;
         float dh[8], rx[8], ry[8];
@@ -252,19 +256,20 @@ double orc_mixed_flux(int C, int MF, double *state, const double *vol, const flo
         const double R0 = (double)r0 + (double)c0;
         const double R1 = ((double)r1 + (double)c1) + ((double)s1 + (double)d1);
         const double R2 = ((double)r2 + (double)c2) + ((double)s2 + (double)d2);
-        const double hn = si[0] + dt * R0 / v;
-        const double un = si[1] + dt * (R1 / v);
-        const double vn = si[2] + dt * (R2 / v);
+        const double dv = dt / v;
+        const double hn = si[0] + R0 * dv;
+        const double un = si[1] + R1 * dv;
+        const double vn = si[2] + R2 * dv;
         si[0] = hn;
         si[1] = un;
         si[2] = vn;
         const double velson = sqrt((double)g * hn);
+        const double ih = 1. / hn;
         double lam = 0.;
         for (int s = 0; s < MF && s < nf; s++) {
             const float *ga = geoA + 4 * ((size_t)MF * i + s);
             const double n0 = ga[0], n1 = ga[1], mesu = (double)sqrtf(ga[0] * ga[0] + ga[1] * ga[1]);
-            const double u_n = fabs(un / hn * n0 + vn / hn * n1);
-            lam += (u_n / mesu + velson) * mesu;
+            lam += fabs((un * n0 + vn * n1) * ih) + velson * mesu;
         }
         const double dl = cfl * v / lam;
         if (dl < dmin || !(dmin == dmin)) dmin = dl;
@@ -280,12 +285,12 @@ double orc_mixed_dt(int C, int MF, const double *state, const double *vol, const
     for (int i = 0; i < C; i++) {
         const double hn = state[4 * (size_t)i], un = state[4 * (size_t)i + 1], vn = state[4 * (size_t)i + 2];
         const double velson = sqrt((double)g * hn);
+        const double ih = 1. / hn;
         double lam = 0.;
         for (int s = 0; s < MF && s < cell_nfaces[i]; s++) {
             const float *ga = geoA + 4 * ((size_t)MF * i + s);
             const double n0 = ga[0], n1 = ga[1], mesu = (double)sqrtf(ga[0] * ga[0] + ga[1] * ga[1]);
-            const double u_n = fabs(un / hn * n0 + vn / hn * n1);
-            lam += (u_n / mesu + velson) * mesu;
+            lam += fabs((un * n0 + vn * n1) * ih) + velson * mesu;
         }
         const double dl = cfl * vol[i] / lam;
         if (dl < dmin) dmin = dl;

@@ -486,8 +486,9 @@ def test_sw_packed_and_mixed_on_hilbert_order(torch, sw_case, oracle_lib):
         same(r.to_old(st[:, k]), model.state[:, k], "mixed %d" % k)
 
 
-@pytest.mark.parametrize("T,hilbert", [(256, True), (512, True), (256, False)])
-def test_sw_mixed_tiled_step_equals_two_kernel_step(torch, oracle_lib, T, hilbert):
+@pytest.mark.parametrize("T,hilbert,stage", [(256, True, False), (512, True, False), (256, False, False),
+                                             (128, True, True), (256, True, True), (128, False, True)])
+def test_sw_mixed_tiled_step_equals_two_kernel_step(torch, oracle_lib, T, hilbert, stage):
     """The fused tiled step (face records in LDS, double-buffered state) gives the bits of
     reconstruction + flux kernels, step after step, including the dt ring."""
     from manapy_amd.device import DeviceMesh, SwMixed
@@ -513,7 +514,7 @@ def test_sw_mixed_tiled_step_equals_two_kernel_step(torch, oracle_lib, T, hilber
         rings = []
         for s in range(25):
             if tiled:
-                p.step_tiled(m, 9.81, 0.8, 2, ring, s, T=T)
+                p.step_tiled(m, 9.81, 0.8, 2, ring, s, T=T, stage=stage)
             else:
                 call('mfv_sw_recon_mixed', m.ptr, p.ptr, 2)
                 call('mfv_sw_flux_mixed', m.ptr, p.ptr, 9.81, 0.8, P(ring), s)

@@ -7,16 +7,24 @@ import pytest
 from conftest import make_tri_domain
 
 
-def _check(L, nbr, C):
+def _ext(L, t):
+    """Tile t's ext list as global cell ids: its cells, its halo, then the extra cells."""
+    a, n = int(L['tile_cell0'][t]), int(L['tile_ncells'][t])
+    return np.concatenate([np.arange(a, a + n), L['ext_cells'][L['ext_ptr'][t]:L['ext_ptr'][t + 1]]])
+
+
+def _check(L, nbr, lsq, C):
     MF = nbr.shape[0]
-    c0, nc, hp, hc, ln = (L[k] for k in ('tile_cell0', 'tile_ncells', 'halo_ptr', 'halo_cells', 'lnbr'))
+    c0, nc, nh, ln = (L[k] for k in ('tile_cell0', 'tile_ncells', 'tile_nhalo', 'lnbr'))
     covered = np.zeros(C, dtype=bool)
+    K, G = lsq['lsq_cells'].shape[0], lsq['lsq_ghosts'].shape[0]
     for t in range(L['nb_tiles']):
-        a, n = int(c0[t]), int(nc[t])
-        assert 1 <= n <= L['T']
+        a, n, h = int(c0[t]), int(nc[t]), int(nh[t])
+        assert 1 <= n <= L['T'] and h <= L['max_halo']
         covered[a:a + n] = True
-        halo = hc[hp[t]:hp[t + 1]]
-        assert len(halo) <= L['max_halo']
+        ext = _ext(L, t)
+        assert len(ext) == L['tile_next'][t] <= L['max_ext'] and len(set(ext.tolist())) == len(ext)
+        halo = ext[n:n + h]
         assert np.all(np.diff(halo) > 0) and not np.any((halo >= a) & (halo < a + n))
         want = set()
         for i in range(a, a + n):
@@ -25,13 +33,23 @@ def _check(L, nbr, C):
                 if g < 0:
                     assert l == -1
                     continue
-                j = g >> 3
-                if not (a <= j < a + n):
-                    want.add(j)
-                loc = l >> 3
-                got = a + loc if loc < n else int(halo[loc - n])
-                assert got == j and (l & 7) == (g & 7)
+                if not (a <= (g >> 3) < a + n):
+                    want.add(g >> 3)
+                assert ext[l >> 3] == g >> 3 and (l & 7) == (g & 7) and (l >> 3) < n + h
         assert want == set(halo.tolist())
+        # every item's reconstruction inputs resolve through the ext list
+        for q in range(n + h):
+            r = int(L['item_ptr'][t]) + q
+            c = int(ext[q])
+            for jj in range(int(lsq['lsq_ncells'][c])):
+                assert ext[L['lst'][jj, r]] == lsq['lsq_cells'][jj, c]
+            for jj in range(int(lsq['lsq_nghosts'][c])):
+                f = lsq['lsq_ghosts'][jj, c]
+                assert ext[L['lgh'][jj, r]] == lsq['face_cells'][f, 0]
+            for s in range(MF):
+                g = int(nbr[s, c])
+                assert (L['lfn'][s, r] == -1) if g < 0 else (ext[L['lfn'][s, r]] == g >> 3)
+        assert int(L['item_ptr'][t + 1]) - int(L['item_ptr'][t]) == n + h
     assert covered.all()
 
 
@@ -40,12 +58,14 @@ def test_tiles_of_a_renumbered_mesh(T):
     from manapy_amd.renumber import renumbered
     from manapy_amd.device import sw_packed_layout, sw_tiles_layout
     d = renumbered(make_tri_domain(37, 23, 100., 60., 0.1, 2))
+    from manapy_amd.device import gpu_layout
     P = sw_packed_layout(d)
     C = int(d.nbcells)
-    L = sw_tiles_layout(P['nbr'], C, T)
-    _check(L, P['nbr'], C)
+    lsq = gpu_layout(d)
+    L = sw_tiles_layout(P['nbr'], C, T, lsq=lsq)
+    _check(L, P['nbr'], lsq, C)
     # the Hilbert order keeps halos well below the tile size
-    assert np.diff(L['halo_ptr']).mean() < 0.4 * T
+    assert L['tile_nhalo'].mean() < 0.4 * T
 
 
 def test_ensemble_tiles_stay_inside_instances():
@@ -53,7 +73,7 @@ def test_ensemble_tiles_stay_inside_instances():
     doms = [make_tri_domain(n, m, 10., 10., 0.1, s) for n, m, s in ((9, 7, 1), (23, 11, 2), (5, 5, 3))]
     E = ensemble_layout(doms)
     L = ensemble_tiles(E, 256)
-    _check(L, E['nbr'], E['C'])
+    _check(L, E['nbr'], E, E['C'])
     inst_end = E['inst_cell0'] + ((E['inst_ncells'] + 63) // 64) * 64
     for a, n in zip(L['tile_cell0'], L['tile_ncells']):
         k = np.searchsorted(E['inst_cell0'], a, side='right') - 1
