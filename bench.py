@@ -168,15 +168,17 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel=
     _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
     k = [0]
     tune = None
+    ncell = tile
     if kernel == 'auto':
-        (kernel, tile), tune = p.autotune(m, G, CFL, 2, ring)
-        log("mixed: autotuned %s %d %s" % (kernel, tile, {"%s:%d" % c: round(v, 4) for c, v in tune.items()}))
+        (kernel, tile, ncell), tune = p.autotune(m, G, CFL, 2, ring)
+        log("mixed: autotuned %s %d/%d %s" % (kernel, tile, ncell,
+                                              {"%s:%d:%d" % c: round(v, 4) for c, v in tune.items()}))
 
     def step(ev=None):
         if ev is not None:
             ev[0].record()
         if kernel in ('tiled', 'staged'):
-            p.step_tiled(m, G, CFL, 2, ring, k[0], T=tile, stage=(kernel == 'staged'), stream=s)
+            p.step_tiled(m, G, CFL, 2, ring, k[0], T=tile, stage=(kernel == 'staged'), stream=s, N=ncell)
             if ev is not None:
                 ev[1].record()
         else:
@@ -221,6 +223,7 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel=
         bt = tiled_bytes_per_cell(MF, K_avg, G_avg)
         kern = {"k_sw_step32_tiled": {"avg_us": round(float(t_rec.mean()) * 1e6, 2), "bytes_per_cell": round(bt, 1),
                                       "GB/s": round(bt * C / t_rec.mean() / 1e9, 1), "tile": tile,
+                                      "cells_per_tile": ncell,
                                       "states_in_lds": kernel == 'staged'},
                 "k_dt_finish": {"avg_us": round(float(t_flx.mean()) * 1e6, 2)}}
         dom, tk, bk = 'k_sw_step32_tiled', t_rec, bt
@@ -235,7 +238,7 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel=
     return {
         "value": round(mcell, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop / steps * 1e3, 5),
         "kernel": kernel, "tile": tile if kernel != 'split' else None,
-        "autotune_ms_per_step": {"%s:%d" % c: round(v, 4) for c, v in tune.items()} if tune else None,
+        "autotune_ms_per_step": {"%s:%d:%d" % c: round(v, 4) for c, v in tune.items()} if tune else None,
         "speedup_vs_fp64": round(mcell / mcell_fp64, 3),
         "dtype": "f32 flux/gradient arithmetic, f64 state/dt/sums",
         "steps_compared": warmup + steps, "error_vs_fp64": errs,
@@ -243,7 +246,8 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel=
         "kernels": kern,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(bk * C / tk.mean() / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(bk * C / tk.mean() / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(dom if kernel == 'split' else "%s:%s:%d" % (dom, kernel, tile), C),
+                     "traffic": measured_traffic(dom if kernel == 'split' else "%s:%s:%d:%d" % (dom, kernel, tile, ncell),
+                                                 C),
                      "bytes_per_cell": round(bk, 1), "avg_launch_us": round(float(tk.mean()) * 1e6, 2)},
         "step_hbm_frac_214B": round(214.0 * C * steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
     }

@@ -256,15 +256,16 @@ int mfv_sw_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, double *partia
 
 /* Tiles of the fused mixed step. Tile t is the cell run tile_cell0[t] .. + tile_ncells[t] - 1
  * (at most tile_size cells; a multiple of 64 cells except possibly the mesh's last tile),
- * processed by one block of tile_size threads. Its items are its cells followed by its halo
+ * processed by one block of tile_size threads (a tile may hold fewer cells than the block
+ * has threads, so that its cells and halo are reconstructed in one pass). Its items are its cells followed by its halo
  * (the tile_nhalo[t] face neighbours outside it); its ext list is its items followed by every
  * other cell their reconstructions read (stencil cells, ghost owners, face neighbours):
  * the halo and the extra cells are ext_cells[ext_ptr[t] .. ext_ptr[t+1]). Item q of tile t
  * is row item_ptr[t] + q of the per-item tables lst [K][nb_items] (stencil), lgh
  * [G][nb_items] (ghost owners) and lfn [MF][nb_items] (face neighbours, -1 on walls), which
  * hold positions in the ext list. lnbr [MF][C] is nbr with the cell part replaced by the
- * neighbour's position in the item list. LDS per block: (tile_size + max_halo) * MF * 16
- * bytes, plus max_ext * 48 bytes with stage_states; at most 64 KiB. */
+ * neighbour's position in the item list. LDS per block: max_items * MF * 16 bytes, plus
+ * max_ext * 48 bytes with stage_states; at most 64 KiB. */
 typedef struct mfv_sw_tiles {
     int32_t nb_tiles;
     int32_t tile_size;          /* 128, 256 or 512 */
@@ -272,6 +273,8 @@ typedef struct mfv_sw_tiles {
     int32_t max_ext;
     int32_t nb_items;
     int32_t stage_states;       /* 1: copy every cell state a tile reads into LDS first */
+    int32_t max_items;          /* max over tiles of cells + halo */
+    int32_t pad_;
     const int32_t *tile_cell0;  /* [nb_tiles] */
     const int32_t *tile_ncells; /* [nb_tiles] */
     const int32_t *tile_nhalo;  /* [nb_tiles] */
@@ -283,6 +286,26 @@ typedef struct mfv_sw_tiles {
     const uint16_t *lgh;        /* [G][nb_items] */
     const int16_t *lfn;         /* [MF][nb_items] */
 } mfv_sw_tiles;
+
+/* Host-side construction of the tiles (no GPU needed). The cell ranges are the mesh
+ * ({0}, {C}) or the ensemble instances (each starting at a multiple of 64 and padded to a
+ * multiple of 64). Tiles are grown along the cell order by 64-cell groups while they hold at
+ * most N cells and their cells + halo fit the T threads of a block. nbr [MF][C] is
+ * mfv_sw_mixed.nbr; the stencil tables are those of mfv_mesh (host copies). The arrays of out
+ * are malloc'd; release them with mfv_sw_tiles_host_free. No counterpart in the reference. */
+typedef struct mfv_sw_tiles_host {
+    int32_t nb_tiles, tile_size, max_halo, max_ext, nb_items, max_items, ext_len, MF, K, G;
+    int32_t *tile_cell0, *tile_ncells, *tile_nhalo, *ext_ptr, *ext_cells, *item_ptr, *lnbr;
+    uint16_t *lst, *lgh;
+    int16_t *lfn;
+} mfv_sw_tiles_host;
+
+int mfv_sw_tiles_build(int C, int MF, const int32_t *nbr, int K, const int32_t *lsq_cells,
+                       const int32_t *lsq_ncells, int G, const int32_t *lsq_ghosts,
+                       const int32_t *lsq_nghosts, const int32_t *face_cells, int nranges,
+                       const int32_t *range_cell0, const int32_t *range_ncells, int T, int N,
+                       mfv_sw_tiles_host *out);
+void mfv_sw_tiles_host_free(mfv_sw_tiles_host *t);
 
 /* One fused mixed step (reconstruction, fluxes, update, next dt) from p->state into
  * state_out (a second [C][4] buffer); the face records stay in LDS. Gives the same bits as

@@ -37,8 +37,17 @@ class MfvSwTiles(ctypes.Structure):
     """Mirror of ``struct mfv_sw_tiles`` (include/mfv.h)."""
     _fields_ = [('nb_tiles', ctypes.c_int32), ('tile_size', ctypes.c_int32), ('max_halo', ctypes.c_int32),
                 ('max_ext', ctypes.c_int32), ('nb_items', ctypes.c_int32), ('stage_states', ctypes.c_int32),
+                ('max_items', ctypes.c_int32), ('pad_', ctypes.c_int32),
                 ('tile_cell0', P), ('tile_ncells', P), ('tile_nhalo', P), ('ext_ptr', P), ('ext_cells', P),
                 ('item_ptr', P), ('lnbr', P), ('lst', P), ('lgh', P), ('lfn', P)]
+
+
+class MfvSwTilesHost(ctypes.Structure):
+    """Mirror of ``struct mfv_sw_tiles_host`` (include/mfv.h)."""
+    _fields_ = [(k, ctypes.c_int32) for k in ('nb_tiles', 'tile_size', 'max_halo', 'max_ext', 'nb_items', 'max_items',
+                                              'ext_len', 'MF', 'K', 'G')] + \
+        [(k, P) for k in ('tile_cell0', 'tile_ncells', 'tile_nhalo', 'ext_ptr', 'ext_cells', 'item_ptr', 'lnbr', 'lst',
+                          'lgh', 'lfn')]
 
 
 class MfvSwPacked(ctypes.Structure):
@@ -146,6 +155,9 @@ _SIGNATURES = {
     'mfv_sw_batch_dt_init': [MESH_P, SWM_P, SWB_P, c_double, c_double, P],
     'mfv_sw_batch_step': [MESH_P, SWM_P, SWB_P, c_double, c_double, c_int, c_int, c_int, P],
     'mfv_sw_step_mixed_tiled': [MESH_P, SWM_P, SWT_P, P, c_double, c_double, c_int, P, c_int, P],
+    'mfv_sw_tiles_build': [c_int, c_int, P, c_int, P, P, c_int, P, P, P, c_int, P, P, c_int, c_int,
+                           ctypes.POINTER(MfvSwTilesHost)],
+    'mfv_sw_tiles_host_free': [ctypes.POINTER(MfvSwTilesHost)],
     'mfv_sw_batch_step_tiled': [MESH_P, SWM_P, SWB_P, SWT_P, P, c_double, c_double, c_int, c_int, P],
     'mfv_sw_batch_graph_tiled': [MESH_P, SWM_P, SWB_P, SWT_P, P, c_double, c_double, c_int, c_int,
                                  ctypes.POINTER(c_void_p)],
@@ -156,7 +168,7 @@ _SIGNATURES = {
     'mfv_krylov_tick': [ELL_P, KW_P, c_int, c_int, c_int, P],
     'mfv_ell_spmv': [ELL_P, P, P, c_int, P],
 }
-_RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p}
+_RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p, 'mfv_sw_tiles_host_free': None}
 
 _lib = None
 
@@ -189,6 +201,8 @@ def call(name, *args):
     """Call an int-returning entry point; raise MfvError with mfv_last_error() on failure."""
     lib = load()
     rc = getattr(lib, name)(*args)
+    if _RESTYPE.get(name, c_int) is None:  # void entry points
+        return 0
     if rc != 0:
         msg = lib.mfv_last_error().decode(errors='replace')
         raise MfvError("%s failed (%d): %s" % (name, rc, msg))

@@ -715,7 +715,7 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
     extern __shared__ double4 lds4[];
     double4 *s_st = lds4;                                       // [E] when STAGED
     double2 *s_cc = reinterpret_cast<double2 *>(lds4 + (STAGED ? t.E : 0));
-    float4 *lfr = reinterpret_cast<float4 *>(s_cc + (STAGED ? t.E : 0));  // [(T + H) * MF]
+    float4 *lfr = reinterpret_cast<float4 *>(s_cc + (STAGED ? t.E : 0));  // [max items * MF]
     const int tile = xcd_block();
     const int c0 = t.cell0[tile], nc = t.ncells[tile], nh = t.nhalo[tile];
     const int e0 = t.eptr[tile], ne = t.eptr[tile + 1] - e0;
@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
 
 size_t tiles_lds_bytes(const mfv_sw_tiles *t, int MF)
 {
-    const size_t rec = (size_t)(t->tile_size + t->max_halo) * MF * sizeof(float4);
+    const size_t rec = (size_t)t->max_items * MF * sizeof(float4);
     return t->stage_states ? rec + (size_t)t->max_ext * (sizeof(double4) + sizeof(double2)) : rec;
 }
 
@@ -771,7 +771,8 @@ int check_tiles(const mfv_mesh *m, const mfv_sw_tiles *t)
     MFV_REQUIRE(t != nullptr, "tiles descriptor is NULL");
     MFV_REQUIRE(t->tile_size == 128 || t->tile_size == 256 || t->tile_size == 512,
                 "tile_size %d not in {128, 256, 512}", t->tile_size);
-    MFV_REQUIRE(t->nb_tiles >= 1 && t->max_halo >= 0 && t->max_ext >= 0 && t->nb_items >= 1,
+    MFV_REQUIRE(t->nb_tiles >= 1 && t->max_halo >= 0 && t->max_ext >= 0 && t->nb_items >= 1 &&
+                    t->max_items >= 1,
                 "bad tile counts");
     MFV_REQUIRE(t->tile_cell0 && t->tile_ncells && t->tile_nhalo && t->ext_ptr && t->ext_cells && t->item_ptr &&
                     t->lnbr && t->lst && t->lgh && t->lfn,

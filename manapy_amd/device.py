@@ -245,18 +245,82 @@ class SwPacked(object):
         return ctypes.byref(self.desc)
 
 
-def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None):
+def sw_tiles_native(nbr, C, T, N, lsq, ranges=None):
+    """Tiles built by the library's host builder (mfv_sw_tiles_build): the same tables as
+    sw_tiles_layout, tiles grown under the block's item budget. ranges: (cell0s, ncells)
+    of the instances, default the whole mesh."""
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    nbr = i32(nbr)
+    MF = nbr.shape[0]
+    K, G = int(lsq['lsq_cells'].shape[0]), int(lsq['lsq_ghosts'].shape[0])
+    arrs = [i32(lsq[k]) for k in ('lsq_cells', 'lsq_ncells', 'lsq_ghosts', 'lsq_nghosts', 'face_cells')]
+    r0, rn = (np.array([0]), np.array([C])) if ranges is None else ranges
+    r0, rn = i32(r0), i32(rn)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    h = _lib.MfvSwTilesHost()
+    _lib.call('mfv_sw_tiles_build', int(C), MF, P(nbr), K, P(arrs[0]), P(arrs[1]), G, P(arrs[2]), P(arrs[3]),
+              P(arrs[4]), len(r0), P(r0), P(rn), int(T), int(N), ctypes.byref(h))
+    try:
+        nt, I = h.nb_tiles, h.nb_items
+
+        def take(ptr, n, dt, shape=None):
+            a = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dt))), (n,)).copy()
+            return a.reshape(shape) if shape is not None else a
+        out = {'T': int(T), 'nb_tiles': nt, 'MF': MF, 'max_halo': h.max_halo, 'max_ext': h.max_ext, 'items': I,
+               'max_items': h.max_items,
+               'tile_cell0': take(h.tile_cell0, nt, np.int32), 'tile_ncells': take(h.tile_ncells, nt, np.int32),
+               'tile_nhalo': take(h.tile_nhalo, nt, np.int32), 'ext_ptr': take(h.ext_ptr, nt + 1, np.int32),
+               'ext_cells': take(h.ext_cells, max(h.ext_len, 1), np.int32)[:h.ext_len],
+               'item_ptr': take(h.item_ptr, nt + 1, np.int32),
+               'lnbr': take(h.lnbr, MF * C, np.int32, (MF, C)),
+               'lst': take(h.lst, max(K, 1) * I, np.uint16, (max(K, 1), I)),
+               'lgh': take(h.lgh, max(G, 1) * I, np.uint16, (max(G, 1), I)),
+               'lfn': take(h.lfn, MF * I, np.int16, (MF, I))}
+        out['tile_next'] = (np.diff(out['ext_ptr']) + out['tile_ncells']).astype(np.int32)
+    finally:
+        _lib.call('mfv_sw_tiles_host_free', ctypes.byref(h))
+    return out
+
+
+def sw_tiles_fitted(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None, N=None):
+    """sw_tiles_layout, then tiles whose cells + halo exceed the block of T threads are split
+    into runs of 64-cell multiples until every tile is reconstructed in one pass."""
+    L = sw_tiles_layout(nbr, C, T, tile_cell0, tile_ncells, lsq, N)
+    for _ in range(6):
+        items = np.diff(L['item_ptr'])
+        big = items > T
+        if not big.any():
+            break
+        c0, nc = L['tile_cell0'].astype(np.int64), L['tile_ncells'].astype(np.int64)
+        new0, newn = [], []
+        for a, n, b in zip(c0, nc, big):
+            if b and n > 64:
+                h = max(64, (n // 2 + 63) // 64 * 64)
+                new0 += [a, a + h]
+                newn += [h, n - h]
+            else:
+                new0.append(a)
+                newn.append(n)
+        L = sw_tiles_layout(nbr, C, T, np.array(new0), np.array(newn), lsq)
+    return L
+
+
+def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None, N=None):
     """Tiles of the fused mixed step (mfv_sw_tiles in include/mfv.h) from the neighbour codes
     nbr [MF][C] and the least-squares tables ``lsq`` (lsq_cells [K][C], lsq_ncells,
     lsq_ghosts [G][C], lsq_nghosts, face_cells of a gpu_layout): by default consecutive runs
-    of T cells. A tile's items are its cells then its halo (face neighbours outside it);
+    of N cells (N <= T, default T; T is the block size). A tile's items are its cells then its
+    halo (face neighbours outside it);
     its ext list is its items then every other cell their reconstruction reads (stencil
     cells, ghost owners, face neighbours). Tables give positions in the ext list."""
     nbr = np.asarray(nbr)
     MF = nbr.shape[0]
     if tile_cell0 is None:
-        tile_cell0 = np.arange(0, C, T, dtype=np.int64)
-        tile_ncells = np.minimum(T, C - tile_cell0)
+        N = T if N is None else int(N)
+        if N % 64 or not 0 < N <= T:
+            raise ValueError("cells per tile must be a multiple of 64 in (0, T]")
+        tile_cell0 = np.arange(0, C, N, dtype=np.int64)
+        tile_ncells = np.minimum(N, C - tile_cell0)
     tile_cell0 = np.asarray(tile_cell0, dtype=np.int64)
     tile_ncells = np.asarray(tile_ncells, dtype=np.int64)
     if np.any(tile_ncells > T) or np.any(tile_ncells <= 0):
@@ -364,7 +428,7 @@ def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None):
     lfn[fn_ok] = ext_pos(itf[fn_ok], fn[fn_ok])
     if n_ext.max(initial=0) >= 65535:
         raise ValueError("a tile reads more than 65535 cells")
-    out.update({'max_ext': int(n_ext.max()) if nt else 0, 'items': I,
+    out.update({'max_ext': int(n_ext.max()) if nt else 0, 'items': I, 'max_items': int(n_items.max()) if nt else 0,
                 'ext_ptr': ext_ptr.astype(np.int32), 'ext_cells': ext_cells.astype(np.int32),
                 'item_ptr': item_ptr.astype(np.int32), 'tile_next': n_ext.astype(np.int32),
                 'lst': np.ascontiguousarray(lst, dtype=np.uint16) if K else np.zeros((1, max(I, 1)), np.uint16),
@@ -394,13 +458,14 @@ class SwTiles(object):
         d = _lib.MfvSwTiles()
         d.nb_tiles, d.tile_size, d.max_halo = host['nb_tiles'], host['T'], host['max_halo']
         d.max_ext, d.nb_items, d.stage_states = host['max_ext'], host['items'], int(bool(stage))
+        d.max_items = host['max_items']
         for k in self.ARRAYS:
             setattr(d, k, self.t[k].data_ptr())
         self.desc = d
 
     def lds_bytes(self):
         MF = self.host['MF']
-        rec = (self.host['T'] + self.host['max_halo']) * MF * 16
+        rec = self.host['max_items'] * MF * 16
         return rec + (self.host['max_ext'] * 48 if self.desc.stage_states else 0)
 
     @property
@@ -439,34 +504,36 @@ class SwMixed(object):
         self._layouts = {}
         self._lsq = dmesh.host
 
-    def tiles(self, T=256, stage=True):
-        """The fused step's tiles of T cells (built on first use)."""
-        if (T, stage) not in self._tiles:
-            if T not in self._layouts:
-                self._layouts[T] = sw_tiles_layout(self.nbr_host, self.desc.nb_cells, T, lsq=self._lsq)
-            self._tiles[(T, stage)] = SwTiles(self._layouts[T], self.state.device, stage)
-        return self._tiles[(T, stage)]
+    def tiles(self, T=256, stage=True, N=None):
+        """The fused step's tiles of N (default T) cells for blocks of T threads (built on first use)."""
+        N = T if N is None else N
+        if (T, N, stage) not in self._tiles:
+            if (T, N) not in self._layouts:
+                self._layouts[(T, N)] = sw_tiles_native(self.nbr_host, self.desc.nb_cells, T, N, self._lsq)
+            self._tiles[(T, N, stage)] = SwTiles(self._layouts[(T, N)], self.state.device, stage)
+        return self._tiles[(T, N, stage)]
 
-    def step_tiled(self, dmesh, grav, cfl, order, ring, step, T=256, stage=True, stream=None):
+    def step_tiled(self, dmesh, grav, cfl, order, ring, step, T=256, stage=True, stream=None, N=None):
         """One fused step: reads the current state buffer, writes the other, swaps them."""
-        _lib.call('mfv_sw_step_mixed_tiled', dmesh.ptr, self.ptr, self.tiles(T, stage).ptr, self.state2.data_ptr(),
+        _lib.call('mfv_sw_step_mixed_tiled', dmesh.ptr, self.ptr, self.tiles(T, stage, N).ptr, self.state2.data_ptr(),
                   float(grav), float(cfl), int(order), ring.data_ptr(), int(step),
                   stream if stream is not None else stream_ptr())
         self.state, self.state2 = self.state2, self.state
         self.desc.state = self.state.data_ptr()
 
-    CONFIGS = (('split', 256), ('tiled', 256), ('staged', 128), ('staged', 256))
+    CONFIGS = (('split', 256, 256), ('tiled', 256, 256), ('staged', 128, 128), ('staged', 256, 256),
+               ('staged', 512, 512))
 
-    def step(self, dmesh, grav, cfl, order, ring, step, config=('staged', 256), stream=None):
-        """One step with ``config``: ('split', _) runs the reconstruction and flux kernels,
-        ('tiled', T) / ('staged', T) the fused step on tiles of T cells."""
-        kind, T = config
+    def step(self, dmesh, grav, cfl, order, ring, step, config=('staged', 256, 256), stream=None):
+        """One step with ``config``: ('split', _, _) runs the reconstruction and flux kernels,
+        ('tiled', T, N) / ('staged', T, N) the fused step on tiles of N cells, T threads."""
+        kind, T, N = config
         s = stream if stream is not None else stream_ptr()
         if kind == 'split':
             _lib.call('mfv_sw_recon_mixed', dmesh.ptr, self.ptr, int(order), s)
             _lib.call('mfv_sw_flux_mixed', dmesh.ptr, self.ptr, float(grav), float(cfl), ring.data_ptr(), int(step), s)
         else:
-            self.step_tiled(dmesh, grav, cfl, order, ring, step, T=T, stage=(kind == 'staged'), stream=s)
+            self.step_tiled(dmesh, grav, cfl, order, ring, step, T=T, stage=(kind == 'staged'), stream=s, N=N)
 
     def autotune(self, dmesh, grav, cfl, order, ring, steps=10):
         """Times each configuration on ``steps`` steps (state and ring restored after) and
@@ -478,7 +545,7 @@ class SwMixed(object):
         for cfg in self.CONFIGS:
             if cfg[0] != 'split':
                 try:
-                    if self.tiles(cfg[1], cfg[0] == 'staged').lds_bytes() > 64 * 1024:
+                    if self.tiles(cfg[1], cfg[0] == 'staged', cfg[2]).lds_bytes() > 64 * 1024:
                         continue
                 except ValueError:
                     continue

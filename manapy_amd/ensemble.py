@@ -29,8 +29,14 @@ TILES = (256, 512)
 # launch configurations the autotuner compares: the two-kernel step with its block size, the
 # fused tiled step with its tile size, and the fused step with every cell state a tile reads
 # staged in LDS
-CONFIGS = (tuple(('split', b) for b in BLOCKS) + tuple(('tiled', t) for t in TILES) +
-           tuple(('staged', t) for t in (128, 256)))
+CONFIGS = (tuple(('split', b, b) for b in BLOCKS) + tuple(('tiled', t, t) for t in TILES) +
+           (('staged', 128, 128), ('staged', 256, 256), ('staged', 512, 512)))
+
+
+def _config(c):
+    """(kind, block) or (kind, block, cells per tile) -> (kind, block, cells per tile)."""
+    c = tuple(c)
+    return c if len(c) == 3 else (c[0], int(c[1]), int(c[1]))
 
 
 def _cat_cols(arrs, pads, fill, offsets=None):
@@ -102,17 +108,14 @@ def ensemble_layout(domains):
     return out
 
 
-def ensemble_tiles(host, T=256):
-    """Tiles of the fused step for an ensemble layout: runs of at most T cells that start at
-    an instance's first cell and never cross into the next instance."""
-    from .device import sw_tiles_layout
-    c0s, ns = [], []
-    for a, n in zip(host['inst_cell0'], host['inst_ncells']):
-        npad = (int(n) + 63) // 64 * 64
-        for off in range(0, npad, T):
-            c0s.append(int(a) + off)
-            ns.append(min(T, npad - off))
-    return sw_tiles_layout(host['nbr'], host['C'], T, np.array(c0s), np.array(ns), lsq=host)
+def ensemble_tiles(host, T=256, N=None):
+    """Tiles of the fused step for an ensemble layout: runs of at most N (default T) cells,
+    for blocks of T threads, that start at an instance's first cell and never cross into the
+    next instance."""
+    from .device import sw_tiles_native
+    N = T if N is None else int(N)
+    npad = (host['inst_ncells'].astype(np.int64) + 63) // 64 * 64
+    return sw_tiles_native(host['nbr'], host['C'], T, N, host, (host['inst_cell0'], npad))
 
 
 class SwEnsemble(object):
@@ -164,7 +167,7 @@ class SwEnsemble(object):
         b.wave_inst, b.inst_cell0 = self.t['wave_inst'].data_ptr(), self.t['inst_cell0'].data_ptr()
         b.inst_ncells, b.rings = self.t['inst_ncells'].data_ptr(), self.rings.data_ptr()
         self.batch = b
-        self.config = ('tiled', 256)
+        self.config = ('staged', 256, 256)
         self._tiles = {}
         self.step_index = 0
         self._graph = None
@@ -205,21 +208,23 @@ class SwEnsemble(object):
     def block(self):
         return self.config[1]
 
-    def tiles(self, T, stage=False):
-        if (T, stage) not in self._tiles:
+    def tiles(self, T, stage=False, N=None):
+        N = T if N is None else N
+        if (T, N, stage) not in self._tiles:
             from .device import SwTiles
-            lay = next((v.host for (t, _), v in self._tiles.items() if t == T), None)
-            self._tiles[(T, stage)] = SwTiles(lay if lay is not None else ensemble_tiles(self.host, T), self.device,
-                                              stage)
-        return self._tiles[(T, stage)]
+            lay = next((v.host for (t, n, _), v in self._tiles.items() if (t, n) == (T, N)), None)
+            self._tiles[(T, N, stage)] = SwTiles(lay if lay is not None else ensemble_tiles(self.host, T, N),
+                                                 self.device, stage)
+        return self._tiles[(T, N, stage)]
 
     def _swap(self):
         self.state, self.state2 = self.state2, self.state
         self.mixed.state = self.state.data_ptr()
 
     def step(self, nsteps=1, config=None):
-        """Eager steps with ``config`` = ('split', block) or ('tiled', T) (default: self.config)."""
-        kind, size = config or self.config
+        """Eager steps with ``config`` = ('split', block), ('tiled', T[, N]) or ('staged', T[, N])
+        (default: self.config)."""
+        kind, size, ncell = _config(config or self.config)
         s = stream_ptr()
         for _ in range(nsteps):
             if kind == 'split':
@@ -227,7 +232,7 @@ class SwEnsemble(object):
                           ctypes.byref(self.batch), self.grav, self.cfl, self.order, self.step_index, int(size), s)
             else:
                 _lib.call('mfv_sw_batch_step_tiled', ctypes.byref(self.mesh), ctypes.byref(self.mixed),
-                          ctypes.byref(self.batch), self.tiles(int(size), kind == 'staged').ptr,
+                          ctypes.byref(self.batch), self.tiles(int(size), kind == 'staged', int(ncell)).ptr,
                           self.state2.data_ptr(), self.grav,
                           self.cfl, self.order, self.step_index, s)
                 self._swap()
@@ -236,7 +241,7 @@ class SwEnsemble(object):
     def graph(self, nsteps, config=None):
         """The captured graph of ``nsteps`` steps (a multiple of 6), built on first use. A
         replay ends in the state buffer it started from."""
-        key = (int(nsteps),) + tuple(config or self.config) + (self.state.data_ptr(),)
+        key = (int(nsteps),) + _config(config or self.config) + (self.state.data_ptr(),)
         if self._graph is None or self._graph_key != key:
             self.close_graph()
             h = ctypes.c_void_p()
@@ -245,7 +250,7 @@ class SwEnsemble(object):
                           ctypes.byref(self.batch), self.grav, self.cfl, self.order, key[0], key[2], ctypes.byref(h))
             else:
                 _lib.call('mfv_sw_batch_graph_tiled', ctypes.byref(self.mesh), ctypes.byref(self.mixed),
-                          ctypes.byref(self.batch), self.tiles(key[2], key[1] == 'staged').ptr,
+                          ctypes.byref(self.batch), self.tiles(key[2], key[1] == 'staged', key[3]).ptr,
                           self.state2.data_ptr(), self.grav,
                           self.cfl, self.order, key[0], ctypes.byref(h))
             self._graph, self._graph_key = h, key
@@ -306,7 +311,7 @@ class SwEnsemble(object):
             with open(path) as fh:
                 table = json.load(fh)
         if key in table:
-            self.config = tuple(table[key]['config'])
+            self.config = _config(table[key]['config'])
             return table[key]
         saved = (self.state.clone(), self.rings.clone(), self.step_index)
         st = torch.cuda.current_stream()
@@ -324,7 +329,7 @@ class SwEnsemble(object):
         self.step_index = saved[2]
         best = min(res, key=res.get)
         self.config = best
-        entry = {'config': list(best), 'ms_per_step': {"%s:%d" % k: v for k, v in res.items()},
+        entry = {'config': list(best), 'ms_per_step': {"%s:%d:%d" % k: v for k, v in res.items()},
                  'when': time.time()}
         if path:
             table[key] = entry

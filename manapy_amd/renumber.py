@@ -20,18 +20,20 @@ from .base import Struct
 __all__ = ['hilbert_order_2d', 'morton_order_3d', 'cell_order', 'renumbered', 'RenumberedDomain']
 
 
-def _quantize(x, bits):
-    lo, hi = float(np.min(x)), float(np.max(x))
-    span = hi - lo if hi > lo else 1.0
-    q = np.floor((x - lo) / span * ((1 << bits) - 1) + 0.5)
-    return q.astype(np.int64)
+def _quantize(coords, bits):
+    """Integer grid coordinates of the points, one scale for every axis (a square grid cell,
+    so the curve's locality does not depend on the domain's aspect ratio)."""
+    lo = [float(np.min(c)) for c in coords]
+    span = max(float(np.max(c)) - l for c, l in zip(coords, lo))
+    span = span if span > 0 else 1.0
+    return [np.floor((np.asarray(c, dtype=np.float64) - l) / span * ((1 << bits) - 1) + 0.5).astype(np.int64)
+            for c, l in zip(coords, lo)]
 
 
 def hilbert_order_2d(x, y, bits=16):
     """Indices that sort points by their position along a Hilbert curve of order ``bits``
     (the iterative quadrant-rotation construction of the curve index)."""
-    xi = _quantize(np.asarray(x, dtype=np.float64), bits)
-    yi = _quantize(np.asarray(y, dtype=np.float64), bits)
+    xi, yi = _quantize([x, y], bits)
     d = np.zeros(len(xi), dtype=np.int64)
     s = np.int64(1) << (bits - 1)
     n = np.int64(1) << bits
@@ -63,8 +65,8 @@ def _spread3(v):
 
 def morton_order_3d(x, y, z, bits=21):
     """Indices that sort points by their Morton (Z-order) key."""
-    key = (_spread3(_quantize(x, bits)) | (_spread3(_quantize(y, bits)) << np.uint64(1)) |
-           (_spread3(_quantize(z, bits)) << np.uint64(2)))
+    xi, yi, zi = _quantize([x, y, z], bits)
+    key = _spread3(xi) | (_spread3(yi) << np.uint64(1)) | (_spread3(zi) << np.uint64(2))
     return np.argsort(key, kind='stable')
 
 

@@ -22,7 +22,7 @@ def _instances():
 
 
 @pytest.mark.parametrize("config", [("tiled", 256), ("split", 128), ("tiled", 512), ("staged", 128),
-                                    ("staged", 256)])
+                                    ("staged", 256, 192), ("staged", 256)])
 def test_alone_equals_mix_and_graph_equals_eager(config):
     """Fused tiled and two-kernel steps, eager and in graphs, alone and mixed: all the same bits."""
     from manapy_amd.ensemble import SwEnsemble

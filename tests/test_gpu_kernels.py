@@ -486,9 +486,11 @@ def test_sw_packed_and_mixed_on_hilbert_order(torch, sw_case, oracle_lib):
         same(r.to_old(st[:, k]), model.state[:, k], "mixed %d" % k)
 
 
-@pytest.mark.parametrize("T,hilbert,stage", [(256, True, False), (512, True, False), (256, False, False),
-                                             (128, True, True), (256, True, True), (128, False, True)])
-def test_sw_mixed_tiled_step_equals_two_kernel_step(torch, oracle_lib, T, hilbert, stage):
+@pytest.mark.parametrize("T,N,hilbert,stage", [(256, 256, True, False), (512, 512, True, False),
+                                               (256, 256, False, False), (128, 128, True, True),
+                                               (256, 256, True, True), (128, 128, False, True),
+                                               (256, 192, True, True), (128, 64, True, True)])
+def test_sw_mixed_tiled_step_equals_two_kernel_step(torch, oracle_lib, T, N, hilbert, stage):
     """The fused tiled step (face records in LDS, double-buffered state) gives the bits of
     reconstruction + flux kernels, step after step, including the dt ring."""
     from manapy_amd.device import DeviceMesh, SwMixed
@@ -514,7 +516,7 @@ def test_sw_mixed_tiled_step_equals_two_kernel_step(torch, oracle_lib, T, hilber
         rings = []
         for s in range(25):
             if tiled:
-                p.step_tiled(m, 9.81, 0.8, 2, ring, s, T=T, stage=stage)
+                p.step_tiled(m, 9.81, 0.8, 2, ring, s, T=T, stage=stage, N=N)
             else:
                 call('mfv_sw_recon_mixed', m.ptr, p.ptr, 2)
                 call('mfv_sw_flux_mixed', m.ptr, p.ptr, 9.81, 0.8, P(ring), s)

@@ -53,6 +53,27 @@ def _check(L, nbr, lsq, C):
     assert covered.all()
 
 
+@pytest.mark.parametrize("T,N", [(256, 256), (256, 192), (128, 64), (512, 512)])
+def test_native_builder_fits_the_block_and_matches_numpy(T, N):
+    """The library's host builder grows tiles under the block's item budget; for the tiles it
+    chose, every table equals the numpy construction's."""
+    from manapy_amd.renumber import renumbered
+    from manapy_amd.device import sw_packed_layout, sw_tiles_layout, sw_tiles_native, gpu_layout
+    d = renumbered(make_tri_domain(41, 29, 100., 70., 0.1, 5))
+    P = sw_packed_layout(d)
+    C = int(d.nbcells)
+    lsq = gpu_layout(d)
+    L = sw_tiles_native(P['nbr'], C, T, N, lsq)
+    _check(L, P['nbr'], lsq, C)
+    items = np.diff(L['item_ptr'])
+    assert np.all((items <= T) | (L['tile_ncells'] == 64))
+    assert np.all(L['tile_ncells'] <= N) and np.all(L['tile_cell0'] % 64 == 0)
+    R = sw_tiles_layout(P['nbr'], C, T, L['tile_cell0'], L['tile_ncells'], lsq=lsq)
+    for k in ('tile_nhalo', 'ext_ptr', 'ext_cells', 'item_ptr', 'lnbr', 'lst', 'lgh', 'lfn'):
+        assert np.array_equal(np.asarray(L[k]), np.asarray(R[k])), k
+    assert (L['max_ext'], L['max_items'], L['items']) == (R['max_ext'], R['max_items'], R['items'])
+
+
 @pytest.mark.parametrize("T", [256, 512])
 def test_tiles_of_a_renumbered_mesh(T):
     from manapy_amd.renumber import renumbered
