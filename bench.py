@@ -133,6 +133,17 @@ def mixed_bytes_per_cell(MF, K_avg, G_avg):
     return recon, flux
 
 
+def fused_bytes_per_cell(MF, K_avg, G_avg):
+    """Algorithmic HBM bytes per cell of the fused float64 tiled step (records of other cells
+    are their own; the halo's repeated reconstruction is extra work, not extra bytes)."""
+    return (64 + 32                     # own record read, new eta/hu/hv/h written
+            + 4 + 1                     # face count, residual order
+            + 4 * K_avg + 4             # cellnid entries + count
+            + G_avg * (4 + 4 + 16) + 4  # ghostnid entries, owner, ghost centre + count
+            + 4 * MF + 4 * MF           # global and block-local neighbour codes
+            + 32 * MF + 32 * MF)        # cfrec, cfgeo
+
+
 def tiled_bytes_per_cell(MF, K_avg, G_avg):
     """Algorithmic HBM bytes per cell of the fused tiled mixed step (the halo's repeated
     reconstruction is extra work, not extra algorithmic bytes)."""
@@ -400,8 +411,10 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-steps', type=int, default=20)
     ap.add_argument('--cpu-budget', type=float, default=20.0)
-    ap.add_argument('--path', choices=('packed', 'unpacked'), default='packed',
-                    help='packed: per-cell records + per-face MUSCL values (default); unpacked: SoA kernels')
+    ap.add_argument('--path', choices=('auto', 'fused', 'packed', 'unpacked'), default='auto',
+                    help='float64 path: fused (one tiled launch per step, records in LDS), packed (per-cell records '
+                         '+ per-face MUSCL values, two kernels), unpacked (SoA kernels), auto (the faster of fused '
+                         'and packed on this mesh)')
     ap.add_argument('--no-mixed', action='store_true', help='skip the mixed-precision run of the same case')
     ap.add_argument('--ensemble', type=int, default=256, help='instances of the ensemble run (0: skip)')
     ap.add_argument('--ensemble-steps', type=int, default=204)
@@ -461,18 +474,52 @@ def main():
     _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
 
     step_no = [0]
-    if args.path == 'packed':
+    path, fused_T, path_ms = args.path, None, None
+    if path in ('auto', 'fused', 'packed'):
         from manapy_amd.device import SwPacked
         pk = SwPacked(m)
-        _lib.call('mfv_sw_pack', m.ptr, pk.ptr, S[0][0].data_ptr(), S[0][1].data_ptr(), S[0][2].data_ptr(),
-                  tZ.data_ptr(), zgrad.data_ptr(), 2, s)
-        names = ('k_sw_recon', 'k_sw_flux_packed')
+        pk.pack(m, S[0][0], S[0][1], S[0][2], tZ, zgrad, 2, s)
+        if path in ('auto', 'fused'):
+            cands = [('fused', T) for T in (128, 256) if pk.lds_bytes(T) <= 64 * 1024]
+            cands += [('packed', None)] if path == 'auto' else []
+            path_ms = {}
+            ring0 = ring.clone()
+            for kind, T in cands:  # a few steps of each, then the state and ring are restored
+                ring.copy_(ring0)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                for k in range(13):
+                    if k == 3:
+                        e0.record()
+                    if kind == 'fused':
+                        pk.step_tiled(m, G, CFL, 2, ring, k, T=T, stream=s)
+                    else:
+                        _lib.call('mfv_sw_recon', m.ptr, pk.ptr, 2, s)
+                        _lib.call('mfv_sw_flux_packed', m.ptr, pk.ptr, G, CFL, ring.data_ptr(), k, s)
+                e1.record()
+                e1.synchronize()
+                path_ms["%s:%s" % (kind, T)] = e0.elapsed_time(e1) / 10
+                pk.pack(m, S[0][0], S[0][1], S[0][2], tZ, zgrad, 2, s)
+            ring.copy_(ring0)
+            best = min(path_ms, key=path_ms.get)
+            path = best.split(':')[0]
+            fused_T = int(best.split(':')[1]) if path == 'fused' else None
+            log("fp64: path %s %s" % (best, {k: round(v, 4) for k, v in path_ms.items()}))
+        if path == 'fused':
+            names = ('k_sw_step_packed_tiled', 'k_dt_finish')
 
-        def launch_a(k):
-            _lib.call('mfv_sw_recon', m.ptr, pk.ptr, 2, s)
+            def launch_a(k):
+                pk.step_tiled(m, G, CFL, 2, ring, k, T=fused_T, stream=s)
 
-        def launch_b(k):
-            _lib.call('mfv_sw_flux_packed', m.ptr, pk.ptr, G, CFL, ring.data_ptr(), k, s)
+            def launch_b(k):
+                pass
+        else:
+            names = ('k_sw_recon', 'k_sw_flux_packed')
+
+            def launch_a(k):
+                _lib.call('mfv_sw_recon', m.ptr, pk.ptr, 2, s)
+
+            def launch_b(k):
+                _lib.call('mfv_sw_flux_packed', m.ptr, pk.ptr, G, CFL, ring.data_ptr(), k, s)
     else:
         names = ('k_sw_gradients', 'k_sw_step')
 
@@ -512,7 +559,7 @@ def main():
     n_fp64_steps = step_no[0]
     h64 = torch.empty(C, dtype=torch.float64, device='cuda')
     hu64, hv64 = torch.empty_like(h64), torch.empty_like(h64)
-    if args.path == 'packed':
+    if path in ('packed', 'fused'):
         _lib.call('mfv_sw_unpack', m.ptr, pk.ptr, h64.data_ptr(), hu64.data_ptr(), hv64.data_ptr(), s)
     else:
         last = S[n_fp64_steps % 2]
@@ -529,13 +576,16 @@ def main():
 
     K_avg = float(d.cells.cellnid[:, -1].mean())
     G_avg = float(d.cells.ghostnid[:, -1].mean())
-    if args.path == 'packed':
-        bg, bfx = kernel_bytes_per_cell_packed(int(d.cells.faceid.shape[1] - 1), K_avg, G_avg)
+    MF = int(d.cells.faceid.shape[1] - 1)
+    if path == 'fused':
+        bg, bfx = fused_bytes_per_cell(MF, K_avg, G_avg), 0.
+    elif path == 'packed':
+        bg, bfx = kernel_bytes_per_cell_packed(MF, K_avg, G_avg)
     else:
         bg, bfx = kernel_bytes_per_cell(F, C, K_avg, G_avg)
     ach_flux = bfx * C / t_flux.mean() / 1e9
     ach_grad = bg * C / t_grad.mean() / 1e9
-    dom_is_flux = t_flux.mean() >= t_grad.mean()
+    dom_is_flux = path != 'fused' and t_flux.mean() >= t_grad.mean()
     ach, bpc, tk = (ach_flux, bfx, t_flux) if dom_is_flux else (ach_grad, bg, t_grad)
     mcell = C * args.steps / t_loop / 1e6
     step_bytes_survey = 356.0  # FP64 B per cell-update of the whole C2 step (algorithmic model)
@@ -555,16 +605,21 @@ def main():
         "data": "synthetic (seeded jittered mesh, dam break over seeded Gaussian bumps)",
         "config": {"workload": "shallow-water dam break, 2D, MUSCL + LSQ + Barth-Jespersen + Rusanov, walls",
                    "cells": C, "faces": F, "nx": args.nx, "cfl": CFL, "order": 2, "precision": "fp64 plain",
-                   "path": args.path, "cell_order": args.renumber},
+                   "path": path, "tile": fused_T, "cell_order": args.renumber},
         "roofline": {"bound": "hbm", "kernel": names[1] if dom_is_flux else names[0],
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(names[1] if dom_is_flux else names[0], C),
+                     "traffic": measured_traffic(names[1] if dom_is_flux else
+                                                 (names[0] + (":%d" % fused_T if path == 'fused' else '')), C),
                      "bytes_per_cell": round(bpc, 1), "avg_launch_us": round(float(tk.mean()) * 1e6, 2)},
         "kernels": {names[0]: {"avg_us": round(float(t_grad.mean()) * 1e6, 2),
                                "bytes_per_cell": round(bg, 1), "GB/s": round(ach_grad, 1)},
                     names[1]: {"avg_us": round(float(t_flux.mean()) * 1e6, 2),
-                               "bytes_per_cell": round(bfx, 1), "GB/s": round(ach_flux, 1)}},
+                               "bytes_per_cell": round(bfx, 1), "GB/s": round(ach_flux, 1)}}
+        if path != 'fused' else
+        {names[0]: {"avg_us": round(float(t_grad.mean()) * 1e6, 2), "bytes_per_cell": round(bg, 1),
+                    "GB/s": round(ach_grad, 1), "tile": fused_T}},
+        "fp64_path_ms_per_step": path_ms,
         "step_hbm_frac_356B": round(step_bytes_survey * C * args.steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
         "mesh_build_s": round(t_mesh, 2),
     }

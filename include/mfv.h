@@ -311,6 +311,14 @@ void mfv_sw_tiles_host_free(mfv_sw_tiles_host *t);
  * state_out (a second [C][4] buffer); the face records stay in LDS. Gives the same bits as
  * mfv_sw_recon_mixed followed by mfv_sw_flux_mixed. No counterpart call in the reference:
  * it replaces the pair above (explicitscheme_convective_SW + update_SW in one launch). */
+/* The float64 packed step fused on tiles (the same mfv_sw_tiles, built from the packed
+ * neighbour codes): reads p->cellrec, writes cellrec_out (a second [C][8] buffer holding the
+ * same static fields); the records a tile reads and its face values stay in LDS
+ * (max_ext * 64 + max_items * MF * 32 bytes <= 64 KiB). Gives the bits of mfv_sw_recon followed
+ * by mfv_sw_flux_packed. */
+int mfv_sw_step_packed_tiled(const mfv_mesh *m, const mfv_sw_packed *p, const mfv_sw_tiles *t,
+                             double *cellrec_out, double grav, double cfl, int order, double *dt_ring,
+                             int step, void *stream);
 int mfv_sw_step_mixed_tiled(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_tiles *t,
                             double *state_out, double grav, double cfl, int order, double *dt_ring,
                             int step, void *stream);

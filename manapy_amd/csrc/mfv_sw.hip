@@ -497,14 +497,49 @@ __device__ __forceinline__ void load_ehv(const double *__restrict__ cellrec, int
 // the neighbour records in independent groups of four, so a cell waits for a few memory
 // latencies instead of two per stencil entry; the sums keep the stencil order. KMAX = 0:
 // plain loop (stencils wider than the largest KMAX).
-template <int MF, int KMAX>
-__global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, double om)
+// Where a reconstruction reads cell records: by cell index in the record array, or by
+// position in a block's LDS copy of the records its tile reads (stencil, ghost owners and
+// face neighbours then come from the tile's tables).
+struct PackedCells {
+    const Mesh &m;
+    const double *__restrict__ rec;
+    int i;
+    __device__ __forceinline__ int self() const { return i; }
+    __device__ __forceinline__ void ehv(int h, double &cx, double &cy, double &e, double &u, double &v) const
+    {
+        load_ehv(rec, h, cx, cy, e, u, v);
+    }
+    __device__ __forceinline__ int stencil(int j) const { return m.lsq_cells[(size_t)j * m.C + i]; }
+    __device__ __forceinline__ int ghost_owner(int, int f) const { return m.face_cells[2 * f]; }
+    __device__ __forceinline__ int face_nb(int, int code) const { return code >> 3; }
+};
+
+struct PackedStaged {
+    const double *rec;  // LDS copy: 8 doubles per cell of the tile's ext list
+    const uint16_t *__restrict__ lst;
+    const uint16_t *__restrict__ lgh;
+    const int16_t *__restrict__ lfn;
+    size_t row, I;
+    int q;
+    __device__ __forceinline__ int self() const { return q; }
+    __device__ __forceinline__ void ehv(int h, double &cx, double &cy, double &e, double &u, double &v) const
+    {
+        load_ehv(rec, h, cx, cy, e, u, v);
+    }
+    __device__ __forceinline__ int stencil(int j) const { return lst[(size_t)j * I + row]; }
+    __device__ __forceinline__ int ghost_owner(int j, int) const { return lgh[(size_t)j * I + row]; }
+    __device__ __forceinline__ int face_nb(int s, int) const { return lfn[(size_t)s * I + row]; }
+};
+
+// Reconstruction of cell i into out[2 s], out[2 s + 1] = (eta_f, hu_f), (hv_f, z_f) for its nf
+// face slots; returns nf.
+template <int MF, int KMAX, class Acc>
+__device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed &p, const Acc &A, int i,
+                                             double om, double2 *out)
 {
-    int i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= m.C) return;
     const size_t C = m.C;
     double cx, cy, ei, ui, vi;
-    load_ehv(p.cellrec, i, cx, cy, ei, ui, vi);
+    A.ehv(A.self(), cx, cy, ei, ui, vi);
     double ixx = 0., iyy = 0., ixy = 0.;
     double je0 = 0., je1 = 0., ju0 = 0., ju1 = 0., jv0 = 0., jv1 = 0.;
     const int nn = m.lsq_ncells[i];
@@ -524,32 +559,32 @@ __global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, do
     if (KMAX > 0) {
         int cidx[KMAX > 0 ? KMAX : 1];
 #pragma unroll
-        for (int j = 0; j < KMAX; j++) cidx[j] = (j < nn) ? m.lsq_cells[(size_t)j * C + i] : i;
+        for (int j = 0; j < KMAX; j++) cidx[j] = (j < nn) ? A.stencil(j) : A.self();
 #pragma unroll
         for (int j0 = 0; j0 < KMAX; j0 += 4) {
             double px[4], py[4], ec[4], uc[4], vc[4];
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (j0 + k < nn) load_ehv(p.cellrec, cidx[j0 + k], px[k], py[k], ec[k], uc[k], vc[k]);
+                if (j0 + k < nn) A.ehv(cidx[j0 + k], px[k], py[k], ec[k], uc[k], vc[k]);
 #pragma unroll
             for (int k = 0; k < 4; k++)
                 if (j0 + k < nn) accum(px[k], py[k], ec[k], uc[k], vc[k]);
         }
     } else {
         for (int j = 0; j < nn; j++) {
-            int c = m.lsq_cells[(size_t)j * C + i];
+            int c = A.stencil(j);
             double px, py, ec, uc, vc;
-            load_ehv(p.cellrec, c, px, py, ec, uc, vc);
+            A.ehv(c, px, py, ec, uc, vc);
             accum(px, py, ec, uc, vc);
         }
     }
     const int ng = m.lsq_nghosts[i];
     for (int j = 0; j < ng; j++) {
         int f = m.lsq_ghosts[(size_t)j * C + i];
-        int o = m.face_cells[2 * f];
+        int o = A.ghost_owner(j, f);
         const double *g = m.face_ghost + 3 * (size_t)f;
         double px, py, eo, uo, vo;
-        load_ehv(p.cellrec, o, px, py, eo, uo, vo);
+        A.ehv(o, px, py, eo, uo, vo);
         double jx = g[0] - cx, jy = g[1] - cy;
         ixx += jx * jx;
         iyy += jy * jy;
@@ -576,7 +611,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, do
         double ea, ua, va, eb, ub, vb;
         if (code >= 0) {
             double px, py, ej, uj, vj;
-            load_ehv(p.cellrec, code >> 3, px, py, ej, uj, vj);
+            A.ehv(A.face_nb(s, code), px, py, ej, uj, vj);
             if (code & 1) {
                 ea = ej; ua = uj; va = vj;
                 eb = ei; ub = ui; vb = vi;
@@ -631,15 +666,107 @@ __global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, do
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         if (s >= nf) break;
-        double2 *o = reinterpret_cast<double2 *>(p.facerec + 4 * (MF * (size_t)i + s));
         double2 a, b;
         a.x = ei + om * pe * (ex * rx[s] + ey * ry[s]);
         a.y = ui + om * pu * (ux * rx[s] + uy * ry[s]);
         b.x = vi + om * pv * (vx * rx[s] + vy * ry[s]);
         b.y = zf[s];
-        o[0] = a;
-        o[1] = b;
+        out[2 * s] = a;
+        out[2 * s + 1] = b;
     }
+    return nf;
+}
+
+template <int MF, int KMAX>
+__global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, double om)
+{
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    double2 rec[2 * MF];
+    const int nf = sw_recon_core<MF, KMAX>(m, p, PackedCells{m, p.cellrec, i}, i, om, rec);
+    double2 *o = reinterpret_cast<double2 *>(p.facerec + 4 * (MF * (size_t)i));
+#pragma unroll
+    for (int s = 0; s < MF; s++)
+        if (s < nf) {
+            o[2 * s] = rec[2 * s];
+            o[2 * s + 1] = rec[2 * s + 1];
+        }
+}
+
+// One cell of the step: fluxes + bathymetry terms + update_SW from the record rin, new eta, hu,
+// hv, h written into rout (rout may be rin); returns the next-dt candidate. rec(k) is face
+// record k (= MF * cell + slot) of the array the codes index; own: this cell's position there.
+template <int MF, class Recs, class Codes>
+__device__ __forceinline__ double sw_flux_packed_core(const mfv_sw_packed &p, int i, int own,
+                                                      const double *__restrict__ rin, double *rout, Recs rec,
+                                                      Codes code_of, int nf, unsigned ord, double g, double cfl,
+                                                      double dt)
+{
+    const double2 *r2 = reinterpret_cast<const double2 *>(rin);
+    double2 c1 = r2[1], c2 = r2[2], c3 = r2[3];
+    const double hu_i = c1.y, hv_i = c2.x, h_i = c2.y, Z_i = c3.x, vol = c3.y;
+    double r0 = 0., r1 = 0., rr2 = 0., s1 = 0., s2 = 0.;
+    // contributions in the residual order; slot s of the r-th one is in bits 2r..2r+1
+#pragma unroll
+    for (int r = 0; r < MF; r++) {
+        if (r >= nf) break;
+        const int s = (ord >> (2 * r)) & 3;
+        const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
+        double2 ga = gq[0], gb = gq[1];
+        const double n0 = ga.x, n1 = ga.y, mes = gb.x;
+        const double2 *oq = rec(MF * (size_t)own + s);
+        double2 oa = oq[0], ob = oq[1];
+        const int code = code_of(s);
+        double eR, uR, vR, zR;
+        if (code >= 0) {
+            int j = code >> 3, k = (code >> 1) & 3;
+            const double2 *nq = rec(MF * (size_t)j + k);
+            double2 na = nq[0], nb = nq[1];
+            eR = na.x; uR = na.y; vR = nb.x; zR = nb.y;
+        } else {
+            eR = oa.x; uR = -1 * oa.y; vR = -1 * ob.x; zR = ob.y;
+        }
+        double F0, F1, F2, bL, bR;
+        if (code < 0 || (code & 1) == 0) {
+            sw_face(oa.x, oa.y, ob.x, ob.y, h_i, Z_i, eR, uR, vR, zR, h_i, Z_i, n0, n1, mes, g, F0, F1, F2, bL, bR);
+            r0 -= F0; r1 -= F1; rr2 -= F2;
+            s1 -= bL * n0;
+            s2 -= bL * n1;
+        } else {
+            sw_face(eR, uR, vR, zR, h_i, Z_i, oa.x, oa.y, ob.x, ob.y, h_i, Z_i, n0, n1, mes, g, F0, F1, F2, bL, bR);
+            r0 += F0; r1 += F1; rr2 += F2;
+            s1 -= -(bR * n0);
+            s2 -= -(bR * n1);
+        }
+    }
+    const double s0 = 0.;
+    double hn = h_i + dt * (r0 + s0) / vol;
+    double un = hu_i + dt * ((r1 + s1) / vol + 0. + 0.);
+    double vn = hv_i + dt * ((rr2 + s2) / vol + 0. + 0.);
+    double2 w1, w2;
+    w1.x = hn + Z_i;
+    w1.y = un;
+    w2.x = vn;
+    w2.y = hn;
+    reinterpret_cast<double2 *>(rout)[1] = w1;
+    reinterpret_cast<double2 *>(rout)[2] = w2;
+    // time_step_SW of the new state, faces in faceid order
+    const double velson = sqrt(g * hn);
+    double lam = 0.;
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        if (s >= nf) break;
+        const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
+        double2 ga = gq[0], gb = gq[1];
+        double n0 = ga.x, n1 = ga.y, mesu = gb.x;
+        double u_n = fabs(un / hn * n0 + vn / hn * n1);
+        double lam_convect = u_n / mesu + velson;
+        lam += lam_convect * mesu;
+        double mes = sqrt(n0 * n0 + n1 * n1);
+        double lam_diff = 0. * (mes * mes) + 0. * (mes * mes);
+        lam += lam_diff / vol;
+    }
+    return cfl * vol / lam;
 }
 
 // Fluxes + bathymetry terms + update_SW + next dt, in place on cellrec.
@@ -653,78 +780,83 @@ __global__ void __launch_bounds__(kBlock) k_sw_flux_packed(Mesh m, mfv_sw_packed
     if (i < m.C) {
         const size_t C = m.C;
         double *rec = p.cellrec + 8 * (size_t)i;
-        const double2 *r2 = reinterpret_cast<const double2 *>(rec);
-        double2 c1 = r2[1], c2 = r2[2], c3 = r2[3];
-        const double hu_i = c1.y, hv_i = c2.x, h_i = c2.y, Z_i = c3.x, vol = c3.y;
-        const int nf = m.cell_nfaces[i];
-        const unsigned ord = p.order[i];
-        double r0 = 0., r1 = 0., rr2 = 0., s1 = 0., s2 = 0.;
-        // contributions in the residual order; slot s of the r-th one is in bits 2r..2r+1
-#pragma unroll
-        for (int r = 0; r < MF; r++) {
-            if (r >= nf) break;
-            const int s = (ord >> (2 * r)) & 3;
-            const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
-            double2 ga = gq[0], gb = gq[1];
-            const double n0 = ga.x, n1 = ga.y, mes = gb.x;
-            const double2 *oq = reinterpret_cast<const double2 *>(p.facerec + 4 * (MF * (size_t)i + s));
-            double2 oa = oq[0], ob = oq[1];
-            const int code = p.nbr[(size_t)s * C + i];
-            double eR, uR, vR, zR;
-            if (code >= 0) {
-                int j = code >> 3, k = (code >> 1) & 3;
-                const double2 *nq = reinterpret_cast<const double2 *>(p.facerec + 4 * (MF * (size_t)j + k));
-                double2 na = nq[0], nb = nq[1];
-                eR = na.x; uR = na.y; vR = nb.x; zR = nb.y;
-            } else {
-                eR = oa.x; uR = -1 * oa.y; vR = -1 * ob.x; zR = ob.y;
-            }
-            double F0, F1, F2, bL, bR;
-            if (code < 0 || (code & 1) == 0) {
-                sw_face(oa.x, oa.y, ob.x, ob.y, h_i, Z_i, eR, uR, vR, zR, h_i, Z_i, n0, n1, mes, g, F0, F1, F2, bL,
-                        bR);
-                r0 -= F0; r1 -= F1; rr2 -= F2;
-                s1 -= bL * n0;
-                s2 -= bL * n1;
-            } else {
-                sw_face(eR, uR, vR, zR, h_i, Z_i, oa.x, oa.y, ob.x, ob.y, h_i, Z_i, n0, n1, mes, g, F0, F1, F2, bL,
-                        bR);
-                r0 += F0; r1 += F1; rr2 += F2;
-                s1 -= -(bR * n0);
-                s2 -= -(bR * n1);
-            }
-        }
-        const double dt = ring.cur[0];
-        const double s0 = 0.;
-        double hn = h_i + dt * (r0 + s0) / vol;
-        double un = hu_i + dt * ((r1 + s1) / vol + 0. + 0.);
-        double vn = hv_i + dt * ((rr2 + s2) / vol + 0. + 0.);
-        double2 w1, w2;
-        w1.x = hn + Z_i;
-        w1.y = un;
-        w2.x = vn;
-        w2.y = hn;
-        reinterpret_cast<double2 *>(rec)[1] = w1;
-        reinterpret_cast<double2 *>(rec)[2] = w2;
-        // time_step_SW of the new state, faces in faceid order
-        const double velson = sqrt(g * hn);
-        double lam = 0.;
-#pragma unroll
-        for (int s = 0; s < MF; s++) {
-            if (s >= nf) break;
-            const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
-            double2 ga = gq[0], gb = gq[1];
-            double n0 = ga.x, n1 = ga.y, mesu = gb.x;
-            double u_n = fabs(un / hn * n0 + vn / hn * n1);
-            double lam_convect = u_n / mesu + velson;
-            lam += lam_convect * mesu;
-            double mes = sqrt(n0 * n0 + n1 * n1);
-            double lam_diff = 0. * (mes * mes) + 0. * (mes * mes);
-            lam += lam_diff / vol;
-        }
-        dloc = cfl * vol / lam;
+        const double *fr = p.facerec;
+        const int32_t *__restrict__ nbr = p.nbr;
+        auto recs = [fr](size_t k) { return reinterpret_cast<const double2 *>(fr + 4 * k); };
+        auto code_of = [nbr, C, i](int s) { return nbr[(size_t)s * C + i]; };
+        dloc = sw_flux_packed_core<MF>(p, i, i, rec, rec, recs, code_of, m.cell_nfaces[i], p.order[i], g, cfl,
+                                       ring.cur[0]);
     }
     wave_min_part(dtpart, (size_t)i, dloc);
+}
+
+// Fused float64 step on tiles (the layout of mfv_sw_tiles, built from the same neighbour
+// codes): the block copies every cell record its tile reads into LDS, reconstructs its cells
+// and halo into LDS face records, then updates its cells from LDS into cellrec_out (the
+// step is double-buffered: neighbouring tiles still read the old records). The per-cell
+// functions are those of the two-kernel path, so the bits are the same.
+struct TilesP {
+    int E;
+    const int32_t *__restrict__ cell0;
+    const int32_t *__restrict__ ncells;
+    const int32_t *__restrict__ nhalo;
+    const int32_t *__restrict__ eptr;
+    const int32_t *__restrict__ ext;
+    const int32_t *__restrict__ iptr;
+    const int32_t *__restrict__ lnbr;
+    const uint16_t *__restrict__ lst;
+    const uint16_t *__restrict__ lgh;
+    const int16_t *__restrict__ lfn;
+    int I;
+};
+
+template <int MF, int T, int KMAX>
+__global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packed p, TilesP t,
+                                                            double *__restrict__ rec_out, double om, double g,
+                                                            double cfl, DtRing ring, double *__restrict__ dtpart)
+{
+    extern __shared__ double2 lds2[];
+    double *s_rec = reinterpret_cast<double *>(lds2);          // [E][8]
+    double2 *lfr = lds2 + 4 * (size_t)t.E;                      // [items][MF][2]
+    const int tile = xcd_block();
+    const int c0 = t.cell0[tile], nc = t.ncells[tile], nh = t.nhalo[tile];
+    const int e0 = t.eptr[tile], ne = t.eptr[tile + 1] - e0;
+    const size_t r0 = t.iptr[tile];
+    ring_advance(ring);
+    for (int e = threadIdx.x; e < nc + ne; e += T) {
+        const int c = (e < nc) ? c0 + e : t.ext[e0 + (e - nc)];
+        const double2 *src = reinterpret_cast<const double2 *>(p.cellrec + 8 * (size_t)c);
+        double2 *dst = lds2 + 4 * (size_t)e;
+#pragma unroll
+        for (int k = 0; k < 4; k++) dst[k] = src[k];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nc + nh; q += T) {
+        const int c = (q < nc) ? c0 + q : t.ext[e0 + (q - nc)];
+        double2 rec[2 * MF];
+        const int nf = sw_recon_core<MF, KMAX>(m, p, PackedStaged{s_rec, t.lst, t.lgh, t.lfn, r0 + q, (size_t)t.I, q},
+                                               c, om, rec);
+#pragma unroll
+        for (int s = 0; s < MF; s++)
+            if (s < nf) {
+                lfr[2 * (MF * q + s)] = rec[2 * s];
+                lfr[2 * (MF * q + s) + 1] = rec[2 * s + 1];
+            }
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if ((q & ~63) >= nc) return;  // the whole wave lies past the tile: it owns no partial
+    double dloc = 1e300;
+    if (q < nc) {
+        const int i = c0 + q;
+        const int32_t *__restrict__ lnbr = t.lnbr;
+        const size_t C = m.C;
+        auto recs = [lfr](size_t k) { return lfr + 2 * k; };
+        auto code_of = [lnbr, C, i](int s) { return lnbr[(size_t)s * C + i]; };
+        dloc = sw_flux_packed_core<MF>(p, i, q, s_rec + 8 * (size_t)q, rec_out + 8 * (size_t)i, recs, code_of,
+                                       m.cell_nfaces[i], p.order[i], g, cfl, ring.cur[0]);
+    }
+    wave_min_part(dtpart, (size_t)c0 + q, dloc);
 }
 
 }  // namespace
@@ -783,6 +915,72 @@ extern "C" int mfv_sw_recon(const mfv_mesh *m, const mfv_sw_packed *p, int order
     hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p,
                        (double)(order - 1));
     return check_launch("k_sw_recon");
+}
+
+static int check_tiles_packed(const mfv_mesh *m, const mfv_sw_tiles *t)
+{
+    MFV_REQUIRE(t != nullptr, "tiles descriptor is NULL");
+    MFV_REQUIRE(t->tile_size == 128 || t->tile_size == 256 || t->tile_size == 512,
+                "tile_size %d not in {128, 256, 512}", t->tile_size);
+    MFV_REQUIRE(t->nb_tiles >= 1 && t->max_ext >= 1 && t->nb_items >= 1 && t->max_items >= 1, "bad tile counts");
+    MFV_REQUIRE(t->tile_cell0 && t->tile_ncells && t->tile_nhalo && t->ext_ptr && t->ext_cells && t->item_ptr &&
+                    t->lnbr && t->lst && t->lgh && t->lfn,
+                "tiles descriptor has NULL arrays");
+    const size_t lds = (size_t)t->max_ext * 64 + (size_t)t->max_items * m->max_cell_faces * 32;
+    MFV_REQUIRE(lds <= 64 * 1024, "tiles of %d cells need %zu bytes of LDS for the float64 step (> 64 KiB): use "
+                "smaller tiles", t->tile_size, lds);
+    return MFV_OK;
+}
+
+template <int MF, int T>
+static void launch_packed_tiled(const mfv_mesh *m, const mfv_sw_packed *p, const mfv_sw_tiles *t, double *out,
+                                double om, double g, double cfl, DtRing r, double *part, hipStream_t st)
+{
+    const TilesP tp{t->max_ext, t->tile_cell0, t->tile_ncells, t->tile_nhalo, t->ext_ptr, t->ext_cells,
+                    t->item_ptr, t->lnbr, t->lst, t->lgh, t->lfn, t->nb_items};
+    const size_t lds = (size_t)t->max_ext * 64 + (size_t)t->max_items * MF * 32;
+    if (m->lsq_width <= 16)
+        hipLaunchKernelGGL((k_sw_step_packed_tiled<MF, T, 16>), dim3(t->nb_tiles), dim3(T), lds, st, to_kernel(m),
+                           *p, tp, out, om, g, cfl, r, part);
+    else
+        hipLaunchKernelGGL((k_sw_step_packed_tiled<MF, T, 0>), dim3(t->nb_tiles), dim3(T), lds, st, to_kernel(m),
+                           *p, tp, out, om, g, cfl, r, part);
+}
+
+extern "C" int mfv_sw_step_packed_tiled(const mfv_mesh *m, const mfv_sw_packed *p, const mfv_sw_tiles *t,
+                                        double *cellrec_out, double grav, double cfl, int order, double *dt_ring_p,
+                                        int step, void *stream)
+{
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_packed(m, p)) return rc;
+    if (int rc = check_tiles_packed(m, t)) return rc;
+    MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    MFV_REQUIRE(dt_ring_p && cellrec_out && cellrec_out != p->cellrec, "cellrec_out must be a second record buffer");
+    if (m->nb_cells == 0) return MFV_OK;
+    DtRing r = dt_ring(dt_ring_p, step);
+    const size_t nw = ((size_t)m->nb_cells + 63) / 64;
+    double *part = dt_scratch(nw, as_stream(stream));
+    if (!part) return MFV_ENOMEM;
+    const double om = (double)(order - 1);
+    hipStream_t st = as_stream(stream);
+    const bool tri = m->max_cell_faces == 3;
+    switch (t->tile_size) {
+    case 128:
+        if (tri) launch_packed_tiled<3, 128>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        else launch_packed_tiled<4, 128>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        break;
+    case 256:
+        if (tri) launch_packed_tiled<3, 256>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        else launch_packed_tiled<4, 256>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        break;
+    default:
+        if (tri) launch_packed_tiled<3, 512>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        else launch_packed_tiled<4, 512>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        break;
+    }
+    if (int rc = check_launch("k_sw_step_packed_tiled")) return rc;
+    return dt_finish(part, nw, r.next, st);
 }
 
 extern "C" int mfv_sw_flux_packed(const mfv_mesh *m, const mfv_sw_packed *p, double grav, double cfl,

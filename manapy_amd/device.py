@@ -223,6 +223,11 @@ class SwPacked(object):
         C = dmesh.C
         MF = host['MF']
         dev = dmesh.device
+        self.nbr_host = host['nbr']
+        self._lsq = dmesh.host
+        self._tiles = {}
+        self.cellrec2 = None  # second record buffer of the fused tiled step (on first use)
+        self._rec_synced = None
         self.cellrec = torch.zeros((C, 8), dtype=torch.float64, device=dev)
         self.facerec = torch.zeros((C, MF, 4), dtype=torch.float64, device=dev)
         self.cfrec = torch.as_tensor(host['cfrec'], device=dev)
@@ -243,6 +248,39 @@ class SwPacked(object):
     @property
     def ptr(self):
         return ctypes.byref(self.desc)
+
+    def pack(self, dmesh, h, hu, hv, Z, zgrad, order, stream=None):
+        """mfv_sw_pack into the current record buffer (marks the second buffer stale)."""
+        _lib.call('mfv_sw_pack', dmesh.ptr, self.ptr, h.data_ptr(), hu.data_ptr(), hv.data_ptr(), Z.data_ptr(),
+                  zgrad.data_ptr(), int(order), stream if stream is not None else stream_ptr())
+        self._rec_synced = None
+
+    def tiles(self, T=128):
+        """Tiles of the fused float64 step for blocks of T threads (built on first use)."""
+        if T not in self._tiles:
+            self._tiles[T] = SwTiles(sw_tiles_native(self.nbr_host, self.desc.nb_cells, T, T, self._lsq),
+                                     self.cellrec.device, stage=True)
+        return self._tiles[T]
+
+    def lds_bytes(self, T=128):
+        h = self.tiles(T).host
+        return h['max_ext'] * 64 + h['max_items'] * h['MF'] * 32
+
+    def step_tiled(self, dmesh, grav, cfl, order, ring, step, T=128, stream=None):
+        """One fused float64 step: reads the current record buffer, writes the other, swaps
+        them. The second buffer takes a copy of the records (static fields included) after
+        pack(); the static fields change only there."""
+        if self.cellrec2 is None or self._rec_synced is not self.cellrec:
+            if self.cellrec2 is None:
+                import torch
+                self.cellrec2 = torch.empty_like(self.cellrec)
+            self.cellrec2.copy_(self.cellrec)
+        _lib.call('mfv_sw_step_packed_tiled', dmesh.ptr, self.ptr, self.tiles(T).ptr, self.cellrec2.data_ptr(),
+                  float(grav), float(cfl), int(order), ring.data_ptr(), int(step),
+                  stream if stream is not None else stream_ptr())
+        self.cellrec, self.cellrec2 = self.cellrec2, self.cellrec
+        self.desc.cellrec = self.cellrec.data_ptr()
+        self._rec_synced = self.cellrec
 
 
 def sw_tiles_native(nbr, C, T, N, lsq, ranges=None):

@@ -317,6 +317,8 @@ class SwEnsemble(object):
         st = torch.cuda.current_stream()
         res = {}
         for cfg in CONFIGS:
+            if cfg[0] != 'split' and self.tiles(cfg[1], cfg[0] == 'staged', cfg[2]).lds_bytes() > 64 * 1024:
+                continue  # this mesh's tiles do not fit the LDS budget at this size
             self.step(6, cfg)  # warm-up, aligned to the ring period and the state buffers
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(st)

@@ -525,3 +525,38 @@ def test_sw_mixed_tiled_step_equals_two_kernel_step(torch, oracle_lib, T, N, hil
     same(runs[1][0], runs[0][0], "tiled state")
     for a, b in zip(runs[0][1], runs[1][1]):
         same(b, a, "dt ring")
+
+
+@pytest.mark.parametrize("T,hilbert", [(128, True), (256, True), (128, False)])
+def test_sw_packed_tiled_step_equals_oracle(torch, oracle_lib, T, hilbert):
+    """The fused float64 step on tiles (records and face values in LDS) gives the reference
+    sequence's bits, like the two-kernel packed path."""
+    from manapy_amd.device import DeviceMesh, SwPacked
+    from manapy_amd.meshgen import gaussian_bumps
+    from manapy_amd.renumber import renumbered
+    d0 = make_tri_domain(43, 35, 100., 80., 0.1, 17)
+    d = renumbered(d0) if hilbert else d0
+    C = int(d.nbcells)
+    c = d.cells.center
+    Z = gaussian_bumps(c, 6, np.random.default_rng(4), [(0., 100.), (0., 80.)])
+    h0 = np.where(c[:, 0] < 40., 2.0, 1.0)
+    G = 9.81
+    h, hu, hv = h0.copy(), np.zeros(C), np.zeros(C)
+    zg = oracle_lib.z_gradient(d, Z)
+    dts = [oracle_lib.sw_step(d, h, hu, hv, Z, zg, G, 0.8, 2) for _ in range(25)]
+    m = DeviceMesh(d)
+    p = SwPacked(m)
+    th, tz, tZ, tg = (dev(torch, x) for x in (h0, np.zeros(C), Z, zg))
+    p.pack(m, th, tz, tz, tZ, tg, 2)
+    dt0 = dev(torch, np.zeros(1))
+    ring = dev(torch, np.zeros(4))
+    call('mfv_time_step_sw', m.ptr, P(th), P(tz), P(tz), 0.8, G, 0., 0., P(dt0), None)
+    call('mfv_dt_ring_init', P(ring), P(dt0))
+    for s in range(25):
+        p.step_tiled(m, G, 0.8, 2, ring, s, T=T)
+    oh, ohu, ohv = (dev(torch, np.zeros(C)) for _ in range(3))
+    call('mfv_sw_unpack', m.ptr, p.ptr, P(oh), P(ohu), P(ohv))
+    same(host(oh), h, "h")
+    same(host(ohu), hu, "hu")
+    same(host(ohv), hv, "hv")
+    assert host(ring)[24 % 3] == dts[-1]
