@@ -200,7 +200,8 @@ typedef struct mfv_sw_packed {
     int32_t max_cell_faces; /* MF (3 or 4), equal to the mesh's */
     double *cellrec;        /* [C][8]     cx, cy, eta, hu, hv, h, Z, vol */
     double *facerec;        /* [C][MF][4] eta_f, hu_f, hv_f, z_f at each face centre */
-    double *cfrec;          /* [C][MF][4] rx, ry (face centre - cell centre), z_f, 0 */
+    double *cfrec;          /* [C][MF][4] rx, ry (face centre - cell centre), z_f, z_f of the
+                                          neighbour's side (own z_f on a wall) */
     const double *cfgeo;    /* [C][MF][4] n0, n1, mesure, 0 of the face (faces.normal orientation) */
     const int32_t *nbr;     /* [MF][C]    (j << 3) | (k << 1) | side, -1 on walls: j = cell across
                                           the face, k = the face's slot in j's faceid row,
@@ -208,7 +209,8 @@ typedef struct mfv_sw_packed {
     const uint8_t *order;   /* [C]        slot of the r-th residual contribution in bits 2r..2r+1 */
 } mfv_sw_packed;
 
-/* Fills cellrec from (h, hu, hv, Z) and z_f = Z + (order-1)(Z_x rx + Z_y ry) into cfrec. */
+/* Fills cellrec from (h, hu, hv, Z), and z_f = Z + (order-1)(Z_x rx + Z_y ry) of both sides
+ * of every face slot into cfrec. */
 int mfv_sw_pack(const mfv_mesh *m, const mfv_sw_packed *p, const double *h, const double *hu,
                 const double *hv, const double *Z, const double *zgrad, int order, void *stream);
 
@@ -307,18 +309,19 @@ int mfv_sw_tiles_build(int C, int MF, const int32_t *nbr, int K, const int32_t *
                        mfv_sw_tiles_host *out);
 void mfv_sw_tiles_host_free(mfv_sw_tiles_host *t);
 
+/* The float64 packed step fused on tiles (the same mfv_sw_tiles, built from the packed
+ * neighbour codes): reads p->cellrec, writes cellrec_out (a second [C][8] buffer holding the
+ * same static fields); the fields a tile's reconstructions read and its face values stay in
+ * LDS (40 * (max_ext rounded up to even) + 24 * MF * max_items bytes <= 64 KiB). Gives the bits
+ * of mfv_sw_recon followed by mfv_sw_flux_packed. No counterpart call in the reference: it
+ * replaces explicitscheme_convective_SW + update_SW + time_step_SW in one launch. */
+int mfv_sw_step_packed_tiled(const mfv_mesh *m, const mfv_sw_packed *p, const mfv_sw_tiles *t,
+                             double *cellrec_out, double grav, double cfl, int order, double *dt_ring,
+                             int step, void *stream);
 /* One fused mixed step (reconstruction, fluxes, update, next dt) from p->state into
  * state_out (a second [C][4] buffer); the face records stay in LDS. Gives the same bits as
  * mfv_sw_recon_mixed followed by mfv_sw_flux_mixed. No counterpart call in the reference:
  * it replaces the pair above (explicitscheme_convective_SW + update_SW in one launch). */
-/* The float64 packed step fused on tiles (the same mfv_sw_tiles, built from the packed
- * neighbour codes): reads p->cellrec, writes cellrec_out (a second [C][8] buffer holding the
- * same static fields); the records a tile reads and its face values stay in LDS
- * (max_ext * 64 + max_items * MF * 32 bytes <= 64 KiB). Gives the bits of mfv_sw_recon followed
- * by mfv_sw_flux_packed. */
-int mfv_sw_step_packed_tiled(const mfv_mesh *m, const mfv_sw_packed *p, const mfv_sw_tiles *t,
-                             double *cellrec_out, double grav, double cfl, int order, double *dt_ring,
-                             int step, void *stream);
 int mfv_sw_step_mixed_tiled(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv_sw_tiles *t,
                             double *state_out, double grav, double cfl, int order, double *dt_ring,
                             int step, void *stream);

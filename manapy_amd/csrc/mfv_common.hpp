@@ -29,6 +29,12 @@ inline unsigned grid_for(int n, int block = kBlock) { return (unsigned)((n + blo
         }                                 \
     } while (0)
 
+// Per-face-slot values of one cell (MF <= 4) as native vectors: element s with a
+// lane-dependent s compiles to a select chain, where an array would be put in scratch memory.
+typedef float slot_f __attribute__((ext_vector_type(4)));
+typedef double slot_d __attribute__((ext_vector_type(4)));
+typedef int slot_i __attribute__((ext_vector_type(4)));
+
 // Python min(a, b) / max(a, b): the second argument wins only on a strict comparison.
 __device__ __forceinline__ double pmin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double pmax(double a, double b) { return (b > a) ? b : a; }
@@ -48,6 +54,23 @@ __device__ __forceinline__ void wave_min_part(double *__restrict__ part, size_t 
         m = (o < m) ? o : m;
     }
     if ((threadIdx.x & 63) == 0) part[gid >> 6] = m;
+}
+
+// Cells at positions e and e + T of a tile's ext order (its nc cells from c0, then
+// ext[e0 .. e0 + ne)), with both table loads issued before either is used. Past the end,
+// the second is the first again (writing it twice is harmless).
+__device__ __forceinline__ int2 tile_ext_pair(const int32_t *__restrict__ ext, int c0, int nc, int e0, int ne,
+                                              int e, int T)
+{
+    const int eb = (e + T < nc + ne) ? e + T : e;
+    int2 c = make_int2(c0 + e, c0 + eb);
+    if (ne > 0) {
+        const int xa = ext[e0 + min(max(e - nc, 0), ne - 1)];
+        const int xb = ext[e0 + min(max(eb - nc, 0), ne - 1)];
+        c.x = (e < nc) ? c.x : xa;
+        c.y = (eb < nc) ? c.y : xb;
+    }
+    return c;
 }
 
 // XCD-aware block order. The dispatcher hands consecutive workgroups to the 8 XCDs in

@@ -190,22 +190,19 @@ __global__ void __launch_bounds__(kBlock) k_sw_gradients(Mesh m, const double *_
         d = ex * r1 + ey * r2;
         if (fabs(d) < 1e-8) qe = 1.;
         else {
-            if (d > 0.) qe = pmin(1., (emax - ei) / d);
-            if (d < 0.) qe = pmin(1., (emin - ei) / d);
+            if (d > 0. || d < 0.) qe = pmin(1., ((d > 0.) ? emax - ei : emin - ei) / d);  // one division
         }
         pe = pmin(pe, qe);
         d = ux * r1 + uy * r2;
         if (fabs(d) < 1e-8) qu = 1.;
         else {
-            if (d > 0.) qu = pmin(1., (umax - ui) / d);
-            if (d < 0.) qu = pmin(1., (umin - ui) / d);
+            if (d > 0. || d < 0.) qu = pmin(1., ((d > 0.) ? umax - ui : umin - ui) / d);  // one division
         }
         pu = pmin(pu, qu);
         d = vx * r1 + vy * r2;
         if (fabs(d) < 1e-8) qv = 1.;
         else {
-            if (d > 0.) qv = pmin(1., (vmax - vi) / d);
-            if (d < 0.) qv = pmin(1., (vmin - vi) / d);
+            if (d > 0. || d < 0.) qv = pmin(1., ((d > 0.) ? vmax - vi : vmin - vi) / d);  // one division
         }
         pv = pmin(pv, qv);
     }
@@ -432,7 +429,9 @@ extern "C" int mfv_sw_step_fused(const mfv_mesh *m, const double *state_in, doub
 // loads instead of one scattered load per array:
 //   cellrec [C][8]      cx, cy, eta, hu, hv, h, Z, vol   (64 bytes, one record per cell)
 //   facerec [C][MF][4]  eta_f, hu_f, hv_f, z_f           (MUSCL values at the face centres)
-//   cfrec   [C][MF][4]  rx, ry, z_f, 0                   (face centre - cell centre; static)
+//   cfrec   [C][MF][4]  rx, ry, z_f, z_f across          (face centre - cell centre, the face's
+//                                                     bathymetry on this side and on the
+//                                                     neighbour's side (own on a wall); static)
 //   cfgeo   [C][MF][4]  n0, n1, mesure, 0                (faces.normal / mesure; static)
 //   nbr     [MF][C]     (j << 3) | (k << 1) | side, or -1 on a wall: j = cell across the face,
 //                       k = the slot of the face in j's faceid row, side = 1 when the cell is
@@ -464,6 +463,22 @@ __global__ void __launch_bounds__(kBlock) k_sw_pack(Mesh m, mfv_sw_packed p, con
     for (int s = 0; s < MF && s < nf; s++) {
         double *q = p.cfrec + 4 * (MF * (size_t)i + s);
         q[2] = Z[i] + om * (zgrad[i] * q[0] + zgrad[C + i] * q[1]);
+    }
+}
+
+// cfrec[i][s][3] = z_f of the neighbour's side of face slot s (its cfrec[j][k][2]), own z_f on
+// a wall; after k_sw_pack has written every z_f.
+template <int MF>
+__global__ void __launch_bounds__(kBlock) k_sw_pack_zf_across(Mesh m, mfv_sw_packed p)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    const size_t C = m.C;
+    const int nf = m.cell_nfaces[i];
+    for (int s = 0; s < MF && s < nf; s++) {
+        double *q = p.cfrec + 4 * (MF * (size_t)i + s);
+        const int code = p.nbr[(size_t)s * C + i];
+        q[3] = (code >= 0) ? p.cfrec[4 * (MF * (size_t)(code >> 3) + ((code >> 1) & 3)) + 2] : q[2];
     }
 }
 
@@ -515,7 +530,9 @@ struct PackedCells {
 };
 
 struct PackedStaged {
-    const double *rec;  // LDS copy: 8 doubles per cell of the tile's ext list
+    const double2 *xy;  // LDS copy of the tile's ext list: (cx, cy), (eta, hu), hv
+    const double2 *eu;
+    const double *hv;
     const uint16_t *__restrict__ lst;
     const uint16_t *__restrict__ lgh;
     const int16_t *__restrict__ lfn;
@@ -524,7 +541,12 @@ struct PackedStaged {
     __device__ __forceinline__ int self() const { return q; }
     __device__ __forceinline__ void ehv(int h, double &cx, double &cy, double &e, double &u, double &v) const
     {
-        load_ehv(rec, h, cx, cy, e, u, v);
+        const double2 a = xy[h], b = eu[h];
+        cx = a.x;
+        cy = a.y;
+        e = b.x;
+        u = b.y;
+        v = hv[h];
     }
     __device__ __forceinline__ int stencil(int j) const { return lst[(size_t)j * I + row]; }
     __device__ __forceinline__ int ghost_owner(int j, int) const { return lgh[(size_t)j * I + row]; }
@@ -538,11 +560,22 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
                                              double om, double2 *out)
 {
     const size_t C = m.C;
+    // Loads are issued in batches ahead of the arithmetic: the per-cell counts, the stencil
+    // handles of the K rows every cell has, and the face slots of the [MF]-wide tables, which
+    // are in bounds whatever the cell's face count (loads behind the count-dependent control
+    // flow would each cost a memory latency of their own).
+    const int nn = m.lsq_ncells[i];
+    const int ng = m.lsq_nghosts[i];
+    const int nf = m.cell_nfaces[i];
+    int codes[MF], nbh[MF];
+#pragma unroll
+    for (int s = 0; s < MF; s++) codes[s] = p.nbr[(size_t)s * C + i];
+#pragma unroll
+    for (int s = 0; s < MF; s++) nbh[s] = A.face_nb(s, codes[s]);
     double cx, cy, ei, ui, vi;
     A.ehv(A.self(), cx, cy, ei, ui, vi);
     double ixx = 0., iyy = 0., ixy = 0.;
     double je0 = 0., je1 = 0., ju0 = 0., ju1 = 0., jv0 = 0., jv1 = 0.;
-    const int nn = m.lsq_ncells[i];
     auto accum = [&](double px, double py, double ec, double uc, double vc) {
         double jx = px - cx, jy = py - cy;
         ixx += jx * jx;
@@ -559,7 +592,9 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
     if (KMAX > 0) {
         int cidx[KMAX > 0 ? KMAX : 1];
 #pragma unroll
-        for (int j = 0; j < KMAX; j++) cidx[j] = (j < nn) ? A.stencil(j) : A.self();
+        for (int j = 0; j < KMAX; j++) cidx[j] = (j < m.K) ? A.stencil(j) : 0;
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) cidx[j] = (j < nn) ? cidx[j] : A.self();
 #pragma unroll
         for (int j0 = 0; j0 < KMAX; j0 += 4) {
             double px[4], py[4], ec[4], uc[4], vc[4];
@@ -578,7 +613,13 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
             accum(px, py, ec, uc, vc);
         }
     }
-    const int ng = m.lsq_nghosts[i];
+    double2 cfa[MF], cfb[MF];  // rx, ry, z_f of each slot
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        const double2 *q = reinterpret_cast<const double2 *>(p.cfrec + 4 * (MF * (size_t)i + s));
+        cfa[s] = q[0];
+        cfb[s] = q[1];
+    }
     for (int j = 0; j < ng; j++) {
         int f = m.lsq_ghosts[(size_t)j * C + i];
         int o = A.ghost_owner(j, f);
@@ -597,21 +638,26 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
         jv0 += (jx * dv);
         jv1 += (jy * dv);
     }
+    // face neighbours' values (the own cell's on a wall or a missing slot: not used)
+    double ne[MF], nu[MF], nv[MF];
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        double px, py;
+        A.ehv(codes[s] >= 0 ? nbh[s] : A.self(), px, py, ne[s], nu[s], nv[s]);
+    }
     const double dia = ixx * iyy - ixy * ixy;
     const double ex = (iyy * je0 - ixy * je1) / dia, ey = (ixx * je1 - ixy * je0) / dia;
     const double ux = (iyy * ju0 - ixy * ju1) / dia, uy = (ixx * ju1 - ixy * ju0) / dia;
     const double vx = (iyy * jv0 - ixy * jv1) / dia, vy = (ixx * jv1 - ixy * jv0) / dia;
 
-    const int nf = m.cell_nfaces[i];
     double emax = ei, emin = ei, umax = ui, umin = ui, vmax = vi, vmin = vi;
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         if (s >= nf) break;
-        int code = p.nbr[(size_t)s * C + i];
+        const int code = codes[s];
         double ea, ua, va, eb, ub, vb;
         if (code >= 0) {
-            double px, py, ej, uj, vj;
-            A.ehv(A.face_nb(s, code), px, py, ej, uj, vj);
+            const double ej = ne[s], uj = nu[s], vj = nv[s];
             if (code & 1) {
                 ea = ej; ua = uj; va = vj;
                 eb = ei; ub = ui; vb = vi;
@@ -630,36 +676,27 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
         vmax = pmax(pmax(vmax, va), vb);
         vmin = pmin(pmin(vmin, va), vb);
     }
-    double rx[MF], ry[MF], zf[MF];
     double pe = 1., pu = 1., pv = 1., qe = 1., qu = 1., qv = 1.;
 #pragma unroll
     for (int s = 0; s < MF; s++) {
-        const double2 *q = reinterpret_cast<const double2 *>(p.cfrec + 4 * (MF * (size_t)i + s));
-        double2 a = q[0], b = q[1];
-        rx[s] = a.x;
-        ry[s] = a.y;
-        zf[s] = b.x;
-        if (s >= nf) continue;
-        double r1 = rx[s], r2 = ry[s], d;
+        if (s >= nf) break;
+        double r1 = cfa[s].x, r2 = cfa[s].y, d;
         d = ex * r1 + ey * r2;
         if (fabs(d) < 1e-8) qe = 1.;
         else {
-            if (d > 0.) qe = pmin(1., (emax - ei) / d);
-            if (d < 0.) qe = pmin(1., (emin - ei) / d);
+            if (d > 0. || d < 0.) qe = pmin(1., ((d > 0.) ? emax - ei : emin - ei) / d);  // one division
         }
         pe = pmin(pe, qe);
         d = ux * r1 + uy * r2;
         if (fabs(d) < 1e-8) qu = 1.;
         else {
-            if (d > 0.) qu = pmin(1., (umax - ui) / d);
-            if (d < 0.) qu = pmin(1., (umin - ui) / d);
+            if (d > 0. || d < 0.) qu = pmin(1., ((d > 0.) ? umax - ui : umin - ui) / d);  // one division
         }
         pu = pmin(pu, qu);
         d = vx * r1 + vy * r2;
         if (fabs(d) < 1e-8) qv = 1.;
         else {
-            if (d > 0.) qv = pmin(1., (vmax - vi) / d);
-            if (d < 0.) qv = pmin(1., (vmin - vi) / d);
+            if (d > 0. || d < 0.) qv = pmin(1., ((d > 0.) ? vmax - vi : vmin - vi) / d);  // one division
         }
         pv = pmin(pv, qv);
     }
@@ -667,10 +704,10 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
     for (int s = 0; s < MF; s++) {
         if (s >= nf) break;
         double2 a, b;
-        a.x = ei + om * pe * (ex * rx[s] + ey * ry[s]);
-        a.y = ui + om * pu * (ux * rx[s] + uy * ry[s]);
-        b.x = vi + om * pv * (vx * rx[s] + vy * ry[s]);
-        b.y = zf[s];
+        a.x = ei + om * pe * (ex * cfa[s].x + ey * cfa[s].y);
+        a.y = ui + om * pu * (ux * cfa[s].x + uy * cfa[s].y);
+        b.x = vi + om * pv * (vx * cfa[s].x + vy * cfa[s].y);
+        b.y = cfb[s].x;
         out[2 * s] = a;
         out[2 * s + 1] = b;
     }
@@ -693,15 +730,86 @@ __global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, do
         }
 }
 
+// Face values of the flux gather: own(s) gives this cell's (eta_f, hu_f) and hv_f at slot s,
+// across(code) those of the other side of the face (code = its neighbour code in the numbering
+// the values are stored in), zpair(s, code) the z_f of both sides (own z_f on a wall).
+// GlobalFaceVals: the facerec array (codes are global); StagedFaceVals: a block's LDS face
+// values without z_f (codes are block-local), z_f of both sides from the static cfrec.
+struct GlobalFaceVals {
+    const double *__restrict__ fr;
+    int i;
+    __device__ __forceinline__ void at(size_t k, double2 &eu, double &v) const
+    {
+        const double2 *q = reinterpret_cast<const double2 *>(fr + 4 * k);
+        eu = q[0];
+        v = q[1].x;
+    }
+    template <int MF>
+    __device__ __forceinline__ void own(int s, double2 &eu, double &v) const { at(MF * (size_t)i + s, eu, v); }
+    template <int MF>
+    __device__ __forceinline__ void across(int code, double2 &eu, double &v) const
+    {
+        at(MF * (size_t)(code >> 3) + ((code >> 1) & 3), eu, v);
+    }
+    template <int MF>
+    __device__ __forceinline__ double2 zpair(int s, int code) const
+    {
+        const double zo = fr[4 * (MF * (size_t)i + s) + 3];
+        return make_double2(zo, code >= 0 ? fr[4 * (MF * (size_t)(code >> 3) + ((code >> 1) & 3)) + 3] : zo);
+    }
+};
+
+struct StagedFaceVals {
+    const double2 *eu;  // [items][MF] (eta_f, hu_f)
+    const double *v;    // [items][MF] hv_f
+    const double *__restrict__ cfrec;
+    int i, q;
+    template <int MF>
+    __device__ __forceinline__ void own(int s, double2 &a, double &b) const
+    {
+        a = eu[MF * q + s];
+        b = v[MF * q + s];
+    }
+    template <int MF>
+    __device__ __forceinline__ void across(int code, double2 &a, double &b) const
+    {
+        const int k = MF * (code >> 3) + ((code >> 1) & 3);
+        a = eu[k];
+        b = v[k];
+    }
+    template <int MF>
+    __device__ __forceinline__ double2 zpair(int s, int) const
+    {
+        return reinterpret_cast<const double2 *>(cfrec + 4 * (MF * (size_t)i + s))[1];
+    }
+};
+
 // One cell of the step: fluxes + bathymetry terms + update_SW from the record rin, new eta, hu,
-// hv, h written into rout (rout may be rin); returns the next-dt candidate. rec(k) is face
-// record k (= MF * cell + slot) of the array the codes index; own: this cell's position there.
-template <int MF, class Recs, class Codes>
-__device__ __forceinline__ double sw_flux_packed_core(const mfv_sw_packed &p, int i, int own,
-                                                      const double *__restrict__ rin, double *rout, Recs rec,
-                                                      Codes code_of, int nf, unsigned ord, double g, double cfl,
-                                                      double dt)
+// hv, h written into rout (rout may be rin); returns the next-dt candidate. fv: where the face
+// values come from; code_of(s): the neighbour code of slot s in fv's numbering. The slot data
+// of the [MF]-wide tables (in bounds whatever nf) is loaded in one batch before the arithmetic.
+template <int MF, class FV, class Codes>
+__device__ __forceinline__ double sw_flux_packed_core(const mfv_sw_packed &p, int i, const double *__restrict__ rin,
+                                                      double *rout, const FV &fv, Codes code_of, int nf,
+                                                      unsigned ord, double g, double cfl, double dt)
 {
+    slot_i codes = -1;
+    slot_d gn0 = 0., gn1 = 0., gms = 1., zo = 0., zn = 0.;
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        codes[s] = code_of(s);
+        const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
+        const double2 ga = gq[0], gb = gq[1];
+        gn0[s] = ga.x;
+        gn1[s] = ga.y;
+        gms[s] = gb.x;
+    }
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        const double2 z = fv.template zpair<MF>(s, codes[s]);
+        zo[s] = z.x;
+        zn[s] = z.y;
+    }
     const double2 *r2 = reinterpret_cast<const double2 *>(rin);
     double2 c1 = r2[1], c2 = r2[2], c3 = r2[3];
     const double hu_i = c1.y, hv_i = c2.x, h_i = c2.y, Z_i = c3.x, vol = c3.y;
@@ -711,33 +819,34 @@ __device__ __forceinline__ double sw_flux_packed_core(const mfv_sw_packed &p, in
     for (int r = 0; r < MF; r++) {
         if (r >= nf) break;
         const int s = (ord >> (2 * r)) & 3;
-        const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
-        double2 ga = gq[0], gb = gq[1];
-        const double n0 = ga.x, n1 = ga.y, mes = gb.x;
-        const double2 *oq = rec(MF * (size_t)own + s);
-        double2 oa = oq[0], ob = oq[1];
-        const int code = code_of(s);
-        double eR, uR, vR, zR;
+        const double n0 = gn0[s], n1 = gn1[s], mes = gms[s];
+        const int code = codes[s];
+        double2 oa;
+        double ov;
+        fv.template own<MF>(s, oa, ov);
+        double eN, uN, vN;  // the other side (the mirrored own values on a wall)
         if (code >= 0) {
-            int j = code >> 3, k = (code >> 1) & 3;
-            const double2 *nq = rec(MF * (size_t)j + k);
-            double2 na = nq[0], nb = nq[1];
-            eR = na.x; uR = na.y; vR = nb.x; zR = nb.y;
+            double2 na;
+            fv.template across<MF>(code, na, vN);
+            eN = na.x; uN = na.y;
         } else {
-            eR = oa.x; uR = -1 * oa.y; vR = -1 * ob.x; zR = ob.y;
+            eN = oa.x; uN = -1 * oa.y; vN = -1 * ov;
         }
+        const double zO = zo[s], zN = zn[s];
+        // One evaluation in the face's orientation whichever side the cell is on (a wave holds
+        // cells of both sides, so two branches would each run for the whole wave); the
+        // operands, the flux and the own side's bathymetry term are those of the two-sided
+        // form: the own side's term is bL (own = L) or bR (own = R), and a - b = a + (-b).
+        const bool own_R = code >= 0 && (code & 1);
         double F0, F1, F2, bL, bR;
-        if (code < 0 || (code & 1) == 0) {
-            sw_face(oa.x, oa.y, ob.x, ob.y, h_i, Z_i, eR, uR, vR, zR, h_i, Z_i, n0, n1, mes, g, F0, F1, F2, bL, bR);
-            r0 -= F0; r1 -= F1; rr2 -= F2;
-            s1 -= bL * n0;
-            s2 -= bL * n1;
-        } else {
-            sw_face(eR, uR, vR, zR, h_i, Z_i, oa.x, oa.y, ob.x, ob.y, h_i, Z_i, n0, n1, mes, g, F0, F1, F2, bL, bR);
-            r0 += F0; r1 += F1; rr2 += F2;
-            s1 -= -(bR * n0);
-            s2 -= -(bR * n1);
-        }
+        sw_face(own_R ? eN : oa.x, own_R ? uN : oa.y, own_R ? vN : ov, own_R ? zN : zO, h_i, Z_i,
+                own_R ? oa.x : eN, own_R ? oa.y : uN, own_R ? ov : vN, own_R ? zO : zN, h_i, Z_i, n0, n1, mes, g,
+                F0, F1, F2, bL, bR);
+        r0 += own_R ? F0 : -F0;
+        r1 += own_R ? F1 : -F1;
+        rr2 += own_R ? F2 : -F2;
+        s1 -= own_R ? -(bR * n0) : bL * n0;
+        s2 -= own_R ? -(bR * n1) : bL * n1;
     }
     const double s0 = 0.;
     double hn = h_i + dt * (r0 + s0) / vol;
@@ -756,15 +865,12 @@ __device__ __forceinline__ double sw_flux_packed_core(const mfv_sw_packed &p, in
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         if (s >= nf) break;
-        const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
-        double2 ga = gq[0], gb = gq[1];
-        double n0 = ga.x, n1 = ga.y, mesu = gb.x;
+        double n0 = gn0[s], n1 = gn1[s], mesu = gms[s];
         double u_n = fabs(un / hn * n0 + vn / hn * n1);
         double lam_convect = u_n / mesu + velson;
         lam += lam_convect * mesu;
-        double mes = sqrt(n0 * n0 + n1 * n1);
-        double lam_diff = 0. * (mes * mes) + 0. * (mes * mes);
-        lam += lam_diff / vol;
+        // time_step_SW then adds (Dxx |n|^2 + Dyy |n|^2) / vol with Dxx = Dyy = 0: +0 for finite
+        // normals and volumes, which leaves lam (>= 0) unchanged
     }
     return cfl * vol / lam;
 }
@@ -780,21 +886,21 @@ __global__ void __launch_bounds__(kBlock) k_sw_flux_packed(Mesh m, mfv_sw_packed
     if (i < m.C) {
         const size_t C = m.C;
         double *rec = p.cellrec + 8 * (size_t)i;
-        const double *fr = p.facerec;
         const int32_t *__restrict__ nbr = p.nbr;
-        auto recs = [fr](size_t k) { return reinterpret_cast<const double2 *>(fr + 4 * k); };
         auto code_of = [nbr, C, i](int s) { return nbr[(size_t)s * C + i]; };
-        dloc = sw_flux_packed_core<MF>(p, i, i, rec, rec, recs, code_of, m.cell_nfaces[i], p.order[i], g, cfl,
-                                       ring.cur[0]);
+        dloc = sw_flux_packed_core<MF>(p, i, rec, rec, GlobalFaceVals{p.facerec, i}, code_of, m.cell_nfaces[i],
+                                       p.order[i], g, cfl, ring.cur[0]);
     }
     wave_min_part(dtpart, (size_t)i, dloc);
 }
 
 // Fused float64 step on tiles (the layout of mfv_sw_tiles, built from the same neighbour
-// codes): the block copies every cell record its tile reads into LDS, reconstructs its cells
-// and halo into LDS face records, then updates its cells from LDS into cellrec_out (the
-// step is double-buffered: neighbouring tiles still read the old records). The per-cell
-// functions are those of the two-kernel path, so the bits are the same.
+// codes): the block copies the fields the reconstruction reads (cx, cy, eta, hu, hv) of every
+// cell its tile reads into LDS, reconstructs its cells and halo into LDS face values, then
+// updates its cells into cellrec_out (the step is double-buffered: neighbouring tiles still
+// read the old records). The per-cell functions are those of the two-kernel path, so the bits
+// are the same. LDS: 40 bytes per ext cell + 24 per item face slot (z_f comes from cfrec),
+// so that four blocks of 256 threads fit a CU on Hilbert-ordered triangle meshes.
 struct TilesP {
     int E;
     const int32_t *__restrict__ cell0;
@@ -810,37 +916,55 @@ struct TilesP {
     int I;
 };
 
+// LDS bytes of the fused float64 step: ext cells (E rounded up to even, so the arrays after
+// the odd-sized hv array stay 16-byte aligned) and item face values.
+inline size_t packed_tiled_lds(int max_ext, int max_items, int MF)
+{
+    const size_t E = ((size_t)max_ext + 1) & ~(size_t)1;
+    return E * 40 + (size_t)max_items * MF * 24;
+}
+
 template <int MF, int T, int KMAX>
 __global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packed p, TilesP t,
                                                             double *__restrict__ rec_out, double om, double g,
                                                             double cfl, DtRing ring, double *__restrict__ dtpart)
 {
     extern __shared__ double2 lds2[];
-    double *s_rec = reinterpret_cast<double *>(lds2);          // [E][8]
-    double2 *lfr = lds2 + 4 * (size_t)t.E;                      // [items][MF][2]
+    const int E = (t.E + 1) & ~1;
+    double2 *s_xy = lds2;                                            // [E]
+    double2 *s_eu = lds2 + E;                                        // [E]
+    double *s_v = reinterpret_cast<double *>(lds2 + 2 * E);          // [E]
+    double2 *f_eu = lds2 + 2 * E + E / 2;                            // [items][MF]
     const int tile = xcd_block();
     const int c0 = t.cell0[tile], nc = t.ncells[tile], nh = t.nhalo[tile];
     const int e0 = t.eptr[tile], ne = t.eptr[tile + 1] - e0;
+    double *f_v = reinterpret_cast<double *>(f_eu + MF * (nc + nh));  // [items][MF]
     const size_t r0 = t.iptr[tile];
     ring_advance(ring);
-    for (int e = threadIdx.x; e < nc + ne; e += T) {
-        const int c = (e < nc) ? c0 + e : t.ext[e0 + (e - nc)];
-        const double2 *src = reinterpret_cast<const double2 *>(p.cellrec + 8 * (size_t)c);
-        double2 *dst = lds2 + 4 * (size_t)e;
-#pragma unroll
-        for (int k = 0; k < 4; k++) dst[k] = src[k];
+    for (int e = threadIdx.x; e < nc + ne; e += 2 * T) {  // two cells per pass: one latency for both
+        const int2 c = tile_ext_pair(t.ext, c0, nc, e0, ne, e, T);
+        const int eb = (e + T < nc + ne) ? e + T : e;
+        const double2 *sa = reinterpret_cast<const double2 *>(p.cellrec + 8 * (size_t)c.x);
+        const double2 *sb = reinterpret_cast<const double2 *>(p.cellrec + 8 * (size_t)c.y);
+        const double2 a0 = sa[0], a1 = sa[1], a2 = sa[2], b0 = sb[0], b1 = sb[1], b2 = sb[2];
+        s_xy[e] = a0;
+        s_eu[e] = a1;
+        s_v[e] = a2.x;
+        s_xy[eb] = b0;
+        s_eu[eb] = b1;
+        s_v[eb] = b2.x;
     }
     __syncthreads();
     for (int q = threadIdx.x; q < nc + nh; q += T) {
         const int c = (q < nc) ? c0 + q : t.ext[e0 + (q - nc)];
         double2 rec[2 * MF];
-        const int nf = sw_recon_core<MF, KMAX>(m, p, PackedStaged{s_rec, t.lst, t.lgh, t.lfn, r0 + q, (size_t)t.I, q},
-                                               c, om, rec);
+        const int nf = sw_recon_core<MF, KMAX>(
+            m, p, PackedStaged{s_xy, s_eu, s_v, t.lst, t.lgh, t.lfn, r0 + q, (size_t)t.I, q}, c, om, rec);
 #pragma unroll
         for (int s = 0; s < MF; s++)
             if (s < nf) {
-                lfr[2 * (MF * q + s)] = rec[2 * s];
-                lfr[2 * (MF * q + s) + 1] = rec[2 * s + 1];
+                f_eu[MF * q + s] = rec[2 * s];
+                f_v[MF * q + s] = rec[2 * s + 1].x;
             }
     }
     __syncthreads();
@@ -851,10 +975,10 @@ __global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packe
         const int i = c0 + q;
         const int32_t *__restrict__ lnbr = t.lnbr;
         const size_t C = m.C;
-        auto recs = [lfr](size_t k) { return lfr + 2 * k; };
         auto code_of = [lnbr, C, i](int s) { return lnbr[(size_t)s * C + i]; };
-        dloc = sw_flux_packed_core<MF>(p, i, q, s_rec + 8 * (size_t)q, rec_out + 8 * (size_t)i, recs, code_of,
-                                       m.cell_nfaces[i], p.order[i], g, cfl, ring.cur[0]);
+        dloc = sw_flux_packed_core<MF>(p, i, p.cellrec + 8 * (size_t)i, rec_out + 8 * (size_t)i,
+                                       StagedFaceVals{f_eu, f_v, p.cfrec, i, q}, code_of, m.cell_nfaces[i],
+                                       p.order[i], g, cfl, ring.cur[0]);
     }
     wave_min_part(dtpart, (size_t)c0 + q, dloc);
 }
@@ -881,10 +1005,13 @@ extern "C" int mfv_sw_pack(const mfv_mesh *m, const mfv_sw_packed *p, const doub
     MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
     MFV_REQUIRE(h && hu && hv && Z && zgrad, "NULL array");
     if (m->nb_cells == 0) return MFV_OK;
-    auto k = (m->max_cell_faces == 3) ? k_sw_pack<3> : k_sw_pack<4>;
-    hipLaunchKernelGGL(k, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p, h, hu,
-                       hv, Z, zgrad, (double)(order - 1));
-    return check_launch("k_sw_pack");
+    const bool tri = m->max_cell_faces == 3;
+    hipLaunchKernelGGL(tri ? k_sw_pack<3> : k_sw_pack<4>, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0,
+                       as_stream(stream), to_kernel(m), *p, h, hu, hv, Z, zgrad, (double)(order - 1));
+    if (int rc = check_launch("k_sw_pack")) return rc;
+    hipLaunchKernelGGL(tri ? k_sw_pack_zf_across<3> : k_sw_pack_zf_across<4>, dim3(grid_for(m->nb_cells)),
+                       dim3(kBlock), 0, as_stream(stream), to_kernel(m), *p);
+    return check_launch("k_sw_pack_zf_across");
 }
 
 extern "C" int mfv_sw_unpack(const mfv_mesh *m, const mfv_sw_packed *p, double *h, double *hu, double *hv,
@@ -926,7 +1053,7 @@ static int check_tiles_packed(const mfv_mesh *m, const mfv_sw_tiles *t)
     MFV_REQUIRE(t->tile_cell0 && t->tile_ncells && t->tile_nhalo && t->ext_ptr && t->ext_cells && t->item_ptr &&
                     t->lnbr && t->lst && t->lgh && t->lfn,
                 "tiles descriptor has NULL arrays");
-    const size_t lds = (size_t)t->max_ext * 64 + (size_t)t->max_items * m->max_cell_faces * 32;
+    const size_t lds = packed_tiled_lds(t->max_ext, t->max_items, m->max_cell_faces);
     MFV_REQUIRE(lds <= 64 * 1024, "tiles of %d cells need %zu bytes of LDS for the float64 step (> 64 KiB): use "
                 "smaller tiles", t->tile_size, lds);
     return MFV_OK;
@@ -938,7 +1065,7 @@ static void launch_packed_tiled(const mfv_mesh *m, const mfv_sw_packed *p, const
 {
     const TilesP tp{t->max_ext, t->tile_cell0, t->tile_ncells, t->tile_nhalo, t->ext_ptr, t->ext_cells,
                     t->item_ptr, t->lnbr, t->lst, t->lgh, t->lfn, t->nb_items};
-    const size_t lds = (size_t)t->max_ext * 64 + (size_t)t->max_items * MF * 32;
+    const size_t lds = packed_tiled_lds(t->max_ext, t->max_items, MF);
     if (m->lsq_width <= 16)
         hipLaunchKernelGGL((k_sw_step_packed_tiled<MF, T, 16>), dim3(t->nb_tiles), dim3(T), lds, st, to_kernel(m),
                            *p, tp, out, om, g, cfl, r, part);

@@ -84,13 +84,9 @@ __device__ __forceinline__ void face_dev(float hb, float aL, float aR, float dz,
     F0 = 0.5f * (hsL * qL + hsR * qR) - 0.5f * s * (eR - eL);
     F1 = 0.5f * (hsL * uL * qL + hsR * uR * qR) + 0.5f * (PL + PR) * n0 - 0.5f * s * (hsR * uR - hsL * uL);
     F2 = 0.5f * (hsL * vL * qL + hsR * vR * qR) + 0.5f * (PL + PR) * n1 - 0.5f * s * (hsR * vR - hsL * vL);
-    if (own_is_L) {
-        const float dh = hL - hsL;
-        bo = 0.5f * g * dh * (hL + hsL) + g * (hL + ho) * 0.5f * dzo;
-    } else {
-        const float dh = hR - hsR;
-        bo = 0.5f * g * dh * (hR + hsR) + g * (hR + ho) * 0.5f * dzo;
-    }
+    const float hO = own_is_L ? hL : hR, hsO = own_is_L ? hsL : hsR;
+    const float dh = hO - hsO;
+    bo = 0.5f * g * dh * (hO + hsO) + g * (hO + ho) * 0.5f * dzo;
 }
 
 // Static face geometry and the constant bathymetry differences (Z gradient from zgrad [3][C]):
@@ -149,6 +145,7 @@ __device__ __forceinline__ void accum_grad(float jx, float jy, float de, float d
 // handle (a global cell index, or a position in a block's LDS copy), the stencil, ghost
 // owners and face neighbours as handles.
 struct GlobalCells {
+    static constexpr int kChunk = 4;  // neighbour states loaded per group (global memory latency)
     const Mesh &m;
     const double4 *__restrict__ st;
     int i;
@@ -164,19 +161,29 @@ struct GlobalCells {
 };
 
 // KMAX > 0: the stencil handles are loaded first and the neighbour states in independent
-// groups of four (a few memory latencies per cell instead of two per stencil entry); the
-// sums keep the stencil order. KMAX = 0: plain loop, for stencils wider than KMAX.
+// groups of Cells::kChunk (a few memory latencies per cell instead of two per stencil entry);
+// the sums keep the stencil order. KMAX = 0: plain loop, for stencils wider than KMAX.
 template <int MF, int KMAX, class Cells>
 __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p, const Cells &a, int i, float om,
                                             float4 *out)
 {
     const size_t C = m.C;
+    // Loads in batches ahead of the arithmetic: the counts, the stencil handles of the K rows
+    // every cell has, and every slot of the [MF]-wide tables (in bounds whatever the face
+    // count; loads behind count-dependent control flow would each wait a memory latency).
+    const int nn = m.lsq_ncells[i];
+    const int ng = m.lsq_nghosts[i];
+    const int nf = m.cell_nfaces[i];
+    int codes[MF], nbh[MF];
+#pragma unroll
+    for (int s = 0; s < MF; s++) codes[s] = p.nbr[(size_t)s * C + i];
+#pragma unroll
+    for (int s = 0; s < MF; s++) nbh[s] = a.face_nb(s, codes[s]);
     const double4 si = a.state(a.self());
     const double hi = si.x, ei = si.x + si.w, ui = si.y, vi = si.z;
     const double2 ci = a.center(a.self());
     const double cx = ci.x, cy = ci.y;
     float ixx = 0.f, iyy = 0.f, ixy = 0.f, je0 = 0.f, je1 = 0.f, ju0 = 0.f, ju1 = 0.f, jv0 = 0.f, jv1 = 0.f;
-    const int nn = m.lsq_ncells[i];
     auto accum = [&](const double4 sc, const double2 cc) {
         const float jx = (float)(cc.x - cx);
         const float jy = (float)(cc.y - cy);
@@ -186,19 +193,22 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
     if (KMAX > 0) {
         int h[KMAX > 0 ? KMAX : 1];
 #pragma unroll
-        for (int j = 0; j < KMAX; j++) h[j] = (j < nn) ? a.stencil(j) : a.self();
+        for (int j = 0; j < KMAX; j++) h[j] = (j < m.K) ? a.stencil(j) : 0;
 #pragma unroll
-        for (int j0 = 0; j0 < KMAX; j0 += 4) {
-            double4 sc[4];
-            double2 cc[4];
+        for (int j = 0; j < KMAX; j++) h[j] = (j < nn) ? h[j] : a.self();
+        constexpr int CH = Cells::kChunk;
 #pragma unroll
-            for (int k = 0; k < 4; k++)
+        for (int j0 = 0; j0 < KMAX; j0 += CH) {
+            double4 sc[CH];
+            double2 cc[CH];
+#pragma unroll
+            for (int k = 0; k < CH; k++)
                 if (j0 + k < nn) {
                     sc[k] = a.state(h[j0 + k]);
                     cc[k] = a.center(h[j0 + k]);
                 }
 #pragma unroll
-            for (int k = 0; k < 4; k++)
+            for (int k = 0; k < CH; k++)
                 if (j0 + k < nn) accum(sc[k], cc[k]);
         }
     } else {
@@ -207,7 +217,13 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
             accum(a.state(c), a.center(c));
         }
     }
-    const int ng = m.lsq_nghosts[i];
+    float2 gb[MF];  // face centre - cell centre of each slot
+    float dzo[MF];  // z_f - Z of each slot
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        gb[s] = reinterpret_cast<const float2 *>(p.geoB)[MF * (size_t)i + s];
+        dzo[s] = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s].w;
+    }
     for (int j = 0; j < ng; j++) {
         const int f = m.lsq_ghosts[(size_t)j * C + i];
         const double4 so = a.state(a.ghost_owner(j, f));
@@ -216,24 +232,24 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
         accum_grad((float)(gc[0] - cx), (float)(gc[1] - cy), (float)((so.x + so.w) - ei), (float)(-so.y - ui),
                    (float)(-so.z - vi), ixx, iyy, ixy, je0, je1, ju0, ju1, jv0, jv1);
     }
+    // face neighbours' states (the own state on a wall or a missing slot: not used)
+    double4 sn[MF];
+#pragma unroll
+    for (int s = 0; s < MF; s++) sn[s] = a.state(codes[s] >= 0 ? nbh[s] : a.self());
     const float idia = 1.f / (ixx * iyy - ixy * ixy);
     const float ex = (iyy * je0 - ixy * je1) * idia, ey = (ixx * je1 - ixy * je0) * idia;
     const float ux = (iyy * ju0 - ixy * ju1) * idia, uy = (ixx * ju1 - ixy * ju0) * idia;
     const float vx = (iyy * jv0 - ixy * jv1) * idia, vy = (ixx * jv1 - ixy * jv0) * idia;
-    const int nf = m.cell_nfaces[i];
     float emax = 0.f, emin = 0.f, umax = 0.f, umin = 0.f, vmax = 0.f, vmin = 0.f;  // bounds as deviations
     float dh[MF];
-    int codes[MF];
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         dh[s] = 0.f;
-        codes[s] = -1;
         if (s >= nf) continue;
-        const int code = p.nbr[(size_t)s * C + i];
-        codes[s] = code;
+        const int code = codes[s];
         float eb, ub, vb;
         if (code >= 0) {
-            const double4 sb = a.state(a.face_nb(s, code));
+            const double4 sb = sn[s];
             eb = (float)((sb.x + sb.w) - ei);
             ub = (float)(sb.y - ui);
             vb = (float)(sb.z - vi);
@@ -250,32 +266,29 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
         vmax = fmaxf(vmax, vb);
         vmin = fminf(vmin, vb);
     }
-    float rx[MF], ry[MF];
     float pe = 1.f, pu = 1.f, pv = 1.f;
 #pragma unroll
     for (int s = 0; s < MF; s++) {
-        const float2 q = reinterpret_cast<const float2 *>(p.geoB)[MF * (size_t)i + s];
-        rx[s] = q.x;
-        ry[s] = q.y;
         if (s >= nf) continue;
-        float d = ex * rx[s] + ey * ry[s];
-        if (!(fabsf(d) < 1e-8f)) pe = fminf(pe, d > 0.f ? emax / d : emin / d);
-        d = ux * rx[s] + uy * ry[s];
-        if (!(fabsf(d) < 1e-8f)) pu = fminf(pu, d > 0.f ? umax / d : umin / d);
-        d = vx * rx[s] + vy * ry[s];
-        if (!(fabsf(d) < 1e-8f)) pv = fminf(pv, d > 0.f ? vmax / d : vmin / d);
+        const float rx = gb[s].x, ry = gb[s].y;
+        float d = ex * rx + ey * ry;
+        if (!(fabsf(d) < 1e-8f)) pe = fminf(pe, (d > 0.f ? emax : emin) / d);
+        d = ux * rx + uy * ry;
+        if (!(fabsf(d) < 1e-8f)) pu = fminf(pu, (d > 0.f ? umax : umin) / d);
+        d = vx * rx + vy * ry;
+        if (!(fabsf(d) < 1e-8f)) pv = fminf(pv, (d > 0.f ? vmax : vmin) / d);
     }
     const float hu32 = (float)ui, hv32 = (float)vi, h32 = (float)hi;
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         if (s >= nf) break;
-        const float dzo = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s].w;
-        const float de = om * pe * (ex * rx[s] + ey * ry[s]);
+        const float rx = gb[s].x, ry = gb[s].y;
+        const float de = om * pe * (ex * rx + ey * ry);
         const bool right = codes[s] >= 0 && (codes[s] & 1);
         float4 o;
-        o.x = right ? dh[s] + (de - dzo) : de - dzo;
-        o.y = hu32 + om * pu * (ux * rx[s] + uy * ry[s]);
-        o.z = hv32 + om * pv * (vx * rx[s] + vy * ry[s]);
+        o.x = right ? dh[s] + (de - dzo[s]) : de - dzo[s];
+        o.y = hu32 + om * pu * (ux * rx + uy * ry);
+        o.z = hv32 + om * pv * (vx * rx + vy * ry);
         o.w = right ? dh[s] : h32;
         out[s] = o;
     }
@@ -307,9 +320,9 @@ __global__ void __launch_bounds__(BS) k_sw_recon32(Mesh m, mfv_sw_mixed p, float
 }
 
 // time_step_SW of a float64 state with the float32 face normals (mesure = |n|), faces in
-// faceid order.
+// faceid order; (gx[s], gy[s]) = the normal of slot s (geoA).
 template <int MF>
-__device__ __forceinline__ double sw_dt32(const mfv_sw_mixed &p, int i, int nf, double hn, double un, double vn,
+__device__ __forceinline__ double sw_dt32(const slot_f gx, const slot_f gy, int nf, double hn, double un, double vn,
                                           double vol, float g, double cfl)
 {
     const double velson = sqrt((double)g * hn);
@@ -318,8 +331,7 @@ __device__ __forceinline__ double sw_dt32(const mfv_sw_mixed &p, int i, int nf, 
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         if (s >= nf) break;
-        const float4 ga = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
-        const double n0 = ga.x, n1 = ga.y, mesu = (double)sqrtf(ga.x * ga.x + ga.y * ga.y);
+        const double n0 = gx[s], n1 = gy[s], mesu = (double)sqrtf(gx[s] * gx[s] + gy[s] * gy[s]);
         // (|u.n| / |n| + c) |n| of time_step_SW, with one division per cell
         lam += fabs((un * n0 + vn * n1) * ih) + velson * mesu;
     }
@@ -335,49 +347,53 @@ struct GlobalRecs {
 
 // One cell of the step: compensated float32 residual, float64 update into *out, next-dt
 // candidate. own: this cell's index in the record array; code_of(s): neighbour code of slot s
-// in the record array's numbering.
+// in the record array's numbering. The slot data of the [MF]-wide tables is loaded in one
+// batch before the arithmetic (in bounds whatever nf).
 template <int MF, class Recs, class Codes>
 __device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int own, const double4 si, int nf,
                                               unsigned ord, Recs rec, Codes code_of, float g, double cfl,
                                               double dt, double4 *out)
 {
+    slot_f gx = 0.f, gy = 0.f, gz = 0.f, gw = 0.f;
+    slot_i codes = -1;
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        const float4 q = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
+        gx[s] = q.x;
+        gy[s] = q.y;
+        gz[s] = q.z;
+        gw[s] = q.w;
+        codes[s] = code_of(s);
+    }
+    const double vol = p.vol[i];
     float r0 = 0.f, c0 = 0.f, r1 = 0.f, c1 = 0.f, r2 = 0.f, c2 = 0.f, s1 = 0.f, d1 = 0.f, s2 = 0.f, d2 = 0.f;
 #pragma unroll
     for (int r = 0; r < MF; r++) {
         if (r >= nf) break;
         const int s = (ord >> (2 * r)) & 3;
-        const float4 ga = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
+        const float4 ga = make_float4(gx[s], gy[s], gz[s], gw[s]);
         const float mes = sqrtf(ga.x * ga.x + ga.y * ga.y);
         const float4 ownr = rec(MF * (size_t)own + s);
-        const int code = code_of(s);
+        const int code = codes[s];
+        // One evaluation in the face's orientation whichever side the cell is on (a wave holds
+        // cells of both sides, so per-side branches would each run for the whole wave). On a
+        // wall the other side is the own record mirrored (momenta negated) and dz = 0 (geoA
+        // holds 0 there).
+        const bool wall = code < 0, own_R = !wall && (code & 1);
+        const float4 oth = wall ? ownr : rec(MF * (size_t)(code >> 3) + ((code >> 1) & 3));
+        const float4 L = own_R ? oth : ownr;
+        const float huR = own_R ? ownr.y : (wall ? -ownr.y : oth.y);
+        const float hvR = own_R ? ownr.z : (wall ? -ownr.z : oth.z);
         float F0, F1, F2, bo;
-        if (code < 0) {
-            face_dev(ownr.w, ownr.x, ownr.x, 0.f, ownr.y, ownr.z, -ownr.y, -ownr.z, ga.w, ga.x, ga.y, mes, g, 0.f,
-                     true, F0, F1, F2, bo);
-        } else {
-            const float4 oth = rec(MF * (size_t)(code >> 3) + ((code >> 1) & 3));
-            if ((code & 1) == 0)
-                face_dev(ownr.w, ownr.x, oth.x, ga.z, ownr.y, ownr.z, oth.y, oth.z, ga.w, ga.x, ga.y, mes, g, 0.f,
-                         true, F0, F1, F2, bo);
-            else
-                face_dev(oth.w, oth.x, ownr.x, ga.z, oth.y, oth.z, ownr.y, ownr.z, ga.w, ga.x, ga.y, mes, g, -ownr.w,
-                         false, F0, F1, F2, bo);
-        }
-        if (code < 0 || (code & 1) == 0) {
-            twosum_add(r0, c0, -F0);
-            twosum_add(r1, c1, -F1);
-            twosum_add(r2, c2, -F2);
-            twosum_add(s1, d1, -(bo * ga.x));
-            twosum_add(s2, d2, -(bo * ga.y));
-        } else {
-            twosum_add(r0, c0, F0);
-            twosum_add(r1, c1, F1);
-            twosum_add(r2, c2, F2);
-            twosum_add(s1, d1, bo * ga.x);
-            twosum_add(s2, d2, bo * ga.y);
-        }
+        face_dev(L.w, L.x, own_R ? ownr.x : oth.x, ga.z, L.y, L.z, huR, hvR, ga.w, ga.x, ga.y, mes, g,
+                 own_R ? -ownr.w : 0.f, !own_R, F0, F1, F2, bo);
+        twosum_add(r0, c0, own_R ? F0 : -F0);
+        twosum_add(r1, c1, own_R ? F1 : -F1);
+        twosum_add(r2, c2, own_R ? F2 : -F2);
+        const float b1 = bo * ga.x, b2 = bo * ga.y;
+        twosum_add(s1, d1, own_R ? b1 : -b1);
+        twosum_add(s2, d2, own_R ? b2 : -b2);
     }
-    const double vol = p.vol[i];
     const double R0 = (double)r0 + (double)c0;
     const double R1 = ((double)r1 + (double)c1) + ((double)s1 + (double)d1);
     const double R2 = ((double)r2 + (double)c2) + ((double)s2 + (double)d2);
@@ -386,7 +402,7 @@ __device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int 
     const double un = si.y + R1 * dv;
     const double vn = si.z + R2 * dv;
     *out = make_double4(hn, un, vn, si.w);
-    return sw_dt32<MF>(p, i, nf, hn, un, vn, vol, g, cfl);
+    return sw_dt32<MF>(gx, gy, nf, hn, un, vn, vol, g, cfl);
 }
 
 // The cell step on the global records, in place on p.state.
@@ -556,7 +572,14 @@ __global__ void __launch_bounds__(kBlock) k_sw_dt_batch(Mesh m, mfv_sw_mixed p, 
     const int inst = b.wave_inst[i >> 6];
     if (inst < 0) return;
     const double4 s = reinterpret_cast<const double4 *>(p.state)[i];
-    const double dloc = sw_dt32<MF>(p, i, m.cell_nfaces[i], s.x, s.y, s.z, p.vol[i], g, cfl);
+    slot_f gx = 0.f, gy = 0.f;
+#pragma unroll
+    for (int k = 0; k < MF; k++) {
+        const float4 q = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + k];
+        gx[k] = q.x;
+        gy[k] = q.y;
+    }
+    const double dloc = sw_dt32<MF>(gx, gy, m.cell_nfaces[i], s.x, s.y, s.z, p.vol[i], g, cfl);
     wave_min_part(part, (size_t)i, dloc);
 }
 
@@ -689,6 +712,9 @@ struct TilesK {
 // Reconstruction inputs from a block's LDS copy: handles are positions in the tile's ext
 // list (its cells, its halo, then every other cell its reconstructions read).
 struct StagedCells {
+    // LDS latency is short: groups of two keep the kernel at 87 VGPRs, so five blocks of 256
+    // threads fit a CU where groups of four (109 VGPRs) allowed four
+    static constexpr int kChunk = 2;
     const double4 *st;
     const double2 *cc;
     const uint16_t *__restrict__ lst;
@@ -723,10 +749,16 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
     const double4 *__restrict__ st = reinterpret_cast<const double4 *>(p.state);
     if (!BATCH) ring_advance(ring);
     if (STAGED) {
-        for (int e = threadIdx.x; e < nc + ne; e += T) {
-            const int c = (e < nc) ? c0 + e : t.ext[e0 + (e - nc)];
-            s_st[e] = st[c];
-            s_cc[e] = make_double2(m.cell_center[3 * (size_t)c], m.cell_center[3 * (size_t)c + 1]);
+        for (int e = threadIdx.x; e < nc + ne; e += 2 * T) {  // two cells per pass: one latency for both
+            const int2 c = tile_ext_pair(t.ext, c0, nc, e0, ne, e, T);
+            const int eb = (e + T < nc + ne) ? e + T : e;
+            const double4 sa = st[c.x], sb = st[c.y];
+            const double *ca = m.cell_center + 3 * (size_t)c.x, *cb = m.cell_center + 3 * (size_t)c.y;
+            const double2 pa = make_double2(ca[0], ca[1]), pb = make_double2(cb[0], cb[1]);
+            s_st[e] = sa;
+            s_cc[e] = pa;
+            s_st[eb] = sb;
+            s_cc[eb] = pb;
         }
         __syncthreads();
     }

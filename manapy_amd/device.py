@@ -264,7 +264,7 @@ class SwPacked(object):
 
     def lds_bytes(self, T=128):
         h = self.tiles(T).host
-        return h['max_ext'] * 64 + h['max_items'] * h['MF'] * 32
+        return ((h['max_ext'] + 1) // 2 * 2) * 40 + h['max_items'] * h['MF'] * 24
 
     def step_tiled(self, dmesh, grav, cfl, order, ring, step, T=128, stream=None):
         """One fused float64 step: reads the current record buffer, writes the other, swaps
