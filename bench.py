@@ -155,6 +155,157 @@ def tiled_bytes_per_cell(MF, K_avg, G_avg):
             + 16 * MF + 8 * MF)         # geoA, geoB
 
 
+def time_steps(step, warmup, steps):
+    """Warm-up steps, then ``steps`` steps timed on the host clock between synchronisations and
+    again with HIP events around each step on the current stream. Returns (s per step of the
+    loop, mean event time of one step in s)."""
+    import torch
+    for k in range(warmup):
+        step(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(warmup + k)
+    torch.cuda.synchronize()
+    t_loop = (time.perf_counter() - t0) / steps
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for k, (a, b) in enumerate(evs):
+        a.record()
+        step(warmup + steps + k)
+        b.record()
+    torch.cuda.synchronize()
+    return t_loop, float(np.mean([a.elapsed_time(b) for a, b in evs])) * 1e-3
+
+
+def run_advection(n, steps, warmup, cpu=True):
+    """Advection of a Gaussian blob on n x n x 2 jittered triangles of the unit square (u = 1,
+    v = 0.5 on every face, all-Neumann boundaries, order 1, CFL 0.8): the fused device step
+    (mfv_advection_step) against the oracle's step, bit for bit, after every timed step."""
+    import torch
+    from manapy_amd import _lib
+    from manapy_amd.ddm import Domain
+    from manapy_amd.device import DeviceMesh, stream_ptr
+    from manapy_amd.meshgen import jittered_triangles
+    el, nd = jittered_triangles(n, n, 1.0, 1.0, 0.1, 0)
+    d = Domain(2, mesh=(el, nd))
+    C, F = int(d.nbcells), int(d.nbfaces)
+    c = d.cells.center
+    w0 = np.exp(-((c[:, 0] - .3) ** 2 + (c[:, 1] - .3) ** 2) / .005)
+    m = DeviceMesh(d, lsq=False)
+    s = stream_ptr()
+    dv = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device='cuda')
+    uf, vf, wf = dv(np.full(F, 1.0)), dv(np.full(F, 0.5)), dv(np.zeros(F))
+    uc, vc, wc = dv(np.zeros(C)), dv(np.zeros(C)), dv(np.zeros(C))
+    for a, b in ((uf, uc), (vf, vc), (wf, wc)):
+        _lib.call('mfv_facetocell', m.ptr, a.data_ptr(), b.data_ptr(), s)
+    W = [dv(w0), dv(np.zeros(C))]
+    ring, dt0 = dv(np.zeros(4)), dv(np.zeros(1))
+    _lib.call('mfv_time_step', m.ptr, uc.data_ptr(), vc.data_ptr(), wc.data_ptr(), CFL, dt0.data_ptr(), None, s)
+    _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
+
+    def step(k):
+        _lib.call('mfv_advection_step', m.ptr, W[k % 2].data_ptr(), W[(k + 1) % 2].data_ptr(), uc.data_ptr(),
+                  vc.data_ptr(), wc.data_ptr(), uf.data_ptr(), vf.data_ptr(), wf.data_ptr(), CFL, 1,
+                  ring.data_ptr(), k, None, s)
+    t_loop, t_ev = time_steps(step, warmup, steps)
+    nsteps = warmup + 2 * steps
+    w_gpu = W[nsteps % 2].cpu().numpy()
+    # algorithmic bytes of k_adv_step (order 1): w read + write, volume, u_c v_c w_c (dt),
+    # faceid + order entries; per face (x F/C): cellid, normal (3 doubles), u_f v_f w_f
+    MF = int(d.cells.faceid.shape[1] - 1)
+    bpc = (8 + 8 + 8 + 24 + 4 + (4 + 1) * MF) + (8 + 24 + 24) * F / C
+    out = {"workload": "advection of a Gaussian blob, %dx%dx2 jittered triangles, order 1, CFL 0.8" % (n, n),
+           "cells": C, "faces": F, "steps": steps, "dtype": "f64",
+           "value": round(C / t_loop / 1e6, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop * 1e3, 5),
+           "roofline": {"bound": "latency (the step's arrays, %.1f MB, stay in the 256 MB Infinity Cache)"
+                                 % (bpc * C / 1e6), "kernel": "k_adv_step + k_dt_finish",
+                        "achieved": round(bpc * C / t_ev / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(bpc * C / t_ev / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                        "bytes_per_cell": round(bpc, 1), "avg_step_us": round(t_ev * 1e6, 2)}}
+    if cpu:
+        from oracle import oracle as O
+        u_, v_, w_ = np.full(F, 1.0), np.full(F, 0.5), np.zeros(F)
+        ucc, vcc, wcc = np.zeros(C), np.zeros(C), np.zeros(C)
+        for a, b in ((u_, ucc), (v_, vcc), (w_, wcc)):
+            O.facetocell(d, a, b)
+        w = w0.copy()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            O.advection_step(d, w, ucc, vcc, wcc, u_, v_, w_, CFL, 1)
+        t_cpu = time.perf_counter() - t0
+        out["parity"] = {"steps": nsteps, "bitwise_equal_to_oracle": bool(np.array_equal(w, w_gpu)),
+                         "max_abs_diff": float(np.abs(w - w_gpu).max())}
+        out["cpu_baseline"] = {"value": round(C * nsteps / t_cpu / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1,
+                               "kind": "port", "sample": "%d steps of the same case (%.1f s), single-threaded C "
+                                                         "restatement of the reference loops" % (nsteps, t_cpu)}
+    return out
+
+
+def run_euler(n, steps, warmup, cpu=True, cpu_steps=2):
+    """3D Sod shock tube on n^3 cubes x 6 Kuhn tetrahedra of the unit cube (interior nodes
+    jittered by +/-8 % of h, slip walls, first-order Rusanov, CFL 0.8): the fused device step
+    (mfv_euler3d_step_fused). Parity: the first ``cpu_steps`` steps against the oracle, bit for
+    bit, at full size (the same run times the CPU baseline)."""
+    import torch
+    from manapy_amd import _lib
+    from manapy_amd.ddm import Domain
+    from manapy_amd.device import DeviceMesh, stream_ptr
+    from manapy_amd.meshgen import kuhn_tetrahedra
+    from manapy_amd.solvers.euler import initialisation_euler_3d
+    t0 = time.perf_counter()
+    el, nd = kuhn_tetrahedra(n, 1.0, 0.08, 0)
+    d = Domain(3, mesh=(el, nd))
+    t_mesh = time.perf_counter() - t0
+    C, F = int(d.nbcells), int(d.nbfaces)
+    U0 = initialisation_euler_3d(d.cells.center)
+    m = DeviceMesh(d, lsq=False)
+    s = stream_ptr()
+    dv = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device='cuda')
+    S = [dv(U0), dv(np.zeros((6, C)))]
+    ring, dt0 = dv(np.zeros(4)), dv(np.zeros(1))
+
+    def init():
+        S[0].copy_(dv(U0))
+        _lib.call('mfv_euler3d_time_step', m.ptr, S[0].data_ptr(), CFL, 1.4, dt0.data_ptr(), None, s)
+        _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
+
+    def step(k):
+        _lib.call('mfv_euler3d_step_fused', m.ptr, S[k % 2].data_ptr(), S[(k + 1) % 2].data_ptr(), 1.4, CFL, 0,
+                  ring.data_ptr(), k, s)
+    init()
+    t_loop, t_ev = time_steps(step, warmup, steps)
+    # algorithmic bytes of k_euler_step: U read + write (6 doubles each), volume, face count;
+    # per face slot: order entry + faceid; per face (x F/C): cellid, normal, mesure, name
+    MF = int(d.cells.faceid.shape[1] - 1)
+    bpc = (48 + 48 + 8 + 4 + (1 + 4) * MF) + (8 + 24 + 8 + 4) * F / C
+    out = {"workload": "3D Sod shock tube, %d^3 x 6 Kuhn tetrahedra, first-order Rusanov, slip walls, CFL 0.8" % n,
+           "cells": C, "faces": F, "steps": steps, "dtype": "f64", "mesh_build_s": round(t_mesh, 1),
+           "value": round(C / t_loop / 1e6, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop * 1e3, 5),
+           "roofline": {"bound": "hbm", "kernel": "k_euler_step + k_dt_finish",
+                        "achieved": round(bpc * C / t_ev / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(bpc * C / t_ev / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                        "bytes_per_cell": round(bpc, 1), "avg_step_us": round(t_ev * 1e6, 2)}}
+    if cpu and cpu_steps:
+        from oracle import oracle as O
+        init()
+        for k in range(cpu_steps):
+            step(k)
+        U_gpu = S[cpu_steps % 2].cpu().numpy()
+        U = U0.copy()
+        G_ = np.zeros((6, F))
+        t0 = time.perf_counter()
+        for _ in range(cpu_steps):
+            O.euler_step(d, U, G_, CFL, 1.4, 0)
+        t_cpu = time.perf_counter() - t0
+        out["parity"] = {"steps": cpu_steps, "bitwise_equal_to_oracle": bool(np.array_equal(U, U_gpu)),
+                         "max_abs_diff": float(np.abs(U - U_gpu).max())}
+        out["cpu_baseline"] = {"value": round(C * cpu_steps / t_cpu / 1e6, 3), "unit": "Mcell-updates/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "%d steps of the same case (%.1f s), single-threaded C restatement of the "
+                                         "reference loops" % (cpu_steps, t_cpu)}
+    return out
+
+
 def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel='auto', tile=256):
     """The same case on the mixed-precision path: throughput, per-kernel times and the
     per-field error against the float64 run after the same number of steps. kernel='tiled' /
@@ -419,6 +570,11 @@ def main():
     ap.add_argument('--ensemble', type=int, default=256, help='instances of the ensemble run (0: skip)')
     ap.add_argument('--ensemble-steps', type=int, default=204)
     ap.add_argument('--serving', type=int, default=32, help='requests of the serving run (0: skip)')
+    ap.add_argument('--advection', type=int, default=200,
+                    help='n of the n x n x 2 triangle advection run (0: skip)')
+    ap.add_argument('--advection-steps', type=int, default=100)
+    ap.add_argument('--euler', type=int, default=100, help='n of the n^3 x 6 tetrahedra Euler run (0: skip)')
+    ap.add_argument('--euler-steps', type=int, default=200)
     ap.add_argument('--tile', type=int, choices=(128, 256, 512), default=256, help='cells per tile of the fused step')
     ap.add_argument('--mixed-kernel', choices=('auto', 'split', 'tiled', 'staged'), default='auto',
                     help='mixed-precision step: autotuned (default), two kernels, fused, fused with LDS-staged states')
@@ -636,6 +792,12 @@ def main():
     if args.serving:
         out["serving"] = run_serving(args.serving)
         log("serving: p50 %.3f s, p99 %.3f s" % (out["serving"]["latency_p50_s"], out["serving"]["latency_p99_s"]))
+    if args.advection:
+        out["advection"] = run_advection(args.advection, args.advection_steps, 10, not args.no_cpu)
+        log("advection: %.1f Mcell-updates/s" % out["advection"]["value"])
+    if args.euler:
+        out["euler"] = run_euler(args.euler, args.euler_steps, 5, not args.no_cpu)
+        log("euler: %.1f Mcell-updates/s" % out["euler"]["value"])
     if not args.no_cpu:
         v, n, t = cpu_baseline(d0, Z0, h00, args.cpu_steps, args.cpu_budget)
         out["cpu_baseline"] = {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
