@@ -36,10 +36,10 @@ def measured_traffic(kernel, cells):
             data = json.load(fh)
     except (OSError, ValueError):
         return None
-    if data.get('cells') != cells:
-        return None
     k = data.get('kernels', {}).get(kernel)
-    return int(k['bytes']) if k else None
+    if not k or k.get('cells', data.get('cells')) != cells:
+        return None
+    return int(k['bytes'])
 
 
 def log(msg):
@@ -247,7 +247,8 @@ def run_advection(n, steps, warmup, cpu=True):
            "roofline": {"bound": "latency (the step's arrays, %.1f MB, stay in the 256 MB Infinity Cache)"
                                  % (bpc * C / 1e6), "kernel": "k_adv_step + k_dt_finish",
                         "achieved": round(bpc * C / t_ev / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(bpc * C / t_ev / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                        "frac": round(bpc * C / t_ev / 1e9 / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic('k_adv_step<2, 1>', C),
                         "bytes_per_cell": round(bpc, 1), "avg_step_us": round(t_ev * 1e6, 2)},
            "graph": graph}
     if cpu:
@@ -309,9 +310,10 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2):
     out = {"workload": "3D Sod shock tube, %d^3 x 6 Kuhn tetrahedra, first-order Rusanov, slip walls, CFL 0.8" % n,
            "cells": C, "faces": F, "steps": steps, "dtype": "f64", "mesh_build_s": round(t_mesh, 1),
            "value": round(C / t_loop / 1e6, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop * 1e3, 5),
-           "roofline": {"bound": "hbm", "kernel": "k_euler_step + k_dt_finish",
+           "roofline": {"bound": "hbm", "kernel": "k_euler_step_tet + k_dt_finish",
                         "achieved": round(bpc * C / t_ev / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(bpc * C / t_ev / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                        "frac": round(bpc * C / t_ev / 1e9 / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic('k_euler_step_tet', C),
                         "bytes_per_cell": round(bpc, 1), "avg_step_us": round(t_ev * 1e6, 2)}}
     if cpu and cpu_steps:
         from oracle import oracle as O
