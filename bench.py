@@ -270,7 +270,7 @@ def run_advection(n, steps, warmup, cpu=True):
     return out
 
 
-def run_euler(n, steps, warmup, cpu=True, cpu_steps=2):
+def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
     """3D Sod shock tube on n^3 cubes x 6 Kuhn tetrahedra of the unit cube (interior nodes
     jittered by +/-8 % of h, slip walls, first-order Rusanov, CFL 0.8): the fused device step
     (mfv_euler3d_step_fused). Parity: the first ``cpu_steps`` steps against the oracle, bit for
@@ -284,6 +284,9 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2):
     t0 = time.perf_counter()
     el, nd = kuhn_tetrahedra(n, 1.0, 0.08, 0)
     d = Domain(3, mesh=(el, nd))
+    if renumber == 'morton':  # cells along a Morton curve of their centres (a relabelling: same bits)
+        from manapy_amd.renumber import renumbered
+        d = renumbered(d)
     t_mesh = time.perf_counter() - t0
     C, F = int(d.nbcells), int(d.nbfaces)
     U0 = initialisation_euler_3d(d.cells.center)
@@ -298,22 +301,44 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2):
         _lib.call('mfv_euler3d_time_step', m.ptr, S[0].data_ptr(), CFL, 1.4, dt0.data_ptr(), None, s)
         _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
 
+    from manapy_amd.device import FaceTiles
+    tiles = {}
+    for T in (128, 256):
+        ft = FaceTiles(m, T)
+        if ft.lds_bytes() <= 64 * 1024:
+            tiles[T] = ft
+    path = [None]  # None: the per-cell fused step; T: the face-tiled step
+
     def step(k):
-        _lib.call('mfv_euler3d_step_fused', m.ptr, S[k % 2].data_ptr(), S[(k + 1) % 2].data_ptr(), 1.4, CFL, 0,
-                  ring.data_ptr(), k, s)
+        if path[0] is None:
+            _lib.call('mfv_euler3d_step_fused', m.ptr, S[k % 2].data_ptr(), S[(k + 1) % 2].data_ptr(), 1.4, CFL, 0,
+                      ring.data_ptr(), k, s)
+        else:
+            _lib.call('mfv_euler3d_step_tiled', m.ptr, tiles[path[0]].ptr, S[k % 2].data_ptr(),
+                      S[(k + 1) % 2].data_ptr(), 1.4, CFL, 0, ring.data_ptr(), k, s)
+    probe = {}
+    for cand in [None] + sorted(tiles):
+        path[0] = cand
+        init()
+        probe["tiled:%d" % cand if cand else "fused"] = time_steps(step, 2, 10)[1] * 1e3
+    best = min(probe, key=probe.get)
+    path[0] = None if best == 'fused' else int(best.split(':')[1])
+    log("euler: path %s %s" % (best, {k: round(v, 4) for k, v in probe.items()}))
     init()
     t_loop, t_ev = time_steps(step, warmup, steps)
     # algorithmic bytes of k_euler_step: U read + write (6 doubles each), volume, face count;
     # per face slot: order entry + faceid; per face (x F/C): cellid, normal, mesure, name
     MF = int(d.cells.faceid.shape[1] - 1)
     bpc = (48 + 48 + 8 + 4 + (1 + 4) * MF) + (8 + 24 + 8 + 4) * F / C
+    kname = "k_euler_step_tet" if path[0] is None else "k_euler_step_tiled<%d>" % path[0]
     out = {"workload": "3D Sod shock tube, %d^3 x 6 Kuhn tetrahedra, first-order Rusanov, slip walls, CFL 0.8" % n,
            "cells": C, "faces": F, "steps": steps, "dtype": "f64", "mesh_build_s": round(t_mesh, 1),
            "value": round(C / t_loop / 1e6, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop * 1e3, 5),
-           "roofline": {"bound": "hbm", "kernel": "k_euler_step_tet + k_dt_finish",
+           "path": best, "path_ms_per_step": {k: round(v, 4) for k, v in probe.items()}, "cell_order": renumber,
+           "roofline": {"bound": "hbm", "kernel": kname + " + k_dt_finish",
                         "achieved": round(bpc * C / t_ev / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(bpc * C / t_ev / 1e9 / HBM_PEAK_GBS, 4),
-                        "traffic": measured_traffic('k_euler_step_tet', C),
+                        "traffic": measured_traffic(kname, C),
                         "bytes_per_cell": round(bpc, 1), "avg_step_us": round(t_ev * 1e6, 2)}}
     if cpu and cpu_steps:
         from oracle import oracle as O
@@ -605,6 +630,7 @@ def main():
     ap.add_argument('--advection-steps', type=int, default=100)
     ap.add_argument('--euler', type=int, default=100, help='n of the n^3 x 6 tetrahedra Euler run (0: skip)')
     ap.add_argument('--euler-steps', type=int, default=200)
+    ap.add_argument('--euler-renumber', choices=('morton', 'none'), default='morton')
     ap.add_argument('--tile', type=int, choices=(128, 256, 512), default=256, help='cells per tile of the fused step')
     ap.add_argument('--mixed-kernel', choices=('auto', 'split', 'tiled', 'staged'), default='auto',
                     help='mixed-precision step: autotuned (default), two kernels, fused, fused with LDS-staged states')
@@ -826,7 +852,7 @@ def main():
         out["advection"] = run_advection(args.advection, args.advection_steps, 10, not args.no_cpu)
         log("advection: %.1f Mcell-updates/s" % out["advection"]["value"])
     if args.euler:
-        out["euler"] = run_euler(args.euler, args.euler_steps, 5, not args.no_cpu)
+        out["euler"] = run_euler(args.euler, args.euler_steps, 5, not args.no_cpu, renumber=args.euler_renumber)
         log("euler: %.1f Mcell-updates/s" % out["euler"]["value"])
     if not args.no_cpu:
         v, n, t = cpu_baseline(d0, Z0, h00, args.cpu_steps, args.cpu_budget)

@@ -393,6 +393,50 @@ int mfv_euler3d_update(const mfv_mesh *m, double *U, const double *rez, double g
 int mfv_euler3d_step_fused(const mfv_mesh *m, const double *U_in, double *U_out, double gamma,
                            double cfl, int quirk, double *dt_ring, int step, void *stream);
 
+/* Face tiles of a first-order cell-centred scheme: tiles of consecutive cells, each with its
+ * halo (the face neighbours outside it) and the list of faces that touch its cells, so that a
+ * block evaluates every such face once. Items of tile t: its ncells cells from cell0, then its
+ * halo halo_cells[halo_ptr[t] ..]. Faces of tile t: face_id[face_ptr[t] ..], ascending, with
+ * face_l = item position of faces.cellid[f][0] and face_r = that of faces.cellid[f][1] (-1 on
+ * a boundary face). lface[s][c] = (position of face slot s of cell c in its tile's face list)
+ * << 1 | (c is faces.cellid[f][1]). No counterpart in the reference: the layout of the tiled
+ * steps below. */
+typedef struct mfv_face_tiles {
+    int32_t nb_tiles;
+    int32_t tile_size;          /* cells per tile (and threads per block): 128, 256 or 512 */
+    int32_t max_items;          /* max over tiles of cells + halo */
+    int32_t max_faces;          /* max over tiles of the face count */
+    const int32_t *tile_cell0;  /* [nb_tiles] */
+    const int32_t *tile_ncells; /* [nb_tiles] */
+    const int32_t *halo_ptr;    /* [nb_tiles + 1] */
+    const int32_t *halo_cells;  /* [halo_ptr[nb_tiles]] */
+    const int32_t *face_ptr;    /* [nb_tiles + 1] */
+    const int32_t *face_id;     /* [face_ptr[nb_tiles]] */
+    const int32_t *face_l;      /* [face_ptr[nb_tiles]] */
+    const int32_t *face_r;      /* [face_ptr[nb_tiles]] */
+    const int32_t *lface;       /* [MF][C] */
+} mfv_face_tiles;
+
+/* Host-side construction (no GPU needed) from host copies of cells.faceid ([MF][C] slots,
+ * cell_nfaces) and faces.cellid ([F][2]); tiles of T consecutive cells. The arrays of out are
+ * malloc'd; release them with mfv_face_tiles_host_free. */
+typedef struct mfv_face_tiles_host {
+    int32_t nb_tiles, tile_size, max_items, max_faces, halo_len, face_len, MF, C;
+    int32_t *tile_cell0, *tile_ncells, *halo_ptr, *halo_cells, *face_ptr, *face_id, *face_l, *face_r, *lface;
+} mfv_face_tiles_host;
+
+int mfv_face_tiles_build(int C, int MF, const int32_t *cell_faces, const int32_t *cell_nfaces, int F,
+                         const int32_t *face_cells, int T, mfv_face_tiles_host *out);
+void mfv_face_tiles_host_free(mfv_face_tiles_host *t);
+
+/* The fused Euler step on face tiles (tetrahedra): every face that touches a tile is evaluated
+ * once in LDS and both of its cells in the tile take the flux from there, in the residual
+ * order; same bits as mfv_euler3d_step_fused. LDS: 48 * max_items + 40 * max_faces bytes
+ * (<= 64 KiB). Replaces explicitscheme_euler_3d_rusanov (fvm_utils3d.py:1083-1170) +
+ * update_euler_3d + time_step_euler_3d in one launch. */
+int mfv_euler3d_step_tiled(const mfv_mesh *m, const mfv_face_tiles *t, const double *U_in, double *U_out,
+                           double gamma, double cfl, int quirk, double *dt_ring, int step, void *stream);
+
 /* ---------------- diamond scheme, 2D (ast/functions2d.py, solvers/diffusion) --------- */
 
 /* Diamond-scheme geometry. The pseudo-Laplacian weight of a node incidence is

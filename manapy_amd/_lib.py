@@ -50,6 +50,21 @@ class MfvSwTilesHost(ctypes.Structure):
                           'lgh', 'lfn')]
 
 
+class MfvFaceTiles(ctypes.Structure):
+    """Mirror of ``struct mfv_face_tiles`` (include/mfv.h)."""
+    _fields_ = [(k, ctypes.c_int32) for k in ('nb_tiles', 'tile_size', 'max_items', 'max_faces')] + \
+        [(k, P) for k in ('tile_cell0', 'tile_ncells', 'halo_ptr', 'halo_cells', 'face_ptr', 'face_id', 'face_l',
+                          'face_r', 'lface')]
+
+
+class MfvFaceTilesHost(ctypes.Structure):
+    """Mirror of ``struct mfv_face_tiles_host`` (include/mfv.h)."""
+    _fields_ = [(k, ctypes.c_int32) for k in ('nb_tiles', 'tile_size', 'max_items', 'max_faces', 'halo_len',
+                                              'face_len', 'MF', 'C')] + \
+        [(k, P) for k in ('tile_cell0', 'tile_ncells', 'halo_ptr', 'halo_cells', 'face_ptr', 'face_id', 'face_l',
+                          'face_r', 'lface')]
+
+
 class MfvSwPacked(ctypes.Structure):
     """Mirror of ``struct mfv_sw_packed`` (include/mfv.h)."""
     _fields_ = [('nb_cells', ctypes.c_int32), ('max_cell_faces', ctypes.c_int32), ('cellrec', P), ('facerec', P),
@@ -106,6 +121,7 @@ SWB_P = ctypes.POINTER(MfvSwBatch)
 SWP_P = ctypes.POINTER(MfvSwPacked)
 SWM_P = ctypes.POINTER(MfvSwMixed)
 SWT_P = ctypes.POINTER(MfvSwTiles)
+FT_P = ctypes.POINTER(MfvFaceTiles)
 DP = ctypes.POINTER(ctypes.c_double)
 
 # name -> argtypes (restype is int unless listed in _RESTYPE)
@@ -145,6 +161,9 @@ _SIGNATURES = {
     'mfv_euler3d_rusanov': [MESH_P, P, P, c_double, c_int, P, P],
     'mfv_euler3d_update': [MESH_P, P, P, c_double, P, P],
     'mfv_euler3d_step_fused': [MESH_P, P, P, c_double, c_double, c_int, P, c_int, P],
+    'mfv_euler3d_step_tiled': [MESH_P, FT_P, P, P, c_double, c_double, c_int, P, c_int, P],
+    'mfv_face_tiles_build': [c_int, c_int, P, P, c_int, P, c_int, ctypes.POINTER(MfvFaceTilesHost)],
+    'mfv_face_tiles_host_free': [ctypes.POINTER(MfvFaceTilesHost)],
     'mfv_diamond_widths': [DIA_P, P, P, P, P, P],
     'mfv_diamond_assemble': [DIA_P, P, P, c_int, c_int, P, P, P, P, P, P],
     'mfv_diamond_rhs': [DIA_P, P, P, P, P, P, P],
@@ -169,7 +188,8 @@ _SIGNATURES = {
     'mfv_krylov_tick': [ELL_P, KW_P, c_int, c_int, c_int, P],
     'mfv_ell_spmv': [ELL_P, P, P, c_int, P],
 }
-_RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p, 'mfv_sw_tiles_host_free': None}
+_RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p, 'mfv_sw_tiles_host_free': None,
+            'mfv_face_tiles_host_free': None}
 
 _lib = None
 

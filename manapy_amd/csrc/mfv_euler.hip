@@ -332,6 +332,122 @@ __global__ void __launch_bounds__(kBlock) k_euler_step_tet(Mesh m, const double 
     wave_min_part(dtpart, (size_t)i, d);
 }
 
+// The fused step on face tiles (mfv_face_tiles): the block stages the states of its cells and
+// halo in LDS ([6][max_items]), evaluates every face of its tile once into LDS ([5][max_faces]),
+// then each cell adds its faces' fluxes in the residual order (increasing face id), updates
+// and takes its next-dt candidate. Same expressions and order as k_euler_step, so the same
+// bits; a face is evaluated once per tile instead of once per cell side.
+struct FaceTilesK {
+    const int32_t *__restrict__ cell0;
+    const int32_t *__restrict__ ncells;
+    const int32_t *__restrict__ hptr;
+    const int32_t *__restrict__ halo;
+    const int32_t *__restrict__ fptr;
+    const int32_t *__restrict__ fid;
+    const int32_t *__restrict__ fl;
+    const int32_t *__restrict__ fr;
+    const int32_t *__restrict__ lface;
+    int MI, MFc;  // max items, max faces per tile
+};
+
+template <int T>
+__global__ void __launch_bounds__(T) k_euler_step_tiled(Mesh m, FaceTilesK t, const double *__restrict__ Uin,
+                                                         double *__restrict__ Uout, double gamma, double cfl,
+                                                         int quirk, DtRing ring, double *__restrict__ dtpart)
+{
+    extern __shared__ double lds[];
+    double *sU = lds;               // [6][MI]
+    double *sF = lds + 6 * t.MI;    // [5][MFc]
+    const int tile = xcd_block();
+    const int c0 = t.cell0[tile], nc = t.ncells[tile];
+    const int h0 = t.hptr[tile], ni = nc + (t.hptr[tile + 1] - h0);
+    const int f0 = t.fptr[tile], nfc = t.fptr[tile + 1] - f0;
+    const size_t C = m.C;
+    ring_advance(ring);
+    for (int e = threadIdx.x; e < ni; e += T) {
+        const int c = (e < nc) ? c0 + e : t.halo[h0 + (e - nc)];
+#pragma unroll
+        for (int v = 0; v < 6; v++) sU[v * t.MI + e] = Uin[v * C + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nfc; k += T) {
+        const int f = t.fid[f0 + k], pl = t.fl[f0 + k], pr = t.fr[f0 + k];
+        const int name = m.face_name[f];
+        const double mesu = m.face_mesure[f];
+        const double nrm[3] = {m.face_normal[3 * (size_t)f], m.face_normal[3 * (size_t)f + 1],
+                               m.face_normal[3 * (size_t)f + 2]};
+        const double n[3] = {nrm[0] / mesu, nrm[1] / mesu, nrm[2] / mesu};
+        const Prim6 L = {sU[pl], sU[t.MI + pl], sU[2 * t.MI + pl], sU[3 * t.MI + pl], sU[4 * t.MI + pl],
+                         sU[5 * t.MI + pl]};
+        Prim6 R;
+        if (name == 0) {
+            R = {sU[pr], sU[t.MI + pr], sU[2 * t.MI + pr], sU[3 * t.MI + pr], sU[4 * t.MI + pr], sU[5 * t.MI + pr]};
+        } else {
+            R = euler_ghost(L, nrm, name, quirk, 0.);
+        }
+        double fl[5];
+        euler_flux(L, R, n, mesu, gamma, quirk, fl);
+#pragma unroll
+        for (int v = 0; v < 5; v++) sF[v * t.MFc + k] = fl[v];
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if ((q & ~63) >= nc) return;  // the whole wave lies past the tile: it owns no partial
+    double d = 1e300;
+    if (q < nc) {
+        const int i = c0 + q;
+        slot_i lf;
+        unsigned ord = 0;
+        const int nf = m.cell_nfaces[i];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            lf[s] = t.lface[(size_t)s * C + i];
+            ord |= (unsigned)m.order_sorted[(size_t)s * C + i] << (2 * s);
+        }
+        double nx[4], ny[4], nz[4], ms[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int f = m.cell_faces[(size_t)j * C + i];
+            nx[j] = m.face_normal[3 * (size_t)f];
+            ny[j] = m.face_normal[3 * (size_t)f + 1];
+            nz[j] = m.face_normal[3 * (size_t)f + 2];
+            ms[j] = m.face_mesure[f];
+        }
+        double r[5] = {0., 0., 0., 0., 0.};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {  // faces in increasing id (order_sorted)
+            if (k >= nf) break;
+            const int code = lf[(ord >> (2 * k)) & 3];
+            const int ph = code >> 1;
+            const bool own_R = code & 1;
+            // rez -= fl (own = L) is rez += -fl
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                const double x = sF[v * t.MFc + ph];
+                r[v] += own_R ? x : -x;
+            }
+        }
+        double u[6];
+#pragma unroll
+        for (int v = 0; v < 6; v++) u[v] = sU[v * t.MI + q];
+        euler_update_cell(u, r, gamma, ring.cur[0], m.cell_volume[i]);
+#pragma unroll
+        for (int v = 0; v < 6; v++) Uout[v * C + i] = u[v];
+        // time_step_euler_3d of the new state (euler_cell_dt, first four faces)
+        double lam = 0.;
+        const double velson = sqrt(gamma * fabs(u[1] / u[0]));
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double u_n = fabs((u[2] * nx[j] + u[3] * ny[j] + u[4] * nz[j]) / u[0]) /
+                               sqrt(nx[j] * nx[j] + ny[j] * ny[j] + nz[j] * nz[j]);
+            const double lam_convect = u_n + velson;
+            lam += lam_convect * ms[j];
+        }
+        d = cfl * m.cell_volume[i] / lam;
+    }
+    wave_min_part(dtpart, (size_t)c0 + q, d);
+}
+
 }  // namespace
 
 extern "C" int mfv_euler3d_ghost(const mfv_mesh *m, const double *U, double *G, int quirk, void *stream)
@@ -411,4 +527,56 @@ extern "C" int mfv_euler3d_step_fused(const mfv_mesh *m, const double *U_in, dou
                        dim3(kBlock), 0, as_stream(stream), to_kernel(m), U_in, U_out, gamma, cfl, quirk, r, part);
     if (int rc = check_launch("k_euler_step")) return rc;
     return dt_finish(part, nw, r.next, as_stream(stream));
+}
+
+static int check_face_tiles(const mfv_mesh *m, const mfv_face_tiles *t)
+{
+    MFV_REQUIRE(t != nullptr, "face tiles descriptor is NULL");
+    MFV_REQUIRE(t->tile_size == 128 || t->tile_size == 256 || t->tile_size == 512, "tile_size %d not in {128, 256, "
+                "512}", t->tile_size);
+    MFV_REQUIRE(t->nb_tiles == (m->nb_cells + t->tile_size - 1) / t->tile_size,
+                "face tiles were built for another mesh (%d tiles of %d cells for %d cells)", t->nb_tiles,
+                t->tile_size, m->nb_cells);
+    MFV_REQUIRE(t->max_items >= 1 && t->max_faces >= 1, "bad tile counts");
+    MFV_REQUIRE(t->tile_cell0 && t->tile_ncells && t->halo_ptr && t->halo_cells && t->face_ptr && t->face_id &&
+                    t->face_l && t->face_r && t->lface,
+                "face tiles descriptor has NULL arrays");
+    const size_t lds = 48 * (size_t)t->max_items + 40 * (size_t)t->max_faces;
+    MFV_REQUIRE(lds <= 64 * 1024, "face tiles of %d cells need %zu bytes of LDS (> 64 KiB): use smaller tiles or "
+                "renumber the cells", t->tile_size, lds);
+    return MFV_OK;
+}
+
+extern "C" int mfv_euler3d_step_tiled(const mfv_mesh *m, const mfv_face_tiles *t, const double *U_in, double *U_out,
+                                      double gamma, double cfl, int quirk, double *dt_ring_p, int step, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 1)) return rc;
+    MFV_REQUIRE(m->dim == 3 && m->max_cell_faces == 4, "the tiled Euler step needs tetrahedra");
+    if (int rc = check_face_tiles(m, t)) return rc;
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(U_in && U_out && U_in != U_out && dt_ring_p, "NULL or aliased array");
+    DtRing r = dt_ring(dt_ring_p, step);
+    const size_t nw = waves_for(m->nb_cells);
+    double *part = dt_scratch(nw, as_stream(stream));
+    if (!part) return MFV_ENOMEM;
+    const FaceTilesK tk{t->tile_cell0, t->tile_ncells, t->halo_ptr, t->halo_cells, t->face_ptr, t->face_id,
+                        t->face_l,    t->face_r,      t->lface,    t->max_items, t->max_faces};
+    const size_t lds = 48 * (size_t)t->max_items + 40 * (size_t)t->max_faces;
+    hipStream_t st = as_stream(stream);
+    switch (t->tile_size) {
+    case 128:
+        hipLaunchKernelGGL(k_euler_step_tiled<128>, dim3(t->nb_tiles), dim3(128), lds, st, to_kernel(m), tk, U_in,
+                           U_out, gamma, cfl, quirk, r, part);
+        break;
+    case 256:
+        hipLaunchKernelGGL(k_euler_step_tiled<256>, dim3(t->nb_tiles), dim3(256), lds, st, to_kernel(m), tk, U_in,
+                           U_out, gamma, cfl, quirk, r, part);
+        break;
+    default:
+        hipLaunchKernelGGL(k_euler_step_tiled<512>, dim3(t->nb_tiles), dim3(512), lds, st, to_kernel(m), tk, U_in,
+                           U_out, gamma, cfl, quirk, r, part);
+        break;
+    }
+    if (int rc = check_launch("k_euler_step_tiled")) return rc;
+    return dt_finish(part, nw, r.next, st);
 }

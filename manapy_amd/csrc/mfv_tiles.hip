@@ -243,3 +243,123 @@ extern "C" int mfv_sw_tiles_build(int C, int MF, const int32_t *nbr, int K, cons
     }
     return MFV_OK;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Face tiles (mfv_face_tiles): tiles of T consecutive cells, their halo (face neighbours outside
+// the tile, ascending) and the faces touching their cells (ascending), so that a block
+// evaluates each such face once.
+
+extern "C" void mfv_face_tiles_host_free(mfv_face_tiles_host *t)
+{
+    if (!t) return;
+    free(t->tile_cell0);
+    free(t->tile_ncells);
+    free(t->halo_ptr);
+    free(t->halo_cells);
+    free(t->face_ptr);
+    free(t->face_id);
+    free(t->face_l);
+    free(t->face_r);
+    free(t->lface);
+    memset(t, 0, sizeof(*t));
+}
+
+extern "C" int mfv_face_tiles_build(int C, int MF, const int32_t *cell_faces, const int32_t *cell_nfaces, int F,
+                                    const int32_t *face_cells, int T, mfv_face_tiles_host *out)
+{
+    MFV_REQUIRE(out, "out is NULL");
+    memset(out, 0, sizeof(*out));
+    MFV_REQUIRE(C > 0 && F > 0 && MF >= 1 && MF <= 8, "bad sizes C=%d F=%d MF=%d", C, F, MF);
+    MFV_REQUIRE(cell_faces && cell_nfaces && face_cells, "NULL array");
+    MFV_REQUIRE(T == 128 || T == 256 || T == 512, "tile size %d not in {128, 256, 512}", T);
+    for (int c = 0; c < C; c++) {
+        MFV_REQUIRE(cell_nfaces[c] >= 0 && cell_nfaces[c] <= MF, "cell %d has %d faces (max %d)", c, cell_nfaces[c],
+                    MF);
+        for (int s = 0; s < cell_nfaces[c]; s++) {
+            const int f = cell_faces[(size_t)s * C + c];
+            MFV_REQUIRE(f >= 0 && f < F, "cell %d slot %d: face %d out of range", c, s, f);
+            const int a = face_cells[2 * (size_t)f], b = face_cells[2 * (size_t)f + 1];
+            MFV_REQUIRE((a == c || b == c) && a >= 0 && a < C && b < C, "face %d does not list cell %d", f, c);
+        }
+    }
+    const int nt = (C + T - 1) / T;
+    std::vector<int32_t> cell0(nt), ncells(nt), hptr(nt + 1, 0), fptr(nt + 1, 0), halo, fid, fl, fr;
+    std::vector<int32_t> lface((size_t)MF * C, -1);
+    std::vector<int32_t> cmark(C, -1), pos(C, 0), fmark(F, -1), fpos(F, 0);
+    std::vector<int32_t> th, tf;
+    int max_items = 0, max_faces = 0;
+    for (int t = 0; t < nt; t++) {
+        const int c0 = t * T, nc = std::min(T, C - c0);
+        cell0[t] = c0;
+        ncells[t] = nc;
+        for (int q = 0; q < nc; q++) {
+            cmark[c0 + q] = t;
+            pos[c0 + q] = q;
+        }
+        th.clear();
+        tf.clear();
+        for (int q = 0; q < nc; q++) {
+            const int c = c0 + q;
+            for (int s = 0; s < cell_nfaces[c]; s++) {
+                const int f = cell_faces[(size_t)s * C + c];
+                if (fmark[f] != t) {
+                    fmark[f] = t;
+                    tf.push_back(f);
+                }
+                const int a = face_cells[2 * (size_t)f], b = face_cells[2 * (size_t)f + 1];
+                const int o = (a == c) ? b : a;
+                if (o >= 0 && cmark[o] != t) {
+                    cmark[o] = t;
+                    th.push_back(o);
+                }
+            }
+        }
+        std::sort(th.begin(), th.end());
+        std::sort(tf.begin(), tf.end());
+        for (size_t k = 0; k < th.size(); k++) pos[th[k]] = nc + (int)k;
+        for (size_t k = 0; k < tf.size(); k++) {
+            const int f = tf[k];
+            fpos[f] = (int)k;
+            const int a = face_cells[2 * (size_t)f], b = face_cells[2 * (size_t)f + 1];
+            fid.push_back(f);
+            fl.push_back(pos[a]);
+            fr.push_back(b >= 0 ? pos[b] : -1);
+        }
+        for (int q = 0; q < nc; q++) {
+            const int c = c0 + q;
+            for (int s = 0; s < cell_nfaces[c]; s++) {
+                const int f = cell_faces[(size_t)s * C + c];
+                lface[(size_t)s * C + c] = (fpos[f] << 1) | (face_cells[2 * (size_t)f + 1] == c ? 1 : 0);
+            }
+        }
+        halo.insert(halo.end(), th.begin(), th.end());
+        hptr[t + 1] = (int32_t)halo.size();
+        fptr[t + 1] = (int32_t)fid.size();
+        max_items = std::max(max_items, nc + (int)th.size());
+        max_faces = std::max(max_faces, (int)tf.size());
+    }
+    out->nb_tiles = nt;
+    out->tile_size = T;
+    out->max_items = max_items;
+    out->max_faces = max_faces;
+    out->halo_len = (int32_t)halo.size();
+    out->face_len = (int32_t)fid.size();
+    out->MF = MF;
+    out->C = C;
+    out->tile_cell0 = copy_out(cell0);
+    out->tile_ncells = copy_out(ncells);
+    out->halo_ptr = copy_out(hptr);
+    out->halo_cells = copy_out(halo);
+    out->face_ptr = copy_out(fptr);
+    out->face_id = copy_out(fid);
+    out->face_l = copy_out(fl);
+    out->face_r = copy_out(fr);
+    out->lface = copy_out(lface);
+    if (!out->tile_cell0 || !out->tile_ncells || !out->halo_ptr || !out->halo_cells || !out->face_ptr ||
+        !out->face_id || !out->face_l || !out->face_r || !out->lface) {
+        mfv_face_tiles_host_free(out);
+        mfv::set_error("face tiles: host allocation failed");
+        return MFV_ENOMEM;
+    }
+    return MFV_OK;
+}

@@ -475,6 +475,62 @@ def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None, 
     return out
 
 
+def face_tiles_native(cell_faces, cell_nfaces, face_cells, T):
+    """Face tiles of T consecutive cells (mfv_face_tiles_build): halo, faces touching each tile
+    and the per-cell-slot face positions. cell_faces [MF][C], cell_nfaces [C], face_cells [F][2]."""
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    cf, nf, fc = i32(cell_faces), i32(cell_nfaces), i32(face_cells)
+    MF, C = cf.shape
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    h = _lib.MfvFaceTilesHost()
+    _lib.call('mfv_face_tiles_build', int(C), int(MF), P(cf), P(nf), int(fc.shape[0]), P(fc), int(T),
+              ctypes.byref(h))
+    try:
+        nt = h.nb_tiles
+
+        def take(ptr, n, shape=None):
+            a = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_int32)), (max(n, 1),))[:n].copy()
+            return a.reshape(shape) if shape is not None else a
+        out = {'T': int(T), 'nb_tiles': nt, 'max_items': h.max_items, 'max_faces': h.max_faces, 'MF': MF, 'C': C,
+               'tile_cell0': take(h.tile_cell0, nt), 'tile_ncells': take(h.tile_ncells, nt),
+               'halo_ptr': take(h.halo_ptr, nt + 1), 'halo_cells': take(h.halo_cells, h.halo_len),
+               'face_ptr': take(h.face_ptr, nt + 1), 'face_id': take(h.face_id, h.face_len),
+               'face_l': take(h.face_l, h.face_len), 'face_r': take(h.face_r, h.face_len),
+               'lface': take(h.lface, MF * C, (MF, C))}
+    finally:
+        _lib.call('mfv_face_tiles_host_free', ctypes.byref(h))
+    return out
+
+
+class FaceTiles(object):
+    """Device copy of a face-tiles layout (mfv_face_tiles) for a DeviceMesh, built on the host
+    from its cell/face tables."""
+
+    ARRAYS = ('tile_cell0', 'tile_ncells', 'halo_ptr', 'halo_cells', 'face_ptr', 'face_id', 'face_l', 'face_r',
+              'lface')
+
+    def __init__(self, dmesh, T=256):
+        import torch
+        H = dmesh.host
+        self.host = face_tiles_native(H['cell_faces'], H['cell_nfaces'], H['face_cells'], T)
+        self.t = {}
+        d = _lib.MfvFaceTiles()
+        d.nb_tiles, d.tile_size = self.host['nb_tiles'], T
+        d.max_items, d.max_faces = self.host['max_items'], self.host['max_faces']
+        for k in self.ARRAYS:
+            a = np.ascontiguousarray(self.host[k])
+            self.t[k] = torch.from_numpy(a if a.size else np.zeros(1, dtype=np.int32)).to(dmesh.device)
+            setattr(d, k, self.t[k].data_ptr())
+        self.desc = d
+
+    @property
+    def ptr(self):
+        return ctypes.byref(self.desc)
+
+    def lds_bytes(self):
+        return 48 * self.host['max_items'] + 40 * self.host['max_faces']
+
+
 class SwTiles(object):
     """Device copy of a tiles layout and its mfv_sw_tiles descriptor (stage=True copies every
     cell state a tile reads into LDS before reconstructing)."""

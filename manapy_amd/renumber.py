@@ -90,6 +90,24 @@ def _remap_padded(a, iperm):
     return out
 
 
+class _RenumberedCells(Struct):
+    """Cell tables of a RenumberedDomain. The least-squares stencil tables (cellnid, ghostnid)
+    are remapped on first access: first-order paths never build them (on a 6M-tetrahedra mesh
+    the base Domain's vertex-neighbour table alone takes most of a minute)."""
+
+    def __getattr__(self, name):
+        if name in ('cellnid', 'ghostnid'):
+            base, perm, iperm = self.__dict__['_src']
+            bc = base.cells
+            if name == 'cellnid':
+                v = _remap_padded(bc.cellnid[perm], iperm).astype(bc.cellnid.dtype)
+            else:
+                v = np.ascontiguousarray(bc.ghostnid[perm])
+            setattr(self, name, v)
+            return v
+        raise AttributeError(name)
+
+
 class RenumberedDomain(object):
     """Domain view with cells in ``perm`` order (perm[new] = old); see the module doc."""
 
@@ -111,13 +129,12 @@ class RenumberedDomain(object):
         self.innerfaces = domain.innerfaces
         self.boundaryfaces = domain.boundaryfaces
         bc = domain.cells
-        cells = Struct()
+        cells = _RenumberedCells()
+        cells.__dict__['_src'] = (domain, perm, iperm)
         cells.faceid = np.ascontiguousarray(bc.faceid[perm])
         cells.center = np.ascontiguousarray(bc.center[perm])
         cells.volume = np.ascontiguousarray(bc.volume[perm])
         cells.nodeid = np.ascontiguousarray(bc.nodeid[perm])
-        cells.cellnid = _remap_padded(bc.cellnid[perm], iperm).astype(bc.cellnid.dtype)
-        cells.ghostnid = np.ascontiguousarray(bc.ghostnid[perm])
         cells.cellfid = _remap_padded(bc.cellfid[perm], iperm).astype(bc.cellfid.dtype)
         if hasattr(bc, 'nf'):
             cells.nf = np.ascontiguousarray(bc.nf[perm])

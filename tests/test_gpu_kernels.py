@@ -315,6 +315,40 @@ def test_euler_fused_40_steps(torch, quirk, d3, m3, oracle_lib):
     same(host(S[0]), U, "U after 40 steps")
 
 
+@pytest.mark.parametrize("quirk,T,n", [(0, 128, 6), (1, 256, 6), (1, 128, 11), (0, 256, 11)])
+def test_euler_tiled_40_steps(torch, quirk, T, n, d3, m3, oracle_lib):
+    """The face-tiled Euler step (each face evaluated once per tile in LDS) gives the oracle's
+    bits after 40 steps, and those of the per-cell fused step; n = 11 has a partial last tile."""
+    from manapy_amd.device import DeviceMesh, FaceTiles
+    d, m = (d3, m3) if n == 6 else (make_tet_domain(n, 0.08, 4), None)
+    m = m if m is not None else DeviceMesh(d, lsq=False)
+    C, F = int(d.nbcells), int(d.nbfaces)
+    U0 = sod(d)
+    U = U0.copy()
+    G_ = np.zeros((6, F))
+    for _ in range(40):
+        oracle_lib.euler_step(d, U, G_, 0.8, 1.4, quirk)
+    ft = FaceTiles(m, T)
+    assert ft.lds_bytes() <= 64 * 1024
+    res = []
+    for tiled in (True, False):
+        S = [dev(torch, U0), dev(torch, np.zeros((6, C)))]
+        dt0 = dev(torch, np.zeros(1))
+        ring = dev(torch, np.zeros(4))
+        call('mfv_euler3d_time_step', m.ptr, P(S[0]), 0.8, 1.4, P(dt0), None)
+        call('mfv_dt_ring_init', P(ring), P(dt0))
+        for s in range(40):
+            if tiled:
+                call('mfv_euler3d_step_tiled', m.ptr, ft.ptr, P(S[s % 2]), P(S[(s + 1) % 2]), 1.4, 0.8, quirk, P(ring),
+                     s)
+            else:
+                call('mfv_euler3d_step_fused', m.ptr, P(S[s % 2]), P(S[(s + 1) % 2]), 1.4, 0.8, quirk, P(ring), s)
+        res.append((host(S[0]), host(ring)))
+    same(res[0][0], U, "tiled U after 40 steps")
+    same(res[0][0], res[1][0], "tiled = fused")
+    same(res[0][1], res[1][1], "dt ring")
+
+
 def test_sw_packed_60_steps(torch, sw_case, oracle_lib):
     """The packed path (one 64-byte record per cell, MUSCL values per face slot) gives the
     same bits as the reference step sequence."""
