@@ -124,7 +124,7 @@ def mixed_bytes_per_cell(MF, K_avg, G_avg):
              + 4 * K_avg + 4            # cellnid entries + count
              + G_avg * (4 + 4 + 16) + 4  # ghostnid entries, owner, ghost centre + count
              + 4 * MF + 4               # neighbour codes + face count
-             + 4 * MF + 8 * MF          # geoA.w (z_f - Z), geoB (rx, ry)
+             + 4 * MF + 16 * MF         # geoA.w (z_f - Z), geoB (rx, ry, |n|, 1/|n|)
              + 16 * MF)                 # write face records
     flux = (32 + 24 + 8                 # state read + write, volume
             + 4 + 1                     # face count, residual order
@@ -152,7 +152,7 @@ def tiled_bytes_per_cell(MF, K_avg, G_avg):
             + 4 * K_avg + 4             # cellnid entries + count
             + G_avg * (4 + 4 + 16) + 4  # ghostnid entries, owner, ghost centre + count
             + 4 * MF + 4 * MF           # global and block-local neighbour codes
-            + 16 * MF + 8 * MF)         # geoA, geoB
+            + 16 * MF + 16 * MF)        # geoA, geoB
 
 
 def time_steps(step, warmup, steps):

@@ -237,7 +237,7 @@ typedef struct mfv_sw_mixed {
     const double *vol;      /* [C]        cell volumes (float64) */
     float *facerec;         /* [C][MF][4] face records (mfv_sw_recon_mixed) */
     float *geoA;            /* [C][MF][4] n0, n1, zf_R - zf_L, zf_own - Z_i (mfv_sw_pack_mixed) */
-    float *geoB;            /* [C][MF][2] face centre - cell centre (mfv_sw_pack_mixed) */
+    float *geoB;            /* [C][MF][4] face centre - cell centre, |n|, 1/|n| (mfv_sw_pack_mixed) */
     const int32_t *nbr;     /* [MF][C]    as mfv_sw_packed */
     const uint8_t *order;   /* [C]        as mfv_sw_packed */
 } mfv_sw_mixed;

@@ -11,9 +11,10 @@
 //     same float32 numbers, so the mass flux is the same bits on both sides;
 //   * pressures enter the momentum fluxes as deviations from the own cell's hydrostatic
 //     pressure 1/2 g h_K^2 (whose contribution sum_f n_f vanishes on a closed cell);
-//   * the residual gathers add the float32 face terms with TwoSum compensation (exact error
-//     terms, any order of magnitudes) and finish in float64, so the float64 update conserves
-//     mass to round-off.
+//   * the residual gathers add the float32 face terms in float64 (float64 adds cost what
+//     float32 adds do on this GPU, and a sum of a few float32 terms is then exact or nearly),
+//     so the float64 update conserves mass to round-off.
+//   * the least-squares sums use fused multiply-adds (fmaf), as the CPU model does.
 // dt and the conservation totals are float64.
 #include "mfv_common.hpp"
 
@@ -23,16 +24,6 @@ using namespace mfv;
 
 // Python max(a, b) on floats.
 __device__ __forceinline__ float pmaxf(float a, float b) { return (b > a) ? b : a; }
-
-// Knuth's TwoSum accumulation: s + c carries the sum with the rounding errors in c.
-__device__ __forceinline__ void twosum_add(float &s, float &c, float x)
-{
-    const float t = s + x;
-    const float bp = t - s;
-    const float e = (s - (t - bp)) + (x - bp);
-    s = t;
-    c += e;
-}
 
 // One face in its canonical orientation L -> R (normal n out of L, |n| = mes).
 //   hb          depth of the L cell (float32)
@@ -44,8 +35,8 @@ __device__ __forceinline__ void twosum_add(float &s, float &c, float x)
 // Outputs: mass and momentum fluxes L -> R (momentum with pressures relative to the own cell)
 // and the own side's bathymetry term.
 __device__ __forceinline__ void face_dev(float hb, float aL, float aR, float dz, float huL, float hvL, float huR,
-                                         float hvR, float dzo, float n0, float n1, float mes, float g, float shift,
-                                         bool own_is_L, float &F0, float &F1, float &F2, float &bo)
+                                         float hvR, float dzo, float n0, float n1, float mes, float im, float g,
+                                         float shift, bool own_is_L, float &F0, float &F1, float &F2, float &bo)
 {
     float hL = hb + aL, hR = hb + aR;
     const float cL = pmaxf(0.f, dz), cR = pmaxf(0.f, -dz);
@@ -74,7 +65,6 @@ __device__ __forceinline__ void face_dev(float hb, float aL, float aR, float dz,
         vR = hvR * r;
     }
     const float qL = uL * n0 + vL * n1, qR = uR * n0 + vR * n1;
-    const float im = 1.f / mes;
     const float sL = fabsf(qL) * im + sqrtf(g * hsL), sR = fabsf(qR) * im + sqrtf(g * hsR);
     const float s = pmaxf(sL, sR) * mes;
     const float ho = hb - shift;
@@ -90,7 +80,8 @@ __device__ __forceinline__ void face_dev(float hb, float aL, float aR, float dz,
 }
 
 // Static face geometry and the constant bathymetry differences (Z gradient from zgrad [3][C]):
-//   geoA[i][s] = (n0, n1, zf_R - zf_L, zf_own - Z_i), geoB[i][s] = (rx, ry)
+//   geoA[i][s] = (n0, n1, zf_R - zf_L, zf_own - Z_i), geoB[i][s] = (rx, ry, |n|, 1/|n|) with |n|
+//   the float32 length of the float32 normal (what the flux and dt would compute every step)
 template <int MF>
 __global__ void __launch_bounds__(kBlock) k_sw_pack32(Mesh m, mfv_sw_mixed p, const double *__restrict__ h,
                                                        const double *__restrict__ hu,
@@ -117,9 +108,10 @@ __global__ void __launch_bounds__(kBlock) k_sw_pack32(Mesh m, mfv_sw_mixed p, co
             dz = (L == i) ? zt - zo : zo - zt;
         }
         const double *n = m.face_normal + 3 * (size_t)f;
-        reinterpret_cast<float4 *>(p.geoA)[MF * (size_t)i + s] =
-            make_float4((float)n[0], (float)n[1], (float)dz, (float)(zo - Z[i]));
-        reinterpret_cast<float2 *>(p.geoB)[MF * (size_t)i + s] = make_float2((float)rx, (float)ry);
+        const float n0 = (float)n[0], n1 = (float)n[1];
+        reinterpret_cast<float4 *>(p.geoA)[MF * (size_t)i + s] = make_float4(n0, n1, (float)dz, (float)(zo - Z[i]));
+        const float mes = sqrtf(n0 * n0 + n1 * n1);
+        reinterpret_cast<float4 *>(p.geoB)[MF * (size_t)i + s] = make_float4((float)rx, (float)ry, mes, 1.f / mes);
     }
 }
 
@@ -127,15 +119,15 @@ __device__ __forceinline__ void accum_grad(float jx, float jy, float de, float d
                                            float &ixy, float &je0, float &je1, float &ju0, float &ju1, float &jv0,
                                            float &jv1)
 {
-    ixx += jx * jx;
-    iyy += jy * jy;
-    ixy += jx * jy;
-    je0 += jx * de;
-    je1 += jy * de;
-    ju0 += jx * du;
-    ju1 += jy * du;
-    jv0 += jx * dv;
-    jv1 += jy * dv;
+    ixx = fmaf(jx, jx, ixx);
+    iyy = fmaf(jy, jy, iyy);
+    ixy = fmaf(jx, jy, ixy);
+    je0 = fmaf(jx, de, je0);
+    je1 = fmaf(jy, de, je1);
+    ju0 = fmaf(jx, du, ju0);
+    ju1 = fmaf(jy, du, ju1);
+    jv0 = fmaf(jx, dv, jv0);
+    jv1 = fmaf(jy, dv, jv1);
 }
 
 // Reconstruction of cell i: float32 LSQ gradients of eta, hu, hv from float64 differences,
@@ -221,7 +213,8 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
     float dzo[MF];  // z_f - Z of each slot
 #pragma unroll
     for (int s = 0; s < MF; s++) {
-        gb[s] = reinterpret_cast<const float2 *>(p.geoB)[MF * (size_t)i + s];
+        const float4 q = reinterpret_cast<const float4 *>(p.geoB)[MF * (size_t)i + s];
+        gb[s] = make_float2(q.x, q.y);
         dzo[s] = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s].w;
     }
     for (int j = 0; j < ng; j++) {
@@ -320,10 +313,10 @@ __global__ void __launch_bounds__(BS) k_sw_recon32(Mesh m, mfv_sw_mixed p, float
 }
 
 // time_step_SW of a float64 state with the float32 face normals (mesure = |n|), faces in
-// faceid order; (gx[s], gy[s]) = the normal of slot s (geoA).
+// faceid order; (gx[s], gy[s]) = the normal of slot s (geoA), gm[s] = its length (geoB).
 template <int MF>
-__device__ __forceinline__ double sw_dt32(const slot_f gx, const slot_f gy, int nf, double hn, double un, double vn,
-                                          double vol, float g, double cfl)
+__device__ __forceinline__ double sw_dt32(const slot_f gx, const slot_f gy, const slot_f gm, int nf, double hn,
+                                          double un, double vn, double vol, float g, double cfl)
 {
     const double velson = sqrt((double)g * hn);
     const double ih = 1. / hn;
@@ -331,7 +324,7 @@ __device__ __forceinline__ double sw_dt32(const slot_f gx, const slot_f gy, int 
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         if (s >= nf) break;
-        const double n0 = gx[s], n1 = gy[s], mesu = (double)sqrtf(gx[s] * gx[s] + gy[s] * gy[s]);
+        const double n0 = gx[s], n1 = gy[s], mesu = (double)gm[s];
         // (|u.n| / |n| + c) |n| of time_step_SW, with one division per cell
         lam += fabs((un * n0 + vn * n1) * ih) + velson * mesu;
     }
@@ -349,12 +342,14 @@ struct GlobalRecs {
 // candidate. own: this cell's index in the record array; code_of(s): neighbour code of slot s
 // in the record array's numbering. The slot data of the [MF]-wide tables is loaded in one
 // batch before the arithmetic (in bounds whatever nf).
-template <int MF, class Recs, class Codes>
+// LEN: take |n| and 1/|n| from geoB (the fused step, bound by its arithmetic) or compute them
+// from the normal (the flux kernel, bound by its memory traffic); the values are the same.
+template <int MF, bool LEN, class Recs, class Codes>
 __device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int own, const double4 si, int nf,
                                               unsigned ord, Recs rec, Codes code_of, float g, double cfl,
                                               double dt, double4 *out)
 {
-    slot_f gx = 0.f, gy = 0.f, gz = 0.f, gw = 0.f;
+    slot_f gx = 0.f, gy = 0.f, gz = 0.f, gw = 0.f, gm = 1.f, gi = 1.f;
     slot_i codes = -1;
 #pragma unroll
     for (int s = 0; s < MF; s++) {
@@ -363,16 +358,28 @@ __device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int 
         gy[s] = q.y;
         gz[s] = q.z;
         gw[s] = q.w;
+        if (LEN) {
+            const float4 b = reinterpret_cast<const float4 *>(p.geoB)[MF * (size_t)i + s];
+            gm[s] = b.z;
+            gi[s] = b.w;
+        }
         codes[s] = code_of(s);
     }
     const double vol = p.vol[i];
-    float r0 = 0.f, c0 = 0.f, r1 = 0.f, c1 = 0.f, r2 = 0.f, c2 = 0.f, s1 = 0.f, d1 = 0.f, s2 = 0.f, d2 = 0.f;
+    double r0 = 0., r1 = 0., r2 = 0., s1 = 0., s2 = 0.;
 #pragma unroll
     for (int r = 0; r < MF; r++) {
         if (r >= nf) break;
         const int s = (ord >> (2 * r)) & 3;
         const float4 ga = make_float4(gx[s], gy[s], gz[s], gw[s]);
-        const float mes = sqrtf(ga.x * ga.x + ga.y * ga.y);
+        float mes, im;
+        if (LEN) {
+            mes = gm[s];
+            im = gi[s];
+        } else {
+            mes = sqrtf(ga.x * ga.x + ga.y * ga.y);
+            im = 1.f / mes;
+        }
         const float4 ownr = rec(MF * (size_t)own + s);
         const int code = codes[s];
         // One evaluation in the face's orientation whichever side the cell is on (a wave holds
@@ -385,24 +392,28 @@ __device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int 
         const float huR = own_R ? ownr.y : (wall ? -ownr.y : oth.y);
         const float hvR = own_R ? ownr.z : (wall ? -ownr.z : oth.z);
         float F0, F1, F2, bo;
-        face_dev(L.w, L.x, own_R ? ownr.x : oth.x, ga.z, L.y, L.z, huR, hvR, ga.w, ga.x, ga.y, mes, g,
+        face_dev(L.w, L.x, own_R ? ownr.x : oth.x, ga.z, L.y, L.z, huR, hvR, ga.w, ga.x, ga.y, mes, im, g,
                  own_R ? -ownr.w : 0.f, !own_R, F0, F1, F2, bo);
-        twosum_add(r0, c0, own_R ? F0 : -F0);
-        twosum_add(r1, c1, own_R ? F1 : -F1);
-        twosum_add(r2, c2, own_R ? F2 : -F2);
+        r0 += (double)(own_R ? F0 : -F0);
+        r1 += (double)(own_R ? F1 : -F1);
+        r2 += (double)(own_R ? F2 : -F2);
         const float b1 = bo * ga.x, b2 = bo * ga.y;
-        twosum_add(s1, d1, own_R ? b1 : -b1);
-        twosum_add(s2, d2, own_R ? b2 : -b2);
+        s1 += (double)(own_R ? b1 : -b1);
+        s2 += (double)(own_R ? b2 : -b2);
     }
-    const double R0 = (double)r0 + (double)c0;
-    const double R1 = ((double)r1 + (double)c1) + ((double)s1 + (double)d1);
-    const double R2 = ((double)r2 + (double)c2) + ((double)s2 + (double)d2);
+    const double R0 = r0;
+    const double R1 = r1 + s1;
+    const double R2 = r2 + s2;
     const double dv = dt / vol;
     const double hn = si.x + R0 * dv;
     const double un = si.y + R1 * dv;
     const double vn = si.z + R2 * dv;
     *out = make_double4(hn, un, vn, si.w);
-    return sw_dt32<MF>(gx, gy, nf, hn, un, vn, vol, g, cfl);
+    if (!LEN) {
+#pragma unroll
+        for (int s = 0; s < MF; s++) gm[s] = sqrtf(gx[s] * gx[s] + gy[s] * gy[s]);
+    }
+    return sw_dt32<MF>(gx, gy, gm, nf, hn, un, vn, vol, g, cfl);
 }
 
 // The cell step on the global records, in place on p.state.
@@ -414,7 +425,7 @@ __device__ __forceinline__ double sw_flux32_cell(const Mesh &m, const mfv_sw_mix
     double4 *st = reinterpret_cast<double4 *>(p.state);
     const int32_t *__restrict__ nbr = p.nbr;
     auto code_of = [nbr, C, i](int s) { return nbr[(size_t)s * C + i]; };
-    return flux32_core<MF>(p, i, i, st[i], m.cell_nfaces[i], p.order[i],
+    return flux32_core<MF, false>(p, i, i, st[i], m.cell_nfaces[i], p.order[i],
                            GlobalRecs{reinterpret_cast<const float4 *>(p.facerec)}, code_of, g, cfl, dt, st + i);
 }
 
@@ -572,14 +583,15 @@ __global__ void __launch_bounds__(kBlock) k_sw_dt_batch(Mesh m, mfv_sw_mixed p, 
     const int inst = b.wave_inst[i >> 6];
     if (inst < 0) return;
     const double4 s = reinterpret_cast<const double4 *>(p.state)[i];
-    slot_f gx = 0.f, gy = 0.f;
+    slot_f gx = 0.f, gy = 0.f, gm = 1.f;
 #pragma unroll
     for (int k = 0; k < MF; k++) {
         const float4 q = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + k];
         gx[k] = q.x;
         gy[k] = q.y;
+        gm[k] = reinterpret_cast<const float4 *>(p.geoB)[MF * (size_t)i + k].z;
     }
-    const double dloc = sw_dt32<MF>(gx, gy, m.cell_nfaces[i], s.x, s.y, s.z, p.vol[i], g, cfl);
+    const double dloc = sw_dt32<MF>(gx, gy, gm, m.cell_nfaces[i], s.x, s.y, s.z, p.vol[i], g, cfl);
     wave_min_part(part, (size_t)i, dloc);
 }
 
@@ -787,7 +799,7 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
         auto code_of = [lnbr, C, i](int s) { return lnbr[(size_t)s * C + i]; };
         auto recs = [lfr](size_t k) { return lfr[k]; };
         const double4 si = STAGED ? s_st[q] : st[i];
-        dloc = flux32_core<MF>(p, i, q, si, m.cell_nfaces[i], p.order[i], recs, code_of, g, cfl, dt, out + i);
+        dloc = flux32_core<MF, true>(p, i, q, si, m.cell_nfaces[i], p.order[i], recs, code_of, g, cfl, dt, out + i);
     }
     wave_min_part(part, (size_t)c0 + q, dloc);
 }

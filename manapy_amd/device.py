@@ -583,7 +583,7 @@ class SwMixed(object):
         self.vol = dmesh.t['cell_volume']
         self.facerec = torch.zeros((C, MF, 4), dtype=f32, device=dev)
         self.geoA = torch.zeros((C, MF, 4), dtype=f32, device=dev)
-        self.geoB = torch.zeros((C, MF, 2), dtype=f32, device=dev)
+        self.geoB = torch.zeros((C, MF, 4), dtype=f32, device=dev)
         self.nbr = torch.as_tensor(host['nbr'], device=dev)
         self.order = torch.as_tensor(host['order'], device=dev)
         self.partial = torch.zeros(3 * ((C + 255) // 256) + 3, dtype=torch.float64, device=dev)

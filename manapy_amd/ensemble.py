@@ -153,7 +153,7 @@ class SwEnsemble(object):
         self.state2 = torch.zeros((self.C, 4), dtype=torch.float64, device=dev)
         self.facerec = torch.zeros((self.C, self.MF, 4), dtype=f32, device=dev)
         self.geoA = torch.zeros((self.C, self.MF, 4), dtype=f32, device=dev)
-        self.geoB = torch.zeros((self.C, self.MF, 2), dtype=f32, device=dev)
+        self.geoB = torch.zeros((self.C, self.MF, 4), dtype=f32, device=dev)
         p = _lib.MfvSwMixed()
         p.nb_cells, p.max_cell_faces = self.C, self.MF
         p.state, p.vol, p.facerec = self.state.data_ptr(), self.t['cell_volume'].data_ptr(), self.facerec.data_ptr()

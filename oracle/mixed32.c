@@ -3,12 +3,13 @@
  * float32 arithmetic of the mixed-precision shallow-water kernels (k_sw_pack32,
  * k_sw_recon32, k_sw_flux32 in manapy_amd/csrc/mfv_sw_mixed.hip). Compiled with
  * -ffp-contract=off, float operations round like the GPU's (IEEE division and square
- * root), so a GPU run must reproduce these results bit for bit; mixed32 results are in
- * turn compared with the float64 restatement of the reference scheme (mfv_oracle.c:
- * orc_sw_step) within the stated error.
+ * root; fused multiply-adds only where both sides call fmaf), so a GPU run must reproduce
+ * these results bit for bit; mixed32 results are in turn compared with the float64
+ * restatement of the reference scheme (mfv_oracle.c: orc_sw_step) within the stated error.
  *
  * Arrays follow the GPU layout: state [C][4] (h, hu, hv, Z), geoA [C][MF][4], geoB
- * [C][MF][2], facerec [C][MF][4], per-cell tables column-major [slot][cell].
+ * [C][MF][4] (rx, ry, |n|, 1/|n|), facerec [C][MF][4], per-cell tables column-major
+ * [slot][cell].
  */
 #include <math.h>
 #include <stddef.h>
@@ -17,17 +18,8 @@
 
 static float pmaxf_(float a, float b) { return (b > a) ? b : a; }
 
-static void twosum_add(float *s, float *c, float x)
-{
-    const float t = *s + x;
-    const float bp = t - *s;
-    const float e = (*s - (t - bp)) + (x - bp);
-    *s = t;
-    *c += e;
-}
-
 static void face_dev(float hb, float aL, float aR, float dz, float huL, float hvL, float huR, float hvR, float dzo,
-                     float n0, float n1, float mes, float g, float shift, int own_is_L, float *F0, float *F1,
+                     float n0, float n1, float mes, float im, float g, float shift, int own_is_L, float *F0, float *F1,
                      float *F2, float *bo)
 {
     float hL = hb + aL, hR = hb + aR;
@@ -57,7 +49,6 @@ static void face_dev(float hb, float aL, float aR, float dz, float huL, float hv
         vR = hvR * r;
     }
     const float qL = uL * n0 + vL * n1, qR = uR * n0 + vR * n1;
-    const float im = 1.f / mes;
     const float sL = fabsf(qL) * im + sqrtf(g * hsL), sR = fabsf(qR) * im + sqrtf(g * hsR);
     const float s = pmaxf_(sL, sR) * mes;
     const float ho = hb - shift;
@@ -105,24 +96,26 @@ void orc_mixed_pack(int C, int MF, const int32_t *cell_faces, const int32_t *cel
             ga[1] = (float)face_normal[3 * (size_t)f + 1];
             ga[2] = (float)dz;
             ga[3] = (float)(zo - Z[i]);
-            float *gb = geoB + 2 * ((size_t)MF * i + s);
+            float *gb = geoB + 4 * ((size_t)MF * i + s);
             gb[0] = (float)rx;
             gb[1] = (float)ry;
+            gb[2] = sqrtf(ga[0] * ga[0] + ga[1] * ga[1]);
+            gb[3] = 1.f / gb[2];
         }
     }
 }
 
 static void accum(float jx, float jy, float de, float du, float dv, float *a)
 {
-    a[0] += jx * jx;
-    a[1] += jy * jy;
-    a[2] += jx * jy;
-    a[3] += jx * de;
-    a[4] += jy * de;
-    a[5] += jx * du;
-    a[6] += jy * du;
-    a[7] += jx * dv;
-    a[8] += jy * dv;
+    a[0] = fmaf(jx, jx, a[0]);
+    a[1] = fmaf(jy, jy, a[1]);
+    a[2] = fmaf(jx, jy, a[2]);
+    a[3] = fmaf(jx, de, a[3]);
+    a[4] = fmaf(jy, de, a[4]);
+    a[5] = fmaf(jx, du, a[5]);
+    a[6] = fmaf(jy, du, a[6]);
+    a[7] = fmaf(jx, dv, a[7]);
+    a[8] = fmaf(jy, dv, a[8]);
 }
 
 void orc_mixed_recon(int C, int MF, const double *state, const double *cell_center, const int32_t *lsq_cells,
@@ -182,8 +175,8 @@ void orc_mixed_recon(int C, int MF, const double *state, const double *cell_cent
         }
         float pe = 1.f, pu = 1.f, pv = 1.f;
         for (int s = 0; s < MF; s++) {
-            rx[s] = geoB[2 * ((size_t)MF * i + s)];
-            ry[s] = geoB[2 * ((size_t)MF * i + s) + 1];
+            rx[s] = geoB[4 * ((size_t)MF * i + s)];
+            ry[s] = geoB[4 * ((size_t)MF * i + s) + 1];
             if (s >= nf) continue;
             float d = ex * rx[s] + ey * ry[s];
             if (!(fabsf(d) < 1e-8f)) pe = fminf(pe, d > 0.f ? emax / d : emin / d);
@@ -210,52 +203,53 @@ void orc_mixed_recon(int C, int MF, const double *state, const double *cell_cent
 /* One step (fluxes + float64 update) of every cell in place; returns the next dt (the minimum
  * of the per-cell candidates, as the GPU's wavefront minimum). */
 double orc_mixed_flux(int C, int MF, double *state, const double *vol, const float *facerec, const float *geoA,
-                      const int32_t *cell_nfaces, const int32_t *nbr, const uint8_t *order, float g, double cfl,
-                      double dt)
+                      const float *geoB, const int32_t *cell_nfaces, const int32_t *nbr, const uint8_t *order,
+                      float g, double cfl, double dt)
 {
     double dmin = 1e300;
     for (int i = 0; i < C; i++) {
         double *si = state + 4 * (size_t)i;
         const int nf = cell_nfaces[i];
         const unsigned ord = order[i];
-        float r0 = 0.f, c0 = 0.f, r1 = 0.f, c1 = 0.f, r2 = 0.f, c2 = 0.f, s1 = 0.f, d1 = 0.f, s2 = 0.f, d2 = 0.f;
+        double r0 = 0., r1 = 0., r2 = 0., s1 = 0., s2 = 0.;
         for (int r = 0; r < MF && r < nf; r++) {
             const int s = (ord >> (2 * r)) & 3;
             const float *ga = geoA + 4 * ((size_t)MF * i + s);
-            const float mes = sqrtf(ga[0] * ga[0] + ga[1] * ga[1]);
+            const float *gb = geoB + 4 * ((size_t)MF * i + s);
+            const float mes = gb[2], im = gb[3];
             const float *own = facerec + 4 * ((size_t)MF * i + s);
             const int code = nbr[(size_t)s * C + i];
             float F0, F1, F2, bo;
             if (code < 0) {
-                face_dev(own[3], own[0], own[0], 0.f, own[1], own[2], -own[1], -own[2], ga[3], ga[0], ga[1], mes, g,
+                face_dev(own[3], own[0], own[0], 0.f, own[1], own[2], -own[1], -own[2], ga[3], ga[0], ga[1], mes, im, g,
                          0.f, 1, &F0, &F1, &F2, &bo);
             } else {
                 const float *oth = facerec + 4 * ((size_t)MF * (code >> 3) + ((code >> 1) & 3));
                 if ((code & 1) == 0)
                     face_dev(own[3], own[0], oth[0], ga[2], own[1], own[2], oth[1], oth[2], ga[3], ga[0], ga[1], mes,
-                             g, 0.f, 1, &F0, &F1, &F2, &bo);
+                             im, g, 0.f, 1, &F0, &F1, &F2, &bo);
                 else
                     face_dev(oth[3], oth[0], own[0], ga[2], oth[1], oth[2], own[1], own[2], ga[3], ga[0], ga[1], mes,
-                             g, -own[3], 0, &F0, &F1, &F2, &bo);
+                             im, g, -own[3], 0, &F0, &F1, &F2, &bo);
             }
             if (code < 0 || (code & 1) == 0) {
-                twosum_add(&r0, &c0, -F0);
-                twosum_add(&r1, &c1, -F1);
-                twosum_add(&r2, &c2, -F2);
-                twosum_add(&s1, &d1, -(bo * ga[0]));
-                twosum_add(&s2, &d2, -(bo * ga[1]));
+                r0 += (double)-F0;
+                r1 += (double)-F1;
+                r2 += (double)-F2;
+                s1 += (double)-(bo * ga[0]);
+                s2 += (double)-(bo * ga[1]);
             } else {
-                twosum_add(&r0, &c0, F0);
-                twosum_add(&r1, &c1, F1);
-                twosum_add(&r2, &c2, F2);
-                twosum_add(&s1, &d1, bo * ga[0]);
-                twosum_add(&s2, &d2, bo * ga[1]);
+                r0 += (double)F0;
+                r1 += (double)F1;
+                r2 += (double)F2;
+                s1 += (double)(bo * ga[0]);
+                s2 += (double)(bo * ga[1]);
             }
         }
         const double v = vol[i];
-        const double R0 = (double)r0 + (double)c0;
-        const double R1 = ((double)r1 + (double)c1) + ((double)s1 + (double)d1);
-        const double R2 = ((double)r2 + (double)c2) + ((double)s2 + (double)d2);
+        const double R0 = r0;
+        const double R1 = r1 + s1;
+        const double R2 = r2 + s2;
         const double dv = dt / v;
         const double hn = si[0] + R0 * dv;
         const double un = si[1] + R1 * dv;

@@ -373,7 +373,7 @@ class MixedModel(object):
         C, MF = self.C, self.MF
         self.state = np.zeros((C, 4))
         self.geoA = np.zeros((C, MF, 4), np.float32)
-        self.geoB = np.zeros((C, MF, 2), np.float32)
+        self.geoB = np.zeros((C, MF, 4), np.float32)
         self.facerec = np.zeros((C, MF, 4), np.float32)
         self.vol = np.ascontiguousarray(d.cells.volume, dtype=np.float64)
         self.om = float(order - 1)
@@ -396,5 +396,6 @@ class MixedModel(object):
                               f64(L['face_ghost']), i32(L['cell_nfaces']), i32(Pk['nbr']), _p(self.geoA, np.float32),
                               _p(self.geoB, np.float32), ctypes.c_float(self.om), _p(self.facerec, np.float32))
         return lib().orc_mixed_flux(_I(self.C), _I(self.MF), f64(self.state), f64(self.vol),
-                                    _p(self.facerec, np.float32), _p(self.geoA, np.float32), i32(L['cell_nfaces']),
+                                    _p(self.facerec, np.float32), _p(self.geoA, np.float32),
+                                    _p(self.geoB, np.float32), i32(L['cell_nfaces']),
                                     i32(Pk['nbr']), _p(Pk['order'], np.uint8), ctypes.c_float(g), _D(cfl), _D(dt))
