@@ -56,6 +56,22 @@ __device__ __forceinline__ void wave_min_part(double *__restrict__ part, size_t 
     if ((threadIdx.x & 63) == 0) part[gid >> 6] = m;
 }
 
+// Minimum and maximum of v over the wavefront, as wave-uniform values (for branches every
+// lane takes the same way).
+__device__ __forceinline__ int wave_min_int(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ int wave_max_int(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Cells at positions e and e + T of a tile's ext order (its nc cells from c0, then
 // ext[e0 .. e0 + ne)), with both table loads issued before either is used. Past the end,
 // the second is the first again (writing it twice is harmless).

@@ -595,15 +595,22 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
         for (int j = 0; j < KMAX; j++) cidx[j] = (j < m.K) ? A.stencil(j) : 0;
 #pragma unroll
         for (int j = 0; j < KMAX; j++) cidx[j] = (j < nn) ? cidx[j] : A.self();
+        // Every group's entries are loaded (entries past nn read the own cell, a valid handle)
+        // and only the accumulation depends on nn: conditional loads made the compiler copy
+        // registers for the entries they may skip. Groups past the wave's largest stencil are
+        // skipped; the scheduling barrier keeps the next group's loads from being hoisted
+        // (which would hold all of them in registers).
+        const int nn_max = wave_max_int(nn);
 #pragma unroll
         for (int j0 = 0; j0 < KMAX; j0 += 4) {
+            if (j0 >= nn_max) break;
             double px[4], py[4], ec[4], uc[4], vc[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (j0 + k < nn) A.ehv(cidx[j0 + k], px[k], py[k], ec[k], uc[k], vc[k]);
+            for (int k = 0; k < 4; k++) A.ehv(cidx[j0 + k], px[k], py[k], ec[k], uc[k], vc[k]);
 #pragma unroll
             for (int k = 0; k < 4; k++)
                 if (j0 + k < nn) accum(px[k], py[k], ec[k], uc[k], vc[k]);
+            __builtin_amdgcn_sched_barrier(0);
         }
     } else {
         for (int j = 0; j < nn; j++) {

@@ -189,19 +189,25 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
 #pragma unroll
         for (int j = 0; j < KMAX; j++) h[j] = (j < nn) ? h[j] : a.self();
         constexpr int CH = Cells::kChunk;
+        // Every group's entries are loaded (entries past nn read the own cell, a valid handle)
+        // and only the accumulation depends on nn: conditional loads made the compiler copy
+        // registers for the entries they may skip. Groups past the wave's largest stencil are
+        // skipped; the scheduling barrier keeps the next group's loads from being hoisted.
+        const int nn_max = wave_max_int(nn);
 #pragma unroll
         for (int j0 = 0; j0 < KMAX; j0 += CH) {
+            if (j0 >= nn_max) break;
             double4 sc[CH];
             double2 cc[CH];
 #pragma unroll
-            for (int k = 0; k < CH; k++)
-                if (j0 + k < nn) {
-                    sc[k] = a.state(h[j0 + k]);
-                    cc[k] = a.center(h[j0 + k]);
-                }
+            for (int k = 0; k < CH; k++) {
+                sc[k] = a.state(h[j0 + k]);
+                cc[k] = a.center(h[j0 + k]);
+            }
 #pragma unroll
             for (int k = 0; k < CH; k++)
                 if (j0 + k < nn) accum(sc[k], cc[k]);
+            __builtin_amdgcn_sched_barrier(0);
         }
     } else {
         for (int j = 0; j < nn; j++) {
