@@ -737,32 +737,30 @@ __global__ void __launch_bounds__(kBlock) k_sw_recon(Mesh m, mfv_sw_packed p, do
         }
 }
 
-// Face values of the flux gather: own(s) gives this cell's (eta_f, hu_f) and hv_f at slot s,
-// across(code) those of the other side of the face (code = its neighbour code in the numbering
-// the values are stored in), zpair(s, code) the z_f of both sides (own z_f on a wall).
-// GlobalFaceVals: the facerec array (codes are global); StagedFaceVals: a block's LDS face
-// values without z_f (codes are block-local), z_f of both sides from the static cfrec.
+// Face values of the flux gather, addressed by face-value index: own_idx(s) is this cell's
+// slot s, nb_idx(code) the other side of a face (code = its neighbour code in the numbering
+// the values are stored in); vals(k) gives (eta_f, hu_f) and hv_f of index k; zpair(s, code)
+// the z_f of both sides of slot s (own z_f on a wall). GlobalFaceVals: the facerec array
+// (codes are global); StagedFaceVals: a block's LDS face values without z_f (codes are
+// block-local), z_f of both sides from the static cfrec.
 struct GlobalFaceVals {
     const double *__restrict__ fr;
     int i;
-    __device__ __forceinline__ void at(size_t k, double2 &eu, double &v) const
+    template <int MF>
+    __device__ __forceinline__ int own_idx(int s) const { return MF * i + s; }
+    template <int MF>
+    __device__ __forceinline__ int nb_idx(int code) const { return MF * (code >> 3) + ((code >> 1) & 3); }
+    __device__ __forceinline__ void vals(int k, double2 &eu, double &v) const
     {
-        const double2 *q = reinterpret_cast<const double2 *>(fr + 4 * k);
+        const double2 *q = reinterpret_cast<const double2 *>(fr + 4 * (size_t)k);
         eu = q[0];
         v = q[1].x;
-    }
-    template <int MF>
-    __device__ __forceinline__ void own(int s, double2 &eu, double &v) const { at(MF * (size_t)i + s, eu, v); }
-    template <int MF>
-    __device__ __forceinline__ void across(int code, double2 &eu, double &v) const
-    {
-        at(MF * (size_t)(code >> 3) + ((code >> 1) & 3), eu, v);
     }
     template <int MF>
     __device__ __forceinline__ double2 zpair(int s, int code) const
     {
         const double zo = fr[4 * (MF * (size_t)i + s) + 3];
-        return make_double2(zo, code >= 0 ? fr[4 * (MF * (size_t)(code >> 3) + ((code >> 1) & 3)) + 3] : zo);
+        return make_double2(zo, code >= 0 ? fr[4 * (size_t)nb_idx<MF>(code) + 3] : zo);
     }
 };
 
@@ -772,15 +770,11 @@ struct StagedFaceVals {
     const double *__restrict__ cfrec;
     int i, q;
     template <int MF>
-    __device__ __forceinline__ void own(int s, double2 &a, double &b) const
-    {
-        a = eu[MF * q + s];
-        b = v[MF * q + s];
-    }
+    __device__ __forceinline__ int own_idx(int s) const { return MF * q + s; }
     template <int MF>
-    __device__ __forceinline__ void across(int code, double2 &a, double &b) const
+    __device__ __forceinline__ int nb_idx(int code) const { return MF * (code >> 3) + ((code >> 1) & 3); }
+    __device__ __forceinline__ void vals(int k, double2 &a, double &b) const
     {
-        const int k = MF * (code >> 3) + ((code >> 1) & 3);
         a = eu[k];
         b = v[k];
     }
@@ -794,66 +788,75 @@ struct StagedFaceVals {
 // One cell of the step: fluxes + bathymetry terms + update_SW from the record rin, new eta, hu,
 // hv, h written into rout (rout may be rin); returns the next-dt candidate. fv: where the face
 // values come from; code_of(s): the neighbour code of slot s in fv's numbering. The slot data
-// of the [MF]-wide tables (in bounds whatever nf) is loaded in one batch before the arithmetic.
+// of the [MF]-wide tables (in bounds whatever nf) is loaded in one batch before the arithmetic:
+// in the residual order for the fluxes and, for the dt, the geometry again in faceid order.
 template <int MF, class FV, class Codes>
 __device__ __forceinline__ double sw_flux_packed_core(const mfv_sw_packed &p, int i, const double *__restrict__ rin,
                                                       double *rout, const FV &fv, Codes code_of, int nf,
                                                       unsigned ord, double g, double cfl, double dt)
 {
-    slot_i codes = -1;
-    slot_d gn0 = 0., gn1 = 0., gms = 1., zo = 0., zn = 0.;
+    int sl[MF], cd[MF];
+    double gn0[MF], gn1[MF], gms[MF], zo[MF], zn[MF];
 #pragma unroll
-    for (int s = 0; s < MF; s++) {
-        codes[s] = code_of(s);
+    for (int r = 0; r < MF; r++) {  // slot s of the r-th contribution is in bits 2r..2r+1
+        const int s = min((int)((ord >> (2 * r)) & 3), MF - 1);
+        sl[r] = s;
+        cd[r] = code_of(s);
         const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
         const double2 ga = gq[0], gb = gq[1];
-        gn0[s] = ga.x;
-        gn1[s] = ga.y;
-        gms[s] = gb.x;
+        gn0[r] = ga.x;
+        gn1[r] = ga.y;
+        gms[r] = gb.x;
     }
 #pragma unroll
+    for (int r = 0; r < MF; r++) {
+        const double2 z = fv.template zpair<MF>(sl[r], cd[r]);
+        zo[r] = z.x;
+        zn[r] = z.y;
+    }
+    // the dt's geometry in faceid order, loaded with the rest (after the record store a load
+    // could not be moved above it)
+    double dn0[MF], dn1[MF], dms[MF];
+#pragma unroll
     for (int s = 0; s < MF; s++) {
-        const double2 z = fv.template zpair<MF>(s, codes[s]);
-        zo[s] = z.x;
-        zn[s] = z.y;
+        const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
+        dn0[s] = gq[0].x;
+        dn1[s] = gq[0].y;
+        dms[s] = gq[1].x;
     }
     const double2 *r2 = reinterpret_cast<const double2 *>(rin);
     double2 c1 = r2[1], c2 = r2[2], c3 = r2[3];
     const double hu_i = c1.y, hv_i = c2.x, h_i = c2.y, Z_i = c3.x, vol = c3.y;
     double r0 = 0., r1 = 0., rr2 = 0., s1 = 0., s2 = 0.;
-    // contributions in the residual order; slot s of the r-th one is in bits 2r..2r+1
 #pragma unroll
     for (int r = 0; r < MF; r++) {
         if (r >= nf) break;
-        const int s = (ord >> (2 * r)) & 3;
-        const double n0 = gn0[s], n1 = gn1[s], mes = gms[s];
-        const int code = codes[s];
-        double2 oa;
-        double ov;
-        fv.template own<MF>(s, oa, ov);
-        double eN, uN, vN;  // the other side (the mirrored own values on a wall)
-        if (code >= 0) {
-            double2 na;
-            fv.template across<MF>(code, na, vN);
-            eN = na.x; uN = na.y;
-        } else {
-            eN = oa.x; uN = -1 * oa.y; vN = -1 * ov;
-        }
-        const double zO = zo[s], zN = zn[s];
+        const double n0 = gn0[r], n1 = gn1[r], mes = gms[r];
+        const int code = cd[r];
         // One evaluation in the face's orientation whichever side the cell is on (a wave holds
-        // cells of both sides, so two branches would each run for the whole wave); the
-        // operands, the flux and the own side's bathymetry term are those of the two-sided
-        // form: the own side's term is bL (own = L) or bR (own = R), and a - b = a + (-b).
-        const bool own_R = code >= 0 && (code & 1);
+        // cells of both sides, so two branches would each run for the whole wave): the L and R
+        // face values are read at selected indices, and on a wall R is the own value with the
+        // momenta negated (-1 * x, as the two-sided form). The own side's bathymetry term is
+        // bL (own = L) or bR (own = R); the signs are exact multiplications by -1 or 1.
+        const bool wall = code < 0, own_R = !wall && (code & 1);
+        const int ko = fv.template own_idx<MF>(sl[r]);
+        const int kn = wall ? ko : fv.template nb_idx<MF>(code);
+        double2 eL, eR;
+        double vL, vR;
+        fv.vals(own_R ? kn : ko, eL, vL);
+        fv.vals(own_R ? ko : kn, eR, vR);
+        const double mR = wall ? -1. : 1.;
+        const double zO = zo[r], zN = zn[r];
         double F0, F1, F2, bL, bR;
-        sw_face(own_R ? eN : oa.x, own_R ? uN : oa.y, own_R ? vN : ov, own_R ? zN : zO, h_i, Z_i,
-                own_R ? oa.x : eN, own_R ? oa.y : uN, own_R ? ov : vN, own_R ? zO : zN, h_i, Z_i, n0, n1, mes, g,
-                F0, F1, F2, bL, bR);
-        r0 += own_R ? F0 : -F0;
-        r1 += own_R ? F1 : -F1;
-        rr2 += own_R ? F2 : -F2;
-        s1 -= own_R ? -(bR * n0) : bL * n0;
-        s2 -= own_R ? -(bR * n1) : bL * n1;
+        sw_face(eL.x, eL.y, vL, own_R ? zN : zO, h_i, Z_i, eR.x, mR * eR.y, mR * vR, own_R ? zO : zN, h_i, Z_i, n0,
+                n1, mes, g, F0, F1, F2, bL, bR);
+        const double sg = own_R ? 1. : -1.;  // rez -= F (own = L) is rez += -1 * F
+        r0 += sg * F0;
+        r1 += sg * F1;
+        rr2 += sg * F2;
+        const double b = own_R ? bR : bL;  // s -= b n (own = L), s -= -(b n) (own = R)
+        s1 -= -sg * (b * n0);
+        s2 -= -sg * (b * n1);
     }
     const double s0 = 0.;
     double hn = h_i + dt * (r0 + s0) / vol;
@@ -872,7 +875,7 @@ __device__ __forceinline__ double sw_flux_packed_core(const mfv_sw_packed &p, in
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         if (s >= nf) break;
-        double n0 = gn0[s], n1 = gn1[s], mesu = gms[s];
+        const double n0 = dn0[s], n1 = dn1[s], mesu = dms[s];
         double u_n = fabs(un / hn * n0 + vn / hn * n1);
         double lam_convect = u_n / mesu + velson;
         lam += lam_convect * mesu;

@@ -355,57 +355,66 @@ __device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int 
                                               unsigned ord, Recs rec, Codes code_of, float g, double cfl,
                                               double dt, double4 *out)
 {
-    slot_f gx = 0.f, gy = 0.f, gz = 0.f, gw = 0.f, gm = 1.f, gi = 1.f;
-    slot_i codes = -1;
+    // slot data in the residual order (slot s of the r-th contribution in bits 2r..2r+1)
+    int sl[MF], cd[MF];
+    float4 gr[MF];
+    float gm[MF], gi[MF];
+#pragma unroll
+    for (int r = 0; r < MF; r++) {
+        const int s = min((int)((ord >> (2 * r)) & 3), MF - 1);
+        sl[r] = s;
+        cd[r] = code_of(s);
+        gr[r] = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
+        if (LEN) {
+            const float4 b = reinterpret_cast<const float4 *>(p.geoB)[MF * (size_t)i + s];
+            gm[r] = b.z;
+            gi[r] = b.w;
+        }
+    }
+    // the dt's geometry in faceid order, loaded with the rest (after the state store a load
+    // could not be moved above it)
+    slot_f gx = 0.f, gy = 0.f, gl = 1.f;
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         const float4 q = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
         gx[s] = q.x;
         gy[s] = q.y;
-        gz[s] = q.z;
-        gw[s] = q.w;
-        if (LEN) {
-            const float4 b = reinterpret_cast<const float4 *>(p.geoB)[MF * (size_t)i + s];
-            gm[s] = b.z;
-            gi[s] = b.w;
-        }
-        codes[s] = code_of(s);
+        gl[s] = LEN ? reinterpret_cast<const float4 *>(p.geoB)[MF * (size_t)i + s].z : sqrtf(q.x * q.x + q.y * q.y);
     }
     const double vol = p.vol[i];
     double r0 = 0., r1 = 0., r2 = 0., s1 = 0., s2 = 0.;
 #pragma unroll
     for (int r = 0; r < MF; r++) {
         if (r >= nf) break;
-        const int s = (ord >> (2 * r)) & 3;
-        const float4 ga = make_float4(gx[s], gy[s], gz[s], gw[s]);
+        const float4 ga = gr[r];
         float mes, im;
         if (LEN) {
-            mes = gm[s];
-            im = gi[s];
+            mes = gm[r];
+            im = gi[r];
         } else {
             mes = sqrtf(ga.x * ga.x + ga.y * ga.y);
             im = 1.f / mes;
         }
-        const float4 ownr = rec(MF * (size_t)own + s);
-        const int code = codes[s];
+        const int code = cd[r];
         // One evaluation in the face's orientation whichever side the cell is on (a wave holds
-        // cells of both sides, so per-side branches would each run for the whole wave). On a
-        // wall the other side is the own record mirrored (momenta negated) and dz = 0 (geoA
-        // holds 0 there).
+        // cells of both sides, so per-side branches would each run for the whole wave): the L
+        // and R records are read at selected indices. On a wall the other side is the own
+        // record with the momenta negated (-1 * x) and dz = 0 (geoA holds 0 there). The signs
+        // of the contributions are exact multiplications by -1 or 1.
         const bool wall = code < 0, own_R = !wall && (code & 1);
-        const float4 oth = wall ? ownr : rec(MF * (size_t)(code >> 3) + ((code >> 1) & 3));
-        const float4 L = own_R ? oth : ownr;
-        const float huR = own_R ? ownr.y : (wall ? -ownr.y : oth.y);
-        const float hvR = own_R ? ownr.z : (wall ? -ownr.z : oth.z);
+        const size_t ko = MF * (size_t)own + sl[r];
+        const size_t kn = wall ? ko : MF * (size_t)(code >> 3) + ((code >> 1) & 3);
+        const float4 L = rec(own_R ? kn : ko), R = rec(own_R ? ko : kn);
+        const float mR = wall ? -1.f : 1.f;
         float F0, F1, F2, bo;
-        face_dev(L.w, L.x, own_R ? ownr.x : oth.x, ga.z, L.y, L.z, huR, hvR, ga.w, ga.x, ga.y, mes, im, g,
-                 own_R ? -ownr.w : 0.f, !own_R, F0, F1, F2, bo);
-        r0 += (double)(own_R ? F0 : -F0);
-        r1 += (double)(own_R ? F1 : -F1);
-        r2 += (double)(own_R ? F2 : -F2);
-        const float b1 = bo * ga.x, b2 = bo * ga.y;
-        s1 += (double)(own_R ? b1 : -b1);
-        s2 += (double)(own_R ? b2 : -b2);
+        face_dev(L.w, L.x, R.x, ga.z, L.y, L.z, mR * R.y, mR * R.z, ga.w, ga.x, ga.y, mes, im, g,
+                 own_R ? -R.w : 0.f, !own_R, F0, F1, F2, bo);
+        const float sg = own_R ? 1.f : -1.f;
+        r0 += (double)(sg * F0);
+        r1 += (double)(sg * F1);
+        r2 += (double)(sg * F2);
+        s1 += (double)(sg * (bo * ga.x));
+        s2 += (double)(sg * (bo * ga.y));
     }
     const double R0 = r0;
     const double R1 = r1 + s1;
@@ -415,11 +424,7 @@ __device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int 
     const double un = si.y + R1 * dv;
     const double vn = si.z + R2 * dv;
     *out = make_double4(hn, un, vn, si.w);
-    if (!LEN) {
-#pragma unroll
-        for (int s = 0; s < MF; s++) gm[s] = sqrtf(gx[s] * gx[s] + gy[s] * gy[s]);
-    }
-    return sw_dt32<MF>(gx, gy, gm, nf, hn, un, vn, vol, g, cfl);
+    return sw_dt32<MF>(gx, gy, gl, nf, hn, un, vn, vol, g, cfl);
 }
 
 // The cell step on the global records, in place on p.state.
