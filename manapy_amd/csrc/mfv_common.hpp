@@ -32,7 +32,6 @@ inline unsigned grid_for(int n, int block = kBlock) { return (unsigned)((n + blo
 // Per-face-slot values of one cell (MF <= 4) as native vectors: element s with a
 // lane-dependent s compiles to a select chain, where an array would be put in scratch memory.
 typedef float slot_f __attribute__((ext_vector_type(4)));
-typedef double slot_d __attribute__((ext_vector_type(4)));
 typedef int slot_i __attribute__((ext_vector_type(4)));
 
 // Python min(a, b) / max(a, b): the second argument wins only on a strict comparison.
@@ -56,15 +55,8 @@ __device__ __forceinline__ void wave_min_part(double *__restrict__ part, size_t 
     if ((threadIdx.x & 63) == 0) part[gid >> 6] = m;
 }
 
-// Minimum and maximum of v over the wavefront, as wave-uniform values (for branches every
-// lane takes the same way).
-__device__ __forceinline__ int wave_min_int(int v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return __builtin_amdgcn_readfirstlane(v);
-}
-
+// Maximum of v over the wavefront, as a wave-uniform value (for branches every lane takes the
+// same way).
 __device__ __forceinline__ int wave_max_int(int v)
 {
 #pragma unroll
