@@ -32,6 +32,7 @@ inline unsigned grid_for(int n, int block = kBlock) { return (unsigned)((n + blo
 // Per-face-slot values of one cell (MF <= 4) as native vectors: element s with a
 // lane-dependent s compiles to a select chain, where an array would be put in scratch memory.
 typedef float slot_f __attribute__((ext_vector_type(4)));
+typedef double slot_d __attribute__((ext_vector_type(4)));
 typedef int slot_i __attribute__((ext_vector_type(4)));
 
 // Python min(a, b) / max(a, b): the second argument wins only on a strict comparison.

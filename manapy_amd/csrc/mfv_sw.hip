@@ -557,7 +557,7 @@ struct PackedStaged {
 // face slots; returns nf.
 template <int MF, int KMAX, class Acc>
 __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed &p, const Acc &A, int i,
-                                             double om, double2 *out)
+                                             double om, double2 *out, double2 *zpair = nullptr)
 {
     const size_t C = m.C;
     // Loads are issued in batches ahead of the arithmetic: the per-cell counts, the stencil
@@ -718,6 +718,10 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
         out[2 * s] = a;
         out[2 * s + 1] = b;
     }
+    if (zpair) {  // z_f of both sides of every slot (cfrec[i][s][2..3]), for a caller that keeps them
+#pragma unroll
+        for (int s = 0; s < MF; s++) zpair[s] = cfb[s];
+    }
     return nf;
 }
 
@@ -767,8 +771,8 @@ struct GlobalFaceVals {
 struct StagedFaceVals {
     const double2 *eu;  // [items][MF] (eta_f, hu_f)
     const double *v;    // [items][MF] hv_f
-    const double *__restrict__ cfrec;
-    int i, q;
+    slot_d zo, zn;      // z_f of both sides of the cell's slots, kept in registers since phase A
+    int q;
     template <int MF>
     __device__ __forceinline__ int own_idx(int s) const { return MF * q + s; }
     template <int MF>
@@ -781,17 +785,29 @@ struct StagedFaceVals {
     template <int MF>
     __device__ __forceinline__ double2 zpair(int s, int) const
     {
-        return reinterpret_cast<const double2 *>(cfrec + 4 * (MF * (size_t)i + s))[1];
+        return make_double2(zo[s], zn[s]);
     }
 };
 
-// One cell of the step: fluxes + bathymetry terms + update_SW from the record rin, new eta, hu,
-// hv, h written into rout (rout may be rin); returns the next-dt candidate. fv: where the face
+// The float64 cell values of the update: hu, hv, h, Z, vol of the cell record.
+struct CellVals {
+    double hu, hv, h, Z, vol;
+};
+
+__device__ __forceinline__ CellVals cell_vals(const double *__restrict__ rec)
+{
+    const double2 *r2 = reinterpret_cast<const double2 *>(rec);
+    const double2 c1 = r2[1], c2 = r2[2], c3 = r2[3];
+    return CellVals{c1.y, c2.x, c2.y, c3.x, c3.y};
+}
+
+// One cell of the step: fluxes + bathymetry terms + update_SW from the cell values cv, new eta,
+// hu, hv, h written into the record rout; returns the next-dt candidate. fv: where the face
 // values come from; code_of(s): the neighbour code of slot s in fv's numbering. The slot data
 // of the [MF]-wide tables (in bounds whatever nf) is loaded in one batch before the arithmetic:
 // in the residual order for the fluxes and, for the dt, the geometry again in faceid order.
 template <int MF, class FV, class Codes>
-__device__ __forceinline__ double sw_flux_packed_core(const mfv_sw_packed &p, int i, const double *__restrict__ rin,
+__device__ __forceinline__ double sw_flux_packed_core(const mfv_sw_packed &p, int i, const CellVals cv,
                                                       double *rout, const FV &fv, Codes code_of, int nf,
                                                       unsigned ord, double g, double cfl, double dt)
 {
@@ -824,9 +840,7 @@ __device__ __forceinline__ double sw_flux_packed_core(const mfv_sw_packed &p, in
         dn1[s] = gq[0].y;
         dms[s] = gq[1].x;
     }
-    const double2 *r2 = reinterpret_cast<const double2 *>(rin);
-    double2 c1 = r2[1], c2 = r2[2], c3 = r2[3];
-    const double hu_i = c1.y, hv_i = c2.x, h_i = c2.y, Z_i = c3.x, vol = c3.y;
+    const double hu_i = cv.hu, hv_i = cv.hv, h_i = cv.h, Z_i = cv.Z, vol = cv.vol;
     double r0 = 0., r1 = 0., rr2 = 0., s1 = 0., s2 = 0.;
 #pragma unroll
     for (int r = 0; r < MF; r++) {
@@ -898,7 +912,7 @@ __global__ void __launch_bounds__(kBlock) k_sw_flux_packed(Mesh m, mfv_sw_packed
         double *rec = p.cellrec + 8 * (size_t)i;
         const int32_t *__restrict__ nbr = p.nbr;
         auto code_of = [nbr, C, i](int s) { return nbr[(size_t)s * C + i]; };
-        dloc = sw_flux_packed_core<MF>(p, i, rec, rec, GlobalFaceVals{p.facerec, i}, code_of, m.cell_nfaces[i],
+        dloc = sw_flux_packed_core<MF>(p, i, cell_vals(rec), rec, GlobalFaceVals{p.facerec, i}, code_of, m.cell_nfaces[i],
                                        p.order[i], g, cfl, ring.cur[0]);
     }
     wave_min_part(dtpart, (size_t)i, dloc);
@@ -965,20 +979,33 @@ __global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packe
         s_v[eb] = b2.x;
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < nc + nh; q += T) {
-        const int c = (q < nc) ? c0 + q : t.ext[e0 + (q - nc)];
+    // Phase A. A tile cell is reconstructed and updated by the same thread (q = threadIdx.x is
+    // in the first pass), which keeps the cell's z_f pairs and its h, Z, vol (loaded while the
+    // staging reads the same record lines) in registers for phase B.
+    const int q = threadIdx.x;
+    double2 zp[MF];
+    double hZv[3] = {0., 0., 0.};
+    if (q < nc) {
+        const double *rc = p.cellrec + 8 * (size_t)(c0 + q);
+        hZv[0] = rc[5];
+        hZv[1] = rc[6];
+        hZv[2] = rc[7];
+    }
+    auto recon_item = [&](int qq, double2 *keep) {
+        const int c = (qq < nc) ? c0 + qq : t.ext[e0 + (qq - nc)];
         double2 rec[2 * MF];
         const int nf = sw_recon_core<MF, KMAX>(
-            m, p, PackedStaged{s_xy, s_eu, s_v, t.lst, t.lgh, t.lfn, r0 + q, (size_t)t.I, q}, c, om, rec);
+            m, p, PackedStaged{s_xy, s_eu, s_v, t.lst, t.lgh, t.lfn, r0 + qq, (size_t)t.I, qq}, c, om, rec, keep);
 #pragma unroll
         for (int s = 0; s < MF; s++)
             if (s < nf) {
-                f_eu[MF * q + s] = rec[2 * s];
-                f_v[MF * q + s] = rec[2 * s + 1].x;
+                f_eu[MF * qq + s] = rec[2 * s];
+                f_v[MF * qq + s] = rec[2 * s + 1].x;
             }
-    }
+    };
+    if (q < nc + nh) recon_item(q, zp);
+    for (int qq = q + T; qq < nc + nh; qq += T) recon_item(qq, nullptr);  // items beyond one per thread
     __syncthreads();
-    const int q = threadIdx.x;
     if ((q & ~63) >= nc) return;  // the whole wave lies past the tile: it owns no partial
     double dloc = 1e300;
     if (q < nc) {
@@ -986,9 +1013,15 @@ __global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packe
         const int32_t *__restrict__ lnbr = t.lnbr;
         const size_t C = m.C;
         auto code_of = [lnbr, C, i](int s) { return lnbr[(size_t)s * C + i]; };
-        dloc = sw_flux_packed_core<MF>(p, i, p.cellrec + 8 * (size_t)i, rec_out + 8 * (size_t)i,
-                                       StagedFaceVals{f_eu, f_v, p.cfrec, i, q}, code_of, m.cell_nfaces[i],
-                                       p.order[i], g, cfl, ring.cur[0]);
+        StagedFaceVals fv{f_eu, f_v, 0., 0., q};
+#pragma unroll
+        for (int s = 0; s < MF; s++) {
+            fv.zo[s] = zp[s].x;
+            fv.zn[s] = zp[s].y;
+        }
+        const CellVals cv{s_eu[q].y, s_v[q], hZv[0], hZv[1], hZv[2]};
+        dloc = sw_flux_packed_core<MF>(p, i, cv, rec_out + 8 * (size_t)i, fv, code_of, m.cell_nfaces[i], p.order[i],
+                                       g, cfl, ring.cur[0]);
     }
     wave_min_part(dtpart, (size_t)c0 + q, dloc);
 }
