@@ -130,6 +130,36 @@ __device__ __forceinline__ void accum_grad(float jx, float jy, float de, float d
     jv1 = fmaf(jy, dv, jv1);
 }
 
+// Static geometry of a cell's face slots in faceid order (geoA and the |n|, 1/|n| of geoB), as
+// native vectors: element s with a lane-dependent s is a select chain.
+struct SlotGeo {
+    slot_f nx, ny, dz, dzo, len, ilen;
+};
+
+// LEN: |n| and 1/|n| from geoB, else computed from the normal (the same float32 expressions).
+template <int MF, bool LEN>
+__device__ __forceinline__ SlotGeo load_slot_geo(const mfv_sw_mixed &p, int i)
+{
+    SlotGeo G{0.f, 0.f, 0.f, 0.f, 1.f, 1.f};
+#pragma unroll
+    for (int s = 0; s < MF; s++) {
+        const float4 q = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
+        G.nx[s] = q.x;
+        G.ny[s] = q.y;
+        G.dz[s] = q.z;
+        G.dzo[s] = q.w;
+        if (LEN) {
+            const float4 b = reinterpret_cast<const float4 *>(p.geoB)[MF * (size_t)i + s];
+            G.len[s] = b.z;
+            G.ilen[s] = b.w;
+        } else {
+            G.len[s] = sqrtf(q.x * q.x + q.y * q.y);
+            G.ilen[s] = 1.f / G.len[s];
+        }
+    }
+    return G;
+}
+
 // Reconstruction of cell i: float32 LSQ gradients of eta, hu, hv from float64 differences,
 // Barth-Jespersen limiter on deviations, face records (a, hu_f, hv_f, w) of its nf faces in
 // out[0..nf) -- see the file comment. st is the state the step starts from.
@@ -157,7 +187,7 @@ struct GlobalCells {
 // the sums keep the stencil order. KMAX = 0: plain loop, for stencils wider than KMAX.
 template <int MF, int KMAX, class Cells>
 __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p, const Cells &a, int i, float om,
-                                            float4 *out)
+                                            float4 *out, SlotGeo *keep = nullptr)
 {
     const size_t C = m.C;
     // Loads in batches ahead of the arithmetic: the counts, the stencil handles of the K rows
@@ -220,8 +250,17 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         const float4 q = reinterpret_cast<const float4 *>(p.geoB)[MF * (size_t)i + s];
+        const float4 ga = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
         gb[s] = make_float2(q.x, q.y);
-        dzo[s] = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s].w;
+        dzo[s] = ga.w;
+        if (keep) {  // the slot geometry of the flux step, for a caller that keeps it
+            keep->nx[s] = ga.x;
+            keep->ny[s] = ga.y;
+            keep->dz[s] = ga.z;
+            keep->dzo[s] = ga.w;
+            keep->len[s] = q.z;
+            keep->ilen[s] = q.w;
+        }
     }
     for (int j = 0; j < ng; j++) {
         const int f = m.lsq_ghosts[(size_t)j * C + i];
@@ -348,53 +387,28 @@ struct GlobalRecs {
 // candidate. own: this cell's index in the record array; code_of(s): neighbour code of slot s
 // in the record array's numbering. The slot data of the [MF]-wide tables is loaded in one
 // batch before the arithmetic (in bounds whatever nf).
-// LEN: take |n| and 1/|n| from geoB (the fused step, bound by its arithmetic) or compute them
-// from the normal (the flux kernel, bound by its memory traffic); the values are the same.
-template <int MF, bool LEN, class Recs, class Codes>
+// G: the cell's slot geometry (load_slot_geo, or kept from the reconstruction).
+template <int MF, class Recs, class Codes>
 __device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int own, const double4 si, int nf,
-                                              unsigned ord, Recs rec, Codes code_of, float g, double cfl,
-                                              double dt, double4 *out)
+                                              unsigned ord, Recs rec, Codes code_of, const SlotGeo &G, float g,
+                                              double cfl, double dt, double4 *out)
 {
-    // slot data in the residual order (slot s of the r-th contribution in bits 2r..2r+1)
+    // slots and codes in the residual order (slot s of the r-th contribution in bits 2r..2r+1)
     int sl[MF], cd[MF];
-    float4 gr[MF];
-    float gm[MF], gi[MF];
 #pragma unroll
     for (int r = 0; r < MF; r++) {
         const int s = min((int)((ord >> (2 * r)) & 3), MF - 1);
         sl[r] = s;
         cd[r] = code_of(s);
-        gr[r] = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
-        if (LEN) {
-            const float4 b = reinterpret_cast<const float4 *>(p.geoB)[MF * (size_t)i + s];
-            gm[r] = b.z;
-            gi[r] = b.w;
-        }
-    }
-    // the dt's geometry in faceid order, loaded with the rest (after the state store a load
-    // could not be moved above it)
-    slot_f gx = 0.f, gy = 0.f, gl = 1.f;
-#pragma unroll
-    for (int s = 0; s < MF; s++) {
-        const float4 q = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
-        gx[s] = q.x;
-        gy[s] = q.y;
-        gl[s] = LEN ? reinterpret_cast<const float4 *>(p.geoB)[MF * (size_t)i + s].z : sqrtf(q.x * q.x + q.y * q.y);
     }
     const double vol = p.vol[i];
     double r0 = 0., r1 = 0., r2 = 0., s1 = 0., s2 = 0.;
 #pragma unroll
     for (int r = 0; r < MF; r++) {
         if (r >= nf) break;
-        const float4 ga = gr[r];
-        float mes, im;
-        if (LEN) {
-            mes = gm[r];
-            im = gi[r];
-        } else {
-            mes = sqrtf(ga.x * ga.x + ga.y * ga.y);
-            im = 1.f / mes;
-        }
+        const int s = sl[r];
+        const float4 ga = make_float4(G.nx[s], G.ny[s], G.dz[s], G.dzo[s]);
+        const float mes = G.len[s], im = G.ilen[s];
         const int code = cd[r];
         // One evaluation in the face's orientation whichever side the cell is on (a wave holds
         // cells of both sides, so per-side branches would each run for the whole wave): the L
@@ -402,7 +416,7 @@ __device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int 
         // record with the momenta negated (-1 * x) and dz = 0 (geoA holds 0 there). The signs
         // of the contributions are exact multiplications by -1 or 1.
         const bool wall = code < 0, own_R = !wall && (code & 1);
-        const size_t ko = MF * (size_t)own + sl[r];
+        const size_t ko = MF * (size_t)own + s;
         const size_t kn = wall ? ko : MF * (size_t)(code >> 3) + ((code >> 1) & 3);
         const float4 L = rec(own_R ? kn : ko), R = rec(own_R ? ko : kn);
         const float mR = wall ? -1.f : 1.f;
@@ -424,7 +438,7 @@ __device__ __forceinline__ double flux32_core(const mfv_sw_mixed &p, int i, int 
     const double un = si.y + R1 * dv;
     const double vn = si.z + R2 * dv;
     *out = make_double4(hn, un, vn, si.w);
-    return sw_dt32<MF>(gx, gy, gl, nf, hn, un, vn, vol, g, cfl);
+    return sw_dt32<MF>(G.nx, G.ny, G.len, nf, hn, un, vn, vol, g, cfl);
 }
 
 // The cell step on the global records, in place on p.state.
@@ -436,8 +450,9 @@ __device__ __forceinline__ double sw_flux32_cell(const Mesh &m, const mfv_sw_mix
     double4 *st = reinterpret_cast<double4 *>(p.state);
     const int32_t *__restrict__ nbr = p.nbr;
     auto code_of = [nbr, C, i](int s) { return nbr[(size_t)s * C + i]; };
-    return flux32_core<MF, false>(p, i, i, st[i], m.cell_nfaces[i], p.order[i],
-                           GlobalRecs{reinterpret_cast<const float4 *>(p.facerec)}, code_of, g, cfl, dt, st + i);
+    return flux32_core<MF>(p, i, i, st[i], m.cell_nfaces[i], p.order[i],
+                           GlobalRecs{reinterpret_cast<const float4 *>(p.facerec)}, code_of,
+                           load_slot_geo<MF, false>(p, i), g, cfl, dt, st + i);
 }
 
 template <int MF>
@@ -785,21 +800,26 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
         }
         __syncthreads();
     }
-    for (int q = threadIdx.x; q < nc + nh; q += T) {
-        const int c = (q < nc) ? c0 + q : t.ext[e0 + (q - nc)];
+    // A tile cell is reconstructed and updated by the same thread (q = threadIdx.x is in the
+    // first pass), which keeps the cell's slot geometry in registers for the flux phase.
+    const int q = threadIdx.x;
+    SlotGeo G{0.f, 0.f, 0.f, 0.f, 1.f, 1.f};
+    auto recon_item = [&](int qq, SlotGeo *keep) {
+        const int c = (qq < nc) ? c0 + qq : t.ext[e0 + (qq - nc)];
         float4 rec[MF];
         int nf;
         if (STAGED)
-            nf = recon32_core<MF, KMAX>(m, p, StagedCells{s_st, s_cc, t.lst, t.lgh, t.lfn, r0 + q, (size_t)t.I, q},
-                                        c, om, rec);
+            nf = recon32_core<MF, KMAX>(
+                m, p, StagedCells{s_st, s_cc, t.lst, t.lgh, t.lfn, r0 + qq, (size_t)t.I, qq}, c, om, rec, keep);
         else
-            nf = recon32_cell<MF, KMAX>(m, p, st, c, om, rec);
+            nf = recon32_core<MF, KMAX>(m, p, GlobalCells{m, st, c}, c, om, rec, keep);
 #pragma unroll
         for (int s = 0; s < MF; s++)
-            if (s < nf) lfr[MF * q + s] = rec[s];
-    }
+            if (s < nf) lfr[MF * qq + s] = rec[s];
+    };
+    if (q < nc + nh) recon_item(q, &G);
+    for (int qq = q + T; qq < nc + nh; qq += T) recon_item(qq, nullptr);  // items beyond one per thread
     __syncthreads();
-    const int q = threadIdx.x;
     if ((q & ~63) >= nc) return;  // the whole wave lies past the tile: it owns no partial
     double dloc = 1e300;
     if (q < nc) {
@@ -810,7 +830,7 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
         auto code_of = [lnbr, C, i](int s) { return lnbr[(size_t)s * C + i]; };
         auto recs = [lfr](size_t k) { return lfr[k]; };
         const double4 si = STAGED ? s_st[q] : st[i];
-        dloc = flux32_core<MF, true>(p, i, q, si, m.cell_nfaces[i], p.order[i], recs, code_of, g, cfl, dt, out + i);
+        dloc = flux32_core<MF>(p, i, q, si, m.cell_nfaces[i], p.order[i], recs, code_of, G, g, cfl, dt, out + i);
     }
     wave_min_part(part, (size_t)c0 + q, dloc);
 }
