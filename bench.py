@@ -604,7 +604,7 @@ def run_serving(n_req, seed=1, nsteps=20):
         "latency_p50_s": round(lat['p50_s'], 4), "latency_p99_s": round(lat['p99_s'], 4),
         "latency_mean_s": round(lat['mean_s'], 4), "max_concurrent": eng.max_active,
         "hbm_budget_bytes": budget, "krylov_iterations_mean": float(np.mean([r.iterations for r in done])),
-        "solve_tol": 1e-10, "mesh_build_s": round(t_build, 1),
+        "solve_tol": 1e-10, "mesh_build_s": round(t_build, 1), "solver_tick": "HIP graph replay",
     }
 
 

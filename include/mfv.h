@@ -581,6 +581,14 @@ typedef struct mfv_krylov_work {
 int mfv_krylov_tick(const mfv_ell_batch *e, const mfv_krylov_work *w, int n_iter, int grid_slices,
                     int grid_inst, void *stream);
 
+/* The same tick captured in a graph (replay with mfv_graph_launch, release with
+ * mfv_graph_destroy). The descriptors hold capacities and device pointers only and the kernels
+ * read the live lists and counts from device memory, so one graph serves every later state
+ * of the arena; the tick's result does not depend on the grid sizes. No counterpart in the
+ * reference: it lets a serving loop advance all solves with one launch. */
+int mfv_krylov_graph(const mfv_ell_batch *e, const mfv_krylov_work *w, int n_iter, int grid_slices,
+                     int grid_inst, void **graph_out);
+
 /* y = A x (FP64) on the live slices. */
 int mfv_ell_spmv(const mfv_ell_batch *e, const double *x, double *y, int grid_slices, void *stream);
 

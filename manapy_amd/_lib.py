@@ -186,6 +186,7 @@ _SIGNATURES = {
     'mfv_graph_launch': [c_void_p, P],
     'mfv_graph_destroy': [c_void_p],
     'mfv_krylov_tick': [ELL_P, KW_P, c_int, c_int, c_int, P],
+    'mfv_krylov_graph': [ELL_P, KW_P, c_int, c_int, c_int, ctypes.POINTER(c_void_p)],
     'mfv_ell_spmv': [ELL_P, P, P, c_int, P],
 }
 _RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p, 'mfv_sw_tiles_host_free': None,

@@ -65,6 +65,21 @@ __device__ __forceinline__ int wave_max_int(int v)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// A captured graph (mfv_graph_launch / mfv_graph_destroy) with the capture stream and the dt
+// partials buffer it owns: the library's shared scratch may be reallocated by later calls,
+// which must not move memory a graph still refers to.
+struct MfvGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+    double *part = nullptr;
+};
+void graph_free(MfvGraph *g);
+// Starts a capture on a new stream of g; returns MFV_OK or MFV_EHIP (g is then freed by the
+// caller). graph_finish ends it and instantiates the graph.
+int graph_begin(MfvGraph *g);
+int graph_finish(MfvGraph *g, int rc);
+
 // Cells at positions e and e + T of a tile's ext order (its nc cells from c0, then
 // ext[e0 .. e0 + ne)), with both table loads issued before either is used. Past the end,
 // the second is the first again (writing it twice is harmless).

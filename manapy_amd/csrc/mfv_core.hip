@@ -305,6 +305,43 @@ struct DtScratch {
 static std::mutex g_dt_mu;
 static std::vector<DtScratch> g_dt_scr;
 
+void graph_free(MfvGraph *g)
+{
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->part) (void)hipFree(g->part);
+    delete g;
+}
+
+int graph_begin(MfvGraph *g)
+{
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        set_error("graph capture setup: %s", hipGetErrorString(e));
+        return MFV_EHIP;
+    }
+    return MFV_OK;
+}
+
+int graph_finish(MfvGraph *g, int rc)
+{
+    hipError_t e = hipStreamEndCapture(g->stream, &g->graph);
+    if (rc != MFV_OK) return rc;
+    if (e != hipSuccess) {
+        set_error("graph capture: %s", hipGetErrorString(e));
+        return MFV_EHIP;
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        set_error("graph instantiate: %s", hipGetErrorString(e));
+        return MFV_EHIP;
+    }
+    return MFV_OK;
+}
+
 double *dt_scratch(size_t nwaves, hipStream_t st)
 {
     int dev = 0;

@@ -395,17 +395,21 @@ static int validate_work(const mfv_krylov_work *w)
 
 using namespace mfv;
 
-extern "C" int mfv_krylov_tick(const mfv_ell_batch *e, const mfv_krylov_work *w, int n_iter, int grid_slices,
-                               int grid_inst, void *stream)
+static int check_tick(const mfv_ell_batch *e, const mfv_krylov_work *w, int n_iter, int grid_slices, int grid_inst)
 {
     if (int rc = validate_ell(e)) return rc;
     if (int rc = validate_work(w)) return rc;
     MFV_REQUIRE(n_iter >= 0 && n_iter <= 4096, "n_iter %d out of [0, 4096]", n_iter);
     MFV_REQUIRE(grid_slices >= 1 && grid_slices <= 65536, "grid_slices %d out of [1, 65536]", grid_slices);
     MFV_REQUIRE(grid_inst >= 1 && grid_inst <= 65536, "grid_inst %d out of [1, 65536]", grid_inst);
+    return MFV_OK;
+}
+
+static int tick_launches(const mfv_ell_batch *e, const mfv_krylov_work *w, int n_iter, int grid_slices,
+                         int grid_inst, hipStream_t st)
+{
     const Ell ek = to_kernel(e);
     const Work wk = to_kernel(w);
-    hipStream_t st = as_stream(stream);
     const dim3 gs(grid_slices), gi(grid_inst), b(256);
     hipLaunchKernelGGL(k_ir_correct, gs, b, 0, st, ek, wk);
     hipLaunchKernelGGL(k_ir_residual, gs, b, 0, st, ek, wk);
@@ -421,6 +425,30 @@ extern "C" int mfv_krylov_tick(const mfv_ell_batch *e, const mfv_krylov_work *w,
         hipLaunchKernelGGL(k_bicg_rho, gi, b, 0, st, ek, wk);
     }
     return check_launch("krylov tick");
+}
+
+extern "C" int mfv_krylov_tick(const mfv_ell_batch *e, const mfv_krylov_work *w, int n_iter, int grid_slices,
+                               int grid_inst, void *stream)
+{
+    if (int rc = check_tick(e, w, n_iter, grid_slices, grid_inst)) return rc;
+    return tick_launches(e, w, n_iter, grid_slices, grid_inst, as_stream(stream));
+}
+
+extern "C" int mfv_krylov_graph(const mfv_ell_batch *e, const mfv_krylov_work *w, int n_iter, int grid_slices,
+                                int grid_inst, void **graph_out)
+{
+    if (int rc = check_tick(e, w, n_iter, grid_slices, grid_inst)) return rc;
+    MFV_REQUIRE(graph_out, "graph_out is NULL");
+    *graph_out = nullptr;
+    MfvGraph *g = new MfvGraph();
+    int rc = graph_begin(g);
+    if (rc == MFV_OK) rc = graph_finish(g, tick_launches(e, w, n_iter, grid_slices, grid_inst, g->stream));
+    if (rc != MFV_OK) {
+        graph_free(g);
+        return rc;
+    }
+    *graph_out = g;
+    return MFV_OK;
 }
 
 extern "C" int mfv_ell_spmv(const mfv_ell_batch *e, const double *x, double *y, int grid_slices, void *stream)

@@ -676,25 +676,6 @@ int batch_step_bs(const Mesh &m, const mfv_sw_mixed &p, const BatchK &b, float g
     return check_launch("ensemble step");
 }
 
-// A captured ensemble graph owns its dt partials buffer: the library's shared scratch may be
-// reallocated by later calls, which must not move memory a graph still refers to.
-struct MfvGraph {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipStream_t stream = nullptr;
-    double *part = nullptr;
-};
-
-void graph_free(MfvGraph *g)
-{
-    if (!g) return;
-    if (g->exec) (void)hipGraphExecDestroy(g->exec);
-    if (g->graph) (void)hipGraphDestroy(g->graph);
-    if (g->stream) (void)hipStreamDestroy(g->stream);
-    if (g->part) (void)hipFree(g->part);
-    delete g;
-}
-
 int check_mixed(const mfv_mesh *m, const mfv_sw_mixed *p)
 {
     MFV_REQUIRE(p != nullptr, "mixed descriptor is NULL");

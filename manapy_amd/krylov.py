@@ -20,6 +20,10 @@ from . import _lib
 from .device import require_gpu, stream_ptr
 
 
+def _pow2(n):
+    return 1 << max(0, int(n) - 1).bit_length()
+
+
 class RangeAllocator(object):
     """First-fit allocator of contiguous ranges of ``[0, size)`` with coalescing frees."""
 
@@ -220,20 +224,44 @@ class EllBatch(object):
         nsl, ni = getattr(self, '_n_live', (0, 0))
         return max(1, min(4096, (nsl + 3) // 4)), max(1, ni)
 
-    def tick(self, n_iter=8, grid_slices=None, grid_inst=None, stream=None):
+    def tick(self, n_iter=8, grid_slices=None, grid_inst=None, stream=None, graph=False):
+        """One tick of every live instance. With ``graph=True`` the tick's launches are
+        replayed from a HIP graph captured once per (n_iter, grid sizes rounded up to powers of
+        two): the kernels read the live lists from device memory, so a graph stays valid as
+        instances come and go, and the result is the same bits as the eager tick."""
         gs, gi = self.grids()
-        _lib.call('mfv_krylov_tick', self.eptr, self.wptr, int(n_iter), int(grid_slices or gs),
-                  int(grid_inst or gi), stream_ptr(stream))
+        gs, gi = int(grid_slices or gs), int(grid_inst or gi)
+        if not graph:
+            _lib.call('mfv_krylov_tick', self.eptr, self.wptr, int(n_iter), gs, gi, stream_ptr(stream))
+            return
+        key = (int(n_iter), min(4096, _pow2(gs)), min(65536, _pow2(gi)))
+        graphs = self.__dict__.setdefault('_graphs', {})
+        g = graphs.get(key)
+        if g is None:
+            out = ctypes.c_void_p()
+            _lib.call('mfv_krylov_graph', self.eptr, self.wptr, key[0], key[1], key[2], ctypes.byref(out))
+            g = graphs[key] = out.value
+        _lib.call('mfv_graph_launch', g, stream_ptr(stream))
+
+    def close_graphs(self):
+        for g in self.__dict__.pop('_graphs', {}).values():
+            _lib.call('mfv_graph_destroy', g)
+
+    def __del__(self):
+        try:
+            self.close_graphs()
+        except Exception:
+            pass
 
     def spmv(self, x, y):
         gs, _ = self.grids()
         _lib.call('mfv_ell_spmv', self.eptr, x.data_ptr(), y.data_ptr(), gs, stream_ptr())
 
-    def run(self, insts, n_iter=8, max_ticks=20000):
+    def run(self, insts, n_iter=8, max_ticks=20000, graph=False):
         """Tick until every listed instance is DONE or FAILED; returns their statuses."""
         idx = np.asarray(list(insts), dtype=np.int64)
         for _ in range(max_ticks):
-            self.tick(n_iter)
+            self.tick(n_iter, graph=graph)
             st = self.status.cpu().numpy()[idx]
             if np.all((st == _lib.KS_DONE) | (st == _lib.KS_FAILED)):
                 return st

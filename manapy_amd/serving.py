@@ -73,12 +73,13 @@ class AdvDiffRequest(object):
 
 class ServingEngine(object):
     def __init__(self, max_cells=4 << 20, max_inst=128, width=16, hbm_budget=None, n_iter=6, tol=1e-10,
-                 tol_inner=1e-4, device=None):
+                 tol_inner=1e-4, device=None, graph=True):
         torch = require_gpu()
         self.arena = EllBatch((int(max_cells) + 63) // 64, max_inst=max_inst, width=width, device=device)
         self.device = self.arena.device
         self.hbm_budget = int(hbm_budget) if hbm_budget else None
         self.n_iter = int(n_iter)
+        self.graph = bool(graph)
         self.tol, self.tol_inner = float(tol), float(tol_inner)
         self.active = []
         self.in_use = 0
@@ -192,7 +193,7 @@ class ServingEngine(object):
             for r in self.active:
                 if r.phase == PHASE_ADVECT:
                     self._advect(r)
-            self.arena.tick(self.n_iter)
+            self.arena.tick(self.n_iter, graph=self.graph)
             self.ticks += 1
             st = self.arena.status.cpu().numpy()
             now = clock() - t0

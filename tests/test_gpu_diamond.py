@@ -197,7 +197,7 @@ def test_batched_solves_alone_equal_mix():
     rng = np.random.default_rng(0)
     rhs = [torch.as_tensor(rng.standard_normal(int(d.nbcells)), device='cuda') for d, *_ in doms]
 
-    def solve(which, batch):
+    def solve(which, batch, graph=False):
         insts = []
         for k in which:
             d, P, sig, tau, _ = doms[k]
@@ -209,7 +209,7 @@ def test_batched_solves_alone_equal_mix():
             node_dir, rank, _ = P.diamond_bc()
             batch.assemble_diamond(i, d.device_diamond, node_dir, rank)
             batch.set_system(i, sig, tau, rhs[k], None, tol=1e-10)
-        st = batch.run(insts)
+        st = batch.run(insts, graph=graph)
         assert np.all(st == 4), st
         return [batch.solution(i).cpu().numpy().copy() for i in insts]
 
@@ -221,6 +221,13 @@ def test_batched_solves_alone_equal_mix():
     mixed = solve([2, 0, 3, 1], big)
     for k, x in zip([2, 0, 3, 1], mixed):
         assert np.array_equal(x, alone[k]), k
+    # ticks replayed from HIP graphs: same bits as the eager ticks
+    g = EllBatch(200, max_inst=8)
+    graphed = solve([1, 3, 0, 2], g, graph=True)
+    assert len(g._graphs) >= 1
+    for k, x in zip([1, 3, 0, 2], graphed):
+        assert np.array_equal(x, alone[k]), k
+    g.close_graphs()
 
 
 def _ref(name, module='manapy.ast.functions2d'):

@@ -34,6 +34,12 @@ def test_alone_equals_mix():
     assert len(done) == 3 and eng.max_active >= 2
     for r in done:
         assert np.array_equal(r.result, alone[r.rid]), r.rid
+    # eager solver ticks instead of graph replays: same bits
+    eng = ServingEngine(max_cells=1 << 14, max_inst=4, n_iter=5, graph=False)
+    reqs = [_request(rid, nx, ny, seed, D, vel, n, arrival=0.001 * k)
+            for k, (rid, nx, ny, seed, D, vel, n) in enumerate(specs)]
+    for r in eng.run(reqs):
+        assert np.array_equal(r.result, alone[r.rid]), r.rid
 
 
 def test_request_matches_fp64_direct_solve(oracle_lib):
