@@ -272,8 +272,8 @@ def run_advection(n, steps, warmup, cpu=True):
 
 def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
     """3D Sod shock tube on n^3 cubes x 6 Kuhn tetrahedra of the unit cube (interior nodes
-    jittered by +/-8 % of h, slip walls, first-order Rusanov, CFL 0.8): the fused device step
-    (mfv_euler3d_step_fused). Parity: the first ``cpu_steps`` steps against the oracle, bit for
+    jittered by +/-8 % of h, slip walls, first-order Rusanov, CFL 0.8): the fastest of the
+    fused device steps (per-cell on [6][C], packed records, face tiles). Parity: the first ``cpu_steps`` steps against the oracle, bit for
     bit, at full size (the same run times the CPU baseline)."""
     import torch
     from manapy_amd import _lib
@@ -296,41 +296,53 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
     S = [dv(U0), dv(np.zeros((6, C)))]
     ring, dt0 = dv(np.zeros(4)), dv(np.zeros(1))
 
-    def init():
-        S[0].copy_(dv(U0))
-        _lib.call('mfv_euler3d_time_step', m.ptr, S[0].data_ptr(), CFL, 1.4, dt0.data_ptr(), None, s)
-        _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
-
-    from manapy_amd.device import FaceTiles
+    from manapy_amd.device import EulerTet, FaceTiles
     tiles = {}
     for T in (128, 256):
         ft = FaceTiles(m, T)
         if ft.lds_bytes() <= 64 * 1024:
             tiles[T] = ft
-    path = [None]  # None: the per-cell fused step; T: the face-tiled step
+    et = EulerTet(m)
+    R = [torch.empty((C, 8), dtype=torch.float64, device='cuda') for _ in range(2)]
+    path = ['fused']  # 'fused': per-cell step on [6][C]; 'tiled:T': face tiles; 'packed': [C][8] records
+
+    def init():
+        S[0].copy_(dv(U0))
+        _lib.call('mfv_euler3d_time_step', m.ptr, S[0].data_ptr(), CFL, 1.4, dt0.data_ptr(), None, s)
+        _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
+        _lib.call('mfv_euler3d_pack', m.ptr, S[0].data_ptr(), R[0].data_ptr(), s)
 
     def step(k):
-        if path[0] is None:
+        if path[0] == 'fused':
             _lib.call('mfv_euler3d_step_fused', m.ptr, S[k % 2].data_ptr(), S[(k + 1) % 2].data_ptr(), 1.4, CFL, 0,
                       ring.data_ptr(), k, s)
+        elif path[0] == 'packed':
+            _lib.call('mfv_euler3d_step_packed', m.ptr, et.ptr, R[k % 2].data_ptr(), R[(k + 1) % 2].data_ptr(), 1.4,
+                      CFL, 0, ring.data_ptr(), k, s)
         else:
-            _lib.call('mfv_euler3d_step_tiled', m.ptr, tiles[path[0]].ptr, S[k % 2].data_ptr(),
+            _lib.call('mfv_euler3d_step_tiled', m.ptr, tiles[int(path[0].split(':')[1])].ptr, S[k % 2].data_ptr(),
                       S[(k + 1) % 2].data_ptr(), 1.4, CFL, 0, ring.data_ptr(), k, s)
     probe = {}
-    for cand in [None] + sorted(tiles):
+    for cand in ['fused', 'packed'] + ['tiled:%d' % T for T in sorted(tiles)]:
         path[0] = cand
         init()
-        probe["tiled:%d" % cand if cand else "fused"] = time_steps(step, 2, 10)[1] * 1e3
+        probe[cand] = time_steps(step, 2, 10)[1] * 1e3
     best = min(probe, key=probe.get)
-    path[0] = None if best == 'fused' else int(best.split(':')[1])
+    path[0] = best
     log("euler: path %s %s" % (best, {k: round(v, 4) for k, v in probe.items()}))
     init()
     t_loop, t_ev = time_steps(step, warmup, steps)
-    # algorithmic bytes of k_euler_step: U read + write (6 doubles each), volume, face count;
-    # per face slot: order entry + faceid; per face (x F/C): cellid, normal, mesure, name
     MF = int(d.cells.faceid.shape[1] - 1)
-    bpc = (48 + 48 + 8 + 4 + (1 + 4) * MF) + (8 + 24 + 8 + 4) * F / C
-    kname = "k_euler_step_tet" if path[0] is None else "k_euler_step_tiled<%d>" % path[0]
+    if best == 'packed':
+        # record read + write (64 B each: six conserved/primitive values, volume, pad), the four
+        # new face ids and the rank byte; per face (x F/C): geometry record 32 B + cell pair 8 B
+        bpc = (64 + 64 + 16 + 1) + (32 + 8) * F / C
+        kname = "k_euler_step_packed"
+    else:
+        # U read + write (6 doubles each), volume, face count; per face slot: order entry +
+        # faceid; per face (x F/C): cellid, normal, mesure, name
+        bpc = (48 + 48 + 8 + 4 + (1 + 4) * MF) + (8 + 24 + 8 + 4) * F / C
+        kname = "k_euler_step_tet" if best == 'fused' else "k_euler_step_tiled<%s>" % best.split(':')[1]
     out = {"workload": "3D Sod shock tube, %d^3 x 6 Kuhn tetrahedra, first-order Rusanov, slip walls, CFL 0.8" % n,
            "cells": C, "faces": F, "steps": steps, "dtype": "f64", "mesh_build_s": round(t_mesh, 1),
            "value": round(C / t_loop / 1e6, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop * 1e3, 5),
@@ -345,6 +357,8 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
         init()
         for k in range(cpu_steps):
             step(k)
+        if best == 'packed':
+            _lib.call('mfv_euler3d_unpack', m.ptr, R[cpu_steps % 2].data_ptr(), S[cpu_steps % 2].data_ptr(), s)
         U_gpu = S[cpu_steps % 2].cpu().numpy()
         U = U0.copy()
         G_ = np.zeros((6, F))

@@ -437,6 +437,51 @@ void mfv_face_tiles_host_free(mfv_face_tiles_host *t);
 int mfv_euler3d_step_tiled(const mfv_mesh *m, const mfv_face_tiles *t, const double *U_in, double *U_out,
                            double gamma, double cfl, int quirk, double *dt_ring, int step, void *stream);
 
+/* Tables of the packed Euler step on tetrahedra. The faces are renumbered in the order the
+ * cells, in their storage order, first reach them: a wave's new faces are contiguous and a
+ * face's second reader finds it in cache. A face is one 32-byte geometry record. A cell keeps,
+ * per slot, its face (new id) and what lies across it, so that the neighbour records and the
+ * face records load together; and the slot order of its faces' original ids (the reference's
+ * face-loop order, the residual's summation order). */
+typedef struct mfv_euler_tet {
+    int32_t nb_cells, nb_faces;
+    const int32_t *cell_faces; /* [C][4] the cell's faces in slot order (cells.faceid), new ids */
+    const int32_t *cell_nb;    /* [C][4] per slot: bit 31 boundary face, bit 30 the cell is the face's right
+                                * side, bits 0-29 the cell across (inner face) or faces.name (boundary face) */
+    const uint8_t *cell_rank;  /* [C] bits 2k..2k+1: slot of the cell's k-th smallest original face id */
+    const double *face_geo;    /* [F][4] faces.normal (3), faces.mesure */
+} mfv_euler_tet;
+
+/* Host-side construction (no GPU needed) from host copies of the mfv_mesh tables: cell_faces
+ * [4][C], face_cells [F][2], face_name [F], face_normal [F][3], face_mesure [F]. Every cell
+ * needs four distinct faces it is a side of; an inner face (name 0) two cells; names are in
+ * [0, 2^30) and C < 2^30. The arrays of out are malloc'd; release them with
+ * mfv_euler_tet_host_free. */
+typedef struct mfv_euler_tet_host {
+    int32_t nb_cells, nb_faces;
+    int32_t *cell_faces;
+    int32_t *cell_nb;
+    uint8_t *cell_rank;
+    double *face_geo;
+} mfv_euler_tet_host;
+
+int mfv_euler_tet_build(int C, int F, const int32_t *cell_faces, const int32_t *face_cells, const int32_t *face_name,
+                        const double *face_normal, const double *face_mesure, mfv_euler_tet_host *out);
+void mfv_euler_tet_host_free(mfv_euler_tet_host *t);
+
+/* Cell records of the packed Euler step: rec[c][0..5] = rho, P, rhou, rhov, rhow, rhoE of
+ * U [6][C], rec[c][6] = the cell volume, rec[c][7] = 0 (64 bytes per cell). */
+int mfv_euler3d_pack(const mfv_mesh *m, const double *U, double *rec, void *stream);
+int mfv_euler3d_unpack(const mfv_mesh *m, const double *rec, double *U, void *stream);
+
+/* The fused Euler step on packed records (tetrahedra): each neighbour is one 64-byte record
+ * and each face one 32-byte record in the cells' order; the volume travels in the record
+ * (rec_out[c][6] = rec_in[c][6]). Same expressions and order as mfv_euler3d_step_fused, so
+ * the same bits. Replaces explicitscheme_euler_3d_rusanov (fvm_utils3d.py:1083-1170) +
+ * update_euler_3d_fvc (:1316-1327) + time_step_euler_3d (:809-825) in one launch. */
+int mfv_euler3d_step_packed(const mfv_mesh *m, const mfv_euler_tet *t, const double *rec_in, double *rec_out,
+                            double gamma, double cfl, int quirk, double *dt_ring, int step, void *stream);
+
 /* ---------------- diamond scheme, 2D (ast/functions2d.py, solvers/diffusion) --------- */
 
 /* Diamond-scheme geometry. The pseudo-Laplacian weight of a node incidence is

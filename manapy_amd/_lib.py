@@ -65,6 +65,18 @@ class MfvFaceTilesHost(ctypes.Structure):
                           'face_r', 'lface')]
 
 
+class MfvEulerTet(ctypes.Structure):
+    """Mirror of ``struct mfv_euler_tet`` (include/mfv.h)."""
+    _fields_ = [('nb_cells', ctypes.c_int32), ('nb_faces', ctypes.c_int32)] + \
+        [(k, P) for k in ('cell_faces', 'cell_nb', 'cell_rank', 'face_geo')]
+
+
+class MfvEulerTetHost(ctypes.Structure):
+    """Mirror of ``struct mfv_euler_tet_host`` (include/mfv.h)."""
+    _fields_ = [('nb_cells', ctypes.c_int32), ('nb_faces', ctypes.c_int32)] + \
+        [(k, P) for k in ('cell_faces', 'cell_nb', 'cell_rank', 'face_geo')]
+
+
 class MfvSwPacked(ctypes.Structure):
     """Mirror of ``struct mfv_sw_packed`` (include/mfv.h)."""
     _fields_ = [('nb_cells', ctypes.c_int32), ('max_cell_faces', ctypes.c_int32), ('cellrec', P), ('facerec', P),
@@ -122,6 +134,7 @@ SWP_P = ctypes.POINTER(MfvSwPacked)
 SWM_P = ctypes.POINTER(MfvSwMixed)
 SWT_P = ctypes.POINTER(MfvSwTiles)
 FT_P = ctypes.POINTER(MfvFaceTiles)
+ET_P = ctypes.POINTER(MfvEulerTet)
 DP = ctypes.POINTER(ctypes.c_double)
 
 # name -> argtypes (restype is int unless listed in _RESTYPE)
@@ -164,6 +177,11 @@ _SIGNATURES = {
     'mfv_euler3d_step_tiled': [MESH_P, FT_P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_face_tiles_build': [c_int, c_int, P, P, c_int, P, c_int, ctypes.POINTER(MfvFaceTilesHost)],
     'mfv_face_tiles_host_free': [ctypes.POINTER(MfvFaceTilesHost)],
+    'mfv_euler_tet_build': [c_int, c_int, P, P, P, P, P, ctypes.POINTER(MfvEulerTetHost)],
+    'mfv_euler_tet_host_free': [ctypes.POINTER(MfvEulerTetHost)],
+    'mfv_euler3d_pack': [MESH_P, P, P, P],
+    'mfv_euler3d_unpack': [MESH_P, P, P, P],
+    'mfv_euler3d_step_packed': [MESH_P, ET_P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_diamond_widths': [DIA_P, P, P, P, P, P],
     'mfv_diamond_assemble': [DIA_P, P, P, c_int, c_int, P, P, P, P, P, P],
     'mfv_diamond_rhs': [DIA_P, P, P, P, P, P, P],
@@ -190,7 +208,7 @@ _SIGNATURES = {
     'mfv_ell_spmv': [ELL_P, P, P, c_int, P],
 }
 _RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p, 'mfv_sw_tiles_host_free': None,
-            'mfv_face_tiles_host_free': None}
+            'mfv_face_tiles_host_free': None, 'mfv_euler_tet_host_free': None}
 
 _lib = None
 

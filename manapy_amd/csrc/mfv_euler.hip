@@ -6,6 +6,11 @@
 //   Q2 ghost_value_TubeSchok3D (:493-494): rhovg is overwritten by rho*w_g, rhowg never set
 //      (it keeps its previous content: zero for zero-initialised ghost arrays).
 // quirk = 0 uses the right state in qr and the mirrored velocity in both components.
+#include <stdlib.h>
+
+#include <utility>
+#include <vector>
+
 #include "mfv_common.hpp"
 
 using namespace mfv;
@@ -332,6 +337,162 @@ __global__ void __launch_bounds__(kBlock) k_euler_step_tet(Mesh m, const double 
     wave_min_part(dtpart, (size_t)i, d);
 }
 
+// The fused step on packed records (mfv_euler_tet): the arithmetic and summation orders of
+// k_euler_step_tet; a neighbour is one 64-byte record [C][8] and a face one 32-byte geometry
+// record, in faces numbered by first use along the cell order. The cell's own tables name
+// what lies across each face, so a face's geometry and its neighbour's record load together.
+struct EulerTetK {
+    const int4 *__restrict__ cell_faces;
+    const int4 *__restrict__ cell_nb;
+    const uint8_t *__restrict__ cell_rank;
+    const double *__restrict__ face_geo;
+};
+
+constexpr int kNbBoundary = int(0x80000000u);
+constexpr int kNbOwnRight = 1 << 30;
+constexpr int kNbIndex = (1 << 30) - 1;
+
+__device__ __forceinline__ Prim6 load_rec(const double *__restrict__ R, int c)
+{
+    const double2 *p = reinterpret_cast<const double2 *>(R + 8 * (size_t)c);
+    const double2 a = p[0], b = p[1], e = p[2];
+    return Prim6{a.x, a.y, b.x, b.y, e.x, e.y};
+}
+
+__device__ __forceinline__ void load_geo(const double *__restrict__ G, int f, double &nx, double &ny, double &nz,
+                                         double &ms)
+{
+    const double2 *p = reinterpret_cast<const double2 *>(G + 4 * (size_t)f);
+    const double2 a = p[0], b = p[1];
+    nx = a.x;
+    ny = a.y;
+    nz = b.x;
+    ms = b.y;
+}
+
+// Two faces in residual order (face ids f0, f1; across codes v0, v1): their geometry and the
+// neighbour records load together, then the fluxes are added to r in that order.
+__device__ __forceinline__ void euler_face_pair_packed(const EulerTetK &t, int i, const double *__restrict__ R,
+                                                       const Prim6 &own, int f0, int f1, int v0, int v1,
+                                                       double gamma, int quirk, double (&r)[5])
+{
+    const int f[2] = {f0, f1}, v[2] = {v0, v1};
+    double nx[2], ny[2], nz[2], ms[2];
+    Prim6 nb[2];  // the cell across each face (the own cell on a boundary face: not used)
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        load_geo(t.face_geo, f[k], nx[k], ny[k], nz[k], ms[k]);
+        nb[k] = load_rec(R, v[k] >= 0 ? (v[k] & kNbIndex) : i);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const double nrm[3] = {nx[k], ny[k], nz[k]};
+        const double mesu = ms[k];
+        const double n[3] = {nrm[0] / mesu, nrm[1] / mesu, nrm[2] / mesu};
+        const bool own_L = (v[k] & kNbOwnRight) == 0;
+        const Prim6 L = own_L ? own : nb[k];
+        Prim6 Rs;
+        if (v[k] >= 0)
+            Rs = own_L ? nb[k] : own;
+        else
+            Rs = euler_ghost(L, nrm, v[k] & kNbIndex, quirk, 0.);
+        double fl[5];
+        euler_flux(L, Rs, n, mesu, gamma, quirk, fl);
+#pragma unroll
+        for (int q = 0; q < 5; q++) r[q] += own_L ? -fl[q] : fl[q];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_euler_step_packed(int C, EulerTetK t, const double *__restrict__ Rin,
+                                                               double *__restrict__ Rout, double gamma, double cfl,
+                                                               int quirk, DtRing ring, double *__restrict__ dtpart)
+{
+    int i = xcd_block() * blockDim.x + threadIdx.x;
+    ring_advance(ring);
+    double d = 1e300;
+    if (i < C) {
+        const int4 f4 = t.cell_faces[i];
+        const int4 v4 = t.cell_nb[i];
+        const unsigned ord = t.cell_rank[i];
+        const double2 *rp = reinterpret_cast<const double2 *>(Rin + 8 * (size_t)i);
+        const double2 a = rp[0], b = rp[1], e = rp[2], vv = rp[3];
+        slot_i fs, vs;
+        fs[0] = f4.x;
+        fs[1] = f4.y;
+        fs[2] = f4.z;
+        fs[3] = f4.w;
+        vs[0] = v4.x;
+        vs[1] = v4.y;
+        vs[2] = v4.z;
+        vs[3] = v4.w;
+        slot_i fk, vk;  // in increasing original face id
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            fk[k] = fs[(ord >> (2 * k)) & 3];
+            vk[k] = vs[(ord >> (2 * k)) & 3];
+        }
+        const Prim6 own{a.x, a.y, b.x, b.y, e.x, e.y};
+        const double vol = vv.x;
+        double r[5] = {0., 0., 0., 0., 0.};
+#pragma unroll 1
+        for (int k = 0; k < 4; k += 2)
+            euler_face_pair_packed(t, i, Rin, own, k ? fk[2] : fk[0], k ? fk[3] : fk[1], k ? vk[2] : vk[0],
+                                   k ? vk[3] : vk[1], gamma, quirk, r);
+        // time_step_euler_3d of the new state, faces in slot order; their geometry (in cache
+        // since the fluxes) loads before the record is stored
+        double nx[4], ny[4], nz[4], ms[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) load_geo(t.face_geo, fs[j], nx[j], ny[j], nz[j], ms[j]);
+        double u[6] = {own.rho, own.P, own.ru, own.rv, own.rw, own.rE};
+        euler_update_cell(u, r, gamma, ring.cur[0], vol);
+        double2 *wp = reinterpret_cast<double2 *>(Rout + 8 * (size_t)i);
+        wp[0] = make_double2(u[0], u[1]);
+        wp[1] = make_double2(u[2], u[3]);
+        wp[2] = make_double2(u[4], u[5]);
+        wp[3] = make_double2(vol, 0.);
+        double lam = 0.;
+        const double velson = sqrt(gamma * fabs(u[1] / u[0]));
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double u_n = fabs((u[2] * nx[j] + u[3] * ny[j] + u[4] * nz[j]) / u[0]) /
+                               sqrt(nx[j] * nx[j] + ny[j] * ny[j] + nz[j] * nz[j]);
+            const double lam_convect = u_n + velson;
+            lam += lam_convect * ms[j];
+        }
+        d = cfl * vol / lam;
+    }
+    wave_min_part(dtpart, (size_t)i, d);
+}
+
+__global__ void __launch_bounds__(kBlock) k_euler_pack(int C, const double *__restrict__ U,
+                                                        const double *__restrict__ vol, double *__restrict__ rec)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const size_t n = C;
+    double2 *w = reinterpret_cast<double2 *>(rec + 8 * (size_t)i);
+    w[0] = make_double2(U[i], U[n + i]);
+    w[1] = make_double2(U[2 * n + i], U[3 * n + i]);
+    w[2] = make_double2(U[4 * n + i], U[5 * n + i]);
+    w[3] = make_double2(vol[i], 0.);
+}
+
+__global__ void __launch_bounds__(kBlock) k_euler_unpack(int C, const double *__restrict__ rec,
+                                                          double *__restrict__ U)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const size_t n = C;
+    const double2 *p = reinterpret_cast<const double2 *>(rec + 8 * (size_t)i);
+    const double2 a = p[0], b = p[1], e = p[2];
+    U[i] = a.x;
+    U[n + i] = a.y;
+    U[2 * n + i] = b.x;
+    U[3 * n + i] = b.y;
+    U[4 * n + i] = e.x;
+    U[5 * n + i] = e.y;
+}
+
 // The fused step on face tiles (mfv_face_tiles): the block stages the states of its cells and
 // halo in LDS ([6][max_items]), evaluates every face of its tile once into LDS ([5][max_faces]),
 // then each cell adds its faces' fluxes in the residual order (increasing face id), updates
@@ -578,5 +739,146 @@ extern "C" int mfv_euler3d_step_tiled(const mfv_mesh *m, const mfv_face_tiles *t
         break;
     }
     if (int rc = check_launch("k_euler_step_tiled")) return rc;
+    return dt_finish(part, nw, r.next, st);
+}
+
+extern "C" void mfv_euler_tet_host_free(mfv_euler_tet_host *t)
+{
+    if (!t) return;
+    free(t->cell_faces);
+    free(t->cell_nb);
+    free(t->cell_rank);
+    free(t->face_geo);
+    *t = mfv_euler_tet_host{};
+}
+
+extern "C" int mfv_euler_tet_build(int C, int F, const int32_t *cell_faces, const int32_t *face_cells,
+                                   const int32_t *face_name, const double *face_normal, const double *face_mesure,
+                                   mfv_euler_tet_host *out)
+{
+    MFV_REQUIRE(out, "out is NULL");
+    *out = mfv_euler_tet_host{};
+    MFV_REQUIRE(C >= 0 && F >= 0 && C <= kNbIndex, "sizes out of range (C %d, F %d)", C, F);
+    MFV_REQUIRE((cell_faces || C == 0) && ((face_cells && face_name && face_normal && face_mesure) || F == 0),
+                "NULL table");
+    mfv_euler_tet_host h{};
+    h.nb_cells = C;
+    h.nb_faces = F;
+    h.cell_faces = static_cast<int32_t *>(malloc(sizeof(int32_t) * 4 * (size_t)(C > 0 ? C : 1)));
+    h.cell_nb = static_cast<int32_t *>(malloc(sizeof(int32_t) * 4 * (size_t)(C > 0 ? C : 1)));
+    h.cell_rank = static_cast<uint8_t *>(malloc((size_t)(C > 0 ? C : 1)));
+    h.face_geo = static_cast<double *>(malloc(sizeof(double) * 4 * (size_t)(F > 0 ? F : 1)));
+    std::vector<int32_t> newid((size_t)F, -1);
+    auto fail = [&](int rc) {
+        mfv_euler_tet_host_free(&h);
+        return rc;
+    };
+    if (!h.cell_faces || !h.cell_nb || !h.cell_rank || !h.face_geo) {
+        set_error("mfv_euler_tet_build: out of host memory");
+        return fail(MFV_ENOMEM);
+    }
+    for (int f = 0; f < F; f++) {
+        const int32_t L = face_cells[2 * (size_t)f], R = face_cells[2 * (size_t)f + 1], nm = face_name[f];
+        if (L < 0 || L >= C || nm < 0 || nm > kNbIndex || R >= C || (nm == 0 && R < 0)) {
+            set_error("face %d: cells (%d, %d), name %d: an inner face (name 0) needs two cells, a boundary face "
+                      "a left cell and a name in [1, 2^30)", f, L, R, nm);
+            return fail(MFV_EINVAL);
+        }
+    }
+    int32_t next = 0;
+    for (int i = 0; i < C; i++) {
+        int32_t o[4];
+        for (int s = 0; s < 4; s++) {
+            o[s] = cell_faces[(size_t)s * C + i];
+            if (o[s] < 0 || o[s] >= F) {
+                set_error("cell %d slot %d: face %d outside [0, %d) (the packed Euler step needs four faces per "
+                          "cell)", i, s, o[s], F);
+                return fail(MFV_EINVAL);
+            }
+            const int32_t f = o[s], L = face_cells[2 * (size_t)f], R = face_cells[2 * (size_t)f + 1];
+            if (L != i && R != i) {
+                set_error("cell %d slot %d: the cell is no side of its face %d (cells %d, %d)", i, s, f, L, R);
+                return fail(MFV_EINVAL);
+            }
+            const int32_t right = L == i ? 0 : kNbOwnRight;
+            h.cell_nb[4 * (size_t)i + s] =
+                face_name[f] == 0 ? ((L == i ? R : L) | right) : (kNbBoundary | right | face_name[f]);
+            if (newid[f] < 0) newid[f] = next++;
+            h.cell_faces[4 * (size_t)i + s] = newid[f];
+        }
+        int sl[4] = {0, 1, 2, 3};  // slots by increasing original face id (insertion sort, stable)
+        for (int a = 1; a < 4; a++)
+            for (int b = a; b > 0 && o[sl[b - 1]] > o[sl[b]]; b--) std::swap(sl[b - 1], sl[b]);
+        for (int a = 1; a < 4; a++)
+            if (o[sl[a - 1]] == o[sl[a]]) {
+                set_error("cell %d holds face %d twice", i, o[sl[a]]);
+                return fail(MFV_EINVAL);
+            }
+        h.cell_rank[i] = (uint8_t)(sl[0] | (sl[1] << 2) | (sl[2] << 4) | (sl[3] << 6));
+    }
+    for (int f = 0; f < F; f++)  // faces no cell reaches go last
+        if (newid[f] < 0) newid[f] = next++;
+    for (int f = 0; f < F; f++) {
+        const size_t nf = (size_t)newid[f];
+        for (int k = 0; k < 3; k++) h.face_geo[4 * nf + k] = face_normal[3 * (size_t)f + k];
+        h.face_geo[4 * nf + 3] = face_mesure[f];
+    }
+    *out = h;
+    return MFV_OK;
+}
+
+static int check_euler_tet(const mfv_mesh *m, const mfv_euler_tet *t)
+{
+    MFV_REQUIRE(t != nullptr, "packed Euler tables are NULL");
+    MFV_REQUIRE(t->nb_cells == m->nb_cells && t->nb_faces == m->nb_faces,
+                "packed Euler tables were built for another mesh (%d cells, %d faces; the mesh has %d, %d)",
+                t->nb_cells, t->nb_faces, m->nb_cells, m->nb_faces);
+    MFV_REQUIRE(t->cell_faces && t->cell_nb && t->cell_rank && t->face_geo, "packed Euler tables have NULL arrays");
+    MFV_REQUIRE(((uintptr_t)t->cell_faces & 15) == 0 && ((uintptr_t)t->cell_nb & 15) == 0 &&
+                    ((uintptr_t)t->face_geo & 15) == 0,
+                "packed Euler tables are misaligned");
+    return MFV_OK;
+}
+
+extern "C" int mfv_euler3d_pack(const mfv_mesh *m, const double *U, double *rec, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(U && rec && ((uintptr_t)rec & 15) == 0, "NULL or misaligned array");
+    hipLaunchKernelGGL(k_euler_pack, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), m->nb_cells,
+                       U, m->cell_volume, rec);
+    return check_launch("k_euler_pack");
+}
+
+extern "C" int mfv_euler3d_unpack(const mfv_mesh *m, const double *rec, double *U, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(U && rec && ((uintptr_t)rec & 15) == 0, "NULL or misaligned array");
+    hipLaunchKernelGGL(k_euler_unpack, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                       m->nb_cells, rec, U);
+    return check_launch("k_euler_unpack");
+}
+
+extern "C" int mfv_euler3d_step_packed(const mfv_mesh *m, const mfv_euler_tet *t, const double *rec_in,
+                                       double *rec_out, double gamma, double cfl, int quirk, double *dt_ring_p,
+                                       int step, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    MFV_REQUIRE(m->dim == 3, "the packed Euler step needs a 3D mesh");
+    if (int rc = check_euler_tet(m, t)) return rc;
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(rec_in && rec_out && rec_in != rec_out && dt_ring_p, "NULL or aliased array");
+    MFV_REQUIRE(((uintptr_t)rec_in & 15) == 0 && ((uintptr_t)rec_out & 15) == 0, "misaligned records");
+    DtRing r = dt_ring(dt_ring_p, step);
+    const size_t nw = waves_for(m->nb_cells);
+    hipStream_t st = as_stream(stream);
+    double *part = dt_scratch(nw, st);
+    if (!part) return MFV_ENOMEM;
+    const EulerTetK tk{reinterpret_cast<const int4 *>(t->cell_faces), reinterpret_cast<const int4 *>(t->cell_nb),
+                       t->cell_rank, t->face_geo};
+    hipLaunchKernelGGL(k_euler_step_packed, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, st, m->nb_cells, tk,
+                       rec_in, rec_out, gamma, cfl, quirk, r, part);
+    if (int rc = check_launch("k_euler_step_packed")) return rc;
     return dt_finish(part, nw, r.next, st);
 }

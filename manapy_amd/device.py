@@ -531,6 +531,61 @@ class FaceTiles(object):
         return 48 * self.host['max_items'] + 40 * self.host['max_faces']
 
 
+def euler_tet_native(cell_faces, face_cells, face_name, face_normal, face_mesure):
+    """Tables of the packed Euler step (mfv_euler_tet_build): faces renumbered by first use
+    along the cell order, each cell's four faces in slot order (new ids), what lies across each
+    (``cell_nb``: bit 31 boundary, bit 30 the cell is the right side, bits 0-29 the other cell or
+    the boundary name) and the slot order of their original ids. cell_faces [4][C], face_cells [F][2], face_name [F], face_normal [F][3],
+    face_mesure [F]."""
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    cf, fc, fn = i32(cell_faces), i32(face_cells), i32(face_name)
+    nrm, ms = f64(face_normal), f64(face_mesure)
+    MF, C = cf.shape
+    F = fc.shape[0]
+    if MF != 4:
+        raise ValueError("the packed Euler step needs tetrahedra (4 faces per cell), the mesh has %d slots" % MF)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    h = _lib.MfvEulerTetHost()
+    _lib.call('mfv_euler_tet_build', int(C), int(F), P(cf), P(fc), P(fn), P(nrm), P(ms), ctypes.byref(h))
+    try:
+        def take(ptr, ctype, n, dtype):
+            a = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), (max(n, 1),))[:n]
+            return np.array(a, dtype=dtype)
+        out = {'C': C, 'F': F,
+               'cell_faces': take(h.cell_faces, ctypes.c_int32, 4 * C, np.int32).reshape(C, 4),
+               'cell_nb': take(h.cell_nb, ctypes.c_int32, 4 * C, np.int32).reshape(C, 4),
+               'cell_rank': take(h.cell_rank, ctypes.c_uint8, C, np.uint8),
+               'face_geo': take(h.face_geo, ctypes.c_double, 4 * F, np.float64).reshape(F, 4)}
+    finally:
+        _lib.call('mfv_euler_tet_host_free', ctypes.byref(h))
+    return out
+
+
+class EulerTet(object):
+    """Device copy of the packed Euler tables (mfv_euler_tet) of a tetrahedral DeviceMesh."""
+
+    ARRAYS = ('cell_faces', 'cell_nb', 'cell_rank', 'face_geo')
+
+    def __init__(self, dmesh):
+        import torch
+        H = dmesh.host
+        self.host = euler_tet_native(H['cell_faces'], H['face_cells'], H['face_name'], H['face_normal'],
+                                     H['face_mesure'])
+        self.t = {}
+        d = _lib.MfvEulerTet()
+        d.nb_cells, d.nb_faces = self.host['C'], self.host['F']
+        for k in self.ARRAYS:
+            a = np.ascontiguousarray(self.host[k])
+            self.t[k] = torch.from_numpy(a if a.size else np.zeros(4, dtype=a.dtype)).to(dmesh.device)
+            setattr(d, k, self.t[k].data_ptr())
+        self.desc = d
+
+    @property
+    def ptr(self):
+        return ctypes.byref(self.desc)
+
+
 class SwTiles(object):
     """Device copy of a tiles layout and its mfv_sw_tiles descriptor (stage=True copies every
     cell state a tile reads into LDS before reconstructing)."""

@@ -97,23 +97,41 @@ class EulerSolver3D(object):
         self.compute_new_val()
         return self.dt
 
-    def run(self, nsteps):
+    def run(self, nsteps, packed=None):
         """``nsteps`` fused steps (ghosts computed on the fly) with dt on the device. Under
-        ``quirk`` the unset rhowg of slip faces is taken as zero (zero-initialised ghosts)."""
+        ``quirk`` the unset rhowg of slip faces is taken as zero (zero-initialised ghosts).
+        On tetrahedra (``packed`` None or True) the steps run on 64-byte cell records and
+        faces renumbered by first use (mfv_euler3d_step_packed, same bits)."""
         import torch
         m = self._m
         s = stream_ptr()
         _lib.call('mfv_euler3d_time_step', m.ptr, self.U.data_ptr(), self.cfl, self.gamma, self._clock.dt0.data_ptr(),
                   None, s)
         self._clock.start()
-        a = self.U
-        b = torch.empty_like(a)
-        for k in range(nsteps):
-            _lib.call('mfv_euler3d_step_fused', m.ptr, a.data_ptr(), b.data_ptr(), self.gamma, self.cfl, self.quirk,
-                      self._clock.ring.data_ptr(), k, s)
-            a, b = b, a
-        if nsteps % 2 == 1:
-            self.U.copy_(a)
+        if packed is None:
+            packed = m.MF == 4
+        if packed:
+            from ..device import EulerTet
+            et = getattr(m, '_euler_tet', None)
+            if et is None:
+                et = m._euler_tet = EulerTet(m)
+            a = torch.empty((int(m.C), 8), dtype=torch.float64, device=self.U.device)
+            b = torch.empty_like(a)
+            _lib.call('mfv_euler3d_pack', m.ptr, self.U.data_ptr(), a.data_ptr(), s)
+            for k in range(nsteps):
+                _lib.call('mfv_euler3d_step_packed', m.ptr, et.ptr, a.data_ptr(), b.data_ptr(), self.gamma, self.cfl,
+                          self.quirk, self._clock.ring.data_ptr(), k, s)
+                a, b = b, a
+            _lib.call('mfv_euler3d_unpack', m.ptr, a.data_ptr(), self.U.data_ptr(), s)
+        else:
+            a = self.U
+            b = torch.empty_like(a)
+            for k in range(nsteps):
+                _lib.call('mfv_euler3d_step_fused', m.ptr, a.data_ptr(), b.data_ptr(), self.gamma, self.cfl,
+                          self.quirk, self._clock.ring.data_ptr(), k, s)
+                a, b = b, a
+            if nsteps % 2 == 1:
+                self.U.copy_(a)
         if nsteps:
             self.dt = float(self._clock.ring[(nsteps - 1) % 3])
         return self._clock.time

@@ -349,6 +349,53 @@ def test_euler_tiled_40_steps(torch, quirk, T, n, d3, m3, oracle_lib):
     same(res[0][1], res[1][1], "dt ring")
 
 
+@pytest.mark.parametrize("quirk,n,renumber", [(0, 6, False), (1, 6, False), (1, 11, True), (0, 7, True)])
+def test_euler_packed_40_steps(torch, quirk, n, renumber, d3, m3, oracle_lib):
+    """The packed Euler step (64-byte cell records, faces renumbered by first use) gives the
+    oracle's bits after 40 steps, and the per-cell fused step's dt ring; pack then unpack is
+    the identity and the records carry the cell volume."""
+    from manapy_amd.device import DeviceMesh, EulerTet
+    if n == 6 and not renumber:
+        d, m = d3, m3
+    else:
+        d = make_tet_domain(n, 0.08, 5)
+        if renumber:
+            from manapy_amd.renumber import renumbered
+            d = renumbered(d)
+        m = DeviceMesh(d, lsq=False)
+    C, F = int(d.nbcells), int(d.nbfaces)
+    U0 = sod(d)
+    U = U0.copy()
+    G_ = np.zeros((6, F))
+    for _ in range(40):
+        oracle_lib.euler_step(d, U, G_, 0.8, 1.4, quirk)
+    et = EulerTet(m)
+    tU = dev(torch, U0)
+    R = [torch.full((C, 8), np.nan, dtype=torch.float64, device='cuda') for _ in range(2)]
+    call('mfv_euler3d_pack', m.ptr, P(tU), P(R[0]))
+    rec = host(R[0])
+    same(rec[:, :6].T, U0, "packed state")
+    same(rec[:, 6], d.cells.volume, "packed volume")
+    assert np.all(rec[:, 7] == 0.)
+    back = dev(torch, np.zeros((6, C)))
+    call('mfv_euler3d_unpack', m.ptr, P(R[0]), P(back))
+    same(host(back), U0, "unpack(pack(U))")
+    dt0 = dev(torch, np.zeros(1))
+    rings = [dev(torch, np.zeros(4)) for _ in range(2)]
+    for ring in rings:
+        call('mfv_euler3d_time_step', m.ptr, P(tU), 0.8, 1.4, P(dt0), None)
+        call('mfv_dt_ring_init', P(ring), P(dt0))
+    S = [dev(torch, U0), dev(torch, np.zeros((6, C)))]
+    for s in range(40):
+        call('mfv_euler3d_step_packed', m.ptr, et.ptr, P(R[s % 2]), P(R[(s + 1) % 2]), 1.4, 0.8, quirk, P(rings[0]),
+             s)
+        call('mfv_euler3d_step_fused', m.ptr, P(S[s % 2]), P(S[(s + 1) % 2]), 1.4, 0.8, quirk, P(rings[1]), s)
+    call('mfv_euler3d_unpack', m.ptr, P(R[0]), P(back))
+    same(host(back), U, "packed U after 40 steps")
+    same(host(R[0])[:, 6], d.cells.volume, "volume carried")
+    same(host(rings[0]), host(rings[1]), "dt ring")
+
+
 def test_sw_packed_60_steps(torch, sw_case, oracle_lib):
     """The packed path (one 64-byte record per cell, MUSCL values per face slot) gives the
     same bits as the reference step sequence."""
