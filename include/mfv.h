@@ -437,19 +437,17 @@ void mfv_face_tiles_host_free(mfv_face_tiles_host *t);
 int mfv_euler3d_step_tiled(const mfv_mesh *m, const mfv_face_tiles *t, const double *U_in, double *U_out,
                            double gamma, double cfl, int quirk, double *dt_ring, int step, void *stream);
 
-/* Tables of the packed Euler step on tetrahedra. The faces are renumbered in the order the
- * cells, in their storage order, first reach them: a wave's new faces are contiguous and a
- * face's second reader finds it in cache. A face is one 32-byte geometry record. A cell keeps,
- * per slot, its face (new id) and what lies across it, so that the neighbour records and the
- * face records load together; and the slot order of its faces' original ids (the reference's
- * face-loop order, the residual's summation order). */
+/* Tables of the packed Euler step on tetrahedra, per cell and in the order of the cell's
+ * faces by increasing original id (the reference's face-loop order, the residual's summation
+ * order): what lies across each face and its geometry, so that a wave reads them with
+ * contiguous loads and only the neighbours' records are gathered; plus the slot of each face
+ * in cells.faceid (the time step sums over the faces in slot order). */
 typedef struct mfv_euler_tet {
     int32_t nb_cells, nb_faces;
-    const int32_t *cell_faces; /* [C][4] the cell's faces in slot order (cells.faceid), new ids */
-    const int32_t *cell_nb;    /* [C][4] per slot: bit 31 boundary face, bit 30 the cell is the face's right
-                                * side, bits 0-29 the cell across (inner face) or faces.name (boundary face) */
-    const uint8_t *cell_rank;  /* [C] bits 2k..2k+1: slot of the cell's k-th smallest original face id */
-    const double *face_geo;    /* [F][4] faces.normal (3), faces.mesure */
+    const int32_t *cell_nb;   /* [C][4] k-th face: bit 31 boundary face, bit 30 the cell is the face's right
+                               * side, bits 0-29 the cell across (inner face) or faces.name (boundary face) */
+    const uint8_t *cell_rank; /* [C] bits 2k..2k+1: slot of the k-th face in cells.faceid */
+    const double *cell_geo;   /* [4][2][C][2] k-th face: (normal x, normal y), (normal z, mesure) */
 } mfv_euler_tet;
 
 /* Host-side construction (no GPU needed) from host copies of the mfv_mesh tables: cell_faces
@@ -459,10 +457,9 @@ typedef struct mfv_euler_tet {
  * mfv_euler_tet_host_free. */
 typedef struct mfv_euler_tet_host {
     int32_t nb_cells, nb_faces;
-    int32_t *cell_faces;
     int32_t *cell_nb;
     uint8_t *cell_rank;
-    double *face_geo;
+    double *cell_geo;
 } mfv_euler_tet_host;
 
 int mfv_euler_tet_build(int C, int F, const int32_t *cell_faces, const int32_t *face_cells, const int32_t *face_name,
@@ -475,7 +472,7 @@ int mfv_euler3d_pack(const mfv_mesh *m, const double *U, double *rec, void *stre
 int mfv_euler3d_unpack(const mfv_mesh *m, const double *rec, double *U, void *stream);
 
 /* The fused Euler step on packed records (tetrahedra): each neighbour is one 64-byte record
- * and each face one 32-byte record in the cells' order; the volume travels in the record
+ * and the cell's face tables are read contiguously; the volume travels in the record
  * (rec_out[c][6] = rec_in[c][6]). Same expressions and order as mfv_euler3d_step_fused, so
  * the same bits. Replaces explicitscheme_euler_3d_rusanov (fvm_utils3d.py:1083-1170) +
  * update_euler_3d_fvc (:1316-1327) + time_step_euler_3d (:809-825) in one launch. */

@@ -68,13 +68,13 @@ class MfvFaceTilesHost(ctypes.Structure):
 class MfvEulerTet(ctypes.Structure):
     """Mirror of ``struct mfv_euler_tet`` (include/mfv.h)."""
     _fields_ = [('nb_cells', ctypes.c_int32), ('nb_faces', ctypes.c_int32)] + \
-        [(k, P) for k in ('cell_faces', 'cell_nb', 'cell_rank', 'face_geo')]
+        [(k, P) for k in ('cell_nb', 'cell_rank', 'cell_geo')]
 
 
 class MfvEulerTetHost(ctypes.Structure):
     """Mirror of ``struct mfv_euler_tet_host`` (include/mfv.h)."""
     _fields_ = [('nb_cells', ctypes.c_int32), ('nb_faces', ctypes.c_int32)] + \
-        [(k, P) for k in ('cell_faces', 'cell_nb', 'cell_rank', 'face_geo')]
+        [(k, P) for k in ('cell_nb', 'cell_rank', 'cell_geo')]
 
 
 class MfvSwPacked(ctypes.Structure):

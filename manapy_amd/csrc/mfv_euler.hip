@@ -338,14 +338,15 @@ __global__ void __launch_bounds__(kBlock) k_euler_step_tet(Mesh m, const double 
 }
 
 // The fused step on packed records (mfv_euler_tet): the arithmetic and summation orders of
-// k_euler_step_tet; a neighbour is one 64-byte record [C][8] and a face one 32-byte geometry
-// record, in faces numbered by first use along the cell order. The cell's own tables name
-// what lies across each face, so a face's geometry and its neighbour's record load together.
+// k_euler_step_tet; a neighbour is one 64-byte record [C][8]; the cell's across-codes and
+// face geometry are stored per cell in the residual order, so a wave reads them with
+// contiguous loads and gathers only the neighbours' records. The geometry is parked in LDS
+// by slot for the time step, which sums over the faces in slot order.
 struct EulerTetK {
-    const int4 *__restrict__ cell_faces;
     const int4 *__restrict__ cell_nb;
     const uint8_t *__restrict__ cell_rank;
-    const double *__restrict__ face_geo;
+    const double2 *__restrict__ cell_geo;
+    size_t C;
 };
 
 constexpr int kNbBoundary = int(0x80000000u);
@@ -359,35 +360,34 @@ __device__ __forceinline__ Prim6 load_rec(const double *__restrict__ R, int c)
     return Prim6{a.x, a.y, b.x, b.y, e.x, e.y};
 }
 
-__device__ __forceinline__ void load_geo(const double *__restrict__ G, int f, double &nx, double &ny, double &nz,
-                                         double &ms)
-{
-    const double2 *p = reinterpret_cast<const double2 *>(G + 4 * (size_t)f);
-    const double2 a = p[0], b = p[1];
-    nx = a.x;
-    ny = a.y;
-    nz = b.x;
-    ms = b.y;
-}
-
-// Two faces in residual order (face ids f0, f1; across codes v0, v1): their geometry and the
-// neighbour records load together, then the fluxes are added to r in that order.
+// Faces k0 and k0 + 1 of the residual order (across codes v0, v1): their geometry and the
+// neighbour records load together, the geometry is parked in LDS at the faces' slots
+// (sg[slot][component][thread]), then the fluxes are added to r in that order.
 __device__ __forceinline__ void euler_face_pair_packed(const EulerTetK &t, int i, const double *__restrict__ R,
-                                                       const Prim6 &own, int f0, int f1, int v0, int v1,
-                                                       double gamma, int quirk, double (&r)[5])
+                                                       const Prim6 &own, int k0, int v0, int v1, unsigned ord,
+                                                       double gamma, int quirk, double (&r)[5], double *sg)
 {
-    const int f[2] = {f0, f1}, v[2] = {v0, v1};
-    double nx[2], ny[2], nz[2], ms[2];
+    const int v[2] = {v0, v1};
+    double2 g01[2], g23[2];
     Prim6 nb[2];  // the cell across each face (the own cell on a boundary face: not used)
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-        load_geo(t.face_geo, f[k], nx[k], ny[k], nz[k], ms[k]);
+        g01[k] = t.cell_geo[(size_t)(2 * (k0 + k)) * t.C + i];
+        g23[k] = t.cell_geo[(size_t)(2 * (k0 + k) + 1) * t.C + i];
         nb[k] = load_rec(R, v[k] >= 0 ? (v[k] & kNbIndex) : i);
     }
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-        const double nrm[3] = {nx[k], ny[k], nz[k]};
-        const double mesu = ms[k];
+        const int so = ((ord >> (2 * (k0 + k))) & 3) * 4 * kBlock;
+        sg[so] = g01[k].x;
+        sg[so + kBlock] = g01[k].y;
+        sg[so + 2 * kBlock] = g23[k].x;
+        sg[so + 3 * kBlock] = g23[k].y;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const double nrm[3] = {g01[k].x, g01[k].y, g23[k].x};
+        const double mesu = g23[k].y;
         const double n[3] = {nrm[0] / mesu, nrm[1] / mesu, nrm[2] / mesu};
         const bool own_L = (v[k] & kNbOwnRight) == 0;
         const Prim6 L = own_L ? own : nb[k];
@@ -403,46 +403,26 @@ __device__ __forceinline__ void euler_face_pair_packed(const EulerTetK &t, int i
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_euler_step_packed(int C, EulerTetK t, const double *__restrict__ Rin,
+__global__ void __launch_bounds__(kBlock) k_euler_step_packed(EulerTetK t, const double *__restrict__ Rin,
                                                                double *__restrict__ Rout, double gamma, double cfl,
                                                                int quirk, DtRing ring, double *__restrict__ dtpart)
 {
+    __shared__ double sgeo[16 * kBlock];
     int i = xcd_block() * blockDim.x + threadIdx.x;
     ring_advance(ring);
     double d = 1e300;
-    if (i < C) {
-        const int4 f4 = t.cell_faces[i];
+    if (i < (int)t.C) {
         const int4 v4 = t.cell_nb[i];
         const unsigned ord = t.cell_rank[i];
         const double2 *rp = reinterpret_cast<const double2 *>(Rin + 8 * (size_t)i);
         const double2 a = rp[0], b = rp[1], e = rp[2], vv = rp[3];
-        slot_i fs, vs;
-        fs[0] = f4.x;
-        fs[1] = f4.y;
-        fs[2] = f4.z;
-        fs[3] = f4.w;
-        vs[0] = v4.x;
-        vs[1] = v4.y;
-        vs[2] = v4.z;
-        vs[3] = v4.w;
-        slot_i fk, vk;  // in increasing original face id
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            fk[k] = fs[(ord >> (2 * k)) & 3];
-            vk[k] = vs[(ord >> (2 * k)) & 3];
-        }
         const Prim6 own{a.x, a.y, b.x, b.y, e.x, e.y};
         const double vol = vv.x;
+        double *sg = sgeo + threadIdx.x;
         double r[5] = {0., 0., 0., 0., 0.};
 #pragma unroll 1
         for (int k = 0; k < 4; k += 2)
-            euler_face_pair_packed(t, i, Rin, own, k ? fk[2] : fk[0], k ? fk[3] : fk[1], k ? vk[2] : vk[0],
-                                   k ? vk[3] : vk[1], gamma, quirk, r);
-        // time_step_euler_3d of the new state, faces in slot order; their geometry (in cache
-        // since the fluxes) loads before the record is stored
-        double nx[4], ny[4], nz[4], ms[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) load_geo(t.face_geo, fs[j], nx[j], ny[j], nz[j], ms[j]);
+            euler_face_pair_packed(t, i, Rin, own, k, k ? v4.z : v4.x, k ? v4.w : v4.y, ord, gamma, quirk, r, sg);
         double u[6] = {own.rho, own.P, own.ru, own.rv, own.rw, own.rE};
         euler_update_cell(u, r, gamma, ring.cur[0], vol);
         double2 *wp = reinterpret_cast<double2 *>(Rout + 8 * (size_t)i);
@@ -450,14 +430,16 @@ __global__ void __launch_bounds__(kBlock) k_euler_step_packed(int C, EulerTetK t
         wp[1] = make_double2(u[2], u[3]);
         wp[2] = make_double2(u[4], u[5]);
         wp[3] = make_double2(vol, 0.);
+        // time_step_euler_3d of the new state, faces in slot order
         double lam = 0.;
         const double velson = sqrt(gamma * fabs(u[1] / u[0]));
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const double u_n = fabs((u[2] * nx[j] + u[3] * ny[j] + u[4] * nz[j]) / u[0]) /
-                               sqrt(nx[j] * nx[j] + ny[j] * ny[j] + nz[j] * nz[j]);
+            const double nx = sg[j * 4 * kBlock], ny = sg[j * 4 * kBlock + kBlock];
+            const double nz = sg[j * 4 * kBlock + 2 * kBlock], ms = sg[j * 4 * kBlock + 3 * kBlock];
+            const double u_n = fabs((u[2] * nx + u[3] * ny + u[4] * nz) / u[0]) / sqrt(nx * nx + ny * ny + nz * nz);
             const double lam_convect = u_n + velson;
-            lam += lam_convect * ms[j];
+            lam += lam_convect * ms;
         }
         d = cfl * vol / lam;
     }
@@ -745,10 +727,9 @@ extern "C" int mfv_euler3d_step_tiled(const mfv_mesh *m, const mfv_face_tiles *t
 extern "C" void mfv_euler_tet_host_free(mfv_euler_tet_host *t)
 {
     if (!t) return;
-    free(t->cell_faces);
     free(t->cell_nb);
     free(t->cell_rank);
-    free(t->face_geo);
+    free(t->cell_geo);
     *t = mfv_euler_tet_host{};
 }
 
@@ -764,16 +745,15 @@ extern "C" int mfv_euler_tet_build(int C, int F, const int32_t *cell_faces, cons
     mfv_euler_tet_host h{};
     h.nb_cells = C;
     h.nb_faces = F;
-    h.cell_faces = static_cast<int32_t *>(malloc(sizeof(int32_t) * 4 * (size_t)(C > 0 ? C : 1)));
-    h.cell_nb = static_cast<int32_t *>(malloc(sizeof(int32_t) * 4 * (size_t)(C > 0 ? C : 1)));
-    h.cell_rank = static_cast<uint8_t *>(malloc((size_t)(C > 0 ? C : 1)));
-    h.face_geo = static_cast<double *>(malloc(sizeof(double) * 4 * (size_t)(F > 0 ? F : 1)));
-    std::vector<int32_t> newid((size_t)F, -1);
+    const size_t c1 = (size_t)(C > 0 ? C : 1);
+    h.cell_nb = static_cast<int32_t *>(malloc(sizeof(int32_t) * 4 * c1));
+    h.cell_rank = static_cast<uint8_t *>(malloc(c1));
+    h.cell_geo = static_cast<double *>(malloc(sizeof(double) * 16 * c1));
     auto fail = [&](int rc) {
         mfv_euler_tet_host_free(&h);
         return rc;
     };
-    if (!h.cell_faces || !h.cell_nb || !h.cell_rank || !h.face_geo) {
+    if (!h.cell_nb || !h.cell_rank || !h.cell_geo) {
         set_error("mfv_euler_tet_build: out of host memory");
         return fail(MFV_ENOMEM);
     }
@@ -785,7 +765,7 @@ extern "C" int mfv_euler_tet_build(int C, int F, const int32_t *cell_faces, cons
             return fail(MFV_EINVAL);
         }
     }
-    int32_t next = 0;
+    const size_t CC = (size_t)C;
     for (int i = 0; i < C; i++) {
         int32_t o[4];
         for (int s = 0; s < 4; s++) {
@@ -795,16 +775,11 @@ extern "C" int mfv_euler_tet_build(int C, int F, const int32_t *cell_faces, cons
                           "cell)", i, s, o[s], F);
                 return fail(MFV_EINVAL);
             }
-            const int32_t f = o[s], L = face_cells[2 * (size_t)f], R = face_cells[2 * (size_t)f + 1];
+            const int32_t L = face_cells[2 * (size_t)o[s]], R = face_cells[2 * (size_t)o[s] + 1];
             if (L != i && R != i) {
-                set_error("cell %d slot %d: the cell is no side of its face %d (cells %d, %d)", i, s, f, L, R);
+                set_error("cell %d slot %d: the cell is no side of its face %d (cells %d, %d)", i, s, o[s], L, R);
                 return fail(MFV_EINVAL);
             }
-            const int32_t right = L == i ? 0 : kNbOwnRight;
-            h.cell_nb[4 * (size_t)i + s] =
-                face_name[f] == 0 ? ((L == i ? R : L) | right) : (kNbBoundary | right | face_name[f]);
-            if (newid[f] < 0) newid[f] = next++;
-            h.cell_faces[4 * (size_t)i + s] = newid[f];
         }
         int sl[4] = {0, 1, 2, 3};  // slots by increasing original face id (insertion sort, stable)
         for (int a = 1; a < 4; a++)
@@ -815,13 +790,18 @@ extern "C" int mfv_euler_tet_build(int C, int F, const int32_t *cell_faces, cons
                 return fail(MFV_EINVAL);
             }
         h.cell_rank[i] = (uint8_t)(sl[0] | (sl[1] << 2) | (sl[2] << 4) | (sl[3] << 6));
-    }
-    for (int f = 0; f < F; f++)  // faces no cell reaches go last
-        if (newid[f] < 0) newid[f] = next++;
-    for (int f = 0; f < F; f++) {
-        const size_t nf = (size_t)newid[f];
-        for (int k = 0; k < 3; k++) h.face_geo[4 * nf + k] = face_normal[3 * (size_t)f + k];
-        h.face_geo[4 * nf + 3] = face_mesure[f];
+        for (int k = 0; k < 4; k++) {
+            const size_t f = (size_t)o[sl[k]];
+            const int32_t L = face_cells[2 * f], R = face_cells[2 * f + 1], nm = face_name[f];
+            const int32_t right = L == i ? 0 : kNbOwnRight;
+            h.cell_nb[4 * (size_t)i + k] = nm == 0 ? ((L == i ? R : L) | right) : (kNbBoundary | right | nm);
+            double *g0 = h.cell_geo + 2 * ((size_t)(2 * k) * CC + i);
+            double *g1 = h.cell_geo + 2 * ((size_t)(2 * k + 1) * CC + i);
+            g0[0] = face_normal[3 * f];
+            g0[1] = face_normal[3 * f + 1];
+            g1[0] = face_normal[3 * f + 2];
+            g1[1] = face_mesure[f];
+        }
     }
     *out = h;
     return MFV_OK;
@@ -833,9 +813,8 @@ static int check_euler_tet(const mfv_mesh *m, const mfv_euler_tet *t)
     MFV_REQUIRE(t->nb_cells == m->nb_cells && t->nb_faces == m->nb_faces,
                 "packed Euler tables were built for another mesh (%d cells, %d faces; the mesh has %d, %d)",
                 t->nb_cells, t->nb_faces, m->nb_cells, m->nb_faces);
-    MFV_REQUIRE(t->cell_faces && t->cell_nb && t->cell_rank && t->face_geo, "packed Euler tables have NULL arrays");
-    MFV_REQUIRE(((uintptr_t)t->cell_faces & 15) == 0 && ((uintptr_t)t->cell_nb & 15) == 0 &&
-                    ((uintptr_t)t->face_geo & 15) == 0,
+    MFV_REQUIRE(t->cell_nb && t->cell_rank && t->cell_geo, "packed Euler tables have NULL arrays");
+    MFV_REQUIRE(((uintptr_t)t->cell_nb & 15) == 0 && ((uintptr_t)t->cell_geo & 15) == 0,
                 "packed Euler tables are misaligned");
     return MFV_OK;
 }
@@ -875,10 +854,10 @@ extern "C" int mfv_euler3d_step_packed(const mfv_mesh *m, const mfv_euler_tet *t
     hipStream_t st = as_stream(stream);
     double *part = dt_scratch(nw, st);
     if (!part) return MFV_ENOMEM;
-    const EulerTetK tk{reinterpret_cast<const int4 *>(t->cell_faces), reinterpret_cast<const int4 *>(t->cell_nb),
-                       t->cell_rank, t->face_geo};
-    hipLaunchKernelGGL(k_euler_step_packed, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, st, m->nb_cells, tk,
-                       rec_in, rec_out, gamma, cfl, quirk, r, part);
+    const EulerTetK tk{reinterpret_cast<const int4 *>(t->cell_nb), t->cell_rank,
+                       reinterpret_cast<const double2 *>(t->cell_geo), (size_t)m->nb_cells};
+    hipLaunchKernelGGL(k_euler_step_packed, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, st, tk, rec_in, rec_out,
+                       gamma, cfl, quirk, r, part);
     if (int rc = check_launch("k_euler_step_packed")) return rc;
     return dt_finish(part, nw, r.next, st);
 }

@@ -532,11 +532,12 @@ class FaceTiles(object):
 
 
 def euler_tet_native(cell_faces, face_cells, face_name, face_normal, face_mesure):
-    """Tables of the packed Euler step (mfv_euler_tet_build): faces renumbered by first use
-    along the cell order, each cell's four faces in slot order (new ids), what lies across each
-    (``cell_nb``: bit 31 boundary, bit 30 the cell is the right side, bits 0-29 the other cell or
-    the boundary name) and the slot order of their original ids. cell_faces [4][C], face_cells [F][2], face_name [F], face_normal [F][3],
-    face_mesure [F]."""
+    """Tables of the packed Euler step (mfv_euler_tet_build), per cell and in the order of its
+    faces by increasing original id: what lies across each face (``cell_nb``: bit 31 boundary,
+    bit 30 the cell is the right side, bits 0-29 the other cell or the boundary name), the
+    face's geometry (``cell_geo`` [4][2][C][2]: (nx, ny), (nz, mesure)), and the slot of each
+    face in cells.faceid (``cell_rank``, 2 bits per face). cell_faces [4][C], face_cells [F][2],
+    face_name [F], face_normal [F][3], face_mesure [F]."""
     i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
     f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
     cf, fc, fn = i32(cell_faces), i32(face_cells), i32(face_name)
@@ -553,10 +554,9 @@ def euler_tet_native(cell_faces, face_cells, face_name, face_normal, face_mesure
             a = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), (max(n, 1),))[:n]
             return np.array(a, dtype=dtype)
         out = {'C': C, 'F': F,
-               'cell_faces': take(h.cell_faces, ctypes.c_int32, 4 * C, np.int32).reshape(C, 4),
                'cell_nb': take(h.cell_nb, ctypes.c_int32, 4 * C, np.int32).reshape(C, 4),
                'cell_rank': take(h.cell_rank, ctypes.c_uint8, C, np.uint8),
-               'face_geo': take(h.face_geo, ctypes.c_double, 4 * F, np.float64).reshape(F, 4)}
+               'cell_geo': take(h.cell_geo, ctypes.c_double, 16 * C, np.float64).reshape(4, 2, C, 2)}
     finally:
         _lib.call('mfv_euler_tet_host_free', ctypes.byref(h))
     return out
@@ -565,7 +565,7 @@ def euler_tet_native(cell_faces, face_cells, face_name, face_normal, face_mesure
 class EulerTet(object):
     """Device copy of the packed Euler tables (mfv_euler_tet) of a tetrahedral DeviceMesh."""
 
-    ARRAYS = ('cell_faces', 'cell_nb', 'cell_rank', 'face_geo')
+    ARRAYS = ('cell_nb', 'cell_rank', 'cell_geo')
 
     def __init__(self, dmesh):
         import torch

@@ -1,9 +1,7 @@
-"""Tables of the packed Euler step (mfv_euler_tet_build): faces renumbered by first use along
-the cell order, each cell's faces in slot order under the new ids, the residual order equal
-to the ascending-original-id order of the device layout, and face records carrying the
-reference's normal, mesure, cells and boundary name."""
-import ctypes
-
+"""Tables of the packed Euler step (mfv_euler_tet_build): per cell, its faces in increasing
+original id (the residual order of the device layout), what lies across each (other cell or
+boundary name, and the side), the reference's normal and mesure of each, and the slot of each
+face in cells.faceid."""
 import numpy as np
 import pytest
 
@@ -18,37 +16,20 @@ def _tables(d):
 
 
 @pytest.mark.parametrize("n,renumber", [(4, False), (5, True)])
-def test_first_use_numbering_and_records(n, renumber):
+def test_residual_order_codes_and_geometry(n, renumber):
     d = make_tet_domain(n, 0.08, n)
     if renumber:
         from manapy_amd.renumber import renumbered
         d = renumbered(d)
     L, T = _tables(d)
-    C, F = int(d.nbcells), int(d.nbfaces)
+    C = int(d.nbcells)
     old = L['cell_faces'].T                     # [C][4] original ids, slot order
-    new = T['cell_faces']
-    # new[i][s] = newid[old[i][s]] for one permutation newid of the faces
-    newid = np.full(F, -1)
-    newid[old.ravel()] = new.ravel()
-    assert np.array_equal(np.sort(newid), np.arange(F))
-    assert np.array_equal(newid[old], new)
-    # first use: walking the cells in order, each face not seen before takes the next id
-    seen, nxt = set(), 0
-    for f in new.ravel():
-        if f not in seen:
-            assert f == nxt
-            seen.add(f)
-            nxt += 1
+    ranks = np.stack([(T['cell_rank'] >> (2 * k)) & 3 for k in range(4)], axis=1).astype(np.int64)
     # residual order = the device layout's ascending-original-id order
-    ranks = np.stack([(T['cell_rank'] >> (2 * k)) & 3 for k in range(4)], axis=1)
     assert np.array_equal(ranks, L['order_sorted'].T.astype(np.int64))
-    assert np.all(np.diff(np.take_along_axis(old, ranks, axis=1), axis=1) > 0)
-    # face records in the new order
-    perm = np.argsort(newid)                    # new -> old
-    assert np.array_equal(T['face_geo'][:, :3], L['face_normal'][perm])
-    assert np.array_equal(T['face_geo'][:, 3], L['face_mesure'][perm])
-    # what lies across each slot: the other cell (inner) or the name (boundary), and the side
-    fc, nm = L['face_cells'][old], L['face_name'][old]        # [C][4](2), [C][4]
+    fk = np.take_along_axis(old, ranks, axis=1)  # [C][4] faces in residual order
+    assert np.all(np.diff(fk, axis=1) > 0)
+    fc, nm = L['face_cells'][fk], L['face_name'][fk]
     me = np.arange(C)[:, None]
     right = fc[..., 1] == me
     assert np.all(right | (fc[..., 0] == me))
@@ -58,6 +39,12 @@ def test_first_use_numbering_and_records(n, renumber):
     want = np.where(nm == 0, np.where(right, fc[..., 0], fc[..., 1]), nm)
     assert np.array_equal(v & ((1 << 30) - 1), want)
     assert np.all(nm[fc[..., 1] < 0] >= 1)
+    g = T['cell_geo']                           # [4][2][C][2]
+    for k in range(4):
+        assert np.array_equal(g[k, 0, :, 0], L['face_normal'][fk[:, k], 0])
+        assert np.array_equal(g[k, 0, :, 1], L['face_normal'][fk[:, k], 1])
+        assert np.array_equal(g[k, 1, :, 0], L['face_normal'][fk[:, k], 2])
+        assert np.array_equal(g[k, 1, :, 1], L['face_mesure'][fk[:, k]])
 
 
 def test_builder_rejects_bad_meshes():
