@@ -334,9 +334,9 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
     t_loop, t_ev = time_steps(step, warmup, steps)
     MF = int(d.cells.faceid.shape[1] - 1)
     if best == 'packed':
-        # record read + write (64 B each: six conserved/primitive values, volume, pad), the four
-        # new face ids and the rank byte; per face (x F/C): geometry record 32 B + cell pair 8 B
-        bpc = (64 + 64 + 16 + 1) + (32 + 8) * F / C
+        # record read + write (64 B each: six conserved/primitive values, volume, pad), the
+        # per-cell across-codes (16 B), face geometry (4 x 32 B) and rank byte
+        bpc = 64 + 64 + 16 + 128 + 1
         kname = "k_euler_step_packed"
     else:
         # U read + write (6 doubles each), volume, face count; per face slot: order entry +

@@ -100,8 +100,8 @@ class EulerSolver3D(object):
     def run(self, nsteps, packed=None):
         """``nsteps`` fused steps (ghosts computed on the fly) with dt on the device. Under
         ``quirk`` the unset rhowg of slip faces is taken as zero (zero-initialised ghosts).
-        On tetrahedra (``packed`` None or True) the steps run on 64-byte cell records and
-        faces renumbered by first use (mfv_euler3d_step_packed, same bits)."""
+        On tetrahedra (``packed`` None or True) the steps run on 64-byte cell records with
+        per-cell face tables (mfv_euler3d_step_packed, same bits)."""
         import torch
         m = self._m
         s = stream_ptr()
