@@ -271,7 +271,10 @@ class SwEnsemble(object):
             self.step_index += chunk
 
     def close_graph(self):
+        """Release the captured steps (after the device has finished any replay of them)."""
         if self._graph is not None:
+            import torch
+            torch.cuda.synchronize()
             _lib.call('mfv_graph_destroy', self._graph)
             self._graph = None
 

@@ -244,7 +244,12 @@ class EllBatch(object):
         _lib.call('mfv_graph_launch', g, stream_ptr(stream))
 
     def close_graphs(self):
-        for g in self.__dict__.pop('_graphs', {}).values():
+        """Release the captured ticks (after the device has finished any replay of them)."""
+        graphs = self.__dict__.pop('_graphs', {})
+        if graphs:
+            import torch
+            torch.cuda.synchronize(self.device)
+        for g in graphs.values():
             _lib.call('mfv_graph_destroy', g)
 
     def __del__(self):
