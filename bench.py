@@ -296,12 +296,15 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
     S = [dv(U0), dv(np.zeros((6, C)))]
     ring, dt0 = dv(np.zeros(4)), dv(np.zeros(1))
 
-    from manapy_amd.device import EulerTet, FaceTiles
-    tiles = {}
+    from manapy_amd.device import EulerTet, EulerTetTiles, FaceTiles
+    tiles, ptiles = {}, {}
     for T in (128, 256):
         ft = FaceTiles(m, T)
         if ft.lds_bytes() <= 64 * 1024:
             tiles[T] = ft
+        pt = EulerTetTiles(m, T)
+        if pt.lds_bytes() <= 64 * 1024:
+            ptiles[T] = pt
     et = EulerTet(m)
     R = [torch.empty((C, 8), dtype=torch.float64, device='cuda') for _ in range(2)]
     path = ['fused']  # 'fused': per-cell step on [6][C]; 'tiled:T': face tiles; 'packed': [C][8] records
@@ -319,11 +322,15 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
         elif path[0] == 'packed':
             _lib.call('mfv_euler3d_step_packed', m.ptr, et.ptr, R[k % 2].data_ptr(), R[(k + 1) % 2].data_ptr(), 1.4,
                       CFL, 0, ring.data_ptr(), k, s)
+        elif path[0].startswith('packed-tiled:'):
+            _lib.call('mfv_euler3d_step_packed_tiled', m.ptr, et.ptr, ptiles[int(path[0].split(':')[1])].ptr,
+                      R[k % 2].data_ptr(), R[(k + 1) % 2].data_ptr(), 1.4, CFL, 0, ring.data_ptr(), k, s)
         else:
             _lib.call('mfv_euler3d_step_tiled', m.ptr, tiles[int(path[0].split(':')[1])].ptr, S[k % 2].data_ptr(),
                       S[(k + 1) % 2].data_ptr(), 1.4, CFL, 0, ring.data_ptr(), k, s)
     probe = {}
-    for cand in ['fused', 'packed'] + ['tiled:%d' % T for T in sorted(tiles)]:
+    for cand in (['fused', 'packed'] + ['packed-tiled:%d' % T for T in sorted(ptiles)] +
+                 ['tiled:%d' % T for T in sorted(tiles)]):
         path[0] = cand
         init()
         probe[cand] = time_steps(step, 2, 10)[1] * 1e3
@@ -338,6 +345,11 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
         # per-cell across-codes (16 B), face geometry (4 x 32 B) and rank byte
         bpc = 64 + 64 + 16 + 128 + 1
         kname = "k_euler_step_packed"
+    elif best.startswith('packed-tiled:'):
+        # records read + written, per-cell entry codes (16 B), face geometry for the time step
+        # (4 x 32 B), rank byte; per face (x F/C): one 40-byte entry (geometry, item pair)
+        bpc = 64 + 64 + 16 + 128 + 1 + 40 * F / C
+        kname = "k_euler_step_packed_tiled<%s>" % best.split(':')[1]
     else:
         # U read + write (6 doubles each), volume, face count; per face slot: order entry +
         # faceid; per face (x F/C): cellid, normal, mesure, name
@@ -357,7 +369,7 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
         init()
         for k in range(cpu_steps):
             step(k)
-        if best == 'packed':
+        if best.startswith('packed'):
             _lib.call('mfv_euler3d_unpack', m.ptr, R[cpu_steps % 2].data_ptr(), S[cpu_steps % 2].data_ptr(), s)
         U_gpu = S[cpu_steps % 2].cpu().numpy()
         U = U0.copy()

@@ -479,6 +479,35 @@ int mfv_euler3d_unpack(const mfv_mesh *m, const double *rec, double *U, void *st
 int mfv_euler3d_step_packed(const mfv_mesh *m, const mfv_euler_tet *t, const double *rec_in, double *rec_out,
                             double gamma, double cfl, int quirk, double *dt_ring, int step, void *stream);
 
+/* Face tiles of the packed Euler step (tiles of T consecutive cells, as mfv_face_tiles): the
+ * tile's face entries carry their own geometry and item positions, so a tile reads its faces
+ * with contiguous loads. */
+typedef struct mfv_euler_tet_tiles {
+    int32_t nb_tiles;
+    int32_t tile_size;          /* cells per tile (and threads per block): 128, 256 or 512 */
+    int32_t max_items;          /* max over tiles of cells + halo */
+    int32_t max_faces;          /* max over tiles of the face entries */
+    const int32_t *tile_cell0;  /* [nb_tiles] */
+    const int32_t *tile_ncells; /* [nb_tiles] */
+    const int32_t *halo_ptr;    /* [nb_tiles + 1] */
+    const int32_t *halo_cells;  /* [halo_ptr[nb_tiles]] */
+    const int32_t *face_ptr;    /* [nb_tiles + 1] */
+    const double *face_geo;     /* [face_ptr[nb_tiles]][4] faces.normal (3), faces.mesure of each entry */
+    const int32_t *face_lr;     /* [face_ptr[nb_tiles]][2] left item; right item (inner face) or -faces.name */
+    const int32_t *cell_code;   /* [C][4] k-th face of the cell (increasing original id): entry << 1 | the cell
+                                 * is the face's right side */
+} mfv_euler_tet_tiles;
+
+/* The packed Euler step on face tiles: the block stages its cells' and halo's records in LDS
+ * with contiguous loads (halo gathered once per tile), evaluates every face of the tile once
+ * into LDS, then each cell adds its faces' fluxes in the residual order, updates its record
+ * and takes its next-dt candidate; same bits as mfv_euler3d_step_packed. LDS: 48 * max_items
+ * + 40 * max_faces bytes (<= 64 KiB). Replaces explicitscheme_euler_3d_rusanov
+ * (fvm_utils3d.py:1083-1170) + update_euler_3d_fvc + time_step_euler_3d in one launch. */
+int mfv_euler3d_step_packed_tiled(const mfv_mesh *m, const mfv_euler_tet *t, const mfv_euler_tet_tiles *tt,
+                                  const double *rec_in, double *rec_out, double gamma, double cfl, int quirk,
+                                  double *dt_ring, int step, void *stream);
+
 /* ---------------- diamond scheme, 2D (ast/functions2d.py, solvers/diffusion) --------- */
 
 /* Diamond-scheme geometry. The pseudo-Laplacian weight of a node incidence is

@@ -77,6 +77,13 @@ class MfvEulerTetHost(ctypes.Structure):
         [(k, P) for k in ('cell_nb', 'cell_rank', 'cell_geo')]
 
 
+class MfvEulerTetTiles(ctypes.Structure):
+    """Mirror of ``struct mfv_euler_tet_tiles`` (include/mfv.h)."""
+    _fields_ = [(k, ctypes.c_int32) for k in ('nb_tiles', 'tile_size', 'max_items', 'max_faces')] + \
+        [(k, P) for k in ('tile_cell0', 'tile_ncells', 'halo_ptr', 'halo_cells', 'face_ptr', 'face_geo', 'face_lr',
+                          'cell_code')]
+
+
 class MfvSwPacked(ctypes.Structure):
     """Mirror of ``struct mfv_sw_packed`` (include/mfv.h)."""
     _fields_ = [('nb_cells', ctypes.c_int32), ('max_cell_faces', ctypes.c_int32), ('cellrec', P), ('facerec', P),
@@ -135,6 +142,7 @@ SWM_P = ctypes.POINTER(MfvSwMixed)
 SWT_P = ctypes.POINTER(MfvSwTiles)
 FT_P = ctypes.POINTER(MfvFaceTiles)
 ET_P = ctypes.POINTER(MfvEulerTet)
+ETT_P = ctypes.POINTER(MfvEulerTetTiles)
 DP = ctypes.POINTER(ctypes.c_double)
 
 # name -> argtypes (restype is int unless listed in _RESTYPE)
@@ -182,6 +190,7 @@ _SIGNATURES = {
     'mfv_euler3d_pack': [MESH_P, P, P, P],
     'mfv_euler3d_unpack': [MESH_P, P, P, P],
     'mfv_euler3d_step_packed': [MESH_P, ET_P, P, P, c_double, c_double, c_int, P, c_int, P],
+    'mfv_euler3d_step_packed_tiled': [MESH_P, ET_P, ETT_P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_diamond_widths': [DIA_P, P, P, P, P, P],
     'mfv_diamond_assemble': [DIA_P, P, P, c_int, c_int, P, P, P, P, P, P],
     'mfv_diamond_rhs': [DIA_P, P, P, P, P, P, P],

@@ -446,6 +446,127 @@ __global__ void __launch_bounds__(kBlock) k_euler_step_packed(EulerTetK t, const
     wave_min_part(dtpart, (size_t)i, d);
 }
 
+// The packed step on face tiles: stage the tile's items' records (cells contiguous, halo
+// gathered) in LDS, evaluate each face entry once from its own contiguous record, then every
+// tile cell adds its faces' fluxes in the residual order, updates its record and takes its
+// next-dt candidate (face geometry from the cell's residual-order table, summed in slot
+// order). Same expressions and orders as k_euler_step_packed, so the same bits.
+struct EulerTilesK {
+    const int32_t *__restrict__ cell0;
+    const int32_t *__restrict__ ncells;
+    const int32_t *__restrict__ hptr;
+    const int32_t *__restrict__ halo;
+    const int32_t *__restrict__ fptr;
+    const double2 *__restrict__ fgeo;
+    const int2 *__restrict__ flr;
+    const int4 *__restrict__ code;
+    int MI, MFc;  // max items, max face entries per tile
+};
+
+template <int T>
+__global__ void __launch_bounds__(T) k_euler_step_packed_tiled(EulerTetK t, EulerTilesK tt,
+                                                                const double *__restrict__ Rin,
+                                                                double *__restrict__ Rout, double gamma, double cfl,
+                                                                int quirk, DtRing ring, double *__restrict__ dtpart)
+{
+    extern __shared__ double lds[];
+    double2 *sR = reinterpret_cast<double2 *>(lds);  // [MI][3]: six values per item
+    double *sF = lds + 6 * tt.MI;                    // [5][MFc]
+    const int tile = xcd_block();
+    const int c0 = tt.cell0[tile], nc = tt.ncells[tile];
+    const int h0 = tt.hptr[tile], ni = nc + (tt.hptr[tile + 1] - h0);
+    const int f0 = tt.fptr[tile], nfc = tt.fptr[tile + 1] - f0;
+    ring_advance(ring);
+    for (int e = threadIdx.x; e < ni; e += T) {
+        const int c = (e < nc) ? c0 + e : tt.halo[h0 + (e - nc)];
+        const double2 *rp = reinterpret_cast<const double2 *>(Rin + 8 * (size_t)c);
+        const double2 a = rp[0], b = rp[1], e2 = rp[2];
+        sR[3 * e] = a;
+        sR[3 * e + 1] = b;
+        sR[3 * e + 2] = e2;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nfc; k += T) {
+        const double2 g01 = tt.fgeo[2 * (size_t)(f0 + k)], g23 = tt.fgeo[2 * (size_t)(f0 + k) + 1];
+        const int2 lr = tt.flr[f0 + k];
+        const double nrm[3] = {g01.x, g01.y, g23.x};
+        const double mesu = g23.y;
+        const double n[3] = {nrm[0] / mesu, nrm[1] / mesu, nrm[2] / mesu};
+        const double2 la = sR[3 * lr.x], lb = sR[3 * lr.x + 1], lc = sR[3 * lr.x + 2];
+        const Prim6 L{la.x, la.y, lb.x, lb.y, lc.x, lc.y};
+        Prim6 R;
+        if (lr.y >= 0) {
+            const double2 ra = sR[3 * lr.y], rb = sR[3 * lr.y + 1], rc = sR[3 * lr.y + 2];
+            R = Prim6{ra.x, ra.y, rb.x, rb.y, rc.x, rc.y};
+        } else {
+            R = euler_ghost(L, nrm, -lr.y, quirk, 0.);
+        }
+        double fl[5];
+        euler_flux(L, R, n, mesu, gamma, quirk, fl);
+#pragma unroll
+        for (int v = 0; v < 5; v++) sF[v * tt.MFc + k] = fl[v];
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if ((q & ~63) >= nc) return;  // the whole wave lies past the tile: it owns no partial
+    double d = 1e300;
+    if (q < nc) {
+        const int i = c0 + q;
+        const int4 c4 = tt.code[i];
+        const unsigned ord = t.cell_rank[i];
+        const double vol = Rin[8 * (size_t)i + 6];
+        double2 g01[4], g23[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            g01[k] = t.cell_geo[(size_t)(2 * k) * t.C + i];
+            g23[k] = t.cell_geo[(size_t)(2 * k + 1) * t.C + i];
+        }
+        const int cd[4] = {c4.x, c4.y, c4.z, c4.w};
+        double r[5] = {0., 0., 0., 0., 0.};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {  // faces in increasing original id
+            const int ph = cd[k] >> 1;
+            const bool own_R = cd[k] & 1;
+            // rez -= fl (own = L) is rez += -fl
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                const double x = sF[v * tt.MFc + ph];
+                r[v] += own_R ? x : -x;
+            }
+        }
+        const double2 oa = sR[3 * q], ob = sR[3 * q + 1], oc = sR[3 * q + 2];
+        double u[6] = {oa.x, oa.y, ob.x, ob.y, oc.x, oc.y};
+        euler_update_cell(u, r, gamma, ring.cur[0], vol);
+        double2 *wp = reinterpret_cast<double2 *>(Rout + 8 * (size_t)i);
+        wp[0] = make_double2(u[0], u[1]);
+        wp[1] = make_double2(u[2], u[3]);
+        wp[2] = make_double2(u[4], u[5]);
+        wp[3] = make_double2(vol, 0.);
+        // time_step_euler_3d of the new state: the terms in residual order, summed in slot order
+        const double velson = sqrt(gamma * fabs(u[1] / u[0]));
+        double tm[4];
+        int sl[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double nx = g01[k].x, ny = g01[k].y, nz = g23[k].x;
+            const double u_n = fabs((u[2] * nx + u[3] * ny + u[4] * nz) / u[0]) / sqrt(nx * nx + ny * ny + nz * nz);
+            const double lam_convect = u_n + velson;
+            tm[k] = lam_convect * g23[k].y;
+            sl[k] = (ord >> (2 * k)) & 3;
+        }
+        double lam = 0.;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double tj = tm[0];
+#pragma unroll
+            for (int k = 1; k < 4; k++) tj = sl[k] == j ? tm[k] : tj;
+            lam += tj;
+        }
+        d = cfl * vol / lam;
+    }
+    wave_min_part(dtpart, (size_t)c0 + q, d);
+}
+
 __global__ void __launch_bounds__(kBlock) k_euler_pack(int C, const double *__restrict__ U,
                                                         const double *__restrict__ vol, double *__restrict__ rec)
 {
@@ -859,5 +980,73 @@ extern "C" int mfv_euler3d_step_packed(const mfv_mesh *m, const mfv_euler_tet *t
     hipLaunchKernelGGL(k_euler_step_packed, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, st, tk, rec_in, rec_out,
                        gamma, cfl, quirk, r, part);
     if (int rc = check_launch("k_euler_step_packed")) return rc;
+    return dt_finish(part, nw, r.next, st);
+}
+
+static int check_euler_tiles(const mfv_mesh *m, const mfv_euler_tet_tiles *tt)
+{
+    MFV_REQUIRE(tt != nullptr, "packed Euler tiles are NULL");
+    MFV_REQUIRE(tt->tile_size == 128 || tt->tile_size == 256 || tt->tile_size == 512,
+                "tile_size %d not in {128, 256, 512}", tt->tile_size);
+    MFV_REQUIRE(tt->nb_tiles == (m->nb_cells + tt->tile_size - 1) / tt->tile_size,
+                "packed Euler tiles were built for another mesh (%d tiles of %d cells for %d cells)", tt->nb_tiles,
+                tt->tile_size, m->nb_cells);
+    MFV_REQUIRE(tt->max_items >= 1 && tt->max_faces >= 1, "bad tile counts");
+    MFV_REQUIRE(tt->tile_cell0 && tt->tile_ncells && tt->halo_ptr && tt->halo_cells && tt->face_ptr && tt->face_geo &&
+                    tt->face_lr && tt->cell_code,
+                "packed Euler tiles have NULL arrays");
+    MFV_REQUIRE(((uintptr_t)tt->face_geo & 15) == 0 && ((uintptr_t)tt->face_lr & 7) == 0 &&
+                    ((uintptr_t)tt->cell_code & 15) == 0,
+                "packed Euler tiles are misaligned");
+    const size_t lds = 48 * (size_t)tt->max_items + 40 * (size_t)tt->max_faces;
+    MFV_REQUIRE(lds <= 64 * 1024, "packed Euler tiles of %d cells need %zu bytes of LDS (> 64 KiB): use smaller "
+                "tiles or renumber the cells", tt->tile_size, lds);
+    return MFV_OK;
+}
+
+extern "C" int mfv_euler3d_step_packed_tiled(const mfv_mesh *m, const mfv_euler_tet *t, const mfv_euler_tet_tiles *tt,
+                                             const double *rec_in, double *rec_out, double gamma, double cfl,
+                                             int quirk, double *dt_ring_p, int step, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    MFV_REQUIRE(m->dim == 3, "the packed Euler step needs a 3D mesh");
+    if (int rc = check_euler_tet(m, t)) return rc;
+    if (int rc = check_euler_tiles(m, tt)) return rc;
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(rec_in && rec_out && rec_in != rec_out && dt_ring_p, "NULL or aliased array");
+    MFV_REQUIRE(((uintptr_t)rec_in & 15) == 0 && ((uintptr_t)rec_out & 15) == 0, "misaligned records");
+    DtRing r = dt_ring(dt_ring_p, step);
+    const size_t nw = waves_for(m->nb_cells);
+    hipStream_t st = as_stream(stream);
+    double *part = dt_scratch(nw, st);
+    if (!part) return MFV_ENOMEM;
+    const EulerTetK tk{reinterpret_cast<const int4 *>(t->cell_nb), t->cell_rank,
+                       reinterpret_cast<const double2 *>(t->cell_geo), (size_t)m->nb_cells};
+    const EulerTilesK ek{tt->tile_cell0,
+                         tt->tile_ncells,
+                         tt->halo_ptr,
+                         tt->halo_cells,
+                         tt->face_ptr,
+                         reinterpret_cast<const double2 *>(tt->face_geo),
+                         reinterpret_cast<const int2 *>(tt->face_lr),
+                         reinterpret_cast<const int4 *>(tt->cell_code),
+                         tt->max_items,
+                         tt->max_faces};
+    const size_t lds = 48 * (size_t)tt->max_items + 40 * (size_t)tt->max_faces;
+    switch (tt->tile_size) {
+    case 128:
+        hipLaunchKernelGGL(k_euler_step_packed_tiled<128>, dim3(tt->nb_tiles), dim3(128), lds, st, tk, ek, rec_in,
+                           rec_out, gamma, cfl, quirk, r, part);
+        break;
+    case 256:
+        hipLaunchKernelGGL(k_euler_step_packed_tiled<256>, dim3(tt->nb_tiles), dim3(256), lds, st, tk, ek, rec_in,
+                           rec_out, gamma, cfl, quirk, r, part);
+        break;
+    default:
+        hipLaunchKernelGGL(k_euler_step_packed_tiled<512>, dim3(tt->nb_tiles), dim3(512), lds, st, tk, ek, rec_in,
+                           rec_out, gamma, cfl, quirk, r, part);
+        break;
+    }
+    if (int rc = check_launch("k_euler_step_packed_tiled")) return rc;
     return dt_finish(part, nw, r.next, st);
 }

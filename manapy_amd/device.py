@@ -586,6 +586,55 @@ class EulerTet(object):
         return ctypes.byref(self.desc)
 
 
+def euler_tet_tiles_host(H, T):
+    """Face tiles of the packed Euler step from a host mesh layout ``H`` (gpu_layout): the
+    tiles, halos and face entries of face_tiles_native, each entry with its own geometry
+    (normal, mesure) and item pair (left item; right item, or -name on a boundary face), and
+    per cell the entry codes of its faces in the residual order."""
+    ft = face_tiles_native(H['cell_faces'], H['cell_nfaces'], H['face_cells'], T)
+    fid = ft['face_id'].astype(np.int64)
+    name = H['face_name'][fid].astype(np.int64)
+    if np.any((name == 0) & (ft['face_r'] < 0)) or np.any(name < 0):
+        raise ValueError("an inner face (name 0) needs two cells and names must be >= 0")
+    geo = np.ascontiguousarray(np.column_stack([H['face_normal'][fid], H['face_mesure'][fid]]), dtype=np.float64)
+    lr = np.ascontiguousarray(np.column_stack([ft['face_l'], np.where(name == 0, ft['face_r'], -name)]),
+                              dtype=np.int32)
+    order = H['order_sorted'].astype(np.int64)              # [4][C]: slot of the k-th face
+    code = np.ascontiguousarray(np.take_along_axis(ft['lface'], order, axis=0).T, dtype=np.int32)  # [C][4]
+    out = {k: ft[k] for k in ('T', 'nb_tiles', 'max_items', 'max_faces', 'tile_cell0', 'tile_ncells', 'halo_ptr',
+                              'halo_cells', 'face_ptr')}
+    out.update({'face_geo': geo, 'face_lr': lr, 'cell_code': code})
+    return out
+
+
+class EulerTetTiles(object):
+    """Device copy of the packed Euler step's face tiles (mfv_euler_tet_tiles)."""
+
+    ARRAYS = ('tile_cell0', 'tile_ncells', 'halo_ptr', 'halo_cells', 'face_ptr', 'face_geo', 'face_lr', 'cell_code')
+
+    def __init__(self, dmesh, T=256):
+        import torch
+        if dmesh.MF != 4:
+            raise ValueError("the packed Euler step needs tetrahedra")
+        self.host = euler_tet_tiles_host(dmesh.host, T)
+        self.t = {}
+        d = _lib.MfvEulerTetTiles()
+        d.nb_tiles, d.tile_size = self.host['nb_tiles'], T
+        d.max_items, d.max_faces = self.host['max_items'], self.host['max_faces']
+        for k in self.ARRAYS:
+            a = np.ascontiguousarray(self.host[k])
+            self.t[k] = torch.from_numpy(a if a.size else np.zeros(4, dtype=a.dtype)).to(dmesh.device)
+            setattr(d, k, self.t[k].data_ptr())
+        self.desc = d
+
+    @property
+    def ptr(self):
+        return ctypes.byref(self.desc)
+
+    def lds_bytes(self):
+        return 48 * self.host['max_items'] + 40 * self.host['max_faces']
+
+
 class SwTiles(object):
     """Device copy of a tiles layout and its mfv_sw_tiles descriptor (stage=True copies every
     cell state a tile reads into LDS before reconstructing)."""

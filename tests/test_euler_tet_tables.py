@@ -74,3 +74,37 @@ def test_builder_rejects_bad_meshes():
         euler_tet_native(*args(cell_faces=cf))
     with pytest.raises(ValueError, match="tetrahedra"):
         euler_tet_native(np.zeros((3, 4), np.int32), *args()[1:])
+
+
+@pytest.mark.parametrize("T", [128, 256])
+def test_tiles_entries_and_cell_codes(T):
+    """Each face entry of a tile carries its face's geometry and the items of its two cells
+    (or the boundary name); each cell's k-th code names the entry of its k-th face (increasing
+    original id) in its own tile and the cell's side."""
+    from manapy_amd.device import euler_tet_tiles_host, face_tiles_native, gpu_layout
+    from manapy_amd.renumber import renumbered
+    d = renumbered(make_tet_domain(7, 0.08, 2))
+    L = gpu_layout(d, lsq=False)
+    tt = euler_tet_tiles_host(L, T)
+    ft = face_tiles_native(L['cell_faces'], L['cell_nfaces'], L['face_cells'], T)
+    C = int(d.nbcells)
+    fk = np.take_along_axis(L['cell_faces'].T, L['order_sorted'].T.astype(np.int64), axis=1)
+    for t in range(tt['nb_tiles']):
+        c0, nc = int(tt['tile_cell0'][t]), int(tt['tile_ncells'][t])
+        halo = tt['halo_cells'][tt['halo_ptr'][t]:tt['halo_ptr'][t + 1]]
+        items = np.concatenate([np.arange(c0, c0 + nc), halo])
+        e0, e1 = int(tt['face_ptr'][t]), int(tt['face_ptr'][t + 1])
+        fid = ft['face_id'][e0:e1]
+        assert np.array_equal(tt['face_geo'][e0:e1, :3], L['face_normal'][fid])
+        assert np.array_equal(tt['face_geo'][e0:e1, 3], L['face_mesure'][fid])
+        lr = tt['face_lr'][e0:e1]
+        assert np.array_equal(items[lr[:, 0]], L['face_cells'][fid, 0])
+        inner = L['face_name'][fid] == 0
+        assert np.array_equal(items[lr[inner, 1]], L['face_cells'][fid[inner], 1])
+        assert np.array_equal(lr[~inner, 1], -L['face_name'][fid[~inner]])
+        for i in range(c0, c0 + nc):
+            for k in range(4):
+                code = int(tt['cell_code'][i, k])
+                assert fid[code >> 1] == fk[i, k]
+                assert (code & 1) == int(L['face_cells'][fk[i, k], 1] == i)
+    assert tt['cell_code'].shape == (C, 4)
