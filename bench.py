@@ -346,9 +346,9 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
         bpc = 64 + 64 + 16 + 128 + 1
         kname = "k_euler_step_packed"
     elif best.startswith('packed-tiled:'):
-        # records read + written, per-cell entry codes (16 B), face geometry for the time step
-        # (4 x 32 B), rank byte; per face (x F/C): one 40-byte entry (geometry, item pair)
-        bpc = 64 + 64 + 16 + 128 + 1 + 40 * F / C
+        # records read + written, per-cell entry codes (16 B), rank byte; per face (x F/C): one
+        # 40-byte entry (geometry, item pair)
+        bpc = 64 + 64 + 16 + 1 + 40 * F / C
         kname = "k_euler_step_packed_tiled<%s>" % best.split(':')[1]
     else:
         # U read + write (6 doubles each), volume, face count; per face slot: order entry +

@@ -449,8 +449,8 @@ __global__ void __launch_bounds__(kBlock) k_euler_step_packed(EulerTetK t, const
 // The packed step on face tiles: stage the tile's items' records (cells contiguous, halo
 // gathered) in LDS, evaluate each face entry once from its own contiguous record, then every
 // tile cell adds its faces' fluxes in the residual order, updates its record and takes its
-// next-dt candidate (face geometry from the cell's residual-order table, summed in slot
-// order). Same expressions and orders as k_euler_step_packed, so the same bits.
+// next-dt candidate (face geometry from the tile's entries, still in cache; terms summed in
+// slot order). Same expressions and orders as k_euler_step_packed, so the same bits.
 struct EulerTilesK {
     const int32_t *__restrict__ cell0;
     const int32_t *__restrict__ ncells;
@@ -515,13 +515,13 @@ __global__ void __launch_bounds__(T) k_euler_step_packed_tiled(EulerTetK t, Eule
         const int4 c4 = tt.code[i];
         const unsigned ord = t.cell_rank[i];
         const double vol = Rin[8 * (size_t)i + 6];
-        double2 g01[4], g23[4];
+        const int cd[4] = {c4.x, c4.y, c4.z, c4.w};
+        double2 g01[4], g23[4];  // the faces' geometry from the tile's entries (in cache since phase F)
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            g01[k] = t.cell_geo[(size_t)(2 * k) * t.C + i];
-            g23[k] = t.cell_geo[(size_t)(2 * k + 1) * t.C + i];
+            g01[k] = tt.fgeo[2 * (size_t)(f0 + (cd[k] >> 1))];
+            g23[k] = tt.fgeo[2 * (size_t)(f0 + (cd[k] >> 1)) + 1];
         }
-        const int cd[4] = {c4.x, c4.y, c4.z, c4.w};
         double r[5] = {0., 0., 0., 0., 0.};
 #pragma unroll
         for (int k = 0; k < 4; k++) {  // faces in increasing original id
