@@ -100,8 +100,9 @@ class EulerSolver3D(object):
     def run(self, nsteps, packed=None):
         """``nsteps`` fused steps (ghosts computed on the fly) with dt on the device. Under
         ``quirk`` the unset rhowg of slip faces is taken as zero (zero-initialised ghosts).
-        On tetrahedra (``packed`` None or True) the steps run on 64-byte cell records with
-        per-cell face tables (mfv_euler3d_step_packed, same bits)."""
+        On tetrahedra (``packed`` None or True) the steps run on 64-byte cell records, on face
+        tiles of 256 cells when their LDS fits (mfv_euler3d_step_packed_tiled, else
+        mfv_euler3d_step_packed; same bits)."""
         import torch
         m = self._m
         s = stream_ptr()
@@ -111,16 +112,29 @@ class EulerSolver3D(object):
         if packed is None:
             packed = m.MF == 4
         if packed:
-            from ..device import EulerTet
+            from ..device import EulerTet, EulerTetTiles
             et = getattr(m, '_euler_tet', None)
             if et is None:
                 et = m._euler_tet = EulerTet(m)
+            tt = getattr(m, '_euler_tiles', False)
+            if tt is False:  # face tiles of 256 (or 128) cells when their LDS fits
+                tt = None
+                for T in (256, 128):
+                    cand = EulerTetTiles(m, T)
+                    if cand.lds_bytes() <= 64 * 1024:
+                        tt = cand
+                        break
+                m._euler_tiles = tt
             a = torch.empty((int(m.C), 8), dtype=torch.float64, device=self.U.device)
             b = torch.empty_like(a)
             _lib.call('mfv_euler3d_pack', m.ptr, self.U.data_ptr(), a.data_ptr(), s)
             for k in range(nsteps):
-                _lib.call('mfv_euler3d_step_packed', m.ptr, et.ptr, a.data_ptr(), b.data_ptr(), self.gamma, self.cfl,
-                          self.quirk, self._clock.ring.data_ptr(), k, s)
+                if tt is not None:
+                    _lib.call('mfv_euler3d_step_packed_tiled', m.ptr, et.ptr, tt.ptr, a.data_ptr(), b.data_ptr(),
+                              self.gamma, self.cfl, self.quirk, self._clock.ring.data_ptr(), k, s)
+                else:
+                    _lib.call('mfv_euler3d_step_packed', m.ptr, et.ptr, a.data_ptr(), b.data_ptr(), self.gamma,
+                              self.cfl, self.quirk, self._clock.ring.data_ptr(), k, s)
                 a, b = b, a
             _lib.call('mfv_euler3d_unpack', m.ptr, a.data_ptr(), self.U.data_ptr(), s)
         else:
