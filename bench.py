@@ -270,7 +270,7 @@ def run_advection(n, steps, warmup, cpu=True):
     return out
 
 
-def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
+def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='hilbert'):
     """3D Sod shock tube on n^3 cubes x 6 Kuhn tetrahedra of the unit cube (interior nodes
     jittered by +/-8 % of h, slip walls, first-order Rusanov, CFL 0.8): the fastest of the
     fused device steps (per-cell on [6][C], packed records, face tiles). Parity: the first ``cpu_steps`` steps against the oracle, bit for
@@ -284,9 +284,11 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='morton'):
     t0 = time.perf_counter()
     el, nd = kuhn_tetrahedra(n, 1.0, 0.08, 0)
     d = Domain(3, mesh=(el, nd))
-    if renumber == 'morton':  # cells along a Morton curve of their centres (a relabelling: same bits)
-        from manapy_amd.renumber import renumbered
-        d = renumbered(d)
+    if renumber in ('hilbert', 'morton'):  # cells along a space-filling curve (a relabelling: same bits)
+        from manapy_amd.renumber import hilbert_order_3d, morton_order_3d, renumbered
+        c = d.cells.center
+        order = hilbert_order_3d if renumber == 'hilbert' else morton_order_3d
+        d = renumbered(d, order(c[:, 0], c[:, 1], c[:, 2]))
     t_mesh = time.perf_counter() - t0
     C, F = int(d.nbcells), int(d.nbfaces)
     U0 = initialisation_euler_3d(d.cells.center)
@@ -656,7 +658,7 @@ def main():
     ap.add_argument('--advection-steps', type=int, default=100)
     ap.add_argument('--euler', type=int, default=100, help='n of the n^3 x 6 tetrahedra Euler run (0: skip)')
     ap.add_argument('--euler-steps', type=int, default=200)
-    ap.add_argument('--euler-renumber', choices=('morton', 'none'), default='morton')
+    ap.add_argument('--euler-renumber', choices=('hilbert', 'morton', 'none'), default='hilbert')
     ap.add_argument('--tile', type=int, choices=(128, 256, 512), default=256, help='cells per tile of the fused step')
     ap.add_argument('--mixed-kernel', choices=('auto', 'split', 'tiled', 'staged'), default='auto',
                     help='mixed-precision step: autotuned (default), two kernels, fused, fused with LDS-staged states')
