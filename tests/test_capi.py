@@ -3,6 +3,8 @@ declared in include/mfv.h; the ctypes table binds each of them."""
 import os
 import re
 
+import pytest
+
 from manapy_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -37,3 +39,38 @@ def test_mesh_struct_layout():
     import ctypes
     # 6 int32 + 17 pointers + 2 int32, natural alignment
     assert ctypes.sizeof(_lib.MfvMesh) == 6 * 4 + 17 * 8 + 2 * 4
+
+
+def test_every_mirrored_struct_matches_the_header(tmp_path):
+    """Each ctypes mirror in the binding has the size and field offsets that a C compiler gives
+    the struct of the same name in include/mfv.h."""
+    import ctypes
+    import re as _re
+    import shutil
+    import subprocess
+    if shutil.which('gcc') is None:
+        pytest.skip("no C compiler")
+    mirrors = []
+    for name in dir(_lib):
+        cls = getattr(_lib, name)
+        if isinstance(cls, type) and issubclass(cls, ctypes.Structure) and cls.__doc__:
+            m = _re.search(r'struct (mfv_[a-z0-9_]+)', cls.__doc__)
+            if m:
+                mirrors.append((m.group(1), cls))
+    assert len(mirrors) >= 10
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mfv.h"', 'int main(void) {']
+    for cname, cls in mirrors:
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in cls._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f[0], cname, f[0]))
+    lines += ['return 0;', '}']
+    src = tmp_path / 'layout.c'
+    src.write_text('\n'.join(lines))
+    exe = tmp_path / 'layout'
+    subprocess.run(['gcc', '-std=c11', '-I', os.path.join(ROOT, 'include'), str(src), '-o', str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                  text=True).stdout.splitlines())
+    for cname, cls in mirrors:
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f in cls._fields_:
+            assert int(got['%s.%s' % (cname, f[0])]) == getattr(cls, f[0]).offset, (cname, f[0])
