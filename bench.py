@@ -578,7 +578,7 @@ def run_ensemble(n_inst, steps, seed=0, n_check=4):
     }
 
 
-def run_serving(n_req, seed=1, nsteps=20):
+def run_serving(n_req, seed=1, nsteps=20, n_iter=6):
     """Implicit advection-diffusion requests (meshes log-uniform in [10k, 1M] cells, D
     log-uniform in [1e-3, 1e-1], 20 steps each, FP64 refinement to 1e-10) served with
     continuous batching: first back to back (capacity), then with Poisson arrivals at 80 % of
@@ -610,7 +610,7 @@ def run_serving(n_req, seed=1, nsteps=20):
                 for k, (sp, a) in enumerate(zip(specs, arrivals))]
 
     budget = int(3.0e9)
-    eng = ServingEngine(max_cells=max(1 << 22, total_cells), max_inst=64, hbm_budget=budget)
+    eng = ServingEngine(max_cells=max(1 << 22, total_cells), max_inst=64, hbm_budget=budget, n_iter=n_iter)
     t1 = time.perf_counter()
     log("serving: capacity run")
     done = eng.run(make(np.zeros(n_req)))
@@ -619,7 +619,7 @@ def run_serving(n_req, seed=1, nsteps=20):
     capacity = n_req / makespan
     rate = 0.8 * capacity
     arrivals = np.cumsum(rng.exponential(1. / rate, n_req))
-    eng = ServingEngine(max_cells=max(1 << 22, total_cells), max_inst=64, hbm_budget=budget)
+    eng = ServingEngine(max_cells=max(1 << 22, total_cells), max_inst=64, hbm_budget=budget, n_iter=n_iter)
     t2 = time.perf_counter()
     done = eng.run(make(arrivals))
     span = time.perf_counter() - t2
@@ -633,6 +633,7 @@ def run_serving(n_req, seed=1, nsteps=20):
         "latency_mean_s": round(lat['mean_s'], 4), "max_concurrent": eng.max_active,
         "hbm_budget_bytes": budget, "krylov_iterations_mean": float(np.mean([r.iterations for r in done])),
         "solve_tol": 1e-10, "mesh_build_s": round(t_build, 1), "solver_tick": "HIP graph replay",
+        "bicgstab_iterations_per_tick": n_iter,
     }
 
 
@@ -653,6 +654,7 @@ def main():
     ap.add_argument('--ensemble', type=int, default=256, help='instances of the ensemble run (0: skip)')
     ap.add_argument('--ensemble-steps', type=int, default=204)
     ap.add_argument('--serving', type=int, default=32, help='requests of the serving run (0: skip)')
+    ap.add_argument('--serving-iters', type=int, default=6, help='BiCGStab iterations per solver tick')
     ap.add_argument('--advection', type=int, default=200,
                     help='n of the n x n x 2 triangle advection run (0: skip)')
     ap.add_argument('--advection-steps', type=int, default=100)
@@ -874,7 +876,7 @@ def main():
         out["ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps)
         log("ensemble: %.1f Mcell-updates/s" % out["ensemble"]["value"])
     if args.serving:
-        out["serving"] = run_serving(args.serving)
+        out["serving"] = run_serving(args.serving, n_iter=args.serving_iters)
         log("serving: p50 %.3f s, p99 %.3f s" % (out["serving"]["latency_p50_s"], out["serving"]["latency_p99_s"]))
     if args.advection:
         out["advection"] = run_advection(args.advection, args.advection_steps, 10, not args.no_cpu)
