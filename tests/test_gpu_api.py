@@ -130,13 +130,17 @@ def test_euler_solver(oracle_lib, quirk):
     for _ in range(15):
         a.step()
     b = EulerSolver3D(d, U0=U0, conf=Struct(cfl=0.8, gamma=1.4, quirk=quirk))
-    b.run(15)
+    b.run(15)                   # packed records on face tiles
+    c = EulerSolver3D(d, U0=U0, conf=Struct(cfl=0.8, gamma=1.4, quirk=quirk))
+    c.run(15, packed=False)     # the reference's [6][C] arrays
     U = U0.copy()
     G = np.zeros((6, int(d.nbfaces)))
     for _ in range(15):
         oracle_lib.euler_step(d, U, G, 0.8, 1.4, int(quirk))
     assert np.array_equal(a.U.cpu().numpy(), U)
     assert np.array_equal(b.U.cpu().numpy(), U)
+    assert np.array_equal(c.U.cpu().numpy(), U)
+    assert b.dt == c.dt
 
 
 def test_backend_compile_reference_signatures(oracle_lib):
