@@ -388,6 +388,31 @@ int mfv_euler3d_rusanov(const mfv_mesh *m, const double *U, const double *G, dou
 int mfv_euler3d_update(const mfv_mesh *m, double *U, const double *rez, double gamma,
                        const double *dt_dev, void *stream);
 
+/* The reference's own argument form of the four Euler kernels: the state as six separate
+ * device arrays (cells [C], ghosts [F]) and the residual as five, so a caller that holds the
+ * reference's arrays passes them without packing them into [6][C]. */
+typedef struct mfv_euler_arrays {
+    double *rho, *P, *rhou, *rhov, *rhow, *rhoE;
+} mfv_euler_arrays;
+typedef struct mfv_euler_rez {
+    double *rho, *rhou, *rhov, *rhow, *rhoE;
+} mfv_euler_rez;
+
+/* ghost_value_TubeSchok3D (fvm_utils3d.py:441-500), with its argument list: ghosts <- cells on
+ * the boundary faces (names 1-6); ug, vg, wg (optional, all or none) <- ghost momentum / rho. */
+int mfv_ghost_value_tubeschok3d(const mfv_mesh *m, const mfv_euler_arrays *ghosts, double *ug, double *vg,
+                                double *wg, const mfv_euler_arrays *cells, int quirk, void *stream);
+/* time_step_euler_3d (fvm_utils3d.py:809-825) as the reference computes it: dt_c[K] per cell
+ * (cells->rhoE is not read). */
+int mfv_time_step_euler_3d(const mfv_mesh *m, const mfv_euler_arrays *cells, double cfl, double gamma,
+                           double *dt_c, void *stream);
+/* explicitscheme_euler_3d_rusanov (fvm_utils3d.py:1083-1170): same bits as mfv_euler3d_rusanov. */
+int mfv_explicitscheme_euler_3d_rusanov(const mfv_mesh *m, const mfv_euler_rez *rez, const mfv_euler_arrays *cells,
+                                        const mfv_euler_arrays *ghosts, double gamma, int quirk, void *stream);
+/* update_euler_3d_fvc (fvm_utils3d.py:1316-1327) with the host dt the reference passes. */
+int mfv_update_euler_3d_fvc(const mfv_mesh *m, const mfv_euler_arrays *cells, const mfv_euler_rez *rez,
+                            double gamma, double dtime, void *stream);
+
 /* Fused step: ghosts on the fly + Rusanov fluxes + update + next dt.
  * U_in -> U_out ([6][C]); dt ring as for mfv_advection_step. */
 int mfv_euler3d_step_fused(const mfv_mesh *m, const double *U_in, double *U_out, double gamma,
@@ -610,7 +635,7 @@ typedef struct mfv_ell_batch {
  * TOL_OUT, TOL_IN, MAXIT_IN, MAXIT_OUT, fills x (initial guess) and b (zero on padding rows)
  * and sets status START (which restarts the refinement count OUTER; ITTOT keeps adding up).
  * Ticks then move it through INNER/CORRECT to DONE (||b - op(x)|| <= TOL_OUT ||b||) or FAILED
- * (more than MAXIT_OUT refinements). */
+ * (more than MAXIT_OUT refinements, or a residual or right-hand side that is not finite). */
 #define MFV_KS_IDLE 0
 #define MFV_KS_START 1
 #define MFV_KS_INNER 2

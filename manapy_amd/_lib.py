@@ -84,6 +84,16 @@ class MfvEulerTetTiles(ctypes.Structure):
                           'cell_code')]
 
 
+class MfvEulerArrays(ctypes.Structure):
+    """struct mfv_euler_arrays: six device arrays rho, P, rhou, rhov, rhow, rhoE."""
+    _fields_ = [(k, P) for k in ('rho', 'P', 'rhou', 'rhov', 'rhow', 'rhoE')]
+
+
+class MfvEulerRez(ctypes.Structure):
+    """struct mfv_euler_rez: five device residual arrays rho, rhou, rhov, rhow, rhoE."""
+    _fields_ = [(k, P) for k in ('rho', 'rhou', 'rhov', 'rhow', 'rhoE')]
+
+
 class MfvSwPacked(ctypes.Structure):
     """Mirror of ``struct mfv_sw_packed`` (include/mfv.h)."""
     _fields_ = [('nb_cells', ctypes.c_int32), ('max_cell_faces', ctypes.c_int32), ('cellrec', P), ('facerec', P),
@@ -181,6 +191,13 @@ _SIGNATURES = {
     'mfv_euler3d_time_step': [MESH_P, P, c_double, c_double, P, DP, P],
     'mfv_euler3d_rusanov': [MESH_P, P, P, c_double, c_int, P, P],
     'mfv_euler3d_update': [MESH_P, P, P, c_double, P, P],
+    'mfv_ghost_value_tubeschok3d': [MESH_P, ctypes.POINTER(MfvEulerArrays), P, P, P, ctypes.POINTER(MfvEulerArrays),
+                                    c_int, P],
+    'mfv_time_step_euler_3d': [MESH_P, ctypes.POINTER(MfvEulerArrays), c_double, c_double, P, P],
+    'mfv_explicitscheme_euler_3d_rusanov': [MESH_P, ctypes.POINTER(MfvEulerRez), ctypes.POINTER(MfvEulerArrays),
+                                            ctypes.POINTER(MfvEulerArrays), c_double, c_int, P],
+    'mfv_update_euler_3d_fvc': [MESH_P, ctypes.POINTER(MfvEulerArrays), ctypes.POINTER(MfvEulerRez), c_double,
+                                c_double, P],
     'mfv_euler3d_step_fused': [MESH_P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_euler3d_step_tiled': [MESH_P, FT_P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_face_tiles_build': [c_int, c_int, P, P, c_int, P, c_int, ctypes.POINTER(MfvFaceTilesHost)],

@@ -287,60 +287,79 @@ def _source_srnh(domain, src_h, src_hu, src_hv, src_hc, src_Z, h_c, hu_c, hv_c, 
     A.finish()
 
 
+def _euler_arrays(A, arrays, out=False):
+    """mfv_euler_arrays over six reference arrays (device tensors used in place; NumPy arrays
+    copied, and written back by A.finish() when ``out``)."""
+    ts = [A.dev(x, out=out) for x in arrays]
+    return _lib.MfvEulerArrays(*[t.data_ptr() for t in ts]), ts
+
+
+def _euler_rez(A, arrays, out=False):
+    ts = [A.dev(x, out=out) for x in arrays]
+    return _lib.MfvEulerRez(*[t.data_ptr() for t in ts]), ts
+
+
 @HIPBackend.register('explicitscheme_euler_3d_rusanov')
 def _euler_rusanov(domain, rez_rho, rez_rhou, rez_rhov, rez_rhow, rez_rhoE, rho_c, P_c, rhou_c, rhov_c, rhow_c,
                    rhoE_c, rho_g, P_g, rhou_g, rhov_g, rhow_g, rhoE_g, rho_h, P_h, rhou_h, rhov_h, rhow_h, rhoE_h,
                    cellidf, halofid, normal, mesurf, name, gamma, tangent, binormal, quirk=1):
     """solvers/euler/fvm_utils3d.py:1083-1170 (quirk=1: the reference's flux, Q1)."""
-    import torch
     m = _m(domain)
+    _check_len(cellidf, m.F, "cellidf")
     A = _Args(m.device)
-    U = torch.stack([A.dev(x) for x in (rho_c, P_c, rhou_c, rhov_c, rhow_c, rhoE_c)]).contiguous()
-    G = torch.stack([A.dev(x) for x in (rho_g, P_g, rhou_g, rhov_g, rhow_g, rhoE_g)]).contiguous()
-    rez = torch.empty((5, m.C), dtype=torch.float64, device=m.device)
-    _lib.call('mfv_euler3d_rusanov', m.ptr, U.data_ptr(), G.data_ptr(), float(gamma), int(quirk), rez.data_ptr(),
-              stream_ptr())
-    for k, out in enumerate((rez_rho, rez_rhou, rez_rhov, rez_rhow, rez_rhoE)):
-        t = A.dev(out, out=True)
-        t.copy_(rez[k])
+    rez, keep_r = _euler_rez(A, (rez_rho, rez_rhou, rez_rhov, rez_rhow, rez_rhoE), out=True)
+    U, keep_u = _euler_arrays(A, (rho_c, P_c, rhou_c, rhov_c, rhow_c, rhoE_c))
+    G, keep_g = _euler_arrays(A, (rho_g, P_g, rhou_g, rhov_g, rhow_g, rhoE_g))
+    _lib.call('mfv_explicitscheme_euler_3d_rusanov', m.ptr, ctypes.byref(rez), ctypes.byref(U), ctypes.byref(G),
+              float(gamma), int(quirk), stream_ptr())
     A.finish()
+    del keep_r, keep_u, keep_g
+
+
+@HIPBackend.register('time_step_euler_3d')
+def _euler_time_step(domain, rho, P, rhou, rhov, rhow, cfl, normal, mesure, volume, faceid, gamma, dt_c):
+    """solvers/euler/fvm_utils3d.py:809-825: dt_c[K] per cell (the caller takes the min)."""
+    m = _m(domain)
+    _check_len(faceid, m.C, "faceid")
+    A = _Args(m.device)
+    ts = [A.dev(x) for x in (rho, P, rhou, rhov, rhow)]
+    U = _lib.MfvEulerArrays(*[t.data_ptr() for t in ts], ts[0].data_ptr())
+    out = A.dev(dt_c, out=True)
+    _lib.call('mfv_time_step_euler_3d', m.ptr, ctypes.byref(U), float(cfl), float(gamma), out.data_ptr(),
+              stream_ptr())
+    A.finish()
+    del ts
 
 
 @HIPBackend.register('update_euler_3d_fvc')
 def _euler_update(domain, rho_c, P_c, rhou_c, rhov_c, rhow_c, rhoE_c, rez_rho, rez_rhou, rez_rhov, rez_rhow,
                   rez_rhoE, gamma, dtime, vol):
     """solvers/euler/fvm_utils3d.py:1316-1327."""
-    import torch
     m = _m(domain)
+    _check_len(vol, m.C, "vol")
     A = _Args(m.device)
-    outs = [A.dev(x, out=True) for x in (rho_c, P_c, rhou_c, rhov_c, rhow_c, rhoE_c)]
-    U = torch.stack(outs).contiguous()
-    rez = torch.stack([A.dev(x) for x in (rez_rho, rez_rhou, rez_rhov, rez_rhow, rez_rhoE)]).contiguous()
-    dt = torch.full((1,), float(dtime), dtype=torch.float64, device=m.device)
-    _lib.call('mfv_euler3d_update', m.ptr, U.data_ptr(), rez.data_ptr(), float(gamma), dt.data_ptr(), stream_ptr())
-    for k, t in enumerate(outs):
-        t.copy_(U[k])
+    U, keep_u = _euler_arrays(A, (rho_c, P_c, rhou_c, rhov_c, rhow_c, rhoE_c), out=True)
+    rez, keep_r = _euler_rez(A, (rez_rho, rez_rhou, rez_rhov, rez_rhow, rez_rhoE))
+    _lib.call('mfv_update_euler_3d_fvc', m.ptr, ctypes.byref(U), ctypes.byref(rez), float(gamma), float(dtime),
+              stream_ptr())
     A.finish()
+    del keep_u, keep_r
 
 
 @HIPBackend.register('ghost_value_TubeSchok3D')
 def _euler_ghost(domain, rhog, Pg, rhoug, rhovg, rhowg, ug, vg, wg, rhoEg, rhoc, Pc, rhouc, rhovc, rhowc, rhoEc,
                  cellid, name, normal, mesure, quirk=1):
     """solvers/euler/fvm_utils3d.py:441-500 (quirk=1: the reference's slip ghost, Q2)."""
-    import torch
     m = _m(domain)
+    _check_len(cellid, m.F, "cellid")
     A = _Args(m.device)
-    gouts = [A.dev(x, out=True) for x in (rhog, Pg, rhoug, rhovg, rhowg, rhoEg)]
-    G = torch.stack(gouts).contiguous()
-    U = torch.stack([A.dev(x) for x in (rhoc, Pc, rhouc, rhovc, rhowc, rhoEc)]).contiguous()
-    _lib.call('mfv_euler3d_ghost', m.ptr, U.data_ptr(), G.data_ptr(), int(quirk), stream_ptr())
-    for k, t in enumerate(gouts):
-        t.copy_(G[k])
-    bf = torch.as_tensor(domain.boundaryfaces, dtype=torch.long, device=m.device)
-    for vel, mom in ((ug, 2), (vg, 3), (wg, 4)):
-        t = A.dev(vel, out=True)
-        t[bf] = G[mom][bf] / G[0][bf]
+    G, keep_g = _euler_arrays(A, (rhog, Pg, rhoug, rhovg, rhowg, rhoEg), out=True)
+    vel = [A.dev(x, out=True) for x in (ug, vg, wg)]
+    U, keep_u = _euler_arrays(A, (rhoc, Pc, rhouc, rhovc, rhowc, rhoEc))
+    _lib.call('mfv_ghost_value_tubeschok3d', m.ptr, ctypes.byref(G), *[t.data_ptr() for t in vel], ctypes.byref(U),
+              int(quirk), stream_ptr())
     A.finish()
+    del keep_g, keep_u
 
 
 # ---------------------------------------------------------------------------------------------

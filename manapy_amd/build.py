@@ -26,8 +26,31 @@ def sources():
     return [os.path.join(CSRC, s) for s in SOURCES]
 
 
-def build_library(verbose=False, extra=()):
-    cmd = [HIPCC] + FLAGS + list(extra) + sources() + ['-o', LIB + '.tmp']
+def build_library(verbose=False, extra=(), jobs=None):
+    """Compile every source to an object in parallel (one hipcc per file), then link."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, '_build')
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in FLAGS if f != '-shared'] + list(extra)
+    objs = [os.path.join(objdir, os.path.basename(s) + '.o') for s in sources()]
+
+    def compile_one(pair):
+        src, obj = pair
+        cmd = [HIPCC] + cflags + ['-c', src, '-o', obj]
+        if verbose:
+            print(' '.join(cmd), flush=True)
+        return subprocess.run(cmd, capture_output=True, text=True)
+
+    jobs = jobs or min(len(objs), os.cpu_count() or 4, 8)
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(compile_one, zip(sources(), objs)))
+    for src, r in zip(sources(), results):
+        err = '\n'.join(l for l in r.stderr.splitlines() if 'packed-fp32-ops' not in l)
+        if err.strip():
+            sys.stderr.write(err + '\n')
+        if r.returncode != 0:
+            raise subprocess.CalledProcessError(r.returncode, 'hipcc ' + src)
+    cmd = [HIPCC, '-shared', '-fPIC', '--offload-arch=' + ARCH] + objs + ['-o', LIB + '.tmp']
     if verbose:
         print(' '.join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -295,7 +295,10 @@ namespace mfv {
 
 // Per-wave dt partial minima live in one buffer per (device, stream): work queued on one
 // stream runs in order, so the buffer is reused without a race, and calls on different
-// streams never share one. A buffer only grows (hipFree waits for the device).
+// streams never share one. A buffer only grows. A smaller buffer it replaces is never freed:
+// a graph captured on that stream (by the caller, e.g. torch) may still hold its address and
+// be replayed later, so retired buffers stay allocated until the process ends. Growth is by
+// doubling, so the retired buffers together are smaller than the live one.
 struct DtScratch {
     int dev;
     hipStream_t stream;
@@ -304,6 +307,7 @@ struct DtScratch {
 };
 static std::mutex g_dt_mu;
 static std::vector<DtScratch> g_dt_scr;
+static std::vector<double *> g_dt_retired;
 
 void graph_free(MfvGraph *g)
 {
@@ -359,7 +363,14 @@ double *dt_scratch(size_t nwaves, hipStream_t st)
     }
     if (e->cap < nwaves) {
         const size_t cap = e->cap * 2 > nwaves ? e->cap * 2 : nwaves;
-        if (e->ptr) (void)hipFree(e->ptr);
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+            set_error("dt scratch: the stream is being captured and its partials buffer must grow to %zu doubles; "
+                      "run one uncaptured call of the same size first",
+                      nwaves);
+            return nullptr;
+        }
+        if (e->ptr) g_dt_retired.push_back(e->ptr);
         e->ptr = nullptr;
         e->cap = 0;
         if (hipMalloc(reinterpret_cast<void **>(&e->ptr), cap * sizeof(double)) != hipSuccess) {

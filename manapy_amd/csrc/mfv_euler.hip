@@ -54,6 +54,26 @@ __device__ __forceinline__ void euler_flux(const Prim6 &L, const Prim6 &R, const
     fl[4] = (0.5 * (fl_rhoE + fr_rhoE) - 0.5 * S * (rhoEr - rhoEl)) * mesure;
 }
 
+// Where the k-th field of a state (or residual) lives: either [k][n] rows of one array
+// (Rows) or k separate arrays, as the reference passes them (Arrays). Both give field k of
+// element c at f(k)[c].
+struct Rows {
+    double *p;
+    size_t n;
+    __device__ __forceinline__ double *f(int k) const { return p + (size_t)k * n; }
+};
+template <int N>
+struct Arrays {
+    double *p[N];
+    __device__ __forceinline__ double *f(int k) const { return p[k]; }
+};
+
+template <class A>
+__device__ __forceinline__ Prim6 load6(const A &U, int c)
+{
+    return Prim6{U.f(0)[c], U.f(1)[c], U.f(2)[c], U.f(3)[c], U.f(4)[c], U.f(5)[c]};
+}
+
 __device__ __forceinline__ Prim6 load6(const double *__restrict__ U, size_t n, int c)
 {
     Prim6 p;
@@ -92,24 +112,30 @@ __device__ __forceinline__ Prim6 euler_ghost(const Prim6 &c, const double *n, in
     return g;
 }
 
-__global__ void __launch_bounds__(kBlock) k_euler_ghost(Mesh m, const int32_t *__restrict__ bnd, int nb,
-                                                         const double *__restrict__ U, double *__restrict__ G,
-                                                         int quirk)
+// vel (optional, Arrays<3> of [F]): the ghost velocities ug, vg, wg the reference also writes,
+// each the ghost momentum over the ghost density.
+template <class A>
+__global__ void __launch_bounds__(kBlock) k_euler_ghost(Mesh m, const int32_t *__restrict__ bnd, int nb, A U, A G,
+                                                         int quirk, Arrays<3> vel)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nb) return;
     int f = bnd[k];
     int name = m.face_name[f];
     if (!(name >= 1 && name <= 6)) return;
-    const size_t C = m.C, F = m.F;
-    Prim6 c = load6(U, C, m.face_cells[2 * f]);
-    Prim6 g = euler_ghost(c, m.face_normal + 3 * (size_t)f, name, quirk, G[4 * F + f]);
-    G[f] = g.rho;
-    G[F + f] = g.P;
-    G[2 * F + f] = g.ru;
-    G[3 * F + f] = g.rv;
-    G[4 * F + f] = g.rw;
-    G[5 * F + f] = g.rE;
+    Prim6 c = load6(U, m.face_cells[2 * f]);
+    Prim6 g = euler_ghost(c, m.face_normal + 3 * (size_t)f, name, quirk, G.f(4)[f]);
+    G.f(0)[f] = g.rho;
+    G.f(1)[f] = g.P;
+    G.f(2)[f] = g.ru;
+    G.f(3)[f] = g.rv;
+    G.f(4)[f] = g.rw;
+    G.f(5)[f] = g.rE;
+    if (vel.p[0]) {
+        vel.p[0][f] = g.ru / g.rho;
+        vel.p[1][f] = g.rv / g.rho;
+        vel.p[2][f] = g.rw / g.rho;
+    }
 }
 
 // time_step_euler_3d (fvm_utils3d.py:809-825): first four faces of the cell.
@@ -140,16 +166,24 @@ __global__ void __launch_bounds__(kBlock) k_euler_dt(Mesh m, const double *__res
     wave_min_part(dtpart, (size_t)i, d);
 }
 
+// time_step_euler_3d in the reference's form: the per-cell dt_c array (the caller takes the min).
+__global__ void __launch_bounds__(kBlock) k_euler_dt_cells(Mesh m, Arrays<5> U, double cfl, double gamma,
+                                                            double *__restrict__ dt_c)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.C) return;
+    dt_c[i] = euler_cell_dt(m, i, U.f(0)[i], U.f(1)[i], U.f(2)[i], U.f(3)[i], U.f(4)[i], cfl, gamma);
+}
+
 // Residual of cell i, faces in increasing id (the single all-faces loop of
 // explicitscheme_euler_3d_rusanov). GHOST_ARRAY: right state of boundary faces from G; else
 // computed from the owner with euler_ghost (rhowg_prev = 0 under quirk).
-template <bool GHOST_ARRAY>
-__device__ __forceinline__ void euler_cell_residual(const Mesh &m, int i, const double *__restrict__ U,
-                                                    const double *__restrict__ G, double gamma, int quirk,
-                                                    double (&rez)[5])
+template <bool GHOST_ARRAY, class A>
+__device__ __forceinline__ void euler_cell_residual(const Mesh &m, int i, const A &U, const A &G, double gamma,
+                                                    int quirk, double (&rez)[5])
 {
     for (int v = 0; v < 5; v++) rez[v] = 0.;
-    const size_t C = m.C, F = m.F;
+    const size_t C = m.C;
     const int nf = m.cell_nfaces[i];
     for (int r = 0; r < nf; r++) {
         int s = m.order_sorted[(size_t)r * C + i];
@@ -158,13 +192,13 @@ __device__ __forceinline__ void euler_cell_residual(const Mesh &m, int i, const 
         const double mesu = m.face_mesure[f];
         const double *nrm = m.face_normal + 3 * (size_t)f;
         double n[3] = {nrm[0] / mesu, nrm[1] / mesu, nrm[2] / mesu};
-        Prim6 L = load6(U, C, a);
+        Prim6 L = load6(U, a);
         Prim6 R;
         int name = m.face_name[f];
         if (name == 0) {
-            R = load6(U, C, m.face_cells[2 * f + 1]);
+            R = load6(U, m.face_cells[2 * f + 1]);
         } else if (GHOST_ARRAY) {
-            R = load6(G, F, f);
+            R = load6(G, f);
         } else {
             R = euler_ghost(L, nrm, name, quirk, 0.);
         }
@@ -178,16 +212,14 @@ __device__ __forceinline__ void euler_cell_residual(const Mesh &m, int i, const 
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_euler_rusanov(Mesh m, const double *__restrict__ U,
-                                                           const double *__restrict__ G, double gamma, int quirk,
-                                                           double *__restrict__ rez)
+template <class A, class R>
+__global__ void __launch_bounds__(kBlock) k_euler_rusanov(Mesh m, A U, A G, double gamma, int quirk, R rez)
 {
     int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= m.C) return;
     double r[5];
     euler_cell_residual<true>(m, i, U, G, gamma, quirk, r);
-    const size_t C = m.C;
-    for (int v = 0; v < 5; v++) rez[v * C + i] = r[v];
+    for (int v = 0; v < 5; v++) rez.f(v)[i] = r[v];
 }
 
 __device__ __forceinline__ void euler_update_cell(double (&u)[6], const double (&r)[5], double gamma, double dt,
@@ -202,19 +234,19 @@ __device__ __forceinline__ void euler_update_cell(double (&u)[6], const double (
     u[1] = (gamma - 1) * (u[5] - 0.5 * (u[2] * u[2] + u[3] * u[3] + u[4] * u[4]) / u[0]);
 }
 
-__global__ void __launch_bounds__(kBlock) k_euler_update(int C, double *__restrict__ U,
-                                                          const double *__restrict__ rez, double gamma,
-                                                          const double *__restrict__ dt_dev,
+// dt from dt_dev[0] (device scalar), or from dt_host when dt_dev is NULL.
+template <class A, class R>
+__global__ void __launch_bounds__(kBlock) k_euler_update(int C, A U, R rez, double gamma,
+                                                          const double *__restrict__ dt_dev, double dt_host,
                                                           const double *__restrict__ vol)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C) return;
-    const size_t n = C;
     double u[6], r[5];
-    for (int v = 0; v < 6; v++) u[v] = U[v * n + i];
-    for (int v = 0; v < 5; v++) r[v] = rez[v * n + i];
-    euler_update_cell(u, r, gamma, dt_dev[0], vol[i]);
-    for (int v = 0; v < 6; v++) U[v * n + i] = u[v];
+    for (int v = 0; v < 6; v++) u[v] = U.f(v)[i];
+    for (int v = 0; v < 5; v++) r[v] = rez.f(v)[i];
+    euler_update_cell(u, r, gamma, dt_dev ? dt_dev[0] : dt_host, vol[i]);
+    for (int v = 0; v < 6; v++) U.f(v)[i] = u[v];
 }
 
 __global__ void __launch_bounds__(kBlock) k_euler_step(Mesh m, const double *__restrict__ Uin,
@@ -227,7 +259,8 @@ __global__ void __launch_bounds__(kBlock) k_euler_step(Mesh m, const double *__r
     if (i < m.C) {
         const size_t C = m.C;
         double r[5];
-        euler_cell_residual<false>(m, i, Uin, nullptr, gamma, quirk, r);
+        const Rows rows{const_cast<double *>(Uin), C};
+        euler_cell_residual<false>(m, i, rows, rows, gamma, quirk, r);
         double u[6];
         for (int v = 0; v < 6; v++) u[v] = Uin[v * C + i];
         euler_update_cell(u, r, gamma, ring.cur[0], m.cell_volume[i]);
@@ -721,9 +754,76 @@ extern "C" int mfv_euler3d_ghost(const mfv_mesh *m, const double *U, double *G, 
     MFV_REQUIRE(U && G, "NULL array");
     if (m->nb_bnd == 0) return MFV_OK;
     MFV_REQUIRE(m->bnd_faces, "mesh descriptor lacks bnd_faces");
-    hipLaunchKernelGGL(k_euler_ghost, dim3(grid_for(m->nb_bnd)), dim3(kBlock), 0, as_stream(stream), to_kernel(m),
-                       m->bnd_faces, m->nb_bnd, U, G, quirk);
+    const Rows u{const_cast<double *>(U), (size_t)m->nb_cells}, g{G, (size_t)m->nb_faces};
+    hipLaunchKernelGGL(k_euler_ghost<Rows>, dim3(grid_for(m->nb_bnd)), dim3(kBlock), 0, as_stream(stream),
+                       to_kernel(m), m->bnd_faces, m->nb_bnd, u, g, quirk, Arrays<3>{{nullptr, nullptr, nullptr}});
     return check_launch("k_euler_ghost");
+}
+
+static Arrays<6> arrays6(const mfv_euler_arrays *a)
+{
+    return Arrays<6>{{a->rho, a->P, a->rhou, a->rhov, a->rhow, a->rhoE}};
+}
+
+static bool complete(const mfv_euler_arrays *a, bool need_P = true)
+{
+    return a && a->rho && (a->P || !need_P) && a->rhou && a->rhov && a->rhow && a->rhoE;
+}
+
+extern "C" int mfv_ghost_value_tubeschok3d(const mfv_mesh *m, const mfv_euler_arrays *ghosts, double *ug, double *vg,
+                                           double *wg, const mfv_euler_arrays *cells, int quirk, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    MFV_REQUIRE(m->dim == 3, "Euler 3D needs a 3D mesh");
+    MFV_REQUIRE(complete(ghosts) && complete(cells), "NULL Euler array");
+    MFV_REQUIRE((ug && vg && wg) || (!ug && !vg && !wg), "ug, vg, wg must be all given or all NULL");
+    if (m->nb_bnd == 0) return MFV_OK;
+    MFV_REQUIRE(m->bnd_faces, "mesh descriptor lacks bnd_faces");
+    hipLaunchKernelGGL(k_euler_ghost<Arrays<6>>, dim3(grid_for(m->nb_bnd)), dim3(kBlock), 0, as_stream(stream),
+                       to_kernel(m), m->bnd_faces, m->nb_bnd, arrays6(cells), arrays6(ghosts), quirk,
+                       Arrays<3>{{ug, vg, wg}});
+    return check_launch("k_euler_ghost");
+}
+
+extern "C" int mfv_time_step_euler_3d(const mfv_mesh *m, const mfv_euler_arrays *cells, double cfl, double gamma,
+                                      double *dt_c, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    MFV_REQUIRE(m->dim == 3 && m->max_cell_faces >= 4, "Euler 3D time step needs tetrahedra");
+    MFV_REQUIRE(complete(cells, true) && dt_c, "NULL array");
+    if (m->nb_cells == 0) return MFV_OK;
+    const Arrays<5> u{{cells->rho, cells->P, cells->rhou, cells->rhov, cells->rhow}};
+    hipLaunchKernelGGL(k_euler_dt_cells, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                       to_kernel(m), u, cfl, gamma, dt_c);
+    return check_launch("k_euler_dt_cells");
+}
+
+extern "C" int mfv_explicitscheme_euler_3d_rusanov(const mfv_mesh *m, const mfv_euler_rez *rez,
+                                                   const mfv_euler_arrays *cells, const mfv_euler_arrays *ghosts,
+                                                   double gamma, int quirk, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 1)) return rc;
+    MFV_REQUIRE(m->dim == 3, "Euler 3D needs a 3D mesh");
+    MFV_REQUIRE(rez && rez->rho && rez->rhou && rez->rhov && rez->rhow && rez->rhoE, "NULL residual array");
+    MFV_REQUIRE(complete(cells) && complete(ghosts), "NULL Euler array");
+    if (m->nb_cells == 0) return MFV_OK;
+    const Arrays<5> r{{rez->rho, rez->rhou, rez->rhov, rez->rhow, rez->rhoE}};
+    hipLaunchKernelGGL((k_euler_rusanov<Arrays<6>, Arrays<5>>), dim3(grid_for(m->nb_cells)), dim3(kBlock), 0,
+                       as_stream(stream), to_kernel(m), arrays6(cells), arrays6(ghosts), gamma, quirk, r);
+    return check_launch("k_euler_rusanov");
+}
+
+extern "C" int mfv_update_euler_3d_fvc(const mfv_mesh *m, const mfv_euler_arrays *cells, const mfv_euler_rez *rez,
+                                       double gamma, double dtime, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    MFV_REQUIRE(rez && rez->rho && rez->rhou && rez->rhov && rez->rhow && rez->rhoE, "NULL residual array");
+    MFV_REQUIRE(complete(cells), "NULL Euler array");
+    if (m->nb_cells == 0) return MFV_OK;
+    const Arrays<5> r{{rez->rho, rez->rhou, rez->rhov, rez->rhow, rez->rhoE}};
+    hipLaunchKernelGGL((k_euler_update<Arrays<6>, Arrays<5>>), dim3(grid_for(m->nb_cells)), dim3(kBlock), 0,
+                       as_stream(stream), m->nb_cells, arrays6(cells), r, gamma, nullptr, dtime, m->cell_volume);
+    return check_launch("k_euler_update");
 }
 
 extern "C" int mfv_euler3d_time_step(const mfv_mesh *m, const double *U, double cfl, double gamma, double *dt_dev,
@@ -760,8 +860,9 @@ extern "C" int mfv_euler3d_rusanov(const mfv_mesh *m, const double *U, const dou
     MFV_REQUIRE(m->dim == 3, "Euler 3D needs a 3D mesh");
     if (m->nb_cells == 0) return MFV_OK;
     MFV_REQUIRE(U && G && rez, "NULL array");
-    hipLaunchKernelGGL(k_euler_rusanov, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
-                       to_kernel(m), U, G, gamma, quirk, rez);
+    const Rows u{const_cast<double *>(U), (size_t)m->nb_cells}, g{const_cast<double *>(G), (size_t)m->nb_faces};
+    hipLaunchKernelGGL((k_euler_rusanov<Rows, Rows>), dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                       to_kernel(m), u, g, gamma, quirk, Rows{rez, (size_t)m->nb_cells});
     return check_launch("k_euler_rusanov");
 }
 
@@ -771,8 +872,9 @@ extern "C" int mfv_euler3d_update(const mfv_mesh *m, double *U, const double *re
     if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
     if (m->nb_cells == 0) return MFV_OK;
     MFV_REQUIRE(U && rez && dt_dev, "NULL array");
-    hipLaunchKernelGGL(k_euler_update, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), m->nb_cells,
-                       U, rez, gamma, dt_dev, m->cell_volume);
+    hipLaunchKernelGGL((k_euler_update<Rows, Rows>), dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                       m->nb_cells, Rows{U, (size_t)m->nb_cells}, Rows{const_cast<double *>(rez), (size_t)m->nb_cells},
+                       gamma, dt_dev, 0., m->cell_volume);
     return check_launch("k_euler_update");
 }
 
@@ -820,7 +922,9 @@ extern "C" int mfv_euler3d_step_tiled(const mfv_mesh *m, const mfv_face_tiles *t
     if (m->nb_cells == 0) return MFV_OK;
     MFV_REQUIRE(U_in && U_out && U_in != U_out && dt_ring_p, "NULL or aliased array");
     DtRing r = dt_ring(dt_ring_p, step);
-    const size_t nw = waves_for(m->nb_cells);
+    // Tiles start on 64-cell boundaries and waves past a tile's cells return early, so exactly
+    // ceil(C/64) partials are written: fold those and no stale slot beyond them.
+    const size_t nw = ((size_t)m->nb_cells + 63) / 64;
     double *part = dt_scratch(nw, as_stream(stream));
     if (!part) return MFV_ENOMEM;
     const FaceTilesK tk{t->tile_cell0, t->tile_ncells, t->halo_ptr, t->halo_cells, t->face_ptr, t->face_id,
@@ -1016,7 +1120,8 @@ extern "C" int mfv_euler3d_step_packed_tiled(const mfv_mesh *m, const mfv_euler_
     MFV_REQUIRE(rec_in && rec_out && rec_in != rec_out && dt_ring_p, "NULL or aliased array");
     MFV_REQUIRE(((uintptr_t)rec_in & 15) == 0 && ((uintptr_t)rec_out & 15) == 0, "misaligned records");
     DtRing r = dt_ring(dt_ring_p, step);
-    const size_t nw = waves_for(m->nb_cells);
+    // One partial per 64-cell wave that holds cells (tiles are whole multiples of 64 cells).
+    const size_t nw = ((size_t)m->nb_cells + 63) / 64;
     hipStream_t st = as_stream(stream);
     double *part = dt_scratch(nw, st);
     if (!part) return MFV_ENOMEM;

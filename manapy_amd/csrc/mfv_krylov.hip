@@ -208,7 +208,8 @@ __global__ void __launch_bounds__(256) k_ir_check(Ell e, Work w)
             sc[MFV_KS_OUTER] += 1.;
             const double tol = sc[MFV_KS_TOL_OUT];
             int st;
-            if (resn2 <= tol * tol * bn2) st = MFV_KS_DONE;
+            if (!isfinite(resn2) || !isfinite(bn2)) st = MFV_KS_FAILED;  // NaN/inf data: refinement cannot help
+            else if (resn2 <= tol * tol * bn2) st = MFV_KS_DONE;
             else if (sc[MFV_KS_OUTER] > sc[MFV_KS_MAXIT_OUT]) st = MFV_KS_FAILED;
             else {
                 sc[MFV_KS_RHO] = sum[2];

@@ -28,7 +28,7 @@ from . import _lib
 from .device import DeviceMesh, require_gpu, stream_ptr
 from .krylov import EllBatch
 
-PHASE_WAIT, PHASE_ADVECT, PHASE_SOLVE, PHASE_DONE = 0, 1, 2, 3
+PHASE_WAIT, PHASE_ADVECT, PHASE_SOLVE, PHASE_DONE, PHASE_FAILED, PHASE_REJECTED = 0, 1, 2, 3, 4, 5
 
 
 class AdvDiffRequest(object):
@@ -51,7 +51,12 @@ class AdvDiffRequest(object):
         self.t_admit = self.t_done = None
         self.result = None
         self.iterations = 0
+        self.error = None
         self._dev = None
+
+    @property
+    def failed(self):
+        return self.phase in (PHASE_FAILED, PHASE_REJECTED)
 
     def prepare(self):
         """Host-side tables of the request's mesh (GPU layout and diamond weights), built
@@ -87,6 +92,16 @@ class ServingEngine(object):
         self.max_active = 0
 
     # -- admission --------------------------------------------------------------------------------
+    def never_fits(self, req):
+        """Why ``req`` can never be admitted by this engine (None if it can): its footprint
+        exceeds the HBM budget, or its matrix exceeds the whole arena."""
+        fp = req.footprint(self.arena)
+        if self.hbm_budget is not None and fp > self.hbm_budget:
+            return "footprint %d B exceeds the HBM budget %d B" % (fp, self.hbm_budget)
+        if (req.C + 63) // 64 > self.arena.S:
+            return "%d cells need %d slices, the arena has %d" % (req.C, (req.C + 63) // 64, self.arena.S)
+        return None
+
     def can_admit(self, req):
         if req.footprint(self.arena) + self.in_use > (self.hbm_budget or 1 << 62):
             return False
@@ -152,34 +167,55 @@ class ServingEngine(object):
         a.status[req.inst] = _lib.KS_START
         req.phase = PHASE_SOLVE
 
-    def _finish(self, req, now):
+    def _finish(self, req, now, error=None):
+        """Retire ``req``: its result (or ``error`` if its solve failed), then its arena
+        instance and device tables are released so that the other requests go on."""
         row0 = req._dev['row0']
-        req.result = self.arena.x[row0:row0 + req.C].cpu().numpy().copy()
+        if error is None:
+            req.result = self.arena.x[row0:row0 + req.C].cpu().numpy().copy()
         req.iterations = int(self.arena.scalars(req.inst)[_lib.KS_ITTOT])
         self.arena.release(req.inst)
         self.arena.publish()
         self.in_use -= req.footprint(self.arena)
         req._dev = None
         req.domain._device_mesh = None
-        req.phase = PHASE_DONE
+        req.phase = PHASE_DONE if error is None else PHASE_FAILED
+        req.error = error
         req.t_done = now
         self.active.remove(req)
 
     # -- the loop -------------------------------------------------------------------------------
     def run(self, requests, clock=None, max_ticks=10 ** 7):
         """Serve ``requests`` (sorted by arrival time, seconds from the start); returns them
-        with t_admit / t_done set and ``result`` filled."""
+        with t_admit / t_done set and ``result`` filled.
+
+        A request whose solve fails (status FAILED: no convergence within the outer-iteration
+        limit, or a non-finite residual) is returned with ``phase == PHASE_FAILED`` and
+        ``error`` set; its instance is released and the other requests go on unchanged. A
+        request that can never be admitted (``never_fits``) is returned at once with
+        ``phase == PHASE_REJECTED``."""
         import torch
         pending = sorted(requests, key=lambda r: r.arrival)
         clock = clock or time.perf_counter
         t0 = clock()
         done = []
+        for r in list(pending):
+            why = self.never_fits(r)
+            if why is not None:
+                pending.remove(r)
+                r.phase, r.error = PHASE_REJECTED, "rejected: " + why
+                r.t_admit = r.t_done = r.arrival
+                done.append(r)
         for _ in range(max_ticks):
             now = clock() - t0
             admitted = False
             while pending and pending[0].arrival <= now and self.can_admit(pending[0]):
                 self.admit(pending.pop(0), now)
                 admitted = True
+            if pending and not self.active and not admitted and pending[0].arrival <= now:
+                # an empty arena that cannot take the head request: never_fits covers budget and
+                # size, so only a full instance table is left, which an empty arena cannot have
+                raise _lib.MfvError("request %s cannot be admitted into an empty arena" % pending[0].rid)
             if admitted:
                 self.arena.publish()
                 self.max_active = max(self.max_active, len(self.active))
@@ -207,7 +243,8 @@ class ServingEngine(object):
                     else:
                         r.phase = PHASE_ADVECT
                 elif s == _lib.KS_FAILED:
-                    raise _lib.MfvError("request %s: implicit solve failed at step %d" % (r.rid, r.step))
+                    self._finish(r, now, error="implicit solve failed at step %d" % r.step)
+                    done.append(r)
         else:
             raise _lib.MfvError("serving loop did not finish in %d ticks" % max_ticks)
         torch.cuda.synchronize()
@@ -215,6 +252,6 @@ class ServingEngine(object):
 
 
 def latency_stats(done):
-    lat = np.array([r.t_done - r.arrival for r in done])
+    lat = np.array([r.t_done - r.arrival for r in done if r.phase == PHASE_DONE])
     return {'p50_s': float(np.percentile(lat, 50)), 'p99_s': float(np.percentile(lat, 99)),
             'mean_s': float(lat.mean()), 'max_s': float(lat.max())}

@@ -186,3 +186,49 @@ def test_backend_compile_reference_signatures(oracle_lib):
     assert dt == oracle_lib.time_step(d, uc, vc, np.zeros(C), 0.8)
     with pytest.raises(NotImplementedError):
         be.compile('srnh_scheme')
+
+
+@pytest.mark.parametrize("quirk", [0, 1])
+def test_backend_compile_euler_driver(oracle_lib, quirk):
+    """A reference Euler driver (ghost_value_TubeSchok3D -> time_step_euler_3d with the min over
+    dt_c -> explicitscheme_euler_3d_rusanov -> update_euler_3d_fvc) binding the kernels through
+    domain.backend.compile with the reference's separate NumPy arrays gets the oracle's bits."""
+    from manapy_amd import initialisation_euler_3d
+    d = make_tet_domain(5, 0.08, 3)
+    be = d.backend
+    C, F = int(d.nbcells), int(d.nbfaces)
+    U = initialisation_euler_3d(d.cells.center)
+    U[3] = 0.05 * np.sin(7 * d.cells.center[:, 1])  # some v, w, so the slip quirk shows
+    U[4] = -0.03 * np.cos(5 * d.cells.center[:, 2])
+    U[5] = 0.5 * (U[2] ** 2 + U[3] ** 2 + U[4] ** 2) / U[0] + U[1] / 0.4
+    cell = [U[k].copy() for k in range(6)]
+    ghost = [np.zeros(F) for _ in range(6)]
+    vel = [np.zeros(F) for _ in range(3)]
+    rez = [np.zeros(C) for _ in range(5)]
+    dtc = np.zeros(C)
+    e = np.zeros(0)
+    ghost_k = be.compile('ghost_value_TubeSchok3D')
+    dt_k = be.compile('time_step_euler_3d')
+    flux_k = be.compile('explicitscheme_euler_3d_rusanov')
+    upd_k = be.compile('update_euler_3d_fvc')
+    Ur, Gr = U.copy(), np.zeros((6, F))
+    for _ in range(6):
+        rho, P, ru, rv, rw, rE = cell
+        ghost_k(ghost[0], ghost[1], ghost[2], ghost[3], ghost[4], vel[0], vel[1], vel[2], ghost[5], rho, P, ru, rv,
+                rw, rE, d.faces.cellid, d.faces.name, d.faces.normal, d.faces.mesure, quirk=quirk)
+        dt_k(rho, P, ru, rv, rw, 0.8, d.faces.normal, d.faces.mesure, d.cells.volume, d.cells.faceid, 1.4, dtc)
+        dt = min(1e6, float(dtc.min()))
+        flux_k(*rez, rho, P, ru, rv, rw, rE, *ghost, e, e, e, e, e, e, d.faces.cellid, d.faces.halofid,
+               d.faces.normal, d.faces.mesure, d.faces.name, 1.4, e, e, quirk=quirk)
+        upd_k(rho, P, ru, rv, rw, rE, *rez, 1.4, dt, d.cells.volume)
+        oracle_lib.euler_ghost(d, Ur, Gr, quirk)
+        dtc_ref = oracle_lib.euler_dt(d, Ur, 0.8, 1.4)
+        assert np.array_equal(dtc, dtc_ref)
+        assert np.array_equal(np.stack(ghost), Gr)
+        bf = d.boundaryfaces
+        assert np.array_equal(vel[0][bf], Gr[2, bf] / Gr[0, bf])
+        assert np.array_equal(vel[2][bf], Gr[4, bf] / Gr[0, bf])
+        rez_ref = oracle_lib.euler_rusanov(d, Ur, Gr, 1.4, quirk)
+        assert np.array_equal(np.stack(rez), rez_ref)
+        oracle_lib.euler_update(d, Ur, rez_ref, 1.4, dt)
+        assert np.array_equal(np.stack(cell), Ur)

@@ -436,6 +436,48 @@ def test_euler_packed_tiled_40_steps(torch, quirk, T, n, oracle_lib):
     same(res[0][1], res[1][1], "dt ring")
 
 
+@pytest.mark.parametrize("packed", [False, True])
+def test_euler_tiled_dt_ignores_stale_partials(torch, packed, oracle_lib):
+    """The tiled Euler steps fold exactly the dt partials their waves write. A finer mesh's dt
+    kernel first fills the stream's partials buffer with smaller values; the tiled step on a
+    7986-cell mesh (125 waves, 3 slots short of a 256-thread grid) must still give the oracle's
+    next dt and state."""
+    from manapy_amd.device import DeviceMesh, EulerTet, EulerTetTiles, FaceTiles
+    from manapy_amd.renumber import renumbered
+    big = make_tet_domain(14, 0.08, 3)
+    mb = DeviceMesh(big, lsq=False)
+    Ub = dev(torch, sod(big))
+    dtb = dev(torch, np.zeros(1))
+    call('mfv_euler3d_time_step', mb.ptr, P(Ub), 0.8, 1.4, P(dtb), None)
+    d = renumbered(make_tet_domain(11, 0.08, 6)) if packed else make_tet_domain(11, 0.08, 6)
+    m = DeviceMesh(d, lsq=False)
+    C, F = int(d.nbcells), int(d.nbfaces)
+    assert (C + 63) // 64 == 125
+    U0 = sod(d)
+    U = U0.copy()
+    G_ = np.zeros((6, F))
+    dt_first = float(np.min(oracle_lib.euler_dt(d, U, 0.8, 1.4)))
+    oracle_lib.euler_step(d, U, G_, 0.8, 1.4, 1)
+    dt_next = float(np.min(oracle_lib.euler_dt(d, U, 0.8, 1.4)))
+    assert float(host(dtb)[0]) < dt_next  # the stale slots would win the min
+    ring = dev(torch, np.array([dt_first, 1e6, 1e6, 0.]))
+    if packed:
+        et, tt = EulerTet(m), EulerTetTiles(m, 256)
+        R = [torch.zeros((C, 8), dtype=torch.float64, device='cuda') for _ in range(2)]
+        tU = dev(torch, U0)
+        call('mfv_euler3d_pack', m.ptr, P(tU), P(R[0]))
+        call('mfv_euler3d_step_packed_tiled', m.ptr, et.ptr, tt.ptr, P(R[0]), P(R[1]), 1.4, 0.8, 1, P(ring), 0)
+        out = dev(torch, np.zeros((6, C)))
+        call('mfv_euler3d_unpack', m.ptr, P(R[1]), P(out))
+    else:
+        ft = FaceTiles(m, 256)
+        S = [dev(torch, U0), dev(torch, np.zeros((6, C)))]
+        call('mfv_euler3d_step_tiled', m.ptr, ft.ptr, P(S[0]), P(S[1]), 1.4, 0.8, 1, P(ring), 0)
+        out = S[1]
+    same(host(out), U, "state after one tiled step")
+    assert float(host(ring)[1]) == dt_next
+
+
 def test_sw_packed_60_steps(torch, sw_case, oracle_lib):
     """The packed path (one 64-byte record per cell, MUSCL values per face slot) gives the
     same bits as the reference step sequence."""

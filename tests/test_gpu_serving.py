@@ -77,3 +77,49 @@ def test_admission_by_hbm_budget():
     done = eng.run(reqs)
     assert len(done) == 4 and eng.max_active == 1
     assert all(r.t_done >= r.t_admit >= r.arrival for r in done)
+
+
+def test_failed_request_is_isolated():
+    """A request whose right-hand side holds a NaN fails alone: the other requests of the mix
+    finish with the bits they have when served alone."""
+    from manapy_amd.serving import ServingEngine, PHASE_DONE, PHASE_FAILED
+    specs = [(0, 18, 15, 1, 3e-3, (1.0, 0.5), 4), (1, 14, 20, 2, 5e-2, (-0.4, 0.9), 5),
+             (2, 16, 16, 3, 1e-2, (0.7, -0.7), 3)]
+    alone = {}
+    for rid, nx, ny, seed, D, vel, n in specs:
+        if rid == 1:
+            continue
+        eng = ServingEngine(max_cells=1 << 14, max_inst=4)
+        r = _request(rid, nx, ny, seed, D, vel, n)
+        eng.run([r])
+        alone[rid] = r.result
+    eng = ServingEngine(max_cells=1 << 14, max_inst=4, n_iter=3)
+    reqs = [_request(rid, nx, ny, seed, D, vel, n, arrival=0.001 * k)
+            for k, (rid, nx, ny, seed, D, vel, n) in enumerate(specs)]
+    reqs[1].u0[len(reqs[1].u0) // 2] = np.nan
+    done = eng.run(reqs)
+    assert len(done) == 3
+    by = {r.rid: r for r in done}
+    assert by[1].phase == PHASE_FAILED and by[1].failed and by[1].result is None and "failed" in by[1].error
+    for rid in (0, 2):
+        assert by[rid].phase == PHASE_DONE, by[rid].error
+        assert np.array_equal(by[rid].result, alone[rid]), rid
+    assert not eng.active and eng.in_use == 0
+
+
+def test_request_that_never_fits_is_rejected():
+    """A request above the HBM budget, or larger than the whole arena, is rejected at
+    submission; the others are served."""
+    from manapy_amd.serving import ServingEngine, PHASE_DONE, PHASE_REJECTED
+    small = _request(0, 10, 10, 5, 1e-2, (1.0, 0.0), 2)
+    big = _request(1, 40, 40, 6, 1e-2, (1.0, 0.0), 2)
+    eng = ServingEngine(max_cells=1 << 14, max_inst=4)
+    eng.hbm_budget = int(1.2 * small.footprint(eng.arena))
+    done = eng.run([big, small])
+    by = {r.rid: r for r in done}
+    assert by[1].phase == PHASE_REJECTED and "budget" in by[1].error
+    assert by[0].phase == PHASE_DONE
+    eng = ServingEngine(max_cells=1 << 10, max_inst=4)
+    huge = _request(2, 30, 30, 7, 1e-2, (1.0, 0.0), 2)
+    done = eng.run([huge])
+    assert done[0].phase == PHASE_REJECTED and "slices" in done[0].error

@@ -270,22 +270,54 @@ def test_sw_mass_conservation_first_order(oracle_lib, sw_dom):
     assert np.all(src[0] == 0)
 
 
+def _y_symmetric_triangles(nx, ny, lx, ly):
+    """Structured triangles on [0, lx] x [0, ly] that are mirror-symmetric about y = ly/2: the
+    squares of the lower half are cut along (i, j) -> (i+1, j+1), those of the upper half along
+    (i+1, j) -> (i, j+1), so the reflection maps every triangle onto another one."""
+    from manapy_amd.meshgen import _tags_from_coords
+    assert ny % 2 == 0
+    ii, jj = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing='xy')
+    x, y = (ii * (lx / nx)).ravel().astype(float), (jj * (ly / ny)).ravel().astype(float)
+    x[(ii == nx).ravel()], y[(jj == ny).ravel()] = lx, ly
+    z = np.zeros_like(x)
+    nodes = np.stack([x, y, z, _tags_from_coords(x, y, z, [(0., lx), (0., ly)], 2)], axis=1)
+    el = []
+    for j in range(ny):
+        for i in range(nx):
+            a = j * (nx + 1) + i
+            b, c, d = a + 1, a + nx + 2, a + nx + 1
+            el += [[a, b, c, 3], [a, c, d, 3]] if j < ny // 2 else [[a, b, d, 3], [b, c, d, 3]]
+    return np.array(el, dtype=np.int32), nodes
+
+
 def test_sw_dam_break_mass_and_symmetry(oracle_lib):
-    """A dam break symmetric in y on a y-symmetric structured mesh stays symmetric and
-    conserves mass (walls) over several second-order steps."""
-    from manapy_amd.meshgen import jittered_triangles
-    el, nd = jittered_triangles(30, 10, 100., 20., 0.0, 0)
+    """A dam break symmetric in y on a y-symmetric structured mesh stays symmetric (h, hu
+    even and hv odd under y -> ly - y, to round-off) and conserves mass (walls) over
+    second-order steps."""
+    ly = 20.
+    el, nd = _y_symmetric_triangles(30, 10, 100., ly)
     d = Domain(2, mesh=(el, nd))
     C = int(d.nbcells)
-    h = np.where(d.cells.center[:, 0] < 50., 2.0, 1.0)
-    hu, hv, Z = np.zeros(C), np.zeros(C), np.zeros(C)
+    cc = d.cells.center
+    key = lambda x, y: np.round(x * 1e6).astype(np.int64) * 10 ** 9 + np.round(y * 1e6).astype(np.int64)
+    idx = {k: i for i, k in enumerate(key(cc[:, 0], cc[:, 1]))}
+    mirror = np.array([idx[k] for k in key(cc[:, 0], ly - cc[:, 1])])
+    assert np.all(mirror[mirror] == np.arange(C)) and np.any(mirror != np.arange(C))
+    # a y-dependent bathymetry breaks the symmetry only if the scheme does
+    Z = 0.2 * np.exp(-((cc[:, 0] - 70.) ** 2 + (cc[:, 1] - ly / 2) ** 2) / 20.)
+    h = np.where(cc[:, 0] < 50., 2.0, 1.0) - Z
+    hu, hv = np.zeros(C), np.zeros(C)
     zgrad = oracle_lib.z_gradient(d, Z)
     m0 = (h * d.cells.volume).sum()
-    for _ in range(20):
+    for _ in range(40):
         oracle_lib.sw_step(d, h, hu, hv, Z, zgrad, G, 0.8, 2)
     assert abs((h * d.cells.volume).sum() - m0) < 1e-9 * m0
-    assert h.min() > 0.9 and h.max() < 2.1
+    assert h.min() > 0.7 and h.max() < 2.1
     assert hu.max() > 0.1  # the dam has started to move right
+    assert np.abs(hv).max() > 1e-3  # the off-centre flow past the bump has a y component
+    assert np.abs(h - h[mirror]).max() < 1e-11
+    assert np.abs(hu - hu[mirror]).max() < 1e-11
+    assert np.abs(hv + hv[mirror]).max() < 1e-11
 
 
 # --------------------------------------------------------------------------------- Euler 3D
@@ -458,3 +490,134 @@ def test_mixed_model_lake_at_rest(oracle_lib):
         dt = M.step(9.81, 0.8, dt)
     assert np.abs(M.state[:, 1:3]).max() < 5e-6
     assert np.abs(M.state[:, 0] - h0).max() < 1e-6
+
+
+# ------------------------------------------------------------------------- Euler dt / update
+def _regular_tet():
+    """One regular tetrahedron with vertices v_k = (1,1,1), (1,-1,-1), (-1,1,-1), (-1,-1,1):
+    edge 2 sqrt 2, volume 8/3, face area 2 sqrt 3, and the outward unit normal of the face
+    opposite v_k is -v_k / sqrt 3."""
+    V = np.array([[1., 1., 1.], [1., -1., -1.], [-1., 1., -1.], [-1., -1., 1.]])
+    nd = np.concatenate([V, np.full((4, 1), 1.)], axis=1)
+    el = np.array([[0, 1, 2, 3, 4]], dtype=np.int32)
+    return Domain(3, mesh=(el, nd)), V
+
+
+def test_regular_tet_geometry():
+    d, V = _regular_tet()
+    assert np.isclose(d.cells.volume[0], 8. / 3., rtol=1e-14)
+    assert np.allclose(d.faces.mesure, 2 * np.sqrt(3.), rtol=1e-14)
+
+
+@pytest.mark.parametrize("vel", [(1., 0., 0.), (-1., 0., 0.), (0.3, -0.5, 0.8), (0., 0., 0.)])
+def test_time_step_euler_3d_closed_form(oracle_lib, vel):
+    """time_step_euler_3d (fvm_utils3d.py:809-825) on one regular tetrahedron with a uniform
+    state: dt = cfl |K| / sum_f (|u . n_f| / |n_f| + c) |f| with c = sqrt(gamma P / rho),
+    which here is cfl (8/3) / (2 sqrt 3 (sum_k |u . v_k| / sqrt 3 + 4 c)). A missing
+    normalisation by |n|, a three-face sum or a squared face measure all change it."""
+    d, V = _regular_tet()
+    rho, P, gamma, cfl = 1.3, 0.7, 1.4, 0.8
+    u = np.array(vel)
+    U = np.array([[rho], [P], [rho * u[0]], [rho * u[1]], [rho * u[2]], [P / (gamma - 1) + 0.5 * rho * u @ u]])
+    dtc = oracle_lib.euler_dt(d, U, cfl, gamma)
+    c = np.sqrt(gamma * P / rho)
+    want = cfl * (8. / 3.) / (2 * np.sqrt(3.) * (np.abs(V @ u).sum() / np.sqrt(3.) + 4 * c))
+    assert dtc.shape == (1,)
+    assert abs(dtc[0] - want) <= 1e-14 * want
+
+
+def test_time_step_euler_3d_scales_with_the_cell(oracle_lib):
+    """The same tetrahedron scaled by s: volume s^3, areas s^2, so dt scales by s exactly
+    (a powers-of-two s keeps every operation exact)."""
+    d, V = _regular_tet()
+    el = np.array([[0, 1, 2, 3, 4]], dtype=np.int32)
+    d4 = Domain(3, mesh=(el, np.concatenate([4. * V, np.full((4, 1), 1.)], axis=1)))
+    U = np.array([[0.9], [1.1], [0.2], [-0.4], [0.1], [3.0]])
+    a = oracle_lib.euler_dt(d, U, 0.5, 1.4)[0]
+    b = oracle_lib.euler_dt(d4, U, 0.5, 1.4)[0]
+    assert abs(b - 4 * a) <= 1e-14 * b
+
+
+def test_update_euler_3d_identity(oracle_lib, dom3d):
+    """update_euler_3d_fvc (fvm_utils3d.py:1316-1327): each conserved field moves by
+    dt rez / |K| (rez in the order rho, rhou, rhov, rhow, rhoE against the state order rho,
+    P, rhou, rhov, rhow, rhoE), then P is recomputed from the equation of state
+    P = (gamma - 1)(rhoE - |rho u|^2 / (2 rho)); dt = 0 only recomputes P."""
+    d = dom3d
+    C = int(d.nbcells)
+    rng = np.random.default_rng(8)
+    vol = d.cells.volume
+    U = np.array([_rand_state(rng) for _ in range(C)]).T.copy()
+    U[1] = 123.  # P is an output of the update, not an input
+    rez = rng.standard_normal((5, C)) * vol * np.array([1., 2., 3., 4., 5.])[:, None]
+    U0 = U.copy()
+    oracle_lib.euler_update(d, U, np.zeros((5, C)), GAMMA, 0.)
+    assert np.array_equal(U[[0, 2, 3, 4, 5]], U0[[0, 2, 3, 4, 5]])
+    eos = lambda W: (GAMMA - 1) * (W[5] - 0.5 * (W[2] ** 2 + W[3] ** 2 + W[4] ** 2) / W[0])
+    assert np.allclose(U[1], eos(U0), rtol=1e-14, atol=1e-14)
+    dt = 0.125
+    U0 = U.copy()
+    oracle_lib.euler_update(d, U, rez, GAMMA, dt)
+    for k, v in enumerate((0, 2, 3, 4, 5)):
+        assert np.allclose((U[v] - U0[v]) * vol / dt, rez[k], rtol=1e-9, atol=1e-12 * np.abs(rez[k]).max())
+    assert np.allclose(U[1], eos(U), rtol=1e-13, atol=1e-13)
+    assert not np.allclose(U[1], eos(U0))
+
+
+def test_update_sw_identity(oracle_lib, dom2d):
+    """update_SW for (h, hu, hv) (shallowater/fvm_utils.py:3-14, no Coriolis or wind): each
+    field moves by dt (rez + src) / |K|; dt = 0 changes nothing."""
+    d = dom2d
+    C = int(d.nbcells)
+    rng = np.random.default_rng(9)
+    vol = d.cells.volume
+    h, hu, hv = 1 + rng.random(C), rng.standard_normal(C), rng.standard_normal(C)
+    rez = rng.standard_normal((3, C)) * vol
+    src = rng.standard_normal((3, C)) * vol * np.array([0., 1., 2.])[:, None]
+    S0 = [h.copy(), hu.copy(), hv.copy()]
+    oracle_lib.update_sw(h, hu, hv, rez, src, 0., vol)
+    for a, b in zip((h, hu, hv), S0):
+        assert np.array_equal(a, b)
+    dt = 0.25
+    oracle_lib.update_sw(h, hu, hv, rez, src, dt, vol)
+    for k, (a, b) in enumerate(zip((h, hu, hv), S0)):
+        assert np.allclose((a - b) * vol / dt, rez[k] + src[k], rtol=1e-9, atol=1e-12)
+
+
+def test_barth_limiter_3d_characterisation(oracle_lib, dom3d):
+    """barthlimiter_3d (ast/functions3d.py:364-408): psi in [0, 1]; the limited face values
+    w_K + psi grad w . (x_f - x_K) stay within the face-neighbour bounds (own cell, both cells
+    of every inner face, the ghost of every boundary face); psi < 1 only where some limited
+    face value reaches a bound; a constant field gives psi = 1."""
+    d = dom3d
+    C = int(d.nbcells)
+    rng = np.random.default_rng(4)
+    xc = d.cells.center
+    wc = np.sin(6 * xc[:, 0]) * np.cos(4 * xc[:, 1]) * np.cos(3 * xc[:, 2]) + 0.1 * rng.standard_normal(C)
+    wg = np.zeros(int(d.nbfaces))
+    oracle_lib.ghost_value(d, 0, wc, wg, d.boundaryfaces)
+    g = [np.zeros(C) for _ in range(3)]
+    psi = np.zeros(C)
+    oracle_lib.cell_gradient(d, wc, wg, *g)
+    oracle_lib.barthlimiter(d, wc, wg, *g, psi)
+    assert psi.min() >= 0 and psi.max() <= 1
+    assert (psi < 1).sum() > C // 10
+    fid = d.cells.faceid
+    cid = d.faces.cellid
+    for i in range(C):
+        faces = fid[i, :fid[i, -1]]
+        nb = [wc[cid[f, 1]] if d.faces.name[f] == 0 else wg[f] for f in faces]
+        nb += [wc[cid[f, 0]] for f in faces]
+        lo, hi = min(nb + [wc[i]]), max(nb + [wc[i]])
+        r = d.faces.center[faces, :3] - xc[i, :3]
+        delta = g[0][i] * r[:, 0] + g[1][i] * r[:, 1] + g[2][i] * r[:, 2]
+        vals = wc[i] + psi[i] * delta
+        tol = 1e-12 * (1 + abs(hi) + abs(lo))
+        assert np.all(vals <= hi + tol) and np.all(vals >= lo - tol)
+        if psi[i] < 1:
+            assert np.min(np.minimum(np.abs(vals - hi), np.abs(vals - lo))) < 1e-9
+    wc = np.full(C, -2.5)
+    wg = np.full(int(d.nbfaces), -2.5)
+    oracle_lib.cell_gradient(d, wc, wg, *g)
+    oracle_lib.barthlimiter(d, wc, wg, *g, psi)
+    assert np.all(psi == 1.)
