@@ -25,6 +25,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mcell-updates/s + %peak HBM GB/s at stated FP64 error; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per cell-update of a whole step (each array the step needs counted
+# once at its element width): shallow water second order float64 / float32, Euler first order
+# float64 / float32, advection first order float64; a Krylov (BiCGStab) iteration per matrix
+# row with float32 ELL values.
+SURVEY_B_SW64, SURVEY_B_MIXED = 356.0, 214.0
+SURVEY_B_EULER64, SURVEY_B_EULER32 = 208.0, 124.0
+SURVEY_B_ADV64 = 112.0
+SURVEY_B_KRYLOV32 = 164.0
 
 
 def measured_traffic(kernel, cells):
@@ -98,23 +106,67 @@ def build_case(nx, seed=0):
     return d, Z, h
 
 
-def cpu_baseline(d, Z, h0, steps, budget_s):
-    """The oracle (single-threaded C restatement of the reference loop structure) on the same
-    mesh and initial state, timed over a bounded number of steps."""
+def cpu_baseline(d, Z, h0, block, repeats):
+    """The oracle (single-threaded C restatement of the reference loops) on the same mesh and
+    initial state: one warm-up step, then ``repeats`` timed blocks of ``block`` steps, each
+    continuing from the last. Returns (median Mcell-updates/s over the blocks, the block rates,
+    the total number of steps taken, the state (h, hu, hv) after them)."""
     from oracle import oracle
     C = int(d.nbcells)
     h, hu, hv = h0.copy(), np.zeros(C), np.zeros(C)
     zg = oracle.z_gradient(d, Z)
-    oracle.sw_step(d, h, hu, hv, Z, zg, G, CFL, 2)  # warm-up
-    done = 0
-    t0 = time.perf_counter()
-    while done < steps:
-        oracle.sw_step(d, h, hu, hv, Z, zg, G, CFL, 2)
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    t = time.perf_counter() - t0
-    return C * done / t / 1e6, done, t
+    oracle.sw_step(d, h, hu, hv, Z, zg, G, CFL, 2)  # warm-up (its step counts for the parity run)
+    rates = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        for _ in range(block):
+            oracle.sw_step(d, h, hu, hv, Z, zg, G, CFL, 2)
+        rates.append(C * block / (time.perf_counter() - t0) / 1e6)
+    return float(np.median(rates)), rates, 1 + block * repeats, (h, hu, hv)
+
+
+def cpu_baseline_omp(d, Z, h0, block, repeats):
+    """The same step on all host cores: the oracle's code with OpenMP over cells and faces
+    (oracle/omp_baseline.c, bit-identical to the serial oracle), timed like cpu_baseline."""
+    from oracle import oracle
+    try:
+        omp = oracle.OmpSwStep(d, Z)
+    except OSError as exc:
+        log("all-cores CPU baseline unavailable: %s" % exc)
+        return None
+    C = int(d.nbcells)
+    h, hu, hv = h0.copy(), np.zeros(C), np.zeros(C)
+    omp.step(h, hu, hv, G, CFL, 2)
+    rates = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        for _ in range(block):
+            omp.step(h, hu, hv, G, CFL, 2)
+        rates.append(C * block / (time.perf_counter() - t0) / 1e6)
+    return {"value": round(float(np.median(rates)), 3), "unit": "Mcell-updates/s", "cores": omp.threads(),
+            "kind": "port", "block_rates": [round(x, 3) for x in rates],
+            "sample": "median of %d blocks of %d steps of the same 2M-cell case, OpenMP over cells and faces "
+                      "(the oracle's code, bit-identical to the serial oracle), threads = OMP_NUM_THREADS"
+                      % (repeats, block)}
+
+
+def region_step_time(step, first, n):
+    """Device time per step of ``n`` un-instrumented steps, from HIP events recorded on the
+    launch stream (torch's current stream, the one every step is launched on) around them."""
+    import torch
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for k in range(n):
+        step(first + k)
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) * 1e-3 / n
+
+
+def rel_errors(a, b, vol):
+    """Relative L1 (Variable.norml2 order 1: sum |a - b| |K| / sum |b| |K|) and relative L-inf."""
+    return {"rel_L1": float(np.sum(np.abs(a - b) * vol) / np.sum(np.abs(b) * vol)),
+            "rel_Linf": float(np.abs(a - b).max() / np.abs(b).max())}
 
 
 def mixed_bytes_per_cell(MF, K_avg, G_avg):
@@ -156,9 +208,9 @@ def tiled_bytes_per_cell(MF, K_avg, G_avg):
 
 
 def time_steps(step, warmup, steps):
-    """Warm-up steps, then ``steps`` steps timed on the host clock between synchronisations and
-    again with HIP events around each step on the current stream. Returns (s per step of the
-    loop, mean event time of one step in s)."""
+    """Warm-up steps, then ``steps`` steps timed on the host clock between synchronisations, then
+    ``steps`` more between two HIP events on the launch stream. Returns (s per step of the host
+    loop, device s per step from the events)."""
     import torch
     for k in range(warmup):
         step(k)
@@ -168,13 +220,20 @@ def time_steps(step, warmup, steps):
         step(warmup + k)
     torch.cuda.synchronize()
     t_loop = (time.perf_counter() - t0) / steps
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-    for k, (a, b) in enumerate(evs):
-        a.record()
-        step(warmup + steps + k)
-        b.record()
-    torch.cuda.synchronize()
-    return t_loop, float(np.mean([a.elapsed_time(b) for a, b in evs])) * 1e-3
+    return t_loop, region_step_time(step, warmup + steps, steps)
+
+
+def cpu_blocks(run_block, cells, total, blocks):
+    """Runs ``total`` steps as ``blocks`` timed blocks (the last takes the remainder); returns
+    (median Mcell-updates/s, the block rates)."""
+    rates = []
+    base = total // blocks
+    for b in range(blocks):
+        n = base if b < blocks - 1 else total - base * (blocks - 1)
+        t0 = time.perf_counter()
+        run_block(n)
+        rates.append(cells * n / (time.perf_counter() - t0) / 1e6)
+    return float(np.median(rates)), rates
 
 
 def run_advection(n, steps, warmup, cpu=True):
@@ -241,16 +300,12 @@ def run_advection(n, steps, warmup, cpu=True):
     # faceid + order entries; per face (x F/C): cellid, normal (3 doubles), u_f v_f w_f
     MF = int(d.cells.faceid.shape[1] - 1)
     bpc = (8 + 8 + 8 + 24 + 4 + (4 + 1) * MF) + (8 + 24 + 24) * F / C
+    rl = roofline("k_adv_step + k_dt_finish", SURVEY_B_ADV64, C, t_ev, bpc, measured_traffic('k_adv_step<2, 1>', C))
+    rl["bound"] = "latency (the step's arrays, %.1f MB, stay in the 256 MB Infinity Cache)" % (bpc * C / 1e6)
     out = {"workload": "advection of a Gaussian blob, %dx%dx2 jittered triangles, order 1, CFL 0.8" % (n, n),
            "cells": C, "faces": F, "steps": steps, "dtype": "f64",
            "value": round(C / t_loop / 1e6, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop * 1e3, 5),
-           "roofline": {"bound": "latency (the step's arrays, %.1f MB, stay in the 256 MB Infinity Cache)"
-                                 % (bpc * C / 1e6), "kernel": "k_adv_step + k_dt_finish",
-                        "achieved": round(bpc * C / t_ev / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(bpc * C / t_ev / 1e9 / HBM_PEAK_GBS, 4),
-                        "traffic": measured_traffic('k_adv_step<2, 1>', C),
-                        "bytes_per_cell": round(bpc, 1), "avg_step_us": round(t_ev * 1e6, 2)},
-           "graph": graph}
+           "roofline": rl, "graph": graph}
     if cpu:
         from oracle import oracle as O
         u_, v_, w_ = np.full(F, 1.0), np.full(F, 0.5), np.zeros(F)
@@ -258,19 +313,21 @@ def run_advection(n, steps, warmup, cpu=True):
         for a, b in ((u_, ucc), (v_, vcc), (w_, wcc)):
             O.facetocell(d, a, b)
         w = w0.copy()
-        t0 = time.perf_counter()
-        for _ in range(nsteps):
-            O.advection_step(d, w, ucc, vcc, wcc, u_, v_, w_, CFL, 1)
-        t_cpu = time.perf_counter() - t0
+
+        def block(k):
+            for _ in range(k):
+                O.advection_step(d, w, ucc, vcc, wcc, u_, v_, w_, CFL, 1)
+        v, rates = cpu_blocks(block, C, nsteps, 6)
         out["parity"] = {"steps": nsteps, "bitwise_equal_to_oracle": bool(np.array_equal(w, w_gpu)),
                          "max_abs_diff": float(np.abs(w - w_gpu).max())}
-        out["cpu_baseline"] = {"value": round(C * nsteps / t_cpu / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1,
-                               "kind": "port", "sample": "%d steps of the same case (%.1f s), single-threaded C "
-                                                         "restatement of the reference loops" % (nsteps, t_cpu)}
+        out["cpu_baseline"] = {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+                               "block_rates": [round(x, 3) for x in rates],
+                               "sample": "median of 6 blocks over the %d steps of the parity run, single-threaded "
+                                         "C restatement of the reference loops" % nsteps}
     return out
 
 
-def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='hilbert'):
+def run_euler(n, steps, warmup, cpu=True, cpu_steps=5, renumber='hilbert'):
     """3D Sod shock tube on n^3 cubes x 6 Kuhn tetrahedra of the unit cube (interior nodes
     jittered by +/-8 % of h, slip walls, first-order Rusanov, CFL 0.8): the fastest of the
     fused device steps (per-cell on [6][C], packed records, face tiles). Parity: the first ``cpu_steps`` steps against the oracle, bit for
@@ -361,11 +418,8 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='hilbert'):
            "cells": C, "faces": F, "steps": steps, "dtype": "f64", "mesh_build_s": round(t_mesh, 1),
            "value": round(C / t_loop / 1e6, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop * 1e3, 5),
            "path": best, "path_ms_per_step": {k: round(v, 4) for k, v in probe.items()}, "cell_order": renumber,
-           "roofline": {"bound": "hbm", "kernel": kname + " + k_dt_finish",
-                        "achieved": round(bpc * C / t_ev / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(bpc * C / t_ev / 1e9 / HBM_PEAK_GBS, 4),
-                        "traffic": measured_traffic(kname, C),
-                        "bytes_per_cell": round(bpc, 1), "avg_step_us": round(t_ev * 1e6, 2)}}
+           "roofline": roofline(kname + " + k_dt_finish", SURVEY_B_EULER64, C, t_ev, bpc,
+                                measured_traffic(kname, C))}
     if cpu and cpu_steps:
         from oracle import oracle as O
         init()
@@ -376,24 +430,25 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=2, renumber='hilbert'):
         U_gpu = S[cpu_steps % 2].cpu().numpy()
         U = U0.copy()
         G_ = np.zeros((6, F))
-        t0 = time.perf_counter()
-        for _ in range(cpu_steps):
-            O.euler_step(d, U, G_, CFL, 1.4, 0)
-        t_cpu = time.perf_counter() - t0
+
+        def block(k):
+            for _ in range(k):
+                O.euler_step(d, U, G_, CFL, 1.4, 0)
+        v, rates = cpu_blocks(block, C, cpu_steps, cpu_steps)
         out["parity"] = {"steps": cpu_steps, "bitwise_equal_to_oracle": bool(np.array_equal(U, U_gpu)),
                          "max_abs_diff": float(np.abs(U - U_gpu).max())}
-        out["cpu_baseline"] = {"value": round(C * cpu_steps / t_cpu / 1e6, 3), "unit": "Mcell-updates/s",
-                               "cores": 1, "kind": "port",
-                               "sample": "%d steps of the same case (%.1f s), single-threaded C restatement of the "
-                                         "reference loops" % (cpu_steps, t_cpu)}
+        out["cpu_baseline"] = {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+                               "block_rates": [round(x, 3) for x in rates],
+                               "sample": "median of %d single steps of the same case (the parity run), "
+                                         "single-threaded C restatement of the reference loops" % cpu_steps}
     return out
 
 
-def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel='auto', tile=256):
-    """The same case on the mixed-precision path: throughput, per-kernel times and the
-    per-field error against the float64 run after the same number of steps. kernel='tiled' /
-    'staged' run the fused one-launch step (states from HBM / staged in LDS), 'split' the
-    reconstruction + flux kernels, 'auto' the fastest of them on this mesh."""
+def run_mixed(d, m, Z, h0, zgrad, warmup, steps, mcell_fp64, kernel='auto', tile=256):
+    """The same case on the mixed-precision path: throughput and the step's roofline (its
+    accuracy against float64 is the separate fixed-length leg ``mixed_accuracy``). kernel=
+    'tiled' / 'staged' run the fused one-launch step (states from HBM / staged in LDS), 'split'
+    the reconstruction + flux kernels, 'auto' the fastest of them on this mesh."""
     import torch
     from manapy_amd import _lib
     from manapy_amd.device import SwMixed, stream_ptr
@@ -411,7 +466,6 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel=
     _lib.call('mfv_time_step_sw', m.ptr, th.data_ptr(), tz.data_ptr(), tz.data_ptr(), CFL, G, 0., 0., dt0.data_ptr(),
               None, s)
     _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
-    k = [0]
     tune = None
     ncell = tile
     if kernel == 'auto':
@@ -419,86 +473,124 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, fp64_state, mcell_fp64, kernel=
         log("mixed: autotuned %s %d/%d %s" % (kernel, tile, ncell,
                                               {"%s:%d:%d" % c: round(v, 4) for c, v in tune.items()}))
 
-    def step(ev=None):
-        if ev is not None:
-            ev[0].record()
+    def step(k):
         if kernel in ('tiled', 'staged'):
-            p.step_tiled(m, G, CFL, 2, ring, k[0], T=tile, stage=(kernel == 'staged'), stream=s, N=ncell)
-            if ev is not None:
-                ev[1].record()
+            p.step_tiled(m, G, CFL, 2, ring, k, T=tile, stage=(kernel == 'staged'), stream=s, N=ncell)
         else:
             _lib.call('mfv_sw_recon_mixed', m.ptr, p.ptr, 2, s)
-            if ev is not None:
-                ev[1].record()
-            _lib.call('mfv_sw_flux_mixed', m.ptr, p.ptr, G, CFL, ring.data_ptr(), k[0], s)
-        if ev is not None:
-            ev[2].record()
-        k[0] += 1
+            _lib.call('mfv_sw_flux_mixed', m.ptr, p.ptr, G, CFL, ring.data_ptr(), k, s)
 
-    for _ in range(warmup):
-        step()
+    for k in range(warmup):
+        step(k)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for _ in range(steps):
-        step()
+    for k in range(steps):
+        step(warmup + k)
     torch.cuda.synchronize()
     t_loop = time.perf_counter() - t1
-    oh, ohu, ohv = (torch.empty(C, dtype=torch.float64, device='cuda') for _ in range(3))
-    _lib.call('mfv_sw_unpack_mixed', m.ptr, p.ptr, oh.data_ptr(), ohu.data_ptr(), ohv.data_ptr(), s)
+    t_dev = region_step_time(step, warmup + steps, steps)
     _lib.call('mfv_sw_conservation', m.ptr, p.ptr, p.partial.data_ptr(), p.partial.numel(), p.sums.data_ptr(), s)
     mass1 = float(p.sums[0])
-    vol = d.cells.volume
-    errs = {}
-    for nm, a, b in (('h', oh, fp64_state[0]), ('hu', ohu, fp64_state[1]), ('hv', ohv, fp64_state[2])):
-        a = a.cpu().numpy()
-        errs[nm] = {"rel_L1": float(np.sum(np.abs(a - b) * vol) / np.sum(np.abs(b) * vol)),
-                    "rel_Linf": float(np.abs(a - b).max() / np.abs(b).max())}
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
-    for e in evs:
-        step(e)
-    torch.cuda.synchronize()
-    t_rec = np.array([a.elapsed_time(b) for a, b, _ in evs]) * 1e-3
-    t_flx = np.array([b.elapsed_time(c) for _, b, c in evs]) * 1e-3
     K_avg = float(d.cells.cellnid[:, -1].mean())
     G_avg = float(d.cells.ghostnid[:, -1].mean())
     MF = int(d.cells.faceid.shape[1] - 1)
     mcell = C * steps / t_loop / 1e6
     if kernel in ('tiled', 'staged'):
-        # t_rec brackets the fused kernel, t_flx the one-block dt fold after it
-        bt = tiled_bytes_per_cell(MF, K_avg, G_avg)
-        kern = {"k_sw_step32_tiled": {"avg_us": round(float(t_rec.mean()) * 1e6, 2), "bytes_per_cell": round(bt, 1),
-                                      "GB/s": round(bt * C / t_rec.mean() / 1e9, 1), "tile": tile,
-                                      "cells_per_tile": ncell,
-                                      "states_in_lds": kernel == 'staged'},
-                "k_dt_finish": {"avg_us": round(float(t_flx.mean()) * 1e6, 2)}}
-        dom, tk, bk = 'k_sw_step32_tiled', t_rec, bt
+        kname, bmodel = 'k_sw_step32_tiled + k_dt_finish', tiled_bytes_per_cell(MF, K_avg, G_avg)
+        trace_key = "k_sw_step32_tiled:%s:%d:%d" % (kernel, tile, ncell)
     else:
-        br, bf = mixed_bytes_per_cell(MF, K_avg, G_avg)
-        kern = {"k_sw_recon32": {"avg_us": round(float(t_rec.mean()) * 1e6, 2), "bytes_per_cell": round(br, 1),
-                                 "GB/s": round(br * C / t_rec.mean() / 1e9, 1)},
-                "k_sw_flux32": {"avg_us": round(float(t_flx.mean()) * 1e6, 2), "bytes_per_cell": round(bf, 1),
-                                "GB/s": round(bf * C / t_flx.mean() / 1e9, 1)}}
-        dom = 'k_sw_flux32' if t_flx.mean() >= t_rec.mean() else 'k_sw_recon32'
-        tk, bk = (t_flx, bf) if dom == 'k_sw_flux32' else (t_rec, br)
+        kname, bmodel = 'k_sw_recon32 + k_sw_flux32', sum(mixed_bytes_per_cell(MF, K_avg, G_avg))
+        trace_key = None
     return {
         "value": round(mcell, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop / steps * 1e3, 5),
         "kernel": kernel, "tile": tile if kernel != 'split' else None,
         "autotune_ms_per_step": {"%s:%d:%d" % c: round(v, 4) for c, v in tune.items()} if tune else None,
         "speedup_vs_fp64": round(mcell / mcell_fp64, 3),
-        "dtype": "f32 flux/gradient arithmetic, f64 state/dt/sums",
-        "steps_compared": warmup + steps, "error_vs_fp64": errs,
+        "dtype": "f32 flux/gradient arithmetic and face records, f64 state/gathers/dt/sums",
         "mass_rel_drift": (mass1 - mass0) / mass0,
-        "kernels": kern,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(bk * C / tk.mean() / 1e9, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(bk * C / tk.mean() / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(dom if kernel == 'split' else "%s:%s:%d:%d" % (dom, kernel, tile, ncell),
-                                                 C),
-                     "bytes_per_cell": round(bk, 1), "avg_launch_us": round(float(tk.mean()) * 1e6, 2)},
-        "step_hbm_frac_214B": round(214.0 * C * steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": roofline(kname, SURVEY_B_MIXED, C, t_dev, bmodel,
+                             measured_traffic(trace_key, C) if trace_key else None),
     }
 
 
-def run_ensemble(n_inst, steps, seed=0, n_check=4):
+def roofline(kernel, bytes_per_cell, cells, t_step, model_bytes=None, traffic=None):
+    """Roofline entry of one step: algorithmic bytes per cell-update (the hot-path survey's
+    count) x cells / the device time per step from HIP events over the timed region."""
+    ach = bytes_per_cell * cells / t_step / 1e9
+    r = {"bound": "hbm", "kernel": kernel, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_cell": bytes_per_cell,
+         "avg_launch_us": round(t_step * 1e6, 2)}
+    if model_bytes is not None:
+        r["bytes_per_cell_kernel_model"] = round(model_bytes, 1)
+        r["frac_kernel_model"] = round(model_bytes * cells / t_step / 1e9 / HBM_PEAK_GBS, 4)
+    return r
+
+
+def fp64_sw_run(d, m, Z, h0, zgrad_t, checkpoints, T=256):
+    """The float64 fused step from the initial state; (h, hu, hv) on the host after each
+    checkpoint's number of steps (device numbering)."""
+    import torch
+    from manapy_amd import _lib
+    from manapy_amd.device import SwPacked, stream_ptr
+    C = int(d.nbcells)
+    s = stream_ptr()
+    dv = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device='cuda')
+    S0 = dv(np.stack([h0, np.zeros(C), np.zeros(C)]))
+    tZ = dv(Z)
+    pk = SwPacked(m)
+    pk.pack(m, S0[0], S0[1], S0[2], tZ, zgrad_t, 2, s)
+    dt0, ring = dv(np.zeros(1)), dv(np.zeros(4))
+    _lib.call('mfv_time_step_sw', m.ptr, S0[0].data_ptr(), S0[1].data_ptr(), S0[2].data_ptr(), CFL, G, 0., 0.,
+              dt0.data_ptr(), None, s)
+    _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
+    out = {}
+    o = [torch.empty(C, dtype=torch.float64, device='cuda') for _ in range(3)]
+    for k in range(max(checkpoints)):
+        pk.step_tiled(m, G, CFL, 2, ring, k, T=T, stream=s)
+        if k + 1 in checkpoints:
+            _lib.call('mfv_sw_unpack', m.ptr, pk.ptr, o[0].data_ptr(), o[1].data_ptr(), o[2].data_ptr(), s)
+            out[k + 1] = [x.cpu().numpy() for x in o]
+    return out
+
+
+def mixed_accuracy(d, m, Z, h0, zgrad_t, checkpoints=(25, 100, 500), T=256):
+    """Per-field error of the mixed-precision path against the float64 path on the same case
+    after each checkpoint's number of steps (independent of --steps), and its mass drift."""
+    import torch
+    from manapy_amd import _lib
+    from manapy_amd.device import SwMixed, stream_ptr
+    C = int(d.nbcells)
+    s = stream_ptr()
+    ref = fp64_sw_run(d, m, Z, h0, zgrad_t, checkpoints, T)
+    dv = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device='cuda')
+    p = SwMixed(m)
+    th, tz, tZ = dv(h0), dv(np.zeros(C)), dv(Z)
+    _lib.call('mfv_sw_pack_mixed', m.ptr, p.ptr, th.data_ptr(), tz.data_ptr(), tz.data_ptr(), tZ.data_ptr(),
+              zgrad_t.data_ptr(), 2, s)
+    _lib.call('mfv_sw_conservation', m.ptr, p.ptr, p.partial.data_ptr(), p.partial.numel(), p.sums.data_ptr(), s)
+    mass0 = float(p.sums[0])
+    dt0, ring = dv(np.zeros(1)), dv(np.zeros(4))
+    _lib.call('mfv_time_step_sw', m.ptr, th.data_ptr(), tz.data_ptr(), tz.data_ptr(), CFL, G, 0., 0., dt0.data_ptr(),
+              None, s)
+    _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
+    vol = d.cells.volume
+    o = [torch.empty(C, dtype=torch.float64, device='cuda') for _ in range(3)]
+    res = {}
+    worst = 0.
+    for k in range(max(checkpoints)):
+        p.step_tiled(m, G, CFL, 2, ring, k, T=T, stage=True, stream=s)
+        if k + 1 in checkpoints:
+            _lib.call('mfv_sw_unpack_mixed', m.ptr, p.ptr, o[0].data_ptr(), o[1].data_ptr(), o[2].data_ptr(), s)
+            e = {nm: rel_errors(a.cpu().numpy(), b, vol) for nm, a, b in zip(('h', 'hu', 'hv'), o, ref[k + 1])}
+            worst = max(worst, max(v["rel_L1"] for v in e.values()))
+            res[str(k + 1)] = e
+    _lib.call('mfv_sw_conservation', m.ptr, p.ptr, p.partial.data_ptr(), p.partial.numel(), p.sums.data_ptr(), s)
+    mass1 = float(p.sums[0])
+    return {"steps": list(checkpoints), "error_vs_fp64": res, "max_rel_L1": worst, "target_rel_L1": 1e-5,
+            "meets_target": bool(worst <= 1e-5), "mass_rel_drift": (mass1 - mass0) / mass0}
+
+
+def run_ensemble(n_inst, steps, seed=0, n_check=16):
     """Batched ensemble: n_inst shallow-water instances (sizes log-uniform in [5k, 200k]
     triangles, aspect ratio 1-4, 1-10 bumps, random dam) advanced together by a captured graph
     with the autotuned launch configuration; error of sampled instances against their float64
@@ -529,10 +621,14 @@ def run_ensemble(n_inst, steps, seed=0, n_check=4):
     e.step(6)
     torch.cuda.synchronize()
     g = e.graph(steps)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t1 = time.perf_counter()
+    ev0.record()
     e.run_graph(steps, chunk=steps)
+    ev1.record()
     torch.cuda.synchronize()
     t_loop = time.perf_counter() - t1
+    t_dev = ev0.elapsed_time(ev1) * 1e-3 / steps
     cells = int(sum(int(d.nbcells) for d in doms))
     mcell = cells * steps / t_loop / 1e6
     # float64 reference for a sample of instances (same initial state, same 6 + steps steps)
@@ -572,8 +668,9 @@ def run_ensemble(n_inst, steps, seed=0, n_check=4):
         "value": round(mcell, 3), "unit": "Mcell-updates/s", "instances": n_inst, "cells": cells,
         "steps": steps, "ms_per_step": round(t_loop / steps * 1e3, 4), "graph": "one captured graph of all steps",
         "config": list(tune['config']), "autotune_ms_per_step": tune['ms_per_step'],
-        "step_hbm_frac_214B": round(214.0 * cells * steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
-        "error_vs_fp64_sampled": errs, "max_rel_L1": max(max(x.values()) for x in errs),
+        "roofline": roofline("k_sw_step32_tiled (all instances) + k_batch_dt_fold", SURVEY_B_MIXED, cells, t_dev),
+        "error_vs_fp64_sampled": errs, "instances_checked": [int(i) for i in pick],
+        "max_rel_L1": max(max(x.values()) for x in errs),
         "mass_totals_finite": bool(np.all(np.isfinite(sums))), "build_s": round(t_build, 1),
     }
 
@@ -644,8 +741,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--nx', type=int, default=1000)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--cpu-steps', type=int, default=20)
-    ap.add_argument('--cpu-budget', type=float, default=20.0)
+    ap.add_argument('--cpu-block', type=int, default=4, help='steps per timed block of the CPU baseline')
+    ap.add_argument('--cpu-repeats', type=int, default=5, help='timed blocks of the CPU baseline (median)')
     ap.add_argument('--path', choices=('auto', 'fused', 'packed', 'unpacked'), default='auto',
                     help='float64 path: fused (one tiled launch per step, records in LDS), packed (per-cell records '
                          '+ per-face MUSCL values, two kernels), unpacked (SoA kernels), auto (the faster of fused '
@@ -715,7 +812,6 @@ def main():
               0., dt0.data_ptr(), None, s)
     _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
 
-    step_no = [0]
     path, fused_T, path_ms = args.path, None, None
     if path in ('auto', 'fused', 'packed'):
         from manapy_amd.device import SwPacked
@@ -775,62 +871,38 @@ def main():
             _lib.call('mfv_sw_step_fused', m.ptr, a.data_ptr(), b.data_ptr(), tZ.data_ptr(), grad.data_ptr(),
                       zgrad.data_ptr(), G, CFL, 2, ring.data_ptr(), k, s)
 
-    def step(ev=None):
-        k = step_no[0]
-        if ev is not None:
-            ev[0].record()
+    def step(k):
         launch_a(k)
-        if ev is not None:
-            ev[1].record()
         launch_b(k)
-        if ev is not None:
-            ev[2].record()
-        step_no[0] = k + 1
 
-    for _ in range(args.warmup):
-        step()
+    for k in range(args.warmup):
+        step(k)
     torch.cuda.synchronize()
 
     # timed loop: no events inside, host-side wall clock bracketed by synchronisations
-    torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
+    for k in range(args.steps):
+        step(args.warmup + k)
     torch.cuda.synchronize()
     t_loop = time.perf_counter() - t1
-    n_fp64_steps = step_no[0]
-    h64 = torch.empty(C, dtype=torch.float64, device='cuda')
-    hu64, hv64 = torch.empty_like(h64), torch.empty_like(h64)
-    if path in ('packed', 'fused'):
-        _lib.call('mfv_sw_unpack', m.ptr, pk.ptr, h64.data_ptr(), hu64.data_ptr(), hv64.data_ptr(), s)
-    else:
-        last = S[n_fp64_steps % 2]
-        h64.copy_(last[0]), hu64.copy_(last[1]), hv64.copy_(last[2])
-    fp64_state = [x.cpu().numpy() for x in (h64, hu64, hv64)]
-
-    # per-kernel durations: HIP events on the launch stream over a second timed pass
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    for e in evs:
-        step(e)
-    torch.cuda.synchronize()
-    t_grad = np.array([a.elapsed_time(b) for a, b, _ in evs]) * 1e-3
-    t_flux = np.array([b.elapsed_time(c) for _, b, c in evs]) * 1e-3
+    # the same number of steps again between two HIP events on the launch stream: the device
+    # time per step that the roofline divides by
+    t_dev = region_step_time(step, args.warmup + args.steps, args.steps)
 
     K_avg = float(d.cells.cellnid[:, -1].mean())
     G_avg = float(d.cells.ghostnid[:, -1].mean())
     MF = int(d.cells.faceid.shape[1] - 1)
     if path == 'fused':
-        bg, bfx = fused_bytes_per_cell(MF, K_avg, G_avg), 0.
+        bmodel = fused_bytes_per_cell(MF, K_avg, G_avg)
+        trace_key = names[0] + ":%d" % fused_T
+        kname = names[0] + " + k_dt_finish"
     elif path == 'packed':
-        bg, bfx = kernel_bytes_per_cell_packed(MF, K_avg, G_avg)
+        bmodel = sum(kernel_bytes_per_cell_packed(MF, K_avg, G_avg))
+        trace_key, kname = None, " + ".join(names)
     else:
-        bg, bfx = kernel_bytes_per_cell(F, C, K_avg, G_avg)
-    ach_flux = bfx * C / t_flux.mean() / 1e9
-    ach_grad = bg * C / t_grad.mean() / 1e9
-    dom_is_flux = path != 'fused' and t_flux.mean() >= t_grad.mean()
-    ach, bpc, tk = (ach_flux, bfx, t_flux) if dom_is_flux else (ach_grad, bg, t_grad)
+        bmodel = sum(kernel_bytes_per_cell(F, C, K_avg, G_avg))
+        trace_key, kname = None, " + ".join(names)
     mcell = C * args.steps / t_loop / 1e6
-    step_bytes_survey = 356.0  # FP64 B per cell-update of the whole C2 step (algorithmic model)
 
     out = {
         "metric": METRIC,
@@ -848,29 +920,38 @@ def main():
         "config": {"workload": "shallow-water dam break, 2D, MUSCL + LSQ + Barth-Jespersen + Rusanov, walls",
                    "cells": C, "faces": F, "nx": args.nx, "cfl": CFL, "order": 2, "precision": "fp64 plain",
                    "path": path, "tile": fused_T, "cell_order": args.renumber},
-        "roofline": {"bound": "hbm", "kernel": names[1] if dom_is_flux else names[0],
-                     "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(names[1] if dom_is_flux else
-                                                 (names[0] + (":%d" % fused_T if path == 'fused' else '')), C),
-                     "bytes_per_cell": round(bpc, 1), "avg_launch_us": round(float(tk.mean()) * 1e6, 2)},
-        "kernels": {names[0]: {"avg_us": round(float(t_grad.mean()) * 1e6, 2),
-                               "bytes_per_cell": round(bg, 1), "GB/s": round(ach_grad, 1)},
-                    names[1]: {"avg_us": round(float(t_flux.mean()) * 1e6, 2),
-                               "bytes_per_cell": round(bfx, 1), "GB/s": round(ach_flux, 1)}}
-        if path != 'fused' else
-        {names[0]: {"avg_us": round(float(t_grad.mean()) * 1e6, 2), "bytes_per_cell": round(bg, 1),
-                    "GB/s": round(ach_grad, 1), "tile": fused_T}},
+        "roofline": roofline(kname, SURVEY_B_SW64, C, t_dev, bmodel,
+                             measured_traffic(trace_key, C) if trace_key else None),
         "fp64_path_ms_per_step": path_ms,
-        "step_hbm_frac_356B": round(step_bytes_survey * C * args.steps / t_loop / 1e9 / HBM_PEAK_GBS, 4),
         "mesh_build_s": round(t_mesh, 2),
     }
     log("fp64: %.1f Mcell-updates/s" % mcell)
+    if not args.no_cpu:
+        v, rates, n_cpu, cpu_state = cpu_baseline(d0, Z0, h00, args.cpu_block, args.cpu_repeats)
+        out["cpu_baseline"] = {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+                               "sample": "median of %d blocks of %d steps of the same 2M-cell case (after one "
+                                         "warm-up step), single-threaded C restatement of the reference loops"
+                                         % (args.cpu_repeats, args.cpu_block),
+                               "block_rates": [round(x, 3) for x in rates]}
+        omp = cpu_baseline_omp(d0, Z0, h00, args.cpu_block, args.cpu_repeats)
+        if omp is not None:
+            out["cpu_baseline_all_cores"] = omp
+        # full-size parity: the float64 GPU path over the same number of steps as the oracle took
+        got = fp64_sw_run(d, m, Z, h0, zgrad, (n_cpu,), T=fused_T or 256)[n_cpu]
+        if hasattr(d, 'to_old'):
+            got = [d.to_old(x) for x in got]
+        diff = max(float(np.abs(a - b).max()) for a, b in zip(got, cpu_state))
+        out["parity"] = {"steps": n_cpu, "bitwise_equal_to_oracle": bool(all(np.array_equal(a, b)
+                                                                              for a, b in zip(got, cpu_state))),
+                         "max_abs_diff": diff}
+        log("fp64 parity after %d steps: %s" % (n_cpu, out["parity"]["bitwise_equal_to_oracle"]))
     if not args.no_mixed:
-        out["mixed"] = run_mixed(d, m, Z, h0, zgrad, args.warmup, args.steps, fp64_state, mcell, args.mixed_kernel,
-                                 args.tile)
+        out["mixed"] = run_mixed(d, m, Z, h0, zgrad, args.warmup, args.steps, mcell, args.mixed_kernel, args.tile)
         log("mixed (%s): %.1f Mcell-updates/s" % (out["mixed"]["kernel"], out["mixed"]["value"]))
-        out["mixed_split"] = run_mixed(d, m, Z, h0, zgrad, args.warmup, args.steps, fp64_state, mcell, 'split')
+        out["mixed_accuracy"] = mixed_accuracy(d, m, Z, h0, zgrad)
+        log("mixed accuracy: max rel L1 %.3g over %s steps" % (out["mixed_accuracy"]["max_rel_L1"],
+                                                              out["mixed_accuracy"]["steps"]))
+        out["mixed_split"] = run_mixed(d, m, Z, h0, zgrad, args.warmup, args.steps, mcell, 'split')
         log("mixed (two kernels): %.1f Mcell-updates/s" % out["mixed_split"]["value"])
     if args.ensemble:
         out["ensemble"] = run_ensemble(args.ensemble, args.ensemble_steps)
@@ -884,11 +965,6 @@ def main():
     if args.euler:
         out["euler"] = run_euler(args.euler, args.euler_steps, 5, not args.no_cpu, renumber=args.euler_renumber)
         log("euler: %.1f Mcell-updates/s" % out["euler"]["value"])
-    if not args.no_cpu:
-        v, n, t = cpu_baseline(d0, Z0, h00, args.cpu_steps, args.cpu_budget)
-        out["cpu_baseline"] = {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-                               "sample": "%d steps of the same 2M-cell case (%.1f s), single-threaded C "
-                                         "restatement of the reference loops" % (n, t)}
     print(json.dumps(out), flush=True)
 
 

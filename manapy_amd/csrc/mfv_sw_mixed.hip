@@ -43,6 +43,11 @@ __device__ __forceinline__ void face_dev(float hb, float aL, float aR, float dz,
     const bool wetL = hL > 0.f, wetR = hR > 0.f;
     float eL = aL - cL, eR = aR - cR;  // hydrostatic depths minus hb
     float hsL = hb + eL, hsR = hb + eR;
+    // h - hs of each side from the deviations (= the bathymetry step c up to rounding) while both
+    // are positive: the difference of the two rounded depths would lose the step to cancellation
+    // (an error of ~1e-7 h in the bathymetry term that breaks the lake-at-rest balance)
+    const float dhL = wetL ? ((hsL > 0.f) ? aL - eL : hb + aL) : 0.f;
+    const float dhR = wetR ? ((hsR > 0.f) ? aR - eR : hb + aR) : 0.f;
     if (!wetL) hL = 0.f;
     if (!(hsL > 0.f)) {
         hsL = 0.f;
@@ -75,7 +80,7 @@ __device__ __forceinline__ void face_dev(float hb, float aL, float aR, float dz,
     F1 = 0.5f * (hsL * uL * qL + hsR * uR * qR) + 0.5f * (PL + PR) * n0 - 0.5f * s * (hsR * uR - hsL * uL);
     F2 = 0.5f * (hsL * vL * qL + hsR * vR * qR) + 0.5f * (PL + PR) * n1 - 0.5f * s * (hsR * vR - hsL * vL);
     const float hO = own_is_L ? hL : hR, hsO = own_is_L ? hsL : hsR;
-    const float dh = hO - hsO;
+    const float dh = own_is_L ? dhL : dhR;
     bo = 0.5f * g * dh * (hO + hsO) + g * (hO + ho) * 0.5f * dzo;
 }
 

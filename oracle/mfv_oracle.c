@@ -131,11 +131,24 @@ void orc_explicitscheme_convective(int dim, double *rez_w, const double *w_c, co
 
 /* cell_gradient_2d (functions2d.py:5-104), one-rank non-periodic terms only.
  * Pinned: exact for affine fields; zero for constants. */
+void orc_cell_gradient_2d_range(const double *w_c, const double *w_ghost, const double *centerc,
+                                const int32_t *cellnid, int k1, const int32_t *ghostnid, int g1,
+                                const double *centergf, int gcw, double *w_x, double *w_y, double *w_z, int i0,
+                                int i1);
 void orc_cell_gradient_2d(const double *w_c, const double *w_ghost, const double *centerc,
                           const int32_t *cellnid, int k1, const int32_t *ghostnid, int g1,
                           const double *centergf, int gcw, double *w_x, double *w_y, double *w_z, int C)
 {
-    for (int i = 0; i < C; i++) {
+    orc_cell_gradient_2d_range(w_c, w_ghost, centerc, cellnid, k1, ghostnid, g1, centergf, gcw, w_x, w_y, w_z, 0, C);
+}
+
+/* the loop of orc_cell_gradient_2d over cells [i0, i1) */
+void orc_cell_gradient_2d_range(const double *w_c, const double *w_ghost, const double *centerc,
+                                const int32_t *cellnid, int k1, const int32_t *ghostnid, int g1,
+                                const double *centergf, int gcw, double *w_x, double *w_y, double *w_z, int i0,
+                                int i1)
+{
+    for (int i = i0; i < i1; i++) {
         double i_xx = 0., i_yy = 0., i_xy = 0., j_xw = 0., j_yw = 0.;
         const int32_t *nr = ROW(cellnid, k1, i);
         for (int j = 0; j < nr[k1 - 1]; j++) {
@@ -168,12 +181,23 @@ void orc_cell_gradient_2d(const double *w_c, const double *w_ghost, const double
 
 /* barthlimiter_2d (functions2d.py:280-325), one-rank branches (inner / names 1-4).
  * Pinned: psi in [0,1]; psi = 1 for constants; brute force on tiny meshes. */
+void orc_barthlimiter_2d_range(const double *w_c, const double *w_ghost, const double *w_x, const double *w_y,
+                               double *psi, const int32_t *cellid, const int32_t *faceid, int mf1,
+                               const int32_t *namef, const double *centerc, const double *centerf, int i0, int i1);
 void orc_barthlimiter_2d(const double *w_c, const double *w_ghost, const double *w_x, const double *w_y,
                          double *psi, const int32_t *cellid, const int32_t *faceid, int mf1,
                          const int32_t *namef, const double *centerc, const double *centerf, int C)
 {
-    for (int i = 0; i < C; i++) psi[i] = 1.;
-    for (int i = 0; i < C; i++) {
+    orc_barthlimiter_2d_range(w_c, w_ghost, w_x, w_y, psi, cellid, faceid, mf1, namef, centerc, centerf, 0, C);
+}
+
+/* the loops of orc_barthlimiter_2d over cells [i0, i1) */
+void orc_barthlimiter_2d_range(const double *w_c, const double *w_ghost, const double *w_x, const double *w_y,
+                               double *psi, const int32_t *cellid, const int32_t *faceid, int mf1,
+                               const int32_t *namef, const double *centerc, const double *centerf, int i0, int i1)
+{
+    for (int i = i0; i < i1; i++) psi[i] = 1.;
+    for (int i = i0; i < i1; i++) {
         double w_max = w_c[i], w_min = w_c[i];
         const int32_t *fr = ROW(faceid, mf1, i);
         int nfc = fr[mf1 - 1];
@@ -270,12 +294,23 @@ void orc_barthlimiter_3d(const double *w_c, const double *w_ghost, const double 
 /* shallow water                                                                            */
 
 /* time_step_SW (solvers/shallowater/fvm_utils.py:16-42). Pinned: closed form. */
+double orc_time_step_sw_range(const double *h, const double *hu, const double *hv, double cfl,
+                              const double *normal, const double *mesure, const double *volume,
+                              const int32_t *faceid, int mf1, double grav, double Dxx, double Dyy, int i0, int i1);
 double orc_time_step_sw(const double *h, const double *hu, const double *hv, double cfl, const double *normal,
                         const double *mesure, const double *volume, const int32_t *faceid, int mf1, double grav,
                         double Dxx, double Dyy, int C)
 {
+    return orc_time_step_sw_range(h, hu, hv, cfl, normal, mesure, volume, faceid, mf1, grav, Dxx, Dyy, 0, C);
+}
+
+/* the minimum of orc_time_step_sw over cells [i0, i1) (from the reference's initial 1e6) */
+double orc_time_step_sw_range(const double *h, const double *hu, const double *hv, double cfl,
+                              const double *normal, const double *mesure, const double *volume,
+                              const int32_t *faceid, int mf1, double grav, double Dxx, double Dyy, int i0, int i1)
+{
     double dt = 1e6;
-    for (int i = 0; i < C; i++) {
+    for (int i = i0; i < i1; i++) {
         double velson = sqrt(grav * h[i]);
         double lam = 0.;
         const int32_t *fr = ROW(faceid, mf1, i);
@@ -295,11 +330,22 @@ double orc_time_step_sw(const double *h, const double *hu, const double *hv, dou
 }
 
 /* update_SW (fvm_utils.py:3-14) for h, hu, hv (Coriolis and wind terms zero). */
+void orc_update_sw_range(double *h, double *hu, double *hv, const double *rez_h, const double *rez_hu,
+                         const double *rez_hv, const double *src_h, const double *src_hu, const double *src_hv,
+                         double dtime, const double *vol, int i0, int i1);
 void orc_update_sw(double *h, double *hu, double *hv, const double *rez_h, const double *rez_hu,
                    const double *rez_hv, const double *src_h, const double *src_hu, const double *src_hv,
                    double dtime, const double *vol, int C)
 {
-    for (int i = 0; i < C; i++) {
+    orc_update_sw_range(h, hu, hv, rez_h, rez_hu, rez_hv, src_h, src_hu, src_hv, dtime, vol, 0, C);
+}
+
+/* the loop of orc_update_sw over cells [i0, i1) */
+void orc_update_sw_range(double *h, double *hu, double *hv, const double *rez_h, const double *rez_hu,
+                         const double *rez_hv, const double *src_h, const double *src_hu, const double *src_hv,
+                         double dtime, const double *vol, int i0, int i1)
+{
+    for (int i = i0; i < i1; i++) {
         h[i] += dtime * (rez_h[i] + src_h[i]) / vol[i];
         hu[i] += dtime * ((rez_hu[i] + src_hu[i]) / vol[i] + 0. + 0.);
         hv[i] += dtime * ((rez_hv[i] + src_hv[i]) / vol[i] + 0. + 0.);
@@ -358,6 +404,13 @@ void orc_sw_face(double etaL, double huL, double hvL, double zfL, double hcL, do
  * eta = h + Z is reconstructed with its own limited gradient; Z with an
  * unlimited gradient (zx, zy). rez_* are the convective residuals, src_* the
  * bathymetry terms; both are zeroed first. */
+void orc_sw_face_terms(int f, int inner, const double *h, const double *hu, const double *hv, const double *Z,
+                       const double *ex, const double *ey, const double *epsi, const double *ux, const double *uy,
+                       const double *upsi, const double *vx, const double *vy, const double *vpsi,
+                       const double *zx, const double *zy, const double *centerc, const double *centerf,
+                       const int32_t *cellidf, const double *normalf, const double *mesuref, double grav, double om,
+                       double *flux, double *sL, double *sR);
+
 void orc_sw_residuals(double *rez_h, double *rez_hu, double *rez_hv, double *src_h, double *src_hu, double *src_hv,
                       const double *h, const double *hu, const double *hv, const double *Z,
                       const double *h_g, const double *hu_g, const double *hv_g, const double *Z_g,
@@ -379,20 +432,8 @@ void orc_sw_residuals(double *rez_h, double *rez_hu, double *rez_hv, double *src
     for (int k = 0; k < n_inner; k++) {
         int f = innerfaces[k];
         int a = cellidf[2 * f], b = cellidf[2 * f + 1];
-        double ra0 = centerf[3 * f] - centerc[3 * a], ra1 = centerf[3 * f + 1] - centerc[3 * a + 1];
-        double rb0 = centerf[3 * f] - centerc[3 * b], rb1 = centerf[3 * f + 1] - centerc[3 * b + 1];
-        double etaA = h[a] + Z[a], etaB = h[b] + Z[b];
-        double eL = etaA + om * epsi[a] * (ex[a] * ra0 + ey[a] * ra1);
-        double eR = etaB + om * epsi[b] * (ex[b] * rb0 + ey[b] * rb1);
-        double uL = hu[a] + om * upsi[a] * (ux[a] * ra0 + uy[a] * ra1);
-        double uR = hu[b] + om * upsi[b] * (ux[b] * rb0 + uy[b] * rb1);
-        double vL = hv[a] + om * vpsi[a] * (vx[a] * ra0 + vy[a] * ra1);
-        double vR = hv[b] + om * vpsi[b] * (vx[b] * rb0 + vy[b] * rb1);
-        double zL = Z[a] + om * (zx[a] * ra0 + zy[a] * ra1);
-        double zR = Z[b] + om * (zx[b] * rb0 + zy[b] * rb1);
-        const double *n = normalf + 3 * (int64_t)f;
-        orc_sw_face(eL, uL, vL, zL, h[a], Z[a], eR, uR, vR, zR, h[b], Z[b], n[0], n[1], mesuref[f], grav,
-                    flux, sL, sR);
+        orc_sw_face_terms(f, 1, h, hu, hv, Z, ex, ey, epsi, ux, uy, upsi, vx, vy, vpsi, zx, zy, centerc, centerf,
+                          cellidf, normalf, mesuref, grav, om, flux, sL, sR);
         rez_h[a] -= flux[0]; rez_hu[a] -= flux[1]; rez_hv[a] -= flux[2];
         src_hu[a] -= sL[0]; src_hv[a] -= sL[1];
         rez_h[b] += flux[0]; rez_hu[b] += flux[1]; rez_hv[b] += flux[2];
@@ -401,19 +442,46 @@ void orc_sw_residuals(double *rez_h, double *rez_hu, double *rez_hv, double *src
     for (int k = 0; k < n_bnd; k++) {
         int f = boundaryfaces[k];
         int a = cellidf[2 * f];
-        double ra0 = centerf[3 * f] - centerc[3 * a], ra1 = centerf[3 * f + 1] - centerc[3 * a + 1];
-        double etaA = h[a] + Z[a];
-        double eL = etaA + om * epsi[a] * (ex[a] * ra0 + ey[a] * ra1);
-        double uL = hu[a] + om * upsi[a] * (ux[a] * ra0 + uy[a] * ra1);
-        double vL = hv[a] + om * vpsi[a] * (vx[a] * ra0 + vy[a] * ra1);
-        double zL = Z[a] + om * (zx[a] * ra0 + zy[a] * ra1);
-        /* wall ghost = the Neumann (eta, Z) / nonslip (hu, hv) mirror of the reconstructed face
-         * state; at order 1 this is the cell ghost h_g, -hu, -hv, Z_g of bc_utils.py:45-55 */
-        const double *n = normalf + 3 * (int64_t)f;
-        orc_sw_face(eL, uL, vL, zL, h[a], Z[a], eL, -1 * uL, -1 * vL, zL, h[a], Z[a], n[0], n[1],
-                    mesuref[f], grav, flux, sL, sR);
+        orc_sw_face_terms(f, 0, h, hu, hv, Z, ex, ey, epsi, ux, uy, upsi, vx, vy, vpsi, zx, zy, centerc, centerf,
+                          cellidf, normalf, mesuref, grav, om, flux, sL, sR);
         rez_h[a] -= flux[0]; rez_hu[a] -= flux[1]; rez_hv[a] -= flux[2];
         src_hu[a] -= sL[0]; src_hv[a] -= sL[1];
+    }
+    (void)h_g; (void)hu_g; (void)hv_g; (void)Z_g;
+}
+
+/* The MUSCL face values of face f and its flux and bathymetry terms (orc_sw_face): inner faces
+ * reconstruct from both cells; a boundary (wall) face uses the Neumann (eta, Z) / nonslip
+ * (hu, hv) mirror of the reconstructed owner state as its right side, which at order 1 is the
+ * cell ghost h_g, -hu, -hv, Z_g of bc_utils.py:45-55. */
+void orc_sw_face_terms(int f, int inner, const double *h, const double *hu, const double *hv, const double *Z,
+                       const double *ex, const double *ey, const double *epsi, const double *ux, const double *uy,
+                       const double *upsi, const double *vx, const double *vy, const double *vpsi,
+                       const double *zx, const double *zy, const double *centerc, const double *centerf,
+                       const int32_t *cellidf, const double *normalf, const double *mesuref, double grav, double om,
+                       double *flux, double *sL, double *sR)
+{
+    int a = cellidf[2 * f];
+    double ra0 = centerf[3 * f] - centerc[3 * a], ra1 = centerf[3 * f + 1] - centerc[3 * a + 1];
+    double etaA = h[a] + Z[a];
+    double eL = etaA + om * epsi[a] * (ex[a] * ra0 + ey[a] * ra1);
+    double uL = hu[a] + om * upsi[a] * (ux[a] * ra0 + uy[a] * ra1);
+    double vL = hv[a] + om * vpsi[a] * (vx[a] * ra0 + vy[a] * ra1);
+    double zL = Z[a] + om * (zx[a] * ra0 + zy[a] * ra1);
+    const double *n = normalf + 3 * (int64_t)f;
+    if (inner) {
+        int b = cellidf[2 * f + 1];
+        double rb0 = centerf[3 * f] - centerc[3 * b], rb1 = centerf[3 * f + 1] - centerc[3 * b + 1];
+        double etaB = h[b] + Z[b];
+        double eR = etaB + om * epsi[b] * (ex[b] * rb0 + ey[b] * rb1);
+        double uR = hu[b] + om * upsi[b] * (ux[b] * rb0 + uy[b] * rb1);
+        double vR = hv[b] + om * vpsi[b] * (vx[b] * rb0 + vy[b] * rb1);
+        double zR = Z[b] + om * (zx[b] * rb0 + zy[b] * rb1);
+        orc_sw_face(eL, uL, vL, zL, h[a], Z[a], eR, uR, vR, zR, h[b], Z[b], n[0], n[1], mesuref[f], grav,
+                    flux, sL, sR);
+    } else {
+        orc_sw_face(eL, uL, vL, zL, h[a], Z[a], eL, -1 * uL, -1 * vL, zL, h[a], Z[a], n[0], n[1],
+                    mesuref[f], grav, flux, sL, sR);
     }
 }
 

@@ -27,6 +27,10 @@ static void face_dev(float hb, float aL, float aR, float dz, float huL, float hv
     const int wetL = hL > 0.f, wetR = hR > 0.f;
     float eL = aL - cL, eR = aR - cR;
     float hsL = hb + eL, hsR = hb + eR;
+    /* h - hs of each side from the deviations while both are positive (the difference of the
+       two rounded depths would lose the bathymetry step to cancellation) */
+    const float dhL = wetL ? ((hsL > 0.f) ? aL - eL : hb + aL) : 0.f;
+    const float dhR = wetR ? ((hsR > 0.f) ? aR - eR : hb + aR) : 0.f;
     if (!wetL) hL = 0.f;
     if (!(hsL > 0.f)) {
         hsL = 0.f;
@@ -59,11 +63,9 @@ static void face_dev(float hb, float aL, float aR, float dz, float huL, float hv
     *F1 = 0.5f * (hsL * uL * qL + hsR * uR * qR) + 0.5f * (PL + PR) * n0 - 0.5f * s * (hsR * uR - hsL * uL);
     *F2 = 0.5f * (hsL * vL * qL + hsR * vR * qR) + 0.5f * (PL + PR) * n1 - 0.5f * s * (hsR * vR - hsL * vL);
     if (own_is_L) {
-        const float dh = hL - hsL;
-        *bo = 0.5f * g * dh * (hL + hsL) + g * (hL + ho) * 0.5f * dzo;
+        *bo = 0.5f * g * dhL * (hL + hsL) + g * (hL + ho) * 0.5f * dzo;
     } else {
-        const float dh = hR - hsR;
-        *bo = 0.5f * g * dh * (hR + hsR) + g * (hR + ho) * 0.5f * dzo;
+        *bo = 0.5f * g * dhR * (hR + hsR) + g * (hR + ho) * 0.5f * dzo;
     }
 }
 

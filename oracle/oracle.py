@@ -193,6 +193,38 @@ def sw_step(d, h, hu, hv, Z, zgrad, grav, cfl, order):
     return dt
 
 
+class OmpSwStep(object):
+    """The all-cores CPU baseline of ``sw_step`` (omp_baseline.c): the oracle's own per-cell
+    and per-face code run with OpenMP, bit-identical to ``sw_step``. Holds the mesh arrays and
+    the scratch across steps."""
+
+    def __init__(self, d, Z):
+        C, F = int(d.nbcells), int(d.nbfaces)
+        self.d, self.C, self.F = d, C, F
+        self.lib = ctypes.CDLL(os.path.join(HERE, 'liboracle_omp.so'))
+        self.lib.orc_sw_step_omp.restype = _D
+        self.Z = np.ascontiguousarray(Z, dtype=np.float64)
+        zg = z_gradient(d, self.Z)
+        self.zx, self.zy = np.ascontiguousarray(zg[0]), np.ascontiguousarray(zg[1])
+        self.gc = np.ascontiguousarray(d.faces.ghostcenter, dtype=np.float64)
+        self.keep = [c3(d.cells.center), c3(d.faces.center), c3(d.faces.normal)]
+        self.scratch = np.zeros(26 * C + 13 * F)
+
+    def threads(self):
+        return int(self.lib.orc_omp_threads())
+
+    def step(self, h, hu, hv, grav, cfl, order):
+        d = self.d
+        cc, fc, nf = self.keep
+        return self.lib.orc_sw_step_omp(
+            _I(self.C), _I(self.F), f64(h), f64(hu), f64(hv), f64(self.Z), f64(self.zx), f64(self.zy), f64(cc),
+            f64(fc), i32(d.faces.cellid), f64(nf), f64(d.faces.mesure), i32(d.cells.faceid),
+            _I(d.cells.faceid.shape[1]), i32(d.faces.name), i32(d.cells.cellnid), _I(d.cells.cellnid.shape[1]),
+            i32(d.cells.ghostnid), _I(d.cells.ghostnid.shape[1]), f64(self.gc), _I(self.gc.shape[1]),
+            i32(d.innerfaces), _I(len(d.innerfaces)), i32(d.boundaryfaces), _I(len(d.boundaryfaces)),
+            f64(d.cells.volume), _D(grav), _D(cfl), _I(order), f64(self.scratch))
+
+
 # --- reference shallow-water path: SRNH flux + well-balanced source (oracle/srnh.c) ----------
 def srnh_roe(hroe, uroe, vroe, croe, grav):
     """Eigenvalues (5,) and sign matrix (5, 5) of fvm_utils.py:220-315."""

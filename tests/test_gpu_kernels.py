@@ -554,8 +554,11 @@ def test_sw_packed_first_order_and_quads(torch, oracle_lib):
 
 def test_sw_mixed_precision_error_and_conservation(torch, oracle_lib):
     """Mixed-precision path: bit-identical to its CPU float32 model (oracle/mixed32.c) after
-    200 steps of the dam break over bumps; relative L1 error (Variable.norml2, order 1) of
-    every field against the float64 oracle <= 1e-5; the float64 mass total conserved."""
+    5, 25 and 200 steps of the dam break over bumps; relative L1 error (Variable.norml2, order
+    1) of every field against the float64 oracle <= 1e-5 at each of them (the early steps,
+    where the tangential momentum is still small, are the hard case: <= 1e-6 is asserted to
+    keep the well-balanced deviation form of the bathymetry term); the float64 mass total
+    conserved."""
     from manapy_amd.device import DeviceMesh, SwMixed
     from manapy_amd.meshgen import gaussian_bumps
     d = make_tri_domain(60, 60, 100., 100., 0.1, 13)
@@ -564,14 +567,19 @@ def test_sw_mixed_precision_error_and_conservation(torch, oracle_lib):
     Z = gaussian_bumps(c, 8, np.random.default_rng(5), [(0., 100.), (0., 100.)])
     h0 = np.where(c[:, 0] < 50., 2.0, 1.0)
     G = 9.81
+    checks = (5, 25, 200)
     h, hu, hv = h0.copy(), np.zeros(C), np.zeros(C)
     zg_ref = oracle_lib.z_gradient(d, Z)
-    for _ in range(200):
-        oracle_lib.sw_step(d, h, hu, hv, Z, zg_ref, G, 0.8, 2)
     model = oracle_lib.MixedModel(d, h0, np.zeros(C), np.zeros(C), Z, zg_ref)
     dt_m = oracle_lib.time_step_sw(d, h0, np.zeros(C), np.zeros(C), 0.8, G)
-    for _ in range(200):
+    ref, mod, dts = {}, {}, {}
+    for k in range(1, checks[-1] + 1):
+        oracle_lib.sw_step(d, h, hu, hv, Z, zg_ref, G, 0.8, 2)
         dt_m = model.step(G, 0.8, dt_m)
+        if k in checks:
+            ref[k] = (h.copy(), hu.copy(), hv.copy())
+            mod[k] = model.state.copy()
+            dts[k] = dt_m
     m = DeviceMesh(d)
     p = SwMixed(m)
     th, thu, thv, tZ, zg = (dev(torch, x) for x in (h0, np.zeros(C), np.zeros(C), Z, zg_ref))
@@ -583,20 +591,24 @@ def test_sw_mixed_precision_error_and_conservation(torch, oracle_lib):
     ring = dev(torch, np.zeros(4))
     call('mfv_time_step_sw', m.ptr, P(th), P(thu), P(thv), 0.8, G, 0., 0., P(dt0), None)
     call('mfv_dt_ring_init', P(ring), P(dt0))
-    for s in range(200):
+    vol = d.cells.volume
+    oh, ohu, ohv = (dev(torch, np.zeros(C)) for _ in range(3))
+    for s in range(checks[-1]):
         call('mfv_sw_recon_mixed', m.ptr, p.ptr, 2)
         call('mfv_sw_flux_mixed', m.ptr, p.ptr, G, 0.8, P(ring), s)
-    same(host(p.state), model.state, "mixed state vs its float32 model")
-    assert host(ring)[200 % 3] == dt_m
-    oh, ohu, ohv = (dev(torch, np.zeros(C)) for _ in range(3))
-    call('mfv_sw_unpack_mixed', m.ptr, p.ptr, P(oh), P(ohu), P(ohv))
+        k = s + 1
+        if k not in checks:
+            continue
+        same(host(p.state), mod[k], "mixed state vs its float32 model after %d steps" % k)
+        assert host(ring)[k % 3] == dts[k]
+        call('mfv_sw_unpack_mixed', m.ptr, p.ptr, P(oh), P(ohu), P(ohv))
+        errs = {}
+        for nm, a, b in (('h', host(oh), ref[k][0]), ('hu', host(ohu), ref[k][1]), ('hv', host(ohv), ref[k][2])):
+            errs[nm] = float(np.sum(np.abs(a - b) * vol) / np.sum(np.abs(b) * vol))
+        print("mixed-precision relative L1 after %d steps:" % k, errs)
+        assert max(errs.values()) <= 1e-5, (k, errs)
+        assert max(errs.values()) <= 1e-6, (k, errs)
     call('mfv_sw_conservation', m.ptr, p.ptr, P(p.partial), p.partial.numel(), P(p.sums))
-    vol = d.cells.volume
-    errs = {}
-    for nm, a, b in (('h', host(oh), h), ('hu', host(ohu), hu), ('hv', host(ohv), hv)):
-        errs[nm] = float(np.sum(np.abs(a - b) * vol) / np.sum(np.abs(b) * vol))
-    print("mixed-precision relative L1 after 200 steps:", errs)
-    assert max(errs.values()) <= 1e-5, errs
     assert abs(host(p.sums)[0] - mass0) <= 1e-12 * mass0
     assert abs(mass0 - np.sum(h0 * vol)) <= 1e-12 * mass0
 

@@ -478,8 +478,9 @@ def test_mixed_model_close_to_fp64_and_mass_exact(oracle_lib):
 def test_mixed_model_lake_at_rest(oracle_lib):
     """eta = h + Z constant and no velocity stays at rest over bumps to float32 resolution:
     the hydrostatic balance cancels exactly in exact arithmetic (the float64 oracle keeps
-    |hu|, |hv| ~ 1e-14); with float32 face arithmetic the balance holds to its rounding,
-    ~1e-7 of the hydrostatic terms per step."""
+    |hu|, |hv| ~ 1e-14); with float32 face arithmetic it holds to the rounding of the
+    deviations (~1e-7 of the bathymetry step, not of the depth: the source term's h - hs is
+    formed from deviations, and a difference of the two rounded depths leaves ~3e-6 here)."""
     d, Z, _ = _dam_case(30)
     C = int(d.nbcells)
     h0 = 2.0 - Z
@@ -488,8 +489,8 @@ def test_mixed_model_lake_at_rest(oracle_lib):
     dt = oracle_lib.time_step_sw(d, h0, np.zeros(C), np.zeros(C), 0.8, 9.81)
     for _ in range(50):
         dt = M.step(9.81, 0.8, dt)
-    assert np.abs(M.state[:, 1:3]).max() < 5e-6
-    assert np.abs(M.state[:, 0] - h0).max() < 1e-6
+    assert np.abs(M.state[:, 1:3]).max() < 3e-7
+    assert np.abs(M.state[:, 0] - h0).max() < 3e-8
 
 
 # ------------------------------------------------------------------------- Euler dt / update
@@ -621,3 +622,22 @@ def test_barth_limiter_3d_characterisation(oracle_lib, dom3d):
     oracle_lib.cell_gradient(d, wc, wg, *g)
     oracle_lib.barthlimiter(d, wc, wg, *g, psi)
     assert np.all(psi == 1.)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_omp_baseline_equals_serial_oracle(oracle_lib, order):
+    """The all-cores CPU baseline (OpenMP over cells and faces, each face's terms formed once
+    and gathered per cell in the scatter's order) gives the serial oracle's bits and dt."""
+    d, Z, h0 = _dam_case(30)
+    C = int(d.nbcells)
+    zg = oracle_lib.z_gradient(d, Z)
+    a = [h0.copy(), np.zeros(C), np.zeros(C)]
+    b = [h0.copy(), np.zeros(C), np.zeros(C)]
+    omp = oracle_lib.OmpSwStep(d, Z)
+    assert omp.threads() >= 1
+    for _ in range(12):
+        dt_a = oracle_lib.sw_step(d, a[0], a[1], a[2], Z, zg, G, 0.8, order)
+        dt_b = omp.step(b[0], b[1], b[2], G, 0.8, order)
+        assert dt_a == dt_b
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
