@@ -420,8 +420,60 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=5, renumber='hilbert'):
            "path": best, "path_ms_per_step": {k: round(v, 4) for k, v in probe.items()}, "cell_order": renumber,
            "roofline": roofline(kname + " + k_dt_finish", SURVEY_B_EULER64, C, t_ev, bpc,
                                 measured_traffic(kname, C))}
+    # FP32-bulk variant on the same tiles: throughput, then 200-step (or --euler-steps) accuracy
+    # against the float64 step and the float64 conservation sums
+    ptile = ptiles.get(256) or ptiles.get(128)
+    if ptile is not None:
+        from manapy_amd.device import Euler32
+        e32 = Euler32(m, et=et, tt=ptile)
+        T32 = int(ptile.desc.tile_size)
+
+        def init32():
+            S[0].copy_(dv(U0))
+            _lib.call('mfv_euler3d_time_step', m.ptr, S[0].data_ptr(), CFL, 1.4, dt0.data_ptr(), None, s)
+            _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
+            e32.pack(S[0], s)
+
+        def step32(k):
+            e32.step(1.4, CFL, 0, ring, k, sums=False, stream=s)
+        init32()
+        t32_loop, t32_ev = time_steps(step32, warmup, steps)
+        # accuracy: both paths from the initial state for ``steps`` steps
+        path[0] = best
+        init()
+        for k in range(steps):
+            step(k)
+        if best.startswith('packed'):
+            _lib.call('mfv_euler3d_unpack', m.ptr, R[steps % 2].data_ptr(), S[steps % 2].data_ptr(), s)
+        U64 = S[steps % 2].cpu().numpy()
+        init32()
+        for k in range(steps):
+            e32.step(1.4, CFL, 0, ring, k, sums=(k == steps - 1), stream=s)
+        U32t = torch.empty((6, C), dtype=torch.float64, device='cuda')
+        e32.unpack(U32t, s)
+        U32 = U32t.cpu().numpy()
+        cons32 = e32.cons.cpu().numpy()
+        vol = d.cells.volume
+        cons64 = np.array([np.sum(U64[k] * vol) for k in (0, 2, 3, 4, 5)])
+        errs = {nm: rel_errors(U32[k], U64[k], vol) for k, nm in enumerate(('rho', 'P', 'rhou', 'rhov', 'rhow',
+                                                                             'rhoE'))}
+        b32 = 32 + 32 + 16 + 1 + 24 * F / C
+        out["fp32_bulk"] = {
+            "value": round(C / t32_loop / 1e6, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t32_loop * 1e3, 5),
+            "speedup_vs_fp64": round(t_loop / t32_loop, 3), "tile": T32,
+            "dtype": "f32 records, geometry and face fluxes; f64 gathers, update, dt and conservation sums",
+            "roofline": roofline("k_euler_step32_tiled<%d> + k_dt_finish" % T32, SURVEY_B_EULER32, C, t32_ev, b32,
+                                 measured_traffic("k_euler_step32_tiled<%d>" % T32, C)),
+            "accuracy": {"steps": steps, "error_vs_fp64": errs,
+                         "max_rel_L1": max(v["rel_L1"] for v in errs.values()), "target_rel_L1": 1e-5,
+                         "conservation_sums_f64": {"fields": ["rho", "rhou", "rhov", "rhow", "rhoE"],
+                                                   "fp32_bulk": cons32.tolist(), "fp64_run": cons64.tolist(),
+                                                   "rel_diff": ((cons32 - cons64) / np.abs(cons64)).tolist()}}}
+        log("euler fp32-bulk: %.1f Mcell-updates/s, max rel L1 %.3g" % (out["fp32_bulk"]["value"],
+                                                                       out["fp32_bulk"]["accuracy"]["max_rel_L1"]))
     if cpu and cpu_steps:
         from oracle import oracle as O
+        path[0] = best
         init()
         for k in range(cpu_steps):
             step(k)

@@ -533,6 +533,23 @@ int mfv_euler3d_step_packed_tiled(const mfv_mesh *m, const mfv_euler_tet *t, con
                                   const double *rec_in, double *rec_out, double gamma, double cfl, int quirk,
                                   double *dt_ring, int step, void *stream);
 
+/* FP32-bulk Euler (the float32 variant of explicitscheme_euler_3d_rusanov + update_euler_3d_fvc
+ * + time_step_euler_3d, fvm_utils3d.py:838-886, 1083-1170, 1316-1327, 809-825): float32 records
+ * [C][8] = (rho, P, rhou, rhov, rhow, rhoE, |K|, 0) and float32 face geometry per tile entry
+ * [entries][4] = (nx, ny, nz, |f|); float32 face fluxes on the same face tiles (the pressure of
+ * the momentum flux as 1/2 (P_L - P_R) n, added by both cells: the same scheme in exact
+ * arithmetic, since sum_f P_K n_f = 0 on a closed cell, and zero where the pressure is uniform);
+ * float64 gathers in the residual order, float64 update, float32 store; float64 dt ring and, when
+ * cons is not NULL, float64 conservation sums cons[5] = sum_K (rho, rhou, rhov, rhow, rhoE) |K|
+ * of the new state (block partials folded in a fixed order). The CPU model oracle/euler32.c
+ * gives the same records and dt bit for bit. LDS: 24 * max_items + 24 * max_faces bytes. */
+int mfv_euler32_pack(const mfv_mesh *m, const double *U, float *rec, void *stream);
+int mfv_euler32_unpack(const mfv_mesh *m, const float *rec, double *U, void *stream);
+int mfv_euler32_tile_geometry(const mfv_euler_tet_tiles *tt, int n_entries, float *geo32, void *stream);
+int mfv_euler3d_step32_tiled(const mfv_mesh *m, const mfv_euler_tet *t, const mfv_euler_tet_tiles *tt,
+                             const float *geo32, const float *rec_in, float *rec_out, double gamma, double cfl,
+                             int quirk, double *dt_ring, int step, double *cons, void *stream);
+
 /* ---------------- diamond scheme, 2D (ast/functions2d.py, solvers/diffusion) --------- */
 
 /* Diamond-scheme geometry. The pseudo-Laplacian weight of a node incidence is

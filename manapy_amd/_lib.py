@@ -208,6 +208,10 @@ _SIGNATURES = {
     'mfv_euler3d_unpack': [MESH_P, P, P, P],
     'mfv_euler3d_step_packed': [MESH_P, ET_P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_euler3d_step_packed_tiled': [MESH_P, ET_P, ETT_P, P, P, c_double, c_double, c_int, P, c_int, P],
+    'mfv_euler32_pack': [MESH_P, P, P, P],
+    'mfv_euler32_unpack': [MESH_P, P, P, P],
+    'mfv_euler32_tile_geometry': [ETT_P, c_int, P, P],
+    'mfv_euler3d_step32_tiled': [MESH_P, ET_P, ETT_P, P, P, P, c_double, c_double, c_int, P, c_int, P, P],
     'mfv_diamond_widths': [DIA_P, P, P, P, P, P],
     'mfv_diamond_assemble': [DIA_P, P, P, c_int, c_int, P, P, P, P, P, P],
     'mfv_diamond_rhs': [DIA_P, P, P, P, P, P, P],

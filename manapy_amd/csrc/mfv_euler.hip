@@ -600,6 +600,264 @@ __global__ void __launch_bounds__(T) k_euler_step_packed_tiled(EulerTetK t, Eule
     wave_min_part(dtpart, (size_t)c0 + q, d);
 }
 
+// ---------------------------------------------------------------------------------------------
+// FP32-bulk step: float32 records [C][8] = (rho, P, rhou, rhov, rhow, rhoE, |K|, 0) and float32
+// face geometry per tile entry (nx, ny, nz, |f|); float32 face fluxes (pressure of the momentum
+// flux in deviation form, 1/2 (P_L - P_R) n, added by both cells); float64 per-cell gather in the
+// residual order, float64 update, float32 store; float64 dt and conservation sums. The CPU
+// model oracle/euler32.c makes the same operations in the same order.
+
+struct Rusanov32 {
+    float F[5];  // mass, momentum without pressure, energy (x |f|)
+    float pd;    // 1/2 (P_L - P_R) |f|
+};
+
+__device__ __forceinline__ float pmax32(float a, float b) { return (b > a) ? b : a; }
+
+__device__ __forceinline__ Rusanov32 flux32(const float (&L)[6], const float (&R)[6], float4 g, float gamma,
+                                            int quirk)
+{
+    const float mes = g.w;
+    const float n0 = g.x / mes, n1 = g.y / mes, n2 = g.z / mes;
+    const float rl = L[0], Pl = L[1], ul = L[2], vl = L[3], wl = L[4], El = L[5];
+    const float rr = R[0], Pr = R[1], ur = R[2], vr = R[3], wr = R[4], Er = R[5];
+    const float ql = ul * n0 + vl * n1 + wl * n2;
+    const float qr = quirk ? (ur * n0 + vl * n1 + wl * n2) : (ur * n0 + vr * n1 + wr * n2);
+    const float qrt = ur * n0 + vr * n1 + wr * n2;
+    const float cl = sqrtf(gamma * Pl / rl), cr = sqrtf(gamma * Pr / rr);
+    const float al = ql / rl, ar = qr / rr;
+    const float Ll = pmax32(pmax32(fabsf(al - cl), fabsf(al)), fabsf(al + cl));
+    const float Lr = pmax32(pmax32(fabsf(ar - cr), fabsf(ar)), fabsf(ar + cr));
+    const float S = (Ll > Lr) ? Ll : Lr;
+    const float sl = ql / rl, sr = qrt / rr;
+    Rusanov32 o;
+    o.F[0] = (0.5f * (ql + qrt) - 0.5f * S * (rr - rl)) * mes;
+    o.F[1] = (0.5f * (ul * sl + ur * sr) - 0.5f * S * (ur - ul)) * mes;
+    o.F[2] = (0.5f * (vl * sl + vr * sr) - 0.5f * S * (vr - vl)) * mes;
+    o.F[3] = (0.5f * (wl * sl + wr * sr) - 0.5f * S * (wr - wl)) * mes;
+    o.F[4] = (0.5f * ((El + Pl) * sl + (Er + Pr) * sr) - 0.5f * S * (Er - El)) * mes;
+    o.pd = 0.5f * (Pl - Pr) * mes;
+    return o;
+}
+
+// ghost_value_TubeSchok3D in float32 (quirk: rhovg = rho w_g, rhowg = 0)
+__device__ __forceinline__ void ghost32(const float (&c)[6], float4 g, int name, int quirk, float (&o)[6])
+{
+#pragma unroll
+    for (int k = 0; k < 6; k++) o[k] = c[k];
+    if (name >= 3 && name <= 6) {
+        const float u = c[2] / c[0], v = c[3] / c[0], w = c[4] / c[0];
+        const float nn = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+        const float s0 = g.x / nn, s1 = g.y / nn, s2 = g.z / nn;
+        const float un = u * s0 + v * s1 + w * s2;
+        const float ug = u - 2.f * un * s0, vg = v - 2.f * un * s1, wg = w - 2.f * un * s2;
+        o[2] = c[0] * ug;
+        if (quirk) {
+            o[3] = c[0] * wg;
+            o[4] = 0.f;
+        } else {
+            o[3] = c[0] * vg;
+            o[4] = c[0] * wg;
+        }
+    }
+}
+
+// Block sum of v[5] in a fixed tree (so the sums do not depend on timing); result in lane 0 of
+// wave 0. red: 5 * (T / 64) doubles of LDS.
+template <int T>
+__device__ __forceinline__ void block_sum5(double (&v)[5], double *red)
+{
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < 5; k++) red[k * (T / 64) + w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            double a = 0.;
+            for (int j = 0; j < T / 64; j++) a += red[k * (T / 64) + j];
+            v[k] = a;
+        }
+}
+
+template <int T>
+__global__ void __launch_bounds__(T) k_euler_step32_tiled(EulerTetK t, EulerTilesK tt, const float4 *__restrict__ geo,
+                                                           const float *__restrict__ Rin, float *__restrict__ Rout,
+                                                           float gamma32, double gamma, double cfl, int quirk,
+                                                           DtRing ring, double *__restrict__ dtpart,
+                                                           double *__restrict__ conspart)
+{
+    extern __shared__ float lds32[];
+    float2 *sR = reinterpret_cast<float2 *>(lds32);  // [MI][3]: six values per item
+    float *sF = lds32 + 6 * tt.MI;                   // [6][MFc]
+    const int tile = xcd_block();
+    const int c0 = tt.cell0[tile], nc = tt.ncells[tile];
+    const int h0 = tt.hptr[tile], ni = nc + (tt.hptr[tile + 1] - h0);
+    const int f0 = tt.fptr[tile], nfc = tt.fptr[tile + 1] - f0;
+    ring_advance(ring);
+    for (int e = threadIdx.x; e < ni; e += T) {
+        const int c = (e < nc) ? c0 + e : tt.halo[h0 + (e - nc)];
+        const float4 *rp = reinterpret_cast<const float4 *>(Rin + 8 * (size_t)c);
+        const float4 a = rp[0], b = rp[1];
+        sR[3 * e] = make_float2(a.x, a.y);
+        sR[3 * e + 1] = make_float2(a.z, a.w);
+        sR[3 * e + 2] = make_float2(b.x, b.y);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nfc; k += T) {
+        const float4 g = geo[f0 + k];
+        const int2 lr = tt.flr[f0 + k];
+        const float2 la = sR[3 * lr.x], lb = sR[3 * lr.x + 1], lc = sR[3 * lr.x + 2];
+        const float L[6] = {la.x, la.y, lb.x, lb.y, lc.x, lc.y};
+        float R[6];
+        if (lr.y >= 0) {
+            const float2 ra = sR[3 * lr.y], rb = sR[3 * lr.y + 1], rc = sR[3 * lr.y + 2];
+            R[0] = ra.x; R[1] = ra.y; R[2] = rb.x; R[3] = rb.y; R[4] = rc.x; R[5] = rc.y;
+        } else {
+            ghost32(L, g, -lr.y, quirk, R);
+        }
+        const Rusanov32 o = flux32(L, R, g, gamma32, quirk);
+#pragma unroll
+        for (int v = 0; v < 5; v++) sF[v * tt.MFc + k] = o.F[v];
+        sF[5 * tt.MFc + k] = o.pd;
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    double d = 1e300;
+    double cs[5] = {0., 0., 0., 0., 0.};
+    if (q < nc) {
+        const int i = c0 + q;
+        const int4 c4 = tt.code[i];
+        const unsigned ord = t.cell_rank[i];
+        const float vol32 = Rin[8 * (size_t)i + 6];
+        const int cd[4] = {c4.x, c4.y, c4.z, c4.w};
+        float4 g[4];  // the faces' geometry from the tile's entries (in cache since the face phase)
+#pragma unroll
+        for (int k = 0; k < 4; k++) g[k] = geo[f0 + (cd[k] >> 1)];
+        double r[5] = {0., 0., 0., 0., 0.};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {  // faces in increasing original id
+            const int ph = cd[k] >> 1;
+            const bool own_R = cd[k] & 1;
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                const double x = (double)sF[v * tt.MFc + ph];
+                r[v] += own_R ? x : -x;
+            }
+            const float pd = sF[5 * tt.MFc + ph];
+            r[1] += (double)(pd * (g[k].x / g[k].w));
+            r[2] += (double)(pd * (g[k].y / g[k].w));
+            r[3] += (double)(pd * (g[k].z / g[k].w));
+        }
+        const float2 oa = sR[3 * q], ob = sR[3 * q + 1], oc = sR[3 * q + 2];
+        const double vol = (double)vol32;
+        const double dv = ring.cur[0] / vol;
+        const double rho = (double)oa.x + r[0] * dv;
+        const double mx = (double)ob.x + r[1] * dv;
+        const double my = (double)ob.y + r[2] * dv;
+        const double mz = (double)oc.x + r[3] * dv;
+        const double E = (double)oc.y + r[4] * dv;
+        const double P = (gamma - 1.) * (E - 0.5 * (mx * mx + my * my + mz * mz) / rho);
+        const float w0 = (float)rho, w1 = (float)P, w2 = (float)mx, w3 = (float)my, w4 = (float)mz, w5 = (float)E;
+        float4 *wp = reinterpret_cast<float4 *>(Rout + 8 * (size_t)i);
+        wp[0] = make_float4(w0, w1, w2, w3);
+        wp[1] = make_float4(w4, w5, vol32, 0.f);
+        cs[0] = (double)w0 * vol;
+        cs[1] = (double)w2 * vol;
+        cs[2] = (double)w3 * vol;
+        cs[3] = (double)w4 * vol;
+        cs[4] = (double)w5 * vol;
+        // time_step_euler_3d of the stored state in float64, terms summed in slot order
+        const double ur = w0, uP = w1, ux = w2, uy = w3, uz = w4;
+        const double velson = sqrt(gamma * fabs(uP / ur));
+        double tm[4];
+        int sl[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double nx = g[k].x, ny = g[k].y, nz = g[k].z;
+            const double u_n = fabs((ux * nx + uy * ny + uz * nz) / ur) / sqrt(nx * nx + ny * ny + nz * nz);
+            tm[k] = (u_n + velson) * (double)g[k].w;
+            sl[k] = (ord >> (2 * k)) & 3;
+        }
+        double lam = 0.;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double tj = tm[0];
+#pragma unroll
+            for (int k = 1; k < 4; k++) tj = sl[k] == j ? tm[k] : tj;
+            lam += tj;
+        }
+        d = cfl * vol / lam;
+    }
+    if ((q & ~63) < nc) wave_min_part(dtpart, (size_t)c0 + q, d);
+    if (conspart) {
+        __syncthreads();  // the flux LDS is free: reuse it for the block sums
+        block_sum5<T>(cs, reinterpret_cast<double *>(lds32));
+        if (q == 0)
+#pragma unroll
+            for (int k = 0; k < 5; k++) conspart[(size_t)k * gridDim.x + tile] = cs[k];
+    }
+}
+
+// cons[k] = sum over tiles of the block partials (one block, fixed order per thread, tree fold)
+__global__ void __launch_bounds__(256) k_cons_fold(const double *__restrict__ part, int n, double *__restrict__ cons)
+{
+    __shared__ double sh[256];
+    for (int k = 0; k < 5; k++) {
+        double a = 0.;
+        for (int j = threadIdx.x; j < n; j += 256) a += part[(size_t)k * n + j];
+        sh[threadIdx.x] = a;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) cons[k] = sh[0];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_euler32_pack(int C, const double *__restrict__ U,
+                                                          const double *__restrict__ vol, float *__restrict__ rec)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const size_t n = C;
+    float4 *w = reinterpret_cast<float4 *>(rec + 8 * (size_t)i);
+    w[0] = make_float4((float)U[i], (float)U[n + i], (float)U[2 * n + i], (float)U[3 * n + i]);
+    w[1] = make_float4((float)U[4 * n + i], (float)U[5 * n + i], (float)vol[i], 0.f);
+}
+
+__global__ void __launch_bounds__(kBlock) k_euler32_unpack(int C, const float *__restrict__ rec,
+                                                            double *__restrict__ U)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const size_t n = C;
+    const float4 *p = reinterpret_cast<const float4 *>(rec + 8 * (size_t)i);
+    const float4 a = p[0], b = p[1];
+    U[i] = a.x;
+    U[n + i] = a.y;
+    U[2 * n + i] = a.z;
+    U[3 * n + i] = a.w;
+    U[4 * n + i] = b.x;
+    U[5 * n + i] = b.y;
+}
+
+__global__ void __launch_bounds__(kBlock) k_euler32_geo(int n, const double2 *__restrict__ g64,
+                                                         float4 *__restrict__ g32)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double2 a = g64[2 * (size_t)k], b = g64[2 * (size_t)k + 1];
+    g32[k] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+}
+
 __global__ void __launch_bounds__(kBlock) k_euler_pack(int C, const double *__restrict__ U,
                                                         const double *__restrict__ vol, double *__restrict__ rec)
 {
@@ -1154,4 +1412,94 @@ extern "C" int mfv_euler3d_step_packed_tiled(const mfv_mesh *m, const mfv_euler_
     }
     if (int rc = check_launch("k_euler_step_packed_tiled")) return rc;
     return dt_finish(part, nw, r.next, st);
+}
+
+extern "C" int mfv_euler32_pack(const mfv_mesh *m, const double *U, float *rec, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    MFV_REQUIRE(U && rec, "NULL array");
+    MFV_REQUIRE(((uintptr_t)rec & 15) == 0, "misaligned records");
+    if (m->nb_cells == 0) return MFV_OK;
+    hipLaunchKernelGGL(k_euler32_pack, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream), m->nb_cells,
+                       U, m->cell_volume, rec);
+    return check_launch("k_euler32_pack");
+}
+
+extern "C" int mfv_euler32_unpack(const mfv_mesh *m, const float *rec, double *U, void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    MFV_REQUIRE(U && rec, "NULL array");
+    MFV_REQUIRE(((uintptr_t)rec & 15) == 0, "misaligned records");
+    if (m->nb_cells == 0) return MFV_OK;
+    hipLaunchKernelGGL(k_euler32_unpack, dim3(grid_for(m->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                       m->nb_cells, rec, U);
+    return check_launch("k_euler32_unpack");
+}
+
+extern "C" int mfv_euler32_tile_geometry(const mfv_euler_tet_tiles *tt, int n_entries, float *geo32, void *stream)
+{
+    MFV_REQUIRE(tt && tt->face_geo && geo32 && n_entries >= 0, "NULL argument");
+    MFV_REQUIRE(((uintptr_t)geo32 & 15) == 0 && ((uintptr_t)tt->face_geo & 15) == 0, "misaligned geometry");
+    if (n_entries == 0) return MFV_OK;
+    hipLaunchKernelGGL(k_euler32_geo, dim3(grid_for(n_entries)), dim3(kBlock), 0, as_stream(stream), n_entries,
+                       reinterpret_cast<const double2 *>(tt->face_geo), reinterpret_cast<float4 *>(geo32));
+    return check_launch("k_euler32_geo");
+}
+
+extern "C" int mfv_euler3d_step32_tiled(const mfv_mesh *m, const mfv_euler_tet *t, const mfv_euler_tet_tiles *tt,
+                                        const float *geo32, const float *rec_in, float *rec_out, double gamma,
+                                        double cfl, int quirk, double *dt_ring_p, int step, double *cons,
+                                        void *stream)
+{
+    if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
+    MFV_REQUIRE(m->dim == 3, "the Euler step needs a 3D mesh");
+    if (int rc = check_euler_tet(m, t)) return rc;
+    if (int rc = check_euler_tiles(m, tt)) return rc;
+    if (m->nb_cells == 0) return MFV_OK;
+    MFV_REQUIRE(geo32 && rec_in && rec_out && rec_in != rec_out && dt_ring_p, "NULL or aliased array");
+    MFV_REQUIRE(((uintptr_t)rec_in & 15) == 0 && ((uintptr_t)rec_out & 15) == 0 && ((uintptr_t)geo32 & 15) == 0,
+                "misaligned records or geometry");
+    DtRing r = dt_ring(dt_ring_p, step);
+    const size_t nw = ((size_t)m->nb_cells + 63) / 64;  // waves with cells (tiles are multiples of 64 cells)
+    hipStream_t st = as_stream(stream);
+    double *part = dt_scratch(nw + 5 * (size_t)tt->nb_tiles, st);
+    if (!part) return MFV_ENOMEM;
+    double *conspart = cons ? part + nw : nullptr;
+    const EulerTetK tk{reinterpret_cast<const int4 *>(t->cell_nb), t->cell_rank,
+                       reinterpret_cast<const double2 *>(t->cell_geo), (size_t)m->nb_cells};
+    const EulerTilesK ek{tt->tile_cell0,
+                         tt->tile_ncells,
+                         tt->halo_ptr,
+                         tt->halo_cells,
+                         tt->face_ptr,
+                         reinterpret_cast<const double2 *>(tt->face_geo),
+                         reinterpret_cast<const int2 *>(tt->face_lr),
+                         reinterpret_cast<const int4 *>(tt->cell_code),
+                         tt->max_items,
+                         tt->max_faces};
+    const size_t lds = 24 * (size_t)tt->max_items + 24 * (size_t)tt->max_faces;
+    MFV_REQUIRE(lds >= 5 * 8 * 8, "tile too small for the conservation sums");
+    const float4 *g = reinterpret_cast<const float4 *>(geo32);
+    const float g32 = (float)gamma;
+    switch (tt->tile_size) {
+    case 128:
+        hipLaunchKernelGGL(k_euler_step32_tiled<128>, dim3(tt->nb_tiles), dim3(128), lds, st, tk, ek, g, rec_in,
+                           rec_out, g32, gamma, cfl, quirk, r, part, conspart);
+        break;
+    case 256:
+        hipLaunchKernelGGL(k_euler_step32_tiled<256>, dim3(tt->nb_tiles), dim3(256), lds, st, tk, ek, g, rec_in,
+                           rec_out, g32, gamma, cfl, quirk, r, part, conspart);
+        break;
+    default:
+        hipLaunchKernelGGL(k_euler_step32_tiled<512>, dim3(tt->nb_tiles), dim3(512), lds, st, tk, ek, g, rec_in,
+                           rec_out, g32, gamma, cfl, quirk, r, part, conspart);
+        break;
+    }
+    if (int rc = check_launch("k_euler_step32_tiled")) return rc;
+    if (int rc = dt_finish(part, nw, r.next, st)) return rc;
+    if (cons) {
+        hipLaunchKernelGGL(k_cons_fold, dim3(1), dim3(256), 0, st, conspart, tt->nb_tiles, cons);
+        return check_launch("k_cons_fold");
+    }
+    return MFV_OK;
 }

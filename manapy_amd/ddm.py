@@ -448,7 +448,7 @@ class Domain(object):
         self._backend_name = get('backend')
         self._backend = None
         if isinstance(mesh, str):
-            from .meshio_gmsh import read_gmsh
+            from .gmsh import read_gmsh
             elements, nodes = read_gmsh(mesh, dim)
         else:
             elements, nodes = mesh
@@ -1015,6 +1015,38 @@ class Domain(object):
             from .backend import HIPBackend
             self._backend = HIPBackend.for_domain(self)
         return self._backend
+
+    # -- output (ddm/domain.py:624-849): VTU pieces written in the background -------------------
+    def _writer(self):
+        if getattr(self, '_vtu', None) is None:
+            from .vtu import VtuWriter
+            self._vtu = VtuWriter(self, 'results')
+        return self._vtu
+
+    def save_on_cell(self, dt=0, time=0, niter=0, miter=0, value=None):
+        if value is None:
+            raise ValueError("value must be given")
+        return self._writer().save(miter, ['w'], [value], on_cells=True)
+
+    def save_on_node(self, dt=0, time=0, niter=0, miter=0, value=None):
+        if value is None:
+            raise ValueError("value must be given")
+        return self._writer().save(miter, ['w'], [value], on_cells=False)
+
+    def save_on_cell_multi(self, dt=0, time=0, niter=0, miter=0, variables=None, values=None, file_format="vtu"):
+        if file_format != "vtu":
+            raise NotImplementedError("only VTU output is written")
+        return self._writer().save(miter, list(variables), list(values), on_cells=True)
+
+    def save_on_node_multi(self, dt=0, time=0, niter=0, miter=0, variables=None, values=None, file_format="vtu"):
+        if file_format != "vtu":
+            raise NotImplementedError("only VTU output is written")
+        return self._writer().save(miter, list(variables), list(values), on_cells=False)
+
+    def wait_outputs(self):
+        """Join every pending background VTU write."""
+        if getattr(self, '_vtu', None) is not None:
+            self._vtu.wait()
 
     def __str__(self):
         return ('\n> dim   :: %d\n> total cells  :: %d\n> total nodes  :: %d\n> total faces  :: %d\n'

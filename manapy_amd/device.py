@@ -635,6 +635,44 @@ class EulerTetTiles(object):
         return 48 * self.host['max_items'] + 40 * self.host['max_faces']
 
 
+class Euler32(object):
+    """FP32-bulk Euler state on face tiles: float32 records [C][8], the tiles' float32 face
+    geometry, and the float64 conservation sums of the last step (mfv_euler3d_step32_tiled)."""
+
+    def __init__(self, dmesh, T=256, et=None, tt=None):
+        import torch
+        self.m = dmesh
+        self.et = et if et is not None else EulerTet(dmesh)
+        self.tt = tt if tt is not None else EulerTetTiles(dmesh, T)
+        n = int(self.tt.host['face_ptr'][-1])
+        self.geo32 = torch.empty((max(n, 1), 4), dtype=torch.float32, device=dmesh.device)
+        _lib.call('mfv_euler32_tile_geometry', self.tt.ptr, n, self.geo32.data_ptr(), stream_ptr())
+        C = int(dmesh.C)
+        self.rec = [torch.zeros((C, 8), dtype=torch.float32, device=dmesh.device) for _ in range(2)]
+        self.cons = torch.zeros(5, dtype=torch.float64, device=dmesh.device)
+        self.k = 0
+
+    def lds_bytes(self):
+        return 24 * self.tt.host['max_items'] + 24 * self.tt.host['max_faces']
+
+    def pack(self, U, stream=None):
+        """float64 [6][C] state -> the current float32 records."""
+        self.k = 0
+        _lib.call('mfv_euler32_pack', self.m.ptr, U.data_ptr(), self.rec[0].data_ptr(),
+                  stream if stream is not None else stream_ptr())
+
+    def unpack(self, U, stream=None):
+        _lib.call('mfv_euler32_unpack', self.m.ptr, self.rec[self.k % 2].data_ptr(), U.data_ptr(),
+                  stream if stream is not None else stream_ptr())
+
+    def step(self, gamma, cfl, quirk, ring, step, sums=True, stream=None):
+        a, b = self.rec[self.k % 2], self.rec[(self.k + 1) % 2]
+        _lib.call('mfv_euler3d_step32_tiled', self.m.ptr, self.et.ptr, self.tt.ptr, self.geo32.data_ptr(),
+                  a.data_ptr(), b.data_ptr(), float(gamma), float(cfl), int(quirk), ring.data_ptr(), int(step),
+                  self.cons.data_ptr() if sums else None, stream if stream is not None else stream_ptr())
+        self.k += 1
+
+
 class SwTiles(object):
     """Device copy of a tiles layout and its mfv_sw_tiles descriptor (stage=True copies every
     cell state a tile reads into LDS before reconstructing)."""

@@ -1,4 +1,4 @@
-"""ORACLE build recipe (test infrastructure): compiles the C restatements (mfv_oracle.c, mixed32.c, srnh.c)
+"""ORACLE build recipe (test infrastructure): compiles the C restatements (mfv_oracle.c, mixed32.c, srnh.c, euler32.c)
 to oracle/liboracle.so with strict IEEE semantics (no contraction, no fast-math), and the
 OpenMP CPU baseline (omp_baseline.c) to oracle/liboracle_omp.so."""
 import os
@@ -17,7 +17,7 @@ def build_oracle(verbose=False):
     OpenMP all-cores baseline of the shallow-water step). -O3 without fast-math or contraction
     keeps every result bit-identical to -O0."""
     cmds = [(['gcc'] + FLAGS + [os.path.join(HERE, 'mfv_oracle.c'), os.path.join(HERE, 'mixed32.c'),
-                                os.path.join(HERE, 'srnh.c'), '-lm'], LIB),
+                                os.path.join(HERE, 'srnh.c'), os.path.join(HERE, 'euler32.c'), '-lm'], LIB),
             (['gcc'] + FLAGS + ['-fopenmp', os.path.join(HERE, 'omp_baseline.c'), '-lm'], LIB_OMP)]
     for cmd, out in cmds:
         cmd = cmd + ['-o', out + '.tmp']

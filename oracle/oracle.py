@@ -32,6 +32,7 @@ def lib():
         _lib.orc_time_step_sw.restype = _D
         _lib.orc_mixed_flux.restype = _D
         _lib.orc_mixed_dt.restype = _D
+        _lib.orc_euler32_step.restype = _D
     return _lib
 
 
@@ -191,6 +192,42 @@ def sw_step(d, h, hu, hv, Z, zgrad, grav, cfl, order):
     rez, src = sw_residuals(d, h, hu, hv, Z, grad, zgrad, grav, order)
     update_sw(h, hu, hv, rez, src, dt, d.cells.volume)
     return dt
+
+
+class Euler32Model(object):
+    """CPU model of the FP32-bulk Euler step (euler32.c) on a Domain of tetrahedra: float32
+    records and geometry, float32 face fluxes, float64 gathers / update / dt / sums."""
+
+    def __init__(self, d, U, quirk=0, gamma=1.4):
+        C, F = int(d.nbcells), int(d.nbfaces)
+        self.d, self.C, self.F, self.quirk, self.gamma = d, C, F, int(quirk), float(gamma)
+        self.rec = [np.zeros((C, 8), np.float32) for _ in range(2)]
+        self.geo = np.zeros((F, 4), np.float32)
+        lib().orc_euler32_pack(_I(C), _I(F), f64(np.ascontiguousarray(U, dtype=np.float64)), f64(d.cells.volume),
+                               f64(c3(d.faces.normal)), f64(d.faces.mesure), _p(self.rec[0], np.float32),
+                               _p(self.geo, np.float32))
+        self.fid = np.ascontiguousarray(d.cells.faceid, dtype=np.int32)
+        self.sorted = np.ascontiguousarray(np.sort(self.fid[:, :4], axis=1), dtype=np.int32)
+        self.fc = np.ascontiguousarray(d.faces.cellid, dtype=np.int32)
+        self.name = np.ascontiguousarray(d.faces.name, dtype=np.int32)
+        self.flx = np.zeros((F, 6), np.float32)
+        self.cons = np.zeros(5)
+        self.k = 0
+
+    @property
+    def state(self):
+        """[C][8] float32 records of the current state."""
+        return self.rec[self.k % 2]
+
+    def step(self, cfl, dt):
+        """One step with the given dt; returns the next dt."""
+        a, b = self.rec[self.k % 2], self.rec[(self.k + 1) % 2]
+        nxt = lib().orc_euler32_step(_I(self.C), _I(self.F), _p(a, np.float32), _p(b, np.float32),
+                                     _p(self.geo, np.float32), i32(self.fc), i32(self.name), i32(self.fid),
+                                     _I(self.fid.shape[1]), i32(self.sorted), _D(self.gamma), _D(cfl), _D(dt),
+                                     _I(self.quirk), _p(self.flx, np.float32), f64(self.cons))
+        self.k += 1
+        return nxt
 
 
 class OmpSwStep(object):

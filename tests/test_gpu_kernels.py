@@ -478,6 +478,48 @@ def test_euler_tiled_dt_ignores_stale_partials(torch, packed, oracle_lib):
     assert float(host(ring)[1]) == dt_next
 
 
+@pytest.mark.parametrize("quirk,T,n", [(0, 256, 8), (1, 128, 8), (0, 128, 11), (1, 256, 11)])
+def test_euler32_tiled_matches_model_and_fp64(torch, quirk, T, n, oracle_lib):
+    """FP32-bulk Euler step on face tiles: the float32 records and the dt ring equal its CPU
+    model (oracle/euler32.c) bit for bit after 200 steps; every field within 1e-5 relative L1 of
+    the float64 oracle; the float64 conservation sums agree with the model's and track the
+    float64 run's (relative drift printed)."""
+    from manapy_amd.device import DeviceMesh, Euler32
+    from manapy_amd.renumber import renumbered
+    d = renumbered(make_tet_domain(n, 0.08, 6))
+    m = DeviceMesh(d, lsq=False)
+    C, F = int(d.nbcells), int(d.nbfaces)
+    U0 = sod(d)
+    U = U0.copy()
+    G_ = np.zeros((6, F))
+    model = oracle_lib.Euler32Model(d, U0, quirk)
+    dt = float(np.min(oracle_lib.euler_dt(d, U0, 0.8, 1.4)))
+    e = Euler32(m, T)
+    assert e.lds_bytes() <= 64 * 1024
+    tU = dev(torch, U0)
+    e.pack(tU)
+    same(host(e.rec[0]), model.state, "packed float32 records")
+    ring = dev(torch, np.array([dt, 1e6, 1e6, 0.]))
+    dt_m = dt
+    vol = d.cells.volume
+    for s in range(200):
+        oracle_lib.euler_step(d, U, G_, 0.8, 1.4, quirk)
+        dt_m = model.step(0.8, dt_m)
+        e.step(1.4, 0.8, quirk, ring, s)
+    same(host(e.rec[e.k % 2]), model.state, "float32 records after 200 steps")
+    assert host(ring)[200 % 3] == dt_m
+    cons = host(e.cons)
+    assert np.allclose(cons, model.cons, rtol=1e-12, atol=1e-15)
+    cons64 = np.array([np.sum(U[k] * vol) for k in (0, 2, 3, 4, 5)])
+    print("FP32-bulk conservation sums relative to float64:", (cons - cons64) / np.abs(cons64))
+    out = dev(torch, np.zeros((6, C)))
+    e.unpack(out)
+    got = host(out)
+    for k in range(6):
+        err = np.sum(np.abs(got[k] - U[k]) * vol) / np.sum(np.abs(U[k]) * vol)
+        assert err <= 1e-5, (k, err)
+
+
 def test_sw_packed_60_steps(torch, sw_case, oracle_lib):
     """The packed path (one 64-byte record per cell, MUSCL values per face slot) gives the
     same bits as the reference step sequence."""
