@@ -365,6 +365,17 @@ int mfv_sw_batch_graph_tiled(const mfv_mesh *m, const mfv_sw_mixed *p, const mfv
 int mfv_graph_launch(void *graph, void *stream);
 int mfv_graph_destroy(void *graph);
 
+/* term_friction_SW (shallowater/fvm_utils.py:512-529): hu, hv /= 1 + time g eta^2 |u| / h^(4/3)
+ * (implicit Manning friction; the reference calls it with the step's dt as time). */
+int mfv_term_friction_sw(int C, const double *h, double *hu, double *hv, double grav, double eta, double time,
+                         void *stream);
+/* term_coriolis_SW (fvm_utils.py:498-503), its signs kept: corio_hu = f hu, corio_hv = -f hv, with
+ * f = f_c[i], or f_const in every cell when f_c is NULL. */
+int mfv_term_coriolis_sw(int C, const double *hu, const double *hv, double *corio_hu, double *corio_hv,
+                         const double *f_c, double f_const, void *stream);
+/* term_wind_SW (fvm_utils.py:531-544): Tx, Ty = 1.25 (0.75 + 0.067 |w|) 1e-3 |w| (wx, wy). */
+int mfv_term_wind_sw(int C, double *Tx, double *Ty, const double *wind_wx, const double *wind_wy, void *stream);
+
 /* Sets dt[0] = 1e6, the initial value of the reference's dt min-loops. */
 int mfv_dt_reset(double *dt, void *stream);
 
@@ -704,6 +715,32 @@ int mfv_krylov_graph(const mfv_ell_batch *e, const mfv_krylov_work *w, int n_ite
 
 /* y = A x (FP64) on the live slices. */
 int mfv_ell_spmv(const mfv_ell_batch *e, const double *x, double *y, int grid_slices, void *stream);
+
+/* ---------------- mesh topology on the host (ddm/domain.py:130-612) ------------------- */
+
+/* The connectivity of a Domain built in C++ (create_2dfaces / create_3dfaces, the face
+ * numbering of domain.py:314-315, create_cell_faceid, create_cellsOfFace,
+ * create_NeighborCellByFace, create_node_cellid; ddm_utils2d.py:12-97, 482-505,
+ * ddm_utils3d.py:766-810) with the reference's numbering and orderings: face rows = nodes in
+ * descending order, -1 padding, then the node count (2 in 2D), faces numbered in the
+ * lexicographic order of their rows; faces_cellid[f][0] = first (lowest) cell listing f;
+ * cellfid in faceid order; nodes_cellid and cells_cellnid ascending. Tables are padded
+ * [rows][width + 1] with the count last and -1 padding. Host arrays, malloc'ed; free them with
+ * mfv_domain_topology_free. elements: [C][ew] with the vertex count in column ew - 1. The
+ * vertex-neighbour table is built only when want_cellnid is set (cellnid_width = -1 otherwise). */
+typedef struct mfv_domain_topology {
+    int32_t nb_faces, max_cell_faces, max_face_nodes, node_cells_width, cellnid_width;
+    int32_t *faces_nodeid;  /* [F][max_face_nodes + 1] */
+    int32_t *cells_faceid;  /* [C][max_cell_faces + 1] */
+    int32_t *faces_cellid;  /* [F][2] */
+    int32_t *cells_cellfid; /* [C][max_cell_faces + 1] */
+    int32_t *nodes_cellid;  /* [N][node_cells_width + 1] */
+    int32_t *cells_cellnid; /* [C][cellnid_width + 1] */
+} mfv_domain_topology;
+
+int mfv_domain_topology_build(int dim, int C, int ew, const int32_t *elements, int N, int want_cellnid,
+                              mfv_domain_topology *out);
+void mfv_domain_topology_free(mfv_domain_topology *t);
 
 #ifdef __cplusplus
 }

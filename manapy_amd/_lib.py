@@ -94,6 +94,14 @@ class MfvEulerRez(ctypes.Structure):
     _fields_ = [(k, P) for k in ('rho', 'rhou', 'rhov', 'rhow', 'rhoE')]
 
 
+class MfvDomainTopology(ctypes.Structure):
+    """struct mfv_domain_topology: host tables of the mesh connectivity (malloc'ed)."""
+    _fields_ = [(k, ctypes.c_int32) for k in ('nb_faces', 'max_cell_faces', 'max_face_nodes', 'node_cells_width',
+                                              'cellnid_width')] + \
+               [(k, P) for k in ('faces_nodeid', 'cells_faceid', 'faces_cellid', 'cells_cellfid', 'nodes_cellid',
+                                 'cells_cellnid')]
+
+
 class MfvSwPacked(ctypes.Structure):
     """Mirror of ``struct mfv_sw_packed`` (include/mfv.h)."""
     _fields_ = [('nb_cells', ctypes.c_int32), ('max_cell_faces', ctypes.c_int32), ('cellrec', P), ('facerec', P),
@@ -178,6 +186,9 @@ _SIGNATURES = {
     'mfv_update_sw5': [MESH_P] + [P] * 18 + [c_double, c_double, c_double, P, P],
     'mfv_sw_step_fused': [MESH_P, P, P, P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_dt_reset': [P, P],
+    'mfv_term_friction_sw': [c_int, P, P, P, c_double, c_double, c_double, P],
+    'mfv_term_coriolis_sw': [c_int, P, P, P, P, P, c_double, P],
+    'mfv_term_wind_sw': [c_int, P, P, P, P, P],
     'mfv_sw_pack': [MESH_P, SWP_P, P, P, P, P, P, c_int, P],
     'mfv_sw_unpack': [MESH_P, SWP_P, P, P, P, P],
     'mfv_sw_recon': [MESH_P, SWP_P, c_int, P],
@@ -233,12 +244,14 @@ _SIGNATURES = {
     'mfv_sw_batch_graph': [MESH_P, SWM_P, SWB_P, c_double, c_double, c_int, c_int, c_int, ctypes.POINTER(c_void_p)],
     'mfv_graph_launch': [c_void_p, P],
     'mfv_graph_destroy': [c_void_p],
+    'mfv_domain_topology_build': [c_int, c_int, c_int, P, c_int, c_int, ctypes.POINTER(MfvDomainTopology)],
+    'mfv_domain_topology_free': [ctypes.POINTER(MfvDomainTopology)],
     'mfv_krylov_tick': [ELL_P, KW_P, c_int, c_int, c_int, P],
     'mfv_krylov_graph': [ELL_P, KW_P, c_int, c_int, c_int, ctypes.POINTER(c_void_p)],
     'mfv_ell_spmv': [ELL_P, P, P, c_int, P],
 }
 _RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p, 'mfv_sw_tiles_host_free': None,
-            'mfv_face_tiles_host_free': None, 'mfv_euler_tet_host_free': None}
+            'mfv_face_tiles_host_free': None, 'mfv_euler_tet_host_free': None, 'mfv_domain_topology_free': None}
 
 _lib = None
 

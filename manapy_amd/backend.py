@@ -240,6 +240,48 @@ def _update_sw(domain, h_c, hu_c, hv_c, hc_c, Z_c, rez_h, rez_hu, rez_hv, rez_hc
     A.finish()
 
 
+@HIPBackend.register('term_friction_SW')
+def _friction_sw(domain, h_c, hu_c, hv_c, grav, eta, time):
+    """solvers/shallowater/fvm_utils.py:512-529 (in place on hu_c, hv_c)."""
+    m = _m(domain)
+    A = _Args(m.device)
+    h = A.dev(h_c)
+    hu, hv = A.dev(hu_c, out=True), A.dev(hv_c, out=True)
+    _lib.call('mfv_term_friction_sw', int(h.numel()), h.data_ptr(), hu.data_ptr(), hv.data_ptr(), float(grav),
+              float(eta), float(time), stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('term_coriolis_SW')
+def _coriolis_sw(domain, hu_c, hv_c, corio_hu, corio_hv, f_c):
+    """solvers/shallowater/fvm_utils.py:498-503. ``f_c`` per cell, as the kernel reads it (a
+    scalar, as ShallowWaterSolver passes, is taken in every cell)."""
+    m = _m(domain)
+    A = _Args(m.device)
+    hu, hv = A.dev(hu_c), A.dev(hv_c)
+    cu, cv = A.dev(corio_hu, out=True), A.dev(corio_hv, out=True)
+    if np.ndim(f_c) == 0 and not hasattr(f_c, 'data_ptr'):
+        _lib.call('mfv_term_coriolis_sw', int(hu.numel()), hu.data_ptr(), hv.data_ptr(), cu.data_ptr(), cv.data_ptr(),
+                  None, float(f_c), stream_ptr())
+    else:
+        f = A.dev(f_c)
+        _lib.call('mfv_term_coriolis_sw', int(hu.numel()), hu.data_ptr(), hv.data_ptr(), cu.data_ptr(), cv.data_ptr(),
+                  f.data_ptr(), 0., stream_ptr())
+    A.finish()
+
+
+@HIPBackend.register('term_wind_SW')
+def _wind_sw(domain, Tx_wind, Ty_wind, wind_wx, wind_wy):
+    """solvers/shallowater/fvm_utils.py:531-544 (the kernel's own signature)."""
+    m = _m(domain)
+    A = _Args(m.device)
+    tx, ty = A.dev(Tx_wind, out=True), A.dev(Ty_wind, out=True)
+    wx, wy = A.dev(wind_wx), A.dev(wind_wy)
+    _lib.call('mfv_term_wind_sw', int(tx.numel()), tx.data_ptr(), ty.data_ptr(), wx.data_ptr(), wy.data_ptr(),
+              stream_ptr())
+    A.finish()
+
+
 def _no_halo(halofaces):
     if len(halofaces) != 0:
         raise NotImplementedError("halo faces: one GPU holds one subdomain (no MPI halo path)")

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libmfv.so')
 SOURCES = ['mfv_core.hip', 'mfv_sw.hip', 'mfv_sw_mixed.hip', 'mfv_srnh.hip', 'mfv_euler.hip', 'mfv_diamond.hip',
-           'mfv_krylov.hip', 'mfv_tiles.hip']
+           'mfv_krylov.hip', 'mfv_tiles.hip', 'mfv_domain.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('MFV_ARCH', 'gfx950')
 

@@ -355,3 +355,82 @@ extern "C" int mfv_update_sw5(const mfv_mesh *m, double *h_c, double *hu_c, doub
                        m->cell_volume);
     return check_launch("k_update_sw5");
 }
+
+// ---------------------------------------------------------------------------------------------
+// term_friction_SW, term_coriolis_SW, term_wind_SW (shallowater/fvm_utils.py:498-544)
+
+__global__ void __launch_bounds__(kBlock) k_term_friction_sw(int C, const double *__restrict__ h,
+                                                              double *__restrict__ hu, double *__restrict__ hv,
+                                                              double grav, double eta, double time)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const double ufric = hu[i] / h[i];
+    const double vfric = hv[i] / h[i];
+    const double hfric = h[i];
+    const double A = 1 + time * grav * (eta * eta) * (sqrt(ufric * ufric + vfric * vfric) / pow(hfric, 4. / 3.));
+    const double hutild = hu[i] / A;
+    const double hvtild = hv[i] / A;
+    hu[i] = hutild;
+    hv[i] = hvtild;
+}
+
+__global__ void __launch_bounds__(kBlock) k_term_coriolis_sw(int C, const double *__restrict__ hu,
+                                                              const double *__restrict__ hv,
+                                                              double *__restrict__ corio_hu,
+                                                              double *__restrict__ corio_hv,
+                                                              const double *__restrict__ f_c, double f_const)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const double f = f_c ? f_c[i] : f_const;
+    corio_hu[i] = f * hu[i];
+    corio_hv[i] = -f * hv[i];
+}
+
+__global__ void __launch_bounds__(kBlock) k_term_wind_sw(int C, double *__restrict__ Tx, double *__restrict__ Ty,
+                                                          const double *__restrict__ wx,
+                                                          const double *__restrict__ wy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    const double ro_a = 1.25;
+    const double u = wx[i], v = wy[i];
+    const double nor = sqrt(u * u + v * v);
+    const double Cw = ro_a * (0.75 + 0.067 * nor) * 1e-03;
+    Tx[i] = Cw * u * nor;
+    Ty[i] = Cw * v * nor;
+}
+
+extern "C" int mfv_term_friction_sw(int C, const double *h, double *hu, double *hv, double grav, double eta,
+                                    double time, void *stream)
+{
+    MFV_REQUIRE(C >= 0, "negative cell count");
+    if (C == 0) return MFV_OK;
+    MFV_REQUIRE(h && hu && hv, "NULL array");
+    hipLaunchKernelGGL(k_term_friction_sw, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), C, h, hu, hv, grav,
+                       eta, time);
+    return check_launch("k_term_friction_sw");
+}
+
+extern "C" int mfv_term_coriolis_sw(int C, const double *hu, const double *hv, double *corio_hu, double *corio_hv,
+                                    const double *f_c, double f_const, void *stream)
+{
+    MFV_REQUIRE(C >= 0, "negative cell count");
+    if (C == 0) return MFV_OK;
+    MFV_REQUIRE(hu && hv && corio_hu && corio_hv, "NULL array");
+    hipLaunchKernelGGL(k_term_coriolis_sw, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), C, hu, hv,
+                       corio_hu, corio_hv, f_c, f_const);
+    return check_launch("k_term_coriolis_sw");
+}
+
+extern "C" int mfv_term_wind_sw(int C, double *Tx, double *Ty, const double *wind_wx, const double *wind_wy,
+                                void *stream)
+{
+    MFV_REQUIRE(C >= 0, "negative cell count");
+    if (C == 0) return MFV_OK;
+    MFV_REQUIRE(Tx && Ty && wind_wx && wind_wy, "NULL array");
+    hipLaunchKernelGGL(k_term_wind_sw, dim3(grid_for(C)), dim3(kBlock), 0, as_stream(stream), C, Tx, Ty, wind_wx,
+                       wind_wy);
+    return check_launch("k_term_wind_sw");
+}

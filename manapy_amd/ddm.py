@@ -407,6 +407,41 @@ def normal_faces_of_cell(centerc, centerf, faceid, normal, maxcellfid):
     return nf
 
 
+def native_topology(nodeid, nbnodes, dim, want_cellnid):
+    """The connectivity from the library's host builder (``mfv_domain_topology_build``, C++):
+    the arrays of ``build_faces`` and of the node/vertex-neighbour tables, identical to the
+    NumPy construction. None when libmfv.so is not built."""
+    import ctypes
+    try:
+        from . import _lib
+        _lib.load()
+    except Exception:
+        return None
+    el = np.ascontiguousarray(nodeid, dtype=I32)
+    C = len(el)
+    t = _lib.MfvDomainTopology()
+    _lib.call('mfv_domain_topology_build', int(dim), C, int(el.shape[1]), el.ctypes.data_as(ctypes.c_void_p),
+              int(nbnodes), int(bool(want_cellnid)), ctypes.byref(t))
+    try:
+        F = int(t.nb_faces)
+
+        def grab(ptr, rows, cols):
+            if rows == 0 or cols <= 0:
+                return np.zeros((rows, max(cols, 0)), dtype=I32)
+            a = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_int32)), shape=(rows, cols))
+            return a.copy()
+        out = {'faces_nodeid': grab(t.faces_nodeid, F, t.max_face_nodes + 1),
+               'faceid': grab(t.cells_faceid, C, t.max_cell_faces + 1),
+               'cellid': grab(t.faces_cellid, F, 2),
+               'cellfid': grab(t.cells_cellfid, C, t.max_cell_faces + 1),
+               'node_cellid': grab(t.nodes_cellid, int(nbnodes), t.node_cells_width + 1),
+               'cellnid': grab(t.cells_cellnid, C, t.cellnid_width + 1) if t.cellnid_width >= 0 else None,
+               'maxcellfid': int(t.max_cell_faces), 'maxfacenid': int(t.max_face_nodes)}
+    finally:
+        _lib.call('mfv_domain_topology_free', ctypes.byref(t))
+    return out
+
+
 # ---------------------------------------------------------------------------------------------
 
 class Domain(object):
@@ -419,6 +454,8 @@ class Domain(object):
 
     _parameters = [
         ('backend', 'str', "hip", False, 'kernel backend: hip (device) only'),
+        ('topology', 'str', "auto", False, "connectivity builder: 'native' (C++, libmfv.so), 'numpy', or 'auto' "
+                                          "(native when the library is built); the arrays are identical"),
         ('multithreading', 'str', 'single', None, 'accepted for API compatibility'),
         ('signature', 'bool', True, False, 'accepted for API compatibility'),
         ('cache', 'bool', False, False, 'accepted for API compatibility'),
@@ -447,6 +484,9 @@ class Domain(object):
         self._rank = 0
         self._backend_name = get('backend')
         self._backend = None
+        self._topology_mode = str(get('topology'))
+        if self._topology_mode not in ('auto', 'native', 'numpy'):
+            raise ValueError("topology must be 'auto', 'native' or 'numpy'")
         if isinstance(mesh, str):
             from .gmsh import read_gmsh
             elements, nodes = read_gmsh(mesh, dim)
@@ -518,10 +558,29 @@ class Domain(object):
         self._cells.center = c
         self._cells.volume = v
 
+    def _native(self):
+        """The C++ connectivity (faces, node cells, 2D vertex neighbours), built once."""
+        if self._topology_mode == 'numpy':
+            return None
+        if getattr(self, '_topo', None) is None:
+            self._topo = native_topology(self._cells.nodeid, int(self._nbnodes), int(self._dim),
+                                         want_cellnid=(int(self._dim) == 2))
+            if self._topo is None and self._topology_mode == 'native':
+                raise RuntimeError("topology='native' needs libmfv.so (run __graft_entry__.build())")
+        return self._topo
+
     def _make_neighbors(self):
         """``create_node_cellid`` (``ddm_utils2d.py:55-97``): node -> cells
         (ascending) now; cell -> vertex neighbours (ascending) in 2D now, in 3D
         on first access."""
+        topo = self._native()
+        if topo is not None:
+            self._nodes.cellid = topo['node_cellid']
+            if self._dim == 2:
+                self._cells.cellnid = topo['cellnid']
+            else:
+                self._cells.lazy('cellnid', self._build_cellnid)
+            return
         nodeid = self._cells.nodeid
         C = len(nodeid)
         N = int(self._nbnodes)
@@ -531,7 +590,6 @@ class Domain(object):
         cells_rep = np.repeat(np.arange(C, dtype=np.int64), maxnv).reshape(C, maxnv)[mask]
         nodes_rep = nodeid[:, :maxnv][mask].astype(np.int64)
         order = np.argsort(nodes_rep, kind='stable')
-        self._node_cells_sorted = (nodes_rep[order], cells_rep[order])
         self._nodes.cellid = _pad_rows(nodes_rep[order], cells_rep[order], N)
         if self._dim == 2:
             self._build_cellnid()
@@ -571,8 +629,14 @@ class Domain(object):
             self._nbOfQuad = int(np.sum(nodeid[:, -1] == 8))
 
     def _create_faces_cons(self):
-        fn, faceid, cellid, cellfid, maxcellfid, maxfacenid = build_faces(
-            self._cells.nodeid, int(self._nbnodes), int(self._dim))
+        topo = self._native()
+        if topo is not None:
+            fn, faceid, cellid, cellfid = topo['faces_nodeid'], topo['faceid'], topo['cellid'], topo['cellfid']
+            maxcellfid, maxfacenid = topo['maxcellfid'], topo['maxfacenid']
+            self._topo = None  # the tables now live in the containers
+        else:
+            fn, faceid, cellid, cellfid, maxcellfid, maxfacenid = build_faces(
+                self._cells.nodeid, int(self._nbnodes), int(self._dim))
         self._faces.nodeid = fn
         self._cells.faceid = faceid
         self._faces.cellid = cellid

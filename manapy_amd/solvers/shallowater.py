@@ -74,11 +74,15 @@ class ShallowWaterSolver(object):
         if self.Dxx != 0 or self.Dyy != 0:
             raise NotImplementedError("the reference's shallow-water diffusion path is broken (self.diffusion "
                                       "unset, system.py:146-147); not provided")
-        if self.Mann != 0 or self.fc != 0 or self.wind:
-            raise NotImplementedError("friction, Coriolis and wind terms are not on the accelerated path")
         self.flux = str(get("flux")).lower()
         if self.flux not in ('srnh', 'rusanov'):
             raise ValueError("flux must be 'srnh' or 'rusanov'")
+        if self.wind:
+            raise NotImplementedError("wind: the reference calls term_wind_SW with arguments that do not match its "
+                                      "signature (system.py:245-246) and passes zero wind to update_SW "
+                                      "(system.py:237-242); the kernel itself is bound by HIPBackend.compile")
+        if (self.Mann != 0 or self.fc != 0) and self.flux != 'srnh':
+            raise NotImplementedError("friction and Coriolis terms run with the reference's scheme (flux='srnh')")
         self.quirk = int(bool(get("quirk")))
         import torch
         m = self.domain.device_mesh
@@ -205,12 +209,29 @@ class ShallowWaterSolver(object):
                   self.Z.cell.data_ptr(), self._grad.data_ptr(), self._zgrad.data_ptr(), self.grav, self.order,
                   self._rez.data_ptr(), self._src.data_ptr(), s)
 
+    def update_term_friction(self):
+        """``system.py:230-231`` -> term_friction_SW with the current dt."""
+        _lib.call('mfv_term_friction_sw', int(self.domain.nbcells), self.h.cell.data_ptr(), self.hu.cell.data_ptr(),
+                  self.hv.cell.data_ptr(), self.grav, float(self.Mann), float(self.dt), stream_ptr())
+
+    def update_term_coriolis(self):
+        """``system.py:233-234`` -> term_coriolis_SW; the reference passes the scalar fc where
+        the kernel reads f_c[i], so fc is taken in every cell."""
+        _lib.call('mfv_term_coriolis_sw', int(self.domain.nbcells), self.hu.cell.data_ptr(), self.hv.cell.data_ptr(),
+                  self.hu.coriolis.data_ptr(), self.hv.coriolis.data_ptr(), None, float(self.fc), stream_ptr())
+
     def compute_fluxes(self):
-        """``system.py:248-281`` (halo exchange is empty on one subdomain; the second dt
-        evaluation of the reference, Q4, gives the same value and is skipped)."""
+        """``system.py:248-281`` (halo exchange is empty on one subdomain). With friction, the
+        reference's second stepper call (Q4) follows it and its dt is the one the update uses;
+        without, that call gives the driver's dt again and is skipped."""
+        if self.Mann != 0:
+            self.update_term_friction()
+            self.stepper()
         self.update_ghost_values()
         self.explicit_convective()
         self.update_term_source()
+        if self.fc != 0:
+            self.update_term_coriolis()
 
     def compute_new_val(self):
         """``system.py:237-242`` -> update_SW."""
@@ -264,6 +285,14 @@ class ShallowWaterSolver(object):
         import torch
         if nsteps <= 0:
             return 0.
+        if self.Mann != 0 or self.fc != 0:  # friction takes the dt on the host, as the reference does
+            t = 0.
+            for _ in range(nsteps):
+                self.stepper()
+                self.compute_fluxes()
+                self.compute_new_val()
+                t += float(self.dt)
+            return t
         dts = torch.zeros(nsteps, dtype=torch.float64, device=self._m.device)
         for k in range(nsteps):
             dk = dts[k:k + 1]

@@ -346,13 +346,31 @@ def srnh_gradients(d, S, G):
     return hx, hy, hcx, hcy, hcpsi
 
 
-def srnh_step(d, S, grav, cfl, order, kinds=SRNH_WALLS, quirk=True):
+def term_friction_sw(h, hu, hv, grav, eta, time):
+    """term_friction_SW in place on hu, hv."""
+    lib().orc_term_friction_sw(f64(h), f64(hu), f64(hv), _D(grav), _D(eta), _D(time), _I(len(h)))
+
+
+def term_coriolis_sw(hu, hv, corio_hu, corio_hv, f_c):
+    lib().orc_term_coriolis_sw(f64(hu), f64(hv), f64(corio_hu), f64(corio_hv), f64(f_c), _I(len(hu)))
+
+
+def term_wind_sw(Tx, Ty, wx, wy):
+    lib().orc_term_wind_sw(f64(Tx), f64(Ty), f64(wx), f64(wy), _I(len(Tx)))
+
+
+def srnh_step(d, S, grav, cfl, order, kinds=SRNH_WALLS, quirk=True, Mann=0., fc=0.):
     """ShallowWaterSolver: stepper -> compute_fluxes -> compute_new_val
-    (solvers/shallowater/system.py:186-281) with Dxx = Dyy = 0, no friction, Coriolis or
-    wind, in place on S [5][C] (h, hu, hv, hc, Z). Returns dt."""
+    (solvers/shallowater/system.py:186-281) with Dxx = Dyy = 0, in place on S [5][C] (h, hu,
+    hv, hc, Z). Mann != 0: friction with the driver's dt, then compute_fluxes' second stepper
+    call, whose dt the update uses; fc != 0: term_coriolis_SW with f_c = fc in every cell
+    (the reference passes the scalar, which the kernel indexes). Returns the dt of the update."""
     C = int(d.nbcells)
     dt = time_step_sw(d, np.ascontiguousarray(S[0]), np.ascontiguousarray(S[1]), np.ascontiguousarray(S[2]), cfl,
                       grav)
+    if Mann != 0:
+        term_friction_sw(S[0], S[1], S[2], grav, Mann, dt)
+        dt = time_step_sw(d, S[0], S[1], S[2], cfl, grav)
     G = srnh_ghosts(d, S, kinds)
     if order == 2:
         grads = srnh_gradients(d, S, G)
@@ -360,7 +378,10 @@ def srnh_step(d, S, grav, cfl, order, kinds=SRNH_WALLS, quirk=True):
         grads = tuple(np.zeros(C) for _ in range(5))
     rez = srnh_convective(d, S, G, grads, grav, order)
     src = srnh_source(d, S, G, grav, quirk)
-    srnh_update(d, S, rez, src, dt)
+    corio_hu, corio_hv = np.zeros(C), np.zeros(C)
+    if fc != 0:
+        term_coriolis_sw(S[1], S[2], corio_hu, corio_hv, np.full(C, float(fc)))
+    srnh_update(d, S, rez, src, dt, corio_hu=corio_hu, corio_hv=corio_hv)
     return dt
 
 

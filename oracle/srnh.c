@@ -278,3 +278,42 @@ void orc_update_sw5(double *state, const double *rez, const double *src, const d
         Z[i] += dtime * (r[4][i] + s[4][i]) / vol[i];
     }
 }
+
+/* term_friction_SW (shallowater/fvm_utils.py:512-529): implicit Manning friction of the cell
+ * momenta, hu /= A, hv /= A with A = 1 + time g eta^2 |u| / h^(4/3). */
+void orc_term_friction_sw(const double *h, double *hu, double *hv, double grav, double eta, double time, int C)
+{
+    for (int i = 0; i < C; i++) {
+        double ufric = hu[i] / h[i];
+        double vfric = hv[i] / h[i];
+        double hfric = h[i];
+        double A = 1 + time * grav * (eta * eta) * (sqrt(ufric * ufric + vfric * vfric) / pow(hfric, 4. / 3.));
+        double hutild = hu[i] / A;
+        double hvtild = hv[i] / A;
+        hu[i] = hutild;
+        hv[i] = hvtild;
+    }
+}
+
+/* term_coriolis_SW (fvm_utils.py:498-503), with its signs: corio_hu = f hu, corio_hv = -f hv. */
+void orc_term_coriolis_sw(const double *hu, const double *hv, double *corio_hu, double *corio_hv, const double *f_c,
+                          int C)
+{
+    for (int i = 0; i < C; i++) {
+        corio_hu[i] = f_c[i] * hu[i];
+        corio_hv[i] = -f_c[i] * hv[i];
+    }
+}
+
+/* term_wind_SW (fvm_utils.py:531-544): wind stress C |w| w with C = rho_air (0.75 + 0.067 |w|) 1e-3. */
+void orc_term_wind_sw(double *Tx, double *Ty, const double *wx, const double *wy, int C)
+{
+    const double ro_a = 1.25;
+    for (int i = 0; i < C; i++) {
+        double u = wx[i], v = wy[i];
+        double nor = sqrt(u * u + v * v);
+        double Cw = ro_a * (0.75 + 0.067 * nor) * 1e-03;
+        Tx[i] = Cw * u * nor;
+        Ty[i] = Cw * v * nor;
+    }
+}

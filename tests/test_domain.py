@@ -167,3 +167,30 @@ def test_order_tables(dom2d):
         bnd = sorted(f for f in faces if d.faces.name[f] != 0)
         assert split == inner + bnd
         assert [faces[L['order_sorted'][r, i]] for r in range(nf)] == sorted(faces)
+
+
+@pytest.mark.parametrize("case", ["tri", "hybrid", "tet", "cube"])
+def test_native_topology_equals_numpy(case):
+    """The C++ connectivity builder (mfv_domain_topology_build) gives every array of the NumPy
+    construction: face numbering and node rows, cells.faceid, faces.cellid, cells.cellfid,
+    nodes.cellid, cells.cellnid, and everything derived from them."""
+    import os
+    from manapy_amd.ddm import Domain
+    from manapy_amd.gmsh import read_gmsh
+    from manapy_amd.meshgen import jittered_triangles, kuhn_tetrahedra
+    here = os.path.dirname(os.path.abspath(__file__))
+    mesh, dim = {"tri": (lambda: jittered_triangles(13, 9, 1., 1., 0.1, 2), 2),
+                 "hybrid": (lambda: read_gmsh(os.path.join(here, 'golden', 'meshes', 'carre_hybrid.msh.gz'), 2), 2),
+                 "tet": (lambda: kuhn_tetrahedra(4, 1.0, 0.08, 1), 3),
+                 "cube": (lambda: read_gmsh(os.path.join(here, 'golden', 'meshes', 'cube.msh.gz'), 3), 3)}[case]
+    el, nd = mesh()
+    a = Domain(dim, mesh=(el, nd), topology='native')
+    b = Domain(dim, mesh=(el, nd), topology='numpy')
+    for cont in ('cells', 'faces', 'nodes'):
+        ca, cb = getattr(a, cont), getattr(b, cont)
+        for k in sorted(set(ca.keys()) | set(cb.keys())):
+            x, y = getattr(ca, k), getattr(cb, k)
+            if isinstance(x, np.ndarray):
+                assert x.dtype == y.dtype and np.array_equal(x, y), (cont, k)
+    for k in ('innerfaces', 'boundaryfaces'):
+        assert np.array_equal(getattr(a, k), getattr(b, k))

@@ -159,3 +159,70 @@ def test_backend_compile_srnh_reference_signatures(oracle_lib):
     Sref = S.copy()
     oracle_lib.srnh_update(d, Sref, ref, sref, 0.01)
     assert np.array_equal(np.array(S2), Sref)
+
+
+def test_friction_coriolis_wind_kernels(oracle_lib):
+    """term_friction_SW (pow on the device and the host may differ in the last bit: 1e-15 of
+    the field), term_coriolis_SW and term_wind_SW (bit for bit) through HIPBackend.compile with
+    the kernels' own signatures."""
+    d, S = _case(7)
+    C = int(d.nbcells)
+    rng = np.random.default_rng(2)
+    h, hu, hv = S[0] + 0.1 * rng.random(C), rng.standard_normal(C), rng.standard_normal(C)
+    be = d.backend
+    hu_g, hv_g = hu.copy(), hv.copy()
+    be.compile('term_friction_SW')(h, hu_g, hv_g, G, 0.03, 0.7)
+    hu_r, hv_r = hu.copy(), hv.copy()
+    oracle_lib.term_friction_sw(h, hu_r, hv_r, G, 0.03, 0.7)
+    assert _close(hu_g, hu_r, 1e-15) and _close(hv_g, hv_r, 1e-15)
+    assert not np.allclose(hu_r, hu)
+    f = rng.random(C)
+    cu, cv, cu_r, cv_r = (np.zeros(C) for _ in range(4))
+    be.compile('term_coriolis_SW')(hu, hv, cu, cv, f)
+    oracle_lib.term_coriolis_sw(hu, hv, cu_r, cv_r, f)
+    assert np.array_equal(cu, cu_r) and np.array_equal(cv, cv_r)
+    be.compile('term_coriolis_SW')(hu, hv, cu, cv, 0.3)
+    oracle_lib.term_coriolis_sw(hu, hv, cu_r, cv_r, np.full(C, 0.3))
+    assert np.array_equal(cu, cu_r) and np.array_equal(cv, cv_r)
+    wx, wy = rng.standard_normal(C), rng.standard_normal(C)
+    tx, ty, tx_r, ty_r = (np.zeros(C) for _ in range(4))
+    be.compile('term_wind_SW')(tx, ty, wx, wy)
+    oracle_lib.term_wind_sw(tx_r, ty_r, wx, wy)
+    assert np.array_equal(tx, tx_r) and np.array_equal(ty, ty_r)
+
+
+def test_srnh_solver_with_friction_and_coriolis(oracle_lib):
+    """ShallowWaterSolver with Mann and fc: friction with the driver's dt, the second stepper
+    call of compute_fluxes (whose dt the update uses), Coriolis into update_SW; 15 steps of the
+    reference sequence and of run() against the oracle's sequence."""
+    import torch
+    from manapy_amd import Struct, Variable, ShallowWaterSolver
+    d, S0 = _case(9)
+    walls = {"in": "nonslip", "out": "nonslip", "upper": "nonslip", "bottom": "nonslip"}
+    res = {}
+    for mode in ("sequence", "run"):
+        h, hu, hv = Variable(domain=d), Variable(domain=d, BC=walls), Variable(domain=d, BC=walls)
+        hc, Z = Variable(domain=d), Variable(domain=d)
+        for var, k in ((h, 0), (hc, 3), (Z, 4)):
+            var.cell[:] = torch.as_tensor(S0[k])
+        sol = ShallowWaterSolver(h=h, hvel=(hu, hv), hc=hc, Z=Z,
+                                 conf=Struct(order=2, cfl=0.8, flux='srnh', Mann=0.03, fc=1e-2))
+        if mode == "sequence":
+            t = 0.
+            for _ in range(15):
+                sol.stepper()
+                sol.compute_fluxes()
+                t += sol.dt
+                sol.compute_new_val()
+        else:
+            t = sol.run(15)
+        res[mode] = ([v.cell.cpu().numpy() for v in (h, hu, hv, hc, Z)], t)
+    S = S0.copy()
+    t_ref = 0.
+    for _ in range(15):
+        t_ref += oracle_lib.srnh_step(d, S, G, 0.8, 2, Mann=0.03, fc=1e-2)
+    for mode, (fields, t) in res.items():
+        for k in range(5):
+            assert _close(fields[k], S[k], 1e-12), (mode, k)
+        assert abs(t - t_ref) <= 1e-12 * t_ref, mode
+    assert np.abs(S[2]).max() > 0.

@@ -186,3 +186,30 @@ def test_srnh_step_conserves_mass_with_walls_interior(oracle_lib, order):
                                   Gh[3][f], S[4][L], Gh[4][f], d.faces.normal[f], d.faces.mesure[f], G)
         wall += fl[0]
     assert abs(np.sum(rez[0]) + wall) <= 1e-12 * max(1., abs(wall))
+
+
+def test_term_friction_closed_form(oracle_lib):
+    """term_friction_SW (fvm_utils.py:512-529): with g time eta^2 = 1, h = 8 (h^(4/3) = 16) and
+    |u| = 3, A = 1 + 3/16, so the momenta shrink by 16/19 and keep their direction; a cell at
+    rest is unchanged."""
+    h = np.array([8., 8., 2.])
+    hu = np.array([1.8 * 8, -3. * 8, 0.])
+    hv = np.array([2.4 * 8, 0., 0.])
+    hu0, hv0 = hu.copy(), hv.copy()
+    oracle_lib.term_friction_sw(h, hu, hv, 2.0, 0.5, 2.0)
+    assert np.allclose(hu[:2], hu0[:2] * 16. / 19., rtol=1e-14)
+    assert np.allclose(hv[:2], hv0[:2] * 16. / 19., rtol=1e-14)
+    assert hu[2] == 0. and hv[2] == 0.
+
+
+def test_term_coriolis_and_wind_closed_forms(oracle_lib):
+    """term_coriolis_SW keeps the reference's signs (corio_hu = f hu, corio_hv = -f hv);
+    term_wind_SW gives 1.25 (0.75 + 0.067 |w|) 1e-3 |w| w: for w = (3, 4), 0.0203438 and 0.027125."""
+    hu, hv, f = np.array([1., -2.]), np.array([3., 4.]), np.array([0.5, 0.25])
+    cu, cv = np.zeros(2), np.zeros(2)
+    oracle_lib.term_coriolis_sw(hu, hv, cu, cv, f)
+    assert np.array_equal(cu, [0.5, -0.5]) and np.array_equal(cv, [-1.5, -1.0])
+    tx, ty = np.zeros(2), np.zeros(2)
+    oracle_lib.term_wind_sw(tx, ty, np.array([3., 0.]), np.array([4., 0.]))
+    c = 1.25 * (0.75 + 0.067 * 5.) * 1e-3
+    assert np.allclose([tx[0], ty[0]], [c * 15., c * 20.], rtol=1e-15) and tx[1] == 0. and ty[1] == 0.
