@@ -318,6 +318,18 @@ void mfv_sw_tiles_host_free(mfv_sw_tiles_host *t);
 int mfv_sw_step_packed_tiled(const mfv_mesh *m, const mfv_sw_packed *p, const mfv_sw_tiles *t,
                              double *cellrec_out, double grav, double cfl, int order, double *dt_ring,
                              int step, void *stream);
+
+/* mfv_sw_step_packed_tiled with no device allocation of its own. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Its scratch (the per-wave dt partials) is a
+ * static device array of MFV_NOALLOC_MAX_WAVES doubles (8 MiB), one per GPU, held until the
+ * process ends, so it takes at most 64 * MFV_NOALLOC_MAX_WAVES cells (larger meshes are
+ * rejected with MFV_EINVAL). Same bits as mfv_sw_step_packed_tiled. Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams, other threads) race. */
+#define MFV_NOALLOC_MAX_WAVES (1 << 20)
+int mfv_sw_step_packed_tiled_noalloc(const mfv_mesh *m, const mfv_sw_packed *p, const mfv_sw_tiles *t,
+                                     double *cellrec_out, double grav, double cfl, int order, double *dt_ring,
+                                     int step, void *stream);
 /* One fused mixed step (reconstruction, fluxes, update, next dt) from p->state into
  * state_out (a second [C][4] buffer); the face records stay in LDS. Gives the same bits as
  * mfv_sw_recon_mixed followed by mfv_sw_flux_mixed. No counterpart call in the reference:

@@ -234,6 +234,7 @@ _SIGNATURES = {
     'mfv_sw_batch_step': [MESH_P, SWM_P, SWB_P, c_double, c_double, c_int, c_int, c_int, P],
     'mfv_sw_step_mixed_tiled': [MESH_P, SWM_P, SWT_P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_sw_step_packed_tiled': [MESH_P, SWP_P, SWT_P, P, c_double, c_double, c_int, P, c_int, P],
+    'mfv_sw_step_packed_tiled_noalloc': [MESH_P, SWP_P, SWT_P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_sw_tiles_build': [c_int, c_int, P, c_int, P, P, c_int, P, P, P, c_int, P, P, c_int, c_int,
                            ctypes.POINTER(MfvSwTilesHost)],
     'mfv_sw_tiles_host_free': [ctypes.POINTER(MfvSwTilesHost)],

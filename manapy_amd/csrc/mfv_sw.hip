@@ -9,6 +9,8 @@
 //
 // State layout: [3][C] = h, hu, hv. Gradient layout: [9][C] = (x, y, psi) of eta = h+Z,
 // hu, hv. Z gradient: [2][C], unlimited, computed once per mesh.
+#include <mutex>
+
 #include "mfv_common.hpp"
 
 using namespace mfv;
@@ -1132,6 +1134,85 @@ extern "C" int mfv_sw_step_packed_tiled(const mfv_mesh *m, const mfv_sw_packed *
     const size_t nw = ((size_t)m->nb_cells + 63) / 64;
     double *part = dt_scratch(nw, as_stream(stream));
     if (!part) return MFV_ENOMEM;
+    const double om = (double)(order - 1);
+    hipStream_t st = as_stream(stream);
+    const bool tri = m->max_cell_faces == 3;
+    switch (t->tile_size) {
+    case 128:
+        if (tri) launch_packed_tiled<3, 128>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        else launch_packed_tiled<4, 128>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        break;
+    case 256:
+        if (tri) launch_packed_tiled<3, 256>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        else launch_packed_tiled<4, 256>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        break;
+    default:
+        if (tri) launch_packed_tiled<3, 512>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        else launch_packed_tiled<4, 512>(m, p, t, cellrec_out, om, grav, cfl, r, part, st);
+        break;
+    }
+    if (int rc = check_launch("k_sw_step_packed_tiled")) return rc;
+    return dt_finish(part, nw, r.next, st);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The fused float64 step without device allocation. Its only scratch is the per-wave dt
+// partials; here they live in a static device array (one copy per GPU, for the life of the
+// process) instead of the per-stream buffer that dt_scratch grows with hipMalloc. Host code
+// never passes the array's name to a kernel (on the host it only names the symbol): the
+// device address comes from hipGetSymbolAddress, once per device, and is kept per device.
+// Every wave with cells overwrites its slot before the fold reads it, so nothing needs
+// clearing between calls; calls that overlap on one GPU (several streams or threads) share
+// the array and race.
+static __device__ __align__(256) double scratch[MFV_NOALLOC_MAX_WAVES];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[64];
+
+static double *noalloc_partials()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("no current device");
+        return nullptr;
+    }
+    if (dev < 0 || dev >= 64) {
+        set_error("device ordinal %d out of range", dev);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_dev[dev] = static_cast<double *>(ptr);
+    }
+    return g_scratch_dev[dev];
+}
+
+extern "C" int mfv_sw_step_packed_tiled_noalloc(const mfv_mesh *m, const mfv_sw_packed *p, const mfv_sw_tiles *t,
+                                                double *cellrec_out, double grav, double cfl, int order,
+                                                double *dt_ring_p, int step, void *stream)
+{
+    MFV_REQUIRE(m != nullptr, "mesh descriptor is NULL");
+    MFV_REQUIRE(m->nb_cells >= 0 && ((size_t)m->nb_cells + 63) / 64 <= (size_t)MFV_NOALLOC_MAX_WAVES,
+                "%d cells need %zu dt partials: the allocation-free step holds at most %d (%d cells); use "
+                "mfv_sw_step_packed_tiled",
+                m->nb_cells, ((size_t)m->nb_cells + 63) / 64, MFV_NOALLOC_MAX_WAVES, 64 * MFV_NOALLOC_MAX_WAVES);
+    if (int rc = validate_mesh(m, 1, 0, 0)) return rc;
+    if (int rc = check_packed(m, p)) return rc;
+    if (int rc = check_tiles_packed(m, t)) return rc;
+    MFV_REQUIRE(m->dim == 2, "shallow water needs a 2D mesh");
+    MFV_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+    MFV_REQUIRE(dt_ring_p && cellrec_out && cellrec_out != p->cellrec, "cellrec_out must be a second record buffer");
+    if (m->nb_cells == 0) return MFV_OK;
+    double *part = noalloc_partials();
+    if (!part) return MFV_EHIP;
+    DtRing r = dt_ring(dt_ring_p, step);
+    const size_t nw = ((size_t)m->nb_cells + 63) / 64;
     const double om = (double)(order - 1);
     hipStream_t st = as_stream(stream);
     const bool tri = m->max_cell_faces == 3;

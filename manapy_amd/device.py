@@ -266,16 +266,18 @@ class SwPacked(object):
         h = self.tiles(T).host
         return ((h['max_ext'] + 1) // 2 * 2) * 40 + h['max_items'] * h['MF'] * 24
 
-    def step_tiled(self, dmesh, grav, cfl, order, ring, step, T=128, stream=None):
+    def step_tiled(self, dmesh, grav, cfl, order, ring, step, T=128, stream=None, noalloc=False):
         """One fused float64 step: reads the current record buffer, writes the other, swaps
         them. The second buffer takes a copy of the records (static fields included) after
-        pack(); the static fields change only there."""
+        pack(); the static fields change only there. noalloc: the entry point that allocates
+        nothing (mfv_sw_step_packed_tiled_noalloc; same bits)."""
         if self.cellrec2 is None or self._rec_synced is not self.cellrec:
             if self.cellrec2 is None:
                 import torch
                 self.cellrec2 = torch.empty_like(self.cellrec)
             self.cellrec2.copy_(self.cellrec)
-        _lib.call('mfv_sw_step_packed_tiled', dmesh.ptr, self.ptr, self.tiles(T).ptr, self.cellrec2.data_ptr(),
+        _lib.call('mfv_sw_step_packed_tiled_noalloc' if noalloc else 'mfv_sw_step_packed_tiled', dmesh.ptr, self.ptr,
+                  self.tiles(T).ptr, self.cellrec2.data_ptr(),
                   float(grav), float(cfl), int(order), ring.data_ptr(), int(step),
                   stream if stream is not None else stream_ptr())
         self.cellrec, self.cellrec2 = self.cellrec2, self.cellrec

@@ -74,3 +74,15 @@ def test_every_mirrored_struct_matches_the_header(tmp_path):
         assert int(got[cname]) == ctypes.sizeof(cls), cname
         for f in cls._fields_:
             assert int(got['%s.%s' % (cname, f[0])]) == getattr(cls, f[0]).offset, (cname, f[0])
+
+
+def test_noalloc_step_rejects_meshes_beyond_its_static_scratch():
+    """The allocation-free step checks the mesh size against its static partials array before
+    touching the device."""
+    import ctypes
+    lib = _lib.load()
+    m = _lib.MfvMesh()
+    m.nb_cells = 64 * (1 << 20) + 1
+    rc = lib.mfv_sw_step_packed_tiled_noalloc(ctypes.byref(m), None, None, None, 9.81, 0.8, 2, None, 0, None)
+    assert rc == -1
+    assert 'allocation-free' in lib.mfv_last_error().decode()
