@@ -754,6 +754,39 @@ int mfv_domain_topology_build(int dim, int C, int ew, const int32_t *elements, i
                               mfv_domain_topology *out);
 void mfv_domain_topology_free(mfv_domain_topology *t);
 
+/* The 2D geometry of a Domain from its topology (Compute_2dcentervolumeOfCell,
+ * create_info_2dfaces, create_NormalFacesOfCell, face_info_2d ghost mirrors and per-node ghost
+ * lists, create_node_ghostid, variables_2d, face_gradient_info_2d, dist_ortho_function;
+ * ddm_utils2d.py:99-223, 433-479, 507-744), the same floating-point expressions in the same
+ * order as the reference's loops, on host threads. Boundary faces are those named 1-4, in
+ * ascending order. Host arrays, malloc'ed; free them with mfv_domain_geometry_2d_free. */
+typedef struct mfv_domain_geometry_2d {
+    int32_t ghost_width, ghostnid_width;
+    double *cell_center;        /* [C][3] */
+    double *cell_volume;        /* [C] */
+    int32_t *face_name;         /* [F] */
+    double *face_normal;        /* [F][3] */
+    double *face_mesure;        /* [F] */
+    double *face_center;        /* [F][3] */
+    double *cell_nf;            /* [C][max_cell_faces][3] */
+    double *face_ghostcenter;   /* [F][3] (x, y, gamma; -1 off the boundary) */
+    double *dist_ortho;         /* [F] */
+    double *airDiamond, *param1, *param2, *param3, *param4; /* [F] */
+    double *f_1, *f_2, *f_3, *f_4; /* [F][2] */
+    double *lambda_x, *lambda_y, *R_x, *R_y; /* [N] */
+    int32_t *number;            /* [N] */
+    double *node_ghostcenter;   /* [N][ghost_width][5] */
+    double *node_ghostfaceinfo; /* [N][ghost_width][4] */
+    int32_t *node_ghostid;      /* [N][ghost_width + 1] */
+    int32_t *cell_ghostnid;     /* [C][ghostnid_width + 1] */
+} mfv_domain_geometry_2d;
+
+int mfv_domain_geometry_2d_build(int C, int ew, const int32_t *nodeid, int N, const double *vertex, int F,
+                                 const int32_t *faces_nodeid, int mf1, const int32_t *cells_faceid,
+                                 const int32_t *faces_cellid, const int32_t *node_cellid, int ncw1,
+                                 mfv_domain_geometry_2d *out);
+void mfv_domain_geometry_2d_free(mfv_domain_geometry_2d *g);
+
 #ifdef __cplusplus
 }
 #endif

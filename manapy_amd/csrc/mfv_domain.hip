@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "mfv_common.hpp"
@@ -272,5 +273,315 @@ extern "C" int mfv_domain_topology_build(int dim, int C, int ew, const int32_t *
         set_error("out of host memory");
         return MFV_ENOMEM;
     }
+    return MFV_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// 2D geometry, ghost mirrors and diamond coefficients (ddm_utils2d.py:122-223, 433-479, 507-705,
+// 716-744), the same floating-point expressions in the same order as the NumPy construction, so
+// the arrays are identical; loops over cells, faces and nodes run on host threads.
+
+namespace {
+
+template <class F>
+void parallel_for(int n, F body)
+{
+    const int hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = std::max(1, std::min(hw, n / 16384 + 1));
+    if (nt == 1) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++) {
+        const int a = (int)((int64_t)n * t / nt), b = (int)((int64_t)n * (t + 1) / nt);
+        th.emplace_back([=] { body(a, b); });
+    }
+    for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+extern "C" void mfv_domain_geometry_2d_free(mfv_domain_geometry_2d *g)
+{
+    if (!g) return;
+    double **d[] = {&g->cell_center, &g->cell_volume, &g->face_normal, &g->face_mesure, &g->face_center,
+                    &g->cell_nf, &g->face_ghostcenter, &g->dist_ortho, &g->airDiamond, &g->param1, &g->param2,
+                    &g->param3, &g->param4, &g->f_1, &g->f_2, &g->f_3, &g->f_4, &g->lambda_x, &g->lambda_y,
+                    &g->R_x, &g->R_y, &g->node_ghostcenter, &g->node_ghostfaceinfo};
+    for (double **p : d) free(*p);
+    free(g->face_name);
+    free(g->number);
+    free(g->node_ghostid);
+    free(g->cell_ghostnid);
+    memset(g, 0, sizeof(*g));
+}
+
+extern "C" int mfv_domain_geometry_2d_build(int C, int ew, const int32_t *nodeid, int N, const double *vertex, int F,
+                                      const int32_t *faces_nodeid, int mf1, const int32_t *cells_faceid,
+                                      const int32_t *faces_cellid, const int32_t *node_cellid, int ncw1,
+                                      mfv_domain_geometry_2d *out)
+{
+    MFV_REQUIRE(out, "out is NULL");
+    memset(out, 0, sizeof(*out));
+    MFV_REQUIRE(C > 0 && N > 0 && F > 0 && nodeid && vertex && faces_nodeid && cells_faceid && faces_cellid &&
+                    node_cellid && mf1 >= 2 && ncw1 >= 2,
+                "bad arguments");
+    const int mf = mf1 - 1;
+    auto X = [&](int n) { return vertex[4 * (size_t)n]; };
+    auto Y = [&](int n) { return vertex[4 * (size_t)n + 1]; };
+    std::vector<double> cc((size_t)C * 3, 0.), vol(C, 0.);
+    // Compute_2dcentervolumeOfCell (ddm_utils2d.py:433-479)
+    parallel_for(C, [&](int a, int b) {
+        for (int i = a; i < b; i++) {
+            const int32_t *s = nodeid + (size_t)i * ew;
+            const int nv = s[ew - 1];
+            if (nv == 3) {
+                const double x1 = X(s[0]), y1 = Y(s[0]), x2 = X(s[1]), y2 = Y(s[1]), x3 = X(s[2]), y3 = Y(s[2]);
+                cc[3 * (size_t)i] = 1. / 3 * ((x1 + x2) + x3);
+                cc[3 * (size_t)i + 1] = 1. / 3 * ((y1 + y2) + y3);
+                vol[i] = (1. / 2) * fabs((x1 - x2) * (y1 - y3) - (x1 - x3) * (y1 - y2));
+            } else {
+                double xs[4], ys[4];
+                for (int k = 0; k < 4; k++) {
+                    xs[k] = X(s[k]);
+                    ys[k] = Y(s[k]);
+                }
+                const double cx = (((xs[0] + xs[1]) + xs[2]) + xs[3]) / 4;
+                const double cy = (((ys[0] + ys[1]) + ys[2]) + ys[3]) / 4;
+                cc[3 * (size_t)i] = cx;
+                cc[3 * (size_t)i + 1] = cy;
+                double v = 0.;
+                for (int k = 0; k < 4; k++) {
+                    const double x1 = xs[k], y1 = ys[k], x2 = xs[(k + 1) % 4], y2 = ys[(k + 1) % 4];
+                    v += (1. / 2) * fabs((x1 - x2) * (y1 - cy) - (x1 - cx) * (y1 - y2));
+                }
+                vol[i] = v;
+            }
+        }
+    });
+    // create_info_2dfaces (ddm_utils2d.py:507-551): names, outward non-unit normals, lengths
+    std::vector<int32_t> name(F, 0);
+    std::vector<double> nrm((size_t)F * 3, 0.), mes(F), cf((size_t)F * 3);
+    parallel_for(F, [&](int a, int b) {
+        for (int f = a; f < b; f++) {
+            const int n0 = faces_nodeid[(size_t)f * 3], n1 = faces_nodeid[(size_t)f * 3 + 1];
+            if (faces_cellid[2 * (size_t)f + 1] == -1) {
+                const int ta = (int)vertex[4 * (size_t)n0 + 3], tb = (int)vertex[4 * (size_t)n1 + 3];
+                int nm = 100;
+                if (ta == tb) {
+                    nm = ta;
+                } else {
+                    const int tags[4] = {3, 4, 33, 44};
+                    for (int t : tags)
+                        if ((ta == t && tb != 0) || (ta != 0 && tb == t)) {
+                            nm = t;
+                            break;
+                        }
+                }
+                name[f] = nm;
+            }
+            const double normx = Y(n0) - Y(n1), normy = X(n1) - X(n0);
+            for (int k = 0; k < 3; k++)
+                cf[3 * (size_t)f + k] = 0.5 * (vertex[4 * (size_t)n0 + k] + vertex[4 * (size_t)n1 + k]);
+            const int c0 = faces_cellid[2 * (size_t)f];
+            const double sx = cc[3 * (size_t)c0] - cf[3 * (size_t)f], sy = cc[3 * (size_t)c0 + 1] - cf[3 * (size_t)f + 1];
+            const bool flip = (sx * normx + sy * normy) > 0;
+            const double nx = flip ? -1 * normx : normx, ny = flip ? -1 * normy : normy;
+            nrm[3 * (size_t)f] = nx;
+            nrm[3 * (size_t)f + 1] = ny;
+            mes[f] = sqrt(nx * nx + ny * ny);
+        }
+    });
+    // create_NormalFacesOfCell (ddm_utils2d.py:99-120)
+    std::vector<double> nf((size_t)C * mf * 3, 0.);
+    parallel_for(C, [&](int a, int b) {
+        for (int i = a; i < b; i++) {
+            const int32_t *fr = cells_faceid + (size_t)i * mf1;
+            for (int j = 0; j < fr[mf]; j++) {
+                const int f = fr[j];
+                const double d0 = cc[3 * (size_t)i] - cf[3 * (size_t)f], d1 = cc[3 * (size_t)i + 1] - cf[3 * (size_t)f + 1],
+                             d2 = cc[3 * (size_t)i + 2] - cf[3 * (size_t)f + 2];
+                const double *n = &nrm[3 * (size_t)f];
+                const bool inward = ((d0 * n[0] + d1 * n[1]) + d2 * n[2]) < 0.;
+                for (int k = 0; k < 3; k++) nf[((size_t)i * mf + j) * 3 + k] = inward ? n[k] : -1.0 * n[k];
+            }
+        }
+    });
+    // boundary faces (names 1-4, ascending) and face_info_2d ghost mirrors (ddm_utils2d.py:163-223)
+    std::vector<int32_t> bf;
+    for (int f = 0; f < F; f++)
+        if (name[f] >= 1 && name[f] <= 4) bf.push_back(f);
+    const int nb = (int)bf.size();
+    std::vector<double> gc((size_t)F * 3, -1.);
+    parallel_for(nb, [&](int a, int b) {
+        for (int q = a; q < b; q++) {
+            const int f = bf[q];
+            const int nod1 = faces_nodeid[(size_t)f * 3 + 1], nod2 = faces_nodeid[(size_t)f * 3];
+            const double x10 = X(nod1), x11 = Y(nod1), x20 = X(nod2), x21 = Y(nod2);
+            const int c0 = faces_cellid[2 * (size_t)f];
+            const double v10 = cc[3 * (size_t)c0], v11 = cc[3 * (size_t)c0 + 1];
+            const double gamma = ((v10 - x20) * (x10 - x20) + (v11 - x21) * (x11 - x21)) /
+                                 ((x10 - x20) * (x10 - x20) + (x11 - x21) * (x11 - x21));
+            const double kk0 = gamma * x10 + (1 - gamma) * x20, kk1 = gamma * x11 + (1 - gamma) * x21;
+            gc[3 * (size_t)f] = 2 * kk0 + (-1 * v10);
+            gc[3 * (size_t)f + 1] = 2 * kk1 + (-1 * v11);
+            gc[3 * (size_t)f + 2] = gamma;
+        }
+    });
+    // per-node ghost lists, by boundary face then by the node's slot in it (nod1, nod2)
+    std::vector<int32_t> gcount(N, 0);
+    for (int q = 0; q < nb; q++)
+        for (int s = 0; s < 2; s++) gcount[faces_nodeid[(size_t)bf[q] * 3 + (s == 0 ? 1 : 0)]]++;
+    int g = 1;
+    for (int v = 0; v < N; v++) g = std::max(g, (int)gcount[v]);
+    std::vector<double> gcn((size_t)N * g * 5, -1.), gfi((size_t)N * g * 4, -1.);
+    std::vector<int32_t> gid((size_t)N * (g + 1), -1), gfill(N, 0);
+    for (int q = 0; q < nb; q++) {
+        const int f = bf[q];
+        for (int s = 0; s < 2; s++) {
+            const int v = faces_nodeid[(size_t)f * 3 + (s == 0 ? 1 : 0)];
+            const int p = gfill[v]++;
+            double *r = &gcn[((size_t)v * g + p) * 5];
+            r[0] = gc[3 * (size_t)f];
+            r[1] = gc[3 * (size_t)f + 1];
+            r[2] = (double)faces_cellid[2 * (size_t)f];
+            r[3] = (double)name[f];
+            r[4] = (double)f;
+            double *t = &gfi[((size_t)v * g + p) * 4];
+            t[0] = cf[3 * (size_t)f];
+            t[1] = cf[3 * (size_t)f + 1];
+            t[2] = nrm[3 * (size_t)f];
+            t[3] = nrm[3 * (size_t)f + 1];
+            gid[(size_t)v * (g + 1) + p] = f;
+        }
+    }
+    for (int v = 0; v < N; v++) gid[(size_t)v * (g + 1) + g] = gcount[v];
+    // cells.ghostnid: the sorted union of the ghost faces of the cell's nodes
+    std::vector<std::vector<int32_t>> cg(C);
+    int gw = 0;
+    parallel_for(C, [&](int a, int b) {
+        for (int i = a; i < b; i++) {
+            const int32_t *s = nodeid + (size_t)i * ew;
+            std::vector<int32_t> &l = cg[i];
+            for (int k = 0; k < s[ew - 1]; k++)
+                for (int p = 0; p < gcount[s[k]]; p++) l.push_back(gid[(size_t)s[k] * (g + 1) + p]);
+            std::sort(l.begin(), l.end());
+            l.erase(std::unique(l.begin(), l.end()), l.end());
+        }
+    });
+    for (int i = 0; i < C; i++) gw = std::max(gw, (int)cg[i].size());
+    std::vector<int32_t> gnid((size_t)C * (gw + 1), -1);
+    for (int i = 0; i < C; i++) {
+        std::copy(cg[i].begin(), cg[i].end(), &gnid[(size_t)i * (gw + 1)]);
+        gnid[(size_t)i * (gw + 1) + gw] = (int32_t)cg[i].size();
+    }
+    // variables_2d (ddm_utils2d.py:608-705): the node's cells, then its ghost mirror centres
+    std::vector<double> lx(N), ly(N), Rx(N), Ry(N);
+    std::vector<int32_t> number(N);
+    parallel_for(N, [&](int a, int b) {
+        for (int v = a; v < b; v++) {
+            double Ixx = 0., Iyy = 0., Ixy = 0., rx = 0., ry = 0.;
+            int num = 0;
+            const double vx = X(v), vy = Y(v);
+            const int nc = node_cellid[(size_t)v * ncw1 + ncw1 - 1];
+            for (int j = 0; j < nc; j++) {
+                const int c = node_cellid[(size_t)v * ncw1 + j];
+                const double dx = cc[3 * (size_t)c] - vx, dy = cc[3 * (size_t)c + 1] - vy;
+                Ixx += dx * dx;
+                Iyy += dy * dy;
+                Ixy += dx * dy;
+                rx += dx;
+                ry += dy;
+                num++;
+            }
+            for (int j = 0; j < gcount[v]; j++) {
+                const int f = gid[(size_t)v * (g + 1) + j];
+                const double dx = gc[3 * (size_t)f] - vx, dy = gc[3 * (size_t)f + 1] - vy;
+                Ixx += dx * dx;
+                Iyy += dy * dy;
+                Ixy += dx * dy;
+                rx += dx;
+                ry += dy;
+                num++;
+            }
+            const double D = Ixx * Iyy - Ixy * Ixy;
+            lx[v] = (Ixy * ry - Iyy * rx) / D;
+            ly[v] = (Ixy * rx - Ixx * ry) / D;
+            Rx[v] = rx;
+            Ry[v] = ry;
+            number[v] = num;
+        }
+    });
+    // face_gradient_info_2d (ddm_utils2d.py:553-605) and dist_ortho (ddm_utils2d.py:716-744)
+    std::vector<double> air(F), p1(F), p2(F), p3(F), p4(F), f1((size_t)F * 2), f2((size_t)F * 2), f3((size_t)F * 2),
+        f4((size_t)F * 2), dort(F, 0.);
+    parallel_for(F, [&](int a, int b) {
+        for (int f = a; f < b; f++) {
+            const int n0 = faces_nodeid[(size_t)f * 3], n1 = faces_nodeid[(size_t)f * 3 + 1];
+            const double xy1[2] = {X(n0), Y(n0)}, xy2[2] = {X(n1), Y(n1)};
+            const int c0 = faces_cellid[2 * (size_t)f], c1 = faces_cellid[2 * (size_t)f + 1];
+            const double v1[2] = {cc[3 * (size_t)c0], cc[3 * (size_t)c0 + 1]};
+            double v2[2] = {gc[3 * (size_t)f], gc[3 * (size_t)f + 1]};
+            if (name[f] == 0) {
+                v2[0] = cc[3 * (size_t)c1];
+                v2[1] = cc[3 * (size_t)c1 + 1];
+            }
+            const double F1[2] = {v1[0] - xy1[0], v1[1] - xy1[1]}, F2[2] = {xy2[0] - v1[0], xy2[1] - v1[1]},
+                         F3[2] = {v2[0] - xy2[0], v2[1] - xy2[1]}, F4[2] = {xy1[0] - v2[0], xy1[1] - v2[1]};
+            const double nx = nrm[3 * (size_t)f], ny = nrm[3 * (size_t)f + 1];
+            const double ar = 0.5 * ((xy2[0] - xy1[0]) * (v2[1] - v1[1]) + (v1[0] - v2[0]) * (xy2[1] - xy1[1]));
+            const double k = 1. / (2. * ar);
+            air[f] = ar;
+            p1[f] = k * ((F1[1] + F2[1]) * nx - (F1[0] + F2[0]) * ny);
+            p2[f] = k * ((F2[1] + F3[1]) * nx - (F2[0] + F3[0]) * ny);
+            p3[f] = k * ((F3[1] + F4[1]) * nx - (F3[0] + F4[0]) * ny);
+            p4[f] = k * ((F4[1] + F1[1]) * nx - (F4[0] + F1[0]) * ny);
+            for (int q = 0; q < 2; q++) {
+                f1[2 * (size_t)f + q] = F1[q];
+                f2[2 * (size_t)f + q] = F2[q];
+                f3[2 * (size_t)f + q] = F3[q];
+                f4[2 * (size_t)f + q] = F4[q];
+            }
+            auto proj = [&](int K) {
+                const double vx = cc[3 * (size_t)K] - cf[3 * (size_t)f], vy = cc[3 * (size_t)K + 1] - cf[3 * (size_t)f + 1];
+                const double s = vx * nx + vy * ny;
+                const double px = cc[3 * (size_t)K] - s * nx, py = cc[3 * (size_t)K + 1] - s * ny;
+                const double dx = cc[3 * (size_t)K] - px, dy = cc[3 * (size_t)K + 1] - py;
+                return sqrt(dx * dx + dy * dy);
+            };
+            if (name[f] == 0) dort[f] = proj(c0) + proj(c1);
+            else if (name[f] >= 1 && name[f] <= 4) dort[f] = 2 * proj(c0);
+        }
+    });
+    out->ghost_width = g;
+    out->ghostnid_width = gw;
+    out->cell_center = copy_out(cc);
+    out->cell_volume = copy_out(vol);
+    out->face_name = copy_out(name);
+    out->face_normal = copy_out(nrm);
+    out->face_mesure = copy_out(mes);
+    out->face_center = copy_out(cf);
+    out->cell_nf = copy_out(nf);
+    out->face_ghostcenter = copy_out(gc);
+    out->dist_ortho = copy_out(dort);
+    out->airDiamond = copy_out(air);
+    out->param1 = copy_out(p1);
+    out->param2 = copy_out(p2);
+    out->param3 = copy_out(p3);
+    out->param4 = copy_out(p4);
+    out->f_1 = copy_out(f1);
+    out->f_2 = copy_out(f2);
+    out->f_3 = copy_out(f3);
+    out->f_4 = copy_out(f4);
+    out->lambda_x = copy_out(lx);
+    out->lambda_y = copy_out(ly);
+    out->R_x = copy_out(Rx);
+    out->R_y = copy_out(Ry);
+    out->number = copy_out(number);
+    out->node_ghostcenter = copy_out(gcn);
+    out->node_ghostfaceinfo = copy_out(gfi);
+    out->node_ghostid = copy_out(gid);
+    out->cell_ghostnid = copy_out(gnid);
     return MFV_OK;
 }

@@ -442,6 +442,53 @@ def native_topology(nodeid, nbnodes, dim, want_cellnid):
     return out
 
 
+def native_geometry_2d(d):
+    """The 2D geometry from the library's host builder (``mfv_domain_geometry_2d_build``, C++
+    on host threads): the arrays of the NumPy stages below, identical."""
+    import ctypes
+    from . import _lib
+    nodeid = np.ascontiguousarray(d._cells.nodeid, dtype=I32)
+    vertex = np.ascontiguousarray(d._nodes.vertex, dtype=F64)
+    fn = np.ascontiguousarray(d._faces.nodeid, dtype=I32)
+    faceid = np.ascontiguousarray(d._cells.faceid, dtype=I32)
+    cellid = np.ascontiguousarray(d._faces.cellid, dtype=I32)
+    ncid = np.ascontiguousarray(d._nodes.cellid, dtype=I32)
+    C, N, F = len(nodeid), len(vertex), len(fn)
+    if fn.shape[1] != 3:
+        raise ValueError("2D face rows have two nodes and the count")
+    mf = faceid.shape[1] - 1
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    g = _lib.MfvDomainGeometry2d()
+    _lib.call('mfv_domain_geometry_2d_build', C, nodeid.shape[1], ptr(nodeid), N, ptr(vertex), F, ptr(fn),
+              faceid.shape[1], ptr(faceid), ptr(cellid), ptr(ncid), ncid.shape[1], ctypes.byref(g))
+    try:
+        def grab(p, shape, ct=ctypes.c_double):
+            n = int(np.prod(shape))
+            if n == 0:
+                return np.zeros(shape, dtype=np.float64 if ct is ctypes.c_double else I32)
+            return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=shape).copy()
+        gw, nw = int(g.ghost_width), int(g.ghostnid_width)
+        out = {'center': grab(g.cell_center, (C, 3)), 'volume': grab(g.cell_volume, (C,)),
+               'name': grab(g.face_name, (F,), ctypes.c_int32), 'normal': grab(g.face_normal, (F, 3)),
+               'mesure': grab(g.face_mesure, (F,)), 'fcenter': grab(g.face_center, (F, 3)),
+               'nf': grab(g.cell_nf, (C, mf, 3)), 'ghostcenter': grab(g.face_ghostcenter, (F, 3)),
+               'dist_ortho': grab(g.dist_ortho, (F,)), 'airDiamond': grab(g.airDiamond, (F,))}
+        for k in ('param1', 'param2', 'param3', 'param4'):
+            out[k] = grab(getattr(g, k), (F,))
+        for k in ('f_1', 'f_2', 'f_3', 'f_4'):
+            out[k] = grab(getattr(g, k), (F, 2))
+        for k in ('lambda_x', 'lambda_y', 'R_x', 'R_y'):
+            out[k] = grab(getattr(g, k), (N,))
+        out['number'] = grab(g.number, (N,), ctypes.c_int32)
+        out['node_ghostcenter'] = grab(g.node_ghostcenter, (N, gw, 5))
+        out['node_ghostfaceinfo'] = grab(g.node_ghostfaceinfo, (N, gw, 4))
+        out['ghostid'] = grab(g.node_ghostid, (N, gw + 1), ctypes.c_int32)
+        out['ghostnid'] = grab(g.cell_ghostnid, (C, nw + 1), ctypes.c_int32)
+    finally:
+        _lib.call('mfv_domain_geometry_2d_free', ctypes.byref(g))
+    return out
+
+
 # ---------------------------------------------------------------------------------------------
 
 class Domain(object):
@@ -503,10 +550,16 @@ class Domain(object):
         self._halos = Container('halos')
         self._read_partition(elements, vertex)
         self._define_bounds()
-        self._compute_cells_info()
+        self._geo = None
+        native2d = self._dim == 2 and self._native() is not None
+        if not native2d:
+            self._compute_cells_info()
         self._make_neighbors()
         self._define_eltypes()
         self._create_faces_cons()
+        if native2d:  # the 2D geometry in C++ (the NumPy stages below then only copy it in)
+            self._geo = native_geometry_2d(self)
+            self._cells.center, self._cells.volume = self._geo['center'], self._geo['volume']
         self._define_face_info()
         self._make_halo_info()
         self._update_boundaries()
@@ -647,7 +700,14 @@ class Domain(object):
 
     def _define_face_info(self):
         self._nodes.name = self._nodes.vertex[:, 3].astype(I32)
-        if self._dim == 2:
+        if self._geo is not None:
+            g = self._geo
+            self._faces.name = g['name']
+            self._faces.normal = g['normal']
+            self._faces.mesure = g['mesure']
+            self._faces.center = g['fcenter']
+            self._cells.nf = g['nf']
+        elif self._dim == 2:
             namef, normal, mesure, centerf = face_info_2d_geometry(
                 self._faces.cellid, self._faces.nodeid, self._nodes.name, self._nodes.vertex, self._cells.center)
             self._faces.name = namef
@@ -724,6 +784,13 @@ class Domain(object):
         """``face_info_2d`` (``ddm_utils2d.py:163-223``) / ``face_info_3d``
         (``ddm_utils3d.py:87-157``): mirror centres of boundary faces and the
         per-node ghost lists (boundary-face order)."""
+        if self._geo is not None:
+            g = self._geo
+            self._faces.ghostcenter = g['ghostcenter']
+            self._nodes.ghostcenter = g['node_ghostcenter']
+            self._nodes.ghostfaceinfo = g['node_ghostfaceinfo']
+            self._cells.halonid = np.zeros((int(self._nbcells), 2), dtype=I32)
+            return
         F = int(self._nbfaces)
         bf = self._boundaryfaces.astype(np.int64)
         cellid = self._faces.cellid
@@ -825,6 +892,16 @@ class Domain(object):
         N = int(self._nbnodes)
         C = int(self._nbcells)
         gcn = self._nodes.ghostcenter
+        if self._geo is not None:
+            self._nodes.ghostid = self._geo['ghostid']
+            self._halos.sizehaloghost = 0
+            self._nodes.haloghostcenter = -1 * np.ones((N, 1, gcn.shape[2]), dtype=F64)
+            self._nodes.haloghostfaceinfo = -1 * np.ones((N, 1, self._nodes.ghostfaceinfo.shape[2]), dtype=F64)
+            self._cells.haloghostcenter = np.zeros((0, 3), dtype=F64)
+            self._cells.ghostnid = self._geo['ghostnid']
+            self._nodes.haloghostid = np.zeros((N, 2), dtype=I32)
+            self._cells.haloghostnid = np.zeros((C, 2), dtype=I32)
+            return
         fid = gcn[:, :, -1]
         valid = fid != -1
         g = gcn.shape[1]
@@ -859,7 +936,16 @@ class Domain(object):
             self._nodes._lazy.pop(a, None)
         for a in ('airDiamond', 'param1', 'param2', 'param3', 'param4', 'f_1', 'f_2', 'f_3', 'f_4'):
             self._faces._lazy.pop(a, None)
-        if self._dim == 2:
+        if self._geo is not None:
+            g = self._geo
+            N = int(self._nbnodes)
+            self._nodes.lambda_x, self._nodes.lambda_y = g['lambda_x'], g['lambda_y']
+            self._nodes.lambda_z = np.zeros(N)
+            self._nodes.R_x, self._nodes.R_y, self._nodes.R_z = g['R_x'], g['R_y'], np.zeros(N)
+            self._nodes.number = g['number']
+            for k in ('airDiamond', 'param1', 'param2', 'param3', 'param4', 'f_1', 'f_2', 'f_3', 'f_4'):
+                setattr(self._faces, k, g[k])
+        elif self._dim == 2:
             self._variables_2d()
             self._face_gradient_info_2d()
         else:
@@ -1002,6 +1088,10 @@ class Domain(object):
     def _compute_orthocenter(self):
         """``dist_ortho_function`` (``ddm_utils2d.py:716-744``), non-unit
         normal kept as in the reference."""
+        if self._geo is not None:
+            self._faces.dist_ortho = self._geo['dist_ortho']
+            self._geo = None  # every stage has taken its arrays
+            return
         F = int(self._nbfaces)
         d = np.zeros(F, dtype=F64)
         cellid = self._faces.cellid

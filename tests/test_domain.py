@@ -173,7 +173,9 @@ def test_order_tables(dom2d):
 def test_native_topology_equals_numpy(case):
     """The C++ connectivity builder (mfv_domain_topology_build) gives every array of the NumPy
     construction: face numbering and node rows, cells.faceid, faces.cellid, cells.cellfid,
-    nodes.cellid, cells.cellnid, and everything derived from them."""
+    nodes.cellid, cells.cellnid, and everything derived from them. In 2D the geometry comes
+    from C++ too (mfv_domain_geometry_2d_build: centers, volumes, normals, ghost mirrors, the
+    per-node ghost lists, the diamond weights and distances), bit for bit the NumPy stages'."""
     import os
     from manapy_amd.ddm import Domain
     from manapy_amd.gmsh import read_gmsh
