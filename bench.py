@@ -356,14 +356,17 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=5, renumber='hilbert'):
     ring, dt0 = dv(np.zeros(4)), dv(np.zeros(1))
 
     from manapy_amd.device import EulerTet, EulerTetTiles, FaceTiles
-    tiles, ptiles = {}, {}
-    for T in (128, 256):
-        ft = FaceTiles(m, T)
-        if ft.lds_bytes() <= 64 * 1024:
-            tiles[T] = ft
+    tiles, ptiles, ctiles = {}, {}, {}
+    for T in (128, 256, 512):
+        if T < 512:
+            ft = FaceTiles(m, T)
+            if ft.lds_bytes() <= 64 * 1024:
+                tiles[T] = ft
         pt = EulerTetTiles(m, T)
         if pt.lds_bytes() <= 64 * 1024:
             ptiles[T] = pt
+        if T > 128 and pt.lds_bytes(cells=True) <= 64 * 1024:
+            ctiles[T] = pt
     et = EulerTet(m)
     R = [torch.empty((C, 8), dtype=torch.float64, device='cuda') for _ in range(2)]
     path = ['fused']  # 'fused': per-cell step on [6][C]; 'tiled:T': face tiles; 'packed': [C][8] records
@@ -384,12 +387,15 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=5, renumber='hilbert'):
         elif path[0].startswith('packed-tiled:'):
             _lib.call('mfv_euler3d_step_packed_tiled', m.ptr, et.ptr, ptiles[int(path[0].split(':')[1])].ptr,
                       R[k % 2].data_ptr(), R[(k + 1) % 2].data_ptr(), 1.4, CFL, 0, ring.data_ptr(), k, s)
+        elif path[0].startswith('cells-tiled:'):
+            _lib.call('mfv_euler3d_step_cells_tiled', m.ptr, et.ptr, ctiles[int(path[0].split(':')[1])].ptr,
+                      R[k % 2].data_ptr(), R[(k + 1) % 2].data_ptr(), 1.4, CFL, 0, ring.data_ptr(), k, s)
         else:
             _lib.call('mfv_euler3d_step_tiled', m.ptr, tiles[int(path[0].split(':')[1])].ptr, S[k % 2].data_ptr(),
                       S[(k + 1) % 2].data_ptr(), 1.4, CFL, 0, ring.data_ptr(), k, s)
     probe = {}
     for cand in (['fused', 'packed'] + ['packed-tiled:%d' % T for T in sorted(ptiles)] +
-                 ['tiled:%d' % T for T in sorted(tiles)]):
+                 ['cells-tiled:%d' % T for T in sorted(ctiles)] + ['tiled:%d' % T for T in sorted(tiles)]):
         path[0] = cand
         init()
         probe[cand] = time_steps(step, 2, 10)[1] * 1e3
@@ -409,6 +415,10 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=5, renumber='hilbert'):
         # 40-byte entry (geometry, item pair)
         bpc = 64 + 64 + 16 + 1 + 40 * F / C
         kname = "k_euler_step_packed_tiled<%s>" % best.split(':')[1]
+    elif best.startswith('cells-tiled:'):
+        # the same tables as the face-tiled step (the fluxes stay in registers)
+        bpc = 64 + 64 + 16 + 1 + 40 * F / C
+        kname = "k_euler_step_cells_tiled<%s>" % best.split(':')[1]
     else:
         # U read + write (6 doubles each), volume, face count; per face slot: order entry +
         # faceid; per face (x F/C): cellid, normal, mesure, name

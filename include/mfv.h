@@ -555,6 +555,14 @@ typedef struct mfv_euler_tet_tiles {
 int mfv_euler3d_step_packed_tiled(const mfv_mesh *m, const mfv_euler_tet *t, const mfv_euler_tet_tiles *tt,
                                   const double *rec_in, double *rec_out, double gamma, double cfl, int quirk,
                                   double *dt_ring, int step, void *stream);
+/* The same step on the same tiles with one cell phase: after staging the records, each cell
+ * evaluates its own four faces' fluxes (an inner face of two tile cells is evaluated by both,
+ * from the same arguments), so the LDS holds only the records (48 * max_items bytes <= 64 KiB)
+ * and tiles of 512 cells fit. Same bits as mfv_euler3d_step_packed_tiled. Replaces the same
+ * reference functions. */
+int mfv_euler3d_step_cells_tiled(const mfv_mesh *m, const mfv_euler_tet *t, const mfv_euler_tet_tiles *tt,
+                                 const double *rec_in, double *rec_out, double gamma, double cfl, int quirk,
+                                 double *dt_ring, int step, void *stream);
 
 /* FP32-bulk Euler (the float32 variant of explicitscheme_euler_3d_rusanov + update_euler_3d_fvc
  * + time_step_euler_3d, fvm_utils3d.py:838-886, 1083-1170, 1316-1327, 809-825): float32 records
@@ -580,14 +588,16 @@ int mfv_euler3d_step32_tiled(const mfv_mesh *m, const mfv_euler_tet *t, const mf
  * (ast/functions2d.py:221, :362), evaluated once per incidence with that expression order. */
 typedef struct mfv_diamond {
     int32_t nb_cells, nb_faces, nb_nodes, max_cell_faces;
+    int32_t dim;                    /* 2 or 3 */
     const int32_t *cell_faces;      /* [MF][C] as mfv_mesh */
     const int32_t *cell_nfaces;     /* [C] */
     const uint8_t *order_sorted;    /* [MF][C] as mfv_mesh (faces of a cell by ascending id) */
     const double *cell_volume;      /* [C] */
     const int32_t *face_cells;      /* [F][2] */
-    const int32_t *face_nodes;      /* [F][2] faces.nodeid[:, :2] */
+    const int32_t *face_nodes;      /* 2D [F][2] faces.nodeid[:, :2]; 3D [F][4] nodes A, B, C, D (D = C
+                                     * on a triangle) */
     const double *face_param;       /* [F][4] faces.param1..param4 */
-    const double *face_fk;          /* [F][8] faces.f_1..f_4 (x, y) */
+    const double *face_fk;          /* 2D [F][8] faces.f_1..f_4 (x, y); 3D [F][6] faces.f_1, f_2 (n1, n2) */
     const double *face_air;         /* [F]    faces.airDiamond */
     const double *face_normal;      /* [F][3] */
     const int32_t *node_cell_ptr;   /* [N+1]  CSR of nodes.cellid */
@@ -595,7 +605,7 @@ typedef struct mfv_diamond {
     const double *node_cell_alpha;  /*        alpha of each incident cell centre */
     const int32_t *node_ghost_ptr;  /* [N+1]  CSR of the valid nodes.ghostcenter entries */
     const int32_t *node_ghost_face; /*        boundary face of the ghost centre */
-    const int32_t *node_ghost_owner;/*        its owner cell (ghostcenter[..][2]) */
+    const int32_t *node_ghost_owner;/*        its owner cell (ghostcenter[..][2]; [..][3] in 3D) */
     const double *node_ghost_alpha; /*        alpha of the ghost centre */
 } mfv_diamond;
 
@@ -603,13 +613,14 @@ typedef struct mfv_diamond {
  *   node_dir[N]      1 if nodes.oldname is in BCdirichlet (search_element(BCdirichlet, ..) == 1)
  *   face_dir_rank[F] position of the face in the variable's dirichletfaces list, -1 if absent */
 
-/* get_triplet_2d + compute_2dmatrix_size (ast/functions2d.py:328-664), first pass: for each
- * 64-row slice the largest number of distinct columns in its rows -> slice_w[ceil(C/64)];
- * info[0] |= 1 if a row has more than 32 distinct columns, info[1] = max(info[1], width). */
+/* get_triplet_2d + compute_2dmatrix_size (ast/functions2d.py:328-664), or in 3D get_triplet_3d
+ * + compute_3dmatrix_size (ast/functions3d.py:410-782), first pass: for each 64-row slice the
+ * largest number of distinct columns in its rows -> slice_w[ceil(C/64)]; info[0] |= 1 if a row
+ * has more than 32 (2D) or 128 (3D) distinct columns, info[1] = max(info[1], width). */
 int mfv_diamond_widths(const mfv_diamond *d, const uint8_t *node_dir, const int32_t *face_dir_rank,
                        int32_t *slice_w, int32_t *info, void *stream);
 
-/* get_triplet_2d, second pass: the matrix rows with the reference's duplicate triplets summed
+/* get_triplet_2d / get_triplet_3d, second pass: the matrix rows with the reference's duplicate triplets summed
  * in insertion order, written as ELL slices (entry j of row K at ((K/64)*width + j)*64 + K%64,
  * columns ascending and offset by row0, unused slots (row0+K, 0)), vals32 = (float)vals,
  * diag[K] = A[K][K]. Rows C..64*ceil(C/64)-1 are padding (no entries, diag 0). */
@@ -617,11 +628,13 @@ int mfv_diamond_assemble(const mfv_diamond *d, const uint8_t *node_dir, const in
                          int row0, int width, const int32_t *slice_w, int32_t *cols, double *vals,
                          float *vals32, double *diag, void *stream);
 
-/* get_rhs_glob_2d (ast/functions2d.py:740-817): Dirichlet lifting rhs[C]. */
+/* get_rhs_glob_2d (ast/functions2d.py:740-817) or get_rhs_glob_3d (ast/functions3d.py:865-946):
+ * Dirichlet lifting rhs[C]. */
 int mfv_diamond_rhs(const mfv_diamond *d, const uint8_t *node_dir, const int32_t *face_dir_rank,
                     const double *Pbordnode, const double *Pbordface, double *rhs, void *stream);
 
-/* centertovertex_2d (ast/functions2d.py:199-277): w_node[N]. */
+/* centertovertex_2d (ast/functions2d.py:199-277) or centertovertex_3d (ast/functions3d.py:251-362):
+ * w_node[N]. */
 int mfv_centertovertex(const mfv_diamond *d, const double *w_c, const double *w_ghost, double *w_node,
                        void *stream);
 
@@ -632,6 +645,13 @@ int mfv_face_gradient(const mfv_diamond *d, int kind, const double *w_c, const d
                       const double *w_node, const int8_t *face_bc, const uint8_t *node_dir,
                       const double *Pbordnode, const double *Pbordface, double *wx, double *wy,
                       void *stream);
+
+/* 3D: kind 0 face_gradient_3d (ast/functions3d.py:122-248); kind 1 compute_P_gradient_3d_diamond
+ * (:949-1108). face_bc, node_dir, Pbord* as mfv_face_gradient. */
+int mfv_face_gradient_3d(const mfv_diamond *d, int kind, const double *w_c, const double *w_ghost,
+                         const double *w_node, const int8_t *face_bc, const uint8_t *node_dir,
+                         const double *Pbordnode, const double *Pbordface, double *wx, double *wy, double *wz,
+                         void *stream);
 
 /* explicitscheme_dissipative (solvers/diffusion/fvm_utils.py:5-26); wz may be NULL (zeros).
  * Needs mesh->order_sorted. */

@@ -127,7 +127,8 @@ class MfvSwMixed(ctypes.Structure):
 class MfvDiamond(ctypes.Structure):
     """Mirror of ``struct mfv_diamond`` (include/mfv.h)."""
     _fields_ = [('nb_cells', ctypes.c_int32), ('nb_faces', ctypes.c_int32), ('nb_nodes', ctypes.c_int32),
-                ('max_cell_faces', ctypes.c_int32), ('cell_faces', P), ('cell_nfaces', P), ('order_sorted', P),
+                ('max_cell_faces', ctypes.c_int32), ('dim', ctypes.c_int32), ('cell_faces', P), ('cell_nfaces', P),
+                ('order_sorted', P),
                 ('cell_volume', P), ('face_cells', P), ('face_nodes', P), ('face_param', P), ('face_fk', P),
                 ('face_air', P), ('face_normal', P), ('node_cell_ptr', P), ('node_cell', P),
                 ('node_cell_alpha', P), ('node_ghost_ptr', P), ('node_ghost_face', P), ('node_ghost_owner', P),
@@ -229,6 +230,7 @@ _SIGNATURES = {
     'mfv_euler3d_unpack': [MESH_P, P, P, P],
     'mfv_euler3d_step_packed': [MESH_P, ET_P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_euler3d_step_packed_tiled': [MESH_P, ET_P, ETT_P, P, P, c_double, c_double, c_int, P, c_int, P],
+    'mfv_euler3d_step_cells_tiled': [MESH_P, ET_P, ETT_P, P, P, c_double, c_double, c_int, P, c_int, P],
     'mfv_euler32_pack': [MESH_P, P, P, P],
     'mfv_euler32_unpack': [MESH_P, P, P, P],
     'mfv_euler32_tile_geometry': [ETT_P, c_int, P, P],
@@ -238,6 +240,7 @@ _SIGNATURES = {
     'mfv_diamond_rhs': [DIA_P, P, P, P, P, P, P],
     'mfv_centertovertex': [DIA_P, P, P, P, P],
     'mfv_face_gradient': [DIA_P, c_int, P, P, P, P, P, P, P, P, P, P],
+    'mfv_face_gradient_3d': [DIA_P, c_int, P, P, P, P, P, P, P, P, P, P, P],
     'mfv_explicitscheme_dissipative': [MESH_P, P, P, P, c_double, c_double, c_double, P, P],
     'mfv_time_step_diffusion': [MESH_P, P, P, P, c_double, c_double, c_double, c_double, P, DP, P],
     'mfv_sw_batch_dt_init': [MESH_P, SWM_P, SWB_P, c_double, c_double, P],

@@ -405,7 +405,8 @@ def _euler_ghost(domain, rhog, Pg, rhoug, rhovg, rhowg, ug, vg, wg, rhoEg, rhoc,
 
 
 # ---------------------------------------------------------------------------------------------
-# diamond scheme (ast/functions2d.py, solvers/diffusion, solvers/advecdiff)
+# diamond scheme (ast/functions2d.py, ast/functions3d.py, solvers/diffusion, solvers/advecdiff);
+# the 3D functions share the 2D signatures
 
 def _i32dev(a, device):
     import torch
@@ -427,22 +428,25 @@ def _no_halo(*lists):
         raise NotImplementedError("one subdomain, non-periodic: halo/periodic face lists must be empty")
 
 
-@HIPBackend.register('compute_2dmatrix_size')
+@HIPBackend.register('compute_2dmatrix_size', 'compute_3dmatrix_size')
 def _matrix_size(domain, nodeidf, halofid, cellnid, halonid, periodicnid, centergn, halocentergn, oldnamen,
                  BCdirichlet, matrixinnerfaces, halofaces, dirichletfaces):
-    """ast/functions2d.py:582-664 (an index count, evaluated on the host)."""
-    from .solvers.ls import compute_2dmatrix_size
+    """ast/functions2d.py:582-664 / functions3d.py:410-495 (an index count, evaluated on the
+    host)."""
+    from .solvers.ls import compute_2dmatrix_size, compute_3dmatrix_size
     _no_halo(halofaces)
-    return compute_2dmatrix_size(domain, BCdirichlet, matrixinnerfaces, dirichletfaces)
+    size = compute_2dmatrix_size if int(domain.dim) == 2 else compute_3dmatrix_size
+    return size(domain, BCdirichlet, matrixinnerfaces, dirichletfaces)
 
 
-@HIPBackend.register('get_triplet_2d')
+@HIPBackend.register('get_triplet_2d', 'get_triplet_3d')
 def _get_triplet_2d(domain, cellfid, nodeidf, vertexn, halofid, haloext, oldnamen, volume, cellnid, centerc,
                     centerh, halonid, periodicnid, centergn, halocentergn, airDiamond, lambda_x, lambda_y,
                     lambda_z, number, R_x, R_y, R_z, param1, param2, param3, param4, shift, nbelements,
                     loctoglob, BCdirichlet, a_loc, irn_loc, jcn_loc, matrixinnerfaces, halofaces,
                     dirichletfaces):
-    """ast/functions2d.py:328-580. The triplets come back with duplicates already summed (in
+    """ast/functions2d.py:328-580 / functions3d.py:498-782. The triplets come back with
+    duplicates already summed (in
     the reference's insertion order), one per distinct (row, col); the remaining slots of the
     caller's arrays are (0, 0, 0.0), so any sparse constructor that sums duplicates builds the
     same matrix."""
@@ -454,7 +458,8 @@ def _get_triplet_2d(domain, cellfid, nodeidf, vertexn, halofid, haloext, oldname
     if not np.array_equal(np.sort(np.asarray(matrixinnerfaces)), np.sort(np.asarray(domain.innerfaces))):
         raise NotImplementedError("matrixinnerfaces must be the domain's inner faces (no periodic faces)")
     node_dir, rank, _ = _bc_dev(domain, BCdirichlet, dirichletfaces, [])
-    batch = EllBatch((C + 63) // 64, max_inst=1, device=m.device)
+    W = 32 if int(domain.dim) == 2 else domain.device_diamond.width(node_dir, rank)
+    batch = EllBatch((C + 63) // 64, max_inst=1, width=W, device=m.device)
     inst = batch.alloc(C)
     batch.assemble_diamond(inst, domain.device_diamond, node_dir, rank)
     W = batch.W
@@ -480,10 +485,10 @@ def _get_triplet_2d(domain, cellfid, nodeidf, vertexn, halofid, haloext, oldname
     return n
 
 
-@HIPBackend.register('get_rhs_glob_2d')
+@HIPBackend.register('get_rhs_glob_2d', 'get_rhs_glob_3d')
 def _get_rhs_glob_2d(domain, cellfid, nodeidf, oldname, volume, centergn, loctoglob, param1, param2, param3, param4,
                      Pbordnode, Pbordface, rhs, BCdirichlet, centergf, matrixinnerfaces, halofaces, dirichletfaces):
-    """ast/functions2d.py:740-817."""
+    """ast/functions2d.py:740-817 / functions3d.py:865-946."""
     _no_halo(halofaces)
     m = _m(domain)
     dd = domain.device_diamond
@@ -496,11 +501,11 @@ def _get_rhs_glob_2d(domain, cellfid, nodeidf, oldname, volume, centergn, loctog
     A.finish()
 
 
-@HIPBackend.register('centertovertex_2d')
+@HIPBackend.register('centertovertex_2d', 'centertovertex_3d')
 def _centertovertex_2d(domain, w_c, w_ghost, w_halo, w_haloghost, centerc, centerh, cellid, ghostid, haloghostid,
                        periodicid, haloid, vertexn, centergf, halocenterg, R_x, R_y, R_z, lambda_x, lambda_y,
                        lambda_z, number, shift, w_n):
-    """ast/functions2d.py:199-277."""
+    """ast/functions2d.py:199-277 / functions3d.py:251-362."""
     m = _m(domain)
     dd = domain.device_diamond
     A = _Args(m.device)
@@ -510,11 +515,11 @@ def _centertovertex_2d(domain, w_c, w_ghost, w_halo, w_haloghost, centerc, cente
     A.finish()
 
 
-@HIPBackend.register('face_gradient_2d')
+@HIPBackend.register('face_gradient_2d', 'face_gradient_3d')
 def _face_gradient_2d(domain, w_c, w_ghost, w_halo, w_node, cellidf, nodeidf, centergf, halofid, centerc, centerh,
                       vertexn, airDiamond, normalf, f_1, f_2, f_3, f_4, shift, wx_face, wy_face, wz_face, innerfaces,
                       halofaces, dirichletfaces, neumann, periodicfaces):
-    """ast/functions2d.py:108-196."""
+    """ast/functions2d.py:108-196 / functions3d.py:122-248."""
     _no_halo(halofaces, periodicfaces)
     m = _m(domain)
     dd = domain.device_diamond
@@ -522,16 +527,21 @@ def _face_gradient_2d(domain, w_c, w_ghost, w_halo, w_node, cellidf, nodeidf, ce
     A = _Args(m.device)
     wc, wg, wn = A.dev(w_c), A.dev(w_ghost), A.dev(w_node)
     wx, wy = A.dev(wx_face, out=True), A.dev(wy_face, out=True)
-    _lib.call('mfv_face_gradient', dd.ptr, 0, wc.data_ptr(), wg.data_ptr(), wn.data_ptr(), face_bc.data_ptr(), None,
-              None, None, wx.data_ptr(), wy.data_ptr(), stream_ptr())
+    args = [dd.ptr, 0, wc.data_ptr(), wg.data_ptr(), wn.data_ptr(), face_bc.data_ptr(), None, None, None,
+            wx.data_ptr(), wy.data_ptr()]
+    if int(domain.dim) == 2:
+        _lib.call('mfv_face_gradient', *args, stream_ptr())
+    else:
+        wz = A.dev(wz_face, out=True)
+        _lib.call('mfv_face_gradient_3d', *args, wz.data_ptr(), stream_ptr())
     A.finish()
 
 
-@HIPBackend.register('compute_P_gradient_2d_diamond')
+@HIPBackend.register('compute_P_gradient_2d_diamond', 'compute_P_gradient_3d_diamond')
 def _p_gradient_2d(domain, P_c, P_ghost, P_halo, P_node, cellidf, nodeidf, centergf, halofid, centerc, centerh,
                    oldname, airDiamond, f_1, f_2, f_3, f_4, normalf, shift, Pbordnode, Pbordface, Px_face, Py_face,
                    Pz_face, BCdirichlet, innerfaces, halofaces, neumannfaces, dirichletfaces, periodicfaces):
-    """ast/functions2d.py:820-936."""
+    """ast/functions2d.py:820-936 / functions3d.py:949-1108 (f_1, f_2 are the 3D n1, n2)."""
     _no_halo(halofaces, periodicfaces)
     m = _m(domain)
     dd = domain.device_diamond
@@ -540,8 +550,13 @@ def _p_gradient_2d(domain, P_c, P_ghost, P_halo, P_node, cellidf, nodeidf, cente
     pc, pg, pn = A.dev(P_c), A.dev(P_ghost), A.dev(P_node)
     bn, bf = A.dev(Pbordnode), A.dev(Pbordface)
     px, py = A.dev(Px_face, out=True), A.dev(Py_face, out=True)
-    _lib.call('mfv_face_gradient', dd.ptr, 1, pc.data_ptr(), pg.data_ptr(), pn.data_ptr(), face_bc.data_ptr(),
-              node_dir.data_ptr(), bn.data_ptr(), bf.data_ptr(), px.data_ptr(), py.data_ptr(), stream_ptr())
+    args = [dd.ptr, 1, pc.data_ptr(), pg.data_ptr(), pn.data_ptr(), face_bc.data_ptr(), node_dir.data_ptr(),
+            bn.data_ptr(), bf.data_ptr(), px.data_ptr(), py.data_ptr()]
+    if int(domain.dim) == 2:
+        _lib.call('mfv_face_gradient', *args, stream_ptr())
+    else:
+        pz = A.dev(Pz_face, out=True)
+        _lib.call('mfv_face_gradient_3d', *args, pz.data_ptr(), stream_ptr())
     A.finish()
 
 

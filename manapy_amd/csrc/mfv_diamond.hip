@@ -1,5 +1,5 @@
-// libmfv diamond scheme (2D): matrix assembly into ELL slices, Dirichlet lifting of the
-// right-hand side, node interpolation, diamond face gradients, the dissipative residual
+// libmfv diamond scheme (2D and 3D): matrix assembly into ELL slices, Dirichlet lifting of
+// the right-hand side, node interpolation, diamond face gradients, the dissipative residual
 // and the diffusion / advection-diffusion time step.
 //
 // Every output element is produced by one thread that gathers its terms in the order the
@@ -11,7 +11,7 @@
 namespace mfv {
 
 struct Diamond {
-    int C, F, N, MF;
+    int C, F, N, MF, dim;
     const int32_t *__restrict__ cell_faces;
     const int32_t *__restrict__ cell_nfaces;
     const uint8_t *__restrict__ order_sorted;
@@ -37,6 +37,7 @@ static int validate_diamond(const mfv_diamond *d)
     MFV_REQUIRE(d->nb_cells >= 0 && d->nb_faces >= 0 && d->nb_nodes >= 0, "negative diamond sizes");
     MFV_REQUIRE(d->max_cell_faces >= 1 && d->max_cell_faces <= 8, "max_cell_faces %d out of [1, 8]",
                 d->max_cell_faces);
+    MFV_REQUIRE(d->dim == 2 || d->dim == 3, "diamond dim %d not in {2, 3}", d->dim);
     MFV_REQUIRE(d->cell_faces && d->cell_nfaces && d->order_sorted && d->cell_volume && d->face_cells &&
                     d->face_nodes && d->face_param && d->face_fk && d->face_air && d->face_normal &&
                     d->node_cell_ptr && d->node_cell && d->node_cell_alpha && d->node_ghost_ptr &&
@@ -52,6 +53,7 @@ static Diamond to_kernel(const mfv_diamond *d)
     k.F = d->nb_faces;
     k.N = d->nb_nodes;
     k.MF = d->max_cell_faces;
+    k.dim = d->dim;
     k.cell_faces = d->cell_faces;
     k.cell_nfaces = d->cell_nfaces;
     k.order_sorted = d->order_sorted;
@@ -106,13 +108,17 @@ __device__ __forceinline__ int next_dirichlet_face(const Diamond &d, int K, int 
 // Dirichlet faces (dirichletfaces list order). Duplicate (row, col) entries are added in that
 // order, which is what summing the reference's COO triplets in insertion order gives.
 
-constexpr int kRowCap = 32;  // unique columns per row (vertex-neighbour stencil + self)
+// unique columns per row (vertex-neighbour stencil + self): a triangle's stencil stays under
+// 32; a tetrahedron's reaches about 110 on unstructured meshes
+constexpr int kRowCap2 = 32;
+constexpr int kRowCap3 = 128;
 
 struct RowAcc {
     int32_t (*col)[64];
     double (*val)[64];
     int lane;
     int n;
+    int cap;
     bool overflow;
     __device__ __forceinline__ void add(int c, double v)
     {
@@ -121,7 +127,7 @@ struct RowAcc {
                 val[j][lane] += v;
                 return;
             }
-        if (n < kRowCap) {
+        if (n < cap) {
             col[n][lane] = c;
             val[n][lane] = v;
             n++;
@@ -131,8 +137,8 @@ struct RowAcc {
     }
 };
 
-__device__ void diamond_row(const Diamond &d, int K, const uint8_t *__restrict__ node_dir,
-                            const int32_t *__restrict__ face_dir_rank, RowAcc &acc)
+__device__ void diamond_row2(const Diamond &d, int K, const uint8_t *__restrict__ node_dir,
+                             const int32_t *__restrict__ face_dir_rank, RowAcc &acc)
 {
     const int nf = d.cell_nfaces[K];
     const double volK = d.vol[K];
@@ -177,18 +183,78 @@ __device__ void diamond_row(const Diamond &d, int K, const uint8_t *__restrict__
     }
 }
 
+// get_triplet_3d (ast/functions3d.py:498-782), row K: for a matrix-inner face the left row
+// takes (K,K) -param3/|K|, the node entries and (K,R) param3/|K|; the right row takes the node
+// entries, (K,L) param3/|K| and (K,K) -param3/|K|. The face's nodes A, B, C, D (D = C on a
+// triangle) carry param1, param2, -param1, -param2. A Dirichlet face adds -param3/|K| to the
+// diagonal twice, as the reference does.
+__device__ void diamond_row3(const Diamond &d, int K, const uint8_t *__restrict__ node_dir,
+                             const int32_t *__restrict__ face_dir_rank, RowAcc &acc)
+{
+    const int nf = d.cell_nfaces[K];
+    const double volK = d.vol[K];
+    for (int jj = 0; jj < nf; jj++) {
+        const int f = sorted_face(d, K, jj);
+        const int L = d.face_cells[2 * f];
+        const int R = d.face_cells[2 * f + 1];
+        if (R < 0) continue;
+        const bool left = (L == K);
+        const double p1 = d.param[4 * f + 0], p2 = d.param[4 * f + 1], p3 = d.param[4 * f + 2];
+        const double prm[4] = {p1, p2, -1. * p1, -1. * p2};
+        if (left) acc.add(K, -1. * p3 / volK);
+        for (int s = 0; s < 4; s++) {
+            const int n = d.face_nodes[4 * f + s];
+            if (node_dir[n]) continue;
+            const double p = prm[s];
+            for (int q = d.nc_ptr[n]; q < d.nc_ptr[n + 1]; q++) {
+                const double a = d.nc_alpha[q];
+                acc.add(d.nc[q], left ? (a / volK * p) : (-1. * a / volK * p));
+            }
+            for (int q = d.ng_ptr[n]; q < d.ng_ptr[n + 1]; q++) {
+                const double a = d.ng_alpha[q];
+                acc.add(d.ng_owner[q], left ? (a / volK * p) : (-1. * a / volK * p));
+            }
+        }
+        if (left) {
+            acc.add(R, p3 / volK);
+        } else {
+            acc.add(L, p3 / volK);
+            acc.add(K, -1. * p3 / volK);
+        }
+    }
+    int rank = -1;
+    for (;;) {
+        int nr;
+        const int f = next_dirichlet_face(d, K, nf, face_dir_rank, rank, nr);
+        if (f < 0) break;
+        rank = nr;
+        acc.add(K, -1. * d.param[4 * f + 2] / volK);
+        acc.add(K, -1. * d.param[4 * f + 2] / volK);
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void diamond_row(const Diamond &d, int K, const uint8_t *__restrict__ node_dir,
+                                            const int32_t *__restrict__ face_dir_rank, RowAcc &acc)
+{
+    if (DIM == 2) diamond_row2(d, K, node_dir, face_dir_rank, acc);
+    else diamond_row3(d, K, node_dir, face_dir_rank, acc);
+}
+
 // One wave = one 64-row slice. Writes the used width of each slice (max unique columns of
-// its rows) and raises *overflow if a row exceeds kRowCap.
+// its rows) and raises *overflow if a row exceeds CAP. The 3D rows' accumulators take 96 KiB
+// of LDS (gfx950 gives a workgroup up to 160 KiB).
+template <int DIM, int CAP>
 __global__ void __launch_bounds__(64) k_diamond_width(Diamond d, const uint8_t *__restrict__ node_dir,
                                                       const int32_t *__restrict__ face_dir_rank,
                                                       int32_t *__restrict__ slice_w, int32_t *__restrict__ info)
 {
-    __shared__ int32_t s_col[kRowCap][64];
-    __shared__ double s_val[kRowCap][64];
+    __shared__ int32_t s_col[CAP][64];
+    __shared__ double s_val[CAP][64];
     const int lane = threadIdx.x;
     const int K = blockIdx.x * 64 + lane;
-    RowAcc acc{s_col, s_val, lane, 0, false};
-    if (K < d.C) diamond_row(d, K, node_dir, face_dir_rank, acc);
+    RowAcc acc{s_col, s_val, lane, 0, CAP, false};
+    if (K < d.C) diamond_row<DIM>(d, K, node_dir, face_dir_rank, acc);
     int w = acc.n;
     int ov = acc.overflow ? 1 : 0;
 #pragma unroll
@@ -207,19 +273,20 @@ __global__ void __launch_bounds__(64) k_diamond_width(Diamond d, const uint8_t *
 // columns ascending, global column = row0 + local column; slots past the row's length (up to
 // the slice width) hold (row0 + K, 0). Rows past C (padding of the last slice) are all padding
 // with diag 0.
+template <int DIM, int CAP>
 __global__ void __launch_bounds__(64) k_diamond_fill(Diamond d, const uint8_t *__restrict__ node_dir,
                                                      const int32_t *__restrict__ face_dir_rank, int row0,
                                                      int width, const int32_t *__restrict__ slice_w,
                                                      int32_t *__restrict__ cols, double *__restrict__ vals,
                                                      float *__restrict__ vals32, double *__restrict__ diag)
 {
-    __shared__ int32_t s_col[kRowCap][64];
-    __shared__ double s_val[kRowCap][64];
+    __shared__ int32_t s_col[CAP][64];
+    __shared__ double s_val[CAP][64];
     const int lane = threadIdx.x;
     const int slice = blockIdx.x;
     const int K = slice * 64 + lane;
-    RowAcc acc{s_col, s_val, lane, 0, false};
-    if (K < d.C) diamond_row(d, K, node_dir, face_dir_rank, acc);
+    RowAcc acc{s_col, s_val, lane, 0, CAP, false};
+    if (K < d.C) diamond_row<DIM>(d, K, node_dir, face_dir_rank, acc);
     const int n = acc.n;
     // insertion sort of this lane's entries by column
     for (int i = 1; i < n; i++) {
@@ -299,6 +366,51 @@ __global__ void __launch_bounds__(kBlock) k_diamond_rhs(Diamond d, const uint8_t
     rhs[K] = r;
 }
 
+// get_rhs_glob_3d (ast/functions3d.py:865-946): the Dirichlet nodes of the matrix-inner faces
+// (parameters param1, param2, -param1, -param2 on nodes A, B, C, D), then per Dirichlet face
+// its boundary nodes (centergn[node][0][3] != -1) and -2 param3/|K| V_K.
+__global__ void __launch_bounds__(kBlock) k_diamond_rhs3(Diamond d, const uint8_t *__restrict__ node_dir,
+                                                          const int32_t *__restrict__ face_dir_rank,
+                                                          const double *__restrict__ Pn,
+                                                          const double *__restrict__ Pf, double *__restrict__ rhs)
+{
+    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K >= d.C) return;
+    const int nf = d.cell_nfaces[K];
+    const double volK = d.vol[K];
+    double r = 0.;
+    for (int jj = 0; jj < nf; jj++) {
+        const int f = sorted_face(d, K, jj);
+        const int L = d.face_cells[2 * f];
+        if (d.face_cells[2 * f + 1] < 0) continue;
+        const bool left = (L == K);
+        const double p1 = d.param[4 * f + 0], p2 = d.param[4 * f + 1];
+        const double prm[4] = {p1, p2, -1. * p1, -1. * p2};
+        for (int s = 0; s < 4; s++) {
+            const int n = d.face_nodes[4 * f + s];
+            if (node_dir[n]) {
+                const double V = Pn[n];
+                r += left ? (-1. * V * prm[s] / volK) : (V * prm[s] / volK);
+            }
+        }
+    }
+    int rank = -1;
+    for (;;) {
+        int nr;
+        const int f = next_dirichlet_face(d, K, nf, face_dir_rank, rank, nr);
+        if (f < 0) break;
+        rank = nr;
+        const double p1 = d.param[4 * f + 0], p2 = d.param[4 * f + 1];
+        const double prm[4] = {p1, p2, -1. * p1, -1. * p2};
+        for (int s = 0; s < 4; s++) {
+            const int n = d.face_nodes[4 * f + s];
+            if (d.ng_ptr[n + 1] > d.ng_ptr[n]) r += -1. * Pn[n] * prm[s] / volK;
+        }
+        r += -2. * d.param[4 * f + 2] / volK * Pf[f];
+    }
+    rhs[K] = r;
+}
+
 // ---------------------------------------------------------------------------------------------
 // node values (centertovertex_2d, ast/functions2d.py:199-277)
 
@@ -360,6 +472,58 @@ __global__ void __launch_bounds__(kBlock) k_face_gradient(Diamond d, int kind, c
         wx[f] = -1 / a2 * sx;
         wy[f] = 1 / a2 * sy;
     }
+}
+
+// 3D: kind 0 = face_gradient_3d (ast/functions3d.py:122-248), kind 1 =
+// compute_P_gradient_3d_diamond (:949-1108): (n1 (V_A - V_C) + n2 (V_B - V_D) + n (V_R - V_L))
+// / airDiamond per component (negated for kind 1); face_fk holds n1 and n2 (faces.f_1, f_2).
+__global__ void __launch_bounds__(kBlock) k_face_gradient3(Diamond d, int kind, const double *__restrict__ w_c,
+                                                            const double *__restrict__ w_ghost,
+                                                            const double *__restrict__ w_node,
+                                                            const int8_t *__restrict__ face_bc,
+                                                            const uint8_t *__restrict__ node_dir,
+                                                            const double *__restrict__ Pn,
+                                                            const double *__restrict__ Pf, double *__restrict__ wx,
+                                                            double *__restrict__ wy, double *__restrict__ wz)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= d.F) return;
+    const int L = d.face_cells[2 * f];
+    const int R = d.face_cells[2 * f + 1];
+    const int bc = (R >= 0) ? 0 : face_bc[f];
+    if (R < 0 && bc == 0) return;
+    int nd[4];
+    double V[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        nd[s] = d.face_nodes[4 * f + s];
+        V[s] = w_node[nd[s]];
+    }
+    const double VL = w_c[L];
+    double VR = (R >= 0) ? w_c[R] : w_ghost[f];
+    if (kind == 1) {
+        if (bc == 2) {
+#pragma unroll
+            for (int s = 0; s < 4; s++) V[s] = Pn[nd[s]];
+            VR = 2. * Pf[f] - VL;
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                if (node_dir[nd[s]]) V[s] = Pn[nd[s]];
+        }
+    }
+    const double *k = d.fk + 6 * f;
+    const double *nr = d.normal + 3 * f;
+    const double air = d.air[f];
+    double g[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double sum = k[c] * (V[0] - V[2]) + k[3 + c] * (V[1] - V[3]) + nr[c] * (VR - VL);
+        g[c] = (kind == 0) ? sum / air : -1. * sum / air;
+    }
+    wx[f] = g[0];
+    wy[f] = g[1];
+    wz[f] = g[2];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -426,8 +590,12 @@ extern "C" int mfv_diamond_widths(const mfv_diamond *d, const uint8_t *node_dir,
     MFV_REQUIRE(node_dir && face_dir_rank && slice_w && info, "NULL array");
     const int S = (d->nb_cells + 63) / 64;
     if (S == 0) return MFV_OK;
-    hipLaunchKernelGGL(k_diamond_width, dim3(S), dim3(64), 0, as_stream(stream), to_kernel(d), node_dir,
-                       face_dir_rank, slice_w, info);
+    if (d->dim == 2)
+        hipLaunchKernelGGL((k_diamond_width<2, kRowCap2>), dim3(S), dim3(64), 0, as_stream(stream), to_kernel(d),
+                           node_dir, face_dir_rank, slice_w, info);
+    else
+        hipLaunchKernelGGL((k_diamond_width<3, kRowCap3>), dim3(S), dim3(64), 0, as_stream(stream), to_kernel(d),
+                           node_dir, face_dir_rank, slice_w, info);
     return check_launch("k_diamond_width");
 }
 
@@ -437,12 +605,17 @@ extern "C" int mfv_diamond_assemble(const mfv_diamond *d, const uint8_t *node_di
 {
     if (int rc = validate_diamond(d)) return rc;
     MFV_REQUIRE(node_dir && face_dir_rank && slice_w && cols && vals && vals32 && diag, "NULL array");
-    MFV_REQUIRE(width >= 1 && width <= kRowCap, "ELL width %d out of [1, %d]", width, kRowCap);
+    const int cap = d->dim == 2 ? kRowCap2 : kRowCap3;
+    MFV_REQUIRE(width >= 1 && width <= cap, "ELL width %d out of [1, %d]", width, cap);
     MFV_REQUIRE(row0 >= 0 && row0 % 64 == 0, "row0 %d must be a non-negative multiple of 64", row0);
     const int S = (d->nb_cells + 63) / 64;
     if (S == 0) return MFV_OK;
-    hipLaunchKernelGGL(k_diamond_fill, dim3(S), dim3(64), 0, as_stream(stream), to_kernel(d), node_dir,
-                       face_dir_rank, row0, width, slice_w, cols, vals, vals32, diag);
+    if (d->dim == 2)
+        hipLaunchKernelGGL((k_diamond_fill<2, kRowCap2>), dim3(S), dim3(64), 0, as_stream(stream), to_kernel(d),
+                           node_dir, face_dir_rank, row0, width, slice_w, cols, vals, vals32, diag);
+    else
+        hipLaunchKernelGGL((k_diamond_fill<3, kRowCap3>), dim3(S), dim3(64), 0, as_stream(stream), to_kernel(d),
+                           node_dir, face_dir_rank, row0, width, slice_w, cols, vals, vals32, diag);
     return check_launch("k_diamond_fill");
 }
 
@@ -452,8 +625,12 @@ extern "C" int mfv_diamond_rhs(const mfv_diamond *d, const uint8_t *node_dir, co
     if (int rc = validate_diamond(d)) return rc;
     if (d->nb_cells == 0) return MFV_OK;
     MFV_REQUIRE(node_dir && face_dir_rank && Pbordnode && Pbordface && rhs, "NULL array");
-    hipLaunchKernelGGL(k_diamond_rhs, dim3(grid_for(d->nb_cells)), dim3(kBlock), 0, as_stream(stream),
-                       to_kernel(d), node_dir, face_dir_rank, Pbordnode, Pbordface, rhs);
+    if (d->dim == 2)
+        hipLaunchKernelGGL(k_diamond_rhs, dim3(grid_for(d->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                           to_kernel(d), node_dir, face_dir_rank, Pbordnode, Pbordface, rhs);
+    else
+        hipLaunchKernelGGL(k_diamond_rhs3, dim3(grid_for(d->nb_cells)), dim3(kBlock), 0, as_stream(stream),
+                           to_kernel(d), node_dir, face_dir_rank, Pbordnode, Pbordface, rhs);
     return check_launch("k_diamond_rhs");
 }
 
@@ -474,6 +651,7 @@ extern "C" int mfv_face_gradient(const mfv_diamond *d, int kind, const double *w
                                  void *stream)
 {
     if (int rc = validate_diamond(d)) return rc;
+    MFV_REQUIRE(d->dim == 2, "mfv_face_gradient is the 2D scheme: use mfv_face_gradient_3d");
     MFV_REQUIRE(kind == 0 || kind == 1, "face gradient kind %d not in {0, 1}", kind);
     if (d->nb_faces == 0) return MFV_OK;
     MFV_REQUIRE(w_c && w_ghost && w_node && face_bc && wx && wy, "NULL array");
@@ -481,6 +659,22 @@ extern "C" int mfv_face_gradient(const mfv_diamond *d, int kind, const double *w
     hipLaunchKernelGGL(k_face_gradient, dim3(grid_for(d->nb_faces)), dim3(kBlock), 0, as_stream(stream),
                        to_kernel(d), kind, w_c, w_ghost, w_node, face_bc, node_dir, Pbordnode, Pbordface, wx, wy);
     return check_launch("k_face_gradient");
+}
+
+extern "C" int mfv_face_gradient_3d(const mfv_diamond *d, int kind, const double *w_c, const double *w_ghost,
+                                    const double *w_node, const int8_t *face_bc, const uint8_t *node_dir,
+                                    const double *Pbordnode, const double *Pbordface, double *wx, double *wy,
+                                    double *wz, void *stream)
+{
+    if (int rc = validate_diamond(d)) return rc;
+    MFV_REQUIRE(d->dim == 3, "mfv_face_gradient_3d needs a 3D diamond descriptor");
+    MFV_REQUIRE(kind == 0 || kind == 1, "face gradient kind %d not in {0, 1}", kind);
+    if (d->nb_faces == 0) return MFV_OK;
+    MFV_REQUIRE(w_c && w_ghost && w_node && face_bc && wx && wy && wz, "NULL array");
+    if (kind == 1) MFV_REQUIRE(node_dir && Pbordnode && Pbordface, "kind 1 needs node_dir, Pbordnode, Pbordface");
+    hipLaunchKernelGGL(k_face_gradient3, dim3(grid_for(d->nb_faces)), dim3(kBlock), 0, as_stream(stream),
+                       to_kernel(d), kind, w_c, w_ghost, w_node, face_bc, node_dir, Pbordnode, Pbordface, wx, wy, wz);
+    return check_launch("k_face_gradient3");
 }
 
 extern "C" int mfv_explicitscheme_dissipative(const mfv_mesh *m, const double *wx, const double *wy,

@@ -600,6 +600,113 @@ __global__ void __launch_bounds__(T) k_euler_step_packed_tiled(EulerTetK t, Eule
     wave_min_part(dtpart, (size_t)c0 + q, d);
 }
 
+// The packed step on face tiles with a single cell phase: the tile's items' records are staged
+// in LDS as above, then every tile cell evaluates the fluxes of its own four faces from its
+// faces' entries (geometry and item pair) and the staged records, in the residual order, and
+// updates its record. An inner face of two tile cells is evaluated by both, with the same
+// arguments, so both get the same bits as the face phase of k_euler_step_packed_tiled; the LDS
+// holds only the records (48 B per item), which lets tiles of 512 cells keep several blocks
+// per CU and cuts the halo share of the staged records.
+template <int T>
+__global__ void __launch_bounds__(T) k_euler_step_cells_tiled(EulerTetK t, EulerTilesK tt,
+                                                               const double *__restrict__ Rin,
+                                                               double *__restrict__ Rout, double gamma, double cfl,
+                                                               int quirk, DtRing ring, double *__restrict__ dtpart)
+{
+    extern __shared__ double lds[];
+    double2 *sR = reinterpret_cast<double2 *>(lds);  // [MI][3]: six values per item
+    const int tile = xcd_block();
+    const int c0 = tt.cell0[tile], nc = tt.ncells[tile];
+    const int h0 = tt.hptr[tile], ni = nc + (tt.hptr[tile + 1] - h0);
+    const int f0 = tt.fptr[tile];
+    ring_advance(ring);
+    for (int e = threadIdx.x; e < ni; e += T) {
+        const int c = (e < nc) ? c0 + e : tt.halo[h0 + (e - nc)];
+        const double2 *rp = reinterpret_cast<const double2 *>(Rin + 8 * (size_t)c);
+        const double2 a = rp[0], b = rp[1], e2 = rp[2];
+        sR[3 * e] = a;
+        sR[3 * e + 1] = b;
+        sR[3 * e + 2] = e2;
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if ((q & ~63) >= nc) return;  // the whole wave lies past the tile: it owns no partial
+    double d = 1e300;
+    if (q < nc) {
+        const int i = c0 + q;
+        const int4 c4 = tt.code[i];
+        const unsigned ord = t.cell_rank[i];
+        const double vol = Rin[8 * (size_t)i + 6];
+        const int cd[4] = {c4.x, c4.y, c4.z, c4.w};
+        double r[5] = {0., 0., 0., 0., 0.};
+        // the next face's entry is loaded while the current flux is evaluated
+        size_t ent = (size_t)(f0 + (cd[0] >> 1));
+        double2 n01 = tt.fgeo[2 * ent], n23 = tt.fgeo[2 * ent + 1];
+        int2 nlr = tt.flr[ent];
+#pragma unroll 1
+        for (int k = 0; k < 4; k++) {  // faces in increasing original id
+            const bool own_R = cd[k] & 1;
+            const double2 g01 = n01, g23 = n23;
+            const int2 lr = nlr;
+            if (k < 3) {
+                ent = (size_t)(f0 + (cd[k + 1] >> 1));
+                n01 = tt.fgeo[2 * ent];
+                n23 = tt.fgeo[2 * ent + 1];
+                nlr = tt.flr[ent];
+            }
+            const double nrm[3] = {g01.x, g01.y, g23.x};
+            const double mesu = g23.y;
+            const double n[3] = {nrm[0] / mesu, nrm[1] / mesu, nrm[2] / mesu};
+            const double2 la = sR[3 * lr.x], lb = sR[3 * lr.x + 1], lc = sR[3 * lr.x + 2];
+            const Prim6 L{la.x, la.y, lb.x, lb.y, lc.x, lc.y};
+            Prim6 R;
+            if (lr.y >= 0) {
+                const double2 ra = sR[3 * lr.y], rb = sR[3 * lr.y + 1], rc = sR[3 * lr.y + 2];
+                R = Prim6{ra.x, ra.y, rb.x, rb.y, rc.x, rc.y};
+            } else {
+                R = euler_ghost(L, nrm, -lr.y, quirk, 0.);
+            }
+            double fl[5];
+            euler_flux(L, R, n, mesu, gamma, quirk, fl);
+            // rez -= fl (own = L) is rez += -fl
+#pragma unroll
+            for (int v = 0; v < 5; v++) r[v] += own_R ? fl[v] : -fl[v];
+        }
+        const double2 oa = sR[3 * q], ob = sR[3 * q + 1], oc = sR[3 * q + 2];
+        double u[6] = {oa.x, oa.y, ob.x, ob.y, oc.x, oc.y};
+        euler_update_cell(u, r, gamma, ring.cur[0], vol);
+        double2 *wp = reinterpret_cast<double2 *>(Rout + 8 * (size_t)i);
+        wp[0] = make_double2(u[0], u[1]);
+        wp[1] = make_double2(u[2], u[3]);
+        wp[2] = make_double2(u[4], u[5]);
+        wp[3] = make_double2(vol, 0.);
+        // time_step_euler_3d of the new state: the terms in residual order, summed in slot order
+        const double velson = sqrt(gamma * fabs(u[1] / u[0]));
+        double tm[4];
+        int sl[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {  // the faces' geometry again (in cache since the flux loop)
+            const size_t ent = (size_t)(f0 + (cd[k] >> 1));
+            const double2 g01 = tt.fgeo[2 * ent], g23 = tt.fgeo[2 * ent + 1];
+            const double nx = g01.x, ny = g01.y, nz = g23.x;
+            const double u_n = fabs((u[2] * nx + u[3] * ny + u[4] * nz) / u[0]) / sqrt(nx * nx + ny * ny + nz * nz);
+            const double lam_convect = u_n + velson;
+            tm[k] = lam_convect * g23.y;
+            sl[k] = (ord >> (2 * k)) & 3;
+        }
+        double lam = 0.;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double tj = tm[0];
+#pragma unroll
+            for (int k = 1; k < 4; k++) tj = sl[k] == j ? tm[k] : tj;
+            lam += tj;
+        }
+        d = cfl * vol / lam;
+    }
+    wave_min_part(dtpart, (size_t)c0 + q, d);
+}
+
 // ---------------------------------------------------------------------------------------------
 // FP32-bulk step: float32 records [C][8] = (rho, P, rhou, rhov, rhow, rhoE, |K|, 0) and float32
 // face geometry per tile entry (nx, ny, nz, |f|); float32 face fluxes (pressure of the momentum
@@ -1345,7 +1452,8 @@ extern "C" int mfv_euler3d_step_packed(const mfv_mesh *m, const mfv_euler_tet *t
     return dt_finish(part, nw, r.next, st);
 }
 
-static int check_euler_tiles(const mfv_mesh *m, const mfv_euler_tet_tiles *tt)
+// item_bytes / face_bytes: the LDS a kernel takes per staged item and per face entry
+static int check_euler_tiles(const mfv_mesh *m, const mfv_euler_tet_tiles *tt, size_t item_bytes, size_t face_bytes)
 {
     MFV_REQUIRE(tt != nullptr, "packed Euler tiles are NULL");
     MFV_REQUIRE(tt->tile_size == 128 || tt->tile_size == 256 || tt->tile_size == 512,
@@ -1360,20 +1468,20 @@ static int check_euler_tiles(const mfv_mesh *m, const mfv_euler_tet_tiles *tt)
     MFV_REQUIRE(((uintptr_t)tt->face_geo & 15) == 0 && ((uintptr_t)tt->face_lr & 7) == 0 &&
                     ((uintptr_t)tt->cell_code & 15) == 0,
                 "packed Euler tiles are misaligned");
-    const size_t lds = 48 * (size_t)tt->max_items + 40 * (size_t)tt->max_faces;
+    const size_t lds = item_bytes * (size_t)tt->max_items + face_bytes * (size_t)tt->max_faces;
     MFV_REQUIRE(lds <= 64 * 1024, "packed Euler tiles of %d cells need %zu bytes of LDS (> 64 KiB): use smaller "
                 "tiles or renumber the cells", tt->tile_size, lds);
     return MFV_OK;
 }
 
-extern "C" int mfv_euler3d_step_packed_tiled(const mfv_mesh *m, const mfv_euler_tet *t, const mfv_euler_tet_tiles *tt,
-                                             const double *rec_in, double *rec_out, double gamma, double cfl,
-                                             int quirk, double *dt_ring_p, int step, void *stream)
+static int euler_step_packed_tiled(bool cells, const mfv_mesh *m, const mfv_euler_tet *t,
+                                   const mfv_euler_tet_tiles *tt, const double *rec_in, double *rec_out, double gamma,
+                                   double cfl, int quirk, double *dt_ring_p, int step, void *stream)
 {
     if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
     MFV_REQUIRE(m->dim == 3, "the packed Euler step needs a 3D mesh");
     if (int rc = check_euler_tet(m, t)) return rc;
-    if (int rc = check_euler_tiles(m, tt)) return rc;
+    if (int rc = check_euler_tiles(m, tt, 48, cells ? 0 : 40)) return rc;
     if (m->nb_cells == 0) return MFV_OK;
     MFV_REQUIRE(rec_in && rec_out && rec_in != rec_out && dt_ring_p, "NULL or aliased array");
     MFV_REQUIRE(((uintptr_t)rec_in & 15) == 0 && ((uintptr_t)rec_out & 15) == 0, "misaligned records");
@@ -1395,23 +1503,41 @@ extern "C" int mfv_euler3d_step_packed_tiled(const mfv_mesh *m, const mfv_euler_
                          reinterpret_cast<const int4 *>(tt->cell_code),
                          tt->max_items,
                          tt->max_faces};
-    const size_t lds = 48 * (size_t)tt->max_items + 40 * (size_t)tt->max_faces;
+    const size_t lds = 48 * (size_t)tt->max_items + (cells ? 0 : 40 * (size_t)tt->max_faces);
+#define MFV_EULER_TILED(K, TS)                                                                                        \
+    hipLaunchKernelGGL(K<TS>, dim3(tt->nb_tiles), dim3(TS), lds, st, tk, ek, rec_in, rec_out, gamma, cfl, quirk, r, \
+                       part)
     switch (tt->tile_size) {
     case 128:
-        hipLaunchKernelGGL(k_euler_step_packed_tiled<128>, dim3(tt->nb_tiles), dim3(128), lds, st, tk, ek, rec_in,
-                           rec_out, gamma, cfl, quirk, r, part);
+        if (cells) MFV_EULER_TILED(k_euler_step_cells_tiled, 128);
+        else MFV_EULER_TILED(k_euler_step_packed_tiled, 128);
         break;
     case 256:
-        hipLaunchKernelGGL(k_euler_step_packed_tiled<256>, dim3(tt->nb_tiles), dim3(256), lds, st, tk, ek, rec_in,
-                           rec_out, gamma, cfl, quirk, r, part);
+        if (cells) MFV_EULER_TILED(k_euler_step_cells_tiled, 256);
+        else MFV_EULER_TILED(k_euler_step_packed_tiled, 256);
         break;
     default:
-        hipLaunchKernelGGL(k_euler_step_packed_tiled<512>, dim3(tt->nb_tiles), dim3(512), lds, st, tk, ek, rec_in,
-                           rec_out, gamma, cfl, quirk, r, part);
+        if (cells) MFV_EULER_TILED(k_euler_step_cells_tiled, 512);
+        else MFV_EULER_TILED(k_euler_step_packed_tiled, 512);
         break;
     }
-    if (int rc = check_launch("k_euler_step_packed_tiled")) return rc;
+#undef MFV_EULER_TILED
+    if (int rc = check_launch(cells ? "k_euler_step_cells_tiled" : "k_euler_step_packed_tiled")) return rc;
     return dt_finish(part, nw, r.next, st);
+}
+
+extern "C" int mfv_euler3d_step_packed_tiled(const mfv_mesh *m, const mfv_euler_tet *t, const mfv_euler_tet_tiles *tt,
+                                             const double *rec_in, double *rec_out, double gamma, double cfl,
+                                             int quirk, double *dt_ring_p, int step, void *stream)
+{
+    return euler_step_packed_tiled(false, m, t, tt, rec_in, rec_out, gamma, cfl, quirk, dt_ring_p, step, stream);
+}
+
+extern "C" int mfv_euler3d_step_cells_tiled(const mfv_mesh *m, const mfv_euler_tet *t, const mfv_euler_tet_tiles *tt,
+                                            const double *rec_in, double *rec_out, double gamma, double cfl,
+                                            int quirk, double *dt_ring_p, int step, void *stream)
+{
+    return euler_step_packed_tiled(true, m, t, tt, rec_in, rec_out, gamma, cfl, quirk, dt_ring_p, step, stream);
 }
 
 extern "C" int mfv_euler32_pack(const mfv_mesh *m, const double *U, float *rec, void *stream)
@@ -1454,7 +1580,7 @@ extern "C" int mfv_euler3d_step32_tiled(const mfv_mesh *m, const mfv_euler_tet *
     if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
     MFV_REQUIRE(m->dim == 3, "the Euler step needs a 3D mesh");
     if (int rc = check_euler_tet(m, t)) return rc;
-    if (int rc = check_euler_tiles(m, tt)) return rc;
+    if (int rc = check_euler_tiles(m, tt, 24, 24)) return rc;
     if (m->nb_cells == 0) return MFV_OK;
     MFV_REQUIRE(geo32 && rec_in && rec_out && rec_in != rec_out && dt_ring_p, "NULL or aliased array");
     MFV_REQUIRE(((uintptr_t)rec_in & 15) == 0 && ((uintptr_t)rec_out & 15) == 0 && ((uintptr_t)geo32 & 15) == 0,

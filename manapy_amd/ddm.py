@@ -1137,6 +1137,8 @@ class Domain(object):
     outnodes = property(lambda self: self._outnodes)
     bottomnodes = property(lambda self: self._bottomnodes)
     uppernodes = property(lambda self: self._uppernodes)
+    frontnodes = property(lambda self: self._frontnodes)
+    backnodes = property(lambda self: self._backnodes)
     halonodes = property(lambda self: self._halonodes)
     boundaryfaces = property(lambda self: self._boundaryfaces)
     boundarynodes = property(lambda self: self._boundarynodes)

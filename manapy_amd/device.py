@@ -633,8 +633,10 @@ class EulerTetTiles(object):
     def ptr(self):
         return ctypes.byref(self.desc)
 
-    def lds_bytes(self):
-        return 48 * self.host['max_items'] + 40 * self.host['max_faces']
+    def lds_bytes(self, cells=False):
+        """LDS of a block: the staged records, plus the face fluxes unless cells (the
+        single-phase step, mfv_euler3d_step_cells_tiled)."""
+        return 48 * self.host['max_items'] + (0 if cells else 40 * self.host['max_faces'])
 
 
 class Euler32(object):

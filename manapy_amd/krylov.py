@@ -62,8 +62,8 @@ class RangeAllocator(object):
 class EllBatch(object):
     def __init__(self, max_slices, max_inst=64, width=32, device=None):
         torch = require_gpu()
-        if width < 1 or width > 32:
-            raise ValueError("ELL width must be in [1, 32]")
+        if width < 1 or width > 128:  # the diamond assembly holds 32 columns per row in 2D, 128 in 3D
+            raise ValueError("ELL width must be in [1, 128]")
         self.S, self.I, self.W = int(max_slices), int(max_inst), int(width)
         self.device = torch.device(device if device is not None else 'cuda')
         S, I, W = self.S, self.I, self.W
@@ -169,7 +169,7 @@ class EllBatch(object):
     # -- matrix and system ------------------------------------------------------------------------
     def assemble_diamond(self, inst, ddia, node_dir, face_dir_rank):
         """Writes the diamond matrix of a DeviceDiamond into the instance's slices
-        (get_triplet_2d, ast/functions2d.py:328-580)."""
+        (get_triplet_2d, ast/functions2d.py:328-580; get_triplet_3d, ast/functions3d.py:498-782)."""
         import torch
         row0, n, ns = self.rows(inst)
         if n != ddia.dmesh.C:

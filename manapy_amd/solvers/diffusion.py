@@ -3,9 +3,9 @@
 ``DiffusionSolver`` (``solvers/diffusion/system.py:19-143``) and
 ``AdvectionDiffusionSolver`` (``solvers/advecdiff/system.py:19-175``) keep the
 reference's interface (``stepper`` / ``compute_fluxes`` / ``compute_new_val``); each
-call is a libmfv kernel: node interpolation (``centertovertex_2d``), face gradient
-(``face_gradient_2d``), the dissipative residual (``explicitscheme_dissipative``), the
-convective residual (``explicitscheme_convective_2d``) and the time steps of
+call is a libmfv kernel: node interpolation (``centertovertex_2d`` / ``_3d``), face gradient
+(``face_gradient_2d`` / ``_3d``), the dissipative residual (``explicitscheme_dissipative``),
+the convective residual (``explicitscheme_convective_2d`` / ``_3d``) and the time steps of
 ``solvers/diffusion/fvm_utils.py:30-49`` / ``solvers/advecdiff/fvm_utils.py:317-340``.
 """
 import ctypes
@@ -42,8 +42,6 @@ class DiffusionSolver(object):
         self.comm = var.comm
         self.domain = var.domain
         self.dim = var.dim
-        if int(self.dim) != 2:
-            raise NotImplementedError("the diamond face gradient is implemented for 2D meshes")
         self.u, self.v = vel[0], vel[1]
         if len(vel) == 3 and vel[2] is not None:
             if not isinstance(vel[2], Variable):

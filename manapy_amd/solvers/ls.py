@@ -1,9 +1,9 @@
 """Diamond-scheme linear systems (``solvers/ls/system.py``) solved on the GPU.
 
 ``LinearSolver`` mirrors the reference base class (``:36-234``): ``update_ghost_values``
-(Dirichlet data -> ``Pbordnode`` / ``Pbordface``), ``assembly`` (``get_triplet_2d`` +
-``get_rhs_glob_2d``) and ``compute_Sol_gradient`` (``compute_P_gradient_2d_diamond``),
-all as libmfv kernels. ``HIPKrylovSolver.__call__(rhs=None)`` solves
+(Dirichlet data -> ``Pbordnode`` / ``Pbordface``), ``assembly`` (``get_triplet_{2,3}d`` +
+``get_rhs_glob_{2,3}d``) and ``compute_Sol_gradient`` (``compute_P_gradient_{2,3}d_diamond``),
+all as libmfv kernels, on 2D and 3D meshes. ``HIPKrylovSolver.__call__(rhs=None)`` solves
 ``A x = rhs0 + rhs`` like ``ScipySolver.__call__`` (``:699-718``) and stores ``x`` in
 ``var.cell``; the factorisation is replaced by FP32 BiCGStab inside FP64 iterative
 refinement down to ``||b - A x|| <= tol ||b||`` (default 1e-10).
@@ -32,6 +32,21 @@ def compute_2dmatrix_size(domain, BCdirichlet, matrixinnerfaces, dirichletfaces)
     return int(4 * len(mif) + per_node[fn[mif, 0]].sum() + per_node[fn[mif, 1]].sum() + 2 * len(dirichletfaces))
 
 
+def compute_3dmatrix_size(domain, BCdirichlet, matrixinnerfaces, dirichletfaces):
+    """Number of COO triplets of get_triplet_3d (``ast/functions3d.py:410-495``): the four node
+    slots of a face (a triangle's third node twice) each count their cells and ghost centres
+    twice unless the node is a Dirichlet node."""
+    from ..diamond import face_nodes4
+    fn = face_nodes4(domain)
+    on = domain.nodes.oldname
+    ncell = domain.nodes.cellid[:, -1].astype(np.int64)
+    nghost = (domain.nodes.ghostcenter[:, :, -1] != -1).sum(axis=1).astype(np.int64)
+    free = ~np.isin(on, np.asarray(BCdirichlet, dtype=np.int64))
+    per_node = np.where(free, 2 * ncell + 2 * nghost, 0)
+    mif = np.asarray(matrixinnerfaces, dtype=np.int64)
+    return int(4 * len(mif) + sum(per_node[fn[mif, k]].sum() for k in range(4)) + 2 * len(dirichletfaces))
+
+
 class LinearSolver(object):
     _parameters = [('scheme', 'str', 'diamond', 'fv4', 'scheme diamond or fv4.'),
                    ('reordering', 'bool', False, False, 'reordering the matrix only in serial case.')]
@@ -52,14 +67,15 @@ class LinearSolver(object):
         self.var = var
         self.domain = domain
         self.dim = int(domain.dim)
-        if self.dim != 2:
-            raise NotImplementedError("the diamond scheme is implemented for 2D meshes")
+        if self.dim not in (2, 3):
+            raise ValueError("dim must be 2 or 3")
         self.comm = comm
         self.localsize = int(domain.nbcells)
         self.globalsize = self.localsize
         self.backend = domain.backend
         self.matrixinnerfaces = np.sort(np.asarray(domain.innerfaces, dtype=np.int32))
-        self.dataSize = compute_2dmatrix_size(domain, var.BCdirichlet, self.matrixinnerfaces, var.dirichletfaces)
+        size = compute_2dmatrix_size if self.dim == 2 else compute_3dmatrix_size
+        self.dataSize = size(domain, var.BCdirichlet, self.matrixinnerfaces, var.dirichletfaces)
         import torch
         dm = domain.device_mesh
         self._dev = dm.device
@@ -84,11 +100,14 @@ class LinearSolver(object):
                 self.domain.Pbordnode[n] = 1.
 
     def assembly(self):
-        """get_triplet_2d (into the ELL arena) + get_rhs_glob_2d (``:154-171``)."""
+        """get_triplet_{2,3}d (into the ELL arena) + get_rhs_glob_{2,3}d (``:154-171``)."""
         dd = self.domain.device_diamond
         node_dir, face_dir_rank, _ = self.var.diamond_bc()
         if self._batch is None:
-            self._batch = EllBatch((self.localsize + 63) // 64, max_inst=1, device=self._dev)
+            # the arena is as wide as the widest row (a triangle's stencil fits 32 columns, a
+            # tetrahedron's takes up to about 110)
+            W = 32 if self.dim == 2 else dd.width(node_dir, face_dir_rank)
+            self._batch = EllBatch((self.localsize + 63) // 64, max_inst=1, width=W, device=self._dev)
             self._inst = self._batch.alloc(self.localsize)
             self._batch.publish()
         self.width = self._batch.assemble_diamond(self._inst, dd, node_dir, face_dir_rank)
@@ -97,14 +116,18 @@ class LinearSolver(object):
                   stream_ptr())
 
     def compute_Sol_gradient(self):
-        """``compute_P_gradient_2d_diamond`` into var.gradfacex / gradfacey (``:191-201``);
-        var.node must hold the node interpolation of var.cell."""
+        """``compute_P_gradient_{2,3}d_diamond`` into var.gradfacex / gradfacey (/ gradfacez)
+        (``:191-201``); var.node must hold the node interpolation of var.cell."""
         dd = self.domain.device_diamond
         node_dir, _, face_bc = self.var.diamond_bc()
         v = self.var
-        _lib.call('mfv_face_gradient', dd.ptr, 1, v.cell.data_ptr(), v.ghost.data_ptr(), v.node.data_ptr(),
-                  face_bc.data_ptr(), node_dir.data_ptr(), self.domain.Pbordnode.data_ptr(),
-                  self.domain.Pbordface.data_ptr(), v.gradfacex.data_ptr(), v.gradfacey.data_ptr(), stream_ptr())
+        args = [dd.ptr, 1, v.cell.data_ptr(), v.ghost.data_ptr(), v.node.data_ptr(), face_bc.data_ptr(),
+                node_dir.data_ptr(), self.domain.Pbordnode.data_ptr(), self.domain.Pbordface.data_ptr(),
+                v.gradfacex.data_ptr(), v.gradfacey.data_ptr()]
+        if self.dim == 2:
+            _lib.call('mfv_face_gradient', *args, stream_ptr())
+        else:
+            _lib.call('mfv_face_gradient_3d', *args, v.gradfacez.data_ptr(), stream_ptr())
 
     def matrix_coo(self):
         """The assembled matrix as host (data, row, col), duplicates summed (for checks)."""

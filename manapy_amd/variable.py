@@ -38,6 +38,8 @@ class Boundary(object):
             faces.update({"front": domain.frontfaces, "back": domain.backfaces})
         nodes = {"in": domain.innodes, "out": domain.outnodes, "bottom": domain.bottomnodes,
                  "upper": domain.uppernodes}
+        if int(domain.dim) == 3:
+            nodes.update({"front": domain.frontnodes, "back": domain.backnodes})
         self._BCfaces = np.asarray(faces[BCloc], dtype=np.int32)
         self.BCvaluenode = None
         self._BCnodes = np.asarray(nodes.get(BCloc, np.zeros(0, np.int32)), dtype=np.int32)
@@ -190,19 +192,24 @@ class Variable(object):
         _lib.call('mfv_facetocell', dm.ptr, self.face.data_ptr(), self.cell.data_ptr(), stream_ptr())
 
     def interpolate_celltonode(self):
-        """``ast/core.py:429-440`` -> ``centertovertex_2d`` (pseudo-Laplacian weights)."""
+        """``ast/core.py:429-440`` -> ``centertovertex_2d`` / ``centertovertex_3d``
+        (pseudo-Laplacian weights)."""
         dd = self._domain.device_diamond
         _lib.call('mfv_centertovertex', dd.ptr, self.cell.data_ptr(), self.ghost.data_ptr(), self.node.data_ptr(),
                   stream_ptr())
 
     def compute_face_gradient(self):
-        """``ast/core.py:480-490`` -> ``face_gradient_2d`` on the inner, Dirichlet and Neumann
-        faces (uses ``node``: call interpolate_celltonode first, as the reference does)."""
+        """``ast/core.py:480-490`` -> ``face_gradient_2d`` / ``face_gradient_3d`` on the inner,
+        Dirichlet and Neumann faces (uses ``node``: call interpolate_celltonode first, as the
+        reference does)."""
         dd = self._domain.device_diamond
         node_dir, _, face_bc = self.diamond_bc()
-        _lib.call('mfv_face_gradient', dd.ptr, 0, self.cell.data_ptr(), self.ghost.data_ptr(), self.node.data_ptr(),
-                  face_bc.data_ptr(), None, None, None, self.gradfacex.data_ptr(), self.gradfacey.data_ptr(),
-                  stream_ptr())
+        args = [dd.ptr, 0, self.cell.data_ptr(), self.ghost.data_ptr(), self.node.data_ptr(), face_bc.data_ptr(),
+                None, None, None, self.gradfacex.data_ptr(), self.gradfacey.data_ptr()]
+        if int(self._dim) == 2:
+            _lib.call('mfv_face_gradient', *args, stream_ptr())
+        else:
+            _lib.call('mfv_face_gradient_3d', *args, self.gradfacez.data_ptr(), stream_ptr())
 
     def compute_cell_gradient(self):
         """``ast/core.py:442-478``: least-squares gradient then Barth-Jespersen limiter."""

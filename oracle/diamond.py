@@ -14,6 +14,8 @@ Follows, loop for loop and in insertion order:
   * insertion-order duplicate sum of the COO triplets (what summing them one by one gives)
 One subdomain, no periodic faces: the halo/periodic loops are empty.
 
+The 3D functions at the end of the file follow functions3d.py the same way.
+
 Pinned in tests/test_diamond.py: the assembled operator is exact on affine fields
 (the diamond scheme's defining property), rows of the pure-Neumann operator sum to zero,
 the Dirichlet problem with affine data is solved exactly by SuperLU (the reference
@@ -99,8 +101,10 @@ def update_ghost_values(d, bcs, values):
     from Dirichlet values (constants or lambdas of x, y, z), 1.0 on Neumann nodes."""
     Pbordnode = np.zeros(int(d.nbnodes))
     Pbordface = np.zeros(int(d.nbfaces))
-    index = {"in": 1, "out": 2, "upper": 3, "bottom": 4}
+    index = {"in": 1, "out": 2, "upper": 3, "bottom": 4, "front": 5, "back": 6}
     faces = {"in": d.infaces, "out": d.outfaces, "upper": d.upperfaces, "bottom": d.bottomfaces}
+    if int(d.dim) == 3:
+        faces.update({"front": d.frontfaces, "back": d.backfaces})
     for loc, bct in bcs.items():
         nodes = np.nonzero(d.nodes.oldname == index[loc])[0]
         if bct == "dirichlet":
@@ -188,12 +192,14 @@ def face_gradient_2d(d, w_c, w_ghost, w_node, faces_boundary):
     return wx, wy
 
 
-def explicitscheme_dissipative(d, wx, wy, Dxx, Dyy):
+def explicitscheme_dissipative(d, wx, wy, Dxx, Dyy, wz=None, Dzz=0.):
+    """wz = None: the 2D call (zero z gradient, Dzz = 0)."""
     dis = np.zeros(int(d.nbcells))
     cid = d.faces.cellid
     nrm = d.faces.normal
+    wz = np.zeros(int(d.nbfaces)) if wz is None else wz
     for i in range(int(d.nbfaces)):
-        q = Dxx * wx[i] * nrm[i][0] + Dyy * wy[i] * nrm[i][1] + 0. * 0. * nrm[i][2]
+        q = Dxx * wx[i] * nrm[i][0] + Dyy * wy[i] * nrm[i][1] + Dzz * wz[i] * nrm[i][2]
         if d.faces.name[i] == 0:
             dis[cid[i][0]] += q
             dis[cid[i][1]] -= q
@@ -202,7 +208,7 @@ def explicitscheme_dissipative(d, wx, wy, Dxx, Dyy):
     return dis
 
 
-def time_step_diffusion(d, Dxx, Dyy, cfl):
+def time_step_diffusion(d, Dxx, Dyy, cfl, Dzz=0.):
     """``solvers/diffusion/fvm_utils.py:30-49`` (2D: Dzz = 0)."""
     dt = 1e6
     nrm = d.faces.normal
@@ -211,7 +217,7 @@ def time_step_diffusion(d, Dxx, Dyy, cfl):
         for j in range(d.cells.faceid[i][-1]):
             n = nrm[d.cells.faceid[i][j]]
             mes = np.sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2])
-            lam += (Dxx * (mes * mes) + Dyy * (mes * mes) + 0. * (mes * mes)) / d.cells.volume[i]
+            lam += (Dxx * (mes * mes) + Dyy * (mes * mes) + Dzz * (mes * mes)) / d.cells.volume[i]
         dt = min(dt, cfl * d.cells.volume[i] / lam)
     return dt
 
@@ -268,4 +274,172 @@ def coo_sum_in_order(data, row, col):
     for v, r, c in zip(data.tolist(), row.tolist(), col.tolist()):
         k = (r, c)
         out[k] = out[k] + v if k in out else v
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# 3D diamond scheme (one subdomain: the halo, haloghost and periodic loops are empty)
+#   * compute_3dmatrix_size        ast/functions3d.py:410-495
+#   * get_triplet_3d               ast/functions3d.py:498-782
+#   * get_rhs_glob_3d              ast/functions3d.py:865-946
+#   * centertovertex_3d            ast/functions3d.py:251-362
+#   * face_gradient_3d             ast/functions3d.py:122-248
+#   * compute_P_gradient_3d_diamond  ast/functions3d.py:949-1108
+# A face has 3 or 4 nodes; a triangle's fourth node is its third (nodes[3] = nodeidf[i][2]).
+
+def _alpha3(d, nod, cx, cy, cz):
+    n = d.nodes
+    xdiff = cx - n.vertex[nod][0]
+    ydiff = cy - n.vertex[nod][1]
+    zdiff = cz - n.vertex[nod][2]
+    return (1. + n.lambda_x[nod] * xdiff + n.lambda_y[nod] * ydiff + n.lambda_z[nod] * zdiff) / (
+        n.number[nod] + n.lambda_x[nod] * n.R_x[nod] + n.lambda_y[nod] * n.R_y[nod] + n.lambda_z[nod] * n.R_z[nod])
+
+
+def face_nodes4(d, i):
+    fn = d.faces.nodeid[i]
+    return [int(fn[0]), int(fn[1]), int(fn[2]), int(fn[3]) if fn[-1] == 4 else int(fn[2])]
+
+
+def compute_3dmatrix_size(d, BCdirichlet, mif, dirichletfaces):
+    ncell = d.nodes.cellid
+    gcn = d.nodes.ghostcenter
+    cmpt = 0
+    for i in mif:
+        cmpt += 1
+        for nod in face_nodes4(d, i):
+            if d.nodes.oldname[nod] not in BCdirichlet:
+                cmpt += 2 * ncell[nod][-1]
+                cmpt += 2 * int(np.sum(gcn[nod][:, -1] != -1))
+        cmpt += 3
+    cmpt += 2 * len(dirichletfaces)
+    return cmpt
+
+
+def get_triplet_3d(d, BCdirichlet, mif, dirichletfaces):
+    """COO entries (data, row, col) in the reference's insertion order."""
+    vol = d.cells.volume
+    cid = d.faces.cellid
+    ncell = d.nodes.cellid
+    gcn = d.nodes.ghostcenter
+    cc = d.cells.center
+    p1, p2, p3 = d.faces.param1, d.faces.param2, d.faces.param3
+    data, row, col = [], [], []
+
+    def add(r, c, v):
+        row.append(r)
+        col.append(c)
+        data.append(v)
+
+    for i in mif:
+        cl, cr = cid[i][0], cid[i][1]
+        params = (p1[i], p2[i], -1. * p1[i], -1. * p2[i])
+        add(cl, cl, -1 * p3[i] / vol[cl])
+        for cmptparam, nod in enumerate(face_nodes4(d, i)):
+            if d.nodes.oldname[nod] not in BCdirichlet:
+                for j in range(ncell[nod][-1]):
+                    c = ncell[nod][j]
+                    a = _alpha3(d, nod, cc[c][0], cc[c][1], cc[c][2])
+                    add(cl, c, a / vol[cl] * params[cmptparam])
+                    add(cr, c, -1. * a / vol[cr] * params[cmptparam])
+                for j in range(len(gcn[nod])):
+                    if gcn[nod][j][-1] != -1:
+                        a = _alpha3(d, nod, gcn[nod][j][0], gcn[nod][j][1], gcn[nod][j][2])
+                        index = np.int32(gcn[nod][j][3])
+                        add(cl, index, a / vol[cl] * params[cmptparam])
+                        add(cr, index, -1. * a / vol[cr] * params[cmptparam])
+        add(cl, cr, p3[i] / vol[cl])
+        add(cr, cl, p3[i] / vol[cr])
+        add(cr, cr, -1. * p3[i] / vol[cr])
+    for i in dirichletfaces:
+        cl = cid[i][0]
+        add(cl, cl, -1 * p3[i] / vol[cl])
+        add(cl, cl, -1. * p3[i] / vol[cl])
+    return np.array(data), np.array(row, dtype=np.int32), np.array(col, dtype=np.int32)
+
+
+def get_rhs_glob_3d(d, Pbordnode, Pbordface, BCdirichlet, mif, dirichletfaces):
+    vol = d.cells.volume
+    cid = d.faces.cellid
+    on = d.nodes.oldname
+    gcn = d.nodes.ghostcenter
+    p1, p2, p3 = d.faces.param1, d.faces.param2, d.faces.param3
+    rhs = np.zeros(int(d.nbcells))
+    for i in mif:
+        cl, cr = cid[i][0], cid[i][1]
+        params = (p1[i], p2[i], -1. * p1[i], -1. * p2[i])
+        for cmpt, nod in enumerate(face_nodes4(d, i)):
+            if on[nod] in BCdirichlet:
+                V = Pbordnode[nod]
+                rhs[cl] += -1. * V * params[cmpt] / vol[cl]
+                rhs[cr] += V * params[cmpt] / vol[cr]
+    for i in dirichletfaces:
+        cl = cid[i][0]
+        params = (p1[i], p2[i], -1. * p1[i], -1. * p2[i])
+        for cmpt, nod in enumerate(face_nodes4(d, i)):
+            if gcn[nod][0][3] != -1:
+                V = Pbordnode[nod]
+                rhs[cl] += -1. * V * params[cmpt] / vol[cl]
+        rhs[cl] += -2. * p3[i] / vol[cl] * Pbordface[i]
+    return rhs
+
+
+def centertovertex_3d(d, w_c, w_ghost):
+    n = d.nodes
+    cc = d.cells.center
+    gc = d.faces.ghostcenter
+    w_n = np.zeros(int(d.nbnodes))
+    for i in range(int(d.nbnodes)):
+        for j in range(n.cellid[i][-1]):
+            c = n.cellid[i][j]
+            w_n[i] += _alpha3(d, i, cc[c][0], cc[c][1], cc[c][2]) * w_c[c]
+        for j in range(n.ghostid[i][-1]):
+            f = n.ghostid[i][j]
+            w_n[i] += _alpha3(d, i, gc[f][0], gc[f][1], gc[f][2]) * w_ghost[f]
+    return w_n
+
+
+def face_gradient_3d(d, w_c, w_ghost, w_node, faces_boundary):
+    """Inner faces and the given boundary faces (dirichlet + neumann lists)."""
+    F = int(d.nbfaces)
+    out = (np.zeros(F), np.zeros(F), np.zeros(F))
+    cid = d.faces.cellid
+    f1, f2, nrm, air = d.faces.f_1, d.faces.f_2, d.faces.normal, d.faces.airDiamond
+
+    def one(i, VR):
+        VA, VB, VC, VD = (w_node[k] for k in face_nodes4(d, i))
+        VL = w_c[cid[i][0]]
+        for k in range(3):
+            out[k][i] = (f1[i][k] * (VA - VC) + f2[i][k] * (VB - VD) + nrm[i][k] * (VR - VL)) / air[i]
+
+    for i in d.innerfaces:
+        one(i, w_c[cid[i][1]])
+    for i in faces_boundary:
+        one(i, w_ghost[i])
+    return out
+
+
+def p_gradient_3d(d, P_c, P_ghost, P_node, Pbordnode, Pbordface, BCdirichlet, neumannfaces, dirichletfaces):
+    """``compute_P_gradient_3d_diamond``: inner faces, Neumann faces (ghost values), Dirichlet
+    faces (the four Dirichlet node values, mirror 2 V_K - P_c)."""
+    F = int(d.nbfaces)
+    out = (np.zeros(F), np.zeros(F), np.zeros(F))
+    cid = d.faces.cellid
+    on = d.nodes.oldname
+    f1, f2, nrm, air = d.faces.f_1, d.faces.f_2, d.faces.normal, d.faces.airDiamond
+
+    def nodeval(n):
+        return Pbordnode[n] if on[n] in BCdirichlet else P_node[n]
+
+    def one(i, VA, VB, VC, VD, VL, VR):
+        for k in range(3):
+            out[k][i] = -1. * (f1[i][k] * (VA - VC) + f2[i][k] * (VB - VD) + nrm[i][k] * (VR - VL)) / air[i]
+
+    for i in d.innerfaces:
+        one(i, *[nodeval(k) for k in face_nodes4(d, i)], P_c[cid[i][0]], P_c[cid[i][1]])
+    for i in neumannfaces:
+        one(i, *[nodeval(k) for k in face_nodes4(d, i)], P_c[cid[i][0]], P_ghost[i])
+    for i in dirichletfaces:
+        VL = P_c[cid[i][0]]
+        one(i, *[Pbordnode[k] for k in face_nodes4(d, i)], VL, 2. * Pbordface[i] - VL)
     return out

@@ -195,3 +195,122 @@ def test_host_diamond_layout_matches_reference_expressions(dd):
         BCdir, dirfaces = _dirichlet_lists(d, bcs)
         mif = dm.matrixinnerfaces(d)
         assert compute_2dmatrix_size(d, BCdir, mif, dirfaces) == dm.compute_2dmatrix_size(d, BCdir, mif, dirfaces)
+
+
+# ---------------------------------------------------------------------------------------------
+# 3D (functions3d.py): the same defining properties on jittered Kuhn tetrahedra
+
+IDX3 = {"in": 1, "out": 2, "upper": 3, "bottom": 4, "front": 5, "back": 6}
+
+
+@pytest.fixture(scope="module")
+def dd3():
+    from conftest import make_tet_domain
+    return make_tet_domain(5, 0.08, 3)
+
+
+def _lists3(d, bcs):
+    faces = {"in": d.infaces, "out": d.outfaces, "upper": d.upperfaces, "bottom": d.bottomfaces,
+             "front": d.frontfaces, "back": d.backfaces}
+    BCdir = [IDX3[k] for k, v in bcs.items() if v == "dirichlet"]
+    dirfaces = np.sort(np.concatenate([faces[k] for k, v in bcs.items() if v == "dirichlet"] or
+                                      [np.zeros(0, np.int32)])).astype(np.int32)
+    neufaces = np.sort(np.concatenate([faces[k] for k, v in bcs.items() if v == "neumann"] or
+                                      [np.zeros(0, np.int32)])).astype(np.int32)
+    return BCdir, dirfaces, neufaces
+
+
+def _matrix3(d, bcs):
+    BCdir, dirfaces, _ = _lists3(d, bcs)
+    mif = dm.matrixinnerfaces(d)
+    data, row, col = dm.get_triplet_3d(d, BCdir, mif, dirfaces)
+    assert len(data) == dm.compute_3dmatrix_size(d, BCdir, mif, dirfaces)
+    C = int(d.nbcells)
+    return sp.csc_matrix((data, (row, col)), shape=(C, C)), BCdir, dirfaces, mif
+
+
+X_DIR = {"in": "dirichlet", "out": "dirichlet", "upper": "neumann", "bottom": "neumann", "front": "neumann",
+         "back": "neumann"}
+
+
+def test_diamond_3d_affine_is_exact(dd3):
+    """get_triplet_3d + get_rhs_glob_3d reproduce u = a + b x with Dirichlet data on the x
+    sides (the mirror 2 V_K - V_L is exact without a tangential gradient) and Neumann on the
+    others (zero flux, ghost = owner value, exact for u independent of y and z): every row of
+    A u - rhs0 vanishes and SuperLU returns u. A single Dirichlet diagonal term (the reference
+    adds -param3/|K| twice), a parameter in the wrong node slot or a dropped z term of the node
+    weights breaks it."""
+    d = dd3
+    f = lambda x, y, z: 0.3 + 1.7 * x
+    A, BCdir, dirfaces, mif = _matrix3(d, X_DIR)
+    Pn, Pf = dm.update_ghost_values(d, X_DIR, {"in": f, "out": f})
+    rhs0 = dm.get_rhs_glob_3d(d, Pn, Pf, BCdir, mif, dirfaces)
+    c = d.cells.center
+    u = f(c[:, 0], c[:, 1], c[:, 2])
+    assert np.abs(A @ u - rhs0).max() < 1e-12 * np.abs(rhs0).max()
+    x = sla.spsolve(A, rhs0)
+    assert np.abs(x - u).max() < 1e-11
+
+
+def test_diamond_3d_neumann_rows_sum_to_zero(dd3):
+    """Pure Neumann: constants are in the kernel of the operator and the lifting is zero."""
+    d = dd3
+    C = int(d.nbcells)
+    A, BCdir, dirfaces, mif = _matrix3(d, {k: "neumann" for k in IDX3})
+    assert np.abs(A @ np.ones(C)).max() < 1e-12 * np.abs(A.diagonal()).max()
+    Pn, Pf = dm.update_ghost_values(d, {k: "neumann" for k in IDX3}, {})
+    assert not np.any(dm.get_rhs_glob_3d(d, Pn, Pf, BCdir, mif, dirfaces))
+
+
+def test_diamond_3d_tangential_dirichlet_is_not_exact(dd3):
+    """With a tangential gradient on the Dirichlet faces the rows of the cells on those faces
+    are not exact (the mirror value is the point reflection through the face centre), while
+    the rows away from them stay exact."""
+    d = dd3
+    f = lambda x, y, z: 0.3 + 1.7 * x + 0.8 * y
+    bcs = {k: "dirichlet" for k in IDX3}
+    A, BCdir, dirfaces, mif = _matrix3(d, bcs)
+    Pn, Pf = dm.update_ghost_values(d, bcs, {k: f for k in IDX3})
+    rhs0 = dm.get_rhs_glob_3d(d, Pn, Pf, BCdir, mif, dirfaces)
+    c = d.cells.center
+    r = np.abs(A @ f(c[:, 0], c[:, 1], c[:, 2]) - rhs0)
+    wall = np.zeros(int(d.nbcells), bool)
+    wall[d.faces.cellid[dirfaces, 0]] = True
+    assert r[~wall].max() < 1e-12 * np.abs(rhs0).max()
+    assert r[wall].max() > 1e-6 * np.abs(rhs0).max()
+
+
+def test_centertovertex_and_face_gradient_3d_are_exact_on_affine_fields(dd3):
+    """The pseudo-Laplacian node weights reproduce affine fields (cell and mirror centres) and
+    the diamond face gradient returns the field's gradient, in x, y and z, on inner and
+    boundary faces."""
+    d = dd3
+    g = lambda p: 0.3 + 1.7 * p[:, 0] - 0.6 * p[:, 1] + 0.9 * p[:, 2]
+    wc, wg = g(d.cells.center), g(d.faces.ghostcenter[:, :3])
+    wn = dm.centertovertex_3d(d, wc, wg)
+    assert np.abs(wn - g(d.nodes.vertex[:, :3])).max() < 1e-12
+    bnd = np.asarray(d.boundaryfaces)
+    grads = dm.face_gradient_3d(d, wc, wg, wn, bnd)
+    fl = np.concatenate([np.asarray(d.innerfaces), bnd])
+    for k, b in enumerate((1.7, -0.6, 0.9)):
+        assert np.abs(grads[k][fl] - b).max() < 1e-11
+
+
+def test_p_gradient_3d_is_minus_the_gradient(dd3):
+    """compute_P_gradient_3d_diamond returns -grad P for P = a + b x with Dirichlet data on the
+    x sides: inner faces, Neumann faces (ghost = owner value) and Dirichlet faces (the
+    Dirichlet node values and the mirror 2 V_K - V_L)."""
+    d = dd3
+    f = lambda x, y, z: 0.3 + 1.7 * x
+    BCdir, dirfaces, neufaces = _lists3(d, X_DIR)
+    Pn, Pf = dm.update_ghost_values(d, X_DIR, {"in": f, "out": f})
+    c = d.cells.center
+    Pc = f(c[:, 0], c[:, 1], c[:, 2])
+    Pg = np.zeros(int(d.nbfaces))
+    Pg[neufaces] = Pc[d.faces.cellid[neufaces, 0]]
+    v = d.nodes.vertex
+    Pnode = f(v[:, 0], v[:, 1], v[:, 2])
+    px, py, pz = dm.p_gradient_3d(d, Pc, Pg, Pnode, Pn, Pf, BCdir, neufaces, dirfaces)
+    fl = np.concatenate([np.asarray(d.innerfaces), neufaces, dirfaces])
+    assert np.abs(px[fl] + 1.7).max() < 1e-11
+    assert np.abs(py[fl]).max() < 1e-11 and np.abs(pz[fl]).max() < 1e-11

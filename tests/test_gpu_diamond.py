@@ -176,7 +176,7 @@ def test_linear_solve_matches_superlu():
         A = sp.csr_matrix((data, (row, col)), shape=(C, C))
         b = rhs0 + src
         relres = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
-        assert relres <= 1.2e-10, (relres, L.stats)
+        assert relres <= 1e-10, (relres, L.stats)
         xs = sla.spsolve(A.tocsc(), b)
         assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 5e-7
         assert L.stats['refinements'] >= 2
@@ -305,3 +305,155 @@ def test_backend_compile_diamond_kernels_reference_signatures():
     dt_a = be.compile(_ref('time_step', 'manapy.solvers.advecdiff.fvm_utils'))(*args)
     assert dt_d == dm.time_step_diffusion(d, 0.3, 0.2, 0.7)
     assert dt_a == dm.time_step_advdiff(d, u, v, w, 0.3, 0.2, 0.7)
+
+
+# ---------------------------------------------------------------------------------------------
+# 3D diamond scheme (ast/functions3d.py) on jittered Kuhn tetrahedra and the reference's cube.msh
+
+LOCS3 = ("in", "out", "upper", "bottom", "front", "back")
+BCS3 = [
+    {"in": "dirichlet", "out": "dirichlet", "upper": "neumann", "bottom": "neumann", "front": "neumann",
+     "back": "neumann"},
+    {k: "dirichlet" for k in LOCS3},
+    {"in": "neumann", "out": "dirichlet", "upper": "nonslip", "bottom": "dirichlet", "front": "dirichlet",
+     "back": "neumann"},
+]
+FUN3 = lambda x, y, z: 1.0 + 0.5 * x - 0.25 * y + 0.4 * z + 0.3 * x * y * z
+
+
+def _values3(bcs):
+    return {k: (FUN3 if k != "bottom" else 0.75) for k, v in bcs.items() if v == "dirichlet"}
+
+
+def _mesh3(kind, seed=0):
+    import os
+    from conftest import make_tet_domain
+    if kind == "kuhn":
+        return make_tet_domain(4, 0.08, 60 + seed)
+    from manapy_amd.ddm import Domain
+    from manapy_amd.gmsh import read_gmsh
+    here = os.path.dirname(os.path.abspath(__file__))
+    return Domain(3, mesh=read_gmsh(os.path.join(here, 'golden', 'meshes', 'cube.msh.gz'), 3))
+
+
+def _oracle_system3(d, var, bcs):
+    from oracle import diamond as dm
+    vals = {k: (lambda x, y, z, f=FUN3: f(x, y, z)) if k != "bottom" else 0.75 for k in _values3(bcs)}
+    Pn, Pf = dm.update_ghost_values(d, {k: ("dirichlet" if v == "dirichlet" else
+                                            ("neumann" if v == "neumann" else "other")) for k, v in bcs.items()},
+                                    vals)
+    mif = dm.matrixinnerfaces(d)
+    data, row, col = dm.get_triplet_3d(d, list(var.BCdirichlet), mif, var.dirichletfaces)
+    rhs0 = dm.get_rhs_glob_3d(d, Pn, Pf, list(var.BCdirichlet), mif, var.dirichletfaces)
+    return data, row, col, Pn, Pf, rhs0
+
+
+@pytest.mark.parametrize("ib,mesh", [(0, "kuhn"), (1, "kuhn"), (2, "kuhn"), (0, "cube")])
+def test_assembly_rhs_bitexact_3d(ib, mesh):
+    """get_triplet_3d (duplicates summed in insertion order) and get_rhs_glob_3d, bit for bit;
+    the matrix size count of compute_3dmatrix_size."""
+    import torch
+    from oracle import diamond as dm
+    from manapy_amd import Variable
+    from manapy_amd.solvers.ls import HIPKrylovSolver
+    bcs = BCS3[ib]
+    d = _mesh3(mesh, ib)
+    P = Variable(domain=d, BC=bcs, values=_values3(bcs))
+    L = HIPKrylovSolver(domain=d, var=P)
+    L.update_ghost_values()
+    L.assembly()
+    torch.cuda.synchronize()
+    data, row, col, Pn, Pf, rhs0 = _oracle_system3(d, P, bcs)
+    assert L.dataSize == len(data)
+    assert np.array_equal(d.Pbordnode.cpu().numpy(), Pn)
+    assert np.array_equal(d.Pbordface.cpu().numpy(), Pf)
+    ref = dm.coo_sum_in_order(data, row, col)
+    gv, gr, gc = L.matrix_coo()
+    got = {(int(r), int(c)): float(v) for v, r, c in zip(gv, gr, gc)}
+    assert got.keys() == ref.keys()
+    bad = [k for k in ref if got[k] != ref[k]]
+    assert not bad, bad[:5]
+    assert np.array_equal(L.rhs0.cpu().numpy(), rhs0)
+
+
+@pytest.mark.parametrize("mesh", ["kuhn", "cube"])
+def test_node_face_gradient_dissipative_bitexact_3d(mesh):
+    """centertovertex_3d, face_gradient_3d (x, y, z), the dissipative residual with Dzz and the
+    diffusion time step in 3D, bit for bit."""
+    import torch
+    from oracle import diamond as dm
+    from manapy_amd import Variable, Struct
+    from manapy_amd.solvers import DiffusionSolver
+    bcs = BCS3[2]
+    d = _mesh3(mesh, 5)
+    C = int(d.nbcells)
+    w0 = np.random.default_rng(3).standard_normal(C)
+    P = Variable(domain=d, BC=bcs, values=_values3(bcs))
+    u, v, w = Variable(domain=d), Variable(domain=d), Variable(domain=d)
+    P.cell[:] = torch.as_tensor(w0)
+    P.update_ghost_value()
+    P.interpolate_celltonode()
+    P.compute_face_gradient()
+    S = DiffusionSolver(P, vel=(u, v, w), conf=Struct(Dxx=0.7, Dyy=0.3, Dzz=0.5, cfl=0.4))
+    S.explicit_dissipative()
+    torch.cuda.synchronize()
+    g = P.ghost.cpu().numpy()
+    wn = dm.centertovertex_3d(d, w0, g)
+    assert np.array_equal(P.node.cpu().numpy(), wn)
+    lists = np.concatenate([P.dirichletfaces, P.neumannfaces]).astype(np.int64)
+    wx, wy, wz = dm.face_gradient_3d(d, w0, g, wn, lists)
+    assert np.array_equal(P.gradfacex.cpu().numpy(), wx)
+    assert np.array_equal(P.gradfacey.cpu().numpy(), wy)
+    assert np.array_equal(P.gradfacez.cpu().numpy(), wz)
+    dis = dm.explicitscheme_dissipative(d, wx, wy, 0.7, 0.3, wz, 0.5)
+    assert np.array_equal(P.dissipative.cpu().numpy(), dis)
+    assert S.stepper() == dm.time_step_diffusion(d, 0.7, 0.3, 0.4, 0.5)
+
+
+def test_p_gradient_bitexact_3d():
+    import torch
+    from oracle import diamond as dm
+    from manapy_amd import Variable
+    from manapy_amd.solvers.ls import HIPKrylovSolver
+    bcs = BCS3[2]
+    d = _mesh3("kuhn", 7)
+    C = int(d.nbcells)
+    P = Variable(domain=d, BC=bcs, values=_values3(bcs))
+    L = HIPKrylovSolver(domain=d, var=P)
+    L.update_ghost_values()
+    P.cell[:] = torch.as_tensor(np.random.default_rng(8).standard_normal(C))
+    P.update_ghost_value()
+    P.interpolate_celltonode()
+    L.compute_Sol_gradient()
+    torch.cuda.synchronize()
+    Pc, Pg, Pnode = P.cell.cpu().numpy(), P.ghost.cpu().numpy(), P.node.cpu().numpy()
+    px, py, pz = dm.p_gradient_3d(d, Pc, Pg, Pnode, d.Pbordnode.cpu().numpy(), d.Pbordface.cpu().numpy(),
+                                  list(P.BCdirichlet), P.neumannfaces, P.dirichletfaces)
+    assert np.array_equal(P.gradfacex.cpu().numpy(), px)
+    assert np.array_equal(P.gradfacey.cpu().numpy(), py)
+    assert np.array_equal(P.gradfacez.cpu().numpy(), pz)
+
+
+@pytest.mark.parametrize("mesh", ["kuhn", "cube"])
+def test_linear_solve_3d_matches_superlu(mesh):
+    """A 3D Poisson problem with Dirichlet data on two sides and a source: the GPU solution meets
+    ||b - A x|| <= 1e-10 ||b|| for the oracle's FP64 matrix and agrees with SuperLU."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    from manapy_amd import Variable
+    from manapy_amd.solvers.ls import ScipySolver
+    bcs = BCS3[0]
+    d = _mesh3(mesh, 9)
+    C = int(d.nbcells)
+    P = Variable(domain=d, BC=bcs, values=_values3(bcs))
+    L = ScipySolver(domain=d, var=P)
+    src = np.full(C, 4.0)
+    L(rhs=src)
+    x = P.cell.cpu().numpy()
+    data, row, col, Pn, Pf, rhs0 = _oracle_system3(d, P, bcs)
+    A = sp.csr_matrix((data, (row, col)), shape=(C, C))
+    b = rhs0 + src
+    relres = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert relres <= 1e-10, (relres, L.stats)
+    xs = sla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 5e-7

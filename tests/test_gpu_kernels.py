@@ -396,10 +396,13 @@ def test_euler_packed_40_steps(torch, quirk, n, renumber, d3, m3, oracle_lib):
     same(host(rings[0]), host(rings[1]), "dt ring")
 
 
-@pytest.mark.parametrize("quirk,T,n", [(0, 128, 6), (1, 256, 6), (1, 128, 11), (0, 256, 11)])
+@pytest.mark.parametrize("quirk,T,n", [(0, 128, 6), (1, 256, 6), (1, 128, 11), (0, 256, 11), (1, 512, 11),
+                                        (0, 512, 6)])
 def test_euler_packed_tiled_40_steps(torch, quirk, T, n, oracle_lib):
-    """The packed Euler step on face tiles (each face once per tile in LDS) gives the oracle's
-    bits after 40 steps and the per-cell packed step's dt ring; n = 11 has a partial last tile."""
+    """The packed Euler step on face tiles (each face once per tile in LDS) and its single-phase
+    variant (each cell evaluates its own faces) give the oracle's bits after 40 steps and the
+    per-cell packed step's dt ring; n = 11 has a partial last tile. Tiles of 512 cells run the
+    single-phase step only (the face fluxes of such a tile do not fit the LDS)."""
     from manapy_amd.device import DeviceMesh, EulerTet, EulerTetTiles
     from manapy_amd.renumber import renumbered
     d = renumbered(make_tet_domain(n, 0.08, 6))
@@ -411,10 +414,11 @@ def test_euler_packed_tiled_40_steps(torch, quirk, T, n, oracle_lib):
     for _ in range(40):
         oracle_lib.euler_step(d, U, G_, 0.8, 1.4, quirk)
     et, tt = EulerTet(m), EulerTetTiles(m, T)
-    assert tt.lds_bytes() <= 64 * 1024
+    assert tt.lds_bytes(cells=(T == 512)) <= 64 * 1024
     tU = dev(torch, U0)
-    res = []
-    for tiled in (True, False):
+    res = {}
+    kinds = ('cells', 'packed') if T == 512 else ('tiled', 'cells', 'packed')
+    for kind in kinds:
         R = [torch.zeros((C, 8), dtype=torch.float64, device='cuda') for _ in range(2)]
         dt0 = dev(torch, np.zeros(1))
         ring = dev(torch, np.zeros(4))
@@ -422,18 +426,21 @@ def test_euler_packed_tiled_40_steps(torch, quirk, T, n, oracle_lib):
         call('mfv_dt_ring_init', P(ring), P(dt0))
         call('mfv_euler3d_pack', m.ptr, P(tU), P(R[0]))
         for s in range(40):
-            if tiled:
+            if kind == 'tiled':
                 call('mfv_euler3d_step_packed_tiled', m.ptr, et.ptr, tt.ptr, P(R[s % 2]), P(R[(s + 1) % 2]), 1.4, 0.8,
+                     quirk, P(ring), s)
+            elif kind == 'cells':
+                call('mfv_euler3d_step_cells_tiled', m.ptr, et.ptr, tt.ptr, P(R[s % 2]), P(R[(s + 1) % 2]), 1.4, 0.8,
                      quirk, P(ring), s)
             else:
                 call('mfv_euler3d_step_packed', m.ptr, et.ptr, P(R[s % 2]), P(R[(s + 1) % 2]), 1.4, 0.8, quirk,
                      P(ring), s)
         back = dev(torch, np.zeros((6, C)))
         call('mfv_euler3d_unpack', m.ptr, P(R[0]), P(back))
-        res.append((host(back), host(ring)))
-    same(res[0][0], U, "packed tiled U after 40 steps")
-    same(res[0][0], res[1][0], "packed tiled = packed")
-    same(res[0][1], res[1][1], "dt ring")
+        res[kind] = (host(back), host(ring))
+    for kind in kinds:
+        same(res[kind][0], U, "%s U after 40 steps" % kind)
+        same(res[kind][1], res['packed'][1], "%s dt ring" % kind)
 
 
 @pytest.mark.parametrize("packed", [False, True])
