@@ -32,17 +32,29 @@ PHASE_WAIT, PHASE_ADVECT, PHASE_SOLVE, PHASE_DONE, PHASE_FAILED, PHASE_REJECTED 
 
 
 class AdvDiffRequest(object):
+    """One solve. ``domain`` is a Domain, or the raw mesh ``(elements, nodes)`` in the partition
+    layout: then the engine builds the Domain and its device tables when the request arrives
+    (on its preparation threads), so the request's latency includes that preprocessing.
+    ``u0`` is an array of cell values, or a callable of the cell centres."""
+
     def __init__(self, rid, domain, u0, velocity=(1.0, 0.5), D=1e-2, nsteps=20, arrival=0., cfl=0.8):
         self.rid = rid
-        self.domain = domain
-        self.u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        if isinstance(domain, tuple):
+            self.mesh, self.domain = domain, None
+            self.C = int(len(domain[0]))
+        else:
+            self.mesh, self.domain = None, domain
+            self.C = int(domain.nbcells)
+        self._u0 = u0
+        self.u0 = None if callable(u0) else np.ascontiguousarray(u0, dtype=np.float64)
         self.velocity = (float(velocity[0]), float(velocity[1]))
         self.D = float(D)
         self.nsteps = int(nsteps)
         self.arrival = float(arrival)
         self.cfl = float(cfl)
-        self.C = int(domain.nbcells)
         self._mesh_host = None
+        self.t_prepared = None
+        self.prepare_s = 0.
         # filled by the engine
         self.inst = None
         self.phase = PHASE_WAIT
@@ -59,13 +71,25 @@ class AdvDiffRequest(object):
         return self.phase in (PHASE_FAILED, PHASE_REJECTED)
 
     def prepare(self):
-        """Host-side tables of the request's mesh (GPU layout and diamond weights), built
-        before the request is submitted so that admission only uploads them."""
+        """Host-side preprocessing: the Domain (when the request came as a raw mesh), then the
+        tables of its GPU layout and diamond weights, so that admission only uploads them."""
         from .device import gpu_layout
         from .diamond import diamond_layout
+        t0 = time.perf_counter()
+        if self.domain is None:
+            from .ddm import Domain
+            self.domain = Domain(2, mesh=self.mesh)
+            self.mesh = None
+        if self.u0 is None:
+            self.u0 = np.ascontiguousarray(self._u0(self.domain.cells.center), dtype=np.float64)
         self._mesh_host = gpu_layout(self.domain, lsq=False)
         self.domain._diamond_host = diamond_layout(self.domain)
+        self.prepare_s = time.perf_counter() - t0
         return self
+
+    @property
+    def prepared(self):
+        return self._mesh_host is not None
 
     def footprint(self, arena):
         """Bytes the request holds on the device while admitted."""
@@ -78,7 +102,7 @@ class AdvDiffRequest(object):
 
 class ServingEngine(object):
     def __init__(self, max_cells=4 << 20, max_inst=128, width=16, hbm_budget=None, n_iter=6, tol=1e-10,
-                 tol_inner=1e-4, device=None, graph=True):
+                 tol_inner=1e-4, device=None, graph=True, prep_threads=8, time_ticks=False):
         torch = require_gpu()
         self.arena = EllBatch((int(max_cells) + 63) // 64, max_inst=max_inst, width=width, device=device)
         self.device = self.arena.device
@@ -90,6 +114,9 @@ class ServingEngine(object):
         self.in_use = 0
         self.ticks = 0
         self.max_active = 0
+        self.prep_threads = int(prep_threads)
+        self.time_ticks = bool(time_ticks)
+        self._tick_events = []
 
     # -- admission --------------------------------------------------------------------------------
     def never_fits(self, req):
@@ -186,72 +213,119 @@ class ServingEngine(object):
 
     # -- the loop -------------------------------------------------------------------------------
     def run(self, requests, clock=None, max_ticks=10 ** 7):
-        """Serve ``requests`` (sorted by arrival time, seconds from the start); returns them
-        with t_admit / t_done set and ``result`` filled.
+        """Serve ``requests`` (by arrival time, seconds from the start); returns them with
+        t_admit / t_done set and ``result`` filled.
 
-        A request whose solve fails (status FAILED: no convergence within the outer-iteration
-        limit, or a non-finite residual) is returned with ``phase == PHASE_FAILED`` and
-        ``error`` set; its instance is released and the other requests go on unchanged. A
-        request that can never be admitted (``never_fits``) is returned at once with
-        ``phase == PHASE_REJECTED``."""
+        A request that is not prepared yet is prepared on one of ``prep_threads`` host
+        threads from its arrival on (the Domain of a raw mesh, the layout tables), while the
+        GPU serves the others; it is admitted, in arrival order, once prepared and once the
+        arena and the HBM budget have room. A request whose solve fails (status FAILED: no
+        convergence within the outer-iteration limit, or a non-finite residual) is returned
+        with ``phase == PHASE_FAILED`` and ``error`` set; its instance is released and the
+        other requests go on unchanged. A request that can never be admitted (``never_fits``)
+        is returned with ``phase == PHASE_REJECTED``."""
         import torch
+        from concurrent.futures import ThreadPoolExecutor, wait, FIRST_COMPLETED
         pending = sorted(requests, key=lambda r: r.arrival)
         clock = clock or time.perf_counter
+        pool = ThreadPoolExecutor(max(1, self.prep_threads))
+        futures = {}
         t0 = clock()
         done = []
-        for r in list(pending):
-            why = self.never_fits(r)
-            if why is not None:
-                pending.remove(r)
-                r.phase, r.error = PHASE_REJECTED, "rejected: " + why
-                r.t_admit = r.t_done = r.arrival
-                done.append(r)
-        for _ in range(max_ticks):
-            now = clock() - t0
-            admitted = False
-            while pending and pending[0].arrival <= now and self.can_admit(pending[0]):
-                self.admit(pending.pop(0), now)
-                admitted = True
-            if pending and not self.active and not admitted and pending[0].arrival <= now:
-                # an empty arena that cannot take the head request: never_fits covers budget and
-                # size, so only a full instance table is left, which an empty arena cannot have
-                raise _lib.MfvError("request %s cannot be admitted into an empty arena" % pending[0].rid)
-            if admitted:
-                self.arena.publish()
-                self.max_active = max(self.max_active, len(self.active))
-            if not self.active:
-                if not pending:
-                    break
-                wait = pending[0].arrival - (clock() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 0.05))
-                continue
-            for r in self.active:
-                if r.phase == PHASE_ADVECT:
-                    self._advect(r)
-            self.arena.tick(self.n_iter, graph=self.graph)
-            self.ticks += 1
-            st = self.arena.status.cpu().numpy()
-            now = clock() - t0
-            for r in list(self.active):
-                s = st[r.inst]
-                if s == _lib.KS_DONE:
-                    r.step += 1
-                    if r.step == r.nsteps:
-                        self._finish(r, now)
+
+        def reject(r, why):
+            r.phase, r.error = PHASE_REJECTED, "rejected: " + why
+            r.t_admit = r.t_done = clock() - t0
+            done.append(r)
+
+        try:
+            for _ in range(max_ticks):
+                now = clock() - t0
+                for r in pending:  # preparation starts at arrival
+                    if r.arrival > now:
+                        break
+                    if id(r) not in futures and not r.prepared:
+                        futures[id(r)] = pool.submit(r.prepare)
+                admitted = False
+                while pending and pending[0].arrival <= now:
+                    r = pending[0]
+                    f = futures.get(id(r))
+                    if f is not None and not f.done():
+                        break
+                    if f is not None and f.exception() is not None:
+                        pending.pop(0)
+                        reject(r, "preparation failed: %s" % f.exception())
+                        continue
+                    r.t_prepared = clock() - t0
+                    why = self.never_fits(r)
+                    if why is not None:
+                        pending.pop(0)
+                        reject(r, why)
+                        continue
+                    if not self.can_admit(r):
+                        if not self.active:
+                            raise _lib.MfvError("request %s cannot be admitted into an empty arena" % r.rid)
+                        break
+                    self.admit(pending.pop(0), now)
+                    admitted = True
+                if admitted:
+                    self.arena.publish()
+                    self.max_active = max(self.max_active, len(self.active))
+                if not self.active:
+                    if not pending:
+                        break
+                    wait_s = pending[0].arrival - (clock() - t0)
+                    f = futures.get(id(pending[0]))
+                    if f is not None and not f.done():
+                        wait([f], timeout=0.05, return_when=FIRST_COMPLETED)
+                    elif wait_s > 0:
+                        time.sleep(min(wait_s, 0.05))
+                    continue
+                for r in self.active:
+                    if r.phase == PHASE_ADVECT:
+                        self._advect(r)
+                if self.time_ticks:
+                    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                    ev[0].record()
+                self.arena.tick(self.n_iter, graph=self.graph)
+                if self.time_ticks:
+                    ev[1].record()
+                    self._tick_events.append(ev)
+                self.ticks += 1
+                st = self.arena.status.cpu().numpy()
+                now = clock() - t0
+                for r in list(self.active):
+                    s = st[r.inst]
+                    if s == _lib.KS_DONE:
+                        r.step += 1
+                        if r.step == r.nsteps:
+                            self._finish(r, now)
+                            done.append(r)
+                        else:
+                            r.phase = PHASE_ADVECT
+                    elif s == _lib.KS_FAILED:
+                        self._finish(r, now, error="implicit solve failed at step %d" % r.step)
                         done.append(r)
-                    else:
-                        r.phase = PHASE_ADVECT
-                elif s == _lib.KS_FAILED:
-                    self._finish(r, now, error="implicit solve failed at step %d" % r.step)
-                    done.append(r)
-        else:
-            raise _lib.MfvError("serving loop did not finish in %d ticks" % max_ticks)
+            else:
+                raise _lib.MfvError("serving loop did not finish in %d ticks" % max_ticks)
+        finally:
+            pool.shutdown(wait=True)
         torch.cuda.synchronize()
         return done
 
+    def tick_seconds(self):
+        """Device time spent in solver ticks (time_ticks=True), from HIP events around each."""
+        import torch
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in self._tick_events) * 1e-3
+
 
 def latency_stats(done):
-    lat = np.array([r.t_done - r.arrival for r in done if r.phase == PHASE_DONE])
+    """Latency from arrival to result of the completed requests (preparation included), and
+    its preparation part."""
+    ok = [r for r in done if r.phase == PHASE_DONE]
+    lat = np.array([r.t_done - r.arrival for r in ok])
+    prep = np.array([r.prepare_s for r in ok])
     return {'p50_s': float(np.percentile(lat, 50)), 'p99_s': float(np.percentile(lat, 99)),
-            'mean_s': float(lat.mean()), 'max_s': float(lat.max())}
+            'mean_s': float(lat.mean()), 'max_s': float(lat.max()),
+            'prepare_p50_s': float(np.percentile(prep, 50)), 'prepare_p99_s': float(np.percentile(prep, 99))}

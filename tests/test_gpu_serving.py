@@ -123,3 +123,30 @@ def test_request_that_never_fits_is_rejected():
     huge = _request(2, 30, 30, 7, 1e-2, (1.0, 0.0), 2)
     done = eng.run([huge])
     assert done[0].phase == PHASE_REJECTED and "slices" in done[0].error
+
+
+def test_raw_mesh_requests_are_prepared_on_arrival():
+    """A request that carries its raw mesh (elements, nodes) and a callable initial field is
+    prepared (Domain, device tables) by the engine's host threads after it arrives, while the
+    GPU serves the others; its result is the same as the request built from a ready Domain,
+    and its latency includes that preparation."""
+    from manapy_amd.meshgen import jittered_triangles
+    from manapy_amd.serving import AdvDiffRequest, ServingEngine
+    blob = lambda c: np.exp(-((c[:, 0] - .3) ** 2 + (c[:, 1] - .4) ** 2) / .01)
+    meshes = [jittered_triangles(n, n, 1.0, 1.0, 0.1, 950 + n) for n in (14, 31, 22)]
+    ref = {}
+    for k, (el, nd) in enumerate(meshes):
+        from manapy_amd.ddm import Domain
+        d = Domain(2, mesh=(el, nd))
+        r = AdvDiffRequest(k, d, blob(d.cells.center), velocity=(0.6, -0.2), D=4e-3, nsteps=5)
+        ServingEngine(max_cells=1 << 13, max_inst=4).run([r])
+        ref[k] = r.result
+    reqs = [AdvDiffRequest(k, (el, nd), blob, velocity=(0.6, -0.2), D=4e-3, nsteps=5, arrival=0.003 * k)
+            for k, (el, nd) in enumerate(meshes)]
+    assert not any(r.prepared for r in reqs)
+    done = ServingEngine(max_cells=1 << 13, max_inst=4, prep_threads=2).run(reqs)
+    assert len(done) == 3
+    for r in done:
+        assert r.prepared and r.prepare_s > 0
+        assert r.t_done - r.arrival >= r.prepare_s
+        assert np.array_equal(r.result, ref[r.rid]), r.rid
