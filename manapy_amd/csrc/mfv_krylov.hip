@@ -152,23 +152,38 @@ __global__ void __launch_bounds__(256) k_ir_residual(Ell e, Work w)
     });
 }
 
+// Threads of a per-instance block: an instance's reduction runs in one block, so a wide block
+// shortens the serial part of the largest instance's sum (1M rows = 15 625 slice partials).
+constexpr int kInstBlock = 1024;
+
 // Sums partial[slice][0..n) over the instance's slices into out[0..n) (thread 0 holds them).
+// Each thread keeps four running sums over its slices (i, i + B, i + 2B, i + 3B, then the next
+// 4B), added in that fixed order, then a tree over the block: the order depends only on the
+// instance's own slice count.
 template <int NSUM>
 __device__ __forceinline__ void inst_reduce(const Ell &e, const Work &w, int inst, double *out)
 {
-    __shared__ double red[NSUM][256];
+    __shared__ double red[NSUM][kInstBlock];
     const int s0 = e.inst_slice0[inst], ns = e.inst_nslices[inst];
-    double acc[NSUM];
+    double acc[4][NSUM];
 #pragma unroll
-    for (int k = 0; k < NSUM; k++) acc[k] = 0.;
-    for (int i = threadIdx.x; i < ns; i += 256) {
+    for (int q = 0; q < 4; q++)
 #pragma unroll
-        for (int k = 0; k < NSUM; k++) acc[k] += w.partial[4 * (size_t)(s0 + i) + k];
+        for (int k = 0; k < NSUM; k++) acc[q][k] = 0.;
+    for (int i = threadIdx.x; i < ns; i += 4 * kInstBlock) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = i + q * kInstBlock;
+            if (j < ns) {
+#pragma unroll
+                for (int k = 0; k < NSUM; k++) acc[q][k] += w.partial[4 * (size_t)(s0 + j) + k];
+            }
+        }
     }
 #pragma unroll
-    for (int k = 0; k < NSUM; k++) red[k][threadIdx.x] = acc[k];
+    for (int k = 0; k < NSUM; k++) red[k][threadIdx.x] = (acc[0][k] + acc[1][k]) + (acc[2][k] + acc[3][k]);
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    for (int h = kInstBlock / 2; h > 0; h >>= 1) {
         if (threadIdx.x < h) {
 #pragma unroll
             for (int k = 0; k < NSUM; k++) red[k][threadIdx.x] += red[k][threadIdx.x + h];
@@ -194,7 +209,7 @@ __device__ __forceinline__ void for_insts(const Ell &e, const Work &w, int want0
     }
 }
 
-__global__ void __launch_bounds__(256) k_ir_check(Ell e, Work w)
+__global__ void __launch_bounds__(kInstBlock) k_ir_check(Ell e, Work w)
 {
     for_insts(e, w, MFV_KS_START, MFV_KS_CORRECT, [&](int inst) {
         double sum[3];
@@ -258,7 +273,7 @@ __global__ void __launch_bounds__(256) k_bicg_spmv_v(Ell e, Work w)
     });
 }
 
-__global__ void __launch_bounds__(256) k_bicg_alpha(Ell e, Work w)
+__global__ void __launch_bounds__(kInstBlock) k_bicg_alpha(Ell e, Work w)
 {
     for_insts(e, w, MFV_KS_INNER, MFV_KS_INNER, [&](int inst) {
         double sum[1];
@@ -305,7 +320,7 @@ __global__ void __launch_bounds__(256) k_bicg_spmv_t(Ell e, Work w)
     });
 }
 
-__global__ void __launch_bounds__(256) k_bicg_omega(Ell e, Work w)
+__global__ void __launch_bounds__(kInstBlock) k_bicg_omega(Ell e, Work w)
 {
     for_insts(e, w, MFV_KS_INNER, MFV_KS_INNER, [&](int inst) {
         double sum[2];
@@ -335,7 +350,7 @@ __global__ void __launch_bounds__(256) k_bicg_x(Ell e, Work w)
     });
 }
 
-__global__ void __launch_bounds__(256) k_bicg_rho(Ell e, Work w)
+__global__ void __launch_bounds__(kInstBlock) k_bicg_rho(Ell e, Work w)
 {
     for_insts(e, w, MFV_KS_INNER, MFV_KS_INNER, [&](int inst) {
         double sum[2];
@@ -411,19 +426,19 @@ static int tick_launches(const mfv_ell_batch *e, const mfv_krylov_work *w, int n
 {
     const Ell ek = to_kernel(e);
     const Work wk = to_kernel(w);
-    const dim3 gs(grid_slices), gi(grid_inst), b(256);
+    const dim3 gs(grid_slices), gi(grid_inst), b(256), bi(kInstBlock);
     hipLaunchKernelGGL(k_ir_correct, gs, b, 0, st, ek, wk);
     hipLaunchKernelGGL(k_ir_residual, gs, b, 0, st, ek, wk);
-    hipLaunchKernelGGL(k_ir_check, gi, b, 0, st, ek, wk);
+    hipLaunchKernelGGL(k_ir_check, gi, bi, 0, st, ek, wk);
     for (int it = 0; it < n_iter; it++) {
         hipLaunchKernelGGL(k_bicg_p, gs, b, 0, st, ek, wk);
         hipLaunchKernelGGL(k_bicg_spmv_v, gs, b, 0, st, ek, wk);
-        hipLaunchKernelGGL(k_bicg_alpha, gi, b, 0, st, ek, wk);
+        hipLaunchKernelGGL(k_bicg_alpha, gi, bi, 0, st, ek, wk);
         hipLaunchKernelGGL(k_bicg_s, gs, b, 0, st, ek, wk);
         hipLaunchKernelGGL(k_bicg_spmv_t, gs, b, 0, st, ek, wk);
-        hipLaunchKernelGGL(k_bicg_omega, gi, b, 0, st, ek, wk);
+        hipLaunchKernelGGL(k_bicg_omega, gi, bi, 0, st, ek, wk);
         hipLaunchKernelGGL(k_bicg_x, gs, b, 0, st, ek, wk);
-        hipLaunchKernelGGL(k_bicg_rho, gi, b, 0, st, ek, wk);
+        hipLaunchKernelGGL(k_bicg_rho, gi, bi, 0, st, ek, wk);
     }
     return check_launch("krylov tick");
 }
