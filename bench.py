@@ -443,20 +443,33 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=5, renumber='hilbert'):
                                 measured_traffic(kname, C))}
     # FP32-bulk variant on the same tiles: throughput, then 200-step (or --euler-steps) accuracy
     # against the float64 step and the float64 conservation sums
-    ptile = ptiles.get(256) or ptiles.get(128)
-    if ptile is not None:
+    # tiles whose FP32-bulk LDS (24 B per item and per face entry) fits: 256 and 512 cells
+    cand32 = {}
+    for T in (256, 512, 128):
+        tt = ptiles.get(T) or ctiles.get(T)
+        if tt is not None and 24 * tt.host['max_items'] + 24 * tt.host['max_faces'] <= 64 * 1024:
+            cand32[T] = tt
+    if cand32:
         from manapy_amd.device import Euler32
-        e32 = Euler32(m, et=et, tt=ptile)
-        T32 = int(ptile.desc.tile_size)
+        e32s = {T: Euler32(m, et=et, tt=tt) for T, tt in cand32.items()}
+        cur = {}
 
         def init32():
             S[0].copy_(dv(U0))
             _lib.call('mfv_euler3d_time_step', m.ptr, S[0].data_ptr(), CFL, 1.4, dt0.data_ptr(), None, s)
             _lib.call('mfv_dt_ring_init', ring.data_ptr(), dt0.data_ptr(), s)
-            e32.pack(S[0], s)
+            cur['e'].pack(S[0], s)
 
         def step32(k):
-            e32.step(1.4, CFL, 0, ring, k, sums=False, stream=s)
+            cur['e'].step(1.4, CFL, 0, ring, k, sums=False, stream=s)
+        probe32 = {}
+        for T, e in e32s.items():
+            cur['e'] = e
+            init32()
+            probe32[T] = time_steps(step32, 2, 10)[1] * 1e3
+        T32 = min(probe32, key=probe32.get)
+        e32 = cur['e'] = e32s[T32]
+        log("euler fp32-bulk: tiles %d %s" % (T32, {k: round(v, 4) for k, v in probe32.items()}))
         init32()
         t32_loop, t32_ev = time_steps(step32, warmup, steps)
         # accuracy: both paths from the initial state for ``steps`` steps
@@ -482,6 +495,7 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=5, renumber='hilbert'):
         out["fp32_bulk"] = {
             "value": round(C / t32_loop / 1e6, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t32_loop * 1e3, 5),
             "speedup_vs_fp64": round(t_loop / t32_loop, 3), "tile": T32,
+            "tile_ms_per_step": {str(k): round(v, 4) for k, v in probe32.items()},
             "dtype": "f32 records, geometry and face fluxes; f64 gathers, update, dt and conservation sums",
             "roofline": roofline("k_euler_step32_tiled<%d> + k_dt_finish" % T32, SURVEY_B_EULER32, C, t32_ev, b32,
                                  measured_traffic("k_euler_step32_tiled<%d>" % T32, C)),

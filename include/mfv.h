@@ -689,6 +689,8 @@ typedef struct mfv_ell_batch {
     const int32_t *n_live_slices; /* device scalar */
     const int32_t *live_inst;     /* [I] */
     const int32_t *n_live_inst;   /* device scalar */
+    const int16_t *cols16;        /* [S][width][64] column - row of the narrow slices, or NULL */
+    const uint8_t *slice_narrow;  /* [S] 1: the slice's columns are read from cols16, or NULL */
 } mfv_ell_batch;
 
 /* Per-instance solver state: status[i] and scal[i][MFV_KS_NSCAL]. The caller sets SIGMA, TAU,
@@ -744,6 +746,15 @@ int mfv_krylov_tick(const mfv_ell_batch *e, const mfv_krylov_work *w, int n_iter
  * reference: it lets a serving loop advance all solves with one launch. */
 int mfv_krylov_graph(const mfv_ell_batch *e, const mfv_krylov_work *w, int n_iter, int grid_slices,
                      int grid_inst, void **graph_out);
+
+/* 16-bit columns for the slices [slice0, slice0 + nslices) of an assembled ELL batch (no
+ * counterpart in the reference: a storage form of the matrix of solvers/ls/system.py:154-171).
+ * A slice is narrow when every used slot's column lies within +-32767 of its row (a banded
+ * numbering, such as a mesh's row-major or Cuthill-McKee order, keeps most slices narrow); its
+ * cols16 = column - row and slice_narrow = 1, and the SpMVs then read 2 bytes per slot instead
+ * of 4 (same columns, same results). Other slices get slice_narrow = 0. */
+int mfv_ell_narrow_columns(const mfv_ell_batch *e, int slice0, int nslices, int16_t *cols16, uint8_t *slice_narrow,
+                           void *stream);
 
 /* y = A x (FP64) on the live slices. */
 int mfv_ell_spmv(const mfv_ell_batch *e, const double *x, double *y, int grid_slices, void *stream);

@@ -140,7 +140,7 @@ class MfvEllBatch(ctypes.Structure):
     _fields_ = [('nb_slices', ctypes.c_int32), ('width', ctypes.c_int32), ('nb_inst', ctypes.c_int32),
                 ('pad_', ctypes.c_int32), ('slice_inst', P), ('slice_w', P), ('cols', P), ('vals', P),
                 ('vals32', P), ('diag', P), ('inst_slice0', P), ('inst_nslices', P), ('live_slices', P),
-                ('n_live_slices', P), ('live_inst', P), ('n_live_inst', P)]
+                ('n_live_slices', P), ('live_inst', P), ('n_live_inst', P), ('cols16', P), ('slice_narrow', P)]
 
 
 class MfvKrylovWork(ctypes.Structure):
@@ -266,6 +266,7 @@ _SIGNATURES = {
     'mfv_krylov_tick': [ELL_P, KW_P, c_int, c_int, c_int, P],
     'mfv_krylov_graph': [ELL_P, KW_P, c_int, c_int, c_int, ctypes.POINTER(c_void_p)],
     'mfv_ell_spmv': [ELL_P, P, P, c_int, P],
+    'mfv_ell_narrow_columns': [ELL_P, c_int, c_int, P, P, P],
 }
 _RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p, 'mfv_sw_tiles_host_free': None,
             'mfv_face_tiles_host_free': None, 'mfv_euler_tet_host_free': None, 'mfv_domain_topology_free': None,

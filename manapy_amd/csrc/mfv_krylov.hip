@@ -32,6 +32,8 @@ struct Ell {
     const int32_t *__restrict__ n_live_slices;
     const int32_t *__restrict__ live_inst;
     const int32_t *__restrict__ n_live_inst;
+    const int16_t *__restrict__ cols16;
+    const uint8_t *__restrict__ narrow;
 };
 
 struct Work {
@@ -52,9 +54,9 @@ struct Work {
 
 static Ell to_kernel(const mfv_ell_batch *e)
 {
-    return Ell{e->nb_slices,   e->width,      e->nb_inst,      e->slice_inst,    e->slice_w,
-               e->cols,        e->vals,       e->vals32,       e->diag,          e->inst_slice0,
-               e->inst_nslices, e->live_slices, e->n_live_slices, e->live_inst, e->n_live_inst};
+    return Ell{e->nb_slices,    e->width,       e->nb_inst,       e->slice_inst, e->slice_w, e->cols,
+               e->vals,         e->vals32,      e->diag,          e->inst_slice0, e->inst_nslices,
+               e->live_slices,  e->n_live_slices, e->live_inst,   e->n_live_inst, e->cols16, e->slice_narrow};
 }
 
 static Work to_kernel(const mfv_krylov_work *w)
@@ -89,15 +91,26 @@ __device__ __forceinline__ void for_slices(const Ell &e, const Work &w, int want
     }
 }
 
-// sum_j A32[row][j] * x[col_j] in FP32 with explicit fused multiply-adds.
+// sum_j A32[row][j] * x[col_j] in FP32 with explicit fused multiply-adds. A narrow slice reads
+// its columns as 16-bit offsets from the row (the branch is uniform over the wave: one slice).
+__device__ __forceinline__ bool slice_is_narrow(const Ell &e, int s) { return e.narrow && e.narrow[s]; }
+
 __device__ __forceinline__ float ell_row32(const Ell &e, int s, int lane, const float *__restrict__ x)
 {
     const int wdt = e.slice_w[s];
     const size_t base = (size_t)s * e.W * 64 + lane;
     float acc = 0.f;
-    for (int j = 0; j < wdt; j++) {
-        const size_t q = base + (size_t)j * 64;
-        acc = fmaf(e.vals32[q], x[e.cols[q]], acc);
+    if (slice_is_narrow(e, s)) {
+        const int row = s * 64 + lane;
+        for (int j = 0; j < wdt; j++) {
+            const size_t q = base + (size_t)j * 64;
+            acc = fmaf(e.vals32[q], x[row + e.cols16[q]], acc);
+        }
+    } else {
+        for (int j = 0; j < wdt; j++) {
+            const size_t q = base + (size_t)j * 64;
+            acc = fmaf(e.vals32[q], x[e.cols[q]], acc);
+        }
     }
     return acc;
 }
@@ -107,11 +120,46 @@ __device__ __forceinline__ double ell_row64(const Ell &e, int s, int lane, const
     const int wdt = e.slice_w[s];
     const size_t base = (size_t)s * e.W * 64 + lane;
     double acc = 0.;
-    for (int j = 0; j < wdt; j++) {
-        const size_t q = base + (size_t)j * 64;
-        acc = fma(e.vals[q], x[e.cols[q]], acc);
+    if (slice_is_narrow(e, s)) {
+        const int row = s * 64 + lane;
+        for (int j = 0; j < wdt; j++) {
+            const size_t q = base + (size_t)j * 64;
+            acc = fma(e.vals[q], x[row + e.cols16[q]], acc);
+        }
+    } else {
+        for (int j = 0; j < wdt; j++) {
+            const size_t q = base + (size_t)j * 64;
+            acc = fma(e.vals[q], x[e.cols[q]], acc);
+        }
     }
     return acc;
+}
+
+// One wave per slice: the slice is narrow when every used slot's column is within +-32767 of its
+// row; then its 16-bit offsets are written. Slots past the slice width are never read.
+__global__ void __launch_bounds__(256) k_ell_narrow(Ell e, int slice0, int nslices, int16_t *__restrict__ cols16,
+                                                     uint8_t *__restrict__ narrow)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nslices) return;
+    const int s = slice0 + i;
+    const int wdt = e.slice_w[s];
+    const int row = s * 64 + lane;
+    const size_t base = (size_t)s * e.W * 64 + lane;
+    int wide = 0;
+    for (int j = 0; j < wdt; j++) {
+        const int d = e.cols[base + (size_t)j * 64] - row;
+        wide |= (d < -32767 || d > 32767) ? 1 : 0;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) wide |= __shfl_xor(wide, off, 64);
+    if (!wide)
+        for (int j = 0; j < wdt; j++) {
+            const size_t q = base + (size_t)j * 64;
+            cols16[q] = (int16_t)(e.cols[q] - row);
+        }
+    if (lane == 0) narrow[s] = wide ? 0 : 1;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -391,6 +439,7 @@ static int validate_ell(const mfv_ell_batch *e)
 {
     MFV_REQUIRE(e != nullptr, "ELL batch descriptor is NULL");
     MFV_REQUIRE(e->nb_slices >= 1 && e->width >= 1 && e->nb_inst >= 1, "ELL batch sizes must be positive");
+    MFV_REQUIRE((e->cols16 == nullptr) == (e->slice_narrow == nullptr), "cols16 and slice_narrow go together");
     MFV_REQUIRE((long long)e->nb_slices * 64 < (1LL << 31), "ELL batch rows exceed int32");
     MFV_REQUIRE(e->slice_inst && e->slice_w && e->cols && e->vals && e->vals32 && e->diag && e->inst_slice0 &&
                     e->inst_nslices && e->live_slices && e->n_live_slices && e->live_inst && e->n_live_inst,
@@ -465,6 +514,19 @@ extern "C" int mfv_krylov_graph(const mfv_ell_batch *e, const mfv_krylov_work *w
     }
     *graph_out = g;
     return MFV_OK;
+}
+
+extern "C" int mfv_ell_narrow_columns(const mfv_ell_batch *e, int slice0, int nslices, int16_t *cols16,
+                                      uint8_t *slice_narrow, void *stream)
+{
+    if (int rc = validate_ell(e)) return rc;
+    MFV_REQUIRE(cols16 && slice_narrow, "NULL array");
+    MFV_REQUIRE(slice0 >= 0 && nslices >= 0 && (long long)slice0 + nslices <= e->nb_slices,
+                "slices [%d, %d) outside the batch's %d", slice0, slice0 + nslices, e->nb_slices);
+    if (nslices == 0) return MFV_OK;
+    hipLaunchKernelGGL(k_ell_narrow, dim3((nslices + 3) / 4), dim3(256), 0, as_stream(stream), to_kernel(e), slice0,
+                       nslices, cols16, slice_narrow);
+    return check_launch("k_ell_narrow");
 }
 
 extern "C" int mfv_ell_spmv(const mfv_ell_batch *e, const double *x, double *y, int grid_slices, void *stream)

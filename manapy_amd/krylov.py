@@ -72,6 +72,8 @@ class EllBatch(object):
         self.slice_inst = torch.full((S,), -1, dtype=i32, device=dev)
         self.slice_w = torch.zeros(S, dtype=i32, device=dev)
         self.cols = torch.zeros(S * W * 64, dtype=i32, device=dev)
+        self.cols16 = torch.zeros(S * W * 64, dtype=torch.int16, device=dev)
+        self.slice_narrow = torch.zeros(S, dtype=torch.uint8, device=dev)
         self.vals = torch.zeros(S * W * 64, dtype=f64, device=dev)
         self.vals32 = torch.zeros(S * W * 64, dtype=f32, device=dev)
         self.diag = torch.zeros(S * 64, dtype=f64, device=dev)
@@ -92,7 +94,7 @@ class EllBatch(object):
         e = _lib.MfvEllBatch()
         e.nb_slices, e.width, e.nb_inst = S, W, I
         for k in ('slice_inst', 'slice_w', 'cols', 'vals', 'vals32', 'diag', 'inst_slice0', 'inst_nslices',
-                  'live_slices', 'n_live_slices', 'live_inst', 'n_live_inst'):
+                  'live_slices', 'n_live_slices', 'live_inst', 'n_live_inst', 'cols16', 'slice_narrow'):
             setattr(e, k, getattr(self, k).data_ptr())
         w = _lib.MfvKrylovWork()
         w.x, w.b = self.x.data_ptr(), self.b.data_ptr()
@@ -115,7 +117,7 @@ class EllBatch(object):
 
     def bytes_per_slice(self):
         """HBM bytes one slice of an instance occupies (matrix slots, vectors, partials)."""
-        return self.W * 64 * (4 + 8 + 4) + 64 * (8 + 8 + 8 + 8 * 4) + 4 * 8 + 8
+        return self.W * 64 * (4 + 2 + 8 + 4) + 64 * (8 + 8 + 8 + 8 * 4) + 4 * 8 + 8 + 1
 
     def free_slices(self):
         return self._ranges.available()
@@ -187,7 +189,20 @@ class EllBatch(object):
         _lib.call('mfv_diamond_assemble', ddia.ptr, node_dir.data_ptr(), face_dir_rank.data_ptr(), row0, self.W,
                   sw.data_ptr(), self.cols[off:].data_ptr(), self.vals[off:].data_ptr(), self.vals32[off:].data_ptr(),
                   self.diag[row0:].data_ptr(), st)
+        self.narrow_columns(inst)
         return wmax
+
+    def narrow_columns(self, inst, enable=True):
+        """16-bit column offsets for the instance's slices whose columns stay within +-32767 of
+        their rows (mfv_ell_narrow_columns); enable=False makes every slice read the int32
+        columns (same results)."""
+        row0, _, ns = self.rows(inst)
+        s0 = row0 // 64
+        if enable:
+            _lib.call('mfv_ell_narrow_columns', self.eptr, s0, ns, self.cols16.data_ptr(), self.slice_narrow.data_ptr(),
+                      stream_ptr())
+        else:
+            self.slice_narrow[s0:s0 + ns].zero_()
 
     def configure(self, inst, sigma, tau, tol=1e-10, tol_in=1e-4, maxit_in=400, maxit_out=60):
         """Operator coefficients and stopping parameters of one instance (counters zeroed)."""

@@ -1,5 +1,6 @@
 """The Gmsh 4.1 reader on the reference's own meshes (mesh/carre.msh, carre_hybrid.msh,
-rectangle.msh, cube.msh, kept gzipped under tests/golden/meshes): node tags follow
+rectangle.msh, cube.msh, carre_structure.msh, rectangle_RTM.msh, cube_bis.msh, kept gzipped under
+tests/golden/meshes): node tags follow
 define_ghost_node's priority (partitions/partitions_utils.py:16-111), checked here against the
 geometry of each mesh; cell volumes add up to the polygon/box area; the element layout is
 convert_2d/3d_cons_to_array's (partitions_utils.py:113-185)."""
@@ -83,3 +84,21 @@ def test_domain_from_gmsh(name, dim, measure):
         sign = np.where(d.faces.cellid[f, 0] == np.arange(int(d.nbcells)), 1., -1.)
         s[ok] += (sign[:, None] * d.faces.normal[f, :3])[ok]
     assert np.abs(s).max() < 1e-12
+
+
+@pytest.mark.parametrize("name,dim,counts,measure", [
+    ("cube_bis", 3, {4: 3196, 5: 81, 8: 729}, 2.0),            # tetrahedra, pyramids, hexahedra
+    ("carre_structure", 2, {3: 1058, 4: 192}, 0.4),            # triangles and quadrangles
+    ("rectangle_RTM", 2, {3: 5206}, 0.008)])
+def test_mixed_element_meshes(name, dim, counts, measure):
+    """Meshes with several element kinds: the element counts per kind, the box tags of the
+    nodes, and cell volumes that add up to the box (the 3D cells split into tetrahedra by
+    Compute_3dcentervolumeOfCell's rules)."""
+    el, nd = read_gmsh(path(name), dim)
+    kinds, n = np.unique(el[:, -1], return_counts=True)
+    assert dict(zip(kinds.tolist(), n.tolist())) == counts
+    box = [(nd[:, k].min(), nd[:, k].max()) for k in range(dim)]
+    assert np.array_equal(nd[:, 3], _expected_tags(nd, box))
+    d = domain_from_gmsh(path(name), dim)
+    assert abs(d.cells.volume.sum() - measure) < 1e-12 * measure
+

@@ -308,7 +308,8 @@ def test_backend_compile_diamond_kernels_reference_signatures():
 
 
 # ---------------------------------------------------------------------------------------------
-# 3D diamond scheme (ast/functions3d.py) on jittered Kuhn tetrahedra and the reference's cube.msh
+# 3D diamond scheme (ast/functions3d.py) on jittered Kuhn tetrahedra, the reference's cube.msh and
+# its mixed tetrahedra / pyramids / hexahedra cube_bis.msh (quadrangular faces: D is a fourth node)
 
 LOCS3 = ("in", "out", "upper", "bottom", "front", "back")
 BCS3 = [
@@ -333,7 +334,7 @@ def _mesh3(kind, seed=0):
     from manapy_amd.ddm import Domain
     from manapy_amd.gmsh import read_gmsh
     here = os.path.dirname(os.path.abspath(__file__))
-    return Domain(3, mesh=read_gmsh(os.path.join(here, 'golden', 'meshes', 'cube.msh.gz'), 3))
+    return Domain(3, mesh=read_gmsh(os.path.join(here, 'golden', 'meshes', kind + '.msh.gz'), 3))
 
 
 def _oracle_system3(d, var, bcs):
@@ -348,7 +349,8 @@ def _oracle_system3(d, var, bcs):
     return data, row, col, Pn, Pf, rhs0
 
 
-@pytest.mark.parametrize("ib,mesh", [(0, "kuhn"), (1, "kuhn"), (2, "kuhn"), (0, "cube")])
+@pytest.mark.parametrize("ib,mesh", [(0, "kuhn"), (1, "kuhn"), (2, "kuhn"), (0, "cube"), (0, "cube_bis"),
+                                     (2, "cube_bis")])
 def test_assembly_rhs_bitexact_3d(ib, mesh):
     """get_triplet_3d (duplicates summed in insertion order) and get_rhs_glob_3d, bit for bit;
     the matrix size count of compute_3dmatrix_size."""
@@ -376,7 +378,7 @@ def test_assembly_rhs_bitexact_3d(ib, mesh):
     assert np.array_equal(L.rhs0.cpu().numpy(), rhs0)
 
 
-@pytest.mark.parametrize("mesh", ["kuhn", "cube"])
+@pytest.mark.parametrize("mesh", ["kuhn", "cube", "cube_bis"])
 def test_node_face_gradient_dissipative_bitexact_3d(mesh):
     """centertovertex_3d, face_gradient_3d (x, y, z), the dissipative residual with Dzz and the
     diffusion time step in 3D, bit for bit."""
@@ -434,7 +436,7 @@ def test_p_gradient_bitexact_3d():
     assert np.array_equal(P.gradfacez.cpu().numpy(), pz)
 
 
-@pytest.mark.parametrize("mesh", ["kuhn", "cube"])
+@pytest.mark.parametrize("mesh", ["kuhn", "cube", "cube_bis"])
 def test_linear_solve_3d_matches_superlu(mesh):
     """A 3D Poisson problem with Dirichlet data on two sides and a source: the GPU solution meets
     ||b - A x|| <= 1e-10 ||b|| for the oracle's FP64 matrix and agrees with SuperLU."""
@@ -457,3 +459,47 @@ def test_linear_solve_3d_matches_superlu(mesh):
     assert relres <= 1e-10, (relres, L.stats)
     xs = sla.spsolve(A.tocsc(), b)
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 5e-7
+
+
+def test_narrow_columns_give_the_same_products():
+    """The 16-bit column offsets (mfv_ell_narrow_columns) are used on the slices of a banded
+    numbering and refused where a column lies farther than 32767 rows away; the FP64 product
+    and a whole solve are bit-identical with and without them."""
+    import torch
+    from manapy_amd import _lib
+    from manapy_amd.device import stream_ptr
+    from manapy_amd.ddm import Domain
+    from manapy_amd.krylov import EllBatch
+    from manapy_amd.meshgen import jittered_triangles
+    el, nd = jittered_triangles(200, 200, 1.0, 1.0, 0.1, 77)     # 80 000 cells, row-major order
+    perm = np.random.default_rng(5).permutation(len(el))
+    for d, expect_narrow in ((Domain(2, mesh=(el, nd)), True), (Domain(2, mesh=(el[perm], nd)), False)):
+        C = int(d.nbcells)
+        dd = d.device_diamond
+        node_dir = torch.zeros(int(d.nbnodes), dtype=torch.uint8, device='cuda')
+        rank = torch.full((int(d.nbfaces),), -1, dtype=torch.int32, device='cuda')
+        b = EllBatch((C + 63) // 64, max_inst=1, width=32)
+        inst = b.alloc(C)
+        b.publish()
+        b.assemble_diamond(inst, dd, node_dir, rank)
+        ns = (C + 63) // 64
+        frac = float(b.slice_narrow[:ns].float().mean())
+        assert (frac == 1.0) if expect_narrow else (frac < 0.5), frac
+        x = torch.as_tensor(np.random.default_rng(1).standard_normal(ns * 64), device='cuda')
+        y1, y2 = torch.zeros_like(x), torch.zeros_like(x)
+        _lib.call('mfv_ell_spmv', b.eptr, x.data_ptr(), y1.data_ptr(), 256, stream_ptr())
+        b.narrow_columns(inst, enable=False)
+        _lib.call('mfv_ell_spmv', b.eptr, x.data_ptr(), y2.data_ptr(), 256, stream_ptr())
+        assert torch.equal(y1, y2)
+        sols = []
+        rhs = torch.as_tensor(np.random.default_rng(2).standard_normal(C), device='cuda')
+        for enable in (True, False):
+            b.narrow_columns(inst, enable=enable)
+            b.set_system(inst, 1.0, -2e-4, rhs)
+            for _ in range(200):
+                b.tick(8)
+                if int(b.status[inst]) in (_lib.KS_DONE, _lib.KS_FAILED):
+                    break
+            assert int(b.status[inst]) == _lib.KS_DONE
+            sols.append(b.x[:C].clone())
+        assert torch.equal(sols[0], sols[1])

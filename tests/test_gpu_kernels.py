@@ -485,7 +485,7 @@ def test_euler_tiled_dt_ignores_stale_partials(torch, packed, oracle_lib):
     assert float(host(ring)[1]) == dt_next
 
 
-@pytest.mark.parametrize("quirk,T,n", [(0, 256, 8), (1, 128, 8), (0, 128, 11), (1, 256, 11)])
+@pytest.mark.parametrize("quirk,T,n", [(0, 256, 8), (1, 128, 8), (0, 128, 11), (1, 256, 11), (1, 512, 11)])
 def test_euler32_tiled_matches_model_and_fp64(torch, quirk, T, n, oracle_lib):
     """FP32-bulk Euler step on face tiles: the float32 records and the dt ring equal its CPU
     model (oracle/euler32.c) bit for bit after 200 steps; every field within 1e-5 relative L1 of

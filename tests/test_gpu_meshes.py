@@ -1,5 +1,7 @@
 """GPU parity on the reference's own Gmsh meshes (mesh/carre.msh: triangles; carre_hybrid.msh:
-triangles and quadrangles around two holes; rectangle.msh; cube.msh: tetrahedra), read by the
+triangles and quadrangles around two holes; rectangle.msh; rectangle_RTM.msh; carre_structure.msh:
+triangles and quadrangles; cube.msh: tetrahedra; cube_bis.msh: tetrahedra, pyramids and
+hexahedra), read by the
 Gmsh 4.1 reader: advection, the float64 shallow-water and Euler steps, bit for bit against the
 oracle; the mixed-precision and FP32-bulk steps bit for bit against their float32 models."""
 import os
@@ -47,7 +49,9 @@ def P(t):
 
 
 @pytest.mark.parametrize("name,dim,order", [("carre", 2, 1), ("carre", 2, 2), ("carre_hybrid", 2, 2),
-                                            ("rectangle", 2, 2), ("cube", 3, 1), ("cube", 3, 2)])
+                                            ("rectangle", 2, 2), ("cube", 3, 1), ("cube", 3, 2),
+                                            ("carre_structure", 2, 2), ("rectangle_RTM", 2, 2),
+                                            ("cube_bis", 3, 1), ("cube_bis", 3, 2)])
 def test_advection_on_reference_meshes(torch, name, dim, order, oracle_lib):
     from manapy_amd.device import DeviceMesh
     d = domain(name, dim)
@@ -86,7 +90,7 @@ def _sw_case(d):
     return Z, h
 
 
-@pytest.mark.parametrize("name", ["carre", "carre_hybrid", "rectangle"])
+@pytest.mark.parametrize("name", ["carre", "carre_hybrid", "rectangle", "carre_structure", "rectangle_RTM"])
 def test_sw_float64_and_mixed_on_reference_meshes(torch, name, oracle_lib):
     """The bench's float64 path (packed records, fused tiled step) and the two-kernel packed
     step against 40 oracle steps of the dam break over bumps (walls, also around the hybrid
