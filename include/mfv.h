@@ -818,6 +818,43 @@ int mfv_domain_geometry_2d_build(int C, int ew, const int32_t *nodeid, int N, co
                                  mfv_domain_geometry_2d *out);
 void mfv_domain_geometry_2d_free(mfv_domain_geometry_2d *g);
 
+/* The 3D geometry of a Domain from its topology (Compute_3dcentervolumeOfCell,
+ * create_info_3dfaces, oriente_3dfacenodeid, face_info_3d ghost mirrors and per-node ghost
+ * lists, create_node_ghostid, variables_3d, face_gradient_info_3d, dist_ortho_function;
+ * ddm_utils3d.py:87-157, 362-403, 459-527, 530-764, 813-981, ddm_utils2d.py:122-161, 716-744),
+ * the same floating-point expressions in the same order as the reference's loops, on host
+ * threads. Cells: tetrahedra, pyramids, hexahedra. Boundary faces are those named 1-6, in
+ * ascending order. face_nodeid is the face table with each face's nodes re-oriented (the
+ * caller's table is not changed). Host arrays, malloc'ed; free them with
+ * mfv_domain_geometry_3d_free. */
+typedef struct mfv_domain_geometry_3d {
+    int32_t ghost_width, ghostnid_width;
+    double *cell_center;        /* [C][3] */
+    double *cell_volume;        /* [C] */
+    int32_t *face_name;         /* [F] */
+    double *face_normal;        /* [F][3] */
+    double *face_mesure;        /* [F] */
+    double *face_center;        /* [F][3] */
+    double *face_tangent;       /* [F][3] */
+    double *face_binormal;      /* [F][3] */
+    int32_t *face_nodeid;       /* [F][fw] oriented */
+    double *face_ghostcenter;   /* [F][4] (x, y, z, gamma; -1 off the boundary) */
+    double *dist_ortho;         /* [F] */
+    double *airDiamond, *param1, *param2, *param3; /* [F] */
+    double *f_1, *f_2;          /* [F][3] */
+    double *lambda_x, *lambda_y, *lambda_z, *R_x, *R_y, *R_z; /* [N] */
+    int32_t *number;            /* [N] */
+    double *node_ghostcenter;   /* [N][ghost_width][6] */
+    double *node_ghostfaceinfo; /* [N][ghost_width][6] */
+    int32_t *node_ghostid;      /* [N][ghost_width + 1] */
+    int32_t *cell_ghostnid;     /* [C][ghostnid_width + 1] */
+} mfv_domain_geometry_3d;
+
+int mfv_domain_geometry_3d_build(int C, int ew, const int32_t *nodeid, int N, const double *vertex, int F, int fw,
+                                 const int32_t *faces_nodeid, const int32_t *faces_cellid, const int32_t *node_cellid,
+                                 int ncw1, mfv_domain_geometry_3d *out);
+void mfv_domain_geometry_3d_free(mfv_domain_geometry_3d *g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,6 +112,16 @@ class MfvDomainGeometry2d(ctypes.Structure):
                                  'cell_ghostnid')]
 
 
+class MfvDomainGeometry3d(ctypes.Structure):
+    """struct mfv_domain_geometry_3d: host arrays of the 3D Domain geometry (malloc'ed)."""
+    _fields_ = [('ghost_width', ctypes.c_int32), ('ghostnid_width', ctypes.c_int32)] + \
+               [(k, P) for k in ('cell_center', 'cell_volume', 'face_name', 'face_normal', 'face_mesure',
+                                 'face_center', 'face_tangent', 'face_binormal', 'face_nodeid', 'face_ghostcenter',
+                                 'dist_ortho', 'airDiamond', 'param1', 'param2', 'param3', 'f_1', 'f_2', 'lambda_x',
+                                 'lambda_y', 'lambda_z', 'R_x', 'R_y', 'R_z', 'number', 'node_ghostcenter',
+                                 'node_ghostfaceinfo', 'node_ghostid', 'cell_ghostnid')]
+
+
 class MfvSwPacked(ctypes.Structure):
     """Mirror of ``struct mfv_sw_packed`` (include/mfv.h)."""
     _fields_ = [('nb_cells', ctypes.c_int32), ('max_cell_faces', ctypes.c_int32), ('cellrec', P), ('facerec', P),
@@ -263,6 +273,9 @@ _SIGNATURES = {
     'mfv_domain_geometry_2d_build': [c_int, c_int, P, c_int, P, c_int, P, c_int, P, P, P, c_int,
                                      ctypes.POINTER(MfvDomainGeometry2d)],
     'mfv_domain_geometry_2d_free': [ctypes.POINTER(MfvDomainGeometry2d)],
+    'mfv_domain_geometry_3d_build': [c_int, c_int, P, c_int, P, c_int, c_int, P, P, P, c_int,
+                                     ctypes.POINTER(MfvDomainGeometry3d)],
+    'mfv_domain_geometry_3d_free': [ctypes.POINTER(MfvDomainGeometry3d)],
     'mfv_krylov_tick': [ELL_P, KW_P, c_int, c_int, c_int, P],
     'mfv_krylov_graph': [ELL_P, KW_P, c_int, c_int, c_int, ctypes.POINTER(c_void_p)],
     'mfv_ell_spmv': [ELL_P, P, P, c_int, P],
@@ -270,7 +283,7 @@ _SIGNATURES = {
 }
 _RESTYPE = {'mfv_version': c_char_p, 'mfv_last_error': c_char_p, 'mfv_sw_tiles_host_free': None,
             'mfv_face_tiles_host_free': None, 'mfv_euler_tet_host_free': None, 'mfv_domain_topology_free': None,
-            'mfv_domain_geometry_2d_free': None}
+            'mfv_domain_geometry_2d_free': None, 'mfv_domain_geometry_3d_free': None}
 
 _lib = None
 

@@ -585,3 +585,383 @@ extern "C" int mfv_domain_geometry_2d_build(int C, int ew, const int32_t *nodeid
     out->cell_ghostnid = copy_out(gnid);
     return MFV_OK;
 }
+
+// ---------------------------------------------------------------------------------------------
+// 3D geometry, face orientation, ghost mirrors and diamond coefficients (ddm_utils3d.py:87-157,
+// 362-403, 459-527, 530-764, 813-981; ddm_utils2d.py:122-161, 716-744), the NumPy construction's
+// expressions in its order, so the arrays are identical; loops run on host threads.
+
+namespace {
+
+struct V3 {
+    double x, y, z;
+};
+
+// outputs are malloc'ed (freed by mfv_domain_geometry_3d_free); a V3 array is a [n][3] double array
+template <typename T>
+T *alloc_out(size_t n)
+{
+    return static_cast<T *>(malloc((n ? n : 1) * sizeof(T)));
+}
+
+V3 *alloc_v3(double *&slot, size_t n)
+{
+    slot = alloc_out<double>(3 * n);
+    return reinterpret_cast<V3 *>(slot);
+}
+
+inline V3 v3sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 v3add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 v3scale(double s, V3 v) { return {s * v.x, s * v.y, s * v.z}; }
+inline V3 v3cross(V3 u, V3 v) { return {u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x}; }
+inline double v3dot(V3 u, V3 v) { return u.x * v.x + u.y * v.y + u.z * v.z; }
+
+// create_info_3dfaces' boundary naming (ddm_utils3d.py:549-593, 615-669)
+int name3d(const int *nm, int k)
+{
+    bool same = true;
+    for (int j = 1; j < k; j++) same = same && nm[j] == nm[0];
+    if (same) return nm[0];
+    const int tags[8] = {5, 6, 3, 4, 55, 66, 33, 44};
+    for (int tag : tags)
+        for (int j = 0; j < k; j++) {
+            if (nm[j] != tag) continue;
+            bool others = true;
+            for (int l = 0; l < k; l++)
+                if (l != j) others = others && nm[l] != 0;
+            if (others) return tag;
+        }
+    return k == 3 ? 100 : 200;
+}
+
+}  // namespace
+
+extern "C" void mfv_domain_geometry_3d_free(mfv_domain_geometry_3d *g)
+{
+    if (!g) return;
+    double **d[] = {&g->cell_center, &g->cell_volume, &g->face_normal, &g->face_mesure, &g->face_center,
+                    &g->face_tangent, &g->face_binormal, &g->face_ghostcenter, &g->dist_ortho, &g->airDiamond,
+                    &g->param1, &g->param2, &g->param3, &g->f_1, &g->f_2, &g->lambda_x, &g->lambda_y,
+                    &g->lambda_z, &g->R_x, &g->R_y, &g->R_z, &g->node_ghostcenter, &g->node_ghostfaceinfo};
+    for (double **p : d) free(*p);
+    free(g->face_name);
+    free(g->face_nodeid);
+    free(g->number);
+    free(g->node_ghostid);
+    free(g->cell_ghostnid);
+    memset(g, 0, sizeof(*g));
+}
+
+extern "C" int mfv_domain_geometry_3d_build(int C, int ew, const int32_t *nodeid, int N, const double *vertex, int F,
+                                            int fw, const int32_t *faces_nodeid, const int32_t *faces_cellid,
+                                            const int32_t *node_cellid, int ncw1, mfv_domain_geometry_3d *out)
+{
+    MFV_REQUIRE(out, "out is NULL");
+    memset(out, 0, sizeof(*out));
+    MFV_REQUIRE(C > 0 && N > 0 && F > 0 && nodeid && vertex && faces_nodeid && faces_cellid && node_cellid &&
+                    ew >= 5 && (fw == 4 || fw == 5) && ncw1 >= 2,
+                "bad arguments");
+    auto P = [&](int n) { return V3{vertex[4 * (size_t)n], vertex[4 * (size_t)n + 1], vertex[4 * (size_t)n + 2]}; };
+    auto tag = [&](int n) { return (int)vertex[4 * (size_t)n + 3]; };
+    // Compute_3dcentervolumeOfCell (ddm_utils3d.py:686-764): tetrahedra, pyramids and hexahedra
+    for (int i = 0; i < C; i++) {
+        const int nv = nodeid[(size_t)i * ew + ew - 1];
+        MFV_REQUIRE(nv == 4 || nv == 5 || nv == 8, "cell %d has %d vertices: tetrahedra, pyramids and hexahedra only",
+                    i, nv);
+    }
+    // outputs of known size are allocated up front and filled in place by the threads
+    V3 *cc = alloc_v3(out->cell_center, C);
+    V3 *nrm = alloc_v3(out->face_normal, F), *cf = alloc_v3(out->face_center, F);
+    V3 *tan = alloc_v3(out->face_tangent, F), *bin = alloc_v3(out->face_binormal, F);
+    V3 *n1v = alloc_v3(out->f_1, F), *n2v = alloc_v3(out->f_2, F);
+    double *vol = out->cell_volume = alloc_out<double>(C);
+    int32_t *name = out->face_name = alloc_out<int32_t>(F);
+    int32_t *fn = out->face_nodeid = alloc_out<int32_t>((size_t)F * fw);
+    double *mes = out->face_mesure = alloc_out<double>(F);
+    double *gc = out->face_ghostcenter = alloc_out<double>((size_t)F * 4);
+    double *air = out->airDiamond = alloc_out<double>(F);
+    double *p1 = out->param1 = alloc_out<double>(F), *p2 = out->param2 = alloc_out<double>(F),
+           *p3 = out->param3 = alloc_out<double>(F);
+    double *dort = out->dist_ortho = alloc_out<double>(F);
+    double *lx = out->lambda_x = alloc_out<double>(N), *ly = out->lambda_y = alloc_out<double>(N),
+           *lz = out->lambda_z = alloc_out<double>(N);
+    double *Rx = out->R_x = alloc_out<double>(N), *Ry = out->R_y = alloc_out<double>(N),
+           *Rz = out->R_z = alloc_out<double>(N);
+    int32_t *number = out->number = alloc_out<int32_t>(N);
+    if (!cc || !vol || !name || !fn || !nrm || !cf || !tan || !bin || !n1v || !n2v || !mes || !gc || !air || !p1 ||
+        !p2 || !p3 || !dort || !lx || !ly || !lz || !Rx || !Ry || !Rz || !number) {
+        mfv_domain_geometry_3d_free(out);
+        set_error("out of host memory");
+        return MFV_ENOMEM;
+    }
+    parallel_for(C, [&](int a, int b) {
+        static const int hexr[6][4] = {{0, 1, 3, 4}, {1, 3, 4, 5}, {4, 5, 3, 7}, {1, 3, 5, 2}, {3, 7, 5, 2}, {5, 7, 6, 2}};
+        static const int pyr[2][4] = {{0, 1, 2, 4}, {0, 2, 3, 4}};
+        for (int i = a; i < b; i++) {
+            const int32_t *s = nodeid + (size_t)i * ew;
+            const int nv = s[ew - 1];
+            V3 vt[8];
+            V3 acc{0., 0., 0.};
+            for (int k = 0; k < nv; k++) {
+                vt[k] = P(s[k]);
+                acc = v3add(acc, vt[k]);
+            }
+            cc[i] = V3{acc.x / nv, acc.y / nv, acc.z / nv};
+            auto tet = [&](int r0, int r1, int r2, int r3) {
+                const V3 u = v3sub(vt[r1], vt[r0]), v = v3sub(vt[r2], vt[r0]), w = v3sub(vt[r3], vt[r0]);
+                return fabs(v3dot(u, v3cross(v, w))) / 6.0;
+            };
+            if (nv == 4) {
+                vol[i] = tet(0, 1, 2, 3);
+            } else {
+                double v = 0.;
+                const int nr = nv == 8 ? 6 : 2;
+                for (int r = 0; r < nr; r++) {
+                    const int *q = nv == 8 ? hexr[r] : pyr[r];
+                    v += tet(q[0], q[1], q[2], q[3]);
+                }
+                vol[i] = v;
+            }
+        }
+    });
+    // create_info_3dfaces (ddm_utils3d.py:530-684): centres, areas, names, outward normals,
+    // tangents and binormals; then oriente_3dfacenodeid (:362-403) on a copy of the face table
+    parallel_for(F, [&](int a, int b) {
+        for (int f = a; f < b; f++) {
+            int32_t *r = &fn[(size_t)f * fw];
+            for (int j = 0; j < fw; j++) r[j] = faces_nodeid[(size_t)f * fw + j];
+            name[f] = 0;
+            const int k = r[fw - 1];
+            V3 v[4];
+            for (int j = 0; j < k; j++) v[j] = P(r[j]);
+            V3 norm;
+            if (k == 3) {
+                const V3 s3 = v3add(v3add(v[0], v[1]), v[2]);
+                cf[f] = v3scale(1. / 3, s3);
+                norm = v3scale(0.5, v3cross(v3sub(v[1], v[0]), v3sub(v[2], v[0])));
+                mes[f] = sqrt(norm.x * norm.x + norm.y * norm.y + norm.z * norm.z);
+            } else {
+                const V3 s4 = v3add(v3add(v3add(v[0], v[1]), v[2]), v[3]);
+                cf[f] = v3scale(1. / 4, s4);
+                const V3 a1 = v3scale(0.5, v3cross(v3sub(v[1], v[0]), v3sub(v[2], v[0])));
+                norm = v3add(a1, a1);  // area2 = area1 (the reference's quadrangle area, ddm_utils3d.py:605)
+                const double mag = sqrt(a1.x * a1.x + a1.y * a1.y + a1.z * a1.z);
+                mes[f] = mag + mag;
+            }
+            if (faces_cellid[2 * (size_t)f + 1] == -1) {
+                int nm[4];
+                for (int j = 0; j < k; j++) nm[j] = tag(r[j]);
+                name[f] = name3d(nm, k);
+            }
+            const V3 snorm = v3sub(cc[faces_cellid[2 * (size_t)f]], cf[f]);
+            const V3 n = v3dot(snorm, norm) > 0 ? v3scale(-1, norm) : norm;
+            nrm[f] = n;
+            const V3 t = v3sub(P(r[1]), P(r[0]));
+            tan[f] = t;
+            bin[f] = v3scale(0.5, v3cross(t, n));
+            if (k == 3) {
+                const int n1 = r[0], n2 = r[1], n3 = r[2];
+                const V3 e1 = v3sub(P(n2), P(n1)), e2 = v3sub(P(n3), P(n1));
+                if (v3dot(v3cross(e1, e2), n) < 0) {
+                    r[1] = n3;
+                    r[2] = n2;
+                }
+            } else {
+                const int s[4] = {r[0], r[1], r[2], r[3]};
+                auto cz = [&](int p, int q) { return P(p).x * P(q).y - P(p).y * P(q).x; };
+                auto orient = [&](int p, int q, int w) { return cz(p, q) + cz(q, w) + cz(w, p); };
+                const bool o123 = orient(s[0], s[1], s[2]) < 0, o134 = orient(s[0], s[2], s[3]) < 0,
+                           o124 = orient(s[0], s[1], s[3]) < 0;
+                if (o123 && !o134) {
+                    if (o124) {
+                        r[2] = s[3];
+                        r[3] = s[2];
+                    } else {
+                        r[0] = s[3];
+                        r[3] = s[0];
+                    }
+                } else if (!o123 && o134) {
+                    if (o124) {
+                        r[1] = s[2];
+                        r[2] = s[1];
+                    } else {
+                        r[0] = s[1];
+                        r[1] = s[0];
+                    }
+                } else if (!o123 && !o134) {
+                    r[0] = s[2];
+                    r[2] = s[0];
+                }
+            }
+        }
+    });
+    // face_info_3d (ddm_utils3d.py:87-157): mirror centres of the boundary faces (names 1-6)
+    std::vector<int32_t> bf;
+    for (int f = 0; f < F; f++)
+        if (name[f] >= 1 && name[f] <= 6) bf.push_back(f);
+    const int nb = (int)bf.size();
+    parallel_for(F, [&](int a, int b) {
+        for (size_t q = 4 * (size_t)a; q < 4 * (size_t)b; q++) gc[q] = -1.;
+    });
+    parallel_for(nb, [&](int a, int b) {
+        for (int q = a; q < b; q++) {
+            const int f = bf[q];
+            const V3 n{nrm[f].x / mes[f], nrm[f].y / mes[f], nrm[f].z / mes[f]};
+            const V3 v1 = cc[faces_cellid[2 * (size_t)f]];
+            const double gamma = v3dot(v3sub(cf[f], v1), n);
+            const V3 kk = v3add(v1, v3scale(gamma, n));
+            const V3 v2 = v3add(v3scale(2, kk), v3scale(-1, v1));
+            gc[4 * (size_t)f] = v2.x;
+            gc[4 * (size_t)f + 1] = v2.y;
+            gc[4 * (size_t)f + 2] = v2.z;
+            gc[4 * (size_t)f + 3] = gamma;
+        }
+    });
+    // per-node ghost lists, by boundary face then by the node's slot in it (nodes 1, 0, 2, 3)
+    const int slot_node[4] = {1, 0, 2, 3};
+    auto nslots = [&](int f) { return fn[(size_t)f * fw + fw - 1] == 4 ? 4 : 3; };
+    std::vector<int32_t> gcount(N, 0);
+    for (int q = 0; q < nb; q++)
+        for (int s = 0; s < nslots(bf[q]); s++) gcount[fn[(size_t)bf[q] * fw + slot_node[s]]]++;
+    int g = 1;
+    for (int v = 0; v < N; v++) g = std::max(g, (int)gcount[v]);
+    double *gcn = alloc_out<double>((size_t)N * g * 6), *gfi = alloc_out<double>((size_t)N * g * 6);
+    int32_t *gid = alloc_out<int32_t>((size_t)N * (g + 1));
+    out->ghost_width = g;
+    out->node_ghostcenter = gcn;
+    out->node_ghostfaceinfo = gfi;
+    out->node_ghostid = gid;
+    if (!gcn || !gfi || !gid) {
+        mfv_domain_geometry_3d_free(out);
+        set_error("out of host memory");
+        return MFV_ENOMEM;
+    }
+    parallel_for(N, [&](int a, int b) {
+        std::fill(gcn + (size_t)a * g * 6, gcn + (size_t)b * g * 6, -1.);
+        std::fill(gfi + (size_t)a * g * 6, gfi + (size_t)b * g * 6, -1.);
+        std::fill(gid + (size_t)a * (g + 1), gid + (size_t)b * (g + 1), -1);
+    });
+    std::vector<int32_t> gfill(N, 0);
+    for (int q = 0; q < nb; q++) {
+        const int f = bf[q];
+        for (int s = 0; s < nslots(f); s++) {
+            const int v = fn[(size_t)f * fw + slot_node[s]];
+            const int p = gfill[v]++;
+            double *r = &gcn[((size_t)v * g + p) * 6];
+            r[0] = gc[4 * (size_t)f];
+            r[1] = gc[4 * (size_t)f + 1];
+            r[2] = gc[4 * (size_t)f + 2];
+            r[3] = (double)faces_cellid[2 * (size_t)f];
+            r[4] = (double)name[f];
+            r[5] = (double)f;
+            double *t = &gfi[((size_t)v * g + p) * 6];
+            t[0] = cf[f].x;
+            t[1] = cf[f].y;
+            t[2] = cf[f].z;
+            t[3] = nrm[f].x;
+            t[4] = nrm[f].y;
+            t[5] = nrm[f].z;
+            gid[(size_t)v * (g + 1) + p] = f;
+        }
+    }
+    for (int v = 0; v < N; v++) gid[(size_t)v * (g + 1) + g] = gcount[v];
+    // cells.ghostnid: the sorted union of the ghost faces of the cell's nodes
+    std::vector<std::vector<int32_t>> cg(C);
+    parallel_for(C, [&](int a, int b) {
+        for (int i = a; i < b; i++) {
+            const int32_t *s = nodeid + (size_t)i * ew;
+            std::vector<int32_t> &l = cg[i];
+            for (int k = 0; k < s[ew - 1]; k++)
+                for (int p = 0; p < gcount[s[k]]; p++) l.push_back(gid[(size_t)s[k] * (g + 1) + p]);
+            std::sort(l.begin(), l.end());
+            l.erase(std::unique(l.begin(), l.end()), l.end());
+        }
+    });
+    int gw = 0;
+    for (int i = 0; i < C; i++) gw = std::max(gw, (int)cg[i].size());
+    int32_t *gnid = alloc_out<int32_t>((size_t)C * (gw + 1));
+    out->ghostnid_width = gw;
+    out->cell_ghostnid = gnid;
+    if (!gnid) {
+        mfv_domain_geometry_3d_free(out);
+        set_error("out of host memory");
+        return MFV_ENOMEM;
+    }
+    parallel_for(C, [&](int a, int b) {
+        for (int i = a; i < b; i++) {
+            int32_t *row = gnid + (size_t)i * (gw + 1);
+            std::fill(row, row + gw, -1);
+            std::copy(cg[i].begin(), cg[i].end(), row);
+            row[gw] = (int32_t)cg[i].size();
+        }
+    });
+    // variables_3d (ddm_utils3d.py:813-981): the node's cells, then its ghost mirror centres
+    parallel_for(N, [&](int a, int b) {
+        for (int v = a; v < b; v++) {
+            double xx = 0., yy = 0., zz = 0., xy = 0., xz = 0., yz = 0., rx = 0., ry = 0., rz = 0.;
+            int num = 0;
+            const V3 vt = P(v);
+            auto add = [&](V3 p) {
+                const double dx = p.x - vt.x, dy = p.y - vt.y, dz = p.z - vt.z;
+                xx += dx * dx;
+                yy += dy * dy;
+                zz += dz * dz;
+                xy += dx * dy;
+                xz += dx * dz;
+                yz += dy * dz;
+                rx += dx;
+                ry += dy;
+                rz += dz;
+                num++;
+            };
+            const int nc = node_cellid[(size_t)v * ncw1 + ncw1 - 1];
+            for (int j = 0; j < nc; j++) add(cc[node_cellid[(size_t)v * ncw1 + j]]);
+            for (int j = 0; j < gcount[v]; j++) {
+                const int f = gid[(size_t)v * (g + 1) + j];
+                add(V3{gc[4 * (size_t)f], gc[4 * (size_t)f + 1], gc[4 * (size_t)f + 2]});
+            }
+            const double D = xx * yy * zz + 2 * xy * xz * yz - xx * yz * yz - yy * xz * xz - zz * xy * xy;
+            lx[v] = ((yz * yz - yy * zz) * rx + (xy * zz - xz * yz) * ry + (xz * yy - xy * yz) * rz) / D;
+            ly[v] = ((xy * zz - xz * yz) * rx + (xz * xz - xx * zz) * ry + (yz * xx - xz * xy) * rz) / D;
+            lz[v] = ((xz * yy - xy * yz) * rx + (yz * xx - xz * xy) * ry + (xy * xy - xx * yy) * rz) / D;
+            Rx[v] = rx;
+            Ry[v] = ry;
+            Rz[v] = rz;
+            number[v] = num;
+        }
+    });
+    // face_gradient_info_3d (ddm_utils3d.py:459-527) and dist_ortho (ddm_utils2d.py:716-744)
+    parallel_for(F, [&](int a, int b) {
+        for (int f = a; f < b; f++) {
+            const int32_t *r = &fn[(size_t)f * fw];
+            const int i1 = r[0], i2 = r[1], i3 = r[2];
+            const int i4 = (fw > 4 && r[fw - 1] == 4) ? r[3] : i3;
+            const int c0 = faces_cellid[2 * (size_t)f], c1 = faces_cellid[2 * (size_t)f + 1];
+            const V3 v1 = cc[c0];
+            const V3 v2 = name[f] == 0 ? cc[c1] : V3{gc[4 * (size_t)f], gc[4 * (size_t)f + 1], gc[4 * (size_t)f + 2]};
+            const V3 s1 = v3sub(v2, P(i2)), s2 = v3sub(P(i4), P(i2)), s3 = v3sub(v1, P(i2));
+            const V3 n1 = v3add(v3scale(0.5, v3cross(s1, s2)), v3scale(0.5, v3cross(s2, s3)));
+            const V3 s4 = v3sub(v2, P(i3)), s5 = v3sub(P(i1), P(i3)), s6 = v3sub(v1, P(i3));
+            const V3 n2 = v3add(v3scale(0.5, v3cross(s4, s5)), v3scale(0.5, v3cross(s5, s6)));
+            const V3 n = nrm[f];
+            const double ar = v3dot(n, v3sub(v2, v1));
+            air[f] = ar;
+            p1[f] = v3dot(n1, n) / ar;
+            p2[f] = v3dot(n2, n) / ar;
+            p3[f] = v3dot(n, n) / ar;
+            n1v[f] = n1;
+            n2v[f] = n2;
+            auto proj = [&](int K) {
+                const double vx = cc[K].x - cf[f].x, vy = cc[K].y - cf[f].y;
+                const double s = vx * n.x + vy * n.y;
+                const double px = cc[K].x - s * n.x, py = cc[K].y - s * n.y;
+                const double dx = cc[K].x - px, dy = cc[K].y - py;
+                return sqrt(dx * dx + dy * dy);
+            };
+            if (name[f] == 0) dort[f] = proj(c0) + proj(c1);
+            else if (name[f] >= 1 && name[f] <= 6) dort[f] = 2 * proj(c0);
+            else dort[f] = 0.;
+        }
+    });
+    return MFV_OK;
+}

@@ -489,6 +489,52 @@ def native_geometry_2d(d):
     return out
 
 
+def native_geometry_3d(d):
+    """The 3D geometry from the library's host builder (``mfv_domain_geometry_3d_build``, C++
+    on host threads): the arrays of the NumPy stages below, identical, with the face table
+    re-oriented as ``oriente_3dfacenodeid`` leaves it."""
+    import ctypes
+    from . import _lib
+    nodeid = np.ascontiguousarray(d._cells.nodeid, dtype=I32)
+    vertex = np.ascontiguousarray(d._nodes.vertex, dtype=F64)
+    fn = np.ascontiguousarray(d._faces.nodeid, dtype=I32)
+    cellid = np.ascontiguousarray(d._faces.cellid, dtype=I32)
+    ncid = np.ascontiguousarray(d._nodes.cellid, dtype=I32)
+    C, N, F, fw = len(nodeid), len(vertex), len(fn), fn.shape[1]
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    g = _lib.MfvDomainGeometry3d()
+    _lib.call('mfv_domain_geometry_3d_build', C, nodeid.shape[1], ptr(nodeid), N, ptr(vertex), F, fw, ptr(fn),
+              ptr(cellid), ptr(ncid), ncid.shape[1], ctypes.byref(g))
+    try:
+        def grab(p, shape, ct=ctypes.c_double):
+            n = int(np.prod(shape))
+            if n == 0:
+                return np.zeros(shape, dtype=np.float64 if ct is ctypes.c_double else I32)
+            return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=shape).copy()
+        gw, nw = int(g.ghost_width), int(g.ghostnid_width)
+        out = {'center': grab(g.cell_center, (C, 3)), 'volume': grab(g.cell_volume, (C,)),
+               'name': grab(g.face_name, (F,), ctypes.c_int32), 'normal': grab(g.face_normal, (F, 3)),
+               'mesure': grab(g.face_mesure, (F,)), 'fcenter': grab(g.face_center, (F, 3)),
+               'tangent': grab(g.face_tangent, (F, 3)), 'binormal': grab(g.face_binormal, (F, 3)),
+               'fnodeid': grab(g.face_nodeid, (F, fw), ctypes.c_int32),
+               'ghostcenter': grab(g.face_ghostcenter, (F, 4)), 'dist_ortho': grab(g.dist_ortho, (F,)),
+               'airDiamond': grab(g.airDiamond, (F,)), 'f_1': grab(g.f_1, (F, 3)), 'f_2': grab(g.f_2, (F, 3))}
+        for k in ('param1', 'param2', 'param3'):
+            out[k] = grab(getattr(g, k), (F,))
+        for k in ('lambda_x', 'lambda_y', 'lambda_z', 'R_x', 'R_y', 'R_z'):
+            out[k] = grab(getattr(g, k), (N,))
+        out['param4'] = np.zeros(F)
+        out['f_3'], out['f_4'] = np.zeros((F, 3)), np.zeros((F, 3))
+        out['number'] = grab(g.number, (N,), ctypes.c_int32)
+        out['node_ghostcenter'] = grab(g.node_ghostcenter, (N, gw, 6))
+        out['node_ghostfaceinfo'] = grab(g.node_ghostfaceinfo, (N, gw, 6))
+        out['ghostid'] = grab(g.node_ghostid, (N, gw + 1), ctypes.c_int32)
+        out['ghostnid'] = grab(g.cell_ghostnid, (C, nw + 1), ctypes.c_int32)
+    finally:
+        _lib.call('mfv_domain_geometry_3d_free', ctypes.byref(g))
+    return out
+
+
 # ---------------------------------------------------------------------------------------------
 
 class Domain(object):
@@ -551,14 +597,14 @@ class Domain(object):
         self._read_partition(elements, vertex)
         self._define_bounds()
         self._geo = None
-        native2d = self._dim == 2 and self._native() is not None
-        if not native2d:
+        native = self._native() is not None
+        if not native:
             self._compute_cells_info()
         self._make_neighbors()
         self._define_eltypes()
         self._create_faces_cons()
-        if native2d:  # the 2D geometry in C++ (the NumPy stages below then only copy it in)
-            self._geo = native_geometry_2d(self)
+        if native:  # the geometry in C++ (the NumPy stages below then only copy it in)
+            self._geo = native_geometry_2d(self) if self._dim == 2 else native_geometry_3d(self)
             self._cells.center, self._cells.volume = self._geo['center'], self._geo['volume']
         self._define_face_info()
         self._make_halo_info()
@@ -570,6 +616,10 @@ class Domain(object):
             self._update_haloghost()
             self._compute_diamondCell_info()
             self._compute_orthocenter()
+        elif self._geo is not None:  # the C++ builder computed these stages already: copy them in
+            self._update_haloghost()
+            self._compute_diamondCell_info()
+            self._geo = None
         else:
             self._cells.lazy('ghostnid', self._update_haloghost)
             self._nodes.lazy('ghostid', self._update_haloghost)
@@ -706,7 +756,13 @@ class Domain(object):
             self._faces.normal = g['normal']
             self._faces.mesure = g['mesure']
             self._faces.center = g['fcenter']
-            self._cells.nf = g['nf']
+            if self._dim == 2:
+                self._cells.nf = g['nf']
+            else:
+                self._faces.tangent = g['tangent']
+                self._faces.binormal = g['binormal']
+                self._cells.nf = np.zeros((int(self._nbcells), int(self._maxcellfid), 3), dtype=F64)
+                self._faces.nodeid[...] = g['fnodeid']
         elif self._dim == 2:
             namef, normal, mesure, centerf = face_info_2d_geometry(
                 self._faces.cellid, self._faces.nodeid, self._nodes.name, self._nodes.vertex, self._cells.center)
@@ -940,8 +996,8 @@ class Domain(object):
             g = self._geo
             N = int(self._nbnodes)
             self._nodes.lambda_x, self._nodes.lambda_y = g['lambda_x'], g['lambda_y']
-            self._nodes.lambda_z = np.zeros(N)
-            self._nodes.R_x, self._nodes.R_y, self._nodes.R_z = g['R_x'], g['R_y'], np.zeros(N)
+            self._nodes.lambda_z = g.get('lambda_z', np.zeros(N))
+            self._nodes.R_x, self._nodes.R_y, self._nodes.R_z = g['R_x'], g['R_y'], g.get('R_z', np.zeros(N))
             self._nodes.number = g['number']
             for k in ('airDiamond', 'param1', 'param2', 'param3', 'param4', 'f_1', 'f_2', 'f_3', 'f_4'):
                 setattr(self._faces, k, g[k])

@@ -169,13 +169,14 @@ def test_order_tables(dom2d):
         assert [faces[L['order_sorted'][r, i]] for r in range(nf)] == sorted(faces)
 
 
-@pytest.mark.parametrize("case", ["tri", "hybrid", "tet", "cube"])
+@pytest.mark.parametrize("case", ["tri", "hybrid", "tet", "cube", "cube_bis"])
 def test_native_topology_equals_numpy(case):
     """The C++ connectivity builder (mfv_domain_topology_build) gives every array of the NumPy
     construction: face numbering and node rows, cells.faceid, faces.cellid, cells.cellfid,
     nodes.cellid, cells.cellnid, and everything derived from them. In 2D the geometry comes
     from C++ too (mfv_domain_geometry_2d_build: centers, volumes, normals, ghost mirrors, the
-    per-node ghost lists, the diamond weights and distances), bit for bit the NumPy stages'."""
+    per-node ghost lists, the diamond weights and distances), bit for bit the NumPy stages', and
+    in 3D likewise (mfv_domain_geometry_3d_build, tetrahedra, pyramids and hexahedra)."""
     import os
     from manapy_amd.ddm import Domain
     from manapy_amd.gmsh import read_gmsh
@@ -184,7 +185,8 @@ def test_native_topology_equals_numpy(case):
     mesh, dim = {"tri": (lambda: jittered_triangles(13, 9, 1., 1., 0.1, 2), 2),
                  "hybrid": (lambda: read_gmsh(os.path.join(here, 'golden', 'meshes', 'carre_hybrid.msh.gz'), 2), 2),
                  "tet": (lambda: kuhn_tetrahedra(4, 1.0, 0.08, 1), 3),
-                 "cube": (lambda: read_gmsh(os.path.join(here, 'golden', 'meshes', 'cube.msh.gz'), 3), 3)}[case]
+                 "cube": (lambda: read_gmsh(os.path.join(here, 'golden', 'meshes', 'cube.msh.gz'), 3), 3),
+                 "cube_bis": (lambda: read_gmsh(os.path.join(here, 'golden', 'meshes', 'cube_bis.msh.gz'), 3), 3)}[case]
     el, nd = mesh()
     a = Domain(dim, mesh=(el, nd), topology='native')
     b = Domain(dim, mesh=(el, nd), topology='numpy')
