@@ -600,6 +600,26 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, mcell_fp64, kernel='auto', tile
     }
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """HBM bandwidth of a plain device-to-device copy of ``nbytes`` (bytes read + written per
+    second, HIP events around ``reps`` copies after two untimed ones): the measured ceiling the
+    roofline fractions can be read against, beside the 8 TB/s of the specification."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device='cuda')
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / reps
+    del src, dst
+    return 2 * nbytes / t / 1e9
+
+
 def roofline(kernel, bytes_per_cell, cells, t_step, model_bytes=None, traffic=None):
     """Roofline entry of one step: algorithmic bytes per cell-update (the hot-path survey's
     count) x cells / the device time per step from HIP events over the timed region."""
@@ -756,7 +776,9 @@ def run_ensemble(n_inst, steps, seed=0, n_check=None):
         "value": round(mcell, 3), "unit": "Mcell-updates/s", "instances": n_inst, "cells": cells,
         "steps": steps, "ms_per_step": round(t_loop / steps * 1e3, 4), "graph": "one captured graph of all steps",
         "config": list(tune['config']), "autotune_ms_per_step": tune['ms_per_step'],
-        "roofline": roofline("k_sw_step32_tiled (all instances) + k_batch_dt_fold", SURVEY_B_MIXED, cells, t_dev),
+        "roofline": roofline("k_sw_step32_tiled (all instances) + k_batch_dt_fold", SURVEY_B_MIXED, cells, t_dev,
+                             traffic=measured_traffic("k_sw_step32_tiled:batch:%s:%d:%d" % tuple(tune['config']),
+                                                      cells)),
         "error_vs_fp64": {"instances_checked": len(pick),
                           "max_rel_L1": {nm: max(x[nm] for x in errs) for nm in ('h', 'hu', 'hv')},
                           "median_rel_L1": {nm: float(np.median([x[nm] for x in errs])) for nm in ('h', 'hu', 'hv')}},
@@ -1035,6 +1057,10 @@ def main():
         "mesh_build_s": round(t_mesh, 2),
     }
     log("fp64: %.1f Mcell-updates/s" % mcell)
+    bw = copy_bandwidth()
+    out["hbm_copy_GBs"] = round(bw, 1)
+    out["roofline"]["frac_of_copy"] = round(out["roofline"]["achieved"] / bw, 4)
+    log("device copy: %.1f GB/s (the headline step reaches %.3f of it)" % (bw, out["roofline"]["frac_of_copy"]))
     if not args.no_cpu:
         v, rates, n_cpu, cpu_state = cpu_baseline(d0, Z0, h00, args.cpu_block, args.cpu_repeats)
         out["cpu_baseline"] = {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",

@@ -546,11 +546,12 @@ typedef struct mfv_euler_tet_tiles {
                                  * is the face's right side */
 } mfv_euler_tet_tiles;
 
-/* The packed Euler step on face tiles: the block stages its cells' and halo's records in LDS
- * with contiguous loads (halo gathered once per tile), evaluates every face of the tile once
- * into LDS, then each cell adds its faces' fluxes in the residual order, updates its record
- * and takes its next-dt candidate; same bits as mfv_euler3d_step_packed. LDS: 48 * max_items
- * + 40 * max_faces bytes (<= 64 KiB). Replaces explicitscheme_euler_3d_rusanov
+/* The packed Euler step on face tiles: the block stages its cells' records in LDS with
+ * contiguous loads, evaluates every face entry of the tile once into LDS (a halo cell's record
+ * is read by the entry that needs it; any entry order works, the host puts the entries with a
+ * halo cell last), then each cell adds its faces' fluxes in the residual order, updates its
+ * record and takes its next-dt candidate; same bits as mfv_euler3d_step_packed. LDS:
+ * 48 * tile_size + 40 * max_faces bytes (<= 64 KiB). Replaces explicitscheme_euler_3d_rusanov
  * (fvm_utils3d.py:1083-1170) + update_euler_3d_fvc + time_step_euler_3d in one launch. */
 int mfv_euler3d_step_packed_tiled(const mfv_mesh *m, const mfv_euler_tet *t, const mfv_euler_tet_tiles *tt,
                                   const double *rec_in, double *rec_out, double gamma, double cfl, int quirk,

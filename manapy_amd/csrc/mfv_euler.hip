@@ -479,11 +479,15 @@ __global__ void __launch_bounds__(kBlock) k_euler_step_packed(EulerTetK t, const
     wave_min_part(dtpart, (size_t)i, d);
 }
 
-// The packed step on face tiles: stage the tile's items' records (cells contiguous, halo
-// gathered) in LDS, evaluate each face entry once from its own contiguous record, then every
-// tile cell adds its faces' fluxes in the residual order, updates its record and takes its
-// next-dt candidate (face geometry from the tile's entries, still in cache; terms summed in
-// slot order). Same expressions and orders as k_euler_step_packed, so the same bits.
+// The packed step on face tiles: stage the tile cells' records (one contiguous read per
+// thread) in LDS, evaluate each face entry once from its own contiguous entry, with the
+// records of tile cells from LDS and those of halo cells read where the entry needs them (the
+// host puts a tile's entries with a halo cell last, so few waves mix both), then every tile
+// cell adds its faces' fluxes in the residual order, updates its record and takes its next-dt
+// candidate (face geometry from the tile's entries, still in cache; terms summed in slot
+// order). LDS holds 48 B per tile cell and 40 B per face entry: 39 KiB for the 256-cell tiles of
+// a Hilbert-ordered tetrahedral mesh, four blocks per CU. Same expressions and orders as
+// k_euler_step_packed, so the same bits.
 struct EulerTilesK {
     const int32_t *__restrict__ cell0;
     const int32_t *__restrict__ ncells;
@@ -503,32 +507,42 @@ __global__ void __launch_bounds__(T) k_euler_step_packed_tiled(EulerTetK t, Eule
                                                                 int quirk, DtRing ring, double *__restrict__ dtpart)
 {
     extern __shared__ double lds[];
-    double2 *sR = reinterpret_cast<double2 *>(lds);  // [MI][3]: six values per item
-    double *sF = lds + 6 * tt.MI;                    // [5][MFc]
+    double2 *sR = reinterpret_cast<double2 *>(lds);  // [T][3]: the tile cells' six values
+    double *sF = lds + 6 * T;                        // [5][MFc]
     const int tile = xcd_block();
     const int c0 = tt.cell0[tile], nc = tt.ncells[tile];
-    const int h0 = tt.hptr[tile], ni = nc + (tt.hptr[tile + 1] - h0);
+    const int h0 = tt.hptr[tile];
     const int f0 = tt.fptr[tile], nfc = tt.fptr[tile + 1] - f0;
     ring_advance(ring);
-    for (int e = threadIdx.x; e < ni; e += T) {
-        const int c = (e < nc) ? c0 + e : tt.halo[h0 + (e - nc)];
-        const double2 *rp = reinterpret_cast<const double2 *>(Rin + 8 * (size_t)c);
-        const double2 a = rp[0], b = rp[1], e2 = rp[2];
-        sR[3 * e] = a;
-        sR[3 * e + 1] = b;
-        sR[3 * e + 2] = e2;
+    double vol = 0.;  // the thread's cell's volume, kept from its record for the update
+    if ((int)threadIdx.x < nc) {
+        const double2 *rp = reinterpret_cast<const double2 *>(Rin + 8 * (size_t)(c0 + threadIdx.x));
+        const double2 a = rp[0], b = rp[1], e2 = rp[2], vv = rp[3];
+        sR[3 * threadIdx.x] = a;
+        sR[3 * threadIdx.x + 1] = b;
+        sR[3 * threadIdx.x + 2] = e2;
+        vol = vv.x;
     }
     __syncthreads();
     for (int k = threadIdx.x; k < nfc; k += T) {
         const double2 g01 = tt.fgeo[2 * (size_t)(f0 + k)], g23 = tt.fgeo[2 * (size_t)(f0 + k) + 1];
         const int2 lr = tt.flr[f0 + k];
+        const bool hl = lr.x >= nc, hr = lr.y >= nc;
+        const int cl = hl ? tt.halo[h0 + (lr.x - nc)] : 0;
+        const int cr = hr ? tt.halo[h0 + (lr.y - nc)] : 0;
+        Prim6 L, R;
+        if (hl) {
+            L = load_rec(Rin, cl);
+        } else {
+            const double2 la = sR[3 * lr.x], lb = sR[3 * lr.x + 1], lc = sR[3 * lr.x + 2];
+            L = Prim6{la.x, la.y, lb.x, lb.y, lc.x, lc.y};
+        }
         const double nrm[3] = {g01.x, g01.y, g23.x};
         const double mesu = g23.y;
         const double n[3] = {nrm[0] / mesu, nrm[1] / mesu, nrm[2] / mesu};
-        const double2 la = sR[3 * lr.x], lb = sR[3 * lr.x + 1], lc = sR[3 * lr.x + 2];
-        const Prim6 L{la.x, la.y, lb.x, lb.y, lc.x, lc.y};
-        Prim6 R;
-        if (lr.y >= 0) {
+        if (hr) {
+            R = load_rec(Rin, cr);
+        } else if (lr.y >= 0) {
             const double2 ra = sR[3 * lr.y], rb = sR[3 * lr.y + 1], rc = sR[3 * lr.y + 2];
             R = Prim6{ra.x, ra.y, rb.x, rb.y, rc.x, rc.y};
         } else {
@@ -547,9 +561,8 @@ __global__ void __launch_bounds__(T) k_euler_step_packed_tiled(EulerTetK t, Eule
         const int i = c0 + q;
         const int4 c4 = tt.code[i];
         const unsigned ord = t.cell_rank[i];
-        const double vol = Rin[8 * (size_t)i + 6];
         const int cd[4] = {c4.x, c4.y, c4.z, c4.w};
-        double2 g01[4], g23[4];  // the faces' geometry from the tile's entries (in cache since phase F)
+        double2 g01[4], g23[4];  // the faces' geometry from the tile's entries (in cache since the face phase)
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             g01[k] = tt.fgeo[2 * (size_t)(f0 + (cd[k] >> 1))];
@@ -1452,8 +1465,8 @@ extern "C" int mfv_euler3d_step_packed(const mfv_mesh *m, const mfv_euler_tet *t
     return dt_finish(part, nw, r.next, st);
 }
 
-// item_bytes / face_bytes: the LDS a kernel takes per staged item and per face entry
-static int check_euler_tiles(const mfv_mesh *m, const mfv_euler_tet_tiles *tt, size_t item_bytes, size_t face_bytes)
+// lds: the bytes of LDS a block of the kernel takes on these tiles
+static int check_euler_tiles(const mfv_mesh *m, const mfv_euler_tet_tiles *tt, size_t lds)
 {
     MFV_REQUIRE(tt != nullptr, "packed Euler tiles are NULL");
     MFV_REQUIRE(tt->tile_size == 128 || tt->tile_size == 256 || tt->tile_size == 512,
@@ -1468,7 +1481,6 @@ static int check_euler_tiles(const mfv_mesh *m, const mfv_euler_tet_tiles *tt, s
     MFV_REQUIRE(((uintptr_t)tt->face_geo & 15) == 0 && ((uintptr_t)tt->face_lr & 7) == 0 &&
                     ((uintptr_t)tt->cell_code & 15) == 0,
                 "packed Euler tiles are misaligned");
-    const size_t lds = item_bytes * (size_t)tt->max_items + face_bytes * (size_t)tt->max_faces;
     MFV_REQUIRE(lds <= 64 * 1024, "packed Euler tiles of %d cells need %zu bytes of LDS (> 64 KiB): use smaller "
                 "tiles or renumber the cells", tt->tile_size, lds);
     return MFV_OK;
@@ -1481,7 +1493,11 @@ static int euler_step_packed_tiled(bool cells, const mfv_mesh *m, const mfv_eule
     if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
     MFV_REQUIRE(m->dim == 3, "the packed Euler step needs a 3D mesh");
     if (int rc = check_euler_tet(m, t)) return rc;
-    if (int rc = check_euler_tiles(m, tt, 48, cells ? 0 : 40)) return rc;
+    if (!tt) return check_euler_tiles(m, tt, 0);
+    // the single-phase step stages the halo's records too; the face-phase step holds the tile
+    // cells' records and the face fluxes, and reads the halo's records in the face phase
+    const size_t lds = cells ? 48 * (size_t)tt->max_items : 48 * (size_t)tt->tile_size + 40 * (size_t)tt->max_faces;
+    if (int rc = check_euler_tiles(m, tt, lds)) return rc;
     if (m->nb_cells == 0) return MFV_OK;
     MFV_REQUIRE(rec_in && rec_out && rec_in != rec_out && dt_ring_p, "NULL or aliased array");
     MFV_REQUIRE(((uintptr_t)rec_in & 15) == 0 && ((uintptr_t)rec_out & 15) == 0, "misaligned records");
@@ -1503,7 +1519,6 @@ static int euler_step_packed_tiled(bool cells, const mfv_mesh *m, const mfv_eule
                          reinterpret_cast<const int4 *>(tt->cell_code),
                          tt->max_items,
                          tt->max_faces};
-    const size_t lds = 48 * (size_t)tt->max_items + (cells ? 0 : 40 * (size_t)tt->max_faces);
 #define MFV_EULER_TILED(K, TS)                                                                                        \
     hipLaunchKernelGGL(K<TS>, dim3(tt->nb_tiles), dim3(TS), lds, st, tk, ek, rec_in, rec_out, gamma, cfl, quirk, r, \
                        part)
@@ -1580,7 +1595,7 @@ extern "C" int mfv_euler3d_step32_tiled(const mfv_mesh *m, const mfv_euler_tet *
     if (int rc = validate_mesh(m, 0, 0, 0)) return rc;
     MFV_REQUIRE(m->dim == 3, "the Euler step needs a 3D mesh");
     if (int rc = check_euler_tet(m, t)) return rc;
-    if (int rc = check_euler_tiles(m, tt, 24, 24)) return rc;
+    if (int rc = check_euler_tiles(m, tt, tt ? 24 * (size_t)tt->max_items + 24 * (size_t)tt->max_faces : 0)) return rc;
     if (m->nb_cells == 0) return MFV_OK;
     MFV_REQUIRE(geo32 && rec_in && rec_out && rec_in != rec_out && dt_ring_p, "NULL or aliased array");
     MFV_REQUIRE(((uintptr_t)rec_in & 15) == 0 && ((uintptr_t)rec_out & 15) == 0 && ((uintptr_t)geo32 & 15) == 0,

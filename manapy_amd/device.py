@@ -592,7 +592,9 @@ def euler_tet_tiles_host(H, T):
     """Face tiles of the packed Euler step from a host mesh layout ``H`` (gpu_layout): the
     tiles, halos and face entries of face_tiles_native, each entry with its own geometry
     (normal, mesure) and item pair (left item; right item, or -name on a boundary face), and
-    per cell the entry codes of its faces in the residual order."""
+    per cell the entry codes of its faces in the residual order. Within a tile, the entries
+    whose cells are both tile cells come first (the face-phase step reads the records of halo
+    cells from HBM, so the waves of the first entries take them all from LDS)."""
     ft = face_tiles_native(H['cell_faces'], H['cell_nfaces'], H['face_cells'], T)
     fid = ft['face_id'].astype(np.int64)
     name = H['face_name'][fid].astype(np.int64)
@@ -602,10 +604,27 @@ def euler_tet_tiles_host(H, T):
     lr = np.ascontiguousarray(np.column_stack([ft['face_l'], np.where(name == 0, ft['face_r'], -name)]),
                               dtype=np.int32)
     order = H['order_sorted'].astype(np.int64)              # [4][C]: slot of the k-th face
-    code = np.ascontiguousarray(np.take_along_axis(ft['lface'], order, axis=0).T, dtype=np.int32)  # [C][4]
+    code = np.take_along_axis(ft['lface'], order, axis=0).T.astype(np.int64)  # [C][4]
+    fp = ft['face_ptr'].astype(np.int64)
+    nt = len(fp) - 1
+    et = np.repeat(np.arange(nt), np.diff(fp))
+    nct = ft['tile_ncells'].astype(np.int64)[et]
+    halo = (lr[:, 0] >= nct) | (lr[:, 1] >= nct)
+    perm = np.argsort(2 * et + halo, kind='stable')
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    geo, lr = np.ascontiguousarray(geo[perm]), np.ascontiguousarray(lr[perm])
+    C = code.shape[0]
+    ncs = ft['tile_ncells'].astype(np.int64)
+    ct = np.repeat(np.arange(nt), ncs)
+    if len(ct) != C or np.any(ft['tile_cell0'] != np.cumsum(ncs) - ncs):
+        raise ValueError("face tiles do not cover the cells in order")
+    base = fp[ct][:, None]
+    code = ((inv[base + (code >> 1)] - base) << 1) | (code & 1)
+    code = np.ascontiguousarray(code, dtype=np.int32)
     out = {k: ft[k] for k in ('T', 'nb_tiles', 'max_items', 'max_faces', 'tile_cell0', 'tile_ncells', 'halo_ptr',
                               'halo_cells', 'face_ptr')}
-    out.update({'face_geo': geo, 'face_lr': lr, 'cell_code': code})
+    out.update({'face_geo': geo, 'face_lr': lr, 'cell_code': code, 'face_id': ft['face_id'][perm]})
     return out
 
 
@@ -634,9 +653,12 @@ class EulerTetTiles(object):
         return ctypes.byref(self.desc)
 
     def lds_bytes(self, cells=False):
-        """LDS of a block: the staged records, plus the face fluxes unless cells (the
-        single-phase step, mfv_euler3d_step_cells_tiled)."""
-        return 48 * self.host['max_items'] + (0 if cells else 40 * self.host['max_faces'])
+        """LDS of a block: the face-phase step holds the tile cells' records and the face
+        fluxes; the single-phase step (cells, mfv_euler3d_step_cells_tiled) the records of the
+        tile cells and of their halo."""
+        if cells:
+            return 48 * self.host['max_items']
+        return 48 * self.host['T'] + 40 * self.host['max_faces']
 
 
 class Euler32(object):

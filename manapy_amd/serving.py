@@ -102,7 +102,7 @@ class AdvDiffRequest(object):
 
 class ServingEngine(object):
     def __init__(self, max_cells=4 << 20, max_inst=128, width=16, hbm_budget=None, n_iter=6, tol=1e-10,
-                 tol_inner=1e-4, device=None, graph=True, prep_threads=8, time_ticks=False):
+                 tol_inner=1e-3, device=None, graph=True, prep_threads=8, time_ticks=False):
         torch = require_gpu()
         self.arena = EllBatch((int(max_cells) + 63) // 64, max_inst=max_inst, width=width, device=device)
         self.device = self.arena.device

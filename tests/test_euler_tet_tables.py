@@ -79,8 +79,9 @@ def test_builder_rejects_bad_meshes():
 @pytest.mark.parametrize("T", [128, 256])
 def test_tiles_entries_and_cell_codes(T):
     """Each face entry of a tile carries its face's geometry and the items of its two cells
-    (or the boundary name); each cell's k-th code names the entry of its k-th face (increasing
-    original id) in its own tile and the cell's side."""
+    (or the boundary name); a tile's entries are its faces, those with a halo cell last; each
+    cell's k-th code names the entry of its k-th face (increasing original id) in its own tile
+    and the cell's side."""
     from manapy_amd.device import euler_tet_tiles_host, face_tiles_native, gpu_layout
     from manapy_amd.renumber import renumbered
     d = renumbered(make_tet_domain(7, 0.08, 2))
@@ -94,7 +95,8 @@ def test_tiles_entries_and_cell_codes(T):
         halo = tt['halo_cells'][tt['halo_ptr'][t]:tt['halo_ptr'][t + 1]]
         items = np.concatenate([np.arange(c0, c0 + nc), halo])
         e0, e1 = int(tt['face_ptr'][t]), int(tt['face_ptr'][t + 1])
-        fid = ft['face_id'][e0:e1]
+        fid = tt['face_id'][e0:e1]
+        assert np.array_equal(np.sort(fid), np.sort(ft['face_id'][e0:e1]))
         assert np.array_equal(tt['face_geo'][e0:e1, :3], L['face_normal'][fid])
         assert np.array_equal(tt['face_geo'][e0:e1, 3], L['face_mesure'][fid])
         lr = tt['face_lr'][e0:e1]
@@ -102,6 +104,8 @@ def test_tiles_entries_and_cell_codes(T):
         inner = L['face_name'][fid] == 0
         assert np.array_equal(items[lr[inner, 1]], L['face_cells'][fid[inner], 1])
         assert np.array_equal(lr[~inner, 1], -L['face_name'][fid[~inner]])
+        has_halo = (lr[:, 0] >= nc) | (lr[:, 1] >= nc)
+        assert np.all(np.diff(has_halo.astype(int)) >= 0)
         for i in range(c0, c0 + nc):
             for k in range(4):
                 code = int(tt['cell_code'][i, k])
