@@ -520,7 +520,10 @@ extern "C" int mfv_domain_geometry_2d_build(int C, int ew, const int32_t *nodeid
         for (int f = a; f < b; f++) {
             const int n0 = faces_nodeid[(size_t)f * 3], n1 = faces_nodeid[(size_t)f * 3 + 1];
             const double xy1[2] = {X(n0), Y(n0)}, xy2[2] = {X(n1), Y(n1)};
-            const int c0 = faces_cellid[2 * (size_t)f], c1 = faces_cellid[2 * (size_t)f + 1];
+            // a boundary face left unnamed (its nodes untagged) keeps name 0 with no second cell:
+            // the NumPy construction's index -1 then takes the last cell, and so does this one
+            const int c0 = faces_cellid[2 * (size_t)f], c1r = faces_cellid[2 * (size_t)f + 1];
+            const int c1 = c1r < 0 ? C + c1r : c1r;
             const double v1[2] = {cc[3 * (size_t)c0], cc[3 * (size_t)c0 + 1]};
             double v2[2] = {gc[3 * (size_t)f], gc[3 * (size_t)f + 1]};
             if (name[f] == 0) {
@@ -936,7 +939,9 @@ extern "C" int mfv_domain_geometry_3d_build(int C, int ew, const int32_t *nodeid
             const int32_t *r = &fn[(size_t)f * fw];
             const int i1 = r[0], i2 = r[1], i3 = r[2];
             const int i4 = (fw > 4 && r[fw - 1] == 4) ? r[3] : i3;
-            const int c0 = faces_cellid[2 * (size_t)f], c1 = faces_cellid[2 * (size_t)f + 1];
+            // an unnamed boundary face: the NumPy construction's index -1 takes the last cell
+            const int c0 = faces_cellid[2 * (size_t)f], c1r = faces_cellid[2 * (size_t)f + 1];
+            const int c1 = c1r < 0 ? C + c1r : c1r;
             const V3 v1 = cc[c0];
             const V3 v2 = name[f] == 0 ? cc[c1] : V3{gc[4 * (size_t)f], gc[4 * (size_t)f + 1], gc[4 * (size_t)f + 2]};
             const V3 s1 = v3sub(v2, P(i2)), s2 = v3sub(P(i4), P(i2)), s3 = v3sub(v1, P(i2));
