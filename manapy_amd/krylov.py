@@ -134,6 +134,7 @@ class EllBatch(object):
         self._inst[inst] = (s0, ns, int(nrows))
         self.slice_inst[s0:s0 + ns] = inst
         self.slice_w[s0:s0 + ns] = 0
+        self.slice_narrow[s0:s0 + ns] = 0  # int32 columns until narrow_columns has checked this matrix
         self.status[inst] = _lib.KS_IDLE
         return inst
 
