@@ -832,15 +832,22 @@ __device__ __forceinline__ double sw_flux_packed_core(const mfv_sw_packed &p, in
         zo[r] = z.x;
         zn[r] = z.y;
     }
-    // the dt's geometry in faceid order, loaded with the rest (after the record store a load
-    // could not be moved above it)
+    // the dt's geometry in faceid order: slot s's values are those loaded for any r with
+    // sl[r] == s (the first nf residual positions hold each of the cell's slots once), picked
+    // from registers instead of loaded again
     double dn0[MF], dn1[MF], dms[MF];
 #pragma unroll
     for (int s = 0; s < MF; s++) {
-        const double2 *gq = reinterpret_cast<const double2 *>(p.cfgeo + 4 * (MF * (size_t)i + s));
-        dn0[s] = gq[0].x;
-        dn1[s] = gq[0].y;
-        dms[s] = gq[1].x;
+        double a = gn0[0], b = gn1[0], c = gms[0];
+#pragma unroll
+        for (int r = 1; r < MF; r++) {
+            a = sl[r] == s ? gn0[r] : a;
+            b = sl[r] == s ? gn1[r] : b;
+            c = sl[r] == s ? gms[r] : c;
+        }
+        dn0[s] = a;
+        dn1[s] = b;
+        dms[s] = c;
     }
     const double hu_i = cv.hu, hv_i = cv.hv, h_i = cv.h, Z_i = cv.Z, vol = cv.vol;
     double r0 = 0., r1 = 0., rr2 = 0., s1 = 0., s2 = 0.;
