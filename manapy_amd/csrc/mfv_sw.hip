@@ -559,7 +559,7 @@ struct PackedStaged {
 // face slots; returns nf.
 template <int MF, int KMAX, class Acc>
 __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed &p, const Acc &A, int i,
-                                             double om, double2 *out, double2 *zpair = nullptr)
+                                             double om, double2 *out, double2 *zpair = nullptr, int *lcode = nullptr)
 {
     const size_t C = m.C;
     // Loads are issued in batches ahead of the arithmetic: the per-cell counts, the stencil
@@ -723,6 +723,10 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
     if (zpair) {  // z_f of both sides of every slot (cfrec[i][s][2..3]), for a caller that keeps them
 #pragma unroll
         for (int s = 0; s < MF; s++) zpair[s] = cfb[s];
+    }
+    if (lcode) {  // the neighbour codes in the accessor's numbering: (position << 3) | (k << 1) | side
+#pragma unroll
+        for (int s = 0; s < MF; s++) lcode[s] = codes[s] >= 0 ? (nbh[s] << 3) | (codes[s] & 7) : -1;
     }
     return nf;
 }
@@ -1000,11 +1004,15 @@ __global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packe
         hZv[1] = rc[6];
         hZv[2] = rc[7];
     }
-    auto recon_item = [&](int qq, double2 *keep) {
+    int lc[MF];  // the tile cell's block-local neighbour codes (t.lnbr's values), from phase A
+    int nf_q = 0;
+    auto recon_item = [&](int qq, double2 *keep, int *lkeep) {
         const int c = (qq < nc) ? c0 + qq : t.ext[e0 + (qq - nc)];
         double2 rec[2 * MF];
         const int nf = sw_recon_core<MF, KMAX>(
-            m, p, PackedStaged{s_xy, s_eu, s_v, t.lst, t.lgh, t.lfn, r0 + qq, (size_t)t.I, qq}, c, om, rec, keep);
+            m, p, PackedStaged{s_xy, s_eu, s_v, t.lst, t.lgh, t.lfn, r0 + qq, (size_t)t.I, qq}, c, om, rec, keep,
+            lkeep);
+        if (keep) nf_q = nf;
 #pragma unroll
         for (int s = 0; s < MF; s++)
             if (s < nf) {
@@ -1012,16 +1020,17 @@ __global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packe
                 f_v[MF * qq + s] = rec[2 * s + 1].x;
             }
     };
-    if (q < nc + nh) recon_item(q, zp);
-    for (int qq = q + T; qq < nc + nh; qq += T) recon_item(qq, nullptr);  // items beyond one per thread
+    if (q < nc + nh) recon_item(q, zp, lc);
+    for (int qq = q + T; qq < nc + nh; qq += T) recon_item(qq, nullptr, nullptr);  // items beyond one per thread
     __syncthreads();
     if ((q & ~63) >= nc) return;  // the whole wave lies past the tile: it owns no partial
     double dloc = 1e300;
     if (q < nc) {
         const int i = c0 + q;
-        const int32_t *__restrict__ lnbr = t.lnbr;
-        const size_t C = m.C;
-        auto code_of = [lnbr, C, i](int s) { return lnbr[(size_t)s * C + i]; };
+        // the codes of t.lnbr, (lfn << 3) | (nbr & 7), kept from the reconstruction; a slot index
+        // picks one by selects (an array indexed by a lane-dependent slot would live in scratch)
+        const int l0 = lc[0], l1 = lc[1], l2 = lc[2], l3 = lc[MF - 1];
+        auto code_of = [l0, l1, l2, l3](int s) { return s == 0 ? l0 : s == 1 ? l1 : s == 2 ? l2 : l3; };
         StagedFaceVals fv{f_eu, f_v, 0., 0., q};
 #pragma unroll
         for (int s = 0; s < MF; s++) {
@@ -1029,8 +1038,8 @@ __global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packe
             fv.zn[s] = zp[s].y;
         }
         const CellVals cv{s_eu[q].y, s_v[q], hZv[0], hZv[1], hZv[2]};
-        dloc = sw_flux_packed_core<MF>(p, i, cv, rec_out + 8 * (size_t)i, fv, code_of, m.cell_nfaces[i], p.order[i],
-                                       g, cfl, ring.cur[0]);
+        dloc = sw_flux_packed_core<MF>(p, i, cv, rec_out + 8 * (size_t)i, fv, code_of, nf_q, p.order[i], g, cfl,
+                                       ring.cur[0]);
     }
     wave_min_part(dtpart, (size_t)c0 + q, dloc);
 }
