@@ -139,13 +139,14 @@ __device__ __forceinline__ void accum_grad(float jx, float jy, float de, float d
 // native vectors: element s with a lane-dependent s is a select chain.
 struct SlotGeo {
     slot_f nx, ny, dz, dzo, len, ilen;
+    int lc0, lc1, lc2, lc3, nf;  // neighbour codes in the accessor's numbering, face count
 };
 
 // LEN: |n| and 1/|n| from geoB, else computed from the normal (the same float32 expressions).
 template <int MF, bool LEN>
 __device__ __forceinline__ SlotGeo load_slot_geo(const mfv_sw_mixed &p, int i)
 {
-    SlotGeo G{0.f, 0.f, 0.f, 0.f, 1.f, 1.f};
+    SlotGeo G{0.f, 0.f, 0.f, 0.f, 1.f, 1.f, -1, -1, -1, -1, 0};
 #pragma unroll
     for (int s = 0; s < MF; s++) {
         const float4 q = reinterpret_cast<const float4 *>(p.geoA)[MF * (size_t)i + s];
@@ -266,6 +267,16 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
             keep->len[s] = q.z;
             keep->ilen[s] = q.w;
         }
+    }
+    if (keep) {  // the codes (position << 3) | (k << 1) | side in the accessor's numbering
+        int lc[MF];
+#pragma unroll
+        for (int s = 0; s < MF; s++) lc[s] = codes[s] >= 0 ? (nbh[s] << 3) | (codes[s] & 7) : -1;
+        keep->lc0 = lc[0];
+        keep->lc1 = lc[1];
+        keep->lc2 = lc[2];
+        keep->lc3 = lc[MF - 1];
+        keep->nf = nf;
     }
     for (int j = 0; j < ng; j++) {
         const int f = m.lsq_ghosts[(size_t)j * C + i];
@@ -736,8 +747,7 @@ struct TilesK {
 // Reconstruction inputs from a block's LDS copy: handles are positions in the tile's ext
 // list (its cells, its halo, then every other cell its reconstructions read).
 struct StagedCells {
-    // LDS latency is short: groups of two keep the kernel at 87 VGPRs, so five blocks of 256
-    // threads fit a CU where groups of four (109 VGPRs) allowed four
+    // LDS latency is short: stencil groups of two hold fewer registers than groups of four
     static constexpr int kChunk = 2;
     const double4 *st;
     const double2 *cc;
@@ -789,7 +799,7 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
     // A tile cell is reconstructed and updated by the same thread (q = threadIdx.x is in the
     // first pass), which keeps the cell's slot geometry in registers for the flux phase.
     const int q = threadIdx.x;
-    SlotGeo G{0.f, 0.f, 0.f, 0.f, 1.f, 1.f};
+    SlotGeo G{0.f, 0.f, 0.f, 0.f, 1.f, 1.f, -1, -1, -1, -1, 0};
     auto recon_item = [&](int qq, SlotGeo *keep) {
         const int c = (qq < nc) ? c0 + qq : t.ext[e0 + (qq - nc)];
         float4 rec[MF];
@@ -813,10 +823,18 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
         const double dt = BATCH ? b.rings[4 * (size_t)b.wave_inst[i >> 6] + kcur] : ring.cur[0];
         const int32_t *__restrict__ lnbr = t.lnbr;
         const size_t C = m.C;
-        auto code_of = [lnbr, C, i](int s) { return lnbr[(size_t)s * C + i]; };
+        // staged: the codes of t.lnbr ((lfn << 3) | (nbr & 7)) and the face count kept from the
+        // reconstruction of the same cell; unstaged: the reconstruction numbered the neighbours
+        // globally, so the block-local codes are loaded
+        const int l0 = G.lc0, l1 = G.lc1, l2 = G.lc2, l3 = G.lc3;
+        auto code_of = [lnbr, C, i, l0, l1, l2, l3](int s) {
+            if (STAGED) return s == 0 ? l0 : s == 1 ? l1 : s == 2 ? l2 : l3;
+            return lnbr[(size_t)s * C + i];
+        };
         auto recs = [lfr](size_t k) { return lfr[k]; };
         const double4 si = STAGED ? s_st[q] : st[i];
-        dloc = flux32_core<MF>(p, i, q, si, m.cell_nfaces[i], p.order[i], recs, code_of, G, g, cfl, dt, out + i);
+        const int nf = STAGED ? G.nf : m.cell_nfaces[i];
+        dloc = flux32_core<MF>(p, i, q, si, nf, p.order[i], recs, code_of, G, g, cfl, dt, out + i);
     }
     wave_min_part(part, (size_t)c0 + q, dloc);
 }
