@@ -527,6 +527,12 @@ struct PackedCells {
         load_ehv(rec, h, cx, cy, e, u, v);
     }
     __device__ __forceinline__ int stencil(int j) const { return m.lsq_cells[(size_t)j * m.C + i]; }
+    template <int KM>
+    __device__ __forceinline__ void stencils(int K, int (&h)[KM]) const
+    {
+#pragma unroll
+        for (int j = 0; j < KM; j++) h[j] = (j < K) ? stencil(j) : 0;
+    }
     __device__ __forceinline__ int ghost_owner(int, int f) const { return m.face_cells[2 * f]; }
     __device__ __forceinline__ int face_nb(int, int code) const { return code >> 3; }
 };
@@ -539,7 +545,7 @@ struct PackedStaged {
     const uint16_t *__restrict__ lgh;
     const int16_t *__restrict__ lfn;
     size_t row, I;
-    int q;
+    int q, kp;
     __device__ __forceinline__ int self() const { return q; }
     __device__ __forceinline__ void ehv(int h, double &cx, double &cy, double &e, double &u, double &v) const
     {
@@ -550,7 +556,12 @@ struct PackedStaged {
         u = b.y;
         v = hv[h];
     }
-    __device__ __forceinline__ int stencil(int j) const { return lst[(size_t)j * I + row]; }
+    __device__ __forceinline__ int stencil(int j) const { return lst[row * kp + j]; }
+    template <int KM>
+    __device__ __forceinline__ void stencils(int K, int (&h)[KM]) const
+    {
+        stencil_row<KM>(lst, row, kp, K, h);
+    }
     __device__ __forceinline__ int ghost_owner(int j, int) const { return lgh[(size_t)j * I + row]; }
     __device__ __forceinline__ int face_nb(int s, int) const { return lfn[(size_t)s * I + row]; }
 };
@@ -592,9 +603,9 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
         jv1 += (jy * dv);
     };
     if (KMAX > 0) {
-        int cidx[KMAX > 0 ? KMAX : 1];
-#pragma unroll
-        for (int j = 0; j < KMAX; j++) cidx[j] = (j < m.K) ? A.stencil(j) : 0;
+        constexpr int KA = KMAX > 0 ? KMAX : 8;
+        int cidx[KA];
+        A.template stencils<KA>(m.K, cidx);
 #pragma unroll
         for (int j = 0; j < KMAX; j++) cidx[j] = (j < nn) ? cidx[j] : A.self();
         // Every group's entries are loaded (entries past nn read the own cell, a valid handle)
@@ -1009,9 +1020,8 @@ __global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packe
     auto recon_item = [&](int qq, double2 *keep, int *lkeep) {
         const int c = (qq < nc) ? c0 + qq : t.ext[e0 + (qq - nc)];
         double2 rec[2 * MF];
-        const int nf = sw_recon_core<MF, KMAX>(
-            m, p, PackedStaged{s_xy, s_eu, s_v, t.lst, t.lgh, t.lfn, r0 + qq, (size_t)t.I, qq}, c, om, rec, keep,
-            lkeep);
+        const PackedStaged acc{s_xy, s_eu, s_v, t.lst, t.lgh, t.lfn, r0 + qq, (size_t)t.I, qq, MFV_SW_TILES_KP(m.K)};
+        const int nf = sw_recon_core<MF, KMAX>(m, p, acc, c, om, rec, keep, lkeep);
         if (keep) nf_q = nf;
 #pragma unroll
         for (int s = 0; s < MF; s++)

@@ -184,6 +184,12 @@ struct GlobalCells {
         return make_double2(m.cell_center[3 * (size_t)h], m.cell_center[3 * (size_t)h + 1]);
     }
     __device__ __forceinline__ int stencil(int j) const { return m.lsq_cells[(size_t)j * m.C + i]; }
+    template <int KM>
+    __device__ __forceinline__ void stencils(int K, int (&h)[KM]) const
+    {
+#pragma unroll
+        for (int j = 0; j < KM; j++) h[j] = (j < K) ? stencil(j) : 0;
+    }
     __device__ __forceinline__ int ghost_owner(int, int f) const { return m.face_cells[2 * f]; }
     __device__ __forceinline__ int face_nb(int, int code) const { return code >> 3; }
 };
@@ -219,9 +225,9 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
                    je1, ju0, ju1, jv0, jv1);
     };
     if (KMAX > 0) {
-        int h[KMAX > 0 ? KMAX : 1];
-#pragma unroll
-        for (int j = 0; j < KMAX; j++) h[j] = (j < m.K) ? a.stencil(j) : 0;
+        constexpr int KA = KMAX > 0 ? KMAX : 8;
+        int h[KA];
+        a.template stencils<KA>(m.K, h);
 #pragma unroll
         for (int j = 0; j < KMAX; j++) h[j] = (j < nn) ? h[j] : a.self();
         constexpr int CH = Cells::kChunk;
@@ -755,11 +761,16 @@ struct StagedCells {
     const uint16_t *__restrict__ lgh;
     const int16_t *__restrict__ lfn;
     size_t row, I;
-    int q;
+    int q, kp;
     __device__ __forceinline__ int self() const { return q; }
     __device__ __forceinline__ double4 state(int h) const { return st[h]; }
     __device__ __forceinline__ double2 center(int h) const { return cc[h]; }
-    __device__ __forceinline__ int stencil(int j) const { return lst[(size_t)j * I + row]; }
+    __device__ __forceinline__ int stencil(int j) const { return lst[row * kp + j]; }
+    template <int KM>
+    __device__ __forceinline__ void stencils(int K, int (&h)[KM]) const
+    {
+        stencil_row<KM>(lst, row, kp, K, h);
+    }
     __device__ __forceinline__ int ghost_owner(int j, int) const { return lgh[(size_t)j * I + row]; }
     __device__ __forceinline__ int face_nb(int s, int) const { return lfn[(size_t)s * I + row]; }
 };
@@ -804,11 +815,12 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
         const int c = (qq < nc) ? c0 + qq : t.ext[e0 + (qq - nc)];
         float4 rec[MF];
         int nf;
-        if (STAGED)
-            nf = recon32_core<MF, KMAX>(
-                m, p, StagedCells{s_st, s_cc, t.lst, t.lgh, t.lfn, r0 + qq, (size_t)t.I, qq}, c, om, rec, keep);
-        else
+        if (STAGED) {
+            const StagedCells acc{s_st, s_cc, t.lst, t.lgh, t.lfn, r0 + qq, (size_t)t.I, qq, MFV_SW_TILES_KP(m.K)};
+            nf = recon32_core<MF, KMAX>(m, p, acc, c, om, rec, keep);
+        } else {
             nf = recon32_core<MF, KMAX>(m, p, GlobalCells{m, st, c}, c, om, rec, keep);
+        }
 #pragma unroll
         for (int s = 0; s < MF; s++)
             if (s < nf) lfr[MF * qq + s] = rec[s];

@@ -170,7 +170,8 @@ extern "C" int mfv_sw_tiles_build(int C, int MF, const int32_t *nbr, int K, cons
     // 3. tables
     std::vector<int32_t> cell0(nt), ncells(nt), nhalo(nt), eptr(nt + 1, 0), iptr(nt + 1, 0), ext;
     std::vector<int32_t> lnbr((size_t)MF * C, -1);
-    std::vector<uint16_t> lst((size_t)std::max(K, 1) * std::max(I, 1), 0);
+    const int KP = MFV_SW_TILES_KP(K);  // stencil row stride: whole 16-byte vectors
+    std::vector<uint16_t> lst((size_t)KP * std::max(I, 1), 0);
     std::vector<uint16_t> lgh((size_t)std::max(G, 1) * std::max(I, 1), 0);
     std::vector<int16_t> lfn((size_t)MF * std::max(I, 1), -1);
     int max_halo = 0, max_ext = 0, max_items = 0;
@@ -205,7 +206,7 @@ extern "C" int mfv_sw_tiles_build(int C, int MF, const int32_t *nbr, int K, cons
             const int c = (q < tl.nc) ? tl.c0 + q : tl.halo[q - tl.nc];
             const size_t row = (size_t)iptr[t] + q;
             for (int j = 0; j < lsq_ncells[c]; j++)
-                lst[(size_t)j * I + row] = (uint16_t)pos[lsq_cells[(size_t)j * C + c]];
+                lst[row * KP + j] = (uint16_t)pos[lsq_cells[(size_t)j * C + c]];
             for (int j = 0; j < lsq_nghosts[c]; j++)
                 lgh[(size_t)j * I + row] = (uint16_t)pos[face_cells[2 * (size_t)lsq_ghosts[(size_t)j * C + c]]];
             for (int s = 0; s < MF; s++) {

@@ -313,13 +313,19 @@ def sw_tiles_native(nbr, C, T, N, lsq, ranges=None):
                'ext_cells': take(h.ext_cells, max(h.ext_len, 1), np.int32)[:h.ext_len],
                'item_ptr': take(h.item_ptr, nt + 1, np.int32),
                'lnbr': take(h.lnbr, MF * C, np.int32, (MF, C)),
-               'lst': take(h.lst, max(K, 1) * I, np.uint16, (max(K, 1), I)),
+               'lst': take(h.lst, sw_tiles_kp(K) * I, np.uint16, (I, sw_tiles_kp(K))),
                'lgh': take(h.lgh, max(G, 1) * I, np.uint16, (max(G, 1), I)),
                'lfn': take(h.lfn, MF * I, np.int16, (MF, I))}
         out['tile_next'] = (np.diff(out['ext_ptr']) + out['tile_ncells']).astype(np.int32)
     finally:
         _lib.call('mfv_sw_tiles_host_free', ctypes.byref(h))
     return out
+
+
+def sw_tiles_kp(K):
+    """Row stride of the tiles' stencil table (MFV_SW_TILES_KP): K rounded up to a multiple
+    of 8, at least 8, so that a row is whole 16-byte vectors."""
+    return 8 if K <= 8 else (int(K) + 7) & ~7
 
 
 def sw_tiles_fitted(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None, N=None):
@@ -460,6 +466,8 @@ def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None, 
     it = np.broadcast_to(item_tile, sten.shape)
     lst = np.zeros(sten.shape, dtype=np.int64)
     lst[sten_ok] = ext_pos(it[sten_ok], sten[sten_ok])
+    lst_rows = np.zeros((I, sw_tiles_kp(K)), dtype=np.uint16)  # one row per item, padded
+    lst_rows[:, :K] = lst.T
     lgh = np.zeros(gown.shape, dtype=np.int64)
     itg = np.broadcast_to(item_tile, gown.shape)
     lgh[g_ok] = ext_pos(itg[g_ok], gown[g_ok])
@@ -471,7 +479,7 @@ def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None, 
     out.update({'max_ext': int(n_ext.max()) if nt else 0, 'items': I, 'max_items': int(n_items.max()) if nt else 0,
                 'ext_ptr': ext_ptr.astype(np.int32), 'ext_cells': ext_cells.astype(np.int32),
                 'item_ptr': item_ptr.astype(np.int32), 'tile_next': n_ext.astype(np.int32),
-                'lst': np.ascontiguousarray(lst, dtype=np.uint16) if K else np.zeros((1, max(I, 1)), np.uint16),
+                'lst': lst_rows,
                 'lgh': np.ascontiguousarray(lgh, dtype=np.uint16) if G else np.zeros((1, max(I, 1)), np.uint16),
                 'lfn': np.ascontiguousarray(lfn, dtype=np.int16)})
     return out

@@ -42,7 +42,7 @@ def _check(L, nbr, lsq, C):
             r = int(L['item_ptr'][t]) + q
             c = int(ext[q])
             for jj in range(int(lsq['lsq_ncells'][c])):
-                assert ext[L['lst'][jj, r]] == lsq['lsq_cells'][jj, c]
+                assert ext[L['lst'][r, jj]] == lsq['lsq_cells'][jj, c]
             for jj in range(int(lsq['lsq_nghosts'][c])):
                 f = lsq['lsq_ghosts'][jj, c]
                 assert ext[L['lgh'][jj, r]] == lsq['face_cells'][f, 0]
