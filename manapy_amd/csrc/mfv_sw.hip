@@ -1009,11 +1009,13 @@ __global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packe
     const int q = threadIdx.x;
     double2 zp[MF];
     double hZv[3] = {0., 0., 0.};
+    unsigned ord_q = 0;  // the residual order, loaded here so that phase B starts with the slot loads
     if (q < nc) {
         const double *rc = p.cellrec + 8 * (size_t)(c0 + q);
         hZv[0] = rc[5];
         hZv[1] = rc[6];
         hZv[2] = rc[7];
+        ord_q = p.order[c0 + q];
     }
     int lc[MF];  // the tile cell's block-local neighbour codes (t.lnbr's values), from phase A
     int nf_q = 0;
@@ -1048,7 +1050,7 @@ __global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packe
             fv.zn[s] = zp[s].y;
         }
         const CellVals cv{s_eu[q].y, s_v[q], hZv[0], hZv[1], hZv[2]};
-        dloc = sw_flux_packed_core<MF>(p, i, cv, rec_out + 8 * (size_t)i, fv, code_of, nf_q, p.order[i], g, cfl,
+        dloc = sw_flux_packed_core<MF>(p, i, cv, rec_out + 8 * (size_t)i, fv, code_of, nf_q, ord_q, g, cfl,
                                        ring.cur[0]);
     }
     wave_min_part(dtpart, (size_t)c0 + q, dloc);

@@ -810,6 +810,8 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
     // A tile cell is reconstructed and updated by the same thread (q = threadIdx.x is in the
     // first pass), which keeps the cell's slot geometry in registers for the flux phase.
     const int q = threadIdx.x;
+    // the residual order, loaded here so that the flux phase starts with the slot loads
+    const unsigned ord_q = (q < nc) ? p.order[c0 + q] : 0u;
     SlotGeo G{0.f, 0.f, 0.f, 0.f, 1.f, 1.f, -1, -1, -1, -1, 0};
     auto recon_item = [&](int qq, SlotGeo *keep) {
         const int c = (qq < nc) ? c0 + qq : t.ext[e0 + (qq - nc)];
@@ -846,7 +848,7 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
         auto recs = [lfr](size_t k) { return lfr[k]; };
         const double4 si = STAGED ? s_st[q] : st[i];
         const int nf = STAGED ? G.nf : m.cell_nfaces[i];
-        dloc = flux32_core<MF>(p, i, q, si, nf, p.order[i], recs, code_of, G, g, cfl, dt, out + i);
+        dloc = flux32_core<MF>(p, i, q, si, nf, ord_q, recs, code_of, G, g, cfl, dt, out + i);
     }
     wave_min_part(part, (size_t)c0 + q, dloc);
 }
