@@ -386,11 +386,23 @@ double *dt_scratch(size_t nwaves, hipStream_t st)
 __global__ void __launch_bounds__(1024) k_dt_finish(const double *__restrict__ part, size_t n, double *__restrict__ out)
 {
     __shared__ double sh[1024];
-    double m = __longlong_as_double(0x7ff0000000000000LL);
-    for (size_t j = threadIdx.x; j < n; j += 1024) {
-        const double v = part[j];
-        m = (v < m) ? v : m;
+    // four partials in flight per thread (the minimum of a set does not depend on the order)
+    double m0 = __longlong_as_double(0x7ff0000000000000LL), m1 = m0, m2 = m0, m3 = m0;
+    size_t j = threadIdx.x;
+    for (; j + 3 * 1024 < n; j += 4 * 1024) {
+        const double a = part[j], b = part[j + 1024], c = part[j + 2048], e = part[j + 3072];
+        m0 = (a < m0) ? a : m0;
+        m1 = (b < m1) ? b : m1;
+        m2 = (c < m2) ? c : m2;
+        m3 = (e < m3) ? e : m3;
     }
+    for (; j < n; j += 1024) {
+        const double v = part[j];
+        m0 = (v < m0) ? v : m0;
+    }
+    m0 = (m1 < m0) ? m1 : m0;
+    m2 = (m3 < m2) ? m3 : m2;
+    const double m = (m2 < m0) ? m2 : m0;
     sh[threadIdx.x] = m;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {

@@ -591,11 +591,23 @@ __global__ void __launch_bounds__(256) k_batch_dt_fold(const double *__restrict_
     if (inst >= b.I) return;
     const int w0 = b.inst_cell0[inst] >> 6;
     const int w1 = (b.inst_cell0[inst] + b.inst_ncells[inst] + 63) >> 6;
-    double m = __longlong_as_double(0x7ff0000000000000LL);
-    for (int w = w0 + lane; w < w1; w += 64) {
-        const double v = part[w];
-        m = (v < m) ? v : m;
+    // four partials in flight per lane (the minimum of a set does not depend on the order)
+    double m0 = __longlong_as_double(0x7ff0000000000000LL), m1 = m0, m2 = m0, m3 = m0;
+    int w = w0 + lane;
+    for (; w + 192 < w1; w += 256) {
+        const double a = part[w], c = part[w + 64], e = part[w + 128], f = part[w + 192];
+        m0 = (a < m0) ? a : m0;
+        m1 = (c < m1) ? c : m1;
+        m2 = (e < m2) ? e : m2;
+        m3 = (f < m3) ? f : m3;
     }
+    for (; w < w1; w += 64) {
+        const double v = part[w];
+        m0 = (v < m0) ? v : m0;
+    }
+    m0 = (m1 < m0) ? m1 : m0;
+    m2 = (m3 < m2) ? m3 : m2;
+    double m = (m2 < m0) ? m2 : m0;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_xor(m, off, 64);
