@@ -76,6 +76,24 @@ def test_every_mirrored_struct_matches_the_header(tmp_path):
             assert int(got['%s.%s' % (cname, f[0])]) == getattr(cls, f[0]).offset, (cname, f[0])
 
 
+def test_tile_stencil_row_stride_matches_the_header(tmp_path):
+    """The Python builders pad the tiles' stencil rows to the stride MFV_SW_TILES_KP gives."""
+    import shutil
+    import subprocess
+    from manapy_amd.device import sw_tiles_kp
+    if shutil.which('gcc') is None:
+        pytest.skip("no C compiler")
+    ks = list(range(0, 40))
+    src = tmp_path / 'kp.c'
+    src.write_text('#include <stdio.h>\n#include "mfv.h"\nint main(void) {\n' +
+                   ''.join('printf("%%d\\n", MFV_SW_TILES_KP(%d));\n' % k for k in ks) + 'return 0;\n}\n')
+    exe = tmp_path / 'kp'
+    subprocess.run(['gcc', '-std=c11', '-I', os.path.join(ROOT, 'include'), str(src), '-o', str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == [sw_tiles_kp(k) for k in ks]
+    assert all(kp % 8 == 0 and kp >= max(k, 8) for k, kp in zip(ks, got))
+
+
 def test_noalloc_step_rejects_meshes_beyond_its_static_scratch():
     """The allocation-free step checks the mesh size against its static partials array before
     touching the device."""
