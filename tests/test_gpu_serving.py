@@ -150,3 +150,23 @@ def test_raw_mesh_requests_are_prepared_on_arrival():
         assert r.prepared and r.prepare_s > 0
         assert r.t_done - r.arrival >= r.prepare_s
         assert np.array_equal(r.result, ref[r.rid]), r.rid
+
+
+def test_request_whose_preparation_fails_is_rejected():
+    """A raw-mesh request whose Domain cannot be built (an element names a node that does not
+    exist) is rejected with the builder's error while the others are served."""
+    from manapy_amd.meshgen import jittered_triangles
+    from manapy_amd.serving import AdvDiffRequest, ServingEngine, PHASE_DONE, PHASE_REJECTED
+    blob = lambda c: np.exp(-((c[:, 0] - .3) ** 2 + (c[:, 1] - .4) ** 2) / .01)
+    el, nd = jittered_triangles(12, 12, 1.0, 1.0, 0.1, 961)
+    bad_el = el.copy()
+    bad_el[0, 0] = len(nd) + 5
+    reqs = [AdvDiffRequest(0, (bad_el, nd), blob, velocity=(1.0, 0.0), D=1e-2, nsteps=2, arrival=0.),
+            AdvDiffRequest(1, (el, nd), blob, velocity=(1.0, 0.0), D=1e-2, nsteps=2, arrival=0.)]
+    eng = ServingEngine(max_cells=1 << 14, max_inst=4)
+    done = eng.run(reqs)
+    by = {r.rid: r for r in done}
+    assert len(done) == 2
+    assert by[0].phase == PHASE_REJECTED and "preparation failed" in by[0].error and "out of" in by[0].error
+    assert by[1].phase == PHASE_DONE and by[1].result is not None
+    assert not eng.active and eng.in_use == 0
