@@ -263,10 +263,11 @@ int mfv_sw_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, double *partia
  * (the tile_nhalo[t] face neighbours outside it); its ext list is its items followed by every
  * other cell their reconstructions read (stencil cells, ghost owners, face neighbours):
  * the halo and the extra cells are ext_cells[ext_ptr[t] .. ext_ptr[t+1]). Item q of tile t
- * is row item_ptr[t] + q of the per-item tables lst [nb_items][KP] (stencil, one row per item,
- * KP = K rounded up to a multiple of 8 so that a row is whole 16-byte vectors, unused entries
- * 0), lgh [G][nb_items] (ghost owners) and lfn [MF][nb_items] (face neighbours, -1 on walls),
- * which hold positions in the ext list. lnbr [MF][C] is nbr with the cell part replaced by the
+ * is row item_ptr[t] + q of the per-item tables lst [nb_items][KP] (stencil, one row per item;
+ * KP = K + 1 rounded up to a multiple of 8 so that a row is whole 16-byte vectors; entry KP - 1
+ * holds the item's stencil and ghost counts, lsq_ncells | lsq_nghosts << 8; other unused
+ * entries 0), lgh [G][nb_items] (ghost owners) and lfn [MF][nb_items] (face neighbours, -1 on
+ * walls), which hold positions in the ext list. lnbr [MF][C] is nbr with the cell part replaced by the
  * neighbour's position in the item list. LDS per block: max_items * MF * 16 bytes, plus
  * max_ext * 48 bytes with stage_states; at most 64 KiB. */
 typedef struct mfv_sw_tiles {
@@ -290,9 +291,9 @@ typedef struct mfv_sw_tiles {
     const int16_t *lfn;         /* [MF][nb_items] */
 } mfv_sw_tiles;
 
-/* Row stride of the tiles' stencil table for a stencil width K: K rounded up to a multiple of
- * 8 (at least 8). */
-#define MFV_SW_TILES_KP(K) ((K) <= 8 ? 8 : (((K) + 7) & ~7))
+/* Row stride of the tiles' stencil table for a stencil width K: K + 1 (the handles and the
+ * counts entry) rounded up to a multiple of 8. */
+#define MFV_SW_TILES_KP(K) (((K) + 8) & ~7)
 
 /* Host-side construction of the tiles (no GPU needed). The cell ranges are the mesh
  * ({0}, {C}) or the ensemble instances (each starting at a multiple of 64 and padded to a

@@ -58,19 +58,22 @@ __device__ __forceinline__ void wave_min_part(double *__restrict__ part, size_t 
 
 // Maximum of v over the wavefront, as a wave-uniform value (for branches every lane takes the
 // same way).
-// The first KM stencil handles of one item from a tile stencil table stored row-major with a
-// row stride kp (a multiple of 8 uint16): whole 16-byte vectors, two loads for 16 handles
-// instead of one 2-byte load per handle. Handles at j >= K are 0.
+// The first KM stencil handles of one item, and its stencil and ghost counts, from a tile
+// stencil table stored row-major with a row stride kp (a multiple of 8 uint16) whose last entry
+// holds the counts (stencil | ghost << 8): whole 16-byte vectors, two loads for 16 handles and
+// the counts instead of one load per handle and two for the counts. Handles at j >= K are 0.
 template <int KM>
 __device__ __forceinline__ void stencil_row(const uint16_t *__restrict__ lst, size_t row, int kp, int K,
-                                            int (&h)[KM])
+                                            int (&h)[KM], int &nn, int &ng)
 {
     static_assert(KM % 8 == 0, "whole groups of eight handles");
     const uint4 *r = reinterpret_cast<const uint4 *>(lst + row * (size_t)kp);
+    const int gm = (kp - 1) >> 3;  // the group whose last entry is the counts
+    unsigned cnt = 0u;
 #pragma unroll
     for (int g = 0; g < KM / 8; g++) {
         uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (8 * g < K) v = r[g];  // a row holds kp >= 8 (g + 1) entries whenever 8 g < K
+        if (8 * g < K || g == gm) v = r[g];  // a row holds kp >= 8 (g + 1) entries in both cases
         const unsigned w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -78,7 +81,11 @@ __device__ __forceinline__ void stencil_row(const uint16_t *__restrict__ lst, si
             h[j] = (j < K) ? (int)(w[k] & 0xffffu) : 0;
             h[j + 1] = (j + 1 < K) ? (int)(w[k] >> 16) : 0;
         }
+        if (g == gm) cnt = v.w >> 16;
     }
+    if (gm >= KM / 8) cnt = lst[row * (size_t)kp + kp - 1];
+    nn = (int)(cnt & 0xffu);
+    ng = (int)(cnt >> 8);
 }
 
 __device__ __forceinline__ int wave_max_int(int v)

@@ -528,10 +528,12 @@ struct PackedCells {
     }
     __device__ __forceinline__ int stencil(int j) const { return m.lsq_cells[(size_t)j * m.C + i]; }
     template <int KM>
-    __device__ __forceinline__ void stencils(int K, int (&h)[KM]) const
+    __device__ __forceinline__ void stencil_counts(const Mesh &mm, int c, int (&h)[KM], int &nn, int &ng) const
     {
+        nn = mm.lsq_ncells[c];
+        ng = mm.lsq_nghosts[c];
 #pragma unroll
-        for (int j = 0; j < KM; j++) h[j] = (j < K) ? stencil(j) : 0;
+        for (int j = 0; j < KM; j++) h[j] = (j < mm.K) ? stencil(j) : 0;
     }
     __device__ __forceinline__ int ghost_owner(int, int f) const { return m.face_cells[2 * f]; }
     __device__ __forceinline__ int face_nb(int, int code) const { return code >> 3; }
@@ -558,9 +560,9 @@ struct PackedStaged {
     }
     __device__ __forceinline__ int stencil(int j) const { return lst[row * kp + j]; }
     template <int KM>
-    __device__ __forceinline__ void stencils(int K, int (&h)[KM]) const
+    __device__ __forceinline__ void stencil_counts(const Mesh &mm, int, int (&h)[KM], int &nn, int &ng) const
     {
-        stencil_row<KM>(lst, row, kp, K, h);
+        stencil_row<KM>(lst, row, kp, mm.K, h, nn, ng);
     }
     __device__ __forceinline__ int ghost_owner(int j, int) const { return lgh[(size_t)j * I + row]; }
     __device__ __forceinline__ int face_nb(int s, int) const { return lfn[(size_t)s * I + row]; }
@@ -577,8 +579,10 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
     // handles of the K rows every cell has, and the face slots of the [MF]-wide tables, which
     // are in bounds whatever the cell's face count (loads behind the count-dependent control
     // flow would each cost a memory latency of their own).
-    const int nn = m.lsq_ncells[i];
-    const int ng = m.lsq_nghosts[i];
+    constexpr int KA = KMAX > 0 ? KMAX : 8;
+    int cidx[KA];  // stencil handles (KMAX > 0) and the stencil and ghost counts, in one batch
+    int nn, ng;
+    A.template stencil_counts<KA>(m, i, cidx, nn, ng);
     const int nf = m.cell_nfaces[i];
     int codes[MF], nbh[MF];
 #pragma unroll
@@ -603,9 +607,6 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
         jv1 += (jy * dv);
     };
     if (KMAX > 0) {
-        constexpr int KA = KMAX > 0 ? KMAX : 8;
-        int cidx[KA];
-        A.template stencils<KA>(m.K, cidx);
 #pragma unroll
         for (int j = 0; j < KMAX; j++) cidx[j] = (j < nn) ? cidx[j] : A.self();
         // Every group's entries are loaded (entries past nn read the own cell, a valid handle)

@@ -185,10 +185,12 @@ struct GlobalCells {
     }
     __device__ __forceinline__ int stencil(int j) const { return m.lsq_cells[(size_t)j * m.C + i]; }
     template <int KM>
-    __device__ __forceinline__ void stencils(int K, int (&h)[KM]) const
+    __device__ __forceinline__ void stencil_counts(const Mesh &mm, int c, int (&h)[KM], int &nn, int &ng) const
     {
+        nn = mm.lsq_ncells[c];
+        ng = mm.lsq_nghosts[c];
 #pragma unroll
-        for (int j = 0; j < KM; j++) h[j] = (j < K) ? stencil(j) : 0;
+        for (int j = 0; j < KM; j++) h[j] = (j < mm.K) ? stencil(j) : 0;
     }
     __device__ __forceinline__ int ghost_owner(int, int f) const { return m.face_cells[2 * f]; }
     __device__ __forceinline__ int face_nb(int, int code) const { return code >> 3; }
@@ -205,8 +207,10 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
     // Loads in batches ahead of the arithmetic: the counts, the stencil handles of the K rows
     // every cell has, and every slot of the [MF]-wide tables (in bounds whatever the face
     // count; loads behind count-dependent control flow would each wait a memory latency).
-    const int nn = m.lsq_ncells[i];
-    const int ng = m.lsq_nghosts[i];
+    constexpr int KA = KMAX > 0 ? KMAX : 8;
+    int h[KA];  // stencil handles (KMAX > 0) and the stencil and ghost counts, in one batch
+    int nn, ng;
+    a.template stencil_counts<KA>(m, i, h, nn, ng);
     const int nf = m.cell_nfaces[i];
     int codes[MF], nbh[MF];
 #pragma unroll
@@ -225,9 +229,7 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
                    je1, ju0, ju1, jv0, jv1);
     };
     if (KMAX > 0) {
-        constexpr int KA = KMAX > 0 ? KMAX : 8;
-        int h[KA];
-        a.template stencils<KA>(m.K, h);
+
 #pragma unroll
         for (int j = 0; j < KMAX; j++) h[j] = (j < nn) ? h[j] : a.self();
         constexpr int CH = Cells::kChunk;
@@ -779,9 +781,9 @@ struct StagedCells {
     __device__ __forceinline__ double2 center(int h) const { return cc[h]; }
     __device__ __forceinline__ int stencil(int j) const { return lst[row * kp + j]; }
     template <int KM>
-    __device__ __forceinline__ void stencils(int K, int (&h)[KM]) const
+    __device__ __forceinline__ void stencil_counts(const Mesh &mm, int, int (&h)[KM], int &nn, int &ng) const
     {
-        stencil_row<KM>(lst, row, kp, K, h);
+        stencil_row<KM>(lst, row, kp, mm.K, h, nn, ng);
     }
     __device__ __forceinline__ int ghost_owner(int j, int) const { return lgh[(size_t)j * I + row]; }
     __device__ __forceinline__ int face_nb(int s, int) const { return lfn[(size_t)s * I + row]; }

@@ -59,6 +59,11 @@ extern "C" int mfv_sw_tiles_build(int C, int MF, const int32_t *nbr, int K, cons
     memset(out, 0, sizeof(*out));
     MFV_REQUIRE(C > 0 && MF >= 1 && MF <= 4 && K >= 0 && G >= 0, "bad sizes C=%d MF=%d K=%d G=%d", C, MF, K, G);
     MFV_REQUIRE(nbr && lsq_ncells && lsq_nghosts && face_cells && range_cell0 && range_ncells, "NULL array");
+    for (int c = 0; c < C; c++)
+        MFV_REQUIRE(lsq_ncells[c] >= 0 && lsq_ncells[c] <= K && lsq_ncells[c] < 256 && lsq_nghosts[c] >= 0 &&
+                        lsq_nghosts[c] <= G && lsq_nghosts[c] < 256,
+                    "cell %d: stencil and ghost counts %d, %d out of [0, %d], [0, %d] or above 255", c,
+                    lsq_ncells[c], lsq_nghosts[c], K, G);
     MFV_REQUIRE((K == 0 || lsq_cells) && (G == 0 || lsq_ghosts), "NULL stencil table");
     MFV_REQUIRE(T == 128 || T == 256 || T == 512, "block size %d not in {128, 256, 512}", T);
     MFV_REQUIRE(N >= 64 && N <= T && N % 64 == 0, "cells per tile %d must be a multiple of 64 in [64, %d]", N, T);
@@ -170,7 +175,7 @@ extern "C" int mfv_sw_tiles_build(int C, int MF, const int32_t *nbr, int K, cons
     // 3. tables
     std::vector<int32_t> cell0(nt), ncells(nt), nhalo(nt), eptr(nt + 1, 0), iptr(nt + 1, 0), ext;
     std::vector<int32_t> lnbr((size_t)MF * C, -1);
-    const int KP = MFV_SW_TILES_KP(K);  // stencil row stride: whole 16-byte vectors
+    const int KP = MFV_SW_TILES_KP(K);  // stencil row stride: whole 16-byte vectors, counts last
     std::vector<uint16_t> lst((size_t)KP * std::max(I, 1), 0);
     std::vector<uint16_t> lgh((size_t)std::max(G, 1) * std::max(I, 1), 0);
     std::vector<int16_t> lfn((size_t)MF * std::max(I, 1), -1);
@@ -207,6 +212,7 @@ extern "C" int mfv_sw_tiles_build(int C, int MF, const int32_t *nbr, int K, cons
             const size_t row = (size_t)iptr[t] + q;
             for (int j = 0; j < lsq_ncells[c]; j++)
                 lst[row * KP + j] = (uint16_t)pos[lsq_cells[(size_t)j * C + c]];
+            lst[row * KP + KP - 1] = (uint16_t)(lsq_ncells[c] | lsq_nghosts[c] << 8);
             for (int j = 0; j < lsq_nghosts[c]; j++)
                 lgh[(size_t)j * I + row] = (uint16_t)pos[face_cells[2 * (size_t)lsq_ghosts[(size_t)j * C + c]]];
             for (int s = 0; s < MF; s++) {

@@ -323,9 +323,9 @@ def sw_tiles_native(nbr, C, T, N, lsq, ranges=None):
 
 
 def sw_tiles_kp(K):
-    """Row stride of the tiles' stencil table (MFV_SW_TILES_KP): K rounded up to a multiple
-    of 8, at least 8, so that a row is whole 16-byte vectors."""
-    return 8 if K <= 8 else (int(K) + 7) & ~7
+    """Row stride of the tiles' stencil table (MFV_SW_TILES_KP): K + 1 (the handles and the
+    counts entry) rounded up to a multiple of 8, so that a row is whole 16-byte vectors."""
+    return (int(K) + 8) & ~7
 
 
 def sw_tiles_fitted(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None, N=None):
@@ -466,8 +466,10 @@ def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None, 
     it = np.broadcast_to(item_tile, sten.shape)
     lst = np.zeros(sten.shape, dtype=np.int64)
     lst[sten_ok] = ext_pos(it[sten_ok], sten[sten_ok])
-    lst_rows = np.zeros((I, sw_tiles_kp(K)), dtype=np.uint16)  # one row per item, padded
+    KP = sw_tiles_kp(K)
+    lst_rows = np.zeros((I, KP), dtype=np.uint16)  # one row per item, padded, counts last
     lst_rows[:, :K] = lst.T
+    lst_rows[:, KP - 1] = nn | (ng << 8)
     lgh = np.zeros(gown.shape, dtype=np.int64)
     itg = np.broadcast_to(item_tile, gown.shape)
     lgh[g_ok] = ext_pos(itg[g_ok], gown[g_ok])

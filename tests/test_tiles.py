@@ -43,6 +43,8 @@ def _check(L, nbr, lsq, C):
             c = int(ext[q])
             for jj in range(int(lsq['lsq_ncells'][c])):
                 assert ext[L['lst'][r, jj]] == lsq['lsq_cells'][jj, c]
+            # the row's last entry holds the item's stencil and ghost counts
+            assert L['lst'][r, -1] == int(lsq['lsq_ncells'][c]) | int(lsq['lsq_nghosts'][c]) << 8
             for jj in range(int(lsq['lsq_nghosts'][c])):
                 f = lsq['lsq_ghosts'][jj, c]
                 assert ext[L['lgh'][jj, r]] == lsq['face_cells'][f, 0]
