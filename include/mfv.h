@@ -266,8 +266,9 @@ int mfv_sw_conservation(const mfv_mesh *m, const mfv_sw_mixed *p, double *partia
  * is row item_ptr[t] + q of the per-item tables lst [nb_items][KP] (stencil, one row per item;
  * KP = K + 1 rounded up to a multiple of 8 so that a row is whole 16-byte vectors; entry KP - 1
  * holds the item's stencil and ghost counts, lsq_ncells | lsq_nghosts << 8; other unused
- * entries 0), lgh [G][nb_items] (ghost owners) and lfn [MF][nb_items] (face neighbours, -1 on
- * walls), which hold positions in the ext list. lnbr [MF][C] is nbr with the cell part replaced by the
+ * entries 0) and lgh [G][nb_items] (ghost owners), which hold positions in the ext list, and
+ * lfn [MF][nb_items]: nbr of the item's cell with the cell part replaced by the neighbour's
+ * position in the ext list ((position << 3) | (k << 1) | side, -1 on walls). lnbr [MF][C] is nbr with the cell part replaced by the
  * neighbour's position in the item list. LDS per block: max_items * MF * 16 bytes, plus
  * max_ext * 48 bytes with stage_states; at most 64 KiB. */
 typedef struct mfv_sw_tiles {
@@ -288,7 +289,7 @@ typedef struct mfv_sw_tiles {
     const int32_t *lnbr;        /* [MF][C] */
     const uint16_t *lst;        /* [nb_items][MFV_SW_TILES_KP(K)] */
     const uint16_t *lgh;        /* [G][nb_items] */
-    const int16_t *lfn;         /* [MF][nb_items] */
+    const int32_t *lfn;         /* [MF][nb_items] */
 } mfv_sw_tiles;
 
 /* Row stride of the tiles' stencil table for a stencil width K: K + 1 (the handles and the
@@ -305,7 +306,7 @@ typedef struct mfv_sw_tiles_host {
     int32_t nb_tiles, tile_size, max_halo, max_ext, nb_items, max_items, ext_len, MF, K, G;
     int32_t *tile_cell0, *tile_ncells, *tile_nhalo, *ext_ptr, *ext_cells, *item_ptr, *lnbr;
     uint16_t *lst, *lgh;
-    int16_t *lfn;
+    int32_t *lfn;
 } mfv_sw_tiles_host;
 
 int mfv_sw_tiles_build(int C, int MF, const int32_t *nbr, int K, const int32_t *lsq_cells,

@@ -536,7 +536,11 @@ struct PackedCells {
         for (int j = 0; j < KM; j++) h[j] = (j < mm.K) ? stencil(j) : 0;
     }
     __device__ __forceinline__ int ghost_owner(int, int f) const { return m.face_cells[2 * f]; }
-    __device__ __forceinline__ int face_nb(int, int code) const { return code >> 3; }
+    // the neighbour code of face slot s: nbr's (global cell << 3 | k << 1 | side, -1 on walls)
+    __device__ __forceinline__ int slot_code(const int32_t *__restrict__ nbr, size_t C, int s) const
+    {
+        return nbr[(size_t)s * C + i];
+    }
 };
 
 struct PackedStaged {
@@ -545,7 +549,7 @@ struct PackedStaged {
     const double *hv;
     const uint16_t *__restrict__ lst;
     const uint16_t *__restrict__ lgh;
-    const int16_t *__restrict__ lfn;
+    const int32_t *__restrict__ lfn;
     size_t row, I;
     int q, kp;
     __device__ __forceinline__ int self() const { return q; }
@@ -565,7 +569,11 @@ struct PackedStaged {
         stencil_row<KM>(lst, row, kp, mm.K, h, nn, ng);
     }
     __device__ __forceinline__ int ghost_owner(int j, int) const { return lgh[(size_t)j * I + row]; }
-    __device__ __forceinline__ int face_nb(int s, int) const { return lfn[(size_t)s * I + row]; }
+    // the neighbour code of face slot s with the cell replaced by its ext-list position
+    __device__ __forceinline__ int slot_code(const int32_t *__restrict__, size_t, int s) const
+    {
+        return lfn[(size_t)s * I + row];
+    }
 };
 
 // Reconstruction of cell i into out[2 s], out[2 s + 1] = (eta_f, hu_f), (hv_f, z_f) for its nf
@@ -586,9 +594,9 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
     const int nf = m.cell_nfaces[i];
     int codes[MF], nbh[MF];
 #pragma unroll
-    for (int s = 0; s < MF; s++) codes[s] = p.nbr[(size_t)s * C + i];
+    for (int s = 0; s < MF; s++) codes[s] = A.slot_code(p.nbr, C, s);
 #pragma unroll
-    for (int s = 0; s < MF; s++) nbh[s] = A.face_nb(s, codes[s]);
+    for (int s = 0; s < MF; s++) nbh[s] = codes[s] >> 3;  // the cell across, in the accessor's numbering
     double cx, cy, ei, ui, vi;
     A.ehv(A.self(), cx, cy, ei, ui, vi);
     double ixx = 0., iyy = 0., ixy = 0.;
@@ -738,7 +746,7 @@ __device__ __forceinline__ int sw_recon_core(const Mesh &m, const mfv_sw_packed 
     }
     if (lcode) {  // the neighbour codes in the accessor's numbering: (position << 3) | (k << 1) | side
 #pragma unroll
-        for (int s = 0; s < MF; s++) lcode[s] = codes[s] >= 0 ? (nbh[s] << 3) | (codes[s] & 7) : -1;
+        for (int s = 0; s < MF; s++) lcode[s] = codes[s];
     }
     return nf;
 }
@@ -961,7 +969,7 @@ struct TilesP {
     const int32_t *__restrict__ lnbr;
     const uint16_t *__restrict__ lst;
     const uint16_t *__restrict__ lgh;
-    const int16_t *__restrict__ lfn;
+    const int32_t *__restrict__ lfn;
     int I;
 };
 
@@ -1040,7 +1048,7 @@ __global__ void __launch_bounds__(T) k_sw_step_packed_tiled(Mesh m, mfv_sw_packe
     double dloc = 1e300;
     if (q < nc) {
         const int i = c0 + q;
-        // the codes of t.lnbr, (lfn << 3) | (nbr & 7), kept from the reconstruction; a slot index
+        // the codes of t.lnbr (the tile cell's lfn row) kept from the reconstruction; a slot index
         // picks one by selects (an array indexed by a lane-dependent slot would live in scratch)
         const int l0 = lc[0], l1 = lc[1], l2 = lc[2], l3 = lc[MF - 1];
         auto code_of = [l0, l1, l2, l3](int s) { return s == 0 ? l0 : s == 1 ? l1 : s == 2 ? l2 : l3; };

@@ -193,7 +193,11 @@ struct GlobalCells {
         for (int j = 0; j < KM; j++) h[j] = (j < mm.K) ? stencil(j) : 0;
     }
     __device__ __forceinline__ int ghost_owner(int, int f) const { return m.face_cells[2 * f]; }
-    __device__ __forceinline__ int face_nb(int, int code) const { return code >> 3; }
+    // the neighbour code of face slot s: nbr's (global cell << 3 | k << 1 | side, -1 on walls)
+    __device__ __forceinline__ int slot_code(const int32_t *__restrict__ nbr, size_t C, int s) const
+    {
+        return nbr[(size_t)s * C + i];
+    }
 };
 
 // KMAX > 0: the stencil handles are loaded first and the neighbour states in independent
@@ -214,9 +218,9 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
     const int nf = m.cell_nfaces[i];
     int codes[MF], nbh[MF];
 #pragma unroll
-    for (int s = 0; s < MF; s++) codes[s] = p.nbr[(size_t)s * C + i];
+    for (int s = 0; s < MF; s++) codes[s] = a.slot_code(p.nbr, C, s);
 #pragma unroll
-    for (int s = 0; s < MF; s++) nbh[s] = a.face_nb(s, codes[s]);
+    for (int s = 0; s < MF; s++) nbh[s] = codes[s] >> 3;  // the cell across, in the accessor's numbering
     const double4 si = a.state(a.self());
     const double hi = si.x, ei = si.x + si.w, ui = si.y, vi = si.z;
     const double2 ci = a.center(a.self());
@@ -277,13 +281,10 @@ __device__ __forceinline__ int recon32_core(const Mesh &m, const mfv_sw_mixed &p
         }
     }
     if (keep) {  // the codes (position << 3) | (k << 1) | side in the accessor's numbering
-        int lc[MF];
-#pragma unroll
-        for (int s = 0; s < MF; s++) lc[s] = codes[s] >= 0 ? (nbh[s] << 3) | (codes[s] & 7) : -1;
-        keep->lc0 = lc[0];
-        keep->lc1 = lc[1];
-        keep->lc2 = lc[2];
-        keep->lc3 = lc[MF - 1];
+        keep->lc0 = codes[0];
+        keep->lc1 = codes[1];
+        keep->lc2 = codes[2];
+        keep->lc3 = codes[MF - 1];
         keep->nf = nf;
     }
     for (int j = 0; j < ng; j++) {
@@ -761,7 +762,7 @@ struct TilesK {
     const int32_t *__restrict__ lnbr;
     const uint16_t *__restrict__ lst;
     const uint16_t *__restrict__ lgh;
-    const int16_t *__restrict__ lfn;
+    const int32_t *__restrict__ lfn;
 };
 
 // Reconstruction inputs from a block's LDS copy: handles are positions in the tile's ext
@@ -773,7 +774,7 @@ struct StagedCells {
     const double2 *cc;
     const uint16_t *__restrict__ lst;
     const uint16_t *__restrict__ lgh;
-    const int16_t *__restrict__ lfn;
+    const int32_t *__restrict__ lfn;
     size_t row, I;
     int q, kp;
     __device__ __forceinline__ int self() const { return q; }
@@ -786,7 +787,11 @@ struct StagedCells {
         stencil_row<KM>(lst, row, kp, mm.K, h, nn, ng);
     }
     __device__ __forceinline__ int ghost_owner(int j, int) const { return lgh[(size_t)j * I + row]; }
-    __device__ __forceinline__ int face_nb(int s, int) const { return lfn[(size_t)s * I + row]; }
+    // the neighbour code of face slot s with the cell replaced by its ext-list position
+    __device__ __forceinline__ int slot_code(const int32_t *__restrict__, size_t, int s) const
+    {
+        return lfn[(size_t)s * I + row];
+    }
 };
 
 // STAGED = false: face records in LDS, cell states read from global memory.
@@ -851,7 +856,7 @@ __global__ void __launch_bounds__(T) k_sw_step32_tiled(Mesh m, mfv_sw_mixed p, T
         const double dt = BATCH ? b.rings[4 * (size_t)b.wave_inst[i >> 6] + kcur] : ring.cur[0];
         const int32_t *__restrict__ lnbr = t.lnbr;
         const size_t C = m.C;
-        // staged: the codes of t.lnbr ((lfn << 3) | (nbr & 7)) and the face count kept from the
+        // staged: the codes of t.lnbr (the tile cell's lfn row) and the face count kept from the
         // reconstruction of the same cell; unstaged: the reconstruction numbered the neighbours
         // globally, so the block-local codes are loaded
         const int l0 = G.lc0, l1 = G.lc1, l2 = G.lc2, l3 = G.lc3;

@@ -178,7 +178,7 @@ extern "C" int mfv_sw_tiles_build(int C, int MF, const int32_t *nbr, int K, cons
     const int KP = MFV_SW_TILES_KP(K);  // stencil row stride: whole 16-byte vectors, counts last
     std::vector<uint16_t> lst((size_t)KP * std::max(I, 1), 0);
     std::vector<uint16_t> lgh((size_t)std::max(G, 1) * std::max(I, 1), 0);
-    std::vector<int16_t> lfn((size_t)MF * std::max(I, 1), -1);
+    std::vector<int32_t> lfn((size_t)MF * std::max(I, 1), -1);
     int max_halo = 0, max_ext = 0, max_items = 0;
     std::fill(mark.begin(), mark.end(), -1);
     for (int t = 0; t < nt; t++) {
@@ -217,7 +217,7 @@ extern "C" int mfv_sw_tiles_build(int C, int MF, const int32_t *nbr, int K, cons
                 lgh[(size_t)j * I + row] = (uint16_t)pos[face_cells[2 * (size_t)lsq_ghosts[(size_t)j * C + c]]];
             for (int s = 0; s < MF; s++) {
                 const int code = code_at(s, c);
-                if (code >= 0) lfn[(size_t)s * I + row] = (int16_t)pos[code >> 3];
+                if (code >= 0) lfn[(size_t)s * I + row] = (pos[code >> 3] << 3) | (code & 7);
                 if (q < tl.nc && code >= 0) lnbr[(size_t)s * C + c] = (pos[code >> 3] << 3) | (code & 7);
             }
         }

@@ -315,7 +315,7 @@ def sw_tiles_native(nbr, C, T, N, lsq, ranges=None):
                'lnbr': take(h.lnbr, MF * C, np.int32, (MF, C)),
                'lst': take(h.lst, sw_tiles_kp(K) * I, np.uint16, (I, sw_tiles_kp(K))),
                'lgh': take(h.lgh, max(G, 1) * I, np.uint16, (max(G, 1), I)),
-               'lfn': take(h.lfn, MF * I, np.int16, (MF, I))}
+               'lfn': take(h.lfn, MF * I, np.int32, (MF, I))}
         out['tile_next'] = (np.diff(out['ext_ptr']) + out['tile_ncells']).astype(np.int32)
     finally:
         _lib.call('mfv_sw_tiles_host_free', ctypes.byref(h))
@@ -475,7 +475,7 @@ def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None, 
     lgh[g_ok] = ext_pos(itg[g_ok], gown[g_ok])
     lfn = np.full(fn.shape, -1, dtype=np.int64)
     itf = np.broadcast_to(item_tile, fn.shape)
-    lfn[fn_ok] = ext_pos(itf[fn_ok], fn[fn_ok])
+    lfn[fn_ok] = (ext_pos(itf[fn_ok], fn[fn_ok]) << 3) | (fn_code[fn_ok] & 7)  # local neighbour codes
     if n_ext.max(initial=0) >= 65535:
         raise ValueError("a tile reads more than 65535 cells")
     out.update({'max_ext': int(n_ext.max()) if nt else 0, 'items': I, 'max_items': int(n_items.max()) if nt else 0,
@@ -483,7 +483,7 @@ def sw_tiles_layout(nbr, C, T=256, tile_cell0=None, tile_ncells=None, lsq=None, 
                 'item_ptr': item_ptr.astype(np.int32), 'tile_next': n_ext.astype(np.int32),
                 'lst': lst_rows,
                 'lgh': np.ascontiguousarray(lgh, dtype=np.uint16) if G else np.zeros((1, max(I, 1)), np.uint16),
-                'lfn': np.ascontiguousarray(lfn, dtype=np.int16)})
+                'lfn': np.ascontiguousarray(lfn, dtype=np.int32)})
     return out
 
 

@@ -50,7 +50,8 @@ def _check(L, nbr, lsq, C):
                 assert ext[L['lgh'][jj, r]] == lsq['face_cells'][f, 0]
             for s in range(MF):
                 g = int(nbr[s, c])
-                assert (L['lfn'][s, r] == -1) if g < 0 else (ext[L['lfn'][s, r]] == g >> 3)
+                lf = int(L['lfn'][s, r])
+                assert (lf == -1) if g < 0 else (ext[lf >> 3] == g >> 3 and lf & 7 == g & 7)
         assert int(L['item_ptr'][t + 1]) - int(L['item_ptr'][t]) == n + h
     assert covered.all()
 
