@@ -95,44 +95,62 @@ __device__ __forceinline__ void for_slices(const Ell &e, const Work &w, int want
 // its columns as 16-bit offsets from the row (the branch is uniform over the wave: one slice).
 __device__ __forceinline__ bool slice_is_narrow(const Ell &e, int s) { return e.narrow && e.narrow[s]; }
 
-__device__ __forceinline__ float ell_row32(const Ell &e, int s, int lane, const float *__restrict__ x)
+// Sum over the row's slots j < wdt of val[j] * x[col[j]], accumulated in j order. The slots are
+// taken in groups (8, then 4, then 1) whose column, value and x loads are all issued before the
+// group's multiply-adds, so a slot no longer waits two memory latencies of its own; the
+// accumulation order, hence the bits, are those of the slot-by-slot loop.
+template <int G, class Val, class Col, class X, class Acc>
+__device__ __forceinline__ int ell_group(int j, int wdt, size_t base, const Val *__restrict__ vals, Col col,
+                                         const X *__restrict__ x, Acc &acc)
+{
+    for (; j + G <= wdt; j += G) {
+        Val v[G];
+        int c[G];
+#pragma unroll
+        for (int u = 0; u < G; u++) {
+            const size_t q = base + (size_t)(j + u) * 64;
+            v[u] = vals[q];
+            c[u] = col(q);
+        }
+        X xv[G];
+#pragma unroll
+        for (int u = 0; u < G; u++) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < G; u++) acc = fma(v[u], xv[u], acc);
+    }
+    return j;
+}
+
+template <class Val, class X, class Acc>
+__device__ __forceinline__ Acc ell_row(const Ell &e, int s, int lane, const Val *__restrict__ vals,
+                                       const X *__restrict__ x)
 {
     const int wdt = e.slice_w[s];
     const size_t base = (size_t)s * e.W * 64 + lane;
-    float acc = 0.f;
+    Acc acc = 0;
     if (slice_is_narrow(e, s)) {
         const int row = s * 64 + lane;
-        for (int j = 0; j < wdt; j++) {
-            const size_t q = base + (size_t)j * 64;
-            acc = fmaf(e.vals32[q], x[row + e.cols16[q]], acc);
-        }
+        auto col = [&](size_t q) { return row + e.cols16[q]; };
+        int j = ell_group<8>(0, wdt, base, vals, col, x, acc);
+        j = ell_group<4>(j, wdt, base, vals, col, x, acc);
+        ell_group<1>(j, wdt, base, vals, col, x, acc);
     } else {
-        for (int j = 0; j < wdt; j++) {
-            const size_t q = base + (size_t)j * 64;
-            acc = fmaf(e.vals32[q], x[e.cols[q]], acc);
-        }
+        auto col = [&](size_t q) { return e.cols[q]; };
+        int j = ell_group<8>(0, wdt, base, vals, col, x, acc);
+        j = ell_group<4>(j, wdt, base, vals, col, x, acc);
+        ell_group<1>(j, wdt, base, vals, col, x, acc);
     }
     return acc;
 }
 
+__device__ __forceinline__ float ell_row32(const Ell &e, int s, int lane, const float *__restrict__ x)
+{
+    return ell_row<float, float, float>(e, s, lane, e.vals32, x);
+}
+
 __device__ __forceinline__ double ell_row64(const Ell &e, int s, int lane, const double *__restrict__ x)
 {
-    const int wdt = e.slice_w[s];
-    const size_t base = (size_t)s * e.W * 64 + lane;
-    double acc = 0.;
-    if (slice_is_narrow(e, s)) {
-        const int row = s * 64 + lane;
-        for (int j = 0; j < wdt; j++) {
-            const size_t q = base + (size_t)j * 64;
-            acc = fma(e.vals[q], x[row + e.cols16[q]], acc);
-        }
-    } else {
-        for (int j = 0; j < wdt; j++) {
-            const size_t q = base + (size_t)j * 64;
-            acc = fma(e.vals[q], x[e.cols[q]], acc);
-        }
-    }
-    return acc;
+    return ell_row<double, double, double>(e, s, lane, e.vals, x);
 }
 
 // One wave per slice: the slice is narrow when every used slot's column is within +-32767 of its
