@@ -29,10 +29,10 @@ HBM_PEAK_GBS = 8000.0
 # once at its element width): shallow water second order float64 / float32, Euler first order
 # float64 / float32, advection first order float64; a Krylov (BiCGStab) iteration per matrix
 # row with float32 ELL values.
-SURVEY_B_SW64, SURVEY_B_MIXED = 356.0, 214.0
-SURVEY_B_EULER64, SURVEY_B_EULER32 = 208.0, 124.0
-SURVEY_B_ADV64 = 112.0
-SURVEY_B_KRYLOV32 = 164.0
+ALGO_B_SW64, ALGO_B_MIXED = 356.0, 214.0
+ALGO_B_EULER64, ALGO_B_EULER32 = 208.0, 124.0
+ALGO_B_ADV64 = 112.0
+ALGO_B_KRYLOV32 = 164.0
 
 
 def measured_traffic(kernel, cells):
@@ -311,7 +311,7 @@ def run_advection(n, steps, warmup, cpu=True):
     # faceid + order entries; per face (x F/C): cellid, normal (3 doubles), u_f v_f w_f
     MF = int(d.cells.faceid.shape[1] - 1)
     bpc = (8 + 8 + 8 + 24 + 4 + (4 + 1) * MF) + (8 + 24 + 24) * F / C
-    rl = roofline("k_adv_step + k_dt_finish", SURVEY_B_ADV64, C, t_ev, bpc, measured_traffic('k_adv_step<2, 1>', C))
+    rl = roofline("k_adv_step + k_dt_finish", ALGO_B_ADV64, C, t_ev, bpc, measured_traffic('k_adv_step<2, 1>', C))
     rl["bound"] = "latency (the step's arrays, %.1f MB, stay in the 256 MB Infinity Cache)" % (bpc * C / 1e6)
     out = {"workload": "advection of a Gaussian blob, %dx%dx2 jittered triangles, order 1, CFL 0.8" % (n, n),
            "cells": C, "faces": F, "steps": steps, "dtype": "f64",
@@ -439,7 +439,7 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=5, renumber='hilbert'):
            "cells": C, "faces": F, "steps": steps, "dtype": "f64", "mesh_build_s": round(t_mesh, 1),
            "value": round(C / t_loop / 1e6, 3), "unit": "Mcell-updates/s", "ms_per_step": round(t_loop * 1e3, 5),
            "path": best, "path_ms_per_step": {k: round(v, 4) for k, v in probe.items()}, "cell_order": renumber,
-           "roofline": roofline(kname + " + k_dt_finish", SURVEY_B_EULER64, C, t_ev, bpc,
+           "roofline": roofline(kname + " + k_dt_finish", ALGO_B_EULER64, C, t_ev, bpc,
                                 measured_traffic(kname, C))}
     # FP32-bulk variant on the same tiles: throughput, then 200-step (or --euler-steps) accuracy
     # against the float64 step and the float64 conservation sums
@@ -497,7 +497,7 @@ def run_euler(n, steps, warmup, cpu=True, cpu_steps=5, renumber='hilbert'):
             "speedup_vs_fp64": round(t_loop / t32_loop, 3), "tile": T32,
             "tile_ms_per_step": {str(k): round(v, 4) for k, v in probe32.items()},
             "dtype": "f32 records, geometry and face fluxes; f64 gathers, update, dt and conservation sums",
-            "roofline": roofline("k_euler_step32_tiled<%d> + k_dt_finish" % T32, SURVEY_B_EULER32, C, t32_ev, b32,
+            "roofline": roofline("k_euler_step32_tiled<%d> + k_dt_finish" % T32, ALGO_B_EULER32, C, t32_ev, b32,
                                  measured_traffic("k_euler_step32_tiled<%d>" % T32, C)),
             "accuracy": {"steps": steps, "error_vs_fp64": errs,
                          "max_rel_L1": max(v["rel_L1"] for v in errs.values()), "target_rel_L1": 1e-5,
@@ -595,7 +595,7 @@ def run_mixed(d, m, Z, h0, zgrad, warmup, steps, mcell_fp64, kernel='auto', tile
         "speedup_vs_fp64": round(mcell / mcell_fp64, 3),
         "dtype": "f32 flux/gradient arithmetic and face records, f64 state/gathers/dt/sums",
         "mass_rel_drift": (mass1 - mass0) / mass0,
-        "roofline": roofline(kname, SURVEY_B_MIXED, C, t_dev, bmodel,
+        "roofline": roofline(kname, ALGO_B_MIXED, C, t_dev, bmodel,
                              measured_traffic(trace_key, C) if trace_key else None),
     }
 
@@ -621,8 +621,8 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
 
 
 def roofline(kernel, bytes_per_cell, cells, t_step, model_bytes=None, traffic=None):
-    """Roofline entry of one step: algorithmic bytes per cell-update (the hot-path survey's
-    count) x cells / the device time per step from HIP events over the timed region."""
+    """Roofline entry of one step: algorithmic bytes per cell-update (the ALGO_B_* counts above)
+    x cells / the device time per step from HIP events over the timed region."""
     ach = bytes_per_cell * cells / t_step / 1e9
     r = {"bound": "hbm", "kernel": kernel, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
          "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_cell": bytes_per_cell,
@@ -776,7 +776,7 @@ def run_ensemble(n_inst, steps, seed=0, n_check=None):
         "value": round(mcell, 3), "unit": "Mcell-updates/s", "instances": n_inst, "cells": cells,
         "steps": steps, "ms_per_step": round(t_loop / steps * 1e3, 4), "graph": "one captured graph of all steps",
         "config": list(tune['config']), "autotune_ms_per_step": tune['ms_per_step'],
-        "roofline": roofline("k_sw_step32_tiled (all instances) + k_batch_dt_fold", SURVEY_B_MIXED, cells, t_dev,
+        "roofline": roofline("k_sw_step32_tiled (all instances) + k_batch_dt_fold", ALGO_B_MIXED, cells, t_dev,
                              traffic=measured_traffic("k_sw_step32_tiled:batch:%s:%d:%d" % tuple(tune['config']),
                                                       cells)),
         "error_vs_fp64": {"instances_checked": len(pick),
@@ -841,7 +841,7 @@ def run_serving(n_req, seed=1, nsteps=20, n_iter=6, prep_threads=8, n_capacity=6
     ok = [r for r in done if r.failed is False]
     rows_iters = float(sum(r.C * r.iterations for r in ok))
     t_ticks = eng.tick_seconds()
-    ach = SURVEY_B_KRYLOV32 * rows_iters / t_ticks / 1e9
+    ach = ALGO_B_KRYLOV32 * rows_iters / t_ticks / 1e9
     return {
         "requests": n_req, "completed": len(ok), "failed": len(done) - len(ok), "steps_per_request": nsteps,
         "cells_total": total_cells, "capacity_req_per_s": round(capacity, 3),
@@ -860,7 +860,7 @@ def run_serving(n_req, seed=1, nsteps=20, n_iter=6, prep_threads=8, n_capacity=6
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": krylov_traffic(),
                      "traffic_unit": "HBM bytes per row-iteration (rocprof counters of the serving workload)",
-                     "bytes_per_row_iteration": SURVEY_B_KRYLOV32,
+                     "bytes_per_row_iteration": ALGO_B_KRYLOV32,
                      "tick_device_s": round(t_ticks, 3), "row_iterations": rows_iters},
     }
 
@@ -1051,7 +1051,7 @@ def main():
         "config": {"workload": "shallow-water dam break, 2D, MUSCL + LSQ + Barth-Jespersen + Rusanov, walls",
                    "cells": C, "faces": F, "nx": args.nx, "cfl": CFL, "order": 2, "precision": "fp64 plain",
                    "path": path, "tile": fused_T, "cell_order": args.renumber},
-        "roofline": roofline(kname, SURVEY_B_SW64, C, t_dev, bmodel,
+        "roofline": roofline(kname, ALGO_B_SW64, C, t_dev, bmodel,
                              measured_traffic(trace_key, C) if trace_key else None),
         "fp64_path_ms_per_step": path_ms,
         "mesh_build_s": round(t_mesh, 2),
